@@ -1,0 +1,109 @@
+"""ctypes binding of the C ABI in include/clt_tile_processor.h (libclt_hip.so).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly when the built
+library is missing.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libclt_hip.so")
+
+CLT_OK = 0
+ERRORS = {-1: "CLT_E_ARG", -2: "CLT_E_STATE", -3: "CLT_E_HIP", -4: "CLT_E_NODEVICE",
+          -5: "CLT_E_NOMEM", -6: "CLT_E_UNSUPPORTED"}
+DISP_LAYERS = 8
+DISP_NAMES = ["int", "cm", "strength", "hor", "hor_strength", "vert", "vert_strength", "cm_weight"]
+TIMING_KERNELS = {"convert": 0, "imclt": 1, "corr2d": 2, "textures": 3, "combine_textures": 4}
+
+
+class clt_geom(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "width", "height", "num_cams", "num_colors", "red_row", "red_col",
+        "kernels_hor", "kernels_vert", "kernel_step", "max_tasks")]
+
+
+class clt_pair(ctypes.Structure):
+    _fields_ = [("cam1", ctypes.c_int32), ("cam2", ctypes.c_int32),
+                ("dir", ctypes.c_int32), ("scale", ctypes.c_int32)]
+
+
+class clt_corr_params(ctypes.Structure):
+    _fields_ = [("fat_zero", ctypes.c_float), ("corr_red", ctypes.c_float),
+                ("corr_blue", ctypes.c_float), ("corr_offset", ctypes.c_float),
+                ("twice_diagonal", ctypes.c_int32), ("min_corr", ctypes.c_float),
+                ("strip_height", ctypes.c_int32), ("enhortho_width", ctypes.c_float),
+                ("enhortho_scale", ctypes.c_float)]
+
+
+class clt_tex_params(ctypes.Structure):
+    _fields_ = [("min_shot", ctypes.c_float), ("scale_shot", ctypes.c_float),
+                ("diff_sigma", ctypes.c_float), ("dust_remove", ctypes.c_int32),
+                ("corr_magic_scale", ctypes.c_float), ("keep_weights", ctypes.c_int32),
+                ("port_offsets", (ctypes.c_float * 2) * 16)]
+
+
+_P = ctypes.c_void_p
+_H = ctypes.c_void_p
+_I = ctypes.c_int32
+
+# name -> (restype, argtypes); every entry point declared in include/clt_tile_processor.h
+SIGNATURES = {
+    "clt_create": (_I, [ctypes.POINTER(clt_geom), ctypes.POINTER(_H)]),
+    "clt_destroy": (_I, [_H]),
+    "clt_last_error": (ctypes.c_char_p, [_H]),
+    "clt_device_count": (_I, []),
+    "clt_set_kernels": (_I, [_H, _I, _P, _P]),
+    "clt_clear_kernels": (_I, [_H]),
+    "clt_set_image": (_I, [_H, _I, _P, _I]),
+    "clt_set_tasks": (_I, [_H, _P, _I]),
+    "clt_set_lpf_rbg": (_I, [_H, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "clt_set_corr_sigma": (_I, [_H, ctypes.c_float]),
+    "clt_set_pairs": (_I, [_H, _P, _I]),
+    "clt_set_corr_params": (_I, [_H, ctypes.POINTER(clt_corr_params)]),
+    "clt_set_tex_params": (_I, [_H, ctypes.POINTER(clt_tex_params)]),
+    "clt_lpf_coeffs": (_I, [ctypes.c_double, _P]),
+    "clt_default_corr_params": (None, [ctypes.POINTER(clt_corr_params)]),
+    "clt_default_tex_params": (None, [ctypes.POINTER(clt_tex_params), _I]),
+    "clt_default_quad_pairs": (_I, [_P]),
+    "clt_exec_convert": (_I, [_H, _I]),
+    "clt_exec_imclt": (_I, [_H, _I]),
+    "clt_exec_corr2d": (_I, [_H]),
+    "clt_exec_textures": (_I, [_H, _P]),
+    "clt_exec_combine_textures": (_I, [_H]),
+    "clt_get_rbg": (_I, [_H, _I, _P]),
+    "clt_get_clt": (_I, [_H, _I, _P]),
+    "clt_get_corr2d": (_I, [_H, _P, _P, _P]),
+    "clt_get_textures": (_I, [_H, _P, _P]),
+    "clt_get_combined_textures": (_I, [_H, _P]),
+    "clt_set_async": (_I, [_H, _I]),
+    "clt_sync": (_I, [_H]),
+    "clt_set_timing": (_I, [_H, _I]),
+    "clt_get_timing": (_I, [_H, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
+    "clt_reset_timing": (_I, [_H]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libclt_hip.so and declare every exported function; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libclt_hip.so is not built (%s); run __graft_entry__.build() or "
+                          "python imagej-elphel_amd/build.py" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class CltError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code

@@ -1,0 +1,154 @@
+// clt_device.hpp -- device-side building blocks of the CLT tile processor (gfx950).
+//
+// Work unit of the transform kernels: one 8x8 tile of one camera and one colour, handled by a
+// group of 16 lanes (4 units per 64-lane wavefront).  Separable 8-point transforms run in
+// registers (one 8-vector per lane), the transposes between the horizontal and the vertical pass
+// go through a 1 KiB LDS scratch per unit, partner exchanges between the two horizontal modes use
+// cross-lane shuffles.  CLT coefficients are stored [mode][kh][kv] (horizontal frequency major,
+// the order of the reference CPU path, DttRad2.java:503-519).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace clt {
+
+// Tables computed on the host in double precision from their closed forms and uploaded once.
+struct Tables {
+    float hw[8];        // half window sin(pi*(i+0.5)/16)            (DttRad2.java:684)
+    float civ[8][8];    // orthonormal DCT-IV, symmetric              (DttRad2.java:993-1012)
+    float ciie[8][8];   // DCT-IIe                                    (DttRad2.java:952-968)
+    float siie[8][8];   // DST-IIe                                    (DttRad2.java:1014-1030)
+};
+extern __constant__ Tables c_tab;
+
+struct GeomDev {
+    int width, height, cams, colors, red_row, red_col;
+    int kh, kv, kstep;            // kernel grid and step
+    int tiles_x, tiles_y;
+};
+
+// ---------------------------------------------------------------- lane / wave helpers
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of the
+// logical block space so neighbouring tiles share an L2.
+__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
+    const int nx = 8;
+    int per = nblocks / nx, rem = nblocks % nx;
+    int x = bid % nx, local = bid / nx;
+    return (x < rem) ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
+}
+
+// ---------------------------------------------------------------- 8-point transforms
+// y = DCT-IV(x) (orthonormal; its own inverse)
+__device__ __forceinline__ void dct_iv8(const float* x, float* y) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) s = fmaf(c_tab.civ[k][j], x[j], s);
+        y[k] = s;
+    }
+}
+// y = DST-IV(x) = (-1)^k DCT-IV(reverse x)[k]  (DttRad2.java:88-98)
+__device__ __forceinline__ void dst_iv8(const float* x, float* y) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) s = fmaf(c_tab.civ[k][j], x[7 - j], s);
+        y[k] = (k & 1) ? -s : s;
+    }
+}
+
+// ---------------------------------------------------------------- fold / unfold index algebra
+// DttRad2.java:112-144 get_fold_indices(x, 8): tap b of folded output x reads source index
+// fold_src, window hw[fold_w] (+ derivative term hw[fold_dw] * fold_ds for a shifted window),
+// with sign fold_sc for the cosine (C) and fold_ss for the sine (S) fold.
+__host__ __device__ constexpr int fold_src(int x, int b) {
+    return (x < 4) ? (b == 0 ? 11 - x : 12 + x) : (b == 0 ? x - 4 : 11 - x);
+}
+__host__ __device__ constexpr int fold_w(int x, int b) {
+    return (x < 4) ? (b == 0 ? 4 + x : 3 - x) : (b == 0 ? x - 4 : 11 - x);
+}
+__host__ __device__ constexpr int fold_dw(int x, int b) {
+    return (x < 4) ? (b == 0 ? 3 - x : 4 + x) : (b == 0 ? 11 - x : x - 4);
+}
+__host__ __device__ constexpr int fold_ds(int x, int b) { return (x < 4) ? -1 : 1; }
+__host__ __device__ constexpr int fold_sc(int x, int b) { return (x < 4) ? -1 : (b == 0 ? 1 : -1); }
+__host__ __device__ constexpr int fold_ss(int x, int b) { return (x < 4) ? (b == 0 ? 1 : -1) : 1; }
+
+// DttRad2.java:392-403 get_unfold_index_signs(x, 8): output x of the 16-long inverse reads IDCT
+// index unfold_idx, signs unfold_sc (C) / unfold_ss (S), window hw[x < 8 ? x : 15 - x].
+__host__ __device__ constexpr int unfold_idx(int x) {
+    return (x < 4) ? x + 4 : (x < 8) ? 11 - x : (x < 12) ? 11 - x : x - 12;
+}
+__host__ __device__ constexpr int unfold_sc(int x) { return (x < 4) ? 1 : -1; }
+__host__ __device__ constexpr int unfold_ss(int x) { return (x < 12) ? 1 : -1; }
+__host__ __device__ constexpr int unfold_w(int x) { return (x < 8) ? x : 15 - x; }
+
+// Shifted-window lapped fold of a 16-vector into the cosine and sine 8-vectors
+// (DttRad2.java:263-314 get_shifted_fold_2d, one dimension): ac, as = cos, sin(pi/16 * shift).
+__device__ __forceinline__ void fold16(const float* x, float ac, float as, float* yc, float* ys) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        float w0 = fmaf(as, c_tab.hw[fold_dw(i, 0)] * fold_ds(i, 0), ac * c_tab.hw[fold_w(i, 0)]);
+        float w1 = fmaf(as, c_tab.hw[fold_dw(i, 1)] * fold_ds(i, 1), ac * c_tab.hw[fold_w(i, 1)]);
+        float a = w0 * x[fold_src(i, 0)], b = w1 * x[fold_src(i, 1)];
+        yc[i] = fold_sc(i, 0) * a + fold_sc(i, 1) * b;
+        ys[i] = fold_ss(i, 0) * a + fold_ss(i, 1) * b;
+    }
+}
+
+// Inverse lapped unfold of the cosine/sine IDCT outputs to 16 samples (DttRad2.java:405-434).
+__device__ __forceinline__ void unfold16(const float* oc, const float* os, float* y) {
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        y[r] = c_tab.hw[unfold_w(r)] * (unfold_sc(r) * oc[unfold_idx(r)] + unfold_ss(r) * os[unfold_idx(r)]);
+}
+
+// cos/sin of (k + 0.5) * pi * t for k = 0..7 by rotation recurrence (2 sincospi + 7 rotations).
+__device__ __forceinline__ void half_phase8(float t, float* c, float* s) {
+    float s1, c1, sh, ch;
+    sincospif(t, &s1, &c1);
+    sincospif(0.5f * t, &sh, &ch);
+    c[0] = ch;
+    s[0] = sh;
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        c[k] = fmaf(c[k - 1], c1, -s[k - 1] * s1);
+        s[k] = fmaf(s[k - 1], c1, c[k - 1] * s1);
+    }
+}
+
+// ---------------------------------------------------------------- inverse CLT of one unit
+// Lane (hs = l >> 3, kh = l & 7) enters with p[kv] = CLT[hs][kh][kv], q[kv] = CLT[hs | 2][kh][kv];
+// scratch: 256 floats of LDS private to this unit.  Leaves with row[c] = output row r = l of the
+// 16x16 pixel tile (sum over the 4 modes of unfold(IDTT), ImageDtt.java:3620-3632 without the 1/4).
+__device__ __forceinline__ void imclt_unit(int l, const float* p, const float* q, float* scratch, float* row) {
+    const int hs = l >> 3, kh = l & 7;
+    float oc[8], os[8], col[16];
+    dct_iv8(p, oc);                          // vertical inverse of the C (v) modes
+    dst_iv8(q, os);                          // vertical inverse of the S (v) modes
+    unfold16(oc, os, col);                   // 16 rows for (hs, kh)
+#pragma unroll
+    for (int r = 0; r < 16; r += 4)
+        *reinterpret_cast<float4*>(&scratch[(hs * 8 + kh) * 16 + r]) = make_float4(col[r], col[r + 1], col[r + 2], col[r + 3]);
+    wave_sync();
+    float rc[8], rs[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        rc[k] = scratch[k * 16 + l];
+        rs[k] = scratch[(8 + k) * 16 + l];
+    }
+    wave_sync();
+    dct_iv8(rc, oc);                         // horizontal inverse, C (h) modes
+    dst_iv8(rs, os);                         // horizontal inverse, S (h) modes
+    unfold16(oc, os, row);
+}
+
+}  // namespace clt

@@ -1,0 +1,970 @@
+// clt_kernels.hpp -- the four hot kernels of the tile processor and the texture combiner.
+#pragma once
+#include "clt_device.hpp"
+
+namespace clt {
+
+// ====================================================================================
+// convert_direct: Bayer/mono mosaic -> per tile/camera/colour 16x16 lapped DCT-IV/DST-IV
+// with aberration-kernel multiply and fractional shift (ImageDtt.java:4867-5584 semantics;
+// replaces TileProcessor.cuh convert_correct_tiles).
+// Unit u = (task, camera, colour); 16 lanes per unit, 16 units per 256-thread block.
+// ====================================================================================
+#pragma clang fp contract(off)
+// Integer window corner and residual shift of one unit, in double precision with the operation
+// order of ImageDtt.java:4904-4943 so the integer corner is bit-identical to the CPU reference.
+struct TilePos {
+    int left, top;
+    float rx, ry;
+};
+__device__ __forceinline__ TilePos tile_position(double cx, double cy, const float* ce, int kstep, int ktx, int kty) {
+    double px = cx - 8.0, py = cy - 8.0;
+    if (ce) {
+        double kdx = cx - ((double)(ktx - 1) + 0.5) * kstep;
+        double kdy = cy - ((double)(kty - 1) + 0.5) * kstep;
+        px = cx - 8.0 - ((double)ce[0] + (double)ce[4] * kdx + (double)ce[5] * kdy);
+        py = cy - 8.0 - ((double)ce[1] + (double)ce[6] * kdx + (double)ce[7] * kdy);
+    }
+    TilePos p;
+    p.left = (int)-floor(-px + 0.5);
+    p.top = (int)-floor(-py + 0.5);
+    p.rx = (float)(-(px - p.left));
+    p.ry = (float)(-(py - p.top));
+    return p;
+}
+#pragma clang fp contract(fast)
+
+__device__ __forceinline__ int clamp_parity(int p, int n) {   // ImageDtt.java:4958-4965
+    if (p < 0) return p & 1;
+    if (p >= n) return n - 2 + (p & 1);
+    return p;
+}
+
+__global__ __launch_bounds__(256) void k_convert(const float* __restrict__ images, const float* __restrict__ kernels,
+                                                 const float* __restrict__ koffsets, const int32_t* __restrict__ tasks,
+                                                 int task_words, int num_tasks, float* __restrict__ clt,
+                                                 const float* __restrict__ lpf, int lpf_mask, GeomDev g) {
+    __shared__ __attribute__((aligned(16))) float s_buf[16][256];
+    const int nblocks = gridDim.x;
+    const int blk = xcd_block(blockIdx.x, nblocks);
+    const int unit_in_block = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int total = num_tasks * g.cams * g.colors;
+    int u = blk * 16 + unit_in_block;
+    bool active = u < total;
+    if (!active) u = 0;
+    const int t = u / (g.cams * g.colors), rem = u % (g.cams * g.colors);
+    const int cam = rem / g.colors, color = rem % g.colors;
+    const int32_t* rec = tasks + (size_t)t * task_words;
+    active = active && (rec[0] != 0);
+    if (!__any(active)) return;   // whole wave idle (no block-level barriers in this kernel)
+    const int txy = rec[1];
+    const int tx = txy & 0xffff, ty = (txy >> 16) & 0xffff;
+    const double cx = active ? (double)__int_as_float(rec[2 + 2 * cam]) : 8.0;
+    const double cy = active ? (double)__int_as_float(rec[3 + 2 * cam]) : 8.0;
+
+    // ---- kernel tile (ImageDtt.java:4904-4911) and integer/residual position
+    const float* ce = nullptr;
+    const float* kt = nullptr;
+    int ktx = 0, kty = 0;
+    if (kernels) {
+        ktx = (int)floor(cx / g.kstep + 0.5) + 1;
+        kty = (int)floor(cy / g.kstep + 0.5) + 1;
+        kty = kty < 0 ? 0 : (kty >= g.kv ? g.kv - 1 : kty);
+        ktx = ktx < 0 ? 0 : (ktx >= g.kh ? g.kh - 1 : ktx);
+        size_t kidx = (((size_t)cam * g.kv + kty) * g.kh + ktx) * g.colors + color;
+        ce = koffsets + kidx * 8;
+        kt = kernels + kidx * 256;
+    }
+    const TilePos pos = tile_position(cx, cy, ce, g.kstep, ktx, kty);
+    const int left = pos.left, top = pos.top;
+    const float rx = pos.rx, ry = pos.ry;
+
+    // ---- stage H: lane = window row r; masked 16 pixels, horizontal fold + DCT-IV / DST-IV
+    const float* img = images + (size_t)cam * g.height * g.width;
+    const int r = l;
+    const int prow = top + r;
+    const int pi = clamp_parity(prow, g.height);
+    float x[16];
+    const bool mono = g.colors == 1;
+    const bool row_red = ((prow & 1) == g.red_row);
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int pc = left + j;
+        bool col_red = ((pc & 1) == g.red_col);
+        int pcol = mono ? 0 : (row_red && col_red) ? 0 : (!row_red && !col_red) ? 1 : 2;
+        x[j] = 0.f;
+        if (active && pcol == color) x[j] = img[(size_t)pi * g.width + clamp_parity(pc, g.width)];
+    }
+    float ahs, ahc, avs, avc;
+    sincospif(rx * 0.0625f, &ahs, &ahc);
+    sincospif(ry * 0.0625f, &avs, &avc);
+    float yc[8], ys[8], hc[8], hsn[8];
+    fold16(x, ahc, ahs, yc, ys);
+    dct_iv8(yc, hc);
+    dst_iv8(ys, hsn);
+    float* sb = s_buf[unit_in_block];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        sb[k * 16 + r] = hc[k];
+        sb[(8 + k) * 16 + r] = hsn[k];
+    }
+    wave_sync();
+
+    // ---- stage V: lane = (hs, kh); vertical fold + DCT-IV (mode hs) / DST-IV (mode hs|2)
+    const int hs = l >> 3, kh = l & 7;
+    float col[16];
+#pragma unroll
+    for (int i = 0; i < 16; i += 4) {
+        float4 v = *reinterpret_cast<const float4*>(&sb[(hs * 8 + kh) * 16 + i]);
+        col[i] = v.x; col[i + 1] = v.y; col[i + 2] = v.z; col[i + 3] = v.w;
+    }
+    float zc[8], zs[8], a[8], b[8];
+    fold16(col, avc, avs, zc, zs);
+    dct_iv8(zc, a);    // mode hs
+    dst_iv8(zs, b);    // mode hs | 2
+    const float sigma = hs ? 1.f : -1.f;
+
+    // ---- aberration kernel multiply (ImageDtt.java:5429-5488), partner lane holds modes hs^1
+    if (kt) {
+        float pa[8], pb[8], k0[8], k1[8], k2[8], k3[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            pa[k] = __shfl_xor(a[k], 8, 16) * sigma;
+            pb[k] = __shfl_xor(b[k], 8, 16) * sigma;
+        }
+        const float* kq = kt + kh * 8;
+#pragma unroll
+        for (int k = 0; k < 8; k += 4) {
+            float4 v0 = *reinterpret_cast<const float4*>(kq + k);
+            float4 v1 = *reinterpret_cast<const float4*>(kq + 64 + k);
+            float4 v2 = *reinterpret_cast<const float4*>(kq + 128 + k);
+            float4 v3 = *reinterpret_cast<const float4*>(kq + 192 + k);
+            k0[k] = v0.x; k0[k + 1] = v0.y; k0[k + 2] = v0.z; k0[k + 3] = v0.w;
+            k1[k] = v1.x; k1[k + 1] = v1.y; k1[k + 2] = v1.z; k1[k + 3] = v1.w;
+            k2[k] = v2.x; k2[k + 1] = v2.y; k2[k + 2] = v2.z; k2[k + 3] = v2.w;
+            k3[k] = v3.x; k3[k + 1] = v3.y; k3[k + 2] = v3.z; k3[k + 3] = v3.w;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float oa = a[k] * k0[k] + pa[k] * k1[k] - b[k] * k2[k] - pb[k] * k3[k];
+            float ob = b[k] * k0[k] + pb[k] * k1[k] + a[k] * k2[k] + pa[k] * k3[k];
+            a[k] = oa;
+            b[k] = ob;
+        }
+    }
+    // ---- fractional shift (ImageDtt.java:5490-5584): horizontal by rx (kh), vertical by ry (kv)
+    {
+        float sh, ch, cv[8], sv[8];
+        sincospif((kh + 0.5f) * rx * 0.125f, &sh, &ch);
+        half_phase8(ry * 0.125f, cv, sv);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float pa = __shfl_xor(a[k], 8, 16) * sigma;
+            float pb = __shfl_xor(b[k], 8, 16) * sigma;
+            float ta = a[k] * ch + pa * sh;
+            float tb = b[k] * ch + pb * sh;
+            a[k] = ta * cv[k] - tb * sv[k];
+            b[k] = tb * cv[k] + ta * sv[k];
+        }
+    }
+    if (lpf && ((lpf_mask >> color) & 1)) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float f = lpf[color * 64 + kh * 8 + k];
+            a[k] *= f;
+            b[k] *= f;
+        }
+    }
+    if (active) {
+        const size_t tile = (size_t)ty * g.tiles_x + tx;
+        float* o = clt + (((size_t)cam * g.tiles_x * g.tiles_y + tile) * g.colors + color) * 256 + kh * 8;
+        *reinterpret_cast<float4*>(o + hs * 64) = make_float4(a[0], a[1], a[2], a[3]);
+        *reinterpret_cast<float4*>(o + hs * 64 + 4) = make_float4(a[4], a[5], a[6], a[7]);
+        *reinterpret_cast<float4*>(o + (hs | 2) * 64) = make_float4(b[0], b[1], b[2], b[3]);
+        *reinterpret_cast<float4*>(o + (hs | 2) * 64 + 4) = make_float4(b[4], b[5], b[6], b[7]);
+    }
+}
+
+// ====================================================================================
+// imclt_rbg: inverse CLT + overlap-add, gather form (no read-modify-write, one launch).
+// Block = strip of IMCLT_S output 8x8 blocks wide, IMCLT_R block rows tall; it sweeps down
+// the tile rows keeping the bottom halves of the previous tile row in LDS.  Output image of a
+// camera/colour is (H+8) x (W+8), uncropped (GPUTileProcessor.java:599-626).
+// ====================================================================================
+constexpr int IMCLT_S = 15;   // output blocks per strip (16 tiles incl. the left halo)
+constexpr int IMCLT_R = 16;   // output block rows per segment
+
+__global__ __launch_bounds__(256) void k_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tile_map,
+                                               const float* __restrict__ lpf, float* __restrict__ rbg, GeomDev g) {
+    __shared__ __attribute__((aligned(16))) float s_scr[16][256];
+    __shared__ float s_tile[16][16][17];
+    __shared__ float s_carry[8][IMCLT_S * 8];
+    const int plane = blockIdx.z;               // cam * colors + color
+    const int cam = plane / g.colors, color = plane % g.colors;
+    const int bx0 = blockIdx.x * IMCLT_S, by0 = blockIdx.y * IMCLT_R;
+    const int out_w = g.width + 8, out_h = g.height + 8;
+    const int nby = min(IMCLT_R, g.tiles_y + 1 - by0);
+    const int unit = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int hs = l >> 3, kh = l & 7;
+    const int tx = bx0 - 1 + unit;
+    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+    float lp[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) lp[k] = 0.25f * (lpf ? lpf[kh * 8 + k] : 1.f);
+    for (int i = threadIdx.x; i < 8 * IMCLT_S * 8; i += 256) (&s_carry[0][0])[i] = 0.f;
+    float* out = rbg + (size_t)plane * out_w * out_h;
+    for (int ty = by0 - 1; ty < by0 + nby; ty++) {
+        bool valid = (ty >= 0) && (ty < g.tiles_y) && (tx >= 0) && (tx < g.tiles_x) &&
+                     (tile_map[(size_t)(ty < 0 ? 0 : ty) * g.tiles_x + (tx < 0 ? 0 : tx)] >= 0);
+        float p[8], q[8];
+        if (valid) {
+            const float* src = clt + (((size_t)cam * ntiles + (size_t)ty * g.tiles_x + tx) * g.colors + color) * 256 + kh * 8;
+            float4 v0 = *reinterpret_cast<const float4*>(src + hs * 64);
+            float4 v1 = *reinterpret_cast<const float4*>(src + hs * 64 + 4);
+            float4 v2 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64);
+            float4 v3 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64 + 4);
+            p[0] = v0.x; p[1] = v0.y; p[2] = v0.z; p[3] = v0.w; p[4] = v1.x; p[5] = v1.y; p[6] = v1.z; p[7] = v1.w;
+            q[0] = v2.x; q[1] = v2.y; q[2] = v2.z; q[3] = v2.w; q[4] = v3.x; q[5] = v3.y; q[6] = v3.z; q[7] = v3.w;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                p[k] *= lp[k];
+                q[k] *= lp[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) p[k] = q[k] = 0.f;
+        }
+        float row[16];
+        imclt_unit(l, p, q, s_scr[unit], row);
+#pragma unroll
+        for (int c = 0; c < 16; c++) s_tile[unit][l][c] = row[c];
+        __syncthreads();
+        // assemble: output col (within strip) = s*8+c from tile s+1 (left half) and tile s (right half)
+        if (threadIdx.x < IMCLT_S * 8) {
+            const int colx = threadIdx.x, s = colx >> 3, c = colx & 7;
+            const int x = (bx0 + s) * 8 + c;
+            const bool wr = (ty >= by0) && (x < out_w);
+#pragma unroll
+            for (int rr = 0; rr < 8; rr++) {
+                float top = s_tile[s + 1][rr][c] + s_tile[s][rr][8 + c];
+                float bot = s_tile[s + 1][rr + 8][c] + s_tile[s][rr + 8][8 + c];
+                if (wr) out[(size_t)(ty * 8 + rr) * out_w + x] = s_carry[rr][colx] + top;
+                s_carry[rr][colx] = bot;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ====================================================================================
+// correlate2D: pairwise phase correlation (Correlation2d.java:197-389), combined strip
+// (C2:556-790) and per-tile disparity (ImageDtt.java:2186-2238).
+// Block: 4 waves, TPB tiles; a wave handles one (tile, pair) at a time, lane = frequency bin.
+// ====================================================================================
+constexpr int CORR_MAX_STRIP_PAIRS = 8;
+
+struct CorrArgs {
+    int num_pairs;
+    int cam1[CLT_MAX_PAIRS];
+    int cam2[CLT_MAX_PAIRS];
+    int strip_slot[CLT_MAX_PAIRS];             // index among the strip pairs or -1
+    int num_strip;                             // pairs with a known orientation (<= 8)
+    int strip_dir[CORR_MAX_STRIP_PAIRS];
+    int strip_ss[CORR_MAX_STRIP_PAIRS];
+    int strip_is_hor1[CORR_MAX_STRIP_PAIRS];   // dir 0 and scale 1 (getMaskHorizontal(1))
+    int strip_is_vert1[CORR_MAX_STRIP_PAIRS];  // dir 1 and scale 1 (getMaskVertical(1))
+    float col_w[3];
+    float fat_zero, corr_offset, min_corr;
+    int twice_diagonal, strip_height;
+    float wndy[2], wndx[9], notch[15];
+};
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// LDS floats needed by k_corr2d for TPB tiles
+__host__ __device__ inline int corr_lds_floats(int tpb, int cams, int colors) {
+    int a = tpb * cams * colors * 256, b = tpb * CORR_MAX_STRIP_PAIRS * 240;
+    return (a > b ? a : b) + 4 * 512 + tpb * CORR_MAX_STRIP_PAIRS * CLT_CORR_LEN;
+}
+
+template <int TPB>
+__global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+                                                int task_words, int num_tasks, const float* __restrict__ lpf,
+                                                const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
+                                                float* __restrict__ combo_out, float* __restrict__ disp_out, GeomDev g) {
+    extern __shared__ __attribute__((aligned(16))) float s_dyn[];
+    __shared__ int s_task[TPB], s_valid[TPB];
+    const CorrArgs& A = *args_p;
+    const int C = g.cams, K = g.colors, P = A.num_pairs;
+    const int tile_floats = C * K * 256;
+    const int region1 = max(TPB * tile_floats, TPB * CORR_MAX_STRIP_PAIRS * 240);
+    float* s_clt = s_dyn;                                   // [TPB][C][K][256]   (phase 1-2)
+    float* s_strip = s_dyn;                                 // [TPB][8][240]      (phase 3, aliases s_clt)
+    float* s_wave = s_dyn + region1;                        // [4][512]
+    float* s_corr = s_wave + 4 * 512;                       // [TPB][8][225]
+    const float qnan = __int_as_float(0x7fc00000);
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+
+    if (threadIdx.x < TPB) {
+        const int t = blk * TPB + threadIdx.x;
+        s_task[threadIdx.x] = t;
+        s_valid[threadIdx.x] = (t < num_tasks) ? (tasks[(size_t)t * task_words] != 0) : 0;
+    }
+    __syncthreads();
+    // ---- stage the CLT of the block's tiles (all cameras and colours) in LDS
+#pragma unroll
+    for (int j = 0; j < TPB; j++) {
+        if (!s_valid[j]) continue;
+        const int txy = tasks[(size_t)s_task[j] * task_words + 1];
+        const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
+        for (int i = threadIdx.x; i < tile_floats / 4; i += 256) {
+            const int cam = i / (K * 64), w = i % (K * 64);
+            float4 v = *reinterpret_cast<const float4*>(clt + ((size_t)cam * ntiles + tile) * K * 256 + w * 4);
+            *reinterpret_cast<float4*>(s_clt + j * tile_floats + cam * K * 256 + w * 4) = v;
+        }
+    }
+    // per-lane DCT-IIe / DST-IIe rows: pass 1 uses row y = lane >> 3, pass 2 row x = lane & 7
+    const int y = lane >> 3, xx = lane & 7;
+    float cy_[8], sy_[8], cx_[8], sx_[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        cy_[k] = c_tab.ciie[y][k];
+        sy_[k] = c_tab.siie[y][k];
+        cx_[k] = c_tab.ciie[xx][k];
+        sx_[k] = c_tab.siie[xx][k];
+    }
+    const float lp = lpf[lane];
+    __syncthreads();
+
+    float* S0 = s_wave + wave * 512;
+    float* S1 = S0 + 256;
+    for (int u = wave; u < TPB * P; u += 4) {
+        const int j = u / P, p = u % P;
+        const int t = s_task[j];
+        if (t >= num_tasks) continue;
+        float* go = corr_out + ((size_t)t * P + p) * CLT_CORR_LEN;
+        if (!s_valid[j]) {
+            for (int o = lane; o < CLT_CORR_LEN; o += 64) go[o] = qnan;
+            continue;
+        }
+        const float* c1 = s_clt + j * tile_floats + A.cam1[p] * K * 256;
+        const float* c2 = s_clt + j * tile_floats + A.cam2[p] * K * 256;
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int col = 0; col < K; col++) {
+            float d1[4], d2[4];
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                d1[n] = c1[col * 256 + n * 64 + lane];
+                d2[n] = c2[col * 256 + n * 64 + lane];
+            }
+            const float s1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] + d1[3] * d1[3];
+            const float s2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2] + d2[3] * d2[3];
+            const float a2 = sqrtf(s1 * s2);
+            float fz2 = 0.f;
+            if (A.fat_zero != 0.f) fz2 = wave_sum(a2) * (1.f / 64.f) * A.fat_zero * A.fat_zero;
+            const float scale = (1.f / (a2 + fz2)) * (K == 1 ? 1.f : A.col_w[col]);
+            // product with the ZI sign/index table (Correlation2d.java:86-90)
+            const float t0 = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2] + d1[3] * d2[3];
+            const float t1 = -d1[1] * d2[0] + d1[0] * d2[1] - d1[3] * d2[2] + d1[2] * d2[3];
+            const float t2 = -d1[2] * d2[0] - d1[3] * d2[1] + d1[0] * d2[2] + d1[1] * d2[3];
+            const float t3 = d1[3] * d2[0] - d1[2] * d2[1] - d1[1] * d2[2] + d1[0] * d2[3];
+            acc[0] += t0 * scale;
+            acc[1] += t1 * scale;
+            acc[2] += t2 * scale;
+            acc[3] += t3 * scale;
+        }
+#pragma unroll
+        for (int n = 0; n < 4; n++) S0[n * 64 + lane] = acc[n] * lp;
+        wave_sync();
+        // pass 1 along kv (contiguous index): lane (y, kh); quadrant q uses DST-IIe if q bit 1
+        {
+            const int khh = lane & 7;
+            float tq[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float* src = S0 + q * 64 + khh * 8;
+                float s = 0.f;
+#pragma unroll
+                for (int k = 0; k < 8; k++) s = fmaf((q & 2) ? sy_[k] : cy_[k], src[k], s);
+                tq[q] = s;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) S1[q * 64 + y * 8 + khh] = tq[q];
+        }
+        wave_sync();
+        // pass 2 along kh: lane (y, x); quadrant q uses DST-IIe if q bit 0
+        {
+            float oq[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float* src = S1 + q * 64 + y * 8;
+                float s = 0.f;
+#pragma unroll
+                for (int k = 0; k < 8; k++) s = fmaf((q & 1) ? sx_[k] : cx_[k], src[k], s);
+                oq[q] = s;
+            }
+            wave_sync();
+#pragma unroll
+            for (int q = 0; q < 4; q++) S0[q * 64 + lane] = oq[q];
+        }
+        wave_sync();
+        // unfold the 4 quadrants to 15x15 (DttRad2.java:443-489)
+        const int slot = A.strip_slot[p];
+        for (int o = lane; o < CLT_CORR_LEN; o += 64) {
+            const int row = o / 15, cc = o % 15;
+            const int dy = row - 7, dx = cc - 7;
+            const int ay = dy < 0 ? -dy : dy, ax = dx < 0 ? -dx : dx;
+            float v = S0[ay * 8 + ax];
+            if (ax > 0) v += (dx > 0 ? 1.f : -1.f) * S0[64 + ay * 8 + ax - 1];
+            if (ay > 0) v += (dy > 0 ? 1.f : -1.f) * S0[128 + (ay - 1) * 8 + ax];
+            if (ax > 0 && ay > 0) v += ((dx > 0) == (dy > 0) ? 1.f : -1.f) * S0[192 + (ay - 1) * 8 + ax - 1];
+            v *= 0.25f;
+            go[o] = v;
+            if (slot >= 0) s_corr[(j * CORR_MAX_STRIP_PAIRS + slot) * CLT_CORR_LEN + o] = v;
+        }
+        wave_sync();
+    }
+    __syncthreads();   // s_clt dead from here; s_strip aliases it
+    if (A.num_strip == 0 || wave >= TPB) return;
+    // ---- strips, combination and disparity: one wave per tile
+    const int j = wave;
+    const int t = s_task[j];
+    if (t >= num_tasks) return;
+    const int hw = A.strip_height, slen = hw * 15;
+    if (!s_valid[j]) {
+        if (combo_out)
+            for (int e = lane; e < slen; e += 64) combo_out[(size_t)t * slen + e] = qnan;
+        if (lane == 0 && disp_out) {
+            float* d = disp_out + (size_t)t * CLT_DISP_LAYERS;
+            for (int i = 0; i < CLT_DISP_LAYERS; i++) d[i] = qnan;
+            d[CLT_DISP_STRENGTH] = d[CLT_DISP_HOR_STR] = d[CLT_DISP_VERT_STR] = d[CLT_DISP_CM_WEIGHT] = 0.f;
+        }
+        return;
+    }
+    float* sst = s_strip + j * CORR_MAX_STRIP_PAIRS * 240;
+    const float* sc = s_corr + j * CORR_MAX_STRIP_PAIRS * CLT_CORR_LEN;
+    // scaleRotateInterpoateSingleCorrelation (C2:610-718)
+    for (int sidx = 0; sidx < A.num_strip; sidx++) {
+        const int dir = A.strip_dir[sidx], ss = A.strip_ss[sidx];
+        const int denom = ss * ((dir > 1) ? 1 : 2);
+        const float rdenom = (float)denom;
+        const int ilimit = 7 * denom;
+        const float* corr = sc + sidx * CLT_CORR_LEN;
+        for (int e = lane; e < slen; e += 64) {
+            const int row = e / 15, scol = e % 15 - 7;
+            float acc = 0.f;
+            bool nan = false;
+            for (int down = -1; down < ((row == 0) ? 0 : 2); down += 2) {
+                int xnum, ynum;
+                switch (dir) {
+                case 0: xnum = 2 * scol + (row & 1); ynum = down * row; break;
+                case 1: xnum = down * row; ynum = 2 * scol + (row & 1); break;
+                case 2: xnum = scol - ((down * row) >> 1); ynum = scol + ((down * row + 1) >> 1); break;
+                default: xnum = scol - ((down * row) >> 1); ynum = -scol - ((down * row + 1) >> 1); break;
+                }
+                if ((xnum > ilimit) || (xnum < -ilimit) || (ynum > ilimit) || (ynum < -ilimit)) {
+                    nan = true;   // the reference stores NaN there; a later += keeps it NaN
+                    continue;
+                }
+                const int ix0 = (xnum + ilimit) / denom, iy0 = (ynum + ilimit) / denom;
+                const bool int_x = (xnum % denom) == 0, int_y = (ynum % denom) == 0;
+                float d;
+                if (int_x && int_y) {
+                    d = corr[iy0 * 15 + ix0];
+                } else {
+                    const float dx = (xnum + ilimit) / rdenom - ix0;
+                    const float dy = (ynum + ilimit) / rdenom - iy0;
+                    if (int_y) d = (1.f - dx) * corr[iy0 * 15 + ix0] + dx * corr[iy0 * 15 + ix0 + 1];
+                    else if (int_x) d = (1.f - dy) * corr[iy0 * 15 + ix0] + dy * corr[iy0 * 15 + ix0 + 15];
+                    else
+                        d = ((1.f - dx) * (1.f - dy) * corr[iy0 * 15 + ix0]) + (dx * (1.f - dy) * corr[iy0 * 15 + ix0 + 1]) +
+                            ((1.f - dx) * dy * corr[iy0 * 15 + ix0 + 15]) + (dx * dy * corr[iy0 * 15 + ix0 + 16]);
+                }
+                acc = (row == 0) ? d : acc + 0.5f * d;
+            }
+            sst[sidx * 240 + e] = nan ? qnan : acc;
+        }
+    }
+    wave_sync();
+    // combineInterpolatedCorrelations (C2:730-790); NaN samples stay NaN
+    float* combo = s_wave + wave * 512;   // this wave's scratch is free now
+    {
+        const int ncomb = A.num_strip * (A.twice_diagonal ? 2 : 1);
+        for (int e = lane; e < slen; e += 64) {
+            float v;
+            if (A.corr_offset >= 0.f) {
+                v = 1.f;
+                for (int sidx = 0; sidx < A.num_strip; sidx++) {
+                    float d = sst[sidx * 240 + e] + A.corr_offset;
+                    if (d < 0.f) d = 0.f;
+                    v *= A.twice_diagonal ? d * d : d;
+                }
+                if (v > 0.f) v = powf(v, 1.f / ncomb) - A.corr_offset;
+            } else {
+                v = 0.f;
+                for (int sidx = 0; sidx < A.num_strip; sidx++) v += (A.twice_diagonal ? 2.f : 1.f) * sst[sidx * 240 + e];
+                v *= 1.f / ncomb;
+            }
+            combo[e] = v;
+            if (combo_out) combo_out[(size_t)t * slen + e] = v;
+        }
+    }
+    wave_sync();
+    // getMaxXSOrtho (C2:2502-2651) 1-D profiles: lanes 0..14 horizontal pairs, 32..46 vertical
+    float* c1d = combo + 256;   // [2][16]
+    {
+        const int which = lane >> 5, jj = lane & 31;
+        if (jj < 15) {
+            int npairs = 0;
+            for (int sidx = 0; sidx < A.num_strip; sidx++) npairs += which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx];
+            float v = 0.f;
+            if (npairs > 0) {
+                const float w = 1.f / npairs;
+                for (int i = 0; i < 15; i++) {
+                    if (A.corr_offset >= 0.f) {
+                        float cmb = 1.f;
+                        for (int sidx = 0; sidx < A.num_strip; sidx++) {
+                            if (!(which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx])) continue;
+                            const float* c = sc + sidx * CLT_CORR_LEN;
+                            const float d = which ? 0.5f * (c[jj * 15 + i] + c[jj * 15 + 14 - i])
+                                                  : 0.5f * (c[i * 15 + jj] + c[(14 - i) * 15 + jj]);
+                            cmb *= d + A.corr_offset;
+                        }
+                        cmb = (cmb > 0.f) ? powf(cmb, w) - A.corr_offset : -A.corr_offset;
+                        v += cmb * A.notch[i];
+                    } else {
+                        for (int sidx = 0; sidx < A.num_strip; sidx++) {
+                            if (!(which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx])) continue;
+                            const float* c = sc + sidx * CLT_CORR_LEN;
+                            v += w * (which ? c[jj * 15 + i] : c[i * 15 + jj]) * A.notch[i];
+                        }
+                    }
+                }
+            }
+            c1d[which * 16 + jj] = v;
+        }
+    }
+    wave_sync();
+    if (lane == 0 && disp_out) {
+        float res[CLT_DISP_LAYERS];
+        for (int i = 0; i < CLT_DISP_LAYERS; i++) res[i] = qnan;
+        res[CLT_DISP_STRENGTH] = res[CLT_DISP_HOR_STR] = res[CLT_DISP_VERT_STR] = res[CLT_DISP_CM_WEIGHT] = 0.f;
+        // getMaxXYInt on the strip axis (C2:802-843)
+        int imx = 0;
+        for (int i = 1; i < 15; i++)
+            if (isnan(combo[imx]) || (combo[i] > combo[imx])) imx = i;
+        if (combo[imx] >= A.min_corr) {
+            const int ix = imx - 7;
+            res[CLT_DISP_INT] = (float)(-ix);
+            res[CLT_DISP_STRENGTH] = combo[ix + 7];
+            // getMaxXCm (C2:936-1032): 2-row window_y, 9-sample window_x
+            int dh = hw;
+            float wy_scale = 1.f;
+            if (dh > 2) dh = 2;
+            else if (dh < 2) wy_scale = 1.f / A.wndy[0];
+            float s0 = 0.f, sx = 0.f, sx2 = 0.f;
+            const int x0 = 7 + ix;
+            for (int dy = 0; dy < dh; dy++) {
+                const int odd = dy & 1;
+                const float wy = ((dy == 0) ? wy_scale : 2.f * wy_scale) * A.wndy[dy];
+                for (int adx = odd; adx < 9; adx += 2) {
+                    for (int dir = (adx == 0) ? 1 : -1; dir <= 1; dir += 2) {
+                        const int idx = (adx * dir) >> 1;
+                        const int xq = 2 * idx + odd, x1 = x0 + idx;
+                        if (x1 >= 0 && x1 < 15) {
+                            float d = combo[15 * dy + x1];
+                            if (!isnan(d) && d > 0.f) {
+                                d *= wy * A.wndx[adx];
+                                s0 += d;
+                                sx += d * xq;
+                                sx2 += d * xq * xq;
+                            }
+                        }
+                    }
+                }
+            }
+            if (s0 != 0.f) {
+                res[CLT_DISP_CM] = -(ix + sx / s0 / 2.f);
+                res[CLT_DISP_CM_WEIGHT] = s0;
+            }
+        }
+        for (int which = 0; which < 2; which++) {
+            int npairs = 0;
+            for (int sidx = 0; sidx < A.num_strip; sidx++) npairs += which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx];
+            if (npairs == 0) continue;
+            const float* v = c1d + which * 16;
+            int ic = 0;
+            for (int i = 1; i < 15; i++)
+                if (v[i] > v[ic]) ic = i;
+            float xc = (float)ic;
+            if (ic > 0 && ic < 14) {   // 3-point parabola (PolynomialApproximation, degree 2)
+                const float a = v[ic - 1], b = v[ic], c = v[ic + 1];
+                const float den = a - 2.f * b + c;
+                if (den != 0.f) xc = ic + (a - c) / (2.f * den);
+            }
+            const int icr = (int)floorf(xc + 0.5f);
+            res[which ? CLT_DISP_VERT : CLT_DISP_HOR] = -(xc - 7.f);
+            res[which ? CLT_DISP_VERT_STR : CLT_DISP_HOR_STR] = v[icr] / 8.f;
+        }
+        float* dout = disp_out + (size_t)t * CLT_DISP_LAYERS;
+        for (int i = 0; i < CLT_DISP_LAYERS; i++) dout[i] = res[i];
+    }
+}
+
+// ====================================================================================
+// textures: per tile RGBA (+ port weights, rms) from all cameras (ImageDtt.java:2662-2812,
+// tile_debayer_shot_corr 3022-3147, tile_combine_rgba 2830-3000).  Block (256 threads) = tile,
+// thread = pixel of the 16x16 tile in the pixel stages.  NC = number of cameras.
+// ====================================================================================
+struct TexArgs {
+    float col_w[3];
+    float min_shot, scale_shot, ksigma, corr_magic_scale;
+    int dust_remove, planes;
+    float port_offsets[CLT_MAX_CAMS][2];
+};
+
+__device__ __forceinline__ float block_max(float v, float* s_red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+}
+__device__ __forceinline__ float block_sum(float v, float* s_red) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+}
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+                                                  int task_words, int num_tasks, const float* __restrict__ lpf,
+                                                  const float* __restrict__ extra_disp, const TexArgs* __restrict__ args_p,
+                                                  float* __restrict__ tiles_out, float* __restrict__ stats_out, GeomDev g) {
+    constexpr int NU = NC * 3;   // camera/colour units
+    __shared__ __attribute__((aligned(16))) float s_scr[16][256];
+    __shared__ float s_iclt[NU][256];
+    __shared__ float s_nw[NU][256];
+    __shared__ float s_red[4];
+    const TexArgs& A = *args_p;
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    if (t >= num_tasks) return;
+    const int32_t* rec = tasks + (size_t)t * task_words;
+    const int flags = rec[0];
+    const int planes = A.planes;
+    float* out = tiles_out + (size_t)t * planes * 256;
+    const int i = threadIdx.x;   // pixel index in the 16x16 tile
+    if (flags == 0) {
+        const float qnan = __int_as_float(0x7fc00000);
+        for (int p = 0; p < planes; p++) out[p * 256 + i] = qnan;
+        if (stats_out && i < 4 * NC) stats_out[(size_t)t * 4 * NC + i] = qnan;
+        return;
+    }
+    const int port_mask = flags & 0xff & ((1 << NC) - 1);
+    const int txy = rec[1];
+    const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
+    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+    float extra = extra_disp ? extra_disp[t] : 0.f;
+    if (isnan(extra)) extra = 0.f;
+    const bool do_shift = (extra != 0.f) && !((flags >> 24) & 1);
+    // ---- IMCLT of the NC*3 units (16 lanes each) after LPF * 0.25 (ImageDtt.java:2684-2716)
+    {
+        const int unit = threadIdx.x >> 4, l = threadIdx.x & 15, hs = l >> 3, kh = l & 7;
+        for (int ub = 0; ub < NU; ub += 16) {
+            const int uu = ub + unit;
+            const bool act = uu < NU;
+            const int cam = act ? uu / 3 : 0, color = act ? uu % 3 : 0;
+            const float* src = clt + (((size_t)cam * ntiles + tile) * 3 + color) * 256 + kh * 8;
+            float p[8], q[8];
+            {
+                float4 v0 = *reinterpret_cast<const float4*>(src + hs * 64);
+                float4 v1 = *reinterpret_cast<const float4*>(src + hs * 64 + 4);
+                float4 v2 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64);
+                float4 v3 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64 + 4);
+                p[0] = v0.x; p[1] = v0.y; p[2] = v0.z; p[3] = v0.w; p[4] = v1.x; p[5] = v1.y; p[6] = v1.z; p[7] = v1.w;
+                q[0] = v2.x; q[1] = v2.y; q[2] = v2.z; q[3] = v2.w; q[4] = v3.x; q[5] = v3.y; q[6] = v3.z; q[7] = v3.w;
+            }
+            if (do_shift) {   // fract_shift by extra * port_offset / corr_magic_scale (ID:2663-2677)
+                const float sxp = extra * A.port_offsets[cam][0] / A.corr_magic_scale;
+                const float syp = extra * A.port_offsets[cam][1] / A.corr_magic_scale;
+                const float sigma = hs ? 1.f : -1.f;
+                float sh, ch, cv[8], sv[8];
+                sincospif((kh + 0.5f) * sxp * 0.125f, &sh, &ch);
+                half_phase8(syp * 0.125f, cv, sv);
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const float pa = __shfl_xor(p[k], 8, 16) * sigma;
+                    const float pb = __shfl_xor(q[k], 8, 16) * sigma;
+                    const float ta = p[k] * ch + pa * sh;
+                    const float tb = q[k] * ch + pb * sh;
+                    p[k] = ta * cv[k] - tb * sv[k];
+                    q[k] = tb * cv[k] + ta * sv[k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float f = 0.25f * lpf[kh * 8 + k];
+                p[k] *= f;
+                q[k] *= f;
+            }
+            float row[16];
+            imclt_unit(l, p, q, s_scr[unit], row);
+            if (act) {
+#pragma unroll
+                for (int c = 0; c < 16; c++) s_iclt[uu][l * 16 + c] = row[c];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- un-apply the squared window, debayer, shot-noise correction, re-apply (ID:3022-3045)
+    const int py = i >> 4, px = i & 15;
+    const float wwy = c_tab.hw[py < 8 ? py : 15 - py], wwx = c_tab.hw[px < 8 ? px : 15 - px];
+    const float w2 = (wwy * wwx) * (wwy * wwx);
+#pragma unroll
+    for (int uu = 0; uu < NU; uu++) s_nw[uu][i] = s_iclt[uu][i] / w2;
+    __syncthreads();
+    float deb[NC][3];
+    {
+        const bool top = py == 0, bot = py == 15, lft = px == 0, rgt = px == 15;
+        const bool corner = (top || bot) && (lft || rgt), side = (top || bot || lft || rgt) && !corner;
+        const float kshot = 1.f / sqrtf(A.min_shot);
+#pragma unroll
+        for (int uu = 0; uu < NU; uu++) {
+            const bool rb = (uu % 3) != 2;
+            float s = 0.f;
+#pragma unroll
+            for (int dy = -1; dy <= 1; dy++) {
+#pragma unroll
+                for (int dx = -1; dx <= 1; dx++) {
+                    if ((dy < 0 && top) || (dy > 0 && bot) || (dx < 0 && lft) || (dx > 0 && rgt)) continue;
+                    // kern_rb / kern_g (ImageDtt.java:43-50)
+                    const float k = rb ? ((dy == 0 ? 2.f : 1.f) * (dx == 0 ? 2.f : 1.f) * 0.0625f)
+                                       : ((dy == 0 && dx == 0) ? 0.5f : ((dy == 0) != (dx == 0) ? 0.125f : 0.f));
+                    if (k != 0.f) s += s_nw[uu][i + dy * 16 + dx] * k;
+                }
+            }
+            if (corner) s *= rb ? (16.f / 9.f) : (4.f / 3.f);
+            else if (side) s *= rb ? (4.f / 3.f) : (8.f / 7.f);
+            if (A.scale_shot > 0.f) s = A.scale_shot * ((s > A.min_shot) ? sqrtf(s) : (kshot * s));
+            deb[uu / 3][uu % 3] = s * w2;
+        }
+    }
+    // ---- tile_combine_rgba per pixel (ImageDtt.java:2830-3000); keep_weights rms first
+    float rms[4];
+    {   // s0*s2 - s1^2 cancels badly in float: accumulate in double
+        double sw = 0.0;
+#pragma unroll
+        for (int chn = 0; chn < 3; chn++) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++)
+                if ((port_mask >> ip) & 1) {
+                    const double v = deb[ip][chn];
+                    s0 += 1.0;
+                    s1 += v;
+                    s2 += v * v;
+                }
+            const double r = sqrt(s0 * s2 - s1 * s1) / s0;
+            rms[chn] = (float)r;
+            sw += A.col_w[chn] * r * r;
+        }
+        rms[3] = (float)sqrt(sw);
+    }
+    int used = 0;
+#pragma unroll
+    for (int ip = 0; ip < NC; ip++) used += (port_mask >> ip) & 1;
+    // Port weights are exponentials of squared mismatches that leave the float range for badly
+    // matching ports; they are carried in double like the reference (ImageDtt.java:2883-2950).
+    double pw[NC];
+    float avg[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ip = 0; ip < NC; ip++) pw[ip] = 0.0;
+    if (used > 1) {
+#pragma unroll
+        for (int a = 0; a < NC; a++) {
+#pragma unroll
+            for (int b = a + 1; b < NC; b++) {
+                if (!((port_mask >> a) & 1) || !((port_mask >> b) & 1)) continue;
+                const float dxo = A.port_offsets[b][0] - A.port_offsets[a][0];
+                const float dyo = A.port_offsets[b][1] - A.port_offsets[a][1];
+                const float pd2r = A.ksigma / (dxo * dxo + dyo * dyo);
+                float d = 0.f;
+#pragma unroll
+                for (int chn = 0; chn < 3; chn++) {
+                    const float dc = (deb[a][chn] - deb[b][chn]) / w2;
+                    d += A.col_w[chn] * dc * dc;
+                }
+                const double e = exp(-(double)pd2r * (double)d);
+                pw[a] += e;
+                pw[b] += e;
+            }
+        }
+        // two best ports (written without dynamic register indexing)
+        double bw1 = pw[0];
+        int b1 = 0;
+#pragma unroll
+        for (int ip = 1; ip < NC; ip++)
+            if (pw[ip] > bw1) { bw1 = pw[ip]; b1 = ip; }
+        int b2 = (b1 == 0) ? 1 : 0;
+        double bw2 = (b1 == 0) ? pw[1] : pw[0];
+#pragma unroll
+        for (int ip = 1; ip < NC; ip++)
+            if (ip > b2 && ip != b1 && pw[ip] > bw2) { bw2 = pw[ip]; b2 = ip; }
+        const float w1 = (float)(bw1 / (bw1 + bw2)), wq = 1.f - w1;
+#pragma unroll
+        for (int chn = 0; chn < 3; chn++) {
+            float v1 = 0.f, v2 = 0.f;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++) {
+                if (ip == b1) v1 = deb[ip][chn];
+                if (ip == b2) v2 = deb[ip][chn];
+            }
+            avg[chn] = w1 * v1 + wq * v2;
+        }
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++) {
+            if (!((port_mask >> ip) & 1)) continue;
+            float d2 = 0.f;
+#pragma unroll
+            for (int chn = 0; chn < 3; chn++) {
+                const float dc = (deb[ip][chn] - avg[chn]) / w2;
+                d2 += A.col_w[chn] * dc * dc;
+            }
+            pw[ip] = exp(-(double)A.ksigma * (double)d2);
+        }
+        double k = 0.0;
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++) k += pw[ip];
+        k = 1.0 / k;
+#pragma unroll
+        for (int chn = 0; chn < 3; chn++) {
+            float s = 0.f;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++) s += (float)(k * pw[ip]) * deb[ip][chn];
+            avg[chn] = s;
+        }
+    } else if (used > 0) {
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++)
+            if ((port_mask >> ip) & 1) {
+#pragma unroll
+                for (int chn = 0; chn < 3; chn++) avg[chn] = deb[ip][chn];
+                pw[ip] = 1.0;
+            }
+    }
+    if (NC == 4 && A.dust_remove && used == 4) {   // dust_remove (ImageDtt.java:3002-3019)
+        int wi = 0;
+        double ww = pw[0];
+#pragma unroll
+        for (int ip = 1; ip < NC; ip++)
+            if (pw[ip] < ww) { ww = pw[ip]; wi = ip; }
+        double av = 0.0;
+#pragma unroll
+        for (int ip = 1; ip < NC; ip++)
+            if (ip != wi) av += pw[ip];
+        av /= (NC - 1);
+        const double scale = 1.0 + (av - ww) / (av * (NC - 1));
+#pragma unroll
+        for (int ip = 1; ip < NC; ip++)
+            if (ip != wi) pw[ip] *= scale;
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++)
+            if (ip == wi) pw[ip] *= ww / av;
+    }
+    double alpha = 0.0;
+#pragma unroll
+    for (int ip = 0; ip < NC; ip++)
+        if ((port_mask >> ip) & 1) alpha += pw[ip];
+    alpha *= (double)w2 / used;
+    // final mix of the non-debayered tiles with the port weights (ImageDtt.java:2789-2801)
+    {
+        double sw = 0.0;
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++) sw += pw[ip];
+        if (sw != 0.0) sw = 1.0 / sw;
+#pragma unroll
+        for (int chn = 0; chn < 3; chn++) {
+            float s = 0.f;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++) s += (float)(sw * pw[ip]) * s_iclt[ip * 3 + chn][i];
+            out[chn * 256 + i] = s;
+        }
+        out[3 * 256 + i] = (float)alpha;
+        if (planes > 4) {
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++) out[(4 + ip) * 256 + i] = (float)pw[ip];
+#pragma unroll
+            for (int k = 0; k < 4; k++) out[(4 + NC + k) * 256 + i] = rms[k];
+        }
+    }
+    if (stats_out) {   // max_diff (ImageDtt.java:2965-2981) and ports_rgb (ImageDtt.java:2982-2996)
+        float* st = stats_out + (size_t)t * 4 * NC;
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++) {
+            float d2 = 0.f;
+            if ((port_mask >> ip) & 1) {
+#pragma unroll
+                for (int chn = 0; chn < 3; chn++) {
+                    const float dc = deb[ip][chn] - avg[chn];
+                    d2 += dc * dc * A.col_w[chn];
+                }
+                d2 *= w2;
+            }
+            const float m = block_max(d2, s_red);
+            if (i == 0) st[ip] = sqrtf(m);
+        }
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++)
+#pragma unroll
+            for (int chn = 0; chn < 3; chn++) {
+                const float s = block_sum(deb[ip][chn], s_red);
+                if (i == 0) st[NC + chn * NC + ip] = s / 256.f;
+            }
+    }
+}
+
+// ====================================================================================
+// combine textures: overlap-add of the texture tiles (ImageDtt.java:3858-3989, overlap,
+// sharp_alpha = false), gather form: thread = output pixel.
+// ====================================================================================
+__global__ __launch_bounds__(256) void k_combine_textures(const float* __restrict__ tiles, const int32_t* __restrict__ tile_map,
+                                                          int planes, float* __restrict__ out, GeomDev g) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= g.width || y >= g.height) return;
+    const int tyh = (y + 4) >> 3, txh = (x + 4) >> 3;
+    const int iy = (y + 4) & 7, ix = (x + 4) & 7;
+    int tsk[4], off[4];
+    int n = 0;
+#pragma unroll
+    for (int a = 0; a < 2; a++) {
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int ty = tyh - a, tx = txh - b;
+            if (ty < 0 || ty >= g.tiles_y || tx < 0 || tx >= g.tiles_x) continue;
+            const int tk = tile_map[(size_t)ty * g.tiles_x + tx];
+            if (tk < 0) continue;
+            tsk[n] = tk;
+            off[n] = (iy + 8 * a) * 16 + ix + 8 * b;
+            n++;
+        }
+    }
+    const size_t plane_sz = (size_t)g.width * g.height;
+    for (int p = 0; p < planes; p++) {
+        float s = 0.f;
+        for (int k = 0; k < n; k++) s += tiles[((size_t)tsk[k] * planes + p) * 256 + off[k]];
+        out[p * plane_sz + (size_t)y * g.width + x] = s;
+    }
+}
+
+}  // namespace clt

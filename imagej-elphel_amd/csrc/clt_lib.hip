@@ -1,0 +1,723 @@
+// clt_lib.hip -- C ABI of the MI355X CLT tile processor (include/clt_tile_processor.h).
+//
+// Host side of the drop-in for the reference's GPUTileProcessor (GPUTileProcessor.java): device
+// buffers owned by a handle, H2D/D2H copies, kernel launches on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/clt_tile_processor.h"
+#include "clt_kernels.hpp"
+
+namespace clt {
+__constant__ Tables c_tab;
+}
+using namespace clt;
+
+// ------------------------------------------------------------------------ host math helpers
+// Closed-form tables (DttRad2.java:671-694, 952-1012)
+static void make_tables(Tables* t) {
+    for (int i = 0; i < 8; i++) t->hw[i] = (float)std::sin(M_PI / 16.0 * (i + 0.5));
+    const double sc = std::sqrt(2.0 / 8);
+    for (int j = 0; j < 8; j++)
+        for (int k = 0; k < 8; k++) {
+            t->civ[j][k] = (float)(sc * std::cos((2 * j + 1) * (2 * k + 1) * M_PI / 32.0));
+            t->ciie[j][k] = (float)(sc * std::cos(j * (2 * k + 1) * M_PI / 16.0));
+            t->siie[j][k] = (float)(sc * std::sin((j + 1) * (2 * k + 1) * M_PI / 16.0));
+        }
+}
+
+// setCltLpf (GPUTileProcessor.java:814-848) followed by dttt_iiie * 2n (GTP:788-798), in double.
+static void lpf_coeffs(double sigma, float out[64]) {
+    double fd[64];
+    if (sigma == 0.0) {
+        for (int i = 0; i < 64; i++) fd[i] = (i == 0) ? 1.0 : 0.0;
+    } else {
+        for (int i = 0; i < 8; i++)
+            for (int j = 0; j < 8; j++) fd[i * 8 + j] = std::exp(-(i * i + j * j) / (2 * sigma));
+    }
+    double sum = 0;
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) {
+            double d = fd[i * 8 + j] * std::cos(M_PI * i / 16.0) * std::cos(M_PI * j / 16.0);
+            if (i > 0) d *= 2.0;
+            if (j > 0) d *= 2.0;
+            sum += d;
+        }
+    for (int i = 0; i < 64; i++) fd[i] /= sum;
+    // DCT-IIIe matrix, e_j = 0.5 for j == 0 and j == 7 (DttRad2.java:970-991)
+    double m[8][8];
+    const double sc = std::sqrt(2.0 / 8);
+    for (int k = 0; k < 8; k++)
+        for (int j = 0; j < 8; j++) m[k][j] = sc * ((j == 0 || j == 7) ? 0.5 : 1.0) * std::cos(j * (2 * k + 1) * M_PI / 16.0);
+    double tmp[64], res[64];
+    for (int i = 0; i < 8; i++)
+        for (int k = 0; k < 8; k++) {
+            double s = 0;
+            for (int j = 0; j < 8; j++) s += m[k][j] * fd[i * 8 + j];
+            tmp[k * 8 + i] = s;
+        }
+    for (int i = 0; i < 8; i++)
+        for (int k = 0; k < 8; k++) {
+            double s = 0;
+            for (int j = 0; j < 8; j++) s += m[k][j] * tmp[i * 8 + j];
+            res[i * 8 + k] = s;
+        }
+    for (int i = 0; i < 64; i++) out[i] = (float)(res[i] * 16.0);
+}
+
+// halfFlatTopWindow (Correlation2d.java:1047-1081)
+static void half_flat_top(int n, double hwidth, double blur, bool normalize, bool notch, double scale, float* w) {
+    std::vector<double> d(n);
+    for (int i = 0; i < n; i++) {
+        if (blur <= 0) d[i] = (i < hwidth) ? 1.0 : 0.0;
+        else if (i < hwidth - blur / 2) d[i] = 1.0;
+        else if (i > hwidth + blur / 2) d[i] = 0.0;
+        else d[i] = 0.5 * (1.0 - std::sin(M_PI * (i - hwidth) / blur));
+        if (notch) d[i] = 1.0 - d[i];
+    }
+    if (normalize) {
+        double s = 0;
+        for (int i = 0; i < n; i++) s += ((i == 0) ? 1.0 : 2.0) * d[i];
+        for (int i = 0; i < n; i++) d[i] /= s;
+    }
+    for (int i = 0; i < n; i++) w[i] = (float)(d[i] * scale);
+}
+
+// ------------------------------------------------------------------------ handle
+enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_NUM };
+
+struct clt_handle {
+    clt_geom g{};
+    GeomDev gd{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool async = false, timing = false;
+    int tiles_x = 0, tiles_y = 0, num_tiles = 0, task_words = 0, max_tasks = 0;
+    // device buffers
+    float* d_images = nullptr;     // [C][H][W]
+    float* d_kernels = nullptr;    // [C][KY][KX][K][4][h][v]
+    float* d_koffs = nullptr;      // [C][KY][KX][K][8]
+    int32_t* d_tasks = nullptr;    // [max_tasks][task_words]
+    int32_t* d_tile_map = nullptr; // [tiles] task index or -1
+    float* d_clt = nullptr;        // [C][tiles][K][4][h][v]
+    float* d_lpf_rbg = nullptr;    // [3][64]
+    float* d_lpf_corr = nullptr;   // [64]
+    float* d_rbg = nullptr;        // [C][K][H+8][W+8]
+    float* d_corr = nullptr;       // [max_tasks][P][225]
+    float* d_combo = nullptr;      // [max_tasks][strip_height*15]
+    float* d_disp = nullptr;       // [max_tasks][8]
+    float* d_tex = nullptr;        // [max_tasks][planes][256]
+    float* d_tex_stats = nullptr;  // [max_tasks][4C]
+    float* d_tex_img = nullptr;    // [planes][H][W]
+    float* d_extra = nullptr;      // [max_tasks]
+    CorrArgs* d_corr_args = nullptr;
+    TexArgs* d_tex_args = nullptr;
+    size_t corr_cap = 0, combo_cap = 0, tex_cap = 0, tex_img_cap = 0;
+    // state
+    bool kernels_set[CLT_MAX_CAMS] = {};
+    bool use_kernels = false;
+    bool images_set[CLT_MAX_CAMS] = {};
+    int num_tasks = 0;
+    bool converted = false, corr_done = false, tex_done = false, combined = false;
+    int corr_pairs_at_exec = 0, corr_strip_h_at_exec = 0, tex_planes_at_exec = 0;
+    float corr_sigma = 0.8f;
+    std::vector<clt_pair> pairs;
+    clt_corr_params cp{};
+    clt_tex_params tp{};
+    std::vector<int32_t> h_tile_map;
+    // per kernel: event pairs of the launches since the last timing reset
+    std::vector<hipEvent_t> ev_beg[T_NUM], ev_end[T_NUM];
+    int ev_used[T_NUM] = {};
+};
+
+static int fail(clt_handle* h, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+#define HIPCHK(h, call)                                                                           \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) return fail((h), CLT_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+static int dalloc(clt_handle* h, void** p, size_t bytes) {
+    if (*p) return CLT_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(h, CLT_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return CLT_OK;
+}
+
+static int finish(clt_handle* h) {
+    if (h->async) return CLT_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return CLT_OK;
+}
+
+static void tbegin(clt_handle* h, int which) {
+    if (!h->timing) return;
+    const int u = h->ev_used[which];
+    if (u == (int)h->ev_beg[which].size()) {
+        hipEvent_t b = nullptr, e = nullptr;
+        if (hipEventCreate(&b) != hipSuccess || hipEventCreate(&e) != hipSuccess) return;
+        h->ev_beg[which].push_back(b);
+        h->ev_end[which].push_back(e);
+    }
+    (void)hipEventRecord(h->ev_beg[which][u], h->stream);
+}
+static void tend(clt_handle* h, int which) {
+    if (!h->timing) return;
+    const int u = h->ev_used[which];
+    if (u >= (int)h->ev_end[which].size()) return;
+    (void)hipEventRecord(h->ev_end[which][u], h->stream);
+    h->ev_used[which] = u + 1;
+}
+
+template <int NC>
+static void launch_tex(clt_handle* h, int blocks, const float* extra) {
+    hipLaunchKernelGGL(k_textures<NC>, dim3(blocks), dim3(256), 0, h->stream, h->d_clt, h->d_tasks, h->task_words, h->num_tasks,
+                       h->d_lpf_corr, extra, h->d_tex_args, h->d_tex, h->d_tex_stats, h->gd);
+}
+
+extern "C" {
+
+int clt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+void clt_default_corr_params(clt_corr_params* p) {
+    p->fat_zero = 0.0f;
+    p->corr_red = 0.5f;
+    p->corr_blue = 0.2f;
+    p->corr_offset = 0.1f;
+    p->twice_diagonal = 1;
+    p->min_corr = 0.02f;
+    p->strip_height = 9;
+    p->enhortho_width = 2.0f;
+    p->enhortho_scale = 0.0f;
+}
+
+void clt_default_tex_params(clt_tex_params* p, int32_t num_cams) {
+    std::memset(p, 0, sizeof(*p));
+    p->min_shot = 10.0f;
+    p->scale_shot = 3.0f;
+    p->diff_sigma = 5.0f;
+    p->dust_remove = 1;
+    p->corr_magic_scale = 0.85f;
+    p->keep_weights = 1;
+    if (num_cams == 4) {   // ImageDtt.java:1648-1652
+        const float po[4][2] = {{-0.5f, -0.5f}, {0.5f, -0.5f}, {-0.5f, 0.5f}, {0.5f, 0.5f}};
+        std::memcpy(p->port_offsets, po, sizeof(po));
+    } else {
+        for (int i = 0; i < num_cams && i < CLT_MAX_CAMS; i++) {
+            p->port_offsets[i][0] = (float)(0.5 * std::cos(2 * M_PI * i / num_cams));
+            p->port_offsets[i][1] = (float)(0.5 * std::sin(2 * M_PI * i / num_cams));
+        }
+    }
+}
+
+int clt_default_quad_pairs(clt_pair out[6]) {
+    const clt_pair p[6] = {{0, 1, 0, 1}, {2, 3, 0, 1}, {0, 2, 1, 1}, {1, 3, 1, 1}, {0, 3, 2, 1}, {2, 1, 3, 1}};
+    std::memcpy(out, p, sizeof(p));
+    return 6;
+}
+
+int clt_lpf_coeffs(double sigma, float out[64]) {
+    if (!out || sigma < 0) return CLT_E_ARG;
+    lpf_coeffs(sigma, out);
+    return CLT_OK;
+}
+
+const char* clt_last_error(const clt_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int clt_create(const clt_geom* geom, clt_handle** out) {
+    if (!geom || !out) return CLT_E_ARG;
+    *out = nullptr;
+    const clt_geom& g = *geom;
+    if (g.width <= 16 || g.height <= 16 || (g.width & 7) || (g.height & 7) || g.num_cams < 1 ||
+        g.num_cams > CLT_MAX_CAMS || (g.num_colors != 1 && g.num_colors != 3) || g.red_row < 0 || g.red_row > 1 ||
+        g.red_col < 0 || g.red_col > 1 || g.kernel_step <= 0 || g.kernels_hor < 0 || g.kernels_vert < 0 ||
+        g.width / 8 > 0xffff || g.height / 8 > 0xffff)
+        return CLT_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CLT_E_NODEVICE;
+    clt_handle* h = new clt_handle();
+    h->g = g;
+    (void)hipGetDevice(&h->device);
+    h->tiles_x = g.width / 8;
+    h->tiles_y = g.height / 8;
+    h->num_tiles = h->tiles_x * h->tiles_y;
+    h->task_words = 2 + 2 * g.num_cams;
+    h->max_tasks = g.max_tasks > 0 ? g.max_tasks : h->num_tiles;
+    h->gd = GeomDev{g.width, g.height, g.num_cams, g.num_colors, g.red_row, g.red_col,
+                    g.kernels_hor, g.kernels_vert, g.kernel_step, h->tiles_x, h->tiles_y};
+    clt_default_corr_params(&h->cp);
+    clt_default_tex_params(&h->tp, g.num_cams);
+    h->pairs.resize(0);
+    if (g.num_cams == 4) {
+        clt_pair p[6];
+        clt_default_quad_pairs(p);
+        h->pairs.assign(p, p + 6);
+    }
+    int rc;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return CLT_E_HIP;
+    }
+    Tables tab;
+    make_tables(&tab);
+    const size_t C = g.num_cams, K = g.num_colors;
+    const size_t img = (size_t)g.width * g.height;
+    if ((rc = dalloc(h, (void**)&h->d_images, C * img * 4)) || (rc = dalloc(h, (void**)&h->d_tasks, (size_t)h->max_tasks * h->task_words * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_tile_map, (size_t)h->num_tiles * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_clt, C * h->num_tiles * K * 256 * 4)) || (rc = dalloc(h, (void**)&h->d_lpf_rbg, 3 * 64 * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_lpf_corr, 64 * 4)) || (rc = dalloc(h, (void**)&h->d_corr_args, sizeof(CorrArgs))) ||
+        (rc = dalloc(h, (void**)&h->d_tex_args, sizeof(TexArgs))) || (rc = dalloc(h, (void**)&h->d_extra, (size_t)h->max_tasks * 4))) {
+        std::string e = h->err;
+        clt_destroy(h);
+        return rc;
+    }
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &tab, sizeof(tab)) != hipSuccess ||
+        hipMemset(h->d_images, 0, C * img * 4) != hipSuccess) {
+        clt_destroy(h);
+        return CLT_E_HIP;
+    }
+    h->h_tile_map.assign(h->num_tiles, -1);
+    if (hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice) != hipSuccess) {
+        clt_destroy(h);
+        return CLT_E_HIP;
+    }
+    float lpf[64];
+    lpf_coeffs(h->corr_sigma, lpf);
+    (void)hipMemcpy(h->d_lpf_corr, lpf, sizeof(lpf), hipMemcpyHostToDevice);
+    *out = h;
+    return CLT_OK;
+}
+
+int clt_destroy(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    void* bufs[] = {h->d_images, h->d_kernels, h->d_koffs, h->d_tasks, h->d_tile_map, h->d_clt, h->d_lpf_rbg,
+                    h->d_lpf_corr, h->d_rbg, h->d_corr, h->d_combo, h->d_disp, h->d_tex, h->d_tex_stats,
+                    h->d_tex_img, h->d_extra, h->d_corr_args, h->d_tex_args};
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    for (int i = 0; i < T_NUM; i++) {
+        for (hipEvent_t e : h->ev_beg[i]) (void)hipEventDestroy(e);
+        for (hipEvent_t e : h->ev_end[i]) (void)hipEventDestroy(e);
+    }
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return CLT_OK;
+}
+
+// ------------------------------------------------------------------------ inputs
+int clt_set_kernels(clt_handle* h, int32_t cam, const float* kernels, const float* offsets) {
+    if (!h) return CLT_E_ARG;
+    const clt_geom& g = h->g;
+    if (cam < 0 || cam >= g.num_cams || !kernels || !offsets)
+        return fail(h, CLT_E_ARG, "clt_set_kernels: bad camera %d or null buffer", cam);
+    if (g.kernels_hor <= 0 || g.kernels_vert <= 0) return fail(h, CLT_E_ARG, "clt_set_kernels: geometry has no kernel grid");
+    const size_t nk = (size_t)g.kernels_vert * g.kernels_hor * g.num_colors;
+    const size_t C = g.num_cams;
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_kernels, C * nk * 256 * 4)) || (rc = dalloc(h, (void**)&h->d_koffs, C * nk * 8 * 4))) return rc;
+    // upload layout [v][h] per 8x8 (GPUTileProcessor.java:406-408) -> device [h][v]
+    std::vector<float> tr(nk * 256);
+    for (size_t k = 0; k < nk; k++)
+        for (int q = 0; q < 4; q++)
+            for (int v = 0; v < 8; v++)
+                for (int hh = 0; hh < 8; hh++) tr[k * 256 + q * 64 + hh * 8 + v] = kernels[k * 256 + q * 64 + v * 8 + hh];
+    HIPCHK(h, hipMemcpy(h->d_kernels + cam * nk * 256, tr.data(), nk * 256 * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_koffs + cam * nk * 8, offsets, nk * 8 * 4, hipMemcpyHostToDevice));
+    h->kernels_set[cam] = true;
+    h->use_kernels = true;
+    for (int c = 0; c < g.num_cams; c++) h->use_kernels = h->use_kernels && h->kernels_set[c];
+    return CLT_OK;
+}
+
+int clt_clear_kernels(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    for (int c = 0; c < CLT_MAX_CAMS; c++) h->kernels_set[c] = false;
+    h->use_kernels = false;
+    return CLT_OK;
+}
+
+int clt_set_image(clt_handle* h, int32_t cam, const float* image, int32_t stride) {
+    if (!h) return CLT_E_ARG;
+    const clt_geom& g = h->g;
+    if (cam < 0 || cam >= g.num_cams || !image || stride < g.width)
+        return fail(h, CLT_E_ARG, "clt_set_image: bad camera %d, null image or stride %d < width %d", cam, stride, g.width);
+    HIPCHK(h, hipMemcpy2D(h->d_images + (size_t)cam * g.width * g.height, (size_t)g.width * 4, image, (size_t)stride * 4,
+                          (size_t)g.width * 4, g.height, hipMemcpyHostToDevice));
+    h->images_set[cam] = true;
+    return CLT_OK;
+}
+
+int clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks) {
+    if (!h) return CLT_E_ARG;
+    if (num_tasks < 0 || num_tasks > h->max_tasks || (num_tasks > 0 && !tasks))
+        return fail(h, CLT_E_ARG, "clt_set_tasks: %d tasks (capacity %d)", num_tasks, h->max_tasks);
+    const int32_t* t = (const int32_t*)tasks;
+    std::fill(h->h_tile_map.begin(), h->h_tile_map.end(), -1);
+    for (int i = 0; i < num_tasks; i++) {
+        const int32_t* r = t + (size_t)i * h->task_words;
+        const int tx = r[1] & 0xffff, ty = (r[1] >> 16) & 0xffff;
+        if (tx >= h->tiles_x || ty >= h->tiles_y)
+            return fail(h, CLT_E_ARG, "clt_set_tasks: task %d tile (%d,%d) outside %dx%d", i, tx, ty, h->tiles_x, h->tiles_y);
+        if (r[0] != 0) h->h_tile_map[(size_t)ty * h->tiles_x + tx] = i;
+    }
+    if (num_tasks > 0) HIPCHK(h, hipMemcpy(h->d_tasks, tasks, (size_t)num_tasks * h->task_words * 4, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice));
+    h->num_tasks = num_tasks;
+    h->converted = h->corr_done = h->tex_done = h->combined = false;
+    return CLT_OK;
+}
+
+int clt_set_lpf_rbg(clt_handle* h, float sigma_r, float sigma_b, float sigma_g) {
+    if (!h) return CLT_E_ARG;
+    if (sigma_r < 0 || sigma_b < 0 || sigma_g < 0) return fail(h, CLT_E_ARG, "clt_set_lpf_rbg: negative sigma");
+    float l[3][64];
+    lpf_coeffs(sigma_r, l[0]);
+    lpf_coeffs(sigma_b, l[1]);
+    lpf_coeffs(sigma_g, l[2]);
+    HIPCHK(h, hipMemcpy(h->d_lpf_rbg, l, sizeof(l), hipMemcpyHostToDevice));
+    return CLT_OK;
+}
+
+int clt_set_corr_sigma(clt_handle* h, float sigma) {
+    if (!h) return CLT_E_ARG;
+    if (sigma < 0) return fail(h, CLT_E_ARG, "clt_set_corr_sigma: negative sigma");
+    float l[64];
+    lpf_coeffs(sigma, l);
+    HIPCHK(h, hipMemcpy(h->d_lpf_corr, l, sizeof(l), hipMemcpyHostToDevice));
+    h->corr_sigma = sigma;
+    return CLT_OK;
+}
+
+int clt_set_pairs(clt_handle* h, const clt_pair* pairs, int32_t num_pairs) {
+    if (!h) return CLT_E_ARG;
+    if (num_pairs < 1 || num_pairs > CLT_MAX_PAIRS || !pairs) return fail(h, CLT_E_ARG, "clt_set_pairs: %d pairs", num_pairs);
+    int nstrip = 0;
+    for (int i = 0; i < num_pairs; i++) {
+        const clt_pair& p = pairs[i];
+        if (p.cam1 < 0 || p.cam1 >= h->g.num_cams || p.cam2 < 0 || p.cam2 >= h->g.num_cams || p.dir < -1 || p.dir > 3 ||
+            (p.dir >= 0 && p.scale < 1))
+            return fail(h, CLT_E_ARG, "clt_set_pairs: bad pair %d", i);
+        nstrip += p.dir >= 0;
+    }
+    if (nstrip > CORR_MAX_STRIP_PAIRS) return fail(h, CLT_E_UNSUPPORTED, "clt_set_pairs: %d oriented pairs > %d", nstrip, CORR_MAX_STRIP_PAIRS);
+    h->pairs.assign(pairs, pairs + num_pairs);
+    h->corr_done = false;
+    return CLT_OK;
+}
+
+int clt_set_corr_params(clt_handle* h, const clt_corr_params* p) {
+    if (!h || !p) return CLT_E_ARG;
+    if (p->strip_height < 1 || p->strip_height > 16) return fail(h, CLT_E_ARG, "strip_height %d", p->strip_height);
+    h->cp = *p;
+    return CLT_OK;
+}
+
+int clt_set_tex_params(clt_handle* h, const clt_tex_params* p) {
+    if (!h || !p) return CLT_E_ARG;
+    if (p->diff_sigma <= 0 || p->corr_magic_scale == 0 || p->min_shot <= 0) return fail(h, CLT_E_ARG, "bad texture parameters");
+    h->tp = *p;
+    return CLT_OK;
+}
+
+// ------------------------------------------------------------------------ kernels
+int clt_exec_convert(clt_handle* h, int32_t lpf_mask) {
+    if (!h) return CLT_E_ARG;
+    for (int c = 0; c < h->g.num_cams; c++)
+        if (!h->images_set[c]) return fail(h, CLT_E_STATE, "clt_exec_convert: image of camera %d not set", c);
+    if (h->num_tasks <= 0) return fail(h, CLT_E_STATE, "clt_exec_convert: no tasks");
+    const long units = (long)h->num_tasks * h->g.num_cams * h->g.num_colors;
+    const int blocks = (int)((units + 15) / 16);
+    tbegin(h, T_CONVERT);
+    hipLaunchKernelGGL(k_convert, dim3(blocks), dim3(256), 0, h->stream, h->d_images,
+                       h->use_kernels ? h->d_kernels : nullptr, h->use_kernels ? h->d_koffs : nullptr, h->d_tasks,
+                       h->task_words, h->num_tasks, h->d_clt, lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, h->gd);
+    tend(h, T_CONVERT);
+    HIPCHK(h, hipGetLastError());
+    h->converted = true;
+    h->corr_done = h->tex_done = h->combined = false;
+    return finish(h);
+}
+
+int clt_exec_imclt(clt_handle* h, int32_t apply_lpf) {
+    if (!h) return CLT_E_ARG;
+    if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_imclt: no converted tiles");
+    const clt_geom& g = h->g;
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_rbg, (size_t)g.num_cams * g.num_colors * (g.width + 8) * (g.height + 8) * 4))) return rc;
+    dim3 grid((h->tiles_x + 1 + IMCLT_S - 1) / IMCLT_S, (h->tiles_y + 1 + IMCLT_R - 1) / IMCLT_R, g.num_cams * g.num_colors);
+    tbegin(h, T_IMCLT);
+    hipLaunchKernelGGL(k_imclt, grid, dim3(256), 0, h->stream, h->d_clt, h->d_tile_map, apply_lpf ? h->d_lpf_corr : nullptr,
+                       h->d_rbg, h->gd);
+    tend(h, T_IMCLT);
+    HIPCHK(h, hipGetLastError());
+    return finish(h);
+}
+
+static int ensure_capacity(clt_handle* h, float** p, size_t* cap, size_t floats) {
+    if (*p && *cap >= floats) return CLT_OK;
+    if (*p) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    int rc = dalloc(h, (void**)p, floats * 4);
+    if (rc == CLT_OK) *cap = floats;
+    return rc;
+}
+
+int clt_exec_corr2d(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_corr2d: no converted tiles");
+    if (h->pairs.empty()) return fail(h, CLT_E_STATE, "clt_exec_corr2d: no correlation pairs set");
+    const clt_geom& g = h->g;
+    const int P = (int)h->pairs.size(), hw = h->cp.strip_height;
+    int rc;
+    if ((rc = ensure_capacity(h, &h->d_corr, &h->corr_cap, (size_t)h->max_tasks * P * CLT_CORR_LEN)) ||
+        (rc = ensure_capacity(h, &h->d_combo, &h->combo_cap, (size_t)h->max_tasks * 16 * 15)) ||
+        (rc = dalloc(h, (void**)&h->d_disp, (size_t)h->max_tasks * CLT_DISP_LAYERS * 4)))
+        return rc;
+    CorrArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.num_pairs = P;
+    int ss_min = 1 << 30;
+    for (int i = 0; i < P; i++)
+        if (h->pairs[i].dir >= 0 && h->pairs[i].scale < ss_min) ss_min = h->pairs[i].scale;
+    for (int i = 0; i < P; i++) {
+        const clt_pair& p = h->pairs[i];
+        a.cam1[i] = p.cam1;
+        a.cam2[i] = p.cam2;
+        a.strip_slot[i] = -1;
+        if (p.dir >= 0) {
+            const int s = a.num_strip++;
+            a.strip_slot[i] = s;
+            a.strip_dir[s] = p.dir;
+            a.strip_ss[s] = p.scale / ss_min;
+            a.strip_is_hor1[s] = (p.dir == 0 && p.scale == 1);
+            a.strip_is_vert1[s] = (p.dir == 1 && p.scale == 1);
+        }
+    }
+    if (g.num_colors == 3) {   // ImageDtt.java:1571-1581
+        const double gw = 1.0 / (1.0 + h->cp.corr_red + h->cp.corr_blue);
+        a.col_w[2] = (float)gw;
+        a.col_w[0] = (float)(h->cp.corr_red * gw);
+        a.col_w[1] = (float)(h->cp.corr_blue * gw);
+    } else {
+        a.col_w[0] = 1.f;
+    }
+    a.fat_zero = h->cp.fat_zero;
+    a.corr_offset = h->cp.corr_offset;
+    a.min_corr = h->cp.min_corr;
+    a.twice_diagonal = h->cp.twice_diagonal;
+    a.strip_height = hw;
+    half_flat_top(2, 1.0, 1.0, true, false, 1.0, a.wndy);   // IDP:77-79 corr_wndy_*
+    half_flat_top(9, 6.0, 5.0, true, false, 2.0, a.wndx);   // IDP:80-82 corr_wndx_*, wndx_scale 2.0 (ID:1758)
+    for (int i = 0; i < 15; i++) {                          // createOrtoNotch (C2:2654-2675)
+        double f = ((i < (8 - h->cp.enhortho_width)) || (i > (8 - 2 + h->cp.enhortho_width))) ? 1.0 : h->cp.enhortho_scale;
+        if (i == 7) f = 0.0;
+        a.notch[i] = (float)(f * std::sin(M_PI * (i + 1.0) / 16));
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    const int tpb = (P <= 8) ? 2 : 1;
+    const size_t lds = (size_t)corr_lds_floats(tpb, g.num_cams, g.num_colors) * 4;
+    if (lds > 64 * 1024) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_corr2d: %zu bytes of LDS", lds);
+    const int blocks = (h->num_tasks + tpb - 1) / tpb;
+    tbegin(h, T_CORR);
+    if (tpb == 2)
+        hipLaunchKernelGGL(k_corr2d<2>, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
+                           h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
+    else
+        hipLaunchKernelGGL(k_corr2d<1>, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
+                           h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
+    tend(h, T_CORR);
+    HIPCHK(h, hipGetLastError());
+    h->corr_done = true;
+    h->corr_pairs_at_exec = P;
+    h->corr_strip_h_at_exec = a.num_strip ? hw : 0;
+    int rc2 = finish(h);
+    return rc2;
+}
+
+int clt_exec_textures(clt_handle* h, const float* extra_disparity) {
+    if (!h) return CLT_E_ARG;
+    const clt_geom& g = h->g;
+    if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_textures: no converted tiles");
+    if (g.num_colors != 3 || g.num_cams < 2 || g.num_cams > 8)
+        return fail(h, CLT_E_UNSUPPORTED, "clt_exec_textures: needs a Bayer frame of 2..8 cameras");
+    const int planes = h->tp.keep_weights ? 4 + g.num_cams + 4 : 4;
+    int rc;
+    if ((rc = ensure_capacity(h, &h->d_tex, &h->tex_cap, (size_t)h->max_tasks * planes * 256)) ||
+        (rc = dalloc(h, (void**)&h->d_tex_stats, (size_t)h->max_tasks * 4 * CLT_MAX_CAMS * 4)))
+        return rc;
+    TexArgs a;
+    std::memset(&a, 0, sizeof(a));
+    const double gw = 1.0 / (1.0 + h->cp.corr_red + h->cp.corr_blue);
+    a.col_w[2] = (float)gw;
+    a.col_w[0] = (float)(h->cp.corr_red * gw);
+    a.col_w[1] = (float)(h->cp.corr_blue * gw);
+    a.min_shot = h->tp.min_shot;
+    a.scale_shot = h->tp.scale_shot;
+    a.ksigma = (float)(1.0 / (2.0 * h->tp.diff_sigma * h->tp.diff_sigma));
+    a.corr_magic_scale = h->tp.corr_magic_scale;
+    a.dust_remove = h->tp.dust_remove;
+    a.planes = planes;
+    std::memcpy(a.port_offsets, h->tp.port_offsets, sizeof(a.port_offsets));
+    HIPCHK(h, hipMemcpyAsync(h->d_tex_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    const float* extra = nullptr;
+    if (extra_disparity) {
+        HIPCHK(h, hipMemcpyAsync(h->d_extra, extra_disparity, (size_t)h->num_tasks * 4, hipMemcpyHostToDevice, h->stream));
+        extra = h->d_extra;
+    }
+    tbegin(h, T_TEX);
+    switch (g.num_cams) {
+    case 2: launch_tex<2>(h, h->num_tasks, extra); break;
+    case 3: launch_tex<3>(h, h->num_tasks, extra); break;
+    case 4: launch_tex<4>(h, h->num_tasks, extra); break;
+    case 5: launch_tex<5>(h, h->num_tasks, extra); break;
+    case 6: launch_tex<6>(h, h->num_tasks, extra); break;
+    case 7: launch_tex<7>(h, h->num_tasks, extra); break;
+    default: launch_tex<8>(h, h->num_tasks, extra); break;
+    }
+    tend(h, T_TEX);
+    HIPCHK(h, hipGetLastError());
+    h->tex_done = true;
+    h->tex_planes_at_exec = planes;
+    h->combined = false;
+    return finish(h);
+}
+
+int clt_exec_combine_textures(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    if (!h->tex_done) return fail(h, CLT_E_STATE, "clt_exec_combine_textures: no textures");
+    const clt_geom& g = h->g;
+    const int planes = h->tex_planes_at_exec;
+    int rc;
+    if ((rc = ensure_capacity(h, &h->d_tex_img, &h->tex_img_cap, (size_t)planes * g.width * g.height))) return rc;
+    dim3 grid((g.width + 63) / 64, (g.height + 3) / 4);
+    tbegin(h, T_COMBINE);
+    hipLaunchKernelGGL(k_combine_textures, grid, dim3(256), 0, h->stream, h->d_tex, h->d_tile_map, planes, h->d_tex_img, h->gd);
+    tend(h, T_COMBINE);
+    HIPCHK(h, hipGetLastError());
+    h->combined = true;
+    return finish(h);
+}
+
+// ------------------------------------------------------------------------ outputs
+int clt_get_rbg(clt_handle* h, int32_t cam, float* out) {
+    if (!h || !out) return CLT_E_ARG;
+    if (cam < 0 || cam >= h->g.num_cams) return fail(h, CLT_E_ARG, "clt_get_rbg: camera %d", cam);
+    if (!h->d_rbg) return fail(h, CLT_E_STATE, "clt_get_rbg: imclt not executed");
+    const size_t n = (size_t)h->g.num_colors * (h->g.width + 8) * (h->g.height + 8);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(out, h->d_rbg + cam * n, n * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+int clt_get_clt(clt_handle* h, int32_t cam, float* out) {
+    if (!h || !out) return CLT_E_ARG;
+    if (cam < 0 || cam >= h->g.num_cams) return fail(h, CLT_E_ARG, "clt_get_clt: camera %d", cam);
+    const size_t n = (size_t)h->num_tiles * h->g.num_colors * 256;
+    std::vector<float> tmp(n);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(tmp.data(), h->d_clt + cam * n, n * 4, hipMemcpyDeviceToHost));
+    for (size_t b = 0; b < n / 64; b++)   // device [h][v] -> [v][h] (TileProcessor.cuh:1065-1095 layout)
+        for (int hh = 0; hh < 8; hh++)
+            for (int v = 0; v < 8; v++) out[b * 64 + v * 8 + hh] = tmp[b * 64 + hh * 8 + v];
+    return CLT_OK;
+}
+
+int clt_get_corr2d(clt_handle* h, float* corr, float* combo, float* disparity) {
+    if (!h) return CLT_E_ARG;
+    if (!h->corr_done) return fail(h, CLT_E_STATE, "clt_get_corr2d: corr2d not executed");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const size_t nt = h->num_tasks;
+    if (corr) HIPCHK(h, hipMemcpy(corr, h->d_corr, nt * h->corr_pairs_at_exec * CLT_CORR_LEN * 4, hipMemcpyDeviceToHost));
+    if (combo && h->corr_strip_h_at_exec)
+        HIPCHK(h, hipMemcpy(combo, h->d_combo, nt * h->corr_strip_h_at_exec * 15 * 4, hipMemcpyDeviceToHost));
+    if (disparity && h->corr_strip_h_at_exec)
+        HIPCHK(h, hipMemcpy(disparity, h->d_disp, nt * CLT_DISP_LAYERS * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+int clt_get_textures(clt_handle* h, float* tiles, float* stats) {
+    if (!h) return CLT_E_ARG;
+    if (!h->tex_done) return fail(h, CLT_E_STATE, "clt_get_textures: textures not executed");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const size_t nt = h->num_tasks;
+    if (tiles) HIPCHK(h, hipMemcpy(tiles, h->d_tex, nt * h->tex_planes_at_exec * 256 * 4, hipMemcpyDeviceToHost));
+    if (stats) HIPCHK(h, hipMemcpy(stats, h->d_tex_stats, nt * 4 * h->g.num_cams * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+int clt_get_combined_textures(clt_handle* h, float* out) {
+    if (!h || !out) return CLT_E_ARG;
+    if (!h->combined) return fail(h, CLT_E_STATE, "clt_get_combined_textures: combine not executed");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(out, h->d_tex_img, (size_t)h->tex_planes_at_exec * h->g.width * h->g.height * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+// ------------------------------------------------------------------------ control / timing
+int clt_set_async(clt_handle* h, int32_t async) {
+    if (!h) return CLT_E_ARG;
+    h->async = async != 0;
+    return CLT_OK;
+}
+
+int clt_sync(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return CLT_OK;
+}
+
+int clt_set_timing(clt_handle* h, int32_t enable) {
+    if (!h) return CLT_E_ARG;
+    h->timing = enable != 0;
+    return CLT_OK;
+}
+
+int clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count) {
+    if (!h || which < 0 || which >= T_NUM) return CLT_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double tot = 0;
+    for (int u = 0; u < h->ev_used[which]; u++) {
+        float ms = 0.f;
+        HIPCHK(h, hipEventElapsedTime(&ms, h->ev_beg[which][u], h->ev_end[which][u]));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (count) *count = h->ev_used[which];
+    return CLT_OK;
+}
+
+int clt_reset_timing(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int i = 0; i < T_NUM; i++) h->ev_used[i] = 0;
+    return CLT_OK;
+}
+
+}  // extern "C"

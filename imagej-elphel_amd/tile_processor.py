@@ -1,0 +1,258 @@
+"""GPUTileProcessor: Python mirror of the reference's JCuda wrapper over the C ABI.
+
+Same method names, argument meaning and call order as
+src/main/java/com/elphel/imagej/gpu/GPUTileProcessor.java (GTP), backed by libclt_hip.so.
+Methods without a reference counterpart (correlation, textures, timing) are named in the same
+style.  Errors of the C ABI raise ``CltError`` (the reference raises ``CudaException``).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import CltError
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class TpTask:
+    """GTP:132-166 TpTask: task word, tile (tx, ty), target disparity, port centres xy[C][2]."""
+
+    def __init__(self, tx, ty, target_disparity=0.0, task=1, xy=None):
+        self.tx, self.ty = tx, ty
+        self.target_disparity = target_disparity
+        self.task = task
+        self.xy = xy
+
+    def as_record(self):
+        xy = np.asarray(self.xy, np.float32)
+        rec = np.zeros(2 + xy.size, np.int32)
+        rec[0] = self.task
+        rec[1] = self.tx | (self.ty << 16)
+        rec[2:] = xy.reshape(-1).view(np.int32)
+        return rec
+
+
+class GPUTileProcessor:
+    """GPUTileProcessor() (GTP:243-360) for one frame geometry.
+
+    Defaults are the reference statics (GTP:80-97): 2592x1936, 4 cameras, 3 colours,
+    164x123 kernels, Bayer phase GR/BG (red at odd columns of even rows, TPC:113-115)."""
+
+    DTT_SIZE = 8
+
+    def __init__(self, width=2592, height=1936, num_cams=4, num_colors=3, red_row=0, red_col=1,
+                 kernels=True, kernel_step=16, max_tasks=0):
+        self.lib = _abi.load()
+        g = _abi.clt_geom(width=width, height=height, num_cams=num_cams, num_colors=num_colors,
+                          red_row=red_row, red_col=red_col,
+                          kernels_hor=(width // kernel_step + 2) if kernels else 0,
+                          kernels_vert=(height // kernel_step + 2) if kernels else 0,
+                          kernel_step=kernel_step, max_tasks=max_tasks)
+        self.geom = g
+        self.IMG_WIDTH, self.IMG_HEIGHT, self.NUM_CAMS, self.NUM_COLORS = width, height, num_cams, num_colors
+        self.KERNELS_HOR, self.KERNELS_VERT = g.kernels_hor, g.kernels_vert
+        self.tiles_x, self.tiles_y = width // 8, height // 8
+        h = ctypes.c_void_p()
+        rc = self.lib.clt_create(ctypes.byref(g), ctypes.byref(h))
+        if rc != 0:
+            raise CltError(rc, "clt_create failed (geometry %dx%d, %d cameras)" % (width, height, num_cams))
+        self.h = h
+        self.kernels_set = False
+        self.bayer_set = False
+        self.num_task_tiles = 0
+        self.num_pairs = 6 if num_cams == 4 else 0
+        self.strip_height = 9
+        self.keep_weights = True
+
+    # ------------------------------------------------------------------ plumbing
+    def _chk(self, rc):
+        if rc != 0:
+            raise CltError(rc, self.lib.clt_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.clt_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------------ inputs (GTP:363-527)
+    def setTasks(self, tile_tasks, use_aux=False):
+        """GTP:363-371.  tile_tasks: int32 records [n][2+2C] or a list of TpTask."""
+        if isinstance(tile_tasks, (list, tuple)):
+            recs = np.stack([t.as_record() for t in tile_tasks]) if tile_tasks else np.zeros((0, 2 + 2 * self.NUM_CAMS), np.int32)
+        else:
+            recs = np.ascontiguousarray(tile_tasks, np.int32)
+        if recs.ndim != 2 or recs.shape[1] != 2 + 2 * self.NUM_CAMS:
+            raise ValueError("task records must be [n][%d]" % (2 + 2 * self.NUM_CAMS))
+        self._chk(self.lib.clt_set_tasks(self.h, _ptr(recs), recs.shape[0]))
+        self.num_task_tiles = recs.shape[0]
+        self._tasks = recs
+
+    def setConvolutionKernel(self, kernel, kernel_offsets, ncam):
+        """GTP:373-379: float kernels [KY][KX][K][4][8][8] ([v][h]) and offsets [KY][KX][K][8]."""
+        k = np.ascontiguousarray(kernel, np.float32)
+        o = np.ascontiguousarray(kernel_offsets, np.float32)
+        nk = self.KERNELS_HOR * self.KERNELS_VERT * self.NUM_COLORS
+        if k.size != nk * 256 or o.size != nk * 8:
+            raise ValueError("kernel arrays do not match the %dx%d kernel grid" % (self.KERNELS_HOR, self.KERNELS_VERT))
+        self._chk(self.lib.clt_set_kernels(self.h, ncam, _ptr(k), _ptr(o)))
+
+    def setConvolutionKernels(self, clt_kernels, force=False, offsets=None):
+        """GTP:381-435.  clt_kernels: per camera arrays in upload layout (as produced by
+        synth.clt_kernels) with ``offsets`` given separately."""
+        if self.kernels_set and not force:
+            return
+        for ncam in range(self.NUM_CAMS):
+            self.setConvolutionKernel(clt_kernels[ncam], offsets[ncam], ncam)
+        self.kernels_set = True
+
+    def clearConvolutionKernels(self):
+        self._chk(self.lib.clt_clear_kernels(self.h))
+        self.kernels_set = False
+
+    def setBayerImage(self, bayer_image, ncam):
+        """GTP:437-450: one mosaic [H][W] (or flat) float image."""
+        img = np.ascontiguousarray(bayer_image, np.float32).reshape(self.IMG_HEIGHT, -1)
+        self._chk(self.lib.clt_set_image(self.h, ncam, _ptr(img), img.shape[1]))
+
+    def setBayerImages(self, bayer_data, force=False):
+        """GTP:452-468: bayer_data [C][3][H*W] colour planes (summed into mosaics) or [C][H][W]."""
+        if self.bayer_set and not force:
+            return
+        for ncam in range(self.NUM_CAMS):
+            d = np.asarray(bayer_data[ncam])
+            if d.ndim == 2 and d.shape[0] == self.NUM_COLORS and self.NUM_COLORS == 3:
+                d = d.sum(axis=0)
+            self.setBayerImage(d, ncam)
+        self.bayer_set = True
+
+    def setLpfRbg(self, sigma_r, sigma_b, sigma_g):
+        """GTP:780-812: convert-stage LPF per colour."""
+        self._chk(self.lib.clt_set_lpf_rbg(self.h, sigma_r, sigma_b, sigma_g))
+
+    @staticmethod
+    def setCltLpf(sigma):
+        """GTP:814-848 setCltLpf followed by dttt_iiie * 2n: 64 multipliers ([h][v])."""
+        out = np.zeros(64, np.float32)
+        rc = _abi.load().clt_lpf_coeffs(sigma, _ptr(out))
+        if rc != 0:
+            raise CltError(rc, "clt_lpf_coeffs")
+        return out
+
+    def setCorrSigma(self, sigma):
+        """corr_sigma of ImageDtt (ID:1662-1690): LPF of correlation, textures and imclt."""
+        self._chk(self.lib.clt_set_corr_sigma(self.h, sigma))
+
+    def setCorrPairs(self, pairs):
+        """pairs: [(cam1, cam2, dir, scale)], dir -1 for pairs off the hor/vert/diagonal axes."""
+        arr = (_abi.clt_pair * len(pairs))(*[_abi.clt_pair(*p) for p in pairs])
+        self._chk(self.lib.clt_set_pairs(self.h, arr, len(pairs)))
+        self.num_pairs = len(pairs)
+
+    def setCorrParams(self, **kw):
+        p = _abi.clt_corr_params()
+        self.lib.clt_default_corr_params(ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._chk(self.lib.clt_set_corr_params(self.h, ctypes.byref(p)))
+        self.strip_height = p.strip_height
+
+    def setTexParams(self, port_offsets=None, **kw):
+        p = _abi.clt_tex_params()
+        self.lib.clt_default_tex_params(ctypes.byref(p), self.NUM_CAMS)
+        if port_offsets is not None:
+            for i, (x, y) in enumerate(port_offsets):
+                p.port_offsets[i][0], p.port_offsets[i][1] = x, y
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._chk(self.lib.clt_set_tex_params(self.h, ctypes.byref(p)))
+        self.keep_weights = bool(p.keep_weights)
+
+    # ------------------------------------------------------------------ kernels
+    def execConverCorrectTiles(self, lpf_mask=0):
+        """GTP:530-558 (the reference passes lpf_mask 7; the CPU path applies no LPF here)."""
+        self._chk(self.lib.clt_exec_convert(self.h, lpf_mask))
+
+    def execImcltRbg(self, apply_lpf=False):
+        """GTP:560-597."""
+        self._chk(self.lib.clt_exec_imclt(self.h, int(apply_lpf)))
+
+    def execCorrelate2D(self):
+        self._chk(self.lib.clt_exec_corr2d(self.h))
+
+    def execTextures(self, extra_disparity=None):
+        ed = None if extra_disparity is None else np.ascontiguousarray(extra_disparity, np.float32)
+        self._chk(self.lib.clt_exec_textures(self.h, _ptr(ed)))
+
+    def execCombineTextures(self):
+        self._chk(self.lib.clt_exec_combine_textures(self.h))
+
+    # ------------------------------------------------------------------ outputs
+    def getRBG(self, ncam):
+        """GTP:599-626: [colors][(H+8)*(W+8)] float32."""
+        out = np.empty((self.NUM_COLORS, (self.IMG_HEIGHT + 8) * (self.IMG_WIDTH + 8)), np.float32)
+        self._chk(self.lib.clt_get_rbg(self.h, ncam, _ptr(out)))
+        return out
+
+    def getCLT(self, ncam):
+        """[tilesY*tilesX][colors][4][8][8] float32 ([v][h] blocks)."""
+        out = np.empty((self.tiles_x * self.tiles_y, self.NUM_COLORS, 4, 8, 8), np.float32)
+        self._chk(self.lib.clt_get_clt(self.h, ncam, _ptr(out)))
+        return out
+
+    def getCorr2D(self):
+        """(corr [n][P][15][15], combined strip [n][h][15], disparity [n][8])."""
+        n = self.num_task_tiles
+        corr = np.empty((n, self.num_pairs, 15, 15), np.float32)
+        combo = np.empty((n, self.strip_height, 15), np.float32)
+        disp = np.empty((n, _abi.DISP_LAYERS), np.float32)
+        self._chk(self.lib.clt_get_corr2d(self.h, _ptr(corr), _ptr(combo), _ptr(disp)))
+        return corr, combo, disp
+
+    def getTextures(self):
+        """(tiles [n][planes][16][16], stats [n][4C])."""
+        n = self.num_task_tiles
+        planes = (4 + self.NUM_CAMS + 4) if self.keep_weights else 4
+        tiles = np.empty((n, planes, 16, 16), np.float32)
+        stats = np.empty((n, 4 * self.NUM_CAMS), np.float32)
+        self._chk(self.lib.clt_get_textures(self.h, _ptr(tiles), _ptr(stats)))
+        return tiles, stats
+
+    def getCombinedTextures(self):
+        planes = (4 + self.NUM_CAMS + 4) if self.keep_weights else 4
+        out = np.empty((planes, self.IMG_HEIGHT, self.IMG_WIDTH), np.float32)
+        self._chk(self.lib.clt_get_combined_textures(self.h, _ptr(out)))
+        return out
+
+    # ------------------------------------------------------------------ execution control
+    def setAsync(self, on):
+        self._chk(self.lib.clt_set_async(self.h, int(on)))
+
+    def sync(self):
+        self._chk(self.lib.clt_sync(self.h))
+
+    def setTiming(self, on):
+        self._chk(self.lib.clt_set_timing(self.h, int(on)))
+
+    def resetTiming(self):
+        self._chk(self.lib.clt_reset_timing(self.h))
+
+    def getTiming(self, kernel):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int32()
+        self._chk(self.lib.clt_get_timing(self.h, _abi.TIMING_KERNELS[kernel], ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value

@@ -1,0 +1,192 @@
+/*
+ * clt_tile_processor.h -- C ABI of the MI355X (gfx950) Complex Lapped Transform tile processor.
+ *
+ * This is the drop-in boundary for the GPU path of imagej-elphel's tile processor.  Every entry
+ * point below replaces one method of the reference's JCuda wrapper
+ *   src/main/java/com/elphel/imagej/gpu/GPUTileProcessor.java            (GTP below)
+ * or adds the GPU form of a CPU-only stage of
+ *   src/main/java/com/elphel/imagej/tileprocessor/ImageDtt.java          (ID below)
+ *   src/main/java/com/elphel/imagej/tileprocessor/Correlation2d.java     (C2 below)
+ * The semantics of every result are those of the reference CPU path (ImageDtt); the memory layouts
+ * of the host buffers are those the reference GPU path exchanges with Java (GTP), unless noted.
+ *
+ * Conventions
+ *   - All functions return 0 on success, a negative CLT_E_* code on failure; the message of the
+ *     last failure of a handle is returned by clt_last_error() (GTP throws CudaException instead).
+ *   - Plain C types only; host buffers are borrowed for the duration of a call.
+ *   - A handle is not thread safe.  Every clt_exec_* call returns after the device work finished
+ *     (GTP:550-557 synchronises the context after every launch) unless clt_set_async(h, 1).
+ *   - Colours of a Bayer frame are numbered as in the reference: 0 = red, 1 = blue, 2 = green
+ *     (EyesisCorrections.java:2192, 2219).  A monochrome frame has one colour (0).
+ *   - 8x8 blocks of CLT coefficients exchanged with the host are row-major [vertical][horizontal]
+ *     (the layout of the reference GPU, GTP:406-408, TileProcessor.cuh:1065-1095).
+ */
+#ifndef CLT_TILE_PROCESSOR_H
+#define CLT_TILE_PROCESSOR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLT_DTT_SIZE        8    /* transform size (ECP:2519 transform_size)          */
+#define CLT_TILE_COEFFS   256    /* 4 quadrants x 8 x 8 coefficients per tile/colour   */
+#define CLT_CORR_SIZE      15    /* 2*DTT_SIZE-1 (C2:134)                             */
+#define CLT_CORR_LEN      225
+#define CLT_MAX_CAMS       16
+#define CLT_MAX_PAIRS     120
+#define CLT_DISP_LAYERS     8    /* per-tile disparity record, see CLT_DISP_*          */
+#define CLT_TEX_STATS      16    /* per-tile texture statistics: diff0..3 + 12 tones   */
+
+enum {
+    CLT_OK            =  0,
+    CLT_E_ARG         = -1,  /* invalid argument (null pointer, size out of range)            */
+    CLT_E_STATE       = -2,  /* required input not set yet (kernels, images, tasks)            */
+    CLT_E_HIP         = -3,  /* HIP runtime error                                              */
+    CLT_E_NODEVICE    = -4,  /* no GPU visible                                                 */
+    CLT_E_NOMEM       = -5,  /* device allocation failed                                       */
+    CLT_E_UNSUPPORTED = -6   /* configuration not supported by this entry point                */
+};
+
+/* Per-tile disparity record written by clt_exec_corr2d (ID:80-86 DISPARITY_TITLES subset). */
+enum {
+    CLT_DISP_INT        = 0, /* -argmax on the combined strip axis          (ID:2197)           */
+    CLT_DISP_CM         = 1, /* centre-of-mass disparity                    (ID:2244, C2:936)   */
+    CLT_DISP_STRENGTH   = 2, /* combined strip value at the integer maximum (ID:2196)           */
+    CLT_DISP_HOR        = 3, /* horizontal pairs, parabola                  (ID:2216-2225)      */
+    CLT_DISP_HOR_STR    = 4,
+    CLT_DISP_VERT       = 5, /* vertical pairs, parabola                    (ID:2227-2237)      */
+    CLT_DISP_VERT_STR   = 6,
+    CLT_DISP_CM_WEIGHT  = 7  /* total weight of the centre-of-mass window   (C2:1025)           */
+};
+
+/* Frame geometry: fixed for the life of a handle (the reference injects the same values as
+ * #defines before NVRTC compilation, GTP:80-97, GTP:262-276). */
+typedef struct clt_geom {
+    int32_t width;          /* pixels, multiple of 8 (IMG_WIDTH)                              */
+    int32_t height;         /* pixels, multiple of 8 (IMG_HEIGHT)                             */
+    int32_t num_cams;       /* 1..16 (NUM_CAMS)                                               */
+    int32_t num_colors;     /* 3 = Bayer mosaic (R,B,G), 1 = monochrome (NUM_COLORS)          */
+    int32_t red_row;        /* Bayer phase of red: row parity (reference: 0, TPC:113)         */
+    int32_t red_col;        /* Bayer phase of red: column parity (reference: 1, TPC:114)      */
+    int32_t kernels_hor;    /* aberration kernel grid width  (KERNELS_HOR  = W/16+2)          */
+    int32_t kernels_vert;   /* aberration kernel grid height (KERNELS_VERT = H/16+2)          */
+    int32_t kernel_step;    /* pixels between kernel centres, 16 (ECP:2545)                  */
+    int32_t max_tasks;      /* capacity of the task list; 0 = tilesX * tilesY                 */
+} clt_geom;
+
+/* One correlation pair: cameras and orientation (C2:52-71 PAIRS).  dir: 0 horizontal,
+ * 1 vertical, 2 main diagonal, 3 other diagonal, -1 any other direction (only the 15x15
+ * correlation is produced for such pairs).  scale: baseline scale (1 for a quad camera). */
+typedef struct clt_pair {
+    int32_t cam1, cam2, dir, scale;
+} clt_pair;
+
+/* Correlation parameters (ImageDttParameters IDP:31-112, CLTParameters ECP:2529-2573). */
+typedef struct clt_corr_params {
+    float   fat_zero;        /* phase-correlation fat zero, 0.0            (ECP:2529)          */
+    float   corr_red;        /* red-to-green weight, 0.5                   (ECP:2557)          */
+    float   corr_blue;       /* blue-to-green weight, 0.2                  (ECP:2558)          */
+    float   corr_offset;     /* strip product offset, 0.1                  (IDP:67)            */
+    int32_t twice_diagonal;  /* 1                                          (IDP:68)            */
+    float   min_corr;        /* minimal integer maximum, 0.02              (IDP:69)            */
+    int32_t strip_height;    /* rows of the combined strip, 9              (IDP:73)            */
+    float   enhortho_width;  /* ortho notch width, 2                       (IDP:44)            */
+    float   enhortho_scale;  /* ortho notch scale, 0.0                     (IDP:45)            */
+} clt_corr_params;
+
+/* Texture parameters (ECP:2584-2594, ID:1491-1555). */
+typedef struct clt_tex_params {
+    float   min_shot;          /* 10.0   */
+    float   scale_shot;        /* 3.0    */
+    float   diff_sigma;        /* 5.0    */
+    int32_t dust_remove;       /* 1      */
+    float   corr_magic_scale;  /* 0.85   (ECP corr_magic_scale)                               */
+    int32_t keep_weights;      /* 1: 12 planes per tile (R,B,G,A,w0..w3,rms_r,rms_b,rms_g,rms),
+                                  0: 4 planes (R,B,G,A)                                        */
+    float   port_offsets[CLT_MAX_CAMS][2]; /* per camera {x,y} offset (ID:1648-1652)           */
+} clt_tex_params;
+
+typedef struct clt_handle clt_handle;
+
+/* ---- life cycle: GPUTileProcessor() (GTP:243-360); the reference never frees ---------------- */
+int  clt_create(const clt_geom* geom, clt_handle** out);
+int  clt_destroy(clt_handle* h);
+const char* clt_last_error(const clt_handle* h);
+int  clt_device_count(void);                              /* 0 when no GPU is visible          */
+
+/* ---- inputs ----------------------------------------------------------------------------------- */
+/* setConvolutionKernel(float[] kernel, float[] kernel_offsets, int ncam) (GTP:373-379).
+ * kernels: [kernels_vert][kernels_hor][num_colors][4][8][8] floats, each 8x8 [v][h];
+ * offsets: [kernels_vert][kernels_hor][num_colors][8] CltExtra records (TPC:139-148). */
+int  clt_set_kernels(clt_handle* h, int32_t cam, const float* kernels, const float* offsets);
+/* Drop the kernels: convert without deconvolution (ID:4896 use_kernels == false). */
+int  clt_clear_kernels(clt_handle* h);
+/* setBayerImage(float[] bayer_image, int ncam) (GTP:437-450).  image: [height][stride] floats,
+ * colours of a Bayer frame interleaved as a mosaic (green already scaled by the caller,
+ * TQ:2062-2067), stride >= width. */
+int  clt_set_image(clt_handle* h, int32_t cam, const float* image, int32_t stride);
+/* setTasks(TpTask[], boolean) (GTP:363-371).  tasks: num_tasks records of (2 + 2*num_cams)
+ * 32-bit words: int task (0 = skip; bits 0..7 = ports used by textures), int txy (tx | ty<<16),
+ * float xy[num_cams][2] (port tile centres, GTP:132-166). */
+int  clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks);
+/* setLpfRbg(float,float,float) (GTP:780-812): LPF applied by convert for colours in lpf_mask.
+ * setCltLpf(double) (GTP:814-848) is clt_lpf_coeffs below. */
+int  clt_set_lpf_rbg(clt_handle* h, float sigma_r, float sigma_b, float sigma_g);
+/* Correlation/texture/imclt low-pass filter sigma (corr_sigma, ID:1662-1690, ID:4178-4271). */
+int  clt_set_corr_sigma(clt_handle* h, float sigma);
+int  clt_set_pairs(clt_handle* h, const clt_pair* pairs, int32_t num_pairs);
+int  clt_set_corr_params(clt_handle* h, const clt_corr_params* p);
+int  clt_set_tex_params(clt_handle* h, const clt_tex_params* p);
+/* 64 LPF multipliers for sigma, [h][v] order (GTP:814-848 setCltLpf, then dttt_iiie * 2n). */
+int  clt_lpf_coeffs(double sigma, float out[64]);
+/* Defaults of the reference parameter classes (IDP, ECP); quad camera pairs C2:52-58. */
+void clt_default_corr_params(clt_corr_params* p);
+void clt_default_tex_params(clt_tex_params* p, int32_t num_cams);
+int  clt_default_quad_pairs(clt_pair out[6]);
+
+/* ---- kernels ---------------------------------------------------------------------------------- */
+/* execConverCorrectTiles() (GTP:530-558): convert_direct over the task list. */
+int  clt_exec_convert(clt_handle* h, int32_t lpf_mask);
+/* execImcltRbg() (GTP:560-597): inverse CLT with overlap-add of every converted tile of the last
+ * convert; apply_lpf != 0 multiplies by the corr_sigma LPF first (clt_lpf, ID:4178-4271). */
+int  clt_exec_imclt(clt_handle* h, int32_t apply_lpf);
+/* Phase correlation of every pair for every task (C2:197-389), combined strip (C2:556-790) and
+ * per-tile disparity (ID:2186-2238). */
+int  clt_exec_corr2d(clt_handle* h);
+/* Per-tile textures (ID:2662-2812).  extra_disparity: num_tasks floats or NULL (ID:2663-2677). */
+int  clt_exec_textures(clt_handle* h, const float* extra_disparity);
+/* Overlap-add of the texture tiles into full-frame planes (combineRGBATiles, ID:3858-3989,
+ * overlap = true, sharp_alpha = false). */
+int  clt_exec_combine_textures(clt_handle* h);
+
+/* ---- outputs ---------------------------------------------------------------------------------- */
+/* getRBG(int ncam) (GTP:599-626): [num_colors][height+8][width+8] floats (uncropped; the
+ * reference CPU image is this one cropped by 4 pixels, ID:3617). */
+int  clt_get_rbg(clt_handle* h, int32_t cam, float* out);
+/* CLT of one camera: [tilesY*tilesX][num_colors][4][8][8] floats, 8x8 as [v][h]. */
+int  clt_get_clt(clt_handle* h, int32_t cam, float* out);
+/* corr: [num_tasks][num_pairs][225]; combo: [num_tasks][strip_height*15] (may be NULL);
+ * disparity: [num_tasks][CLT_DISP_LAYERS] (may be NULL). */
+int  clt_get_corr2d(clt_handle* h, float* corr, float* combo, float* disparity);
+/* tiles: [num_tasks][planes][256] with planes = 12 or 4 (keep_weights); stats:
+ * [num_tasks][16] = max_diff[4] (ID:2802-2806) then tones r0..r3,b0..b3,g0..g3 (ID:2807-2811),
+ * or NULL. */
+int  clt_get_textures(clt_handle* h, float* tiles, float* stats);
+/* [planes][height][width] overlap-added texture planes. */
+int  clt_get_combined_textures(clt_handle* h, float* out);
+
+/* ---- execution control and timing (no reference counterpart) --------------------------------- */
+int  clt_set_async(clt_handle* h, int32_t async);     /* 1: exec_* return after launch        */
+int  clt_sync(clt_handle* h);
+int  clt_set_timing(clt_handle* h, int32_t enable);   /* record HIP events around each kernel  */
+/* Accumulated device time (ms) and launch count of a kernel since the last reset:
+ * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures. */
+int  clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count);
+int  clt_reset_timing(clt_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLT_TILE_PROCESSOR_H */

@@ -1,0 +1,381 @@
+"""CPU oracle of the CLT tile processor -- TEST INFRASTRUCTURE ONLY.
+
+ctypes front end of ``clt_oracle.c``, the C restatement of the reference's Java CPU path
+(DttRad2 / ImageDtt / Correlation2d).  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this package, and only as the checker or the timed
+CPU baseline; the product path never does.
+
+Parity pinning of each oracle function (golden tables held by the reference, closed forms,
+invariants) is documented and tested in ``tests/test_oracle.py``.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class Geom(ctypes.Structure):
+    """Mirror of ``clt_geom`` in include/clt_tile_processor.h."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "width", "height", "num_cams", "num_colors", "red_row", "red_col",
+        "kernels_hor", "kernels_vert", "kernel_step", "max_tasks")]
+
+
+class Pair(ctypes.Structure):
+    _fields_ = [("cam1", ctypes.c_int32), ("cam2", ctypes.c_int32),
+                ("dir", ctypes.c_int32), ("scale", ctypes.c_int32)]
+
+
+class CorrParams(ctypes.Structure):
+    _fields_ = [("fat_zero", ctypes.c_float), ("corr_red", ctypes.c_float),
+                ("corr_blue", ctypes.c_float), ("corr_offset", ctypes.c_float),
+                ("twice_diagonal", ctypes.c_int32), ("min_corr", ctypes.c_float),
+                ("strip_height", ctypes.c_int32), ("enhortho_width", ctypes.c_float),
+                ("enhortho_scale", ctypes.c_float)]
+
+
+class TexParams(ctypes.Structure):
+    _fields_ = [("min_shot", ctypes.c_float), ("scale_shot", ctypes.c_float),
+                ("diff_sigma", ctypes.c_float), ("dust_remove", ctypes.c_int32),
+                ("corr_magic_scale", ctypes.c_float), ("keep_weights", ctypes.c_int32),
+                ("port_offsets", (ctypes.c_float * 2) * 16)]
+
+
+# Reference quad pairs, Correlation2d.java:52-58 {first, second, orientation, scale}
+QUAD_PAIRS = [(0, 1, 0, 1), (2, 3, 0, 1), (0, 2, 1, 1), (1, 3, 1, 1), (0, 3, 2, 1), (2, 1, 3, 1)]
+# ImageDtt.java:1648-1652 port_offsets of a quad camera
+QUAD_PORT_OFFSETS = [(-0.5, -0.5), (0.5, -0.5), (-0.5, 0.5), (0.5, 0.5)]
+
+
+def default_corr_params(**kw):
+    """ImageDttParameters / CLTParameters defaults (IDP:44-73, ECP:2529-2558)."""
+    p = CorrParams(fat_zero=0.0, corr_red=0.5, corr_blue=0.2, corr_offset=0.1, twice_diagonal=1,
+                   min_corr=0.02, strip_height=9, enhortho_width=2.0, enhortho_scale=0.0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def default_tex_params(num_cams=4, port_offsets=None, **kw):
+    """CLTParameters defaults (ECP:2584-2594) and the quad port offsets (ID:1648-1652)."""
+    p = TexParams(min_shot=10.0, scale_shot=3.0, diff_sigma=5.0, dust_remove=1,
+                  corr_magic_scale=0.85, keep_weights=1)
+    offs = port_offsets if port_offsets is not None else QUAD_PORT_OFFSETS
+    for i, (x, y) in enumerate(offs):
+        p.port_offsets[i][0] = x
+        p.port_offsets[i][1] = y
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def build():
+    """Compile the oracle libraries with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_LIBS = {}
+_D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
+_I = ctypes.POINTER(ctypes.c_int32)
+_VP = ctypes.c_void_p
+
+
+def lib(precision=64):
+    """Load libclt_oracle64.so (FP64 restatement) or libclt_oracle32.so (FP32 port)."""
+    if precision in _LIBS:
+        return _LIBS[precision]
+    path = os.path.join(_HERE, "libclt_oracle%d.so" % precision)
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    L.orc_real_size.restype = ctypes.c_int
+    L.orc_hwindow.argtypes = [_D]
+    L.orc_tables.argtypes = [_VP, _VP, _VP]
+    L.orc_dttt.argtypes = [ctypes.c_int, _D, ctypes.c_int, _D]
+    L.orc_fold_tile.argtypes = [_D, ctypes.c_double, ctypes.c_double, ctypes.c_int, _D]
+    L.orc_unfold_tile.argtypes = [_D, ctypes.c_int, _D]
+    L.orc_lpf.argtypes = [ctypes.c_double, _D]
+    L.orc_fract_shift.argtypes = [_D, ctypes.c_double, ctypes.c_double]
+    L.orc_convolve_tile.argtypes = [_D, _D]
+    L.orc_convert_one.argtypes = [ctypes.POINTER(Geom), _F, ctypes.c_int, _VP, _VP, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, _D, _D]
+    L.orc_iclt_tile.argtypes = [_D, _VP, _D]
+    L.orc_corr_unfold.argtypes = [_D, _D]
+    L.orc_strip.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]
+    L.orc_combine_strips.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _D]
+    L.orc_max_xy_int_axis.argtypes = [_D, ctypes.c_double, _I]
+    L.orc_half_flat_top_window.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_double, _D]
+    L.orc_max_x_cm.argtypes = [_D, ctypes.c_int, ctypes.c_int, _D]
+    L.orc_max_xs_ortho.argtypes = [_D, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, _D]
+    L.orc_tile_debayer.argtypes = [ctypes.c_int, _D, _D]
+    L.orc_convert_frame.argtypes = [ctypes.POINTER(Geom), _VP, ctypes.c_int, _VP, _VP, _VP,
+                                    ctypes.c_int, ctypes.c_int, _VP, ctypes.c_int, _D, _VP, ctypes.c_int]
+    L.orc_imclt_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, _VP, _D, ctypes.c_int]
+    L.orc_corr_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, ctypes.c_int, ctypes.POINTER(Pair),
+                                 ctypes.c_int, ctypes.POINTER(CorrParams), ctypes.c_double, _D, _VP,
+                                 _VP, ctypes.c_int]
+    L.orc_textures_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, ctypes.c_int,
+                                     ctypes.POINTER(TexParams), ctypes.POINTER(CorrParams),
+                                     ctypes.c_double, _VP, _D, _VP, ctypes.c_int]
+    L.orc_combine_textures.argtypes = [ctypes.POINTER(Geom), _VP, ctypes.c_int, _D, ctypes.c_int, _D]
+    _LIBS[precision] = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(_VP)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def make_geom(width, height, num_cams=4, num_colors=3, red_row=0, red_col=1, kernels=True,
+              kernel_step=16):
+    kh, kv = (width // kernel_step + 2, height // kernel_step + 2) if kernels else (0, 0)
+    return Geom(width=width, height=height, num_cams=num_cams, num_colors=num_colors,
+                red_row=red_row, red_col=red_col, kernels_hor=kh, kernels_vert=kv,
+                kernel_step=kernel_step, max_tasks=0)
+
+
+# ---------------------------------------------------------------- primitives
+def hwindow(precision=64):
+    out = np.zeros(8)
+    lib(precision).orc_hwindow(_dp(out))
+    return out
+
+
+def tables(precision=64):
+    fi = np.zeros((64, 4), np.int32)
+    ui = np.zeros(256, np.int32)
+    uk = np.zeros((4, 256))
+    lib(precision).orc_tables(fi.ctypes.data_as(_VP), ui.ctypes.data_as(_VP), uk.ctypes.data_as(_VP))
+    return fi, ui, uk
+
+
+def dttt(which, x, mode, precision=64):
+    """which: 'iv' (DR:503 dttt_iv), 'iie' (DR:596), 'iiie' (DR:648); x: 64 values."""
+    x = _f64(x).reshape(64)
+    y = np.zeros(64)
+    lib(precision).orc_dttt({"iv": 0, "iie": 1, "iiie": 2}[which], _dp(x), mode, _dp(y))
+    return y
+
+
+def fold_tile(x256, shift_h, shift_v, mode, precision=64):
+    x = _f64(x256).reshape(256)
+    y = np.zeros(64)
+    lib(precision).orc_fold_tile(_dp(x), shift_h, shift_v, mode, _dp(y))
+    return y
+
+
+def unfold_tile(x64, mode, precision=64):
+    x = _f64(x64).reshape(64)
+    y = np.zeros(256)
+    lib(precision).orc_unfold_tile(_dp(x), mode, _dp(y))
+    return y
+
+
+def lpf(sigma, precision=64):
+    out = np.zeros(64)
+    lib(precision).orc_lpf(sigma, _dp(out))
+    return out
+
+
+def fract_shift(clt4x64, sx, sy, precision=64):
+    c = _f64(clt4x64).reshape(256).copy()
+    lib(precision).orc_fract_shift(_dp(c), sx, sy)
+    return c.reshape(4, 64)
+
+
+def convolve_tile(data4x64, kern4x64, precision=64):
+    d = _f64(data4x64).reshape(256).copy()
+    k = _f64(kern4x64).reshape(256)
+    lib(precision).orc_convolve_tile(_dp(d), _dp(k))
+    return d.reshape(4, 64)
+
+
+def convert_one(geom, image, color, cx, cy, kernels=None, offsets=None, do_shift=True, precision=64):
+    """One camera/colour tile: returns (clt[4][64] in the CPU [h][v] order, residual[2])."""
+    image = np.ascontiguousarray(image, np.float32)
+    kern = None if kernels is None else np.ascontiguousarray(kernels, np.float32)
+    offs = None if offsets is None else np.ascontiguousarray(offsets, np.float32)
+    clt = np.zeros(256)
+    res = np.zeros(2)
+    lib(precision).orc_convert_one(ctypes.byref(geom), image.ctypes.data_as(_F), image.shape[-1],
+                                   _vp(kern), _vp(offs), color, cx, cy, int(do_shift), _dp(clt), _dp(res))
+    return clt.reshape(4, 64), res
+
+
+def iclt_tile(clt4x64, lpf64=None, precision=64):
+    c = _f64(clt4x64).reshape(256)
+    l = None if lpf64 is None else _f64(lpf64)
+    out = np.zeros(256)
+    lib(precision).orc_iclt_tile(_dp(c), _vp(l), _dp(out))
+    return out.reshape(16, 16)
+
+
+def corr_unfold(q4x64, precision=64):
+    q = _f64(q4x64).reshape(256)
+    out = np.zeros(225)
+    lib(precision).orc_corr_unfold(_dp(q), _dp(out))
+    return out.reshape(15, 15)
+
+
+def strip(corr225, hwidth, direction, ss=1, precision=64):
+    c = _f64(corr225).reshape(225)
+    out = np.zeros(hwidth * 15)
+    lib(precision).orc_strip(_dp(c), hwidth, direction, ss, _dp(out))
+    return out.reshape(hwidth, 15)
+
+
+def combine_strips(strips, offset=0.1, twice_diagonal=True, precision=64):
+    s = _f64(strips)
+    n, length = s.shape[0], s[0].size
+    out = np.zeros(length)
+    lib(precision).orc_combine_strips(_dp(s), n, length, offset, int(twice_diagonal), _dp(out))
+    return out.reshape(s.shape[1:])
+
+
+def max_xy_int_axis(data15, min_max=0.02, precision=64):
+    d = _f64(data15).reshape(-1)[:15].copy()
+    ix = ctypes.c_int32(0)
+    ok = lib(precision).orc_max_xy_int_axis(_dp(d), min_max, ctypes.byref(ix))
+    return ix.value if ok else None
+
+
+def half_flat_top_window(ihwidth, hwidth, blur, normalize=True, notch=False, scale=1.0):
+    out = np.zeros(ihwidth)
+    lib().orc_half_flat_top_window(ihwidth, hwidth, blur, int(normalize), int(notch), scale, _dp(out))
+    return out
+
+
+def max_x_cm(strip_data, ixcenter, precision=64):
+    s = _f64(strip_data)
+    out = np.zeros(3)
+    ok = lib(precision).orc_max_x_cm(_dp(s.reshape(-1)), s.shape[0], ixcenter, _dp(out))
+    return out if ok else None
+
+
+def max_xs_ortho(corrs, offset=0.1, is_vert=False, enhortho_width=2.0, enhortho_scale=0.0, precision=64):
+    c = _f64(corrs).reshape(-1, 225)
+    out = np.zeros(2)
+    ok = lib(precision).orc_max_xs_ortho(_dp(c), c.shape[0], offset, int(is_vert), enhortho_width,
+                                         enhortho_scale, _dp(out))
+    return out if ok else None
+
+
+def tile_debayer(rb, tile, precision=64):
+    t = _f64(tile).reshape(256)
+    out = np.zeros(256)
+    lib(precision).orc_tile_debayer(int(rb), _dp(t), _dp(out))
+    return out.reshape(16, 16)
+
+
+# ---------------------------------------------------------------- frame level
+def _ptr_array(arrs):
+    if arrs is None:
+        return None, None
+    keep = [np.ascontiguousarray(a, np.float32) for a in arrs]
+    ptrs = (ctypes.c_void_p * len(keep))(*[k.ctypes.data for k in keep])
+    return ptrs, keep
+
+
+def convert_frame(geom, images, tasks, kernels=None, offsets=None, lpf_mask=0, lpf3=None,
+                  no_fract_shift=False, nthreads=None, precision=64, want_residual=False):
+    """ID clt_aberrations_quad_corr convert stage over a task list.
+
+    images: [cams][H][W] float32 mosaics; kernels/offsets: per camera arrays in the GPU upload
+    layout or None; tasks: int32 [ntasks][2+2C] records.  Returns CLT float64
+    [cams][tilesY*tilesX][colors][4][8][8] ([v][h] blocks) and, optionally, residual shifts."""
+    nthreads = nthreads or os.cpu_count()
+    C, K = geom.num_cams, geom.num_colors
+    nt = (geom.width // 8) * (geom.height // 8)
+    img_p, keep_i = _ptr_array(images)
+    ker_p, keep_k = _ptr_array(kernels)
+    off_p, keep_o = _ptr_array(offsets)
+    tasks = np.ascontiguousarray(tasks, np.int32)
+    ntasks = tasks.shape[0]
+    clt = np.zeros((C, nt, K, 4, 8, 8))
+    res = np.zeros((ntasks, C, K, 2)) if want_residual else None
+    l3 = None if lpf3 is None else _f64(lpf3).reshape(-1)
+    lib(precision).orc_convert_frame(ctypes.byref(geom), ctypes.cast(img_p, _VP),
+                                     np.asarray(images).shape[-1],
+                                     None if ker_p is None else ctypes.cast(ker_p, _VP),
+                                     None if off_p is None else ctypes.cast(off_p, _VP),
+                                     tasks.ctypes.data_as(_VP), ntasks, lpf_mask, _vp(l3),
+                                     int(no_fract_shift), _dp(clt), _vp(res), nthreads)
+    del keep_i, keep_k, keep_o
+    return (clt, res) if want_residual else clt
+
+
+def imclt_frame(geom, clt, valid, lpf64=None, nthreads=None, precision=64):
+    """iclt_2d (ID:3559-3666) of every camera/colour: [cams][colors][H+8][W+8] (uncropped)."""
+    nthreads = nthreads or os.cpu_count()
+    clt = _f64(clt)
+    valid = np.ascontiguousarray(valid, np.uint8)
+    out = np.zeros((geom.num_cams, geom.num_colors, geom.height + 8, geom.width + 8))
+    l = None if lpf64 is None else _f64(lpf64)
+    lib(precision).orc_imclt_frame(ctypes.byref(geom), _dp(clt), valid.ctypes.data_as(_VP), _vp(l),
+                                   _dp(out), nthreads)
+    return out
+
+
+def corr_frame(geom, clt, tasks, pairs=None, params=None, corr_sigma=0.8, nthreads=None, precision=64):
+    """Pair correlations [task][pair][15][15], combined strip [task][h][15], disparity [task][8]."""
+    nthreads = nthreads or os.cpu_count()
+    pairs = pairs if pairs is not None else QUAD_PAIRS
+    params = params or default_corr_params()
+    parr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+    tasks = np.ascontiguousarray(tasks, np.int32)
+    ntasks = tasks.shape[0]
+    clt = _f64(clt)
+    corr = np.zeros((ntasks, len(pairs), 15, 15))
+    combo = np.zeros((ntasks, params.strip_height, 15))
+    disp = np.zeros((ntasks, 8))
+    rc = lib(precision).orc_corr_frame(ctypes.byref(geom), _dp(clt), tasks.ctypes.data_as(_VP), ntasks,
+                                       parr, len(pairs), ctypes.byref(params), corr_sigma, _dp(corr),
+                                       _vp(combo), _vp(disp), nthreads)
+    if rc:
+        raise ValueError("orc_corr_frame failed: %d" % rc)
+    return corr, combo, disp
+
+
+def textures_frame(geom, clt, tasks, tex_params=None, corr_params=None, corr_sigma=0.8,
+                   extra_disparity=None, nthreads=None, precision=64):
+    """Texture tiles [task][planes][16][16] and stats [task][4*cams]."""
+    nthreads = nthreads or os.cpu_count()
+    tp = tex_params or default_tex_params(geom.num_cams)
+    cp = corr_params or default_corr_params()
+    tasks = np.ascontiguousarray(tasks, np.int32)
+    ntasks = tasks.shape[0]
+    planes = 4 + geom.num_cams + 4 if tp.keep_weights else 4
+    tiles = np.zeros((ntasks, planes, 16, 16))
+    stats = np.zeros((ntasks, 4 * geom.num_cams))
+    ed = None if extra_disparity is None else _f64(extra_disparity)
+    rc = lib(precision).orc_textures_frame(ctypes.byref(geom), _dp(_f64(clt)), tasks.ctypes.data_as(_VP),
+                                           ntasks, ctypes.byref(tp), ctypes.byref(cp), corr_sigma, _vp(ed),
+                                           _dp(tiles), _vp(stats), nthreads)
+    if rc:
+        raise ValueError("orc_textures_frame failed: %d" % rc)
+    return tiles, stats
+
+
+def combine_textures(geom, tasks, tiles):
+    """combineRGBATiles (ID:3858-3989), overlap: [planes][H][W]."""
+    tasks = np.ascontiguousarray(tasks, np.int32)
+    tiles = _f64(tiles)
+    planes = tiles.shape[1]
+    out = np.zeros((planes, geom.height, geom.width))
+    lib().orc_combine_textures(ctypes.byref(geom), tasks.ctypes.data_as(_VP), tasks.shape[0], _dp(tiles),
+                               planes, _dp(out))
+    return out

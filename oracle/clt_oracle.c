@@ -1,0 +1,1428 @@
+/*
+ * clt_oracle.c -- CPU restatement of imagej-elphel's CLT tile processor (the reference CPU path).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle of the MI355X tile processor: only the
+ * tests, the smoke check and the CPU-baseline leg of the benchmark load it, and only as the checker
+ * (or as the timed CPU baseline).  The product library never links or calls it.
+ *
+ * It restates, function by function, the Java CPU path of the reference (paths relative to
+ * src/main/java/com/elphel/imagej/tileprocessor/):
+ *   DttRad2.java       (DR) transforms, window, fold/unfold tables, correlation unfold
+ *   ImageDtt.java      (ID) extract_correct_tile, convolve_tile, fract_shift, clt_lpf, iclt_2d,
+ *                           the quad correlation/texture driver, tile_debayer, tile_combine_rgba,
+ *                           combineRGBATiles, the worker-thread pool
+ *   Correlation2d.java (C2) phase correlation, strips, strip combination, maxima
+ * Inputs are the float32 buffers of the C ABI (include/clt_tile_processor.h): mosaic images, kernels
+ * in the GPU upload layout, packed task records.  Arithmetic is done in `real`: double by default
+ * (the FP64 oracle), float when compiled with -DORC_FP32 (an FP32 port used to size the expected
+ * single-precision error).  Elementary functions are always evaluated in double.
+ *
+ * Compile with -ffp-contract=off: Java never fuses multiply-add, and the integer tile positions
+ * (ID:4940-4943) are recomputed bit-exactly by the GPU from the same float inputs.
+ *
+ * Parity pinning of every function is listed in the header of tests/test_oracle.py.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdatomic.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/clt_tile_processor.h"
+
+#ifdef ORC_FP32
+typedef float real;
+#else
+typedef double real;
+#endif
+
+#define EXPORT __attribute__((visibility("default")))
+#define N8 8
+
+/* ================================ DttRad2 tables ================================ */
+static double HW[8];                       /* DR:671-694 set_window(1): sin(pi*(i+0.5)/16)      */
+static double CIV[8][8], SIV[8][8];        /* DR:993-1012, DR:1054-1073 (orthonormal, symmetric)  */
+static double CIIE[8][8], SIIE[8][8];      /* DR:952-968 (setup_CIIe), DR:1014-1030 (setup_SIIe)  */
+static double CIIIE[8][8], SIIIE[8][8];    /* DR:970-991 (setup_CIIIe), DR:1032-1052 (setup_SIIIe)*/
+static int    FOLD_INDEX[64][4];           /* DR:146-203 set_fold_2d                             */
+static int    UNFOLD_INDEX[256];           /* DR:405-434 set_unfold_2d                           */
+static double UNFOLD_K[4][256];
+static pthread_once_t tables_once = PTHREAD_ONCE_INIT;
+
+/* DR:112-144 get_fold_indices(x, n=8) */
+static void get_fold_indices(int x, int ind[2][6]) {
+    const int n = 8, n1 = 4;
+    if (x < n1) {
+        ind[0][0] = n + n1 - x - 1; ind[0][1] = n1 + x;     ind[0][2] = -1; ind[0][3] = 1;
+        ind[0][4] = n1 - x - 1;     ind[0][5] = -1;
+        ind[1][0] = n + n1 + x;     ind[1][1] = n1 - x - 1; ind[1][2] = -1; ind[1][3] = -1;
+        ind[1][4] = n1 + x;         ind[1][5] = -1;
+    } else {
+        x -= n1;
+        ind[0][0] = x;              ind[0][1] = x;          ind[0][2] = 1;  ind[0][3] = 1;
+        ind[0][4] = n - x - 1;      ind[0][5] = 1;
+        ind[1][0] = n - x - 1;      ind[1][1] = n - x - 1;  ind[1][2] = -1; ind[1][3] = 1;
+        ind[1][4] = x;              ind[1][5] = 1;
+    }
+}
+
+/* DR:392-403 get_unfold_index_signs(x, n=8) */
+static void get_unfold_index_signs(int x, int is2[3]) {
+    const int n = 8, n1 = 4;
+    int segm = x / n1;
+    x = x % n1;
+    switch (segm) {
+    case 0: is2[0] = x + n1;     is2[1] = 1;  is2[2] = 1;  break;
+    case 1: is2[0] = n - x - 1;  is2[1] = -1; is2[2] = 1;  break;
+    case 2: is2[0] = n1 - x - 1; is2[1] = -1; is2[2] = 1;  break;
+    default: is2[0] = x;         is2[1] = -1; is2[2] = -1; break;
+    }
+}
+
+static void init_tables(void) {
+    const int n = 8;
+    for (int i = 0; i < n; i++) HW[i] = sin(M_PI / (2.0 * n) * (i + 0.5));
+    const double scale = sqrt(2.0 / n);
+    for (int j = 0; j < n; j++) {
+        for (int k = 0; k < n; k++) {
+            CIV[j][k] = scale * cos((2 * j + 1) * (2 * k + 1) * M_PI / (4 * n));
+            SIV[j][k] = scale * sin((2 * j + 1) * (2 * k + 1) * M_PI / (4 * n));
+            CIIE[j][k] = scale * cos(j * (2 * k + 1) * M_PI / (2 * n));
+            SIIE[j][k] = scale * sin((j + 1) * (2 * k + 1) * M_PI / (2 * n));
+        }
+    }
+    for (int j = 0; j < n; j++) {           /* CIIIe[k][j], e_j = 0.5 for j == 0 or n-1 (DR:982) */
+        double ec = ((j == 0) || (j == n - 1)) ? 0.5 : 1.0;
+        double es = (j == n - 1) ? 0.5 : 1.0;  /* SIIIe: only j == n-1 (DR:1043) */
+        for (int k = 0; k < n; k++) {
+            CIIIE[k][j] = scale * ec * cos(j * (2 * k + 1) * M_PI / (2 * n));
+            SIIIE[k][j] = scale * es * sin((j + 1) * (2 * k + 1) * M_PI / (2 * n));
+        }
+    }
+    /* DR:146-203 set_fold_2d: fold_index[n*i + j][k] = 2n*vert_ind[k>>1] + hor_ind[k&1] */
+    for (int i = 0; i < n; i++) {
+        int fv[2][6];
+        get_fold_indices(i, fv);
+        for (int j = 0; j < n; j++) {
+            int fh[2][6];
+            get_fold_indices(j, fh);
+            for (int k = 0; k < 4; k++)
+                FOLD_INDEX[n * i + j][k] = 2 * n * fv[(k >> 1) & 1][0] + fh[k & 1][0];
+        }
+    }
+    /* DR:405-434 set_unfold_2d */
+    for (int i = 0; i < 2 * n; i++) {
+        int iv[3];
+        get_unfold_index_signs(i, iv);
+        double wv = HW[(i < n) ? i : 2 * n - i - 1];
+        double kv[2] = {iv[1] * wv, iv[2] * wv};
+        for (int j = 0; j < 2 * n; j++) {
+            int ih[3];
+            get_unfold_index_signs(j, ih);
+            double wh = HW[(j < n) ? j : 2 * n - j - 1];
+            double kh[2] = {ih[1] * wh, ih[2] * wh};
+            UNFOLD_INDEX[2 * n * i + j] = iv[0] * n + ih[0];
+            for (int mode = 0; mode < 4; mode++)
+                UNFOLD_K[mode][2 * n * i + j] = kv[(mode >> 1) & 1] * kh[mode & 1];
+        }
+    }
+}
+static void tables(void) { pthread_once(&tables_once, init_tables); }
+
+/* ================================ 8-point transforms ================================ */
+/* 1-D y = M x for one of the tables (dct_iv/dst_iv DR:80-98 equal the orthonormal closed forms) */
+static void mat8(const double m[8][8], const real* x, real* y) {
+    for (int i = 0; i < 8; i++) {
+        real s = 0;
+        for (int j = 0; j < 8; j++) s += (real)m[i][j] * x[j];
+        y[i] = s;
+    }
+}
+
+/* Separable 2-D pass pair with the reference's transposed storage (DR:503-519 dttt_iv, DR:596-614
+ * dttt_iie, DR:648-666 dttt_iiie): first pass along the contiguous index with table a0/a1 by
+ * mode bit 0, result transposed, second pass with b by mode bit 1. */
+static void dttt_2d(const real* x, int mode, const double (*c)[8], const double (*s)[8], real* y) {
+    real t[64], line[8];
+    for (int i = 0; i < 8; i++) {
+        mat8((mode & 1) ? s : c, x + 8 * i, line);
+        for (int j = 0; j < 8; j++) t[j * 8 + i] = line[j];
+    }
+    for (int i = 0; i < 8; i++) mat8((mode & 2) ? s : c, t + 8 * i, y + 8 * i);
+}
+static void dttt_iv(const real* x, int mode, real* y) { dttt_2d(x, mode, CIV, SIV, y); }
+static void dttt_iie(const real* x, int mode, real* y) { dttt_2d(x, mode, CIIE, SIIE, y); }
+static void dttt_iiie(const real* x, int mode, real* y) { dttt_2d(x, mode, CIIIE, SIIIE, y); }
+
+/* DR:263-314 get_shifted_fold_2d(8, shift_hor, shift_vert) */
+static void shifted_fold_coeffs(real shift_hor, real shift_vert, real fk[4][64][4]) {
+    const double f = M_PI / 16.0;
+    real ahc = (real)cos(f * shift_hor), ahs = (real)sin(f * shift_hor);
+    real avc = (real)cos(f * shift_vert), avs = (real)sin(f * shift_vert);
+    for (int i = 0; i < 8; i++) {
+        int fi[2][6];
+        get_fold_indices(i, fi);
+        real vw0 = avc * (real)HW[fi[0][1]] + avs * (real)HW[fi[0][4]] * fi[0][5];
+        real vw1 = avc * (real)HW[fi[1][1]] + avs * (real)HW[fi[1][4]] * fi[1][5];
+        real vk[2][2] = {{fi[0][2] * vw0, fi[1][2] * vw1}, {fi[0][3] * vw0, fi[1][3] * vw1}};
+        for (int j = 0; j < 8; j++) {
+            int fj[2][6];
+            get_fold_indices(j, fj);
+            real hw0 = ahc * (real)HW[fj[0][1]] + ahs * (real)HW[fj[0][4]] * fj[0][5];
+            real hw1 = ahc * (real)HW[fj[1][1]] + ahs * (real)HW[fj[1][4]] * fj[1][5];
+            real hk[2][2] = {{fj[0][2] * hw0, fj[1][2] * hw1}, {fj[0][3] * hw0, fj[1][3] * hw1}};
+            for (int mode = 0; mode < 4; mode++)
+                for (int k = 0; k < 4; k++)
+                    fk[mode][8 * i + j][k] = vk[(mode >> 1) & 1][(k >> 1) & 1] * hk[mode & 1][k & 1];
+        }
+    }
+}
+
+/* DR:724-738 fold_tile: 16x16 [row][col] -> 8x8 [v][h] */
+static void fold_tile(const real* x, int mode, real fk[4][64][4], real* y) {
+    for (int i = 0; i < 64; i++) {
+        real s = 0;
+        for (int k = 0; k < 4; k++) s += x[FOLD_INDEX[i][k]] * fk[mode][i][k];
+        y[i] = s;
+    }
+}
+
+/* DR:770-780 unfold_tile: 8x8 [v][h] IDCT output -> 16x16 [row][col] */
+static void unfold_tile(const real* x, int mode, real* y) {
+    for (int i = 0; i < 256; i++) y[i] = (real)UNFOLD_K[mode][i] * x[UNFOLD_INDEX[i]];
+}
+
+/* DR:443-489 corr_unfold_tile: 4 quadrants [y][x] of DCT-IIe/DST-IIe output -> 15x15 */
+static void corr_unfold_tile(real q[4][64], real* r) {
+    const int n = 8, cs = 15;
+    const real sc = (real)0.25;
+    r[cs * n - n] = sc * q[0][0];
+    for (int j = 1; j < n; j++) {
+        r[cs * n - n + j] = sc * (q[0][j] + q[1][j - 1]);
+        r[cs * n - n - j] = sc * (q[0][j] - q[1][j - 1]);
+    }
+    for (int i = 1; i < n; i++) {
+        r[cs * (n + i) - n] = sc * (q[0][i * n] + q[2][(i - 1) * n]);
+        r[cs * (n - i) - n] = sc * (q[0][i * n] - q[2][(i - 1) * n]);
+        for (int j = 1; j < n; j++) {
+            real a = q[0][i * n + j], b = q[1][i * n + j - 1], c = q[2][(i - 1) * n + j],
+                 d = q[3][(i - 1) * n + j - 1];
+            r[cs * (n + i) - n + j] = sc * (a + b + c + d);
+            r[cs * (n + i) - n - j] = sc * (a + -b + c + -d);
+            r[cs * (n - i) - n + j] = sc * (a + b + -c + -d);
+            r[cs * (n - i) - n - j] = sc * (a + -b + -c + d);
+        }
+    }
+}
+
+/* ================================ LPF ================================ */
+/* ID:1662-1690 (filter in clt_aberrations_quad_corr_new) == ID:4178-4220 (clt_lpf) == GTP:814-848
+ * (setCltLpf): exp(-(i^2+j^2)/(2*sigma)) (sigma not squared, as in the reference), normalised by
+ * the DC gain of the DCT-IIIe, then dttt_iiie and * 2n. */
+static void make_lpf(double sigma, real* out) {
+    real fd[64], f[64];
+    if (sigma == 0) {
+        for (int i = 0; i < 64; i++) fd[i] = (i == 0) ? 1 : 0;
+    } else {
+        for (int i = 0; i < 8; i++)
+            for (int j = 0; j < 8; j++) fd[i * 8 + j] = (real)exp(-(i * i + j * j) / (2 * sigma));
+    }
+    real sum = 0;
+    for (int i = 0; i < 8; i++)
+        for (int j = 0; j < 8; j++) {
+            real d = fd[i * 8 + j];
+            d *= (real)(cos(M_PI * i / 16.0) * cos(M_PI * j / 16.0));
+            if (i > 0) d *= 2;
+            if (j > 0) d *= 2;
+            sum += d;
+        }
+    for (int i = 0; i < 64; i++) fd[i] /= sum;
+    dttt_iiie(fd, 0, f);
+    for (int i = 0; i < 64; i++) out[i] = f[i] * 16;
+}
+
+/* ================================ convert (ID:4867-5584) ================================ */
+typedef struct {
+    const clt_geom* g;
+    int tiles_x, tiles_y, num_tiles;
+} frame_t;
+
+static void frame_init(frame_t* f, const clt_geom* g) {
+    f->g = g;
+    f->tiles_x = g->width / 8;
+    f->tiles_y = g->height / 8;
+    f->num_tiles = f->tiles_x * f->tiles_y;
+}
+
+/* Colour of mosaic pixel (EyesisCorrections.java:2219 for the reference phase red_row=0,
+ * red_col=1; blue sits at the opposite parities, green on the remaining diagonal). */
+static int pixel_color(const clt_geom* g, int y, int x) {
+    if (g->num_colors == 1) return 0;
+    int ry = (y & 1) == g->red_row, rx = (x & 1) == g->red_col;
+    if (ry && rx) return 0;
+    if (!ry && !rx) return 1;
+    return 2;
+}
+
+/* Java Math.round(double) */
+static double jround(double x) { return floor(x + 0.5); }
+
+/* ID:4867-5426 extract_correct_tile followed by fract_shift (ID:2047-2058) for one camera/colour.
+ * image: mosaic [height][stride]; kernels/offsets: GPU upload layout or NULL.  clt: [4][64] in the
+ * reference CPU order [h][v]; returns residual shift in res[2]. */
+static void fract_shift(real clt[4][64], real shift_x, real shift_y);
+static void convolve_tile(real data[4][64], const real kern[4][64]);
+
+static void convert_one(const clt_geom* g, const float* image, int stride, const float* kernels,
+                        const float* offsets, int color, double center_x, double center_y,
+                        real clt[4][64], double res[2]) {
+    const int width = g->width, height = g->height;
+    double px = center_x - 8, py = center_y - 8;
+    int use_kernels = (kernels != NULL) && (offsets != NULL);
+    const float* ktile = NULL;
+    if (use_kernels) {
+        const int kstep = g->kernel_step;
+        int ktx = (int)jround(center_x / kstep) + 1;
+        int kty = (int)jround(center_y / kstep) + 1;
+        if (kty < 0) kty = 0; else if (kty >= g->kernels_vert) kty = g->kernels_vert - 1;
+        if (ktx < 0) ktx = 0; else if (ktx >= g->kernels_hor) ktx = g->kernels_hor - 1;
+        size_t kidx = ((size_t)kty * g->kernels_hor + ktx) * g->num_colors + color;
+        const float* ce = offsets + kidx * 8;   /* CltExtra: data_x,data_y,center_x,center_y,
+                                                    dxc_dx,dxc_dy,dyc_dx,dyc_dy (TPC:139-148) */
+        double kdx = center_x - (ktx - 1 + 0.5) * kstep;
+        double kdy = center_y - (kty - 1 + 0.5) * kstep;
+        px = center_x - 8 - ((double)ce[0] + (double)ce[4] * kdx + (double)ce[5] * kdy);
+        py = center_y - 8 - ((double)ce[1] + (double)ce[6] * kdx + (double)ce[7] * kdy);
+        ktile = kernels + kidx * 256;
+    }
+    int left = (int)-jround(-px), top = (int)-jround(-py);  /* ID:4940-4941 */
+    res[0] = -(px - left);
+    res[1] = -(py - top);
+    real tile_in[256];
+    for (int i = 0; i < 16; i++) {                    /* ID:4953-4970, per-pixel parity clamp */
+        int pi = top + i;
+        if (pi < 0) pi &= 1;
+        else if (pi >= height) pi = height - 2 + (pi & 1);
+        for (int j = 0; j < 16; j++) {
+            int pj = left + j;
+            if (pj < 0) pj &= 1;
+            else if (pj >= width) pj = width - 2 + (pj & 1);
+            tile_in[16 * i + j] = (pixel_color(g, pi, pj) == color) ? (real)image[(size_t)pi * stride + pj] : 0;
+        }
+    }
+    real fk[4][64][4];
+    shifted_fold_coeffs((real)res[0], (real)res[1], fk);  /* ID:5071 */
+    for (int m = 0; m < 4; m++) {                         /* ID:5243-5292 */
+        real folded[64];
+        fold_tile(tile_in, m, fk, folded);
+        dttt_iv(folded, m, clt[m]);
+    }
+    if (use_kernels) {                                    /* ID:5402-5410: [v][h] upload -> [h][v] */
+        real kern[4][64];
+        for (int m = 0; m < 4; m++)
+            for (int v = 0; v < 8; v++)
+                for (int h = 0; h < 8; h++) kern[m][h * 8 + v] = (real)ktile[m * 64 + v * 8 + h];
+        convolve_tile(clt, kern);
+    }
+}
+
+/* ID:5429-5488 convolve_tile */
+static void convolve_tile(real data[4][64], const real kern[4][64]) {
+    static const int zi[4][4] = {{0, -1, -2, 3}, {1, 0, -3, -2}, {2, -3, 0, -1}, {3, 2, 1, 0}};
+    real r[4][64];
+    for (int i = 0; i < 64; i++)
+        for (int n = 0; n < 4; n++) {
+            real s = 0;
+            for (int k = 0; k < 4; k++) {
+                if (zi[n][k] < 0) s -= data[-zi[n][k]][i] * kern[k][i];
+                else s += data[zi[n][k]][i] * kern[k][i];
+            }
+            r[n][i] = s;
+        }
+    memcpy(data, r, sizeof(r));
+}
+
+/* ID:5490-5584 fract_shift: phase rotation, horizontal then vertical ([h][v] storage) */
+static void fract_shift(real clt[4][64], real shift_x, real shift_y) {
+    real ch[8], sh[8], cv[8], sv[8];
+    for (int i = 0; i < 8; i++) {
+        ch[i] = (real)cos((i + 0.5) * M_PI * shift_x / 8);
+        sh[i] = (real)sin((i + 0.5) * M_PI * shift_x / 8);
+        cv[i] = (real)cos((i + 0.5) * M_PI * shift_y / 8);
+        sv[i] = (real)sin((i + 0.5) * M_PI * shift_y / 8);
+    }
+    for (int idx = 0; idx < 64; idx++) {
+        int kh = idx >> 3, kv = idx & 7;
+        real c = ch[kh], s = sh[kh];
+        real t0 = clt[0][idx] * c - clt[1][idx] * s;
+        real t1 = clt[1][idx] * c + clt[0][idx] * s;
+        real t2 = clt[2][idx] * c - clt[3][idx] * s;
+        real t3 = clt[3][idx] * c + clt[2][idx] * s;
+        c = cv[kv]; s = sv[kv];
+        clt[0][idx] = t0 * c - t2 * s;
+        clt[2][idx] = t2 * c + t0 * s;
+        clt[1][idx] = t1 * c - t3 * s;
+        clt[3][idx] = t3 * c + t1 * s;
+    }
+}
+
+/* ================================ thread pool (ID:5847-5872) ================================ */
+typedef void (*tile_fn)(void* ctx, int index);
+typedef struct {
+    atomic_int next;
+    int count;
+    tile_fn fn;
+    void* ctx;
+} pool_t;
+
+static void* pool_worker(void* arg) {
+    pool_t* p = (pool_t*)arg;
+    for (int i = atomic_fetch_add(&p->next, 1); i < p->count; i = atomic_fetch_add(&p->next, 1))
+        p->fn(p->ctx, i);
+    return NULL;
+}
+
+static void run_pool(int count, int nthreads, tile_fn fn, void* ctx) {
+    pool_t p;
+    atomic_init(&p.next, 0);
+    p.count = count;
+    p.fn = fn;
+    p.ctx = ctx;
+    if (nthreads <= 1 || count < 2) {
+        pool_worker(&p);
+        return;
+    }
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    int started = 0;
+    for (int t = 0; t < nthreads; t++) {
+        if (pthread_create(&th[t], NULL, pool_worker, &p) != 0) break;
+        started++;
+    }
+    if (started == 0) pool_worker(&p);
+    for (int t = 0; t < started; t++) pthread_join(th[t], NULL);
+}
+
+/* ================================ task records (GTP:132-166) ================================ */
+static int task_words(const clt_geom* g) { return 2 + 2 * g->num_cams; }
+static int32_t task_flags(const clt_geom* g, const int32_t* tasks, int t) { return tasks[(size_t)t * task_words(g)]; }
+static void task_txy(const clt_geom* g, const int32_t* tasks, int t, int* tx, int* ty) {
+    int32_t txy = tasks[(size_t)t * task_words(g) + 1];
+    *tx = txy & 0xffff;
+    *ty = (txy >> 16) & 0xffff;
+}
+static float task_xy(const clt_geom* g, const int32_t* tasks, int t, int cam, int d) {
+    float v;
+    memcpy(&v, &tasks[(size_t)t * task_words(g) + 2 + 2 * cam + d], 4);
+    return v;
+}
+
+/* ================================ frame: convert ================================ */
+typedef struct {
+    frame_t f;
+    const float* const* images;
+    int stride;
+    const float* const* kernels;
+    const float* const* offsets;
+    const int32_t* tasks;
+    int lpf_mask;
+    const double* lpf;     /* [3][64] or NULL */
+    int no_fract_shift;
+    double* clt;           /* [cam][tile][color][4][v][h] */
+    double* residual;      /* [task][cam][color][2] or NULL */
+} convert_ctx;
+
+static void convert_task(void* vctx, int t) {
+    convert_ctx* c = (convert_ctx*)vctx;
+    const clt_geom* g = c->f.g;
+    if (task_flags(g, c->tasks, t) == 0) return;
+    int tx, ty;
+    task_txy(g, c->tasks, t, &tx, &ty);
+    size_t tile = (size_t)ty * c->f.tiles_x + tx;
+    for (int cam = 0; cam < g->num_cams; cam++) {
+        double cx = task_xy(g, c->tasks, t, cam, 0), cy = task_xy(g, c->tasks, t, cam, 1);
+        for (int col = 0; col < g->num_colors; col++) {
+            real clt[4][64];
+            double res[2];
+            convert_one(g, c->images[cam], c->stride, c->kernels ? c->kernels[cam] : NULL,
+                        c->offsets ? c->offsets[cam] : NULL, col, cx, cy, clt, res);
+            if (!c->no_fract_shift) fract_shift(clt, (real)res[0], (real)res[1]);
+            if (c->lpf && ((c->lpf_mask >> col) & 1))
+                for (int m = 0; m < 4; m++)
+                    for (int i = 0; i < 64; i++) clt[m][i] *= (real)c->lpf[col * 64 + i];
+            double* o = c->clt + (((size_t)cam * c->f.num_tiles + tile) * g->num_colors + col) * 256;
+            for (int m = 0; m < 4; m++)
+                for (int v = 0; v < 8; v++)
+                    for (int h = 0; h < 8; h++) o[m * 64 + v * 8 + h] = clt[m][h * 8 + v];
+            if (c->residual) {
+                double* r = c->residual + (((size_t)t * g->num_cams + cam) * g->num_colors + col) * 2;
+                r[0] = res[0];
+                r[1] = res[1];
+            }
+        }
+    }
+}
+
+/* Read one camera/colour CLT tile of the GPU layout back into [4][h*8+v] */
+static void load_clt(const frame_t* f, const double* clt, int cam, size_t tile, int col, real out[4][64]) {
+    const double* p = clt + (((size_t)cam * f->num_tiles + tile) * f->g->num_colors + col) * 256;
+    for (int m = 0; m < 4; m++)
+        for (int v = 0; v < 8; v++)
+            for (int h = 0; h < 8; h++) out[m][h * 8 + v] = (real)p[m * 64 + v * 8 + h];
+}
+
+/* ================================ correlation (C2) ================================ */
+static const int ZI[4][4] = {{0, 1, 2, 3}, {-1, 0, -3, 2}, {-2, -3, 0, 1}, {3, -2, -1, 0}}; /* C2:86-90 */
+
+/* C2:197-231 correlateSingleColorFD */
+static void correlate_single_color_fd(real d1[4][64], real d2[4][64], real fat_zero, real tcorr[4][64]) {
+    real a2[64], sa2 = 0;
+    for (int i = 0; i < 64; i++) {
+        real s1 = 0, s2 = 0;
+        for (int n = 0; n < 4; n++) {
+            s1 += d1[n][i] * d1[n][i];
+            s2 += d2[n][i] * d2[n][i];
+        }
+        a2[i] = (real)sqrt((double)(s1 * s2));
+        sa2 += a2[i];
+    }
+    real fz2 = sa2 / 64 * fat_zero * fat_zero;
+    for (int i = 0; i < 64; i++) {
+        real scale = 1 / (a2[i] + fz2);
+        for (int n = 0; n < 4; n++) {
+            real s = 0;
+            for (int k = 0; k < 4; k++) {
+                if (ZI[n][k] < 0) s -= d1[-ZI[n][k]][i] * d2[k][i];
+                else s += d1[ZI[n][k]][i] * d2[k][i];
+            }
+            tcorr[n][i] = s * scale;
+        }
+    }
+}
+
+/* C2:275-324 correlateCompositeFD (one pair): colour mix, LPF, DCT-IIe/DST-IIe, unfold 15x15 */
+static void correlate_composite_fd(int ncol, real (*clt1)[4][64], real (*clt2)[4][64], const real* lpf,
+                                   const real* col_weights, real fat_zero, real* out225) {
+    real acc[4][64], t[4][64];
+    for (int col = 0; col < ncol; col++) {
+        correlate_single_color_fd(clt1[col], clt2[col], fat_zero, t);
+        real w = (ncol == 1) ? 1 : col_weights[col];
+        for (int n = 0; n < 4; n++)
+            for (int i = 0; i < 64; i++) {
+                if (col == 0) acc[n][i] = t[n][i] * w;
+                else acc[n][i] += t[n][i] * w;
+            }
+    }
+    if (lpf)
+        for (int n = 0; n < 4; n++)
+            for (int i = 0; i < 64; i++) acc[n][i] *= lpf[i];
+    real q[4][64];
+    for (int quadrant = 0; quadrant < 4; quadrant++) {
+        int mode = ((quadrant << 1) & 2) | ((quadrant >> 1) & 1);
+        dttt_iie(acc[quadrant], mode, q[quadrant]);
+    }
+    corr_unfold_tile(q, out225);
+}
+
+/* C2:610-718 scaleRotateInterpoateSingleCorrelation */
+static void strip_single(const real* corr, int hwidth, int dir, int ss, real* strip) {
+    const int center = 7, width = 15;
+    int denom = ss * ((dir > 1) ? 1 : 2);
+    real rdenom = (real)denom;
+    int ilimit = center * denom;
+    for (int i = 0; i < hwidth * width; i++) strip[i] = 0;   /* Java arrays start zeroed */
+    for (int row = 0; row < hwidth; row++) {
+        for (int scol = -center; scol <= center; scol++) {
+            int indx = row * width + scol + center;
+            for (int down = -1; down < ((row == 0) ? 0 : 2); down += 2) {
+                int xnum = 0, ynum = 0;
+                switch (dir) {
+                case 0: xnum = 2 * scol + (row & 1); ynum = down * row; break;
+                case 1: xnum = down * row; ynum = 2 * scol + (row & 1); break;
+                case 2: xnum = scol - ((down * row) >> 1); ynum = scol + ((down * row + 1) >> 1); break;
+                case 3: xnum = scol - ((down * row) >> 1); ynum = -scol - ((down * row + 1) >> 1); break;
+                }
+                if ((xnum > ilimit) || (xnum < -ilimit) || (ynum > ilimit) || (ynum < -ilimit)) {
+                    strip[indx] = (real)NAN;
+                } else {
+                    int ix0 = (xnum + ilimit) / denom;
+                    int iy0 = (ynum + ilimit) / denom;
+                    int int_x = (xnum % denom) == 0;
+                    int int_y = (ynum % denom) == 0;
+                    real d;
+                    if (int_x && int_y) {
+                        d = corr[iy0 * width + ix0];
+                    } else {
+                        real dx = (xnum + ilimit) / rdenom - ix0;
+                        real dy = (ynum + ilimit) / rdenom - iy0;
+                        if (int_y) d = (1 - dx) * corr[iy0 * width + ix0] + dx * corr[iy0 * width + ix0 + 1];
+                        else if (int_x) d = (1 - dy) * corr[iy0 * width + ix0] + dy * corr[iy0 * width + ix0 + width];
+                        else
+                            d = ((1 - dx) * (1 - dy) * corr[iy0 * width + ix0]) + (dx * (1 - dy) * corr[iy0 * width + ix0 + 1]) +
+                                ((1 - dx) * dy * corr[iy0 * width + ix0 + width]) + (dx * dy * corr[iy0 * width + ix0 + width + 1]);
+                    }
+                    if (row == 0) strip[indx] = d;
+                    else strip[indx] += (real)0.5 * d;
+                }
+            }
+        }
+    }
+}
+
+/* C2:730-790 combineInterpolatedCorrelations */
+static int combine_strips(real** strips, int nstrips, int len, real offset, int twice_diagonal, real* combo) {
+    int ncombined = 0, have = 0;
+    if (offset >= 0) {
+        for (int p = 0; p < nstrips; p++) {
+            if (!strips[p]) continue;
+            if (!have) {
+                for (int i = 0; i < len; i++) combo[i] = 1;
+                have = 1;
+            }
+            if (twice_diagonal) {
+                for (int i = 0; i < len; i++) {
+                    real d = strips[p][i] + offset;
+                    if (d < 0) d = 0;
+                    combo[i] *= d * d;
+                }
+                ncombined++;
+            } else {
+                for (int i = 0; i < len; i++) {
+                    real d = strips[p][i] + offset;
+                    if (d < 0) d = 0;
+                    combo[i] *= d;
+                }
+            }
+            ncombined++;
+        }
+        double pwr = 1.0 / ncombined;
+        if (have)
+            for (int i = 0; i < len; i++)
+                if (combo[i] > 0) combo[i] = (real)pow((double)combo[i], pwr) - offset;
+    } else {
+        for (int p = 0; p < nstrips; p++) {
+            if (!strips[p]) continue;
+            if (!have) {
+                for (int i = 0; i < len; i++) combo[i] = 0;
+                have = 1;
+            }
+            if (twice_diagonal) {
+                for (int i = 0; i < len; i++) combo[i] += 2 * strips[p][i];
+                ncombined++;
+            } else {
+                for (int i = 0; i < len; i++) combo[i] += strips[p][i];
+            }
+            ncombined++;
+        }
+        real scale = (real)1 / ncombined;
+        for (int i = 0; i < len; i++) combo[i] *= scale;
+    }
+    return have;
+}
+
+/* C2:802-843 getMaxXYInt, axis only (rectangular strip).  Returns 1 and ix (relative to centre). */
+static int max_xy_int_axis(const real* data, real min_max, int* ix) {
+    int imx = 0;
+    for (int i = 1; i < 15; i++)
+        if (isnan(data[imx]) || (data[i] > data[imx])) imx = i;
+    if (!(data[imx] >= min_max)) return 0;
+    *ix = imx % 15 - 7;
+    return 1;
+}
+
+/* C2:1047-1081 halfFlatTopWindow */
+static void half_flat_top_window(int ihwidth, double hwidth, double blur, int normalize, int notch,
+                                 double scale, double* wnd) {
+    for (int i = 0; i < ihwidth; i++) {
+        if (blur <= 0) wnd[i] = (i < hwidth) ? 1.0 : 0.0;
+        else if (i < hwidth - blur / 2) wnd[i] = 1.0;
+        else if (i > hwidth + blur / 2) wnd[i] = 0.0;
+        else wnd[i] = 0.5 * (1.0 - sin(M_PI * (i - hwidth) / blur));
+        if (notch) wnd[i] = 1.0 - wnd[i];
+    }
+    if (normalize) {
+        double s = 0;
+        for (int i = 0; i < ihwidth; i++) s += ((i == 0) ? 1.0 : 2.0) * wnd[i];
+        s = 1 / s;
+        for (int i = 0; i < ihwidth; i++) wnd[i] *= s;
+    }
+    for (int i = 0; i < ihwidth; i++) wnd[i] *= scale;
+}
+
+/* C2:936-1032 getMaxXCm on a strip with rows, window_y/window_x from IDP defaults (C2:139-152) */
+static int max_x_cm(const real* data, int data_height, int ixcenter, const double* wy_tab, int wy_len,
+                    const double* wx_tab, int wx_len, real rslt[3]) {
+    const int center = 7, data_width = 15;
+    real wy_scale = 1;
+    if (data_height > wy_len) {
+        data_height = wy_len;
+    } else if (data_height < wy_len) {
+        real swy = (real)wy_tab[0];
+        for (int i = 1; i < data_height; i++) swy += (real)wy_tab[i];
+        wy_scale = 1 / swy;
+    }
+    real s0 = 0, sx = 0, sx2 = 0;
+    int x0 = center + ixcenter;
+    for (int dy = 0; dy < data_height; dy++) {
+        int odd = dy & 1;
+        real wy = ((dy == 0) ? wy_scale : (2 * wy_scale)) * (real)wy_tab[dy];
+        int indx0 = data_width * dy;
+        for (int adx = odd; adx < wx_len; adx += 2) {
+            for (int dir = (adx == 0) ? 1 : -1; dir <= 1; dir += 2) {
+                int idx = (adx * dir) >> 1;
+                int x = 2 * idx + odd;
+                int x1 = x0 + idx;
+                if ((x1 >= 0) && (x1 < data_width)) {
+                    real d = data[indx0 + x1];
+                    if (!isnan(d) && (d > 0)) {
+                        d *= wy * (real)wx_tab[adx];
+                        s0 += d;
+                        sx += d * x;
+                        sx2 += d * x * x;
+                    }
+                }
+            }
+        }
+    }
+    if (s0 == 0) return 0;
+    rslt[0] = ixcenter + sx / s0 / 2;
+    rslt[1] = s0;
+    rslt[2] = (real)sqrt((double)(s0 * sx2 - sx * sx)) / s0 / 2;
+    return 1;
+}
+
+/* C2:2654-2675 createOrtoNotch */
+static void ortho_notch(double enhortho_width, double enhortho_scale, double* f) {
+    for (int i = 0; i < 15; i++) {
+        if ((i < (8 - enhortho_width)) || (i > (8 - 2 + enhortho_width))) f[i] = 1.0;
+        else f[i] = enhortho_scale;
+        if (i == 7) f[i] = 0.0;
+        f[i] *= sin(M_PI * (i + 1.0) / 16);
+    }
+}
+
+/* C2:2502-2651 getMaxXSOrtho (offset >= 0 branch and linear branch), 3-point parabola
+ * (PolynomialApproximation.polynomialApproximation1d with 3 points, degree 2, closed form). */
+static int max_xs_ortho(real** corrs, int npairs, real offset, int is_vert, const double* notch, real rslt[2]) {
+    const int cs = 15;
+    if (npairs == 0) return 0;
+    real corr_1d[15];
+    double w = 1.0 / npairs;
+    for (int j = 0; j < cs; j++) {
+        corr_1d[j] = 0;
+        if (offset >= 0) {
+            for (int i = 0; i < cs; i++) {
+                real combo = 1;
+                for (int p = 0; p < npairs; p++) {
+                    const real* c = corrs[p];
+                    real d = is_vert ? (real)0.5 * (c[j * cs + i] + c[j * cs + (cs - i - 1)])
+                                     : (real)0.5 * (c[i * cs + j] + c[(cs - i - 1) * cs + j]);
+                    combo *= d + offset;
+                }
+                if (combo > 0) combo = (real)pow((double)combo, w) - offset;
+                else combo = -offset;
+                corr_1d[j] += combo * (real)notch[i];
+            }
+        } else {
+            for (int p = 0; p < npairs; p++)
+                for (int i = 0; i < cs; i++) {
+                    const real* c = corrs[p];
+                    corr_1d[j] += (real)w * (is_vert ? c[j * cs + i] : c[i * cs + j]) * (real)notch[i];
+                }
+        }
+    }
+    int icenter = 0;
+    for (int i = 1; i < cs; i++)
+        if (corr_1d[i] > corr_1d[icenter]) icenter = i;
+    real xcenter = (real)icenter;
+    if ((icenter > 0) && (icenter < cs - 1)) {
+        real a = corr_1d[icenter - 1], b = corr_1d[icenter], c = corr_1d[icenter + 1];
+        real den = a - 2 * b + c;
+        if (den != 0) xcenter = icenter + (a - c) / (2 * den);
+    }
+    icenter = (int)jround((double)xcenter);
+    rslt[0] = xcenter - 7;
+    rslt[1] = corr_1d[icenter] / 8;
+    return 1;
+}
+
+typedef struct {
+    frame_t f;
+    const double* clt;
+    const int32_t* tasks;
+    const clt_pair* pairs;
+    int npairs;
+    clt_corr_params prm;
+    real lpf[64];
+    real col_weights[3];
+    double wndy[16], wndx[16], notch[15];
+    int wndy_len, wndx_len;
+    double* corr_out;   /* [task][pair][225] */
+    double* combo_out;  /* [task][strip_height*15] or NULL */
+    double* disp_out;   /* [task][8] or NULL */
+} corr_ctx;
+
+/* ID:2101-2238 for one task tile */
+static void corr_task(void* vctx, int t) {
+    corr_ctx* c = (corr_ctx*)vctx;
+    const clt_geom* g = c->f.g;
+    const int K = g->num_colors, hw = c->prm.strip_height, slen = hw * 15;
+    double* corr_o = c->corr_out + (size_t)t * c->npairs * 225;
+    double* combo_o = c->combo_out ? c->combo_out + (size_t)t * slen : NULL;
+    double* disp_o = c->disp_out ? c->disp_out + (size_t)t * 8 : NULL;
+    if (disp_o) {   /* ID:1713-1719 init: NaN, strengths 0 */
+        for (int i = 0; i < 8; i++) disp_o[i] = NAN;
+        disp_o[CLT_DISP_STRENGTH] = disp_o[CLT_DISP_HOR_STR] = disp_o[CLT_DISP_VERT_STR] = 0;
+        disp_o[CLT_DISP_CM_WEIGHT] = 0;
+    }
+    if (combo_o)
+        for (int i = 0; i < slen; i++) combo_o[i] = NAN;
+    if (task_flags(g, c->tasks, t) == 0) {
+        for (int i = 0; i < c->npairs * 225; i++) corr_o[i] = NAN;
+        return;
+    }
+    int tx, ty;
+    task_txy(g, c->tasks, t, &tx, &ty);
+    size_t tile = (size_t)ty * c->f.tiles_x + tx;
+    real corrs[CLT_MAX_PAIRS][225];
+    real (*clt1)[4][64] = malloc(sizeof(real) * 3 * 256);
+    real (*clt2)[4][64] = malloc(sizeof(real) * 3 * 256);
+    for (int p = 0; p < c->npairs; p++) {
+        for (int col = 0; col < K; col++) {
+            load_clt(&c->f, c->clt, c->pairs[p].cam1, tile, col, clt1[col]);
+            load_clt(&c->f, c->clt, c->pairs[p].cam2, tile, col, clt2[col]);
+        }
+        correlate_composite_fd(K, clt1, clt2, c->lpf, c->col_weights, (real)c->prm.fat_zero, corrs[p]);
+        for (int i = 0; i < 225; i++) corr_o[p * 225 + i] = corrs[p][i];
+    }
+    free(clt1);
+    free(clt2);
+    /* strips (C2:556-578): pairs with a known orientation; ss relative to the smallest scale */
+    int ss_min = 1 << 30;
+    for (int p = 0; p < c->npairs; p++)
+        if (c->pairs[p].dir >= 0 && c->pairs[p].scale < ss_min) ss_min = c->pairs[p].scale;
+    real* strips[CLT_MAX_PAIRS];
+    real strip_buf[CLT_MAX_PAIRS][16 * 15];
+    int nstrip = 0;
+    for (int p = 0; p < c->npairs; p++) {
+        strips[p] = NULL;
+        if (c->pairs[p].dir < 0) continue;
+        strip_single(corrs[p], hw, c->pairs[p].dir, c->pairs[p].scale / ss_min, strip_buf[p]);
+        strips[p] = strip_buf[p];
+        nstrip++;
+    }
+    if (nstrip == 0) return;
+    real combo[16 * 15];
+    combine_strips(strips, c->npairs, slen, (real)c->prm.corr_offset, c->prm.twice_diagonal, combo);
+    if (combo_o)
+        for (int i = 0; i < slen; i++) combo_o[i] = combo[i];
+    if (!disp_o) return;
+    int ix;
+    if (max_xy_int_axis(combo, (real)c->prm.min_corr, &ix)) {   /* ID:2186-2208 */
+        disp_o[CLT_DISP_INT] = -ix;
+        disp_o[CLT_DISP_STRENGTH] = combo[ix + 7];
+        real cm[3];
+        if (max_x_cm(combo, hw, ix, c->wndy, c->wndy_len, c->wndx, c->wndx_len, cm)) {
+            disp_o[CLT_DISP_CM] = -cm[0];
+            disp_o[CLT_DISP_CM_WEIGHT] = cm[1];
+        }
+    }
+    for (int vert = 0; vert < 2; vert++) {   /* ID:2211-2238, getMaskHorizontal(1)/getMaskVertical(1) */
+        real* sel[CLT_MAX_PAIRS];
+        int n = 0;
+        for (int p = 0; p < c->npairs; p++)
+            if (c->pairs[p].dir == vert && c->pairs[p].scale == 1) sel[n++] = corrs[p];
+        real r[2];
+        if (max_xs_ortho(sel, n, (real)c->prm.corr_offset, vert, c->notch, r)) {
+            disp_o[vert ? CLT_DISP_VERT : CLT_DISP_HOR] = -r[0];
+            disp_o[vert ? CLT_DISP_VERT_STR : CLT_DISP_HOR_STR] = r[1];
+        }
+    }
+}
+
+static void color_weights(const clt_geom* g, double red, double blue, real* w) {
+    if (g->num_colors == 1) {
+        w[0] = 1;
+        return;
+    }
+    double gw = 1.0 / (1.0 + red + blue);   /* ID:1571-1581 */
+    w[2] = (real)gw;
+    w[0] = (real)(red * gw);
+    w[1] = (real)(blue * gw);
+}
+
+/* ================================ IMCLT (ID:3559-3666) ================================ */
+/* 16x16 inverse of one camera/colour tile, already LPF-ed/scaled: sum over modes */
+static void iclt_tile(real clt[4][64], real scale, const real* lpf, real* out256) {
+    real tmp[64], idct[64], unf[256];
+    for (int i = 0; i < 256; i++) out256[i] = 0;
+    for (int m = 0; m < 4; m++) {
+        for (int i = 0; i < 64; i++) tmp[i] = clt[m][i] * (lpf ? scale * lpf[i] : scale);
+        int idct_mode = ((m << 1) & 2) | ((m >> 1) & 1);
+        dttt_iv(tmp, idct_mode, idct);
+        unfold_tile(idct, m, unf);
+        for (int i = 0; i < 256; i++) out256[i] += unf[i];
+    }
+}
+
+typedef struct {
+    frame_t f;
+    const double* clt;
+    const uint8_t* valid;   /* [tile] */
+    const real* lpf;        /* NULL: none */
+    int cam, color, pass;
+    real* plane;            /* (H+8) x (W+8) */
+} imclt_ctx;
+
+static void imclt_tile_fn(void* vctx, int n) {
+    imclt_ctx* c = (imclt_ctx*)vctx;
+    int nx = (c->f.tiles_x + 1 - (c->pass & 1)) / 2;
+    int tx = 2 * (n % nx) + (c->pass & 1), ty = 2 * (n / nx) + ((c->pass >> 1) & 1);
+    size_t tile = (size_t)ty * c->f.tiles_x + tx;
+    if (!c->valid[tile]) return;
+    real clt[4][64], out[256];
+    load_clt(&c->f, c->clt, c->cam, tile, c->color, clt);
+    iclt_tile(clt, (real)0.25, c->lpf, out);   /* debug_scale = 1/4 for mask 15 (ID:3574) */
+    const int ow = c->f.g->width + 8;
+    for (int i = 0; i < 16; i++)
+        for (int j = 0; j < 16; j++) c->plane[(size_t)(ty * 8 + i) * ow + tx * 8 + j] += out[16 * i + j];
+}
+
+/* ================================ textures (ID:2662-2812, 2830-3147) ================================ */
+static const double KERN_G[9] = {0.0, 0.125, 0.0, 0.125, 0.5, 0.125, 0.0, 0.125, 0.0};          /* ID:43-46 */
+static const double KERN_RB[9] = {0.0625, 0.125, 0.0625, 0.125, 0.25, 0.125, 0.0625, 0.125, 0.0625}; /* ID:47-50 */
+static const int CORN_SIDE[9][9] = {                                                                /* ID:52-61 */
+    {4, 5, 7, 8, -1}, {3, 4, 5, 6, 7, 8, -1}, {3, 4, 6, 7, -1}, {1, 2, 4, 5, 7, 8, -1},
+    {0, 1, 2, 3, 4, 5, 6, 7, 8}, {0, 1, 3, 4, 6, 7, -1}, {1, 2, 4, 5, -1}, {0, 1, 2, 3, 4, 5, -1},
+    {0, 1, 3, 4, -1}};
+
+/* ID:3048-3147 tile_debayer (16x16) */
+static void tile_debayer(int rb, const real* tile, real* rslt) {
+    const int ts = 16, tsm1 = 15;
+    const int neib[9] = {-ts - 1, -ts, -ts + 1, -1, 0, 1, ts - 1, ts, ts + 1};
+    const double* kern = rb ? KERN_RB : KERN_G;
+    real k_corn = rb ? (real)(16.0 / 9.0) : (real)(4.0 / 3.0);
+    real k_side = rb ? (real)(4.0 / 3.0) : (real)(8.0 / 7.0);
+    for (int i = 0; i < 256; i++) rslt[i] = 0;
+    for (int i = 0; i < ts; i++) {
+        for (int j = 0; j < ts; j++) {
+            int side = ((i == 0) ? 0 : (i == tsm1) ? 6 : 3) + ((j == 0) ? 0 : (j == tsm1) ? 2 : 1);
+            int indx = i * ts + j;
+            real s = 0;
+            for (int k = 0; k < 9 && CORN_SIDE[side][k] >= 0; k++) {
+                int dr = CORN_SIDE[side][k];
+                s += tile[indx + neib[dr]] * (real)kern[dr];
+            }
+            if (side == 4) rslt[indx] = s;
+            else if (side == 0 || side == 2 || side == 6 || side == 8) rslt[indx] = s * k_corn;
+            else rslt[indx] = s * k_side;
+        }
+    }
+}
+
+/* ID:3022-3045 tile_debayer_shot_corr */
+static void tile_debayer_shot_corr(int rb, const real* tile, const real* window2, real min_shot,
+                                   real scale_shot, real* out) {
+    real nw[256];
+    for (int i = 0; i < 256; i++) nw[i] = tile[i] / window2[i];
+    tile_debayer(rb, nw, out);
+    if (scale_shot > 0) {
+        real k = (real)(1.0 / sqrt((double)min_shot));
+        for (int i = 0; i < 256; i++)
+            out[i] = scale_shot * ((out[i] > min_shot) ? (real)sqrt((double)out[i]) : (k * out[i]));
+    }
+    for (int i = 0; i < 256; i++) out[i] = out[i] * window2[i];
+}
+
+/* ID:3002-3019 dust_remove */
+static void dust_remove(real pw[][256], int np) {
+    for (int i = 0; i < 256; i++) {
+        int wi = 0;
+        for (int ip = 1; ip < np; ip++)
+            if (pw[ip][i] < pw[wi][i]) wi = ip;
+        real avg = 0;
+        for (int ip = 1; ip < np; ip++)
+            if (ip != wi) avg += pw[ip][i];
+        avg /= (np - 1);
+        real scale = 1 + (avg - pw[wi][i]) / (avg * (np - 1));
+        for (int ip = 1; ip < np; ip++)
+            if (ip != wi) pw[ip][i] *= scale;
+        pw[wi][i] *= pw[wi][i] / avg;
+    }
+}
+
+typedef struct {
+    frame_t f;
+    const double* clt;
+    const int32_t* tasks;
+    const double* extra_disparity;   /* [task] or NULL */
+    clt_tex_params prm;
+    real lpf[64];
+    real col_weights[3];
+    real window2[256];
+    int planes;
+    double* tiles_out;   /* [task][planes][256] */
+    double* stats_out;   /* [task][4*C] or NULL */
+} tex_ctx;
+
+/* ID:2659-2812 for one task tile, then tile_combine_rgba (ID:2830-3000) */
+static void tex_task(void* vctx, int t) {
+    tex_ctx* c = (tex_ctx*)vctx;
+    const clt_geom* g = c->f.g;
+    const int ports = g->num_cams, numcol = 3;
+    double* out = c->tiles_out + (size_t)t * c->planes * 256;
+    double* stats = c->stats_out ? c->stats_out + (size_t)t * 4 * ports : NULL;
+    int32_t flags = task_flags(g, c->tasks, t);
+    if (flags == 0) {
+        for (int i = 0; i < c->planes * 256; i++) out[i] = NAN;
+        if (stats)
+            for (int i = 0; i < 4 * ports; i++) stats[i] = NAN;
+        return;
+    }
+    int port_mask = flags & 0xff & ((1 << ports) - 1);
+    int tx, ty;
+    task_txy(g, c->tasks, t, &tx, &ty);
+    size_t tile = (size_t)ty * c->f.tiles_x + tx;
+    double extra = c->extra_disparity ? c->extra_disparity[t] : 0.0;
+    if (isnan(extra)) extra = 0;
+    int forced = (flags >> 24) & 1;   /* C2:97 getForcedDisparity */
+    real (*iclt)[3][256] = malloc(sizeof(real) * ports * 3 * 256);
+    real (*deb)[3][256] = malloc(sizeof(real) * ports * 3 * 256);
+    for (int ip = 0; ip < ports; ip++) {
+        for (int chn = 0; chn < numcol; chn++) {
+            real clt[4][64];
+            load_clt(&c->f, c->clt, ip, tile, chn, clt);
+            if (extra != 0 && !forced)
+                fract_shift(clt, (real)(extra * c->prm.port_offsets[ip][0] / c->prm.corr_magic_scale),
+                            (real)(extra * c->prm.port_offsets[ip][1] / c->prm.corr_magic_scale));
+            iclt_tile(clt, (real)0.25, c->lpf, iclt[ip][chn]);
+            tile_debayer_shot_corr(chn != 2, iclt[ip][chn], c->window2, (real)c->prm.min_shot,
+                                   (real)c->prm.scale_shot, deb[ip][chn]);
+        }
+    }
+    /* ---- tile_combine_rgba (ID:2830-3000), keep_weights = true ---- */
+    const real* lt_window = c->window2;
+    const real* chw = c->col_weights;
+    int used = 0;
+    for (int ip = 0; ip < ports; ip++) used += (port_mask >> ip) & 1;
+    real (*pw)[256] = calloc((size_t)ports, sizeof(real) * 256);
+    real avg[3][256], alpha[256], rms[4][256];
+    for (int i = 0; i < 256; i++) {   /* rms for keep_weights (ID:2858-2874) */
+        real sw = 0;
+        for (int chn = 0; chn < numcol; chn++) {
+            real s0 = 0, s1 = 0, s2 = 0;
+            for (int ip = 0; ip < ports; ip++)
+                if ((port_mask >> ip) & 1) {
+                    s0 += 1;
+                    s1 += deb[ip][chn][i];
+                    s2 += deb[ip][chn][i] * deb[ip][chn][i];
+                }
+            rms[chn][i] = (real)sqrt((double)(s0 * s2 - s1 * s1)) / s0;
+            sw += chw[chn] * rms[chn][i] * rms[chn][i];
+        }
+        rms[3][i] = (real)sqrt((double)sw);
+    }
+    if (used > 1) {
+        real ksigma = (real)(1.0 / (2.0 * c->prm.diff_sigma * c->prm.diff_sigma));
+        int npairs = 0, pp[CLT_MAX_CAMS * CLT_MAX_CAMS / 2][2];
+        real pd2r[CLT_MAX_CAMS * CLT_MAX_CAMS / 2];
+        for (int i = 0; i < ports; i++)
+            if ((port_mask >> i) & 1)
+                for (int j = i + 1; j < ports; j++)
+                    if ((port_mask >> j) & 1) {
+                        real dx = c->prm.port_offsets[j][0] - c->prm.port_offsets[i][0];
+                        real dy = c->prm.port_offsets[j][1] - c->prm.port_offsets[i][1];
+                        pp[npairs][0] = i;
+                        pp[npairs][1] = j;
+                        pd2r[npairs++] = ksigma / (dx * dx + dy * dy);
+                    }
+        for (int i = 0; i < 256; i++) {
+            for (int ip = 0; ip < ports; ip++) pw[ip][i] = 0;
+            for (int ip = 0; ip < npairs; ip++) {
+                real d = 0;
+                for (int chn = 0; chn < numcol; chn++) {
+                    real dc = deb[pp[ip][0]][chn][i] - deb[pp[ip][1]][chn][i];
+                    dc /= lt_window[i];
+                    d += chw[chn] * dc * dc;
+                }
+                d = (real)exp((double)(-pd2r[ip] * d));
+                pw[pp[ip][0]][i] += d;
+                pw[pp[ip][1]][i] += d;
+            }
+            int b1 = 0;
+            for (int ip = b1 + 1; ip < ports; ip++)
+                if (pw[ip][i] > pw[b1][i]) b1 = ip;
+            int b2 = (b1 == 0) ? 1 : 0;
+            for (int ip = b2 + 1; ip < ports; ip++)
+                if ((ip != b1) && (pw[ip][i] > pw[b2][i])) b2 = ip;
+            real w1 = pw[b1][i] / (pw[b1][i] + pw[b2][i]);
+            real w2 = 1 - w1;
+            for (int chn = 0; chn < numcol; chn++) avg[chn][i] = w1 * deb[b1][chn][i] + w2 * deb[b2][chn][i];
+            for (int ip = 0; ip < ports; ip++)
+                if ((port_mask >> ip) & 1) {
+                    real d2 = 0;
+                    for (int chn = 0; chn < numcol; chn++) {
+                        real dc = deb[ip][chn][i] - avg[chn][i];
+                        dc /= lt_window[i];
+                        d2 += chw[chn] * dc * dc;
+                    }
+                    pw[ip][i] = (real)exp((double)(-ksigma * d2));
+                }
+            real k = 0;
+            for (int ip = 0; ip < ports; ip++) k += pw[ip][i];
+            k = 1 / k;
+            for (int chn = 0; chn < numcol; chn++) {
+                avg[chn][i] = 0;
+                for (int ip = 0; ip < ports; ip++) avg[chn][i] += k * pw[ip][i] * deb[ip][chn][i];
+            }
+        }
+    } else if (used > 0) {
+        for (int ip = 0; ip < ports; ip++)
+            if ((port_mask >> ip) & 1)
+                for (int i = 0; i < 256; i++) {
+                    for (int chn = 0; chn < numcol; chn++) avg[chn][i] = deb[ip][chn][i];
+                    pw[ip][i] = 1;
+                }
+    }
+    if (c->prm.dust_remove && (used == 4)) dust_remove(pw, ports);
+    for (int i = 0; i < 256; i++) {
+        alpha[i] = 0;
+        for (int ip = 0; ip < ports; ip++)
+            if ((port_mask >> ip) & 1) alpha[i] += pw[ip][i];
+        alpha[i] *= lt_window[i] / used;
+    }
+    if (stats) {   /* max_diff (ID:2965-2981), ports_rgb (ID:2982-2996) */
+        for (int ip = 0; ip < ports; ip++) {
+            real md = 0;
+            if ((port_mask >> ip) & 1)
+                for (int i = 0; i < 256; i++) {
+                    real d2 = 0;
+                    for (int chn = 0; chn < numcol; chn++) {
+                        real dc = deb[ip][chn][i] - avg[chn][i];
+                        d2 += dc * dc * chw[chn];
+                    }
+                    d2 *= lt_window[i];
+                    if (d2 > md) md = d2;
+                }
+            stats[ip] = sqrt((double)md);
+        }
+        for (int ip = 0; ip < ports; ip++)
+            for (int chn = 0; chn < numcol; chn++) {
+                real s = 0;
+                for (int i = 0; i < 256; i++) s += deb[ip][chn][i];
+                stats[ports + chn * ports + ip] = s / 256;
+            }
+    }
+    /* ID:2789-2801 final mix of the non-debayered tiles with the port weights */
+    for (int i = 0; i < 256; i++) {
+        real sw = 0;
+        for (int ip = 0; ip < ports; ip++) sw += pw[ip][i];
+        if (sw != 0) sw = 1 / sw;
+        for (int chn = 0; chn < numcol; chn++) {
+            real s = 0;
+            for (int ip = 0; ip < ports; ip++) s += sw * pw[ip][i] * iclt[ip][chn][i];
+            out[chn * 256 + i] = s;
+        }
+        out[3 * 256 + i] = alpha[i];
+        if (c->planes > 4) {
+            for (int ip = 0; ip < ports; ip++) out[(4 + ip) * 256 + i] = pw[ip][i];
+            for (int k = 0; k < 4; k++) out[(4 + ports + k) * 256 + i] = rms[k][i];
+        }
+    }
+    free(pw);
+    free(iclt);
+    free(deb);
+}
+
+/* ================================ exported API ================================ */
+EXPORT int orc_real_size(void) { return (int)sizeof(real); }
+
+EXPORT void orc_hwindow(double* out8) {
+    tables();
+    for (int i = 0; i < 8; i++) out8[i] = HW[i];
+}
+
+EXPORT void orc_tables(int* fold_index, int* unfold_index, double* unfold_k) {
+    tables();
+    if (fold_index) memcpy(fold_index, FOLD_INDEX, sizeof(FOLD_INDEX));
+    if (unfold_index) memcpy(unfold_index, UNFOLD_INDEX, sizeof(UNFOLD_INDEX));
+    if (unfold_k) memcpy(unfold_k, UNFOLD_K, sizeof(UNFOLD_K));
+}
+
+/* which: 0 dttt_iv, 1 dttt_iie, 2 dttt_iiie */
+EXPORT void orc_dttt(int which, const double* x, int mode, double* y) {
+    tables();
+    real xi[64], yo[64];
+    for (int i = 0; i < 64; i++) xi[i] = (real)x[i];
+    if (which == 0) dttt_iv(xi, mode, yo);
+    else if (which == 1) dttt_iie(xi, mode, yo);
+    else dttt_iiie(xi, mode, yo);
+    for (int i = 0; i < 64; i++) y[i] = yo[i];
+}
+
+EXPORT void orc_fold_tile(const double* x256, double shift_h, double shift_v, int mode, double* y64) {
+    tables();
+    real fk[4][64][4], xi[256], yo[64];
+    for (int i = 0; i < 256; i++) xi[i] = (real)x256[i];
+    shifted_fold_coeffs((real)shift_h, (real)shift_v, fk);
+    fold_tile(xi, mode, fk, yo);
+    for (int i = 0; i < 64; i++) y64[i] = yo[i];
+}
+
+EXPORT void orc_unfold_tile(const double* x64, int mode, double* y256) {
+    tables();
+    real xi[64], yo[256];
+    for (int i = 0; i < 64; i++) xi[i] = (real)x64[i];
+    unfold_tile(xi, mode, yo);
+    for (int i = 0; i < 256; i++) y256[i] = yo[i];
+}
+
+EXPORT void orc_lpf(double sigma, double* out64) {
+    tables();
+    real f[64];
+    make_lpf(sigma, f);
+    for (int i = 0; i < 64; i++) out64[i] = f[i];
+}
+
+EXPORT void orc_fract_shift(double* clt256, double sx, double sy) {
+    real c[4][64];
+    for (int i = 0; i < 256; i++) c[i >> 6][i & 63] = (real)clt256[i];
+    fract_shift(c, (real)sx, (real)sy);
+    for (int i = 0; i < 256; i++) clt256[i] = c[i >> 6][i & 63];
+}
+
+EXPORT void orc_convolve_tile(double* data256, const double* kern256) {
+    real d[4][64], k[4][64];
+    for (int i = 0; i < 256; i++) {
+        d[i >> 6][i & 63] = (real)data256[i];
+        k[i >> 6][i & 63] = (real)kern256[i];
+    }
+    convolve_tile(d, k);
+    for (int i = 0; i < 256; i++) data256[i] = d[i >> 6][i & 63];
+}
+
+/* One camera/colour tile, CPU order [4][h*8+v], residual shift; fract_shift applied if do_shift */
+EXPORT void orc_convert_one(const clt_geom* g, const float* image, int stride, const float* kernels,
+                            const float* offsets, int color, double cx, double cy, int do_shift,
+                            double* clt256, double* res2) {
+    tables();
+    real clt[4][64];
+    convert_one(g, image, stride, kernels, offsets, color, cx, cy, clt, res2);
+    if (do_shift) fract_shift(clt, (real)res2[0], (real)res2[1]);
+    for (int i = 0; i < 256; i++) clt256[i] = clt[i >> 6][i & 63];
+}
+
+EXPORT void orc_iclt_tile(const double* clt256, const double* lpf64, double* out256) {
+    tables();
+    real c[4][64], lpf[64], o[256];
+    for (int i = 0; i < 256; i++) c[i >> 6][i & 63] = (real)clt256[i];
+    if (lpf64)
+        for (int i = 0; i < 64; i++) lpf[i] = (real)lpf64[i];
+    iclt_tile(c, (real)0.25, lpf64 ? lpf : NULL, o);
+    for (int i = 0; i < 256; i++) out256[i] = o[i];
+}
+
+EXPORT void orc_corr_unfold(const double* q256, double* out225) {
+    real q[4][64], r[225];
+    for (int i = 0; i < 256; i++) q[i >> 6][i & 63] = (real)q256[i];
+    corr_unfold_tile(q, r);
+    for (int i = 0; i < 225; i++) out225[i] = r[i];
+}
+
+EXPORT void orc_strip(const double* corr225, int hwidth, int dir, int ss, double* strip) {
+    real c[225], s[16 * 15];
+    for (int i = 0; i < 225; i++) c[i] = (real)corr225[i];
+    strip_single(c, hwidth, dir, ss, s);
+    for (int i = 0; i < hwidth * 15; i++) strip[i] = s[i];
+}
+
+EXPORT int orc_combine_strips(const double* strips, int nstrips, int len, double offset, int twice_diagonal,
+                              double* combo) {
+    real* sp[CLT_MAX_PAIRS];
+    real* buf = malloc(sizeof(real) * (size_t)nstrips * len);
+    real* cb = malloc(sizeof(real) * (size_t)len);
+    for (int p = 0; p < nstrips; p++) {
+        sp[p] = buf + (size_t)p * len;
+        for (int i = 0; i < len; i++) sp[p][i] = (real)strips[(size_t)p * len + i];
+    }
+    int r = combine_strips(sp, nstrips, len, (real)offset, twice_diagonal, cb);
+    for (int i = 0; i < len; i++) combo[i] = cb[i];
+    free(buf);
+    free(cb);
+    return r;
+}
+
+EXPORT int orc_max_xy_int_axis(const double* data15, double min_max, int* ix) {
+    real d[15];
+    for (int i = 0; i < 15; i++) d[i] = (real)data15[i];
+    return max_xy_int_axis(d, (real)min_max, ix);
+}
+
+EXPORT void orc_half_flat_top_window(int ihwidth, double hwidth, double blur, int normalize, int notch,
+                                     double scale, double* wnd) {
+    half_flat_top_window(ihwidth, hwidth, blur, normalize, notch, scale, wnd);
+}
+
+EXPORT int orc_max_x_cm(const double* strip, int height, int ixcenter, double* rslt3) {
+    double wy[2], wx[9];
+    half_flat_top_window(2, 1.0, 1.0, 1, 0, 1.0, wy);   /* IDP:77-79 corr_wndy_* */
+    half_flat_top_window(9, 6.0, 5.0, 1, 0, 2.0, wx);   /* IDP:80-82 corr_wndx_*, wndx_scale 2 */
+    real s[16 * 15], r[3];
+    for (int i = 0; i < height * 15; i++) s[i] = (real)strip[i];
+    int ok = max_x_cm(s, height, ixcenter, wy, 2, wx, 9, r);
+    for (int i = 0; i < 3; i++) rslt3[i] = ok ? r[i] : NAN;
+    return ok;
+}
+
+EXPORT int orc_max_xs_ortho(const double* corrs, int npairs, double offset, int is_vert, double enh_w,
+                            double enh_s, double* rslt2) {
+    double notch[15];
+    ortho_notch(enh_w, enh_s, notch);
+    real* sel[CLT_MAX_PAIRS];
+    real* buf = malloc(sizeof(real) * 225 * (size_t)(npairs > 0 ? npairs : 1));
+    for (int p = 0; p < npairs; p++) {
+        sel[p] = buf + p * 225;
+        for (int i = 0; i < 225; i++) sel[p][i] = (real)corrs[p * 225 + i];
+    }
+    real r[2];
+    int ok = max_xs_ortho(sel, npairs, (real)offset, is_vert, notch, r);
+    rslt2[0] = ok ? r[0] : NAN;
+    rslt2[1] = ok ? r[1] : NAN;
+    free(buf);
+    return ok;
+}
+
+EXPORT void orc_tile_debayer(int rb, const double* tile, double* out) {
+    real t[256], o[256];
+    for (int i = 0; i < 256; i++) t[i] = (real)tile[i];
+    tile_debayer(rb, t, o);
+    for (int i = 0; i < 256; i++) out[i] = o[i];
+}
+
+/* ---- frame level ---- */
+EXPORT int orc_convert_frame(const clt_geom* g, const float* const* images, int stride, const float* const* kernels,
+                             const float* const* offsets, const int32_t* tasks, int ntasks, int lpf_mask,
+                             const double* lpf3x64, int no_fract_shift, double* clt_out, double* residual_out,
+                             int nthreads) {
+    tables();
+    convert_ctx c;
+    frame_init(&c.f, g);
+    c.images = images;
+    c.stride = stride;
+    c.kernels = kernels;
+    c.offsets = offsets;
+    c.tasks = tasks;
+    c.lpf_mask = lpf_mask;
+    c.lpf = lpf3x64;
+    c.no_fract_shift = no_fract_shift;
+    c.clt = clt_out;
+    c.residual = residual_out;
+    run_pool(ntasks, nthreads, convert_task, &c);
+    return 0;
+}
+
+/* valid: [tilesY*tilesX] bytes; out: [cams][colors][(H+8)*(W+8)] accumulated (must be zeroed) */
+EXPORT int orc_imclt_frame(const clt_geom* g, const double* clt, const uint8_t* valid, const double* lpf64,
+                           double* out, int nthreads) {
+    tables();
+    frame_t f;
+    frame_init(&f, g);
+    real lpf[64];
+    if (lpf64)
+        for (int i = 0; i < 64; i++) lpf[i] = (real)lpf64[i];
+    size_t plane = (size_t)(g->height + 8) * (g->width + 8);
+    real* buf = malloc(sizeof(real) * plane);
+    for (int cam = 0; cam < g->num_cams; cam++)
+        for (int col = 0; col < g->num_colors; col++) {
+            for (size_t i = 0; i < plane; i++) buf[i] = 0;
+            for (int pass = 0; pass < 4; pass++) {   /* ID:3588-3664: 4-pass checkerboard */
+                imclt_ctx c = {f, clt, valid, lpf64 ? lpf : NULL, cam, col, pass, buf};
+                int nx = (f.tiles_x + 1 - (pass & 1)) / 2;
+                int ny = (f.tiles_y + 1 - ((pass >> 1) & 1)) / 2;
+                run_pool(nx * ny, nthreads, imclt_tile_fn, &c);
+            }
+            double* o = out + ((size_t)cam * g->num_colors + col) * plane;
+            for (size_t i = 0; i < plane; i++) o[i] = buf[i];
+        }
+    free(buf);
+    return 0;
+}
+
+EXPORT int orc_corr_frame(const clt_geom* g, const double* clt, const int32_t* tasks, int ntasks,
+                          const clt_pair* pairs, int npairs, const clt_corr_params* prm, double corr_sigma,
+                          double* corr_out, double* combo_out, double* disp_out, int nthreads) {
+    tables();
+    if (npairs > CLT_MAX_PAIRS || prm->strip_height > 16) return CLT_E_ARG;
+    corr_ctx* c = calloc(1, sizeof(corr_ctx));
+    frame_init(&c->f, g);
+    c->clt = clt;
+    c->tasks = tasks;
+    c->pairs = pairs;
+    c->npairs = npairs;
+    c->prm = *prm;
+    make_lpf(corr_sigma, c->lpf);
+    color_weights(g, prm->corr_red, prm->corr_blue, c->col_weights);
+    half_flat_top_window(2, 1.0, 1.0, 1, 0, 1.0, c->wndy);
+    c->wndy_len = 2;
+    half_flat_top_window(9, 6.0, 5.0, 1, 0, 2.0, c->wndx);
+    c->wndx_len = 9;
+    ortho_notch(prm->enhortho_width, prm->enhortho_scale, c->notch);
+    c->corr_out = corr_out;
+    c->combo_out = combo_out;
+    c->disp_out = disp_out;
+    run_pool(ntasks, nthreads, corr_task, c);
+    free(c);
+    return 0;
+}
+
+EXPORT int orc_textures_frame(const clt_geom* g, const double* clt, const int32_t* tasks, int ntasks,
+                              const clt_tex_params* prm, const clt_corr_params* cprm, double corr_sigma,
+                              const double* extra_disparity, double* tiles_out, double* stats_out, int nthreads) {
+    tables();
+    if (g->num_colors != 3) return CLT_E_UNSUPPORTED;
+    tex_ctx* c = calloc(1, sizeof(tex_ctx));
+    frame_init(&c->f, g);
+    c->clt = clt;
+    c->tasks = tasks;
+    c->extra_disparity = extra_disparity;
+    c->prm = *prm;
+    make_lpf(corr_sigma, c->lpf);
+    color_weights(g, cprm->corr_red, cprm->corr_blue, c->col_weights);
+    for (int i = 0; i < 16; i++)   /* DR:695-712 getWin2d, squared (ID:1725-1726) */
+        for (int j = 0; j < 16; j++) {
+            double w = HW[(i < 8) ? i : 15 - i] * HW[(j < 8) ? j : 15 - j];
+            c->window2[i * 16 + j] = (real)(w * w);
+        }
+    c->planes = prm->keep_weights ? 4 + g->num_cams + 4 : 4;
+    c->tiles_out = tiles_out;
+    c->stats_out = stats_out;
+    run_pool(ntasks, nthreads, tex_task, c);
+    free(c);
+    return 0;
+}
+
+/* ID:3858-3989 combineRGBATiles(overlap = true, sharp_alpha = false) on texture tiles placed by
+ * the task list; out: [planes][height][width] (zeroed by the caller). */
+EXPORT int orc_combine_textures(const clt_geom* g, const int32_t* tasks, int ntasks, const double* tiles,
+                                int planes, double* out) {
+    frame_t f;
+    frame_init(&f, g);
+    const int W = g->width, H = g->height;
+    for (int t = 0; t < ntasks; t++) {
+        if (task_flags(g, tasks, t) == 0) continue;
+        int tx, ty;
+        task_txy(g, tasks, t, &tx, &ty);
+        const double* tt = tiles + (size_t)t * planes * 256;
+        for (int i = 0; i < 16; i++) {
+            int y = ty * 8 + i - 4;
+            if (y < 0 || y >= H) continue;
+            for (int j = 0; j < 16; j++) {
+                int x = tx * 8 + j - 4;
+                if (x < 0 || x >= W) continue;
+                for (int p = 0; p < planes; p++) out[((size_t)p * H + y) * W + x] += tt[p * 256 + i * 16 + j];
+            }
+        }
+    }
+    return 0;
+}

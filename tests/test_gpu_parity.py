@@ -1,0 +1,254 @@
+"""GPU parity of the four kernels against the CPU oracle (FP64 restatement of the reference).
+
+Tolerances (from the acceptance criteria of this path): CLT coefficients and reconstructed images
+within 1e-4 relative RMS, correlation-peak disparities within 0.01 px.  The FP32 error of the
+oracle's own single-precision build is printed next to the GPU error for scale.
+"""
+import numpy as np
+import pytest
+
+from helpers import (gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, task_tiles,
+                     valid_map)
+
+pytestmark = pytest.mark.gpu
+
+CLT_TOL = 1e-4
+IMG_TOL = 1e-4
+DISP_TOL = 0.01
+
+
+@pytest.fixture(scope="module")
+def quad_scene(pk):
+    return pk.synth.make_scene(128, 96, seed=11, disparity=("field", 0.0, 6.0))
+
+
+def _convert_case(pk, orc, sc, lpf_mask=0, lpf_sigmas=None, use_kernels=True):
+    tp = gpu_processor(pk, sc, use_kernels=use_kernels)
+    lpf3 = None
+    if lpf_sigmas is not None:
+        tp.setLpfRbg(*lpf_sigmas)
+        lpf3 = np.stack([orc.lpf(s) for s in lpf_sigmas])
+    tp.execConverCorrectTiles(lpf_mask)
+    g_clt = gpu_clt_cpu_layout(tp, sc)
+    g = oracle_geom(orc, sc)
+    kern = sc.kernels if (use_kernels and sc.kernels is not None) else None
+    offs = sc.offsets if kern is not None else None
+    o64 = orc.convert_frame(g, sc.images, sc.tasks, kernels=kern, offsets=offs, lpf_mask=lpf_mask, lpf3=lpf3)
+    o32 = orc.convert_frame(g, sc.images, sc.tasks, kernels=kern, offsets=offs, lpf_mask=lpf_mask, lpf3=lpf3,
+                            precision=32)
+    tiles = task_tiles(sc)
+    err = rel_rms(g_clt[:, tiles], o64[:, tiles])
+    err32 = rel_rms(o32[:, tiles], o64[:, tiles])
+    return tp, err, err32, o64
+
+
+def test_convert_quad_bayer_kernels(pk, orc, quad_scene):
+    log("case: convert quad 128x96 Bayer GR/BG, kernels, disparity 0..6")
+    tp, err, err32, _ = _convert_case(pk, orc, quad_scene)
+    log("  CLT rel RMS gpu=%.3g  fp32-oracle=%.3g" % (err, err32))
+    assert err < CLT_TOL
+    tp.close()
+
+
+def test_convert_rggb_no_kernels_borders(pk, orc):
+    log("case: convert quad 64x48 RGGB, no kernels, disparity 14 (windows cross the borders)")
+    sc = pk.synth.make_scene(64, 48, seed=5, disparity=14.0, red_row=0, red_col=0, kernels=False)
+    tp, err, err32, _ = _convert_case(pk, orc, sc)
+    log("  CLT rel RMS gpu=%.3g  fp32-oracle=%.3g" % (err, err32))
+    assert err < CLT_TOL
+    tp.close()
+
+
+def test_convert_lpf_mask(pk, orc, quad_scene):
+    log("case: convert with the reference GPU LPF placement (lpf_mask 7, sigma 1.1/1.1/0.7)")
+    tp, err, err32, _ = _convert_case(pk, orc, quad_scene, lpf_mask=7, lpf_sigmas=(1.1, 1.1, 0.7))
+    log("  CLT rel RMS gpu=%.3g  fp32-oracle=%.3g" % (err, err32))
+    assert err < CLT_TOL
+    tp.close()
+
+
+def test_convert_mono16_sparse(pk, orc):
+    log("case: convert 16-camera mono 160x128, 30% sparse tasks")
+    sc = pk.synth.make_scene(160, 128, num_cams=16, num_colors=1, seed=7, disparity=("field", 0.0, 4.0),
+                             task_fraction=0.3)
+    tp, err, err32, _ = _convert_case(pk, orc, sc)
+    log("  CLT rel RMS gpu=%.3g  fp32-oracle=%.3g" % (err, err32))
+    assert err < CLT_TOL
+    tp.close()
+
+
+def test_convert_skips_zero_tasks(pk, orc, quad_scene):
+    log("case: task records with task word 0 are not converted")
+    sc = quad_scene
+    tasks = sc.tasks.copy()
+    tasks[::3, 0] = 0
+    tp = gpu_processor(pk, sc)
+    tp.setTasks(tasks)
+    tp.execConverCorrectTiles()
+    g_clt = gpu_clt_cpu_layout(tp, sc)
+    o = orc.convert_frame(oracle_geom(orc, sc), sc.images, tasks, kernels=sc.kernels, offsets=sc.offsets)
+    live = tasks[:, 0] != 0
+    tiles = ((tasks[:, 1] >> 16) * (sc.width // 8) + (tasks[:, 1] & 0xFFFF))[live]
+    assert rel_rms(g_clt[:, tiles], o[:, tiles]) < CLT_TOL
+    tp.close()
+
+
+def test_imclt(pk, orc, quad_scene):
+    sc = quad_scene
+    for apply_lpf in (False, True):
+        log("case: imclt quad 128x96, lpf=%s" % apply_lpf)
+        tp = gpu_processor(pk, sc)
+        tp.setCorrSigma(0.8)
+        tp.execConverCorrectTiles()
+        tp.execImcltRbg(apply_lpf)
+        g_img = np.stack([tp.getRBG(c) for c in range(sc.num_cams)]).reshape(sc.num_cams, 3, sc.height + 8, sc.width + 8)
+        g_clt = gpu_clt_cpu_layout(tp, sc).astype(np.float64)
+        lpf = orc.lpf(0.8) if apply_lpf else None
+        # same input: the GPU's own CLT through the oracle's inverse
+        o_img = orc.imclt_frame(oracle_geom(orc, sc), g_clt, valid_map(sc), lpf64=lpf)
+        err = rel_rms(g_img, o_img)
+        # end to end against the FP64 convert + inverse
+        o_clt = orc.convert_frame(oracle_geom(orc, sc), sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+        o_img2 = orc.imclt_frame(oracle_geom(orc, sc), o_clt, valid_map(sc), lpf64=lpf)
+        err2 = rel_rms(g_img, o_img2)
+        log("  image rel RMS same-input=%.3g end-to-end=%.3g" % (err, err2))
+        assert err < IMG_TOL and err2 < IMG_TOL
+        tp.close()
+
+
+def test_imclt_sparse_tasks(pk, orc):
+    log("case: imclt with a sparse task list (untouched tiles contribute nothing)")
+    sc = pk.synth.make_scene(128, 96, seed=3, disparity=2.0, task_fraction=0.4)
+    tp = gpu_processor(pk, sc)
+    tp.execConverCorrectTiles()
+    tp.execImcltRbg(False)
+    g_img = np.stack([tp.getRBG(c) for c in range(sc.num_cams)]).reshape(sc.num_cams, 3, sc.height + 8, sc.width + 8)
+    o_clt = orc.convert_frame(oracle_geom(orc, sc), sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+    o_img = orc.imclt_frame(oracle_geom(orc, sc), o_clt, valid_map(sc))
+    err = rel_rms(g_img, o_img)
+    log("  image rel RMS=%.3g" % err)
+    assert err < IMG_TOL
+    tp.close()
+
+
+def _corr_compare(pk, orc, sc, fat_zero=0.0, pairs=None, strip=True):
+    tp = gpu_processor(pk, sc)
+    tp.setCorrSigma(0.8)
+    tp.setCorrParams(fat_zero=fat_zero)
+    if pairs is not None:
+        tp.setCorrPairs(pairs)
+    tp.execConverCorrectTiles()
+    tp.execCorrelate2D()
+    g_corr, g_combo, g_disp = tp.getCorr2D()
+    g_clt = gpu_clt_cpu_layout(tp, sc).astype(np.float64)
+    prm = orc.default_corr_params(fat_zero=fat_zero)
+    o_corr, o_combo, o_disp = orc.corr_frame(oracle_geom(orc, sc), g_clt, sc.tasks, pairs=pairs, params=prm)
+    o_clt = orc.convert_frame(oracle_geom(orc, sc), sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+    e_corr, e_combo, e_disp = orc.corr_frame(oracle_geom(orc, sc), o_clt, sc.tasks, pairs=pairs, params=prm)
+    res = dict(corr=rel_rms(g_corr, o_corr), corr_e2e=rel_rms(g_corr, e_corr))
+    if strip:
+        assert np.array_equal(np.isnan(g_combo), np.isnan(o_combo)), "NaN pattern of the combined strip"
+        res["combo"] = rel_rms(g_combo, o_combo)
+        for name, k in (("int", 0), ("cm", 1), ("hor", 3), ("vert", 5)):
+            for ref, tag in ((o_disp, ""), (e_disp, "_e2e")):
+                both = np.isfinite(ref[:, k]) & np.isfinite(g_disp[:, k])
+                assert np.array_equal(np.isfinite(ref[:, k]), np.isfinite(g_disp[:, k])), name + tag
+                res[name + tag] = float(np.abs(g_disp[both, k] - ref[both, k]).max()) if both.any() else 0.0
+        res["strength"] = rel_rms(g_disp[:, 2], o_disp[:, 2])
+    tp.close()
+    return res
+
+
+def test_corr2d_quad(pk, orc, quad_scene):
+    log("case: correlate2D quad 128x96, 6 pairs, fat_zero 0")
+    r = _corr_compare(pk, orc, quad_scene)
+    log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
+    assert r["corr"] < CLT_TOL and r["corr_e2e"] < CLT_TOL and r["combo"] < CLT_TOL
+    assert r["int"] == 0 and r["cm"] < DISP_TOL and r["hor"] < DISP_TOL and r["vert"] < DISP_TOL
+    assert r["cm_e2e"] < DISP_TOL and r["strength"] < CLT_TOL
+
+
+def test_corr2d_fat_zero(pk, orc, quad_scene):
+    log("case: correlate2D quad, fat_zero 0.05")
+    r = _corr_compare(pk, orc, quad_scene, fat_zero=0.05)
+    log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
+    assert r["corr"] < CLT_TOL and r["cm"] < DISP_TOL
+
+
+def test_corr2d_mono16_all_pairs(pk, orc):
+    log("case: correlate2D 16-camera mono 160x128, 30% tasks, 120 pairs")
+    sc = pk.synth.make_scene(160, 128, num_cams=16, num_colors=1, seed=9, disparity=("field", 0.0, 4.0),
+                             task_fraction=0.3)
+    pairs = [(i, j, -1, 1) for i in range(16) for j in range(i + 1, 16)]
+    r = _corr_compare(pk, orc, sc, pairs=pairs, strip=False)
+    log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
+    assert r["corr"] < CLT_TOL and r["corr_e2e"] < CLT_TOL
+
+
+def _tex_compare(pk, orc, sc, extra=None, tasks=None):
+    tasks = sc.tasks if tasks is None else tasks
+    tp = gpu_processor(pk, sc)
+    tp.setTasks(tasks)
+    tp.setCorrSigma(0.8)
+    tp.execConverCorrectTiles()
+    tp.execTextures(extra)
+    g_tiles, g_stats = tp.getTextures()
+    tp.execCombineTextures()
+    g_img = tp.getCombinedTextures()
+    g_clt = gpu_clt_cpu_layout(tp, sc).astype(np.float64)
+    o_tiles, o_stats = orc.textures_frame(oracle_geom(orc, sc), g_clt, tasks, extra_disparity=extra)
+    o_img = orc.combine_textures(oracle_geom(orc, sc), tasks, o_tiles)
+    live = tasks[:, 0] != 0
+    res = {"rgb": rel_rms(g_tiles[live, :3], o_tiles[live, :3]), "alpha": rel_rms(g_tiles[live, 3], o_tiles[live, 3]),
+           "weights": rel_rms(g_tiles[live, 4:8], o_tiles[live, 4:8]), "rms": rel_rms(g_tiles[live, 8:], o_tiles[live, 8:]),
+           "stats": rel_rms(g_stats[live], o_stats[live]), "image": rel_rms(g_img[:4], o_img[:4])}
+    tp.close()
+    return res
+
+
+def test_textures_quad(pk, orc, quad_scene):
+    log("case: textures quad 128x96, all ports")
+    r = _tex_compare(pk, orc, quad_scene)
+    log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
+    assert r["rgb"] < IMG_TOL and r["alpha"] < IMG_TOL and r["image"] < IMG_TOL
+    assert r["weights"] < 1e-3 and r["rms"] < 1e-3 and r["stats"] < 1e-3
+
+
+def test_textures_extra_disparity_port_masks(pk, orc, quad_scene):
+    log("case: textures with extra disparity, partial port masks, forced-disparity bit")
+    sc = quad_scene
+    rng = np.random.default_rng(4)
+    tasks = sc.tasks.copy()
+    masks = np.array([0xF, 0x7, 0xB, 0x3, 0x1, 0x1000000 | 0xF], np.int32)
+    tasks[:, 0] = masks[rng.integers(0, len(masks), len(tasks))]
+    extra = rng.uniform(-1.5, 1.5, len(tasks)).astype(np.float32)
+    r = _tex_compare(pk, orc, sc, extra=extra.astype(np.float64), tasks=tasks)
+    log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
+    assert r["rgb"] < IMG_TOL and r["alpha"] < IMG_TOL
+    assert r["weights"] < 1e-3
+
+
+def test_config0_pipeline_disparity(pk, orc):
+    """4 x 512x512 Bayer, constant 3.0 px disparity, all tiles: convert + corr + imclt."""
+    log("case: 512x512 quad frame, disparity 3.0 px, full pipeline")
+    sc = pk.synth.make_scene(512, 512, seed=20261203, disparity=3.0)
+    tp = gpu_processor(pk, sc)
+    tp.setCorrSigma(0.8)
+    tp.execConverCorrectTiles()
+    tp.execCorrelate2D()
+    tp.execImcltRbg(True)
+    _, _, g_disp = tp.getCorr2D()
+    g_img = np.stack([tp.getRBG(c) for c in range(4)]).reshape(4, 3, 520, 520)
+    g = oracle_geom(orc, sc)
+    o_clt = orc.convert_frame(g, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+    _, _, o_disp = orc.corr_frame(g, o_clt, sc.tasks)
+    o_img = orc.imclt_frame(g, o_clt, valid_map(sc), lpf64=orc.lpf(0.8))
+    g_clt = gpu_clt_cpu_layout(tp, sc)
+    e_clt = rel_rms(g_clt, o_clt)
+    e_img = rel_rms(g_img, o_img)
+    fin = np.isfinite(o_disp[:, 1]) & np.isfinite(g_disp[:, 1])
+    e_disp = float(np.abs(g_disp[fin, 1] - o_disp[fin, 1]).max())
+    log("  CLT=%.3g image=%.3g cm-disparity max|d|=%.3g px over %d tiles" % (e_clt, e_img, e_disp, fin.sum()))
+    assert e_clt < CLT_TOL and e_img < IMG_TOL and e_disp < DISP_TOL
+    assert np.array_equal(np.isfinite(o_disp[:, 1]), np.isfinite(g_disp[:, 1]))
+    tp.close()
