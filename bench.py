@@ -1,0 +1,156 @@
+"""Benchmark of the CLT tile processor on one MI355X.
+
+Workload (one "step" = one frame through the plain per-frame GPU path): 4 x 2592x1936 Bayer
+(GR/BG) float32 mosaics, 78,408 tiles, dense task list, per-tile disparity a smooth random field
+0..20 px, random per-tile aberration kernels; convert_direct -> correlate2D (6 pairs + combined
+strip + disparity) -> textures (RGBA) -> imclt_rbg (with LPF).  Inputs are resident in HBM before
+the timed region; outputs stay on the device.
+
+Prints one JSON line: tiles/s, HBM roofline of the dominant kernel (HIP events on the library's
+stream) and the CPU baseline (the FP64 C restatement of the reference CPU path, all host cores).
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CLT tiles/s per frame (convert+corr+imclt), % HBM roofline; x vs Java CPU"
+HBM_PEAK_GBS = 8000.0
+W, H, C, K = 2592, 1936, 4, 3
+
+
+def algorithmic_bytes(sc, planes, n_pairs, strip_h):
+    """Compulsory bytes at each kernel's interface (f32), per launch."""
+    T = len(sc.tasks)
+    tw = 4 * (2 + 2 * C)
+    clt = C * T * K * 1024
+    # kernel tiles actually used: round(centre/16)+1, clamped (ImageDtt.java:4906-4911)
+    xy = sc.tasks[:, 2:].view(np.float32).reshape(T, C, 2).astype(np.float64)
+    ktx = np.clip(np.floor(xy[..., 0] / 16 + 0.5).astype(int) + 1, 0, sc.kernels_hor - 1)
+    kty = np.clip(np.floor(xy[..., 1] / 16 + 0.5).astype(int) + 1, 0, sc.kernels_vert - 1)
+    used = np.unique((np.arange(C)[None, :] * sc.kernels_vert + kty) * sc.kernels_hor + ktx).size
+    return {
+        "convert": C * W * H * 4 + used * K * (256 + 8) * 4 + T * tw + clt,
+        "corr2d": clt + T * tw + T * n_pairs * 225 * 4 + T * strip_h * 15 * 4 + T * 8 * 4,
+        "textures": clt + T * tw + T * planes * 1024 + T * 4 * C * 4,
+        "imclt": clt + C * K * (W + 8) * (H + 8) * 4 + (W // 8) * (H // 8) * 4,
+    }
+
+
+def cpu_baseline(sc, min_s=10.0, max_passes=20):
+    """FP64 oracle (C restatement of the reference CPU path, one worker per host core, atomic tile
+    counter as ImageDtt.java:5847-5872) on the same frame, repeated for at least min_s seconds."""
+    import oracle as orc
+    cores = os.cpu_count() or 1
+    g = orc.make_geom(W, H)
+    valid = np.zeros((H // 8) * (W // 8), np.uint8)
+    valid[((sc.tasks[:, 1] >> 16) * (W // 8) + (sc.tasks[:, 1] & 0xFFFF))] = 1
+    tp = orc.default_tex_params(keep_weights=0)
+    lpf = orc.lpf(0.8)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        clt = orc.convert_frame(g, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets, nthreads=cores)
+        orc.corr_frame(g, clt, sc.tasks, nthreads=cores)
+        orc.textures_frame(g, clt, sc.tasks, tex_params=tp, nthreads=cores)
+        orc.imclt_frame(g, clt, valid, lpf64=lpf, nthreads=cores)
+        del clt
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= min_s or n >= max_passes:
+            break
+    return {"value": round(n * len(sc.tasks) / dt, 1), "unit": "tiles/s", "cores": cores, "kind": "port",
+            "sample": "the same 78408-tile frame, convert+corr+textures+imclt in FP64, %d pass(es) in %.1f s"
+                      % (n, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=20261204)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the tile path has no multi-GPU form yet); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    pk = importlib.import_module("imagej-elphel_amd")
+    if pk.load().clt_device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    print("bench: generating the 2592x1936 quad frame ...", file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    sc = pk.synth.make_scene(W, H, seed=args.seed, disparity=("field", 0.0, 20.0))
+    print("bench: scene ready in %.1f s, %d tiles" % (time.perf_counter() - t, len(sc.tasks)), file=sys.stderr, flush=True)
+
+    tp = pk.GPUTileProcessor(W, H, C, K)
+    tp.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
+    tp.setBayerImages(sc.images, force=True)
+    tp.setTasks(sc.tasks)
+    tp.setCorrSigma(0.8)
+    tp.setTexParams(keep_weights=0)
+    tp.setAsync(True)
+
+    def step():
+        tp.execConverCorrectTiles(0)
+        tp.execCorrelate2D()
+        tp.execTextures()
+        tp.execImcltRbg(True)
+
+    for _ in range(args.warmup):
+        step()
+    tp.sync()
+    tp.setTiming(True)
+    tp.resetTiming()
+    tp.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    tp.sync()
+    dt = time.perf_counter() - t0
+    ms_step = 1000 * dt / args.steps
+    T = len(sc.tasks)
+    value = T / (dt / args.steps)
+
+    ab = algorithmic_bytes(sc, planes=4, n_pairs=6, strip_h=9)
+    kernels = {}
+    for name in ("convert", "corr2d", "textures", "imclt"):
+        ms, cnt = tp.getTiming(name)
+        avg = ms / max(cnt, 1)
+        gbs = ab[name] / (avg * 1e-3) / 1e9
+        kernels[name] = {"avg_ms": round(avg, 4), "launches": cnt, "alg_MB": round(ab[name] / 1e6, 1),
+                         "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    total_alg = sum(ab.values())
+    roof = {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": kernels[dom]["frac"], "traffic": None,
+            "frame_alg_MB": round(total_alg / 1e6, 1),
+            "frame_frac": round(total_alg / (sum(k["avg_ms"] for k in kernels.values()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    tp.close()
+    cpu = None
+    if not args.no_cpu:
+        print("bench: timing the CPU baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(sc)
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "tiles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "quad 2592x1936 Bayer frame, 78408 tiles, dense tasks, disparity 0-20 px; "
+                               "convert+correlate2D(6 pairs)+textures(RGBA)+imclt_rbg",
+                   "tiles": T, "cameras": C, "colors": K, "pairs": 6},
+        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

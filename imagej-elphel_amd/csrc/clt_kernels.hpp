@@ -259,7 +259,8 @@ __global__ __launch_bounds__(256) void k_imclt(const float* __restrict__ clt, co
 // ====================================================================================
 // correlate2D: pairwise phase correlation (Correlation2d.java:197-389), combined strip
 // (C2:556-790) and per-tile disparity (ImageDtt.java:2186-2238).
-// Block: 4 waves, TPB tiles; a wave handles one (tile, pair) at a time, lane = frequency bin.
+// One wavefront per task tile (4 tiles per block, no block barriers); lane = frequency bin in
+// the transform-domain product, then lane-parallel DCT-IIe passes through 2 KiB of LDS.
 // ====================================================================================
 constexpr int CORR_MAX_STRIP_PAIRS = 8;
 
@@ -270,9 +271,10 @@ struct CorrArgs {
     int strip_slot[CLT_MAX_PAIRS];             // index among the strip pairs or -1
     int num_strip;                             // pairs with a known orientation (<= 8)
     int strip_dir[CORR_MAX_STRIP_PAIRS];
-    int strip_ss[CORR_MAX_STRIP_PAIRS];
+    int strip_lg[CORR_MAX_STRIP_PAIRS];        // log2 of the interpolation denominator ss*(dir>1?1:2)
     int strip_is_hor1[CORR_MAX_STRIP_PAIRS];   // dir 0 and scale 1 (getMaskHorizontal(1))
     int strip_is_vert1[CORR_MAX_STRIP_PAIRS];  // dir 1 and scale 1 (getMaskVertical(1))
+    int n_hor1, n_vert1;
     float col_w[3];
     float fat_zero, corr_offset, min_corr;
     int twice_diagonal, strip_height;
@@ -285,50 +287,84 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// LDS floats needed by k_corr2d for TPB tiles
-__host__ __device__ inline int corr_lds_floats(int tpb, int cams, int colors) {
-    int a = tpb * cams * colors * 256, b = tpb * CORR_MAX_STRIP_PAIRS * 240;
-    return (a > b ? a : b) + 4 * 512 + tpb * CORR_MAX_STRIP_PAIRS * CLT_CORR_LEN;
+// LDS floats per wavefront of k_corr2d
+__host__ __device__ inline int corr_wave_floats(int num_strip) {
+    return 512 + num_strip * CLT_CORR_LEN + 240 + 32 + 3;
 }
 
-template <int TPB>
+// x^(1/n) for x > 0 with the hardware log2/exp2
+__device__ __forceinline__ float pow_inv(float x, float inv_n) { return exp2f(log2f(x) * inv_n); }
+
+// One sample of scaleRotateInterpoateSingleCorrelation (C2:610-718): the strip element (row,
+// scol) of a pair with orientation dir, interpolation denominator 1 << lg.  NaN outside the range.
+__device__ __forceinline__ float strip_value(const float* corr, int dir, int lg, int row, int scol) {
+    const int mask = (1 << lg) - 1, ilimit = 7 << lg;
+    const float rden = 1.f / (float)(1 << lg);
+    float acc = 0.f;
+    bool nan = false;
+    for (int down = -1; down < ((row == 0) ? 0 : 2); down += 2) {
+        int xnum, ynum;
+        switch (dir) {
+        case 0: xnum = 2 * scol + (row & 1); ynum = down * row; break;
+        case 1: xnum = down * row; ynum = 2 * scol + (row & 1); break;
+        case 2: xnum = scol - ((down * row) >> 1); ynum = scol + ((down * row + 1) >> 1); break;
+        default: xnum = scol - ((down * row) >> 1); ynum = -scol - ((down * row + 1) >> 1); break;
+        }
+        if ((xnum > ilimit) || (xnum < -ilimit) || (ynum > ilimit) || (ynum < -ilimit)) {
+            nan = true;   // the reference stores NaN there; a later += keeps it NaN
+            continue;
+        }
+        const int ix0 = (xnum + ilimit) >> lg, iy0 = (ynum + ilimit) >> lg;
+        const int fx = (xnum + ilimit) & mask, fy = (ynum + ilimit) & mask;   // exact dyadic fractions
+        const float* c = corr + iy0 * 15 + ix0;
+        float d;
+        if (fx == 0 && fy == 0) {
+            d = c[0];
+        } else {
+            const float dx = fx * rden, dy = fy * rden;
+            if (fy == 0) d = (1.f - dx) * c[0] + dx * c[1];
+            else if (fx == 0) d = (1.f - dy) * c[0] + dy * c[15];
+            else d = ((1.f - dx) * (1.f - dy) * c[0]) + (dx * (1.f - dy) * c[1]) + ((1.f - dx) * dy * c[15]) + (dx * dy * c[16]);
+        }
+        acc = (row == 0) ? d : acc + 0.5f * d;
+    }
+    return nan ? __int_as_float(0x7fc00000) : acc;
+}
+
 __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                 int task_words, int num_tasks, const float* __restrict__ lpf,
                                                 const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
                                                 float* __restrict__ combo_out, float* __restrict__ disp_out, GeomDev g) {
     extern __shared__ __attribute__((aligned(16))) float s_dyn[];
-    __shared__ int s_task[TPB], s_valid[TPB];
     const CorrArgs& A = *args_p;
-    const int C = g.cams, K = g.colors, P = A.num_pairs;
-    const int tile_floats = C * K * 256;
-    const int region1 = max(TPB * tile_floats, TPB * CORR_MAX_STRIP_PAIRS * 240);
-    float* s_clt = s_dyn;                                   // [TPB][C][K][256]   (phase 1-2)
-    float* s_strip = s_dyn;                                 // [TPB][8][240]      (phase 3, aliases s_clt)
-    float* s_wave = s_dyn + region1;                        // [4][512]
-    float* s_corr = s_wave + 4 * 512;                       // [TPB][8][225]
-    const float qnan = __int_as_float(0x7fc00000);
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int K = g.colors, P = A.num_pairs;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
-
-    if (threadIdx.x < TPB) {
-        const int t = blk * TPB + threadIdx.x;
-        s_task[threadIdx.x] = t;
-        s_valid[threadIdx.x] = (t < num_tasks) ? (tasks[(size_t)t * task_words] != 0) : 0;
-    }
-    __syncthreads();
-    // ---- stage the CLT of the block's tiles (all cameras and colours) in LDS
-#pragma unroll
-    for (int j = 0; j < TPB; j++) {
-        if (!s_valid[j]) continue;
-        const int txy = tasks[(size_t)s_task[j] * task_words + 1];
-        const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
-        for (int i = threadIdx.x; i < tile_floats / 4; i += 256) {
-            const int cam = i / (K * 64), w = i % (K * 64);
-            float4 v = *reinterpret_cast<const float4*>(clt + ((size_t)cam * ntiles + tile) * K * 256 + w * 4);
-            *reinterpret_cast<float4*>(s_clt + j * tile_floats + cam * K * 256 + w * 4) = v;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+    if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
+    const int wfl = (corr_wave_floats(A.num_strip) + 3) & ~3;
+    float* S0 = s_dyn + wave * wfl;                   // [256]
+    float* S1 = S0 + 256;                             // [256]
+    float* s_corr = S1 + 256;                         // [num_strip][225]
+    float* s_combo = s_corr + A.num_strip * CLT_CORR_LEN;   // [240]
+    float* s_c1d = s_combo + 240;                     // [2][16]
+    const float qnan = __int_as_float(0x7fc00000);
+    const int hw = A.strip_height, slen = hw * 15;
+    const int32_t* rec = tasks + (size_t)t * task_words;
+    if (rec[0] == 0) {
+        for (int o = lane; o < P * CLT_CORR_LEN; o += 64) corr_out[(size_t)t * P * CLT_CORR_LEN + o] = qnan;
+        if (A.num_strip) {
+            if (combo_out)
+                for (int e = lane; e < slen; e += 64) combo_out[(size_t)t * slen + e] = qnan;
+            if (lane < CLT_DISP_LAYERS && disp_out)
+                disp_out[(size_t)t * CLT_DISP_LAYERS + lane] =
+                    (lane == CLT_DISP_STRENGTH || lane == CLT_DISP_HOR_STR || lane == CLT_DISP_VERT_STR ||
+                     lane == CLT_DISP_CM_WEIGHT) ? 0.f : qnan;
         }
+        return;
     }
+    const int txy = rec[1];
+    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+    const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
     // per-lane DCT-IIe / DST-IIe rows: pass 1 uses row y = lane >> 3, pass 2 row x = lane & 7
     const int y = lane >> 3, xx = lane & 7;
     float cy_[8], sy_[8], cx_[8], sx_[8];
@@ -340,28 +376,17 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
         sx_[k] = c_tab.siie[xx][k];
     }
     const float lp = lpf[lane];
-    __syncthreads();
-
-    float* S0 = s_wave + wave * 512;
-    float* S1 = S0 + 256;
-    for (int u = wave; u < TPB * P; u += 4) {
-        const int j = u / P, p = u % P;
-        const int t = s_task[j];
-        if (t >= num_tasks) continue;
-        float* go = corr_out + ((size_t)t * P + p) * CLT_CORR_LEN;
-        if (!s_valid[j]) {
-            for (int o = lane; o < CLT_CORR_LEN; o += 64) go[o] = qnan;
-            continue;
-        }
-        const float* c1 = s_clt + j * tile_floats + A.cam1[p] * K * 256;
-        const float* c2 = s_clt + j * tile_floats + A.cam2[p] * K * 256;
+    float* corr_t = corr_out + (size_t)t * P * CLT_CORR_LEN;
+    for (int p = 0; p < P; p++) {
+        const float* c1 = clt + ((size_t)A.cam1[p] * ntiles + tile) * K * 256 + lane;
+        const float* c2 = clt + ((size_t)A.cam2[p] * ntiles + tile) * K * 256 + lane;
         float acc[4] = {0.f, 0.f, 0.f, 0.f};
         for (int col = 0; col < K; col++) {
             float d1[4], d2[4];
 #pragma unroll
             for (int n = 0; n < 4; n++) {
-                d1[n] = c1[col * 256 + n * 64 + lane];
-                d2[n] = c2[col * 256 + n * 64 + lane];
+                d1[n] = c1[col * 256 + n * 64];
+                d2[n] = c2[col * 256 + n * 64];
             }
             const float s1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] + d1[3] * d1[3];
             const float s2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2] + d2[3] * d2[3];
@@ -417,7 +442,7 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
         // unfold the 4 quadrants to 15x15 (DttRad2.java:443-489)
         const int slot = A.strip_slot[p];
         for (int o = lane; o < CLT_CORR_LEN; o += 64) {
-            const int row = o / 15, cc = o % 15;
+            const int row = o / 15, cc = o - row * 15;
             const int dy = row - 7, dx = cc - 7;
             const int ay = dy < 0 ? -dy : dy, ax = dx < 0 ? -dx : dx;
             float v = S0[ay * 8 + ax];
@@ -425,180 +450,130 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
             if (ay > 0) v += (dy > 0 ? 1.f : -1.f) * S0[128 + (ay - 1) * 8 + ax];
             if (ax > 0 && ay > 0) v += ((dx > 0) == (dy > 0) ? 1.f : -1.f) * S0[192 + (ay - 1) * 8 + ax - 1];
             v *= 0.25f;
-            go[o] = v;
-            if (slot >= 0) s_corr[(j * CORR_MAX_STRIP_PAIRS + slot) * CLT_CORR_LEN + o] = v;
+            corr_t[p * CLT_CORR_LEN + o] = v;
+            if (slot >= 0) s_corr[slot * CLT_CORR_LEN + o] = v;
         }
         wave_sync();
     }
-    __syncthreads();   // s_clt dead from here; s_strip aliases it
-    if (A.num_strip == 0 || wave >= TPB) return;
-    // ---- strips, combination and disparity: one wave per tile
-    const int j = wave;
-    const int t = s_task[j];
-    if (t >= num_tasks) return;
-    const int hw = A.strip_height, slen = hw * 15;
-    if (!s_valid[j]) {
-        if (combo_out)
-            for (int e = lane; e < slen; e += 64) combo_out[(size_t)t * slen + e] = qnan;
-        if (lane == 0 && disp_out) {
-            float* d = disp_out + (size_t)t * CLT_DISP_LAYERS;
-            for (int i = 0; i < CLT_DISP_LAYERS; i++) d[i] = qnan;
-            d[CLT_DISP_STRENGTH] = d[CLT_DISP_HOR_STR] = d[CLT_DISP_VERT_STR] = d[CLT_DISP_CM_WEIGHT] = 0.f;
-        }
-        return;
-    }
-    float* sst = s_strip + j * CORR_MAX_STRIP_PAIRS * 240;
-    const float* sc = s_corr + j * CORR_MAX_STRIP_PAIRS * CLT_CORR_LEN;
-    // scaleRotateInterpoateSingleCorrelation (C2:610-718)
-    for (int sidx = 0; sidx < A.num_strip; sidx++) {
-        const int dir = A.strip_dir[sidx], ss = A.strip_ss[sidx];
-        const int denom = ss * ((dir > 1) ? 1 : 2);
-        const float rdenom = (float)denom;
-        const int ilimit = 7 * denom;
-        const float* corr = sc + sidx * CLT_CORR_LEN;
-        for (int e = lane; e < slen; e += 64) {
-            const int row = e / 15, scol = e % 15 - 7;
-            float acc = 0.f;
-            bool nan = false;
-            for (int down = -1; down < ((row == 0) ? 0 : 2); down += 2) {
-                int xnum, ynum;
-                switch (dir) {
-                case 0: xnum = 2 * scol + (row & 1); ynum = down * row; break;
-                case 1: xnum = down * row; ynum = 2 * scol + (row & 1); break;
-                case 2: xnum = scol - ((down * row) >> 1); ynum = scol + ((down * row + 1) >> 1); break;
-                default: xnum = scol - ((down * row) >> 1); ynum = -scol - ((down * row + 1) >> 1); break;
-                }
-                if ((xnum > ilimit) || (xnum < -ilimit) || (ynum > ilimit) || (ynum < -ilimit)) {
-                    nan = true;   // the reference stores NaN there; a later += keeps it NaN
-                    continue;
-                }
-                const int ix0 = (xnum + ilimit) / denom, iy0 = (ynum + ilimit) / denom;
-                const bool int_x = (xnum % denom) == 0, int_y = (ynum % denom) == 0;
-                float d;
-                if (int_x && int_y) {
-                    d = corr[iy0 * 15 + ix0];
-                } else {
-                    const float dx = (xnum + ilimit) / rdenom - ix0;
-                    const float dy = (ynum + ilimit) / rdenom - iy0;
-                    if (int_y) d = (1.f - dx) * corr[iy0 * 15 + ix0] + dx * corr[iy0 * 15 + ix0 + 1];
-                    else if (int_x) d = (1.f - dy) * corr[iy0 * 15 + ix0] + dy * corr[iy0 * 15 + ix0 + 15];
-                    else
-                        d = ((1.f - dx) * (1.f - dy) * corr[iy0 * 15 + ix0]) + (dx * (1.f - dy) * corr[iy0 * 15 + ix0 + 1]) +
-                            ((1.f - dx) * dy * corr[iy0 * 15 + ix0 + 15]) + (dx * dy * corr[iy0 * 15 + ix0 + 16]);
-                }
-                acc = (row == 0) ? d : acc + 0.5f * d;
-            }
-            sst[sidx * 240 + e] = nan ? qnan : acc;
-        }
-    }
-    wave_sync();
-    // combineInterpolatedCorrelations (C2:730-790); NaN samples stay NaN
-    float* combo = s_wave + wave * 512;   // this wave's scratch is free now
+    if (A.num_strip == 0) return;
+    // ---- combined strip: combineInterpolatedCorrelations of the rotated strips (C2:730-790)
     {
         const int ncomb = A.num_strip * (A.twice_diagonal ? 2 : 1);
+        const float inv_n = 1.f / ncomb;
         for (int e = lane; e < slen; e += 64) {
+            const int row = e / 15, scol = e - row * 15 - 7;
             float v;
             if (A.corr_offset >= 0.f) {
                 v = 1.f;
                 for (int sidx = 0; sidx < A.num_strip; sidx++) {
-                    float d = sst[sidx * 240 + e] + A.corr_offset;
+                    float d = strip_value(s_corr + sidx * CLT_CORR_LEN, A.strip_dir[sidx], A.strip_lg[sidx], row, scol) +
+                              A.corr_offset;
                     if (d < 0.f) d = 0.f;
                     v *= A.twice_diagonal ? d * d : d;
                 }
-                if (v > 0.f) v = powf(v, 1.f / ncomb) - A.corr_offset;
+                if (v > 0.f) v = pow_inv(v, inv_n) - A.corr_offset;
             } else {
                 v = 0.f;
-                for (int sidx = 0; sidx < A.num_strip; sidx++) v += (A.twice_diagonal ? 2.f : 1.f) * sst[sidx * 240 + e];
-                v *= 1.f / ncomb;
+                for (int sidx = 0; sidx < A.num_strip; sidx++)
+                    v += (A.twice_diagonal ? 2.f : 1.f) *
+                         strip_value(s_corr + sidx * CLT_CORR_LEN, A.strip_dir[sidx], A.strip_lg[sidx], row, scol);
+                v *= inv_n;
             }
-            combo[e] = v;
+            s_combo[e] = v;
             if (combo_out) combo_out[(size_t)t * slen + e] = v;
         }
     }
-    wave_sync();
-    // getMaxXSOrtho (C2:2502-2651) 1-D profiles: lanes 0..14 horizontal pairs, 32..46 vertical
-    float* c1d = combo + 256;   // [2][16]
+    // ---- getMaxXSOrtho 1-D profiles (C2:2502-2651): lanes 0..14 horizontal, 32..46 vertical
     {
         const int which = lane >> 5, jj = lane & 31;
-        if (jj < 15) {
-            int npairs = 0;
-            for (int sidx = 0; sidx < A.num_strip; sidx++) npairs += which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx];
+        const int npairs = which ? A.n_vert1 : A.n_hor1;
+        if (jj < 15 && npairs > 0) {
+            const float w = 1.f / npairs;
             float v = 0.f;
-            if (npairs > 0) {
-                const float w = 1.f / npairs;
-                for (int i = 0; i < 15; i++) {
-                    if (A.corr_offset >= 0.f) {
-                        float cmb = 1.f;
-                        for (int sidx = 0; sidx < A.num_strip; sidx++) {
-                            if (!(which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx])) continue;
-                            const float* c = sc + sidx * CLT_CORR_LEN;
-                            const float d = which ? 0.5f * (c[jj * 15 + i] + c[jj * 15 + 14 - i])
-                                                  : 0.5f * (c[i * 15 + jj] + c[(14 - i) * 15 + jj]);
-                            cmb *= d + A.corr_offset;
-                        }
-                        cmb = (cmb > 0.f) ? powf(cmb, w) - A.corr_offset : -A.corr_offset;
-                        v += cmb * A.notch[i];
-                    } else {
-                        for (int sidx = 0; sidx < A.num_strip; sidx++) {
-                            if (!(which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx])) continue;
-                            const float* c = sc + sidx * CLT_CORR_LEN;
-                            v += w * (which ? c[jj * 15 + i] : c[i * 15 + jj]) * A.notch[i];
-                        }
+            for (int i = 0; i < 15; i++) {
+                if (A.notch[i] == 0.f) continue;   // the combination is finite: adds exactly 0
+                if (A.corr_offset >= 0.f) {
+                    float cmb = 1.f;
+                    for (int sidx = 0; sidx < A.num_strip; sidx++) {
+                        if (!(which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx])) continue;
+                        const float* c = s_corr + sidx * CLT_CORR_LEN;
+                        const float d = which ? 0.5f * (c[jj * 15 + i] + c[jj * 15 + 14 - i])
+                                              : 0.5f * (c[i * 15 + jj] + c[(14 - i) * 15 + jj]);
+                        cmb *= d + A.corr_offset;
+                    }
+                    cmb = (cmb > 0.f) ? pow_inv(cmb, w) - A.corr_offset : -A.corr_offset;
+                    v += cmb * A.notch[i];
+                } else {
+                    for (int sidx = 0; sidx < A.num_strip; sidx++) {
+                        if (!(which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx])) continue;
+                        const float* c = s_corr + sidx * CLT_CORR_LEN;
+                        v += w * (which ? c[jj * 15 + i] : c[i * 15 + jj]) * A.notch[i];
                     }
                 }
             }
-            c1d[which * 16 + jj] = v;
+            s_c1d[which * 16 + jj] = v;
         }
     }
     wave_sync();
+    // ---- centre of mass on the strip axis, lane-parallel (getMaxXYInt C2:802-843, getMaxXCm C2:936-1032)
+    int imx = 0;
+    for (int i = 1; i < 15; i++)
+        if (isnan(s_combo[imx]) || (s_combo[i] > s_combo[imx])) imx = i;
+    const bool have_max = s_combo[imx] >= A.min_corr;
+    const int ix = imx - 7;
+    float s0 = 0.f, sx = 0.f, sx2 = 0.f;
+    if (have_max) {
+        // sample k of the 2-row window: lanes 0..8 row 0 (adx even, both signs), 16..27 row 1
+        const int dh = hw > 2 ? 2 : hw;
+        const float wy_scale = (hw < 2) ? 1.f / A.wndy[0] : 1.f;
+        const int dy = lane >> 4, k = lane & 15;
+        if (dy < dh) {
+            const int odd = dy & 1;
+            // enumerate (adx, dir) in the reference order: adx = odd, odd+2, ...; dir -1 then +1 (adx > 0)
+            const int nadx = (9 - odd + 1) / 2;
+            int adx = -1, dir = 1;
+            int cnt = 0;
+            for (int a = odd; a < 9; a += 2) {
+                for (int d = (a == 0) ? 1 : -1; d <= 1; d += 2) {
+                    if (cnt == k) { adx = a; dir = d; }
+                    cnt++;
+                }
+            }
+            (void)nadx;
+            if (adx >= 0) {
+                const float wy = ((dy == 0) ? wy_scale : 2.f * wy_scale) * A.wndy[dy];
+                const int idx = (adx * dir) >> 1;
+                const int xq = 2 * idx + odd, x1 = 7 + ix + idx;
+                if (x1 >= 0 && x1 < 15) {
+                    float d = s_combo[15 * dy + x1];
+                    if (!isnan(d) && d > 0.f) {
+                        d *= wy * A.wndx[adx];
+                        s0 = d;
+                        sx = d * xq;
+                        sx2 = d * xq * xq;
+                    }
+                }
+            }
+        }
+        s0 = wave_sum(s0);
+        sx = wave_sum(sx);
+        sx2 = wave_sum(sx2);
+    }
     if (lane == 0 && disp_out) {
         float res[CLT_DISP_LAYERS];
         for (int i = 0; i < CLT_DISP_LAYERS; i++) res[i] = qnan;
         res[CLT_DISP_STRENGTH] = res[CLT_DISP_HOR_STR] = res[CLT_DISP_VERT_STR] = res[CLT_DISP_CM_WEIGHT] = 0.f;
-        // getMaxXYInt on the strip axis (C2:802-843)
-        int imx = 0;
-        for (int i = 1; i < 15; i++)
-            if (isnan(combo[imx]) || (combo[i] > combo[imx])) imx = i;
-        if (combo[imx] >= A.min_corr) {
-            const int ix = imx - 7;
+        if (have_max) {
             res[CLT_DISP_INT] = (float)(-ix);
-            res[CLT_DISP_STRENGTH] = combo[ix + 7];
-            // getMaxXCm (C2:936-1032): 2-row window_y, 9-sample window_x
-            int dh = hw;
-            float wy_scale = 1.f;
-            if (dh > 2) dh = 2;
-            else if (dh < 2) wy_scale = 1.f / A.wndy[0];
-            float s0 = 0.f, sx = 0.f, sx2 = 0.f;
-            const int x0 = 7 + ix;
-            for (int dy = 0; dy < dh; dy++) {
-                const int odd = dy & 1;
-                const float wy = ((dy == 0) ? wy_scale : 2.f * wy_scale) * A.wndy[dy];
-                for (int adx = odd; adx < 9; adx += 2) {
-                    for (int dir = (adx == 0) ? 1 : -1; dir <= 1; dir += 2) {
-                        const int idx = (adx * dir) >> 1;
-                        const int xq = 2 * idx + odd, x1 = x0 + idx;
-                        if (x1 >= 0 && x1 < 15) {
-                            float d = combo[15 * dy + x1];
-                            if (!isnan(d) && d > 0.f) {
-                                d *= wy * A.wndx[adx];
-                                s0 += d;
-                                sx += d * xq;
-                                sx2 += d * xq * xq;
-                            }
-                        }
-                    }
-                }
-            }
+            res[CLT_DISP_STRENGTH] = s_combo[ix + 7];
             if (s0 != 0.f) {
                 res[CLT_DISP_CM] = -(ix + sx / s0 / 2.f);
                 res[CLT_DISP_CM_WEIGHT] = s0;
             }
         }
         for (int which = 0; which < 2; which++) {
-            int npairs = 0;
-            for (int sidx = 0; sidx < A.num_strip; sidx++) npairs += which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx];
-            if (npairs == 0) continue;
-            const float* v = c1d + which * 16;
+            if ((which ? A.n_vert1 : A.n_hor1) == 0) continue;
+            const float* v = s_c1d + which * 16;
             int ic = 0;
             for (int i = 1; i < 15; i++)
                 if (v[i] > v[ic]) ic = i;
@@ -629,23 +604,6 @@ struct TexArgs {
     float port_offsets[CLT_MAX_CAMS][2];
 };
 
-__device__ __forceinline__ float block_max(float v, float* s_red) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    return fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
-}
-__device__ __forceinline__ float block_sum(float v, float* s_red) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    return (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
-}
-
 template <int NC>
 __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                   int task_words, int num_tasks, const float* __restrict__ lpf,
@@ -655,7 +613,7 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
     __shared__ __attribute__((aligned(16))) float s_scr[16][256];
     __shared__ float s_iclt[NU][256];
     __shared__ float s_nw[NU][256];
-    __shared__ float s_red[4];
+    __shared__ float s_red[4 * 4 * NC];
     const TexArgs& A = *args_p;
     const int t = xcd_block(blockIdx.x, gridDim.x);
     if (t >= num_tasks) return;
@@ -908,7 +866,7 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
         }
     }
     if (stats_out) {   // max_diff (ImageDtt.java:2965-2981) and ports_rgb (ImageDtt.java:2982-2996)
-        float* st = stats_out + (size_t)t * 4 * NC;
+        float v[4 * NC];
 #pragma unroll
         for (int ip = 0; ip < NC; ip++) {
             float d2 = 0.f;
@@ -920,16 +878,30 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
                 }
                 d2 *= w2;
             }
-            const float m = block_max(d2, s_red);
-            if (i == 0) st[ip] = sqrtf(m);
+            v[ip] = d2;
+#pragma unroll
+            for (int chn = 0; chn < 3; chn++) v[NC + chn * NC + ip] = deb[ip][chn];
         }
+        // one reduction pass for all 4*NC values: max for the first NC, sum for the rest
 #pragma unroll
-        for (int ip = 0; ip < NC; ip++)
+        for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
-            for (int chn = 0; chn < 3; chn++) {
-                const float s = block_sum(deb[ip][chn], s_red);
-                if (i == 0) st[NC + chn * NC + ip] = s / 256.f;
+            for (int k = 0; k < 4 * NC; k++) {
+                const float u = __shfl_xor(v[k], o, 64);
+                v[k] = (k < NC) ? fmaxf(v[k], u) : v[k] + u;
             }
+        }
+        if ((i & 63) == 0) {
+#pragma unroll
+            for (int k = 0; k < 4 * NC; k++) s_red[(i >> 6) * 4 * NC + k] = v[k];
+        }
+        __syncthreads();
+        if (i < 4 * NC) {
+            const float* r = s_red + i;
+            float* st = stats_out + (size_t)t * 4 * NC;
+            if (i < NC) st[i] = sqrtf(fmaxf(fmaxf(r[0], r[4 * NC]), fmaxf(r[8 * NC], r[12 * NC])));
+            else st[i] = ((r[0] + r[4 * NC]) + (r[8 * NC] + r[12 * NC])) / 256.f;
+        }
     }
 }
 

@@ -513,11 +513,17 @@ int clt_exec_corr2d(clt_handle* h) {
         a.strip_slot[i] = -1;
         if (p.dir >= 0) {
             const int s = a.num_strip++;
+            const int ss = p.scale / ss_min, denom = ss * ((p.dir > 1) ? 1 : 2);
+            int lg = 0;
+            while ((1 << lg) < denom) lg++;
+            if ((1 << lg) != denom) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_corr2d: pair %d scale ratio %d is not a power of 2", i, ss);
             a.strip_slot[i] = s;
             a.strip_dir[s] = p.dir;
-            a.strip_ss[s] = p.scale / ss_min;
+            a.strip_lg[s] = lg;
             a.strip_is_hor1[s] = (p.dir == 0 && p.scale == 1);
             a.strip_is_vert1[s] = (p.dir == 1 && p.scale == 1);
+            a.n_hor1 += a.strip_is_hor1[s];
+            a.n_vert1 += a.strip_is_vert1[s];
         }
     }
     if (g.num_colors == 3) {   // ImageDtt.java:1571-1581
@@ -541,17 +547,11 @@ int clt_exec_corr2d(clt_handle* h) {
         a.notch[i] = (float)(f * std::sin(M_PI * (i + 1.0) / 16));
     }
     HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
-    const int tpb = (P <= 8) ? 2 : 1;
-    const size_t lds = (size_t)corr_lds_floats(tpb, g.num_cams, g.num_colors) * 4;
-    if (lds > 64 * 1024) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_corr2d: %zu bytes of LDS", lds);
-    const int blocks = (h->num_tasks + tpb - 1) / tpb;
+    const size_t lds = (size_t)4 * ((corr_wave_floats(a.num_strip) + 3) & ~3) * 4;
+    const int blocks = (h->num_tasks + 3) / 4;
     tbegin(h, T_CORR);
-    if (tpb == 2)
-        hipLaunchKernelGGL(k_corr2d<2>, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
-                           h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
-    else
-        hipLaunchKernelGGL(k_corr2d<1>, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
-                           h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
+    hipLaunchKernelGGL(k_corr2d, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
+                       h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
     tend(h, T_CORR);
     HIPCHK(h, hipGetLastError());
     h->corr_done = true;
