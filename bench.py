@@ -71,6 +71,24 @@ def cpu_baseline(sc, min_s=10.0, max_passes=20):
                       % (n, dt)}
 
 
+def copy_calibration(mib):
+    """One hipMemcpy device-to-device of a known size: its FETCH_SIZE / WRITE_SIZE counters give the
+    scale between the counters and real HBM bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("bench: copied %d MiB device to device" % mib, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +96,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
+                    help="run one device-to-device copy of MB MiB first (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the tile path has no multi-GPU form yet); "
@@ -87,6 +107,8 @@ def main():
     if pk.load().clt_device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
     print("bench: generating the 2592x1936 quad frame ...", file=sys.stderr, flush=True)
     t = time.perf_counter()
     sc = pk.synth.make_scene(W, H, seed=args.seed, disparity=("field", 0.0, 20.0))

@@ -40,23 +40,33 @@ __device__ __forceinline__ int clamp_parity(int p, int n) {   // ImageDtt.java:4
     return p;
 }
 
-__global__ __launch_bounds__(256) void k_convert(const float* __restrict__ images, const float* __restrict__ kernels,
+// hardware sin/cos of pi*t (v_sin_f32 / v_cos_f32 take the angle in revolutions)
+__device__ __forceinline__ void sincos_pi(float t, float* s, float* c) {
+    *s = __builtin_amdgcn_sinf(0.5f * t);
+    *c = __builtin_amdgcn_cosf(0.5f * t);
+}
+
+// Block = `colors` wavefronts, wavefront w converts colour w of 4 (task, camera) units, 16 lanes
+// per unit.  Every row of a Bayer colour plane holds samples of one column parity only, so its
+// sine fold transform is the reversed cosine one: DST-IV(fold_S x)[k] = s * DCT-IV(fold_C x)[7-k],
+// s = -1 when the samples sit at odd window columns (same along columns for red and blue).
+__global__ __launch_bounds__(192) void k_convert(const float* __restrict__ images, const float* __restrict__ kernels,
                                                  const float* __restrict__ koffsets, const int32_t* __restrict__ tasks,
                                                  int task_words, int num_tasks, float* __restrict__ clt,
-                                                 const float* __restrict__ lpf, int lpf_mask, GeomDev g) {
-    __shared__ __attribute__((aligned(16))) float s_buf[16][256];
-    const int nblocks = gridDim.x;
-    const int blk = xcd_block(blockIdx.x, nblocks);
+                                                 const float* __restrict__ lpf, int lpf_mask, double inv_kstep, GeomDev g) {
+    __shared__ __attribute__((aligned(16))) float s_buf[12][256];
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int color = threadIdx.x >> 6;                     // wave-uniform
     const int unit_in_block = threadIdx.x >> 4, l = threadIdx.x & 15;
-    const int total = num_tasks * g.cams * g.colors;
-    int u = blk * 16 + unit_in_block;
+    const int total = num_tasks * g.cams;
+    int u = blk * 4 + ((threadIdx.x >> 4) & 3);             // (task, camera) unit
     bool active = u < total;
     if (!active) u = 0;
-    const int t = u / (g.cams * g.colors), rem = u % (g.cams * g.colors);
-    const int cam = rem / g.colors, color = rem % g.colors;
+    const int t = u / g.cams, cam = u % g.cams;
     const int32_t* rec = tasks + (size_t)t * task_words;
     active = active && (rec[0] != 0);
     if (!__any(active)) return;   // whole wave idle (no block-level barriers in this kernel)
+    const bool bayer = g.colors == 3;
     const int txy = rec[1];
     const int tx = txy & 0xffff, ty = (txy >> 16) & 0xffff;
     const double cx = active ? (double)__int_as_float(rec[2 + 2 * cam]) : 8.0;
@@ -67,11 +77,11 @@ __global__ __launch_bounds__(256) void k_convert(const float* __restrict__ image
     const float* kt = nullptr;
     int ktx = 0, kty = 0;
     if (kernels) {
-        ktx = (int)floor(cx / g.kstep + 0.5) + 1;
-        kty = (int)floor(cy / g.kstep + 0.5) + 1;
+        ktx = (int)floor(cx * inv_kstep + 0.5) + 1;        // exact: kstep is a power of two
+        kty = (int)floor(cy * inv_kstep + 0.5) + 1;
         kty = kty < 0 ? 0 : (kty >= g.kv ? g.kv - 1 : kty);
         ktx = ktx < 0 ? 0 : (ktx >= g.kh ? g.kh - 1 : ktx);
-        size_t kidx = (((size_t)cam * g.kv + kty) * g.kh + ktx) * g.colors + color;
+        const size_t kidx = (((size_t)cam * g.kv + kty) * g.kh + ktx) * g.colors + color;
         ce = koffsets + kidx * 8;
         kt = kernels + kidx * 256;
     }
@@ -79,34 +89,56 @@ __global__ __launch_bounds__(256) void k_convert(const float* __restrict__ image
     const int left = pos.left, top = pos.top;
     const float rx = pos.rx, ry = pos.ry;
 
-    // ---- stage H: lane = window row r; masked 16 pixels, horizontal fold + DCT-IV / DST-IV
-    const float* img = images + (size_t)cam * g.height * g.width;
+    // ---- window row r = l: 16 pixels of this colour (others 0), parity-preserving border clamp
     const int r = l;
     const int prow = top + r;
-    const int pi = clamp_parity(prow, g.height);
-    float x[16];
-    const bool mono = g.colors == 1;
     const bool row_red = ((prow & 1) == g.red_row);
+    // column parity (absolute) holding this colour on this row; -1: none (empty R/B row)
+    int cpar;
+    if (!bayer) cpar = 2;                                              // mono: every column
+    else if (color == 0) cpar = row_red ? g.red_col : -1;
+    else if (color == 1) cpar = row_red ? -1 : (g.red_col ^ 1);
+    else cpar = row_red ? (g.red_col ^ 1) : g.red_col;
+    float x[16];
+    const float* img = images + (size_t)cam * g.height * g.width;
+    const bool inside = (left >= 0) && (left + 16 <= g.width) && (top >= 0) && (top + 16 <= g.height);
+    if (active && inside && cpar >= 0) {
+        const float* rowp = img + (size_t)prow * g.width + left;
+        const int jpar = (cpar == 2) ? 2 : (cpar ^ (left & 1));        // window-relative parity
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-        const int pc = left + j;
-        bool col_red = ((pc & 1) == g.red_col);
-        int pcol = mono ? 0 : (row_red && col_red) ? 0 : (!row_red && !col_red) ? 1 : 2;
-        x[j] = 0.f;
-        if (active && pcol == color) x[j] = img[(size_t)pi * g.width + clamp_parity(pc, g.width)];
+        for (int j = 0; j < 16; j++) {
+            const float v = rowp[j];
+            x[j] = (jpar == 2 || (j & 1) == jpar) ? v : 0.f;
+        }
+    } else {
+        const int pi = clamp_parity(prow, g.height);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int pc = left + j;
+            x[j] = 0.f;
+            if (active && cpar >= 0 && (cpar == 2 || (pc & 1) == cpar)) x[j] = img[(size_t)pi * g.width + clamp_parity(pc, g.width)];
+        }
     }
     float ahs, ahc, avs, avc;
-    sincospif(rx * 0.0625f, &ahs, &ahc);
-    sincospif(ry * 0.0625f, &avs, &avc);
-    float yc[8], ys[8], hc[8], hsn[8];
-    fold16(x, ahc, ahs, yc, ys);
-    dct_iv8(yc, hc);
-    dst_iv8(ys, hsn);
+    sincos_pi(rx * 0.0625f, &ahs, &ahc);
+    sincos_pi(ry * 0.0625f, &avs, &avc);
     float* sb = s_buf[unit_in_block];
+    {   // ---- stage H: horizontal fold + DCT-IV (C) / DST-IV (S)
+        float yc[8], ys[8], hc[8], hsn[8];
+        fold16(x, ahc, ahs, yc, ys);
+        dct_iv8(yc, hc);
+        if (bayer) {
+            const float sg = (((cpar ^ left) & 1) ? -1.f : 1.f);        // odd window columns: -1
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        sb[k * 16 + r] = hc[k];
-        sb[(8 + k) * 16 + r] = hsn[k];
+            for (int k = 0; k < 8; k++) hsn[k] = sg * hc[7 - k];
+        } else {
+            dst_iv8(ys, hsn);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            sb[k * 16 + r] = hc[k];
+            sb[(8 + k) * 16 + r] = hsn[k];
+        }
     }
     wave_sync();
 
@@ -118,10 +150,20 @@ __global__ __launch_bounds__(256) void k_convert(const float* __restrict__ image
         float4 v = *reinterpret_cast<const float4*>(&sb[(hs * 8 + kh) * 16 + i]);
         col[i] = v.x; col[i + 1] = v.y; col[i + 2] = v.z; col[i + 3] = v.w;
     }
-    float zc[8], zs[8], a[8], b[8];
-    fold16(col, avc, avs, zc, zs);
-    dct_iv8(zc, a);    // mode hs
-    dst_iv8(zs, b);    // mode hs | 2
+    float a[8], b[8];
+    {
+        float zc[8], zs[8];
+        fold16(col, avc, avs, zc, zs);
+        dct_iv8(zc, a);    // mode hs
+        if (bayer && color != 2) {   // red / blue: rows of one parity only (wave-uniform branch)
+            const int rpar = (color == 0) ? g.red_row : (g.red_row ^ 1);
+            const float sg = (((rpar ^ top) & 1) ? -1.f : 1.f);
+#pragma unroll
+            for (int k = 0; k < 8; k++) b[k] = sg * a[7 - k];
+        } else {
+            dst_iv8(zs, b);   // mode hs | 2
+        }
+    }
     const float sigma = hs ? 1.f : -1.f;
 
     // ---- aberration kernel multiply (ImageDtt.java:5429-5488), partner lane holds modes hs^1
@@ -155,8 +197,9 @@ __global__ __launch_bounds__(256) void k_convert(const float* __restrict__ image
     // ---- fractional shift (ImageDtt.java:5490-5584): horizontal by rx (kh), vertical by ry (kv)
     {
         float sh, ch, cv[8], sv[8];
-        sincospif((kh + 0.5f) * rx * 0.125f, &sh, &ch);
-        half_phase8(ry * 0.125f, cv, sv);
+        sincos_pi((kh + 0.5f) * rx * 0.125f, &sh, &ch);
+#pragma unroll
+        for (int k = 0; k < 8; k++) sincos_pi((k + 0.5f) * ry * 0.125f, &sv[k], &cv[k]);
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             float pa = __shfl_xor(a[k], 8, 16) * sigma;
@@ -331,28 +374,39 @@ __device__ __forceinline__ float strip_value(const float* corr, int dir, int lg,
     return nan ? __int_as_float(0x7fc00000) : acc;
 }
 
+// Default quad camera pairs (Correlation2d.java:52-58) for the specialised kernel
+constexpr int QP_C1[6] = {0, 2, 0, 1, 0, 2};
+constexpr int QP_C2[6] = {1, 3, 2, 3, 3, 1};
+constexpr int QP_DIR[6] = {0, 0, 1, 1, 2, 3};
+constexpr int QP_LG[6] = {1, 1, 1, 1, 0, 0};
+
+// QUAD: 4 cameras, 3 colours, the 6 reference pairs -- the tile's CLT is held in registers and
+// every pair/strip index is a compile-time constant; otherwise the pair table comes from CorrArgs.
+template <bool QUAD>
 __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                 int task_words, int num_tasks, const float* __restrict__ lpf,
                                                 const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
                                                 float* __restrict__ combo_out, float* __restrict__ disp_out, GeomDev g) {
     extern __shared__ __attribute__((aligned(16))) float s_dyn[];
     const CorrArgs& A = *args_p;
-    const int K = g.colors, P = A.num_pairs;
+    const int K = QUAD ? 3 : g.colors;
+    const int P = QUAD ? 6 : A.num_pairs;
+    const int NS = QUAD ? 6 : A.num_strip;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
     if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
-    const int wfl = (corr_wave_floats(A.num_strip) + 3) & ~3;
+    const int wfl = (corr_wave_floats(NS) + 3) & ~3;
     float* S0 = s_dyn + wave * wfl;                   // [256]
     float* S1 = S0 + 256;                             // [256]
-    float* s_corr = S1 + 256;                         // [num_strip][225]
-    float* s_combo = s_corr + A.num_strip * CLT_CORR_LEN;   // [240]
+    float* s_corr = S1 + 256;                         // [NS][225]
+    float* s_combo = s_corr + NS * CLT_CORR_LEN;      // [240]
     float* s_c1d = s_combo + 240;                     // [2][16]
     const float qnan = __int_as_float(0x7fc00000);
     const int hw = A.strip_height, slen = hw * 15;
     const int32_t* rec = tasks + (size_t)t * task_words;
     if (rec[0] == 0) {
         for (int o = lane; o < P * CLT_CORR_LEN; o += 64) corr_out[(size_t)t * P * CLT_CORR_LEN + o] = qnan;
-        if (A.num_strip) {
+        if (NS) {
             if (combo_out)
                 for (int e = lane; e < slen; e += 64) combo_out[(size_t)t * slen + e] = qnan;
             if (lane < CLT_DISP_LAYERS && disp_out)
@@ -365,6 +419,16 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
     const int txy = rec[1];
     const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
     const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
+    // quad: the whole tile (4 cameras x 3 colours x 4 modes) for this lane's frequency bin
+    float D[QUAD ? 4 : 1][3][4];
+    if (QUAD) {
+#pragma unroll
+        for (int c = 0; c < (QUAD ? 4 : 1); c++)
+#pragma unroll
+            for (int col = 0; col < 3; col++)
+#pragma unroll
+                for (int n = 0; n < 4; n++) D[c][col][n] = clt[(((size_t)c * ntiles + tile) * 3 + col) * 256 + n * 64 + lane];
+    }
     // per-lane DCT-IIe / DST-IIe rows: pass 1 uses row y = lane >> 3, pass 2 row x = lane & 7
     const int y = lane >> 3, xx = lane & 7;
     float cy_[8], sy_[8], cx_[8], sx_[8];
@@ -377,106 +441,134 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
     }
     const float lp = lpf[lane];
     float* corr_t = corr_out + (size_t)t * P * CLT_CORR_LEN;
-    for (int p = 0; p < P; p++) {
-        const float* c1 = clt + ((size_t)A.cam1[p] * ntiles + tile) * K * 256 + lane;
-        const float* c2 = clt + ((size_t)A.cam2[p] * ntiles + tile) * K * 256 + lane;
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int col = 0; col < K; col++) {
-            float d1[4], d2[4];
 #pragma unroll
-            for (int n = 0; n < 4; n++) {
-                d1[n] = c1[col * 256 + n * 64];
-                d2[n] = c2[col * 256 + n * 64];
-            }
-            const float s1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] + d1[3] * d1[3];
-            const float s2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2] + d2[3] * d2[3];
-            const float a2 = sqrtf(s1 * s2);
-            float fz2 = 0.f;
-            if (A.fat_zero != 0.f) fz2 = wave_sum(a2) * (1.f / 64.f) * A.fat_zero * A.fat_zero;
-            const float scale = (1.f / (a2 + fz2)) * (K == 1 ? 1.f : A.col_w[col]);
-            // product with the ZI sign/index table (Correlation2d.java:86-90)
-            const float t0 = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2] + d1[3] * d2[3];
-            const float t1 = -d1[1] * d2[0] + d1[0] * d2[1] - d1[3] * d2[2] + d1[2] * d2[3];
-            const float t2 = -d1[2] * d2[0] - d1[3] * d2[1] + d1[0] * d2[2] + d1[1] * d2[3];
-            const float t3 = d1[3] * d2[0] - d1[2] * d2[1] - d1[1] * d2[2] + d1[0] * d2[3];
-            acc[0] += t0 * scale;
-            acc[1] += t1 * scale;
-            acc[2] += t2 * scale;
-            acc[3] += t3 * scale;
-        }
+    for (int p = 0; p < (QUAD ? 6 : 1); p++) {
+        for (int pg = 0; pg < (QUAD ? 1 : P); pg++) {
+            const int pp = QUAD ? p : pg;
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+            const float* c1 = clt + ((size_t)A.cam1[pp] * ntiles + tile) * K * 256 + lane;
+            const float* c2 = clt + ((size_t)A.cam2[pp] * ntiles + tile) * K * 256 + lane;
 #pragma unroll
-        for (int n = 0; n < 4; n++) S0[n * 64 + lane] = acc[n] * lp;
-        wave_sync();
-        // pass 1 along kv (contiguous index): lane (y, kh); quadrant q uses DST-IIe if q bit 1
-        {
-            const int khh = lane & 7;
-            float tq[4];
+            for (int col = 0; col < (QUAD ? 3 : 1); col++) {
+                for (int colg = 0; colg < (QUAD ? 1 : K); colg++) {
+                    const int cc = QUAD ? col : colg;
+                    float d1[4], d2[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const float* src = S0 + q * 64 + khh * 8;
-                float s = 0.f;
-#pragma unroll
-                for (int k = 0; k < 8; k++) s = fmaf((q & 2) ? sy_[k] : cy_[k], src[k], s);
-                tq[q] = s;
+                    for (int n = 0; n < 4; n++) {
+                        if (QUAD) {
+                            d1[n] = D[QUAD ? QP_C1[p] : 0][col][n];
+                            d2[n] = D[QUAD ? QP_C2[p] : 0][col][n];
+                        } else {
+                            d1[n] = c1[cc * 256 + n * 64];
+                            d2[n] = c2[cc * 256 + n * 64];
+                        }
+                    }
+                    const float s1 = d1[0] * d1[0] + d1[1] * d1[1] + d1[2] * d1[2] + d1[3] * d1[3];
+                    const float s2 = d2[0] * d2[0] + d2[1] * d2[1] + d2[2] * d2[2] + d2[3] * d2[3];
+                    const float a2 = sqrtf(s1 * s2);
+                    float fz2 = 0.f;
+                    if (A.fat_zero != 0.f) fz2 = wave_sum(a2) * (1.f / 64.f) * A.fat_zero * A.fat_zero;
+                    const float scale = (1.f / (a2 + fz2)) * (K == 1 ? 1.f : A.col_w[cc]);
+                    // product with the ZI sign/index table (Correlation2d.java:86-90)
+                    const float t0 = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2] + d1[3] * d2[3];
+                    const float t1 = -d1[1] * d2[0] + d1[0] * d2[1] - d1[3] * d2[2] + d1[2] * d2[3];
+                    const float t2 = -d1[2] * d2[0] - d1[3] * d2[1] + d1[0] * d2[2] + d1[1] * d2[3];
+                    const float t3 = d1[3] * d2[0] - d1[2] * d2[1] - d1[1] * d2[2] + d1[0] * d2[3];
+                    acc[0] += t0 * scale;
+                    acc[1] += t1 * scale;
+                    acc[2] += t2 * scale;
+                    acc[3] += t3 * scale;
+                }
             }
 #pragma unroll
-            for (int q = 0; q < 4; q++) S1[q * 64 + y * 8 + khh] = tq[q];
-        }
-        wave_sync();
-        // pass 2 along kh: lane (y, x); quadrant q uses DST-IIe if q bit 0
-        {
-            float oq[4];
+            for (int n = 0; n < 4; n++) S0[n * 64 + lane] = acc[n] * lp;
+            wave_sync();
+            // pass 1 along kv (contiguous index): lane (y, kh); quadrant q uses DST-IIe if q bit 1
+            {
+                const int khh = lane & 7;
+                float tq[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const float* src = S1 + q * 64 + y * 8;
-                float s = 0.f;
+                for (int q = 0; q < 4; q++) {
+                    const float* src = S0 + q * 64 + khh * 8;
+                    float s = 0.f;
 #pragma unroll
-                for (int k = 0; k < 8; k++) s = fmaf((q & 1) ? sx_[k] : cx_[k], src[k], s);
-                oq[q] = s;
+                    for (int k = 0; k < 8; k++) s = fmaf((q & 2) ? sy_[k] : cy_[k], src[k], s);
+                    tq[q] = s;
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) S1[q * 64 + y * 8 + khh] = tq[q];
             }
             wave_sync();
+            // pass 2 along kh: lane (y, x); quadrant q uses DST-IIe if q bit 0
+            {
+                float oq[4];
 #pragma unroll
-            for (int q = 0; q < 4; q++) S0[q * 64 + lane] = oq[q];
+                for (int q = 0; q < 4; q++) {
+                    const float* src = S1 + q * 64 + y * 8;
+                    float s = 0.f;
+#pragma unroll
+                    for (int k = 0; k < 8; k++) s = fmaf((q & 1) ? sx_[k] : cx_[k], src[k], s);
+                    oq[q] = s;
+                }
+                wave_sync();
+#pragma unroll
+                for (int q = 0; q < 4; q++) S0[q * 64 + lane] = oq[q];
+            }
+            wave_sync();
+            // unfold the 4 quadrants to 15x15 (DttRad2.java:443-489)
+            const int slot = QUAD ? p : A.strip_slot[pp];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int o = lane + 64 * j;
+                if (o < CLT_CORR_LEN) {
+                    const int row = o / 15, cc = o - row * 15;
+                    const int dy = row - 7, dx = cc - 7;
+                    const int ay = dy < 0 ? -dy : dy, ax = dx < 0 ? -dx : dx;
+                    float v = S0[ay * 8 + ax];
+                    if (ax > 0) v += (dx > 0 ? 1.f : -1.f) * S0[64 + ay * 8 + ax - 1];
+                    if (ay > 0) v += (dy > 0 ? 1.f : -1.f) * S0[128 + (ay - 1) * 8 + ax];
+                    if (ax > 0 && ay > 0) v += ((dx > 0) == (dy > 0) ? 1.f : -1.f) * S0[192 + (ay - 1) * 8 + ax - 1];
+                    v *= 0.25f;
+                    corr_t[pp * CLT_CORR_LEN + o] = v;
+                    if (slot >= 0) s_corr[slot * CLT_CORR_LEN + o] = v;
+                }
+            }
+            wave_sync();
         }
-        wave_sync();
-        // unfold the 4 quadrants to 15x15 (DttRad2.java:443-489)
-        const int slot = A.strip_slot[p];
-        for (int o = lane; o < CLT_CORR_LEN; o += 64) {
-            const int row = o / 15, cc = o - row * 15;
-            const int dy = row - 7, dx = cc - 7;
-            const int ay = dy < 0 ? -dy : dy, ax = dx < 0 ? -dx : dx;
-            float v = S0[ay * 8 + ax];
-            if (ax > 0) v += (dx > 0 ? 1.f : -1.f) * S0[64 + ay * 8 + ax - 1];
-            if (ay > 0) v += (dy > 0 ? 1.f : -1.f) * S0[128 + (ay - 1) * 8 + ax];
-            if (ax > 0 && ay > 0) v += ((dx > 0) == (dy > 0) ? 1.f : -1.f) * S0[192 + (ay - 1) * 8 + ax - 1];
-            v *= 0.25f;
-            corr_t[p * CLT_CORR_LEN + o] = v;
-            if (slot >= 0) s_corr[slot * CLT_CORR_LEN + o] = v;
-        }
-        wave_sync();
     }
-    if (A.num_strip == 0) return;
+    if (NS == 0) return;
     // ---- combined strip: combineInterpolatedCorrelations of the rotated strips (C2:730-790)
     {
-        const int ncomb = A.num_strip * (A.twice_diagonal ? 2 : 1);
+        const int ncomb = NS * (A.twice_diagonal ? 2 : 1);
         const float inv_n = 1.f / ncomb;
         for (int e = lane; e < slen; e += 64) {
             const int row = e / 15, scol = e - row * 15 - 7;
             float v;
             if (A.corr_offset >= 0.f) {
                 v = 1.f;
-                for (int sidx = 0; sidx < A.num_strip; sidx++) {
-                    float d = strip_value(s_corr + sidx * CLT_CORR_LEN, A.strip_dir[sidx], A.strip_lg[sidx], row, scol) +
-                              A.corr_offset;
-                    if (d < 0.f) d = 0.f;
-                    v *= A.twice_diagonal ? d * d : d;
+#pragma unroll
+                for (int sq = 0; sq < (QUAD ? 6 : 1); sq++) {
+                    for (int sg = 0; sg < (QUAD ? 1 : NS); sg++) {
+                        const int sidx = QUAD ? sq : sg;
+                        const int dir = QUAD ? QP_DIR[sq] : A.strip_dir[sidx];
+                        const int lg = QUAD ? QP_LG[sq] : A.strip_lg[sidx];
+                        float d = strip_value(s_corr + sidx * CLT_CORR_LEN, dir, lg, row, scol) + A.corr_offset;
+                        if (d < 0.f) d = 0.f;
+                        v *= A.twice_diagonal ? d * d : d;
+                    }
                 }
                 if (v > 0.f) v = pow_inv(v, inv_n) - A.corr_offset;
             } else {
                 v = 0.f;
-                for (int sidx = 0; sidx < A.num_strip; sidx++)
-                    v += (A.twice_diagonal ? 2.f : 1.f) *
-                         strip_value(s_corr + sidx * CLT_CORR_LEN, A.strip_dir[sidx], A.strip_lg[sidx], row, scol);
+#pragma unroll
+                for (int sq = 0; sq < (QUAD ? 6 : 1); sq++) {
+                    for (int sg = 0; sg < (QUAD ? 1 : NS); sg++) {
+                        const int sidx = QUAD ? sq : sg;
+                        const int dir = QUAD ? QP_DIR[sq] : A.strip_dir[sidx];
+                        const int lg = QUAD ? QP_LG[sq] : A.strip_lg[sidx];
+                        v += (A.twice_diagonal ? 2.f : 1.f) * strip_value(s_corr + sidx * CLT_CORR_LEN, dir, lg, row, scol);
+                    }
+                }
                 v *= inv_n;
             }
             s_combo[e] = v;
@@ -494,7 +586,7 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
                 if (A.notch[i] == 0.f) continue;   // the combination is finite: adds exactly 0
                 if (A.corr_offset >= 0.f) {
                     float cmb = 1.f;
-                    for (int sidx = 0; sidx < A.num_strip; sidx++) {
+                    for (int sidx = 0; sidx < NS; sidx++) {
                         if (!(which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx])) continue;
                         const float* c = s_corr + sidx * CLT_CORR_LEN;
                         const float d = which ? 0.5f * (c[jj * 15 + i] + c[jj * 15 + 14 - i])
@@ -504,7 +596,7 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
                     cmb = (cmb > 0.f) ? pow_inv(cmb, w) - A.corr_offset : -A.corr_offset;
                     v += cmb * A.notch[i];
                 } else {
-                    for (int sidx = 0; sidx < A.num_strip; sidx++) {
+                    for (int sidx = 0; sidx < NS; sidx++) {
                         if (!(which ? A.strip_is_vert1[sidx] : A.strip_is_hor1[sidx])) continue;
                         const float* c = s_corr + sidx * CLT_CORR_LEN;
                         v += w * (which ? c[jj * 15 + i] : c[i * 15 + jj]) * A.notch[i];
@@ -523,35 +615,31 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
     const int ix = imx - 7;
     float s0 = 0.f, sx = 0.f, sx2 = 0.f;
     if (have_max) {
-        // sample k of the 2-row window: lanes 0..8 row 0 (adx even, both signs), 16..27 row 1
+        // lanes 0..8: the 9 samples of row 0 (adx 0, then 2, 4, 6, 8 with dir -1 / +1), lanes
+        // 16..23: the 8 samples of row 1 (adx 1, 3, 5, 7 with dir -1 / +1)
         const int dh = hw > 2 ? 2 : hw;
         const float wy_scale = (hw < 2) ? 1.f / A.wndy[0] : 1.f;
         const int dy = lane >> 4, k = lane & 15;
-        if (dy < dh) {
+        int adx = -1, dir = 1;
+        if (dy == 0 && k < 9) {
+            adx = (k + 1) & ~1;
+            dir = (k == 0 || (k & 1) == 0) ? 1 : -1;
+        } else if (dy == 1 && k < 8) {
+            adx = 2 * (k >> 1) + 1;
+            dir = (k & 1) ? 1 : -1;
+        }
+        if (dy < dh && adx >= 0) {
             const int odd = dy & 1;
-            // enumerate (adx, dir) in the reference order: adx = odd, odd+2, ...; dir -1 then +1 (adx > 0)
-            const int nadx = (9 - odd + 1) / 2;
-            int adx = -1, dir = 1;
-            int cnt = 0;
-            for (int a = odd; a < 9; a += 2) {
-                for (int d = (a == 0) ? 1 : -1; d <= 1; d += 2) {
-                    if (cnt == k) { adx = a; dir = d; }
-                    cnt++;
-                }
-            }
-            (void)nadx;
-            if (adx >= 0) {
-                const float wy = ((dy == 0) ? wy_scale : 2.f * wy_scale) * A.wndy[dy];
-                const int idx = (adx * dir) >> 1;
-                const int xq = 2 * idx + odd, x1 = 7 + ix + idx;
-                if (x1 >= 0 && x1 < 15) {
-                    float d = s_combo[15 * dy + x1];
-                    if (!isnan(d) && d > 0.f) {
-                        d *= wy * A.wndx[adx];
-                        s0 = d;
-                        sx = d * xq;
-                        sx2 = d * xq * xq;
-                    }
+            const float wy = ((dy == 0) ? wy_scale : 2.f * wy_scale) * A.wndy[dy];
+            const int idx = (adx * dir) >> 1;
+            const int xq = 2 * idx + odd, x1 = 7 + ix + idx;
+            if (x1 >= 0 && x1 < 15) {
+                float d = s_combo[15 * dy + x1];
+                if (!isnan(d) && d > 0.f) {
+                    d *= wy * A.wndx[adx];
+                    s0 = d;
+                    sx = d * xq;
+                    sx2 = d * xq * xq;
                 }
             }
         }
@@ -604,6 +692,177 @@ struct TexArgs {
     float port_offsets[CLT_MAX_CAMS][2];
 };
 
+// Per-pixel port combination of one tile (tile_combine_rgba, ImageDtt.java:2830-3000, with the
+// final mix ImageDtt.java:2789-2801).  ALL: every port of the NC is used (no per-port masking).
+template <int NC, bool ALL>
+__device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, const float (&deb)[NC][3], float w2,
+                                              float iw2, const float (*s_iclt)[256], int i, float* out, int planes,
+                                              float (&avg)[3]) {
+    auto on = [&](int ip) { return ALL || ((port_mask >> ip) & 1); };
+    int used = 0;
+#pragma unroll
+    for (int ip = 0; ip < NC; ip++) used += on(ip) ? 1 : 0;
+    float rms[4];
+    {   // sqrt(s0*s2 - s1^2)/s0 is the population deviation: two-pass form, stable in float
+        float sw = 0.f;
+        const float inv_used = 1.f / (float)used;
+#pragma unroll
+        for (int chn = 0; chn < 3; chn++) {
+            float s1 = 0.f;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++)
+                if (on(ip)) s1 += deb[ip][chn];
+            const float mean = s1 * inv_used;
+            float var = 0.f;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++)
+                if (on(ip)) var += (deb[ip][chn] - mean) * (deb[ip][chn] - mean);
+            rms[chn] = sqrtf(var * inv_used);
+            sw += A.col_w[chn] * rms[chn] * rms[chn];
+        }
+        rms[3] = sqrtf(sw);
+    }
+    // Port weights are exponentials of squared mismatches.  They are computed relative to the
+    // largest one (exp(m - a) with m the smallest exponent), which keeps every ratio the reference
+    // forms in double within float range; absolute weights are rescaled by exp(-m) at the end.
+    float pw[NC], wscale = 1.f;
+    avg[0] = avg[1] = avg[2] = 0.f;
+#pragma unroll
+    for (int ip = 0; ip < NC; ip++) pw[ip] = 0.f;
+    if (used > 1) {
+        constexpr int NP = NC * (NC - 1) / 2 > 0 ? NC * (NC - 1) / 2 : 1;
+        float ea[NP];
+        float m1 = 3.0e38f;
+        int np = 0;
+#pragma unroll
+        for (int a = 0; a < NC; a++) {
+#pragma unroll
+            for (int b = a + 1; b < NC; b++) {
+                const float dxo = A.port_offsets[b][0] - A.port_offsets[a][0];
+                const float dyo = A.port_offsets[b][1] - A.port_offsets[a][1];
+                const float pd2r = A.ksigma / (dxo * dxo + dyo * dyo);
+                float d = 0.f;
+#pragma unroll
+                for (int chn = 0; chn < 3; chn++) {
+                    const float dc = (deb[a][chn] - deb[b][chn]) * iw2;
+                    d += A.col_w[chn] * dc * dc;
+                }
+                float e = pd2r * d;
+                if (!ALL && !(on(a) && on(b))) e = 3.0e38f;
+                m1 = fminf(m1, e);
+                ea[np++] = e;
+            }
+        }
+        np = 0;
+#pragma unroll
+        for (int a = 0; a < NC; a++) {
+#pragma unroll
+            for (int b = a + 1; b < NC; b++) {
+                float e = expf(m1 - ea[np++]);
+                if (!ALL && !(on(a) && on(b))) e = 0.f;
+                pw[a] += e;
+                pw[b] += e;
+            }
+        }
+        // two best ports (written without dynamic register indexing)
+        float bw1 = pw[0];
+        int b1 = 0;
+#pragma unroll
+        for (int ip = 1; ip < NC; ip++)
+            if (pw[ip] > bw1) { bw1 = pw[ip]; b1 = ip; }
+        int b2 = (b1 == 0) ? 1 : 0;
+        float bw2 = (b1 == 0) ? pw[1] : pw[0];
+#pragma unroll
+        for (int ip = 1; ip < NC; ip++)
+            if (ip > b2 && ip != b1 && pw[ip] > bw2) { bw2 = pw[ip]; b2 = ip; }
+        const float w1 = bw1 / (bw1 + bw2), wq = 1.f - w1;
+#pragma unroll
+        for (int chn = 0; chn < 3; chn++) {
+            float v1 = 0.f, v2 = 0.f;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++) {
+                v1 = (ip == b1) ? deb[ip][chn] : v1;
+                v2 = (ip == b2) ? deb[ip][chn] : v2;
+            }
+            avg[chn] = w1 * v1 + wq * v2;
+        }
+        float a2[NC];
+        float m2 = 3.0e38f;
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++) {
+            float d2 = 0.f;
+#pragma unroll
+            for (int chn = 0; chn < 3; chn++) {
+                const float dc = (deb[ip][chn] - avg[chn]) * iw2;
+                d2 += A.col_w[chn] * dc * dc;
+            }
+            a2[ip] = A.ksigma * d2;
+            if (on(ip)) m2 = fminf(m2, a2[ip]);
+        }
+        float ksum = 0.f;
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++) {
+            pw[ip] = on(ip) ? expf(m2 - a2[ip]) : 0.f;
+            ksum += pw[ip];
+        }
+        const float k = 1.f / ksum;
+#pragma unroll
+        for (int chn = 0; chn < 3; chn++) {
+            float s = 0.f;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++) s += k * pw[ip] * deb[ip][chn];
+            avg[chn] = s;
+        }
+        wscale = expf(-m2);
+    } else if (used > 0) {
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++)
+            if (on(ip)) {
+#pragma unroll
+                for (int chn = 0; chn < 3; chn++) avg[chn] = deb[ip][chn];
+                pw[ip] = 1.f;
+            }
+    }
+    if (A.dust_remove && used == 4) {   // dust_remove (ImageDtt.java:3002-3019), scale free
+        int wi = 0;
+        float ww = pw[0];
+#pragma unroll
+        for (int ip = 1; ip < NC; ip++)
+            if (pw[ip] < ww) { ww = pw[ip]; wi = ip; }
+        float av = 0.f;
+#pragma unroll
+        for (int ip = 1; ip < NC; ip++) av += (ip != wi) ? pw[ip] : 0.f;
+        av *= 1.f / (NC - 1);
+        if (av > 0.f) {   // av == 0 only when those weights are below float range: they stay 0
+            const float scale = 1.f + (av - ww) / (av * (NC - 1));
+            const float fw = ww / av;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++) pw[ip] *= (ip == wi) ? fw : ((ip >= 1) ? scale : 1.f);
+        }
+    }
+    float psum = 0.f;
+#pragma unroll
+    for (int ip = 0; ip < NC; ip++) psum += on(ip) ? pw[ip] : 0.f;
+    out[3 * 256 + i] = psum * wscale * w2 / used;
+    float sw = 0.f;
+#pragma unroll
+    for (int ip = 0; ip < NC; ip++) sw += pw[ip];
+    if (sw != 0.f) sw = 1.f / sw;
+#pragma unroll
+    for (int chn = 0; chn < 3; chn++) {
+        float s = 0.f;
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++) s += sw * pw[ip] * s_iclt[ip * 3 + chn][i];
+        out[chn * 256 + i] = s;
+    }
+    if (planes > 4) {
+#pragma unroll
+        for (int ip = 0; ip < NC; ip++) out[(4 + ip) * 256 + i] = pw[ip] * wscale;
+#pragma unroll
+        for (int k = 0; k < 4; k++) out[(4 + NC + k) * 256 + i] = rms[k];
+    }
+}
+
 template <int NC>
 __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                   int task_words, int num_tasks, const float* __restrict__ lpf,
@@ -612,8 +871,7 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
     constexpr int NU = NC * 3;   // camera/colour units
     __shared__ __attribute__((aligned(16))) float s_scr[16][256];
     __shared__ float s_iclt[NU][256];
-    __shared__ float s_nw[NU][256];
-    __shared__ float s_red[4 * 4 * NC];
+    __shared__ float s_nw[NU][18 * 18];   // window-free tiles with a zero border for the 3x3 debayer
     const TexArgs& A = *args_p;
     const int t = xcd_block(blockIdx.x, gridDim.x);
     if (t >= num_tasks) return;
@@ -635,6 +893,12 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
     float extra = extra_disp ? extra_disp[t] : 0.f;
     if (isnan(extra)) extra = 0.f;
     const bool do_shift = (extra != 0.f) && !((flags >> 24) & 1);
+    // zero the border ring of the padded tiles
+    for (int k = i; k < NU * 68; k += 256) {
+        const int uu = k / 68, b = k % 68;
+        const int pos = (b < 18) ? b : (b < 36) ? (17 * 18 + b - 18) : (b < 52) ? ((b - 36 + 1) * 18) : ((b - 52 + 1) * 18 + 17);
+        s_nw[uu][pos] = 0.f;
+    }
     // ---- IMCLT of the NC*3 units (16 lanes each) after LPF * 0.25 (ImageDtt.java:2684-2716)
     {
         const int unit = threadIdx.x >> 4, l = threadIdx.x & 15, hs = l >> 3, kh = l & 7;
@@ -678,195 +942,56 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
             float row[16];
             imclt_unit(l, p, q, s_scr[unit], row);
             if (act) {
+                const float wy = c_tab.hw[l < 8 ? l : 15 - l];
 #pragma unroll
-                for (int c = 0; c < 16; c++) s_iclt[uu][l * 16 + c] = row[c];
+                for (int c = 0; c < 16; c++) {
+                    const float wx = c_tab.hw[c < 8 ? c : 15 - c];
+                    const float iw = 1.f / ((wy * wx) * (wy * wx));
+                    s_iclt[uu][l * 16 + c] = row[c];
+                    s_nw[uu][(l + 1) * 18 + c + 1] = row[c] * iw;   // un-apply the squared window
+                }
             }
         }
     }
     __syncthreads();
-    // ---- un-apply the squared window, debayer, shot-noise correction, re-apply (ID:3022-3045)
+    // ---- debayer with shot-noise correction and re-applied window (ID:3022-3147); the zero border
+    // plays the role of the dropped taps, the corner/side factors renormalise the kernel sums
     const int py = i >> 4, px = i & 15;
     const float wwy = c_tab.hw[py < 8 ? py : 15 - py], wwx = c_tab.hw[px < 8 ? px : 15 - px];
     const float w2 = (wwy * wwx) * (wwy * wwx);
-#pragma unroll
-    for (int uu = 0; uu < NU; uu++) s_nw[uu][i] = s_iclt[uu][i] / w2;
-    __syncthreads();
+    const float iw2 = 1.f / w2;
+    const int nb = (py == 0 || py == 15) + (px == 0 || px == 15);   // 0 inside, 1 side, 2 corner
+    const float k_rb = (nb == 2) ? (16.f / 9.f) : (nb == 1) ? (4.f / 3.f) : 1.f;
+    const float k_g = (nb == 2) ? (4.f / 3.f) : (nb == 1) ? (8.f / 7.f) : 1.f;
     float deb[NC][3];
     {
-        const bool top = py == 0, bot = py == 15, lft = px == 0, rgt = px == 15;
-        const bool corner = (top || bot) && (lft || rgt), side = (top || bot || lft || rgt) && !corner;
         const float kshot = 1.f / sqrtf(A.min_shot);
+        const int c0 = (py + 1) * 18 + px + 1;
 #pragma unroll
         for (int uu = 0; uu < NU; uu++) {
-            const bool rb = (uu % 3) != 2;
-            float s = 0.f;
-#pragma unroll
-            for (int dy = -1; dy <= 1; dy++) {
-#pragma unroll
-                for (int dx = -1; dx <= 1; dx++) {
-                    if ((dy < 0 && top) || (dy > 0 && bot) || (dx < 0 && lft) || (dx > 0 && rgt)) continue;
-                    // kern_rb / kern_g (ImageDtt.java:43-50)
-                    const float k = rb ? ((dy == 0 ? 2.f : 1.f) * (dx == 0 ? 2.f : 1.f) * 0.0625f)
-                                       : ((dy == 0 && dx == 0) ? 0.5f : ((dy == 0) != (dx == 0) ? 0.125f : 0.f));
-                    if (k != 0.f) s += s_nw[uu][i + dy * 16 + dx] * k;
-                }
+            const float* nw = s_nw[uu];
+            float s;
+            if (uu % 3 != 2) {   // kern_rb = (1 2 1)x(1 2 1)/16 (ImageDtt.java:47-50)
+                s = 0.0625f * (nw[c0 - 19] + nw[c0 - 17] + nw[c0 + 17] + nw[c0 + 19]) +
+                    0.125f * (nw[c0 - 18] + nw[c0 - 1] + nw[c0 + 1] + nw[c0 + 18]) + 0.25f * nw[c0];
+                s *= k_rb;
+            } else {             // kern_g = cross (ImageDtt.java:43-46)
+                s = 0.125f * (nw[c0 - 18] + nw[c0 - 1] + nw[c0 + 1] + nw[c0 + 18]) + 0.5f * nw[c0];
+                s *= k_g;
             }
-            if (corner) s *= rb ? (16.f / 9.f) : (4.f / 3.f);
-            else if (side) s *= rb ? (4.f / 3.f) : (8.f / 7.f);
             if (A.scale_shot > 0.f) s = A.scale_shot * ((s > A.min_shot) ? sqrtf(s) : (kshot * s));
             deb[uu / 3][uu % 3] = s * w2;
         }
     }
-    // ---- tile_combine_rgba per pixel (ImageDtt.java:2830-3000); keep_weights rms first
-    float rms[4];
-    {   // s0*s2 - s1^2 cancels badly in float: accumulate in double
-        double sw = 0.0;
-#pragma unroll
-        for (int chn = 0; chn < 3; chn++) {
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
-            for (int ip = 0; ip < NC; ip++)
-                if ((port_mask >> ip) & 1) {
-                    const double v = deb[ip][chn];
-                    s0 += 1.0;
-                    s1 += v;
-                    s2 += v * v;
-                }
-            const double r = sqrt(s0 * s2 - s1 * s1) / s0;
-            rms[chn] = (float)r;
-            sw += A.col_w[chn] * r * r;
-        }
-        rms[3] = (float)sqrt(sw);
-    }
-    int used = 0;
-#pragma unroll
-    for (int ip = 0; ip < NC; ip++) used += (port_mask >> ip) & 1;
-    // Port weights are exponentials of squared mismatches that leave the float range for badly
-    // matching ports; they are carried in double like the reference (ImageDtt.java:2883-2950).
-    double pw[NC];
-    float avg[3] = {0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ip = 0; ip < NC; ip++) pw[ip] = 0.0;
-    if (used > 1) {
-#pragma unroll
-        for (int a = 0; a < NC; a++) {
-#pragma unroll
-            for (int b = a + 1; b < NC; b++) {
-                if (!((port_mask >> a) & 1) || !((port_mask >> b) & 1)) continue;
-                const float dxo = A.port_offsets[b][0] - A.port_offsets[a][0];
-                const float dyo = A.port_offsets[b][1] - A.port_offsets[a][1];
-                const float pd2r = A.ksigma / (dxo * dxo + dyo * dyo);
-                float d = 0.f;
-#pragma unroll
-                for (int chn = 0; chn < 3; chn++) {
-                    const float dc = (deb[a][chn] - deb[b][chn]) / w2;
-                    d += A.col_w[chn] * dc * dc;
-                }
-                const double e = exp(-(double)pd2r * (double)d);
-                pw[a] += e;
-                pw[b] += e;
-            }
-        }
-        // two best ports (written without dynamic register indexing)
-        double bw1 = pw[0];
-        int b1 = 0;
-#pragma unroll
-        for (int ip = 1; ip < NC; ip++)
-            if (pw[ip] > bw1) { bw1 = pw[ip]; b1 = ip; }
-        int b2 = (b1 == 0) ? 1 : 0;
-        double bw2 = (b1 == 0) ? pw[1] : pw[0];
-#pragma unroll
-        for (int ip = 1; ip < NC; ip++)
-            if (ip > b2 && ip != b1 && pw[ip] > bw2) { bw2 = pw[ip]; b2 = ip; }
-        const float w1 = (float)(bw1 / (bw1 + bw2)), wq = 1.f - w1;
-#pragma unroll
-        for (int chn = 0; chn < 3; chn++) {
-            float v1 = 0.f, v2 = 0.f;
-#pragma unroll
-            for (int ip = 0; ip < NC; ip++) {
-                if (ip == b1) v1 = deb[ip][chn];
-                if (ip == b2) v2 = deb[ip][chn];
-            }
-            avg[chn] = w1 * v1 + wq * v2;
-        }
-#pragma unroll
-        for (int ip = 0; ip < NC; ip++) {
-            if (!((port_mask >> ip) & 1)) continue;
-            float d2 = 0.f;
-#pragma unroll
-            for (int chn = 0; chn < 3; chn++) {
-                const float dc = (deb[ip][chn] - avg[chn]) / w2;
-                d2 += A.col_w[chn] * dc * dc;
-            }
-            pw[ip] = exp(-(double)A.ksigma * (double)d2);
-        }
-        double k = 0.0;
-#pragma unroll
-        for (int ip = 0; ip < NC; ip++) k += pw[ip];
-        k = 1.0 / k;
-#pragma unroll
-        for (int chn = 0; chn < 3; chn++) {
-            float s = 0.f;
-#pragma unroll
-            for (int ip = 0; ip < NC; ip++) s += (float)(k * pw[ip]) * deb[ip][chn];
-            avg[chn] = s;
-        }
-    } else if (used > 0) {
-#pragma unroll
-        for (int ip = 0; ip < NC; ip++)
-            if ((port_mask >> ip) & 1) {
-#pragma unroll
-                for (int chn = 0; chn < 3; chn++) avg[chn] = deb[ip][chn];
-                pw[ip] = 1.0;
-            }
-    }
-    if (NC == 4 && A.dust_remove && used == 4) {   // dust_remove (ImageDtt.java:3002-3019)
-        int wi = 0;
-        double ww = pw[0];
-#pragma unroll
-        for (int ip = 1; ip < NC; ip++)
-            if (pw[ip] < ww) { ww = pw[ip]; wi = ip; }
-        double av = 0.0;
-#pragma unroll
-        for (int ip = 1; ip < NC; ip++)
-            if (ip != wi) av += pw[ip];
-        av /= (NC - 1);
-        const double scale = 1.0 + (av - ww) / (av * (NC - 1));
-#pragma unroll
-        for (int ip = 1; ip < NC; ip++)
-            if (ip != wi) pw[ip] *= scale;
-#pragma unroll
-        for (int ip = 0; ip < NC; ip++)
-            if (ip == wi) pw[ip] *= ww / av;
-    }
-    double alpha = 0.0;
-#pragma unroll
-    for (int ip = 0; ip < NC; ip++)
-        if ((port_mask >> ip) & 1) alpha += pw[ip];
-    alpha *= (double)w2 / used;
-    // final mix of the non-debayered tiles with the port weights (ImageDtt.java:2789-2801)
-    {
-        double sw = 0.0;
-#pragma unroll
-        for (int ip = 0; ip < NC; ip++) sw += pw[ip];
-        if (sw != 0.0) sw = 1.0 / sw;
-#pragma unroll
-        for (int chn = 0; chn < 3; chn++) {
-            float s = 0.f;
-#pragma unroll
-            for (int ip = 0; ip < NC; ip++) s += (float)(sw * pw[ip]) * s_iclt[ip * 3 + chn][i];
-            out[chn * 256 + i] = s;
-        }
-        out[3 * 256 + i] = (float)alpha;
-        if (planes > 4) {
-#pragma unroll
-            for (int ip = 0; ip < NC; ip++) out[(4 + ip) * 256 + i] = (float)pw[ip];
-#pragma unroll
-            for (int k = 0; k < 4; k++) out[(4 + NC + k) * 256 + i] = rms[k];
-        }
-    }
+    float avg[3];
+    if (port_mask == (1 << NC) - 1) combine_pixel<NC, true>(A, port_mask, deb, w2, iw2, s_iclt, i, out, planes, avg);
+    else combine_pixel<NC, false>(A, port_mask, deb, w2, iw2, s_iclt, i, out, planes, avg);
     if (stats_out) {   // max_diff (ImageDtt.java:2965-2981) and ports_rgb (ImageDtt.java:2982-2996)
-        float v[4 * NC];
+        // transpose through LDS: value k of pixel i at [k][i] (the NC max_diff rows reuse s_scr, the
+        // 3*NC colour rows reuse s_iclt, whose sizes match); 16 threads then reduce each row
+        float* red_max = &s_scr[0][0];
+        float* red_rgb = &s_iclt[0][0];
+        __syncthreads();
 #pragma unroll
         for (int ip = 0; ip < NC; ip++) {
             float d2 = 0.f;
@@ -878,29 +1003,28 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
                 }
                 d2 *= w2;
             }
-            v[ip] = d2;
+            red_max[ip * 256 + i] = d2;
 #pragma unroll
-            for (int chn = 0; chn < 3; chn++) v[NC + chn * NC + ip] = deb[ip][chn];
-        }
-        // one reduction pass for all 4*NC values: max for the first NC, sum for the rest
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-#pragma unroll
-            for (int k = 0; k < 4 * NC; k++) {
-                const float u = __shfl_xor(v[k], o, 64);
-                v[k] = (k < NC) ? fmaxf(v[k], u) : v[k] + u;
-            }
-        }
-        if ((i & 63) == 0) {
-#pragma unroll
-            for (int k = 0; k < 4 * NC; k++) s_red[(i >> 6) * 4 * NC + k] = v[k];
+            for (int chn = 0; chn < 3; chn++) red_rgb[(chn * NC + ip) * 256 + i] = deb[ip][chn];
         }
         __syncthreads();
-        if (i < 4 * NC) {
-            const float* r = s_red + i;
-            float* st = stats_out + (size_t)t * 4 * NC;
-            if (i < NC) st[i] = sqrtf(fmaxf(fmaxf(r[0], r[4 * NC]), fmaxf(r[8 * NC], r[12 * NC])));
-            else st[i] = ((r[0] + r[4 * NC]) + (r[8 * NC] + r[12 * NC])) / 256.f;
+        // 4*NC values x 256 pixels: 16 threads per value, 16 pixels each, then a 16-lane reduction
+        for (int kb = 0; kb < 4 * NC; kb += 16) {
+            const int k = kb + (i >> 4), j = i & 15;
+            float v = 0.f;
+            if (k < 4 * NC) {
+                const bool is_max = k < NC;
+                const float* r = (is_max ? red_max + k * 256 : red_rgb + (k - NC) * 256) + j * 16;
+                v = r[0];
+#pragma unroll
+                for (int q = 1; q < 16; q++) v = is_max ? fmaxf(v, r[q]) : v + r[q];
+#pragma unroll
+                for (int o = 8; o > 0; o >>= 1) {
+                    const float u = __shfl_xor(v, o, 16);
+                    v = is_max ? fmaxf(v, u) : v + u;
+                }
+                if (j == 0) stats_out[(size_t)t * 4 * NC + k] = is_max ? sqrtf(v) : v / 256.f;
+            }
         }
     }
 }

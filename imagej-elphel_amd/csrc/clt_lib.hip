@@ -253,7 +253,7 @@ int clt_create(const clt_geom* geom, clt_handle** out) {
     const clt_geom& g = *geom;
     if (g.width <= 16 || g.height <= 16 || (g.width & 7) || (g.height & 7) || g.num_cams < 1 ||
         g.num_cams > CLT_MAX_CAMS || (g.num_colors != 1 && g.num_colors != 3) || g.red_row < 0 || g.red_row > 1 ||
-        g.red_col < 0 || g.red_col > 1 || g.kernel_step <= 0 || g.kernels_hor < 0 || g.kernels_vert < 0 ||
+        g.red_col < 0 || g.red_col > 1 || g.kernel_step <= 0 || (g.kernel_step & (g.kernel_step - 1)) || g.kernels_hor < 0 || g.kernels_vert < 0 ||
         g.width / 8 > 0xffff || g.height / 8 > 0xffff)
         return CLT_E_ARG;
     int ndev = 0;
@@ -449,12 +449,13 @@ int clt_exec_convert(clt_handle* h, int32_t lpf_mask) {
     for (int c = 0; c < h->g.num_cams; c++)
         if (!h->images_set[c]) return fail(h, CLT_E_STATE, "clt_exec_convert: image of camera %d not set", c);
     if (h->num_tasks <= 0) return fail(h, CLT_E_STATE, "clt_exec_convert: no tasks");
-    const long units = (long)h->num_tasks * h->g.num_cams * h->g.num_colors;
-    const int blocks = (int)((units + 15) / 16);
+    const long units = (long)h->num_tasks * h->g.num_cams;   // (task, camera); 4 per block
+    const int blocks = (int)((units + 3) / 4);
+    const double inv_kstep = 1.0 / h->g.kernel_step;            // exact: checked power of two
     tbegin(h, T_CONVERT);
-    hipLaunchKernelGGL(k_convert, dim3(blocks), dim3(256), 0, h->stream, h->d_images,
+    hipLaunchKernelGGL(k_convert, dim3(blocks), dim3(64 * h->g.num_colors), 0, h->stream, h->d_images,
                        h->use_kernels ? h->d_kernels : nullptr, h->use_kernels ? h->d_koffs : nullptr, h->d_tasks,
-                       h->task_words, h->num_tasks, h->d_clt, lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, h->gd);
+                       h->task_words, h->num_tasks, h->d_clt, lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, inv_kstep, h->gd);
     tend(h, T_CONVERT);
     HIPCHK(h, hipGetLastError());
     h->converted = true;
@@ -550,8 +551,21 @@ int clt_exec_corr2d(clt_handle* h) {
     const size_t lds = (size_t)4 * ((corr_wave_floats(a.num_strip) + 3) & ~3) * 4;
     const int blocks = (h->num_tasks + 3) / 4;
     tbegin(h, T_CORR);
-    hipLaunchKernelGGL(k_corr2d, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
-                       h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
+    // the reference quad configuration runs the specialised kernel (pairs as compile-time constants)
+    bool quad = g.num_cams == 4 && g.num_colors == 3 && P == 6;
+    if (quad) {
+        clt_pair dq[6];
+        clt_default_quad_pairs(dq);
+        for (int i = 0; i < 6; i++)
+            quad = quad && h->pairs[i].cam1 == dq[i].cam1 && h->pairs[i].cam2 == dq[i].cam2 &&
+                   h->pairs[i].dir == dq[i].dir && h->pairs[i].scale == 1;
+    }
+    if (quad)
+        hipLaunchKernelGGL(k_corr2d<true>, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
+                           h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
+    else
+        hipLaunchKernelGGL(k_corr2d<false>, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
+                           h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
     tend(h, T_CORR);
     HIPCHK(h, hipGetLastError());
     h->corr_done = true;

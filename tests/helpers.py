@@ -55,3 +55,19 @@ def gpu_processor(pk, sc, use_kernels=True, **kw):
 def gpu_clt_cpu_layout(tp, sc):
     """GPU CLT of every camera [C][tiles][K][4][8][8] ([v][h] blocks, the oracle's output layout)."""
     return np.stack([tp.getCLT(c) for c in range(sc.num_cams)])
+
+
+def robust_rel_rms(a, b, frac=1e-3, tol=1e-3):
+    """For outputs of discontinuous selections (argmax/argmin of port weights in the reference's
+    texture combiner): FP32 and FP64 may resolve near-ties differently on a few pixels.
+    Returns (rel RMS over all but the worst `frac` of the entries, fraction of entries whose error
+    exceeds `tol` times the RMS of b)."""
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    m = np.isfinite(b)
+    a, b = a[m], b[m]
+    err = np.abs(a - b)
+    scale = np.sqrt(np.mean(b ** 2)) or 1.0
+    keep = np.argsort(err)[: max(1, int(np.ceil(len(err) * (1 - frac))))]
+    r = float(np.sqrt(np.mean(err[keep] ** 2)) / scale)
+    return r, float(np.mean(err > tol * scale))

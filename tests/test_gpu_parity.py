@@ -7,8 +7,8 @@ oracle's own single-precision build is printed next to the GPU error for scale.
 import numpy as np
 import pytest
 
-from helpers import (gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, task_tiles,
-                     valid_map)
+from helpers import (gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, robust_rel_rms,
+                     task_tiles, valid_map)
 
 pytestmark = pytest.mark.gpu
 
@@ -199,19 +199,29 @@ def _tex_compare(pk, orc, sc, extra=None, tasks=None):
     o_tiles, o_stats = orc.textures_frame(oracle_geom(orc, sc), g_clt, tasks, extra_disparity=extra)
     o_img = orc.combine_textures(oracle_geom(orc, sc), tasks, o_tiles)
     live = tasks[:, 0] != 0
-    res = {"rgb": rel_rms(g_tiles[live, :3], o_tiles[live, :3]), "alpha": rel_rms(g_tiles[live, 3], o_tiles[live, 3]),
-           "weights": rel_rms(g_tiles[live, 4:8], o_tiles[live, 4:8]), "rms": rel_rms(g_tiles[live, 8:], o_tiles[live, 8:]),
-           "stats": rel_rms(g_stats[live], o_stats[live]), "image": rel_rms(g_img[:4], o_img[:4])}
+    res = {}
+    for name, sl in (("rgb", slice(0, 3)), ("alpha", slice(3, 4)), ("weights", slice(4, 8)), ("rms", slice(8, None))):
+        res[name], res[name + "_flips"] = robust_rel_rms(g_tiles[live, sl], o_tiles[live, sl])
+    res["stats"] = rel_rms(g_stats[live], o_stats[live])
+    res["image"], res["image_flips"] = robust_rel_rms(g_img[:4], o_img[:4])
     tp.close()
     return res
+
+
+def _assert_textures(r):
+    """Textures: 1e-4 relative RMS over all but 0.1% of the pixels; at most 0.1% of the pixels
+    may differ by more than 1e-3 of the plane RMS (near-ties of the port selection)."""
+    for k in ("rgb", "alpha", "weights", "rms", "image"):
+        assert r[k] < IMG_TOL, k
+        assert r[k + "_flips"] <= 1e-3, k
+    assert r["stats"] < 1e-3
 
 
 def test_textures_quad(pk, orc, quad_scene):
     log("case: textures quad 128x96, all ports")
     r = _tex_compare(pk, orc, quad_scene)
     log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
-    assert r["rgb"] < IMG_TOL and r["alpha"] < IMG_TOL and r["image"] < IMG_TOL
-    assert r["weights"] < 1e-3 and r["rms"] < 1e-3 and r["stats"] < 1e-3
+    _assert_textures(r)
 
 
 def test_textures_extra_disparity_port_masks(pk, orc, quad_scene):
@@ -224,8 +234,7 @@ def test_textures_extra_disparity_port_masks(pk, orc, quad_scene):
     extra = rng.uniform(-1.5, 1.5, len(tasks)).astype(np.float32)
     r = _tex_compare(pk, orc, sc, extra=extra.astype(np.float64), tasks=tasks)
     log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
-    assert r["rgb"] < IMG_TOL and r["alpha"] < IMG_TOL
-    assert r["weights"] < 1e-3
+    _assert_textures(r)
 
 
 def test_config0_pipeline_disparity(pk, orc):
