@@ -15,6 +15,8 @@ namespace clt {
 // Tables computed on the host in double precision from their closed forms and uploaded once.
 struct Tables {
     float hw[8];        // half window sin(pi*(i+0.5)/16)            (DttRad2.java:684)
+    float hw2[8];       // hw^2 (one factor of the squared 2-D window, ImageDtt.java:2686-2690)
+    float ihw2[8];      // 1 / hw^2
     float civ[8][8];    // orthonormal DCT-IV, symmetric              (DttRad2.java:993-1012)
     float ciie[8][8];   // DCT-IIe                                    (DttRad2.java:952-968)
     float siie[8][8];   // DST-IIe                                    (DttRad2.java:1014-1030)
@@ -126,9 +128,16 @@ __device__ __forceinline__ void half_phase8(float t, float* c, float* s) {
 }
 
 // ---------------------------------------------------------------- inverse CLT of one unit
+// LDS scratch of one 16-lane unit: 16 rows of 20 floats (a 16-byte row store of 8 consecutive
+// lanes covers all 32 banks once), units 336 floats apart (neighbouring units use the other
+// half of the banks when a wavefront reads them together).
+constexpr int IMCLT_SCR_ROW = 20;
+constexpr int IMCLT_SCR_UNIT = 336;
+
 // Lane (hs = l >> 3, kh = l & 7) enters with p[kv] = CLT[hs][kh][kv], q[kv] = CLT[hs | 2][kh][kv];
-// scratch: 256 floats of LDS private to this unit.  Leaves with row[c] = output row r = l of the
-// 16x16 pixel tile (sum over the 4 modes of unfold(IDTT), ImageDtt.java:3620-3632 without the 1/4).
+// scratch: IMCLT_SCR_UNIT floats of LDS private to this unit.  Leaves with row[c] = output row
+// r = l of the 16x16 pixel tile (sum over the 4 modes of unfold(IDTT), ImageDtt.java:3620-3632
+// without the 1/4).
 __device__ __forceinline__ void imclt_unit(int l, const float* p, const float* q, float* scratch, float* row) {
     const int hs = l >> 3, kh = l & 7;
     float oc[8], os[8], col[16];
@@ -137,13 +146,14 @@ __device__ __forceinline__ void imclt_unit(int l, const float* p, const float* q
     unfold16(oc, os, col);                   // 16 rows for (hs, kh)
 #pragma unroll
     for (int r = 0; r < 16; r += 4)
-        *reinterpret_cast<float4*>(&scratch[(hs * 8 + kh) * 16 + r]) = make_float4(col[r], col[r + 1], col[r + 2], col[r + 3]);
+        *reinterpret_cast<float4*>(&scratch[(hs * 8 + kh) * IMCLT_SCR_ROW + r]) =
+            make_float4(col[r], col[r + 1], col[r + 2], col[r + 3]);
     wave_sync();
     float rc[8], rs[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        rc[k] = scratch[k * 16 + l];
-        rs[k] = scratch[(8 + k) * 16 + l];
+        rc[k] = scratch[k * IMCLT_SCR_ROW + l];
+        rs[k] = scratch[(8 + k) * IMCLT_SCR_ROW + l];
     }
     wave_sync();
     dct_iv8(rc, oc);                         // horizontal inverse, C (h) modes

@@ -239,7 +239,7 @@ constexpr int IMCLT_R = 16;   // output block rows per segment
 
 __global__ __launch_bounds__(256) void k_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tile_map,
                                                const float* __restrict__ lpf, float* __restrict__ rbg, GeomDev g) {
-    __shared__ __attribute__((aligned(16))) float s_scr[16][256];
+    __shared__ __attribute__((aligned(16))) float s_scr[16][IMCLT_SCR_UNIT];
     __shared__ float s_tile[16][16][17];
     __shared__ float s_carry[8][IMCLT_S * 8];
     const int plane = blockIdx.z;               // cam * colors + color
@@ -688,43 +688,30 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
 struct TexArgs {
     float col_w[3];
     float min_shot, scale_shot, ksigma, corr_magic_scale;
+    float ksigma_l2;                  // ksigma * log2(e): weights as powers of 2
+    float pair_k_l2[28];              // ksigma / |offset_j - offset_i|^2 * log2(e), pairs i < j
     int dust_remove, planes;
     float port_offsets[CLT_MAX_CAMS][2];
 };
+
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 
 // Per-pixel port combination of one tile (tile_combine_rgba, ImageDtt.java:2830-3000, with the
 // final mix ImageDtt.java:2789-2801).  ALL: every port of the NC is used (no per-port masking).
 template <int NC, bool ALL>
 __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, const float (&deb)[NC][3], float w2,
-                                              float iw2, const float (*s_iclt)[256], int i, float* out, int planes,
+                                              float iw2, const float* s_nw, int c0, float* out, int i, int planes,
                                               float (&avg)[3]) {
     auto on = [&](int ip) { return ALL || ((port_mask >> ip) & 1); };
     int used = 0;
 #pragma unroll
     for (int ip = 0; ip < NC; ip++) used += on(ip) ? 1 : 0;
-    float rms[4];
-    {   // sqrt(s0*s2 - s1^2)/s0 is the population deviation: two-pass form, stable in float
-        float sw = 0.f;
-        const float inv_used = 1.f / (float)used;
-#pragma unroll
-        for (int chn = 0; chn < 3; chn++) {
-            float s1 = 0.f;
-#pragma unroll
-            for (int ip = 0; ip < NC; ip++)
-                if (on(ip)) s1 += deb[ip][chn];
-            const float mean = s1 * inv_used;
-            float var = 0.f;
-#pragma unroll
-            for (int ip = 0; ip < NC; ip++)
-                if (on(ip)) var += (deb[ip][chn] - mean) * (deb[ip][chn] - mean);
-            rms[chn] = sqrtf(var * inv_used);
-            sw += A.col_w[chn] * rms[chn] * rms[chn];
-        }
-        rms[3] = sqrtf(sw);
-    }
-    // Port weights are exponentials of squared mismatches.  They are computed relative to the
-    // largest one (exp(m - a) with m the smallest exponent), which keeps every ratio the reference
-    // forms in double within float range; absolute weights are rescaled by exp(-m) at the end.
+    const float inv_used = fast_rcp((float)used);
+    // Port weights are powers of 2 of scaled squared mismatches (exp(-k d) = 2^(-k log2(e) d)),
+    // computed relative to the largest one (2^(m - a), m the smallest exponent): every ratio the
+    // reference forms in double stays in float range; absolute weights are rescaled by 2^-m.
     float pw[NC], wscale = 1.f;
     avg[0] = avg[1] = avg[2] = 0.f;
 #pragma unroll
@@ -738,16 +725,13 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
         for (int a = 0; a < NC; a++) {
 #pragma unroll
             for (int b = a + 1; b < NC; b++) {
-                const float dxo = A.port_offsets[b][0] - A.port_offsets[a][0];
-                const float dyo = A.port_offsets[b][1] - A.port_offsets[a][1];
-                const float pd2r = A.ksigma / (dxo * dxo + dyo * dyo);
                 float d = 0.f;
 #pragma unroll
                 for (int chn = 0; chn < 3; chn++) {
-                    const float dc = (deb[a][chn] - deb[b][chn]) * iw2;
+                    const float dc = deb[a][chn] - deb[b][chn];
                     d += A.col_w[chn] * dc * dc;
                 }
-                float e = pd2r * d;
+                float e = A.pair_k_l2[np] * (d * iw2 * iw2);
                 if (!ALL && !(on(a) && on(b))) e = 3.0e38f;
                 m1 = fminf(m1, e);
                 ea[np++] = e;
@@ -758,7 +742,7 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
         for (int a = 0; a < NC; a++) {
 #pragma unroll
             for (int b = a + 1; b < NC; b++) {
-                float e = expf(m1 - ea[np++]);
+                float e = fast_exp2(m1 - ea[np++]);
                 if (!ALL && !(on(a) && on(b))) e = 0.f;
                 pw[a] += e;
                 pw[b] += e;
@@ -775,7 +759,7 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
 #pragma unroll
         for (int ip = 1; ip < NC; ip++)
             if (ip > b2 && ip != b1 && pw[ip] > bw2) { bw2 = pw[ip]; b2 = ip; }
-        const float w1 = bw1 / (bw1 + bw2), wq = 1.f - w1;
+        const float w1 = bw1 * fast_rcp(bw1 + bw2), wq = 1.f - w1;
 #pragma unroll
         for (int chn = 0; chn < 3; chn++) {
             float v1 = 0.f, v2 = 0.f;
@@ -788,32 +772,33 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
         }
         float a2[NC];
         float m2 = 3.0e38f;
+        const float k2 = A.ksigma_l2 * iw2 * iw2;
 #pragma unroll
         for (int ip = 0; ip < NC; ip++) {
             float d2 = 0.f;
 #pragma unroll
             for (int chn = 0; chn < 3; chn++) {
-                const float dc = (deb[ip][chn] - avg[chn]) * iw2;
+                const float dc = deb[ip][chn] - avg[chn];
                 d2 += A.col_w[chn] * dc * dc;
             }
-            a2[ip] = A.ksigma * d2;
+            a2[ip] = k2 * d2;
             if (on(ip)) m2 = fminf(m2, a2[ip]);
         }
         float ksum = 0.f;
 #pragma unroll
         for (int ip = 0; ip < NC; ip++) {
-            pw[ip] = on(ip) ? expf(m2 - a2[ip]) : 0.f;
+            pw[ip] = on(ip) ? fast_exp2(m2 - a2[ip]) : 0.f;
             ksum += pw[ip];
         }
-        const float k = 1.f / ksum;
+        const float k = fast_rcp(ksum);
 #pragma unroll
         for (int chn = 0; chn < 3; chn++) {
             float s = 0.f;
 #pragma unroll
-            for (int ip = 0; ip < NC; ip++) s += k * pw[ip] * deb[ip][chn];
-            avg[chn] = s;
+            for (int ip = 0; ip < NC; ip++) s += pw[ip] * deb[ip][chn];
+            avg[chn] = k * s;
         }
-        wscale = expf(-m2);
+        wscale = fast_exp2(-m2);
     } else if (used > 0) {
 #pragma unroll
         for (int ip = 0; ip < NC; ip++)
@@ -834,8 +819,9 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
         for (int ip = 1; ip < NC; ip++) av += (ip != wi) ? pw[ip] : 0.f;
         av *= 1.f / (NC - 1);
         if (av > 0.f) {   // av == 0 only when those weights are below float range: they stay 0
-            const float scale = 1.f + (av - ww) / (av * (NC - 1));
-            const float fw = ww / av;
+            const float rav = fast_rcp(av);
+            const float scale = 1.f + (av - ww) * rav * (1.f / (NC - 1));
+            const float fw = ww * rav;
 #pragma unroll
             for (int ip = 0; ip < NC; ip++) pw[ip] *= (ip == wi) ? fw : ((ip >= 1) ? scale : 1.f);
         }
@@ -843,23 +829,38 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
     float psum = 0.f;
 #pragma unroll
     for (int ip = 0; ip < NC; ip++) psum += on(ip) ? pw[ip] : 0.f;
-    out[3 * 256 + i] = psum * wscale * w2 / used;
+    out[3 * 256 + i] = psum * wscale * w2 * inv_used;
     float sw = 0.f;
 #pragma unroll
     for (int ip = 0; ip < NC; ip++) sw += pw[ip];
-    if (sw != 0.f) sw = 1.f / sw;
+    sw = (sw != 0.f) ? fast_rcp(sw) * w2 : 0.f;   // the window-free tiles times w2 are the IMCLT tiles
 #pragma unroll
     for (int chn = 0; chn < 3; chn++) {
         float s = 0.f;
 #pragma unroll
-        for (int ip = 0; ip < NC; ip++) s += sw * pw[ip] * s_iclt[ip * 3 + chn][i];
-        out[chn * 256 + i] = s;
+        for (int ip = 0; ip < NC; ip++) s += pw[ip] * s_nw[(ip * 3 + chn) * 324 + c0];
+        out[chn * 256 + i] = sw * s;
     }
-    if (planes > 4) {
+    if (planes > 4) {   // keep_weights: weights, then the rms of the ports (ImageDtt.java:2858-2874)
 #pragma unroll
         for (int ip = 0; ip < NC; ip++) out[(4 + ip) * 256 + i] = pw[ip] * wscale;
+        float swr = 0.f;
 #pragma unroll
-        for (int k = 0; k < 4; k++) out[(4 + NC + k) * 256 + i] = rms[k];
+        for (int chn = 0; chn < 3; chn++) {   // population deviation, two-pass form (stable in float)
+            float s1 = 0.f;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++)
+                if (on(ip)) s1 += deb[ip][chn];
+            const float mean = s1 * inv_used;
+            float var = 0.f;
+#pragma unroll
+            for (int ip = 0; ip < NC; ip++)
+                if (on(ip)) var += (deb[ip][chn] - mean) * (deb[ip][chn] - mean);
+            const float r = fast_sqrt(var * inv_used);
+            out[(4 + NC + chn) * 256 + i] = r;
+            swr += A.col_w[chn] * r * r;
+        }
+        out[(4 + NC + 3) * 256 + i] = fast_sqrt(swr);
     }
 }
 
@@ -869,8 +870,7 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
                                                   const float* __restrict__ extra_disp, const TexArgs* __restrict__ args_p,
                                                   float* __restrict__ tiles_out, float* __restrict__ stats_out, GeomDev g) {
     constexpr int NU = NC * 3;   // camera/colour units
-    __shared__ __attribute__((aligned(16))) float s_scr[16][256];
-    __shared__ float s_iclt[NU][256];
+    __shared__ __attribute__((aligned(16))) float s_scr[16][IMCLT_SCR_UNIT];
     __shared__ float s_nw[NU][18 * 18];   // window-free tiles with a zero border for the 3x3 debayer
     const TexArgs& A = *args_p;
     const int t = xcd_block(blockIdx.x, gridDim.x);
@@ -903,6 +903,7 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
     {
         const int unit = threadIdx.x >> 4, l = threadIdx.x & 15, hs = l >> 3, kh = l & 7;
         for (int ub = 0; ub < NU; ub += 16) {
+            if (ub + (threadIdx.x >> 6) * 4 >= NU) continue;   // wavefront without a unit
             const int uu = ub + unit;
             const bool act = uu < NU;
             const int cam = act ? uu / 3 : 0, color = act ? uu % 3 : 0;
@@ -942,14 +943,10 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
             float row[16];
             imclt_unit(l, p, q, s_scr[unit], row);
             if (act) {
-                const float wy = c_tab.hw[l < 8 ? l : 15 - l];
+                const float iwy = c_tab.ihw2[l < 8 ? l : 15 - l];
 #pragma unroll
-                for (int c = 0; c < 16; c++) {
-                    const float wx = c_tab.hw[c < 8 ? c : 15 - c];
-                    const float iw = 1.f / ((wy * wx) * (wy * wx));
-                    s_iclt[uu][l * 16 + c] = row[c];
-                    s_nw[uu][(l + 1) * 18 + c + 1] = row[c] * iw;   // un-apply the squared window
-                }
+                for (int c = 0; c < 16; c++)   // un-apply the squared window
+                    s_nw[uu][(l + 1) * 18 + c + 1] = row[c] * (iwy * c_tab.ihw2[c < 8 ? c : 15 - c]);
             }
         }
     }
@@ -957,16 +954,16 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
     // ---- debayer with shot-noise correction and re-applied window (ID:3022-3147); the zero border
     // plays the role of the dropped taps, the corner/side factors renormalise the kernel sums
     const int py = i >> 4, px = i & 15;
-    const float wwy = c_tab.hw[py < 8 ? py : 15 - py], wwx = c_tab.hw[px < 8 ? px : 15 - px];
-    const float w2 = (wwy * wwx) * (wwy * wwx);
-    const float iw2 = 1.f / w2;
+    const int qy = py < 8 ? py : 15 - py, qx = px < 8 ? px : 15 - px;
+    const float w2 = c_tab.hw2[qy] * c_tab.hw2[qx];
+    const float iw2 = c_tab.ihw2[qy] * c_tab.ihw2[qx];
     const int nb = (py == 0 || py == 15) + (px == 0 || px == 15);   // 0 inside, 1 side, 2 corner
     const float k_rb = (nb == 2) ? (16.f / 9.f) : (nb == 1) ? (4.f / 3.f) : 1.f;
     const float k_g = (nb == 2) ? (4.f / 3.f) : (nb == 1) ? (8.f / 7.f) : 1.f;
     float deb[NC][3];
+    const int c0 = (py + 1) * 18 + px + 1;
     {
-        const float kshot = 1.f / sqrtf(A.min_shot);
-        const int c0 = (py + 1) * 18 + px + 1;
+        const float kshot = fast_rcp(fast_sqrt(A.min_shot));
 #pragma unroll
         for (int uu = 0; uu < NU; uu++) {
             const float* nw = s_nw[uu];
@@ -979,18 +976,18 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
                 s = 0.125f * (nw[c0 - 18] + nw[c0 - 1] + nw[c0 + 1] + nw[c0 + 18]) + 0.5f * nw[c0];
                 s *= k_g;
             }
-            if (A.scale_shot > 0.f) s = A.scale_shot * ((s > A.min_shot) ? sqrtf(s) : (kshot * s));
+            if (A.scale_shot > 0.f) s = A.scale_shot * ((s > A.min_shot) ? fast_sqrt(s) : (kshot * s));
             deb[uu / 3][uu % 3] = s * w2;
         }
     }
     float avg[3];
-    if (port_mask == (1 << NC) - 1) combine_pixel<NC, true>(A, port_mask, deb, w2, iw2, s_iclt, i, out, planes, avg);
-    else combine_pixel<NC, false>(A, port_mask, deb, w2, iw2, s_iclt, i, out, planes, avg);
+    if (port_mask == (1 << NC) - 1) combine_pixel<NC, true>(A, port_mask, deb, w2, iw2, &s_nw[0][0], c0, out, i, planes, avg);
+    else combine_pixel<NC, false>(A, port_mask, deb, w2, iw2, &s_nw[0][0], c0, out, i, planes, avg);
     if (stats_out) {   // max_diff (ImageDtt.java:2965-2981) and ports_rgb (ImageDtt.java:2982-2996)
         // transpose through LDS: value k of pixel i at [k][i] (the NC max_diff rows reuse s_scr, the
-        // 3*NC colour rows reuse s_iclt, whose sizes match); 16 threads then reduce each row
+        // 3*NC colour rows reuse s_nw); 16 threads then reduce each row
         float* red_max = &s_scr[0][0];
-        float* red_rgb = &s_iclt[0][0];
+        float* red_rgb = &s_nw[0][0];
         __syncthreads();
 #pragma unroll
         for (int ip = 0; ip < NC; ip++) {

@@ -22,7 +22,12 @@ using namespace clt;
 // ------------------------------------------------------------------------ host math helpers
 // Closed-form tables (DttRad2.java:671-694, 952-1012)
 static void make_tables(Tables* t) {
-    for (int i = 0; i < 8; i++) t->hw[i] = (float)std::sin(M_PI / 16.0 * (i + 0.5));
+    for (int i = 0; i < 8; i++) {
+        const double w = std::sin(M_PI / 16.0 * (i + 0.5));
+        t->hw[i] = (float)w;
+        t->hw2[i] = (float)(w * w);
+        t->ihw2[i] = (float)(1.0 / (w * w));
+    }
     const double sc = std::sqrt(2.0 / 8);
     for (int j = 0; j < 8; j++)
         for (int k = 0; k < 8; k++) {
@@ -594,7 +599,15 @@ int clt_exec_textures(clt_handle* h, const float* extra_disparity) {
     a.col_w[1] = (float)(h->cp.corr_blue * gw);
     a.min_shot = h->tp.min_shot;
     a.scale_shot = h->tp.scale_shot;
-    a.ksigma = (float)(1.0 / (2.0 * h->tp.diff_sigma * h->tp.diff_sigma));
+    const double ksigma = 1.0 / (2.0 * h->tp.diff_sigma * h->tp.diff_sigma);
+    a.ksigma = (float)ksigma;
+    a.ksigma_l2 = (float)(ksigma * M_LOG2E);
+    for (int i = 0, np = 0; i < g.num_cams; i++)
+        for (int j = i + 1; j < g.num_cams; j++, np++) {   // ImageDtt.java:2881-2891 pair weights
+            const double dx = (double)h->tp.port_offsets[j][0] - h->tp.port_offsets[i][0];
+            const double dy = (double)h->tp.port_offsets[j][1] - h->tp.port_offsets[i][1];
+            a.pair_k_l2[np] = (float)(ksigma / (dx * dx + dy * dy) * M_LOG2E);
+        }
     a.corr_magic_scale = h->tp.corr_magic_scale;
     a.dust_remove = h->tp.dust_remove;
     a.planes = planes;

@@ -37,7 +37,7 @@ extern "C" {
 #define CLT_MAX_CAMS       16
 #define CLT_MAX_PAIRS     120
 #define CLT_DISP_LAYERS     8    /* per-tile disparity record, see CLT_DISP_*          */
-#define CLT_TEX_STATS      16    /* per-tile texture statistics: diff0..3 + 12 tones   */
+#define CLT_TEX_STATS      16    /* per-tile texture statistics of a quad (4*num_cams)  */
 
 enum {
     CLT_OK            =  0,
@@ -170,9 +170,10 @@ int  clt_get_clt(clt_handle* h, int32_t cam, float* out);
 /* corr: [num_tasks][num_pairs][225]; combo: [num_tasks][strip_height*15] (may be NULL);
  * disparity: [num_tasks][CLT_DISP_LAYERS] (may be NULL). */
 int  clt_get_corr2d(clt_handle* h, float* corr, float* combo, float* disparity);
-/* tiles: [num_tasks][planes][256] with planes = 12 or 4 (keep_weights); stats:
- * [num_tasks][16] = max_diff[4] (ID:2802-2806) then tones r0..r3,b0..b3,g0..g3 (ID:2807-2811),
- * or NULL. */
+/* tiles: [num_tasks][planes][256] with planes = 4 + num_cams + 4 when keep_weights (RGBA, one
+ * weight per port, rms r,b,g,combined), else 4 (RGBA); stats:
+ * [num_tasks][4*num_cams] = max_diff[num_cams] (ID:2802-2806) then the mean tones
+ * r[0..C-1], b[0..C-1], g[0..C-1] (ID:2807-2811), or NULL. */
 int  clt_get_textures(clt_handle* h, float* tiles, float* stats);
 /* [planes][height][width] overlap-added texture planes. */
 int  clt_get_combined_textures(clt_handle* h, float* out);
