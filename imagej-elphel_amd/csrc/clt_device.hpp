@@ -36,6 +36,13 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Hardware transcendentals (about 1 ulp, no denormal range handling): used where the argument
+// range is known to be normal and the reference's double result is matched to ~1e-6 anyway.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float fast_log2(float x) { return __builtin_amdgcn_logf(x); }
+
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of the
 // logical block space so neighbouring tiles share an L2.
 __device__ __forceinline__ int xcd_block(int bid, int nblocks) {

@@ -239,9 +239,11 @@ constexpr int IMCLT_R = 16;   // output block rows per segment
 
 __global__ __launch_bounds__(256) void k_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tile_map,
                                                const float* __restrict__ lpf, float* __restrict__ rbg, GeomDev g) {
+    // per unit: IMCLT scratch, then (after the unit's last scratch read) its 16x16 tile, row stride 17
     __shared__ __attribute__((aligned(16))) float s_scr[16][IMCLT_SCR_UNIT];
-    __shared__ float s_tile[16][16][17];
     __shared__ float s_carry[8][IMCLT_S * 8];
+    static_assert(16 * 17 <= IMCLT_SCR_UNIT, "tile must fit the unit scratch");
+    auto s_tile = [&](int u, int r, int c) -> float& { return s_scr[u][r * 17 + c]; };
     const int plane = blockIdx.z;               // cam * colors + color
     const int cam = plane / g.colors, color = plane % g.colors;
     const int bx0 = blockIdx.x * IMCLT_S, by0 = blockIdx.y * IMCLT_R;
@@ -280,7 +282,7 @@ __global__ __launch_bounds__(256) void k_imclt(const float* __restrict__ clt, co
         float row[16];
         imclt_unit(l, p, q, s_scr[unit], row);
 #pragma unroll
-        for (int c = 0; c < 16; c++) s_tile[unit][l][c] = row[c];
+        for (int c = 0; c < 16; c++) s_tile(unit, l, c) = row[c];
         __syncthreads();
         // assemble: output col (within strip) = s*8+c from tile s+1 (left half) and tile s (right half)
         if (threadIdx.x < IMCLT_S * 8) {
@@ -289,8 +291,8 @@ __global__ __launch_bounds__(256) void k_imclt(const float* __restrict__ clt, co
             const bool wr = (ty >= by0) && (x < out_w);
 #pragma unroll
             for (int rr = 0; rr < 8; rr++) {
-                float top = s_tile[s + 1][rr][c] + s_tile[s][rr][8 + c];
-                float bot = s_tile[s + 1][rr + 8][c] + s_tile[s][rr + 8][8 + c];
+                float top = s_tile(s + 1, rr, c) + s_tile(s, rr, 8 + c);
+                float bot = s_tile(s + 1, rr + 8, c) + s_tile(s, rr + 8, 8 + c);
                 if (wr) out[(size_t)(ty * 8 + rr) * out_w + x] = s_carry[rr][colx] + top;
                 s_carry[rr][colx] = bot;
             }
@@ -680,6 +682,356 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
     }
 }
 
+// ------------------------------------------------------------------------------------
+// Quad camera form (4 cameras, 3 colours, the 6 pairs of Correlation2d.java:52-58): one wavefront
+// per tile, the tile's 48 CLT values per frequency bin in registers; pair products pre-normalised
+// per camera/colour; the 2-D DCT-IIe through 1 KiB of LDS; the 15x15 unfold by lane shuffles
+// (lane (ay, ax) writes the up to 4 outputs (7 +- ay, 7 +- ax)); strips with tap addresses shared
+// by the pairs of one orientation.
+// ------------------------------------------------------------------------------------
+constexpr int CQ_WAVE_FLOATS = 256 + 6 * CLT_CORR_LEN + 240 + 32 + 2;   // per wavefront, multiple of 4
+
+// One sample of a strip element (C2:610-718): position of (row, scol) for orientation DIR with
+// interpolation denominator 1 << LG, sample side down = -1 / +1.  Returns false outside the
+// correlation tile (the reference stores NaN); offsets of out-of-range or zero-weight taps point
+// back at the base tap so every read stays inside the tile.
+template <int DIR, int LG>
+__device__ __forceinline__ bool strip_tap(int row, int scol, int down, int& base, int& ox, int& oy, float& fx, float& fy) {
+    constexpr int ilimit = 7 << LG, mask = (1 << LG) - 1;
+    int xnum, ynum;
+    if (DIR == 0) { xnum = 2 * scol + (row & 1); ynum = down * row; }
+    else if (DIR == 1) { xnum = down * row; ynum = 2 * scol + (row & 1); }
+    else if (DIR == 2) { xnum = scol - ((down * row) >> 1); ynum = scol + ((down * row + 1) >> 1); }
+    else { xnum = scol - ((down * row) >> 1); ynum = -scol - ((down * row + 1) >> 1); }
+    const bool ok = (xnum <= ilimit) && (xnum >= -ilimit) && (ynum <= ilimit) && (ynum >= -ilimit);
+    const int mx = (xnum + ilimit) & mask, my = (ynum + ilimit) & mask;
+    base = ok ? ((ynum + ilimit) >> LG) * 15 + ((xnum + ilimit) >> LG) : 112;
+    ox = (ok && mx) ? 1 : 0;
+    oy = (ok && my) ? 15 : 0;
+    fx = (float)mx * (1.f / (1 << LG));
+    fy = (float)my * (1.f / (1 << LG));
+    return ok;
+}
+
+// The taps of one strip element for one orientation: row 0 is one sample, other rows the mean
+// of the samples at -row and +row (C2:640-700).
+template <int DIR, int LG>
+struct StripAt {
+    int a, aox, aoy, b, box, boy;
+    float wa[4], wb[4];
+    bool nan;
+    __device__ __forceinline__ StripAt(int row, int scol) {
+        float fx, fy;
+        const bool oka = strip_tap<DIR, LG>(row, scol, -1, a, aox, aoy, fx, fy);
+        const float ka = (row == 0) ? 1.f : 0.5f;
+        wa[0] = ka * (1.f - fx) * (1.f - fy);
+        wa[1] = ka * fx * (1.f - fy);
+        wa[2] = ka * (1.f - fx) * fy;
+        wa[3] = ka * fx * fy;
+        bool okb = true;
+        if (row > 0) {
+            okb = strip_tap<DIR, LG>(row, scol, 1, b, box, boy, fx, fy);
+            wb[0] = 0.5f * (1.f - fx) * (1.f - fy);
+            wb[1] = 0.5f * fx * (1.f - fy);
+            wb[2] = 0.5f * (1.f - fx) * fy;
+            wb[3] = 0.5f * fx * fy;
+        } else {
+            b = a; box = aox; boy = aoy;
+            wb[0] = wb[1] = wb[2] = wb[3] = 0.f;
+        }
+        nan = !(oka && okb);
+    }
+    __device__ __forceinline__ float operator()(const float* c) const {
+        if (LG == 0) return wa[0] * c[a] + wb[0] * c[b];
+        return wa[0] * c[a] + wa[1] * c[a + aox] + wa[2] * c[a + aoy] + wa[3] * c[a + aox + aoy] +
+               wb[0] * c[b] + wb[1] * c[b + box] + wb[2] * c[b + boy] + wb[3] * c[b + box + boy];
+    }
+};
+
+// first maximum of 16 lanes (lane order), NaN as -inf; returns the index in every lane of the group
+__device__ __forceinline__ int argmax16(float v, int idx) {
+    if (isnan(v)) v = -INFINITY;
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) {
+        const float v2 = __shfl_xor(v, o, 16);
+        const int i2 = __shfl_xor(idx, o, 16);
+        if (v2 > v || (v2 == v && i2 < idx)) { v = v2; idx = i2; }
+    }
+    return idx;
+}
+
+__global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+                                                     int task_words, int num_tasks, const float* __restrict__ lpf,
+                                                     const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
+                                                     float* __restrict__ combo_out, float* __restrict__ disp_out, GeomDev g) {
+    extern __shared__ __attribute__((aligned(16))) float s_dyn[];
+    const CorrArgs& A = *args_p;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+    if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
+    float* S = s_dyn + wave * CQ_WAVE_FLOATS;         // [4][64] transform staging
+    float* s_corr = S + 256;                          // [6][225]
+    float* s_combo = s_corr + 6 * CLT_CORR_LEN;       // [240]
+    float* s_c1d = s_combo + 240;                     // [2][16]
+    const float qnan = __int_as_float(0x7fc00000);
+    const int hw = A.strip_height, slen = hw * 15;
+    const int32_t* rec = tasks + (size_t)t * task_words;
+    if (rec[0] == 0) {
+        for (int o = lane; o < 6 * CLT_CORR_LEN; o += 64) corr_out[(size_t)t * 6 * CLT_CORR_LEN + o] = qnan;
+        if (combo_out)
+            for (int e = lane; e < slen; e += 64) combo_out[(size_t)t * slen + e] = qnan;
+        if (lane < CLT_DISP_LAYERS && disp_out)
+            disp_out[(size_t)t * CLT_DISP_LAYERS + lane] =
+                (lane == CLT_DISP_STRENGTH || lane == CLT_DISP_HOR_STR || lane == CLT_DISP_VERT_STR ||
+                 lane == CLT_DISP_CM_WEIGHT) ? 0.f : qnan;
+        return;
+    }
+    const int txy = rec[1];
+    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+    const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
+    float D[4][3][4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+#pragma unroll
+            for (int n = 0; n < 4; n++) D[c][col][n] = clt[(((size_t)c * ntiles + tile) * 3 + col) * 256 + n * 64 + lane];
+    const bool fz = A.fat_zero != 0.f;
+    float nrm[4][3];   // per bin sum over the 4 modes of the squares (C2:205-212)
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+            nrm[c][col] = D[c][col][0] * D[c][col][0] + D[c][col][1] * D[c][col][1] + D[c][col][2] * D[c][col][2] +
+                          D[c][col][3] * D[c][col][3];
+    if (!fz) {   // 1 / sqrt(n1 n2) * w = (sqrt(w) / sqrt(n1)) * (sqrt(w) / sqrt(n2)): scale each camera once
+#pragma unroll
+        for (int col = 0; col < 3; col++) {
+            const float sw = fast_sqrt(A.col_w[col]);
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const float k = sw * __builtin_amdgcn_rsqf(nrm[c][col]);
+#pragma unroll
+                for (int n = 0; n < 4; n++) D[c][col][n] *= k;
+            }
+        }
+    }
+    const int y = lane >> 3, xx = lane & 7;
+    float cy_[8], sy_[8], cx_[8], sx_[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        cy_[k] = c_tab.ciie[y][k];
+        sy_[k] = c_tab.siie[y][k];
+        cx_[k] = c_tab.ciie[xx][k];
+        sx_[k] = c_tab.siie[xx][k];
+    }
+    const float lp = lpf[lane];
+    // unfold targets of this lane (ay, ax) = (y, xx): (7 +- y, 7 +- xx)
+    const int o_pp = (7 + y) * 15 + 7 + xx, o_pm = (7 + y) * 15 + 7 - xx;
+    const int o_mp = (7 - y) * 15 + 7 + xx, o_mm = (7 - y) * 15 + 7 - xx;
+    float* corr_t = corr_out + (size_t)t * 6 * CLT_CORR_LEN;
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+        const int c1 = QP_C1[p], c2 = QP_C2[p];
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int col = 0; col < 3; col++) {
+            const float* d1 = D[c1][col];
+            const float* d2 = D[c2][col];
+            float scale = 1.f;
+            if (fz) {   // fat zero: mean over the bins of sqrt(n1 n2) (C2:213-218)
+                const float a2 = fast_sqrt(nrm[c1][col] * nrm[c2][col]);
+                const float fz2 = wave_sum(a2) * (1.f / 64.f) * A.fat_zero * A.fat_zero;
+                scale = fast_rcp(a2 + fz2) * A.col_w[col];
+            }
+            // product with the ZI sign/index table (Correlation2d.java:86-90)
+            const float t0 = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2] + d1[3] * d2[3];
+            const float t1 = -d1[1] * d2[0] + d1[0] * d2[1] - d1[3] * d2[2] + d1[2] * d2[3];
+            const float t2 = -d1[2] * d2[0] - d1[3] * d2[1] + d1[0] * d2[2] + d1[1] * d2[3];
+            const float t3 = d1[3] * d2[0] - d1[2] * d2[1] - d1[1] * d2[2] + d1[0] * d2[3];
+            acc[0] = fmaf(t0, scale, acc[0]);
+            acc[1] = fmaf(t1, scale, acc[1]);
+            acc[2] = fmaf(t2, scale, acc[2]);
+            acc[3] = fmaf(t3, scale, acc[3]);
+        }
+        wave_sync();
+#pragma unroll
+        for (int n = 0; n < 4; n++) S[n * 64 + lane] = acc[n] * lp;
+        wave_sync();
+        // pass 1 along kv: lane (y, kh); quadrant q uses DST-IIe if q bit 1 (C2:316-319, DR:595-614)
+        float tq[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 u0 = *reinterpret_cast<const float4*>(S + q * 64 + xx * 8);
+            const float4 u1 = *reinterpret_cast<const float4*>(S + q * 64 + xx * 8 + 4);
+            const float* m = (q & 2) ? sy_ : cy_;
+            tq[q] = m[0] * u0.x + m[1] * u0.y + m[2] * u0.z + m[3] * u0.w + m[4] * u1.x + m[5] * u1.y + m[6] * u1.z + m[7] * u1.w;
+        }
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < 4; q++) S[q * 64 + y * 8 + xx] = tq[q];
+        wave_sync();
+        // pass 2 along kh: lane (y, x); quadrant q uses DST-IIe if q bit 0
+        float oq[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 u0 = *reinterpret_cast<const float4*>(S + q * 64 + y * 8);
+            const float4 u1 = *reinterpret_cast<const float4*>(S + q * 64 + y * 8 + 4);
+            const float* m = (q & 1) ? sx_ : cx_;
+            oq[q] = m[0] * u0.x + m[1] * u0.y + m[2] * u0.z + m[3] * u0.w + m[4] * u1.x + m[5] * u1.y + m[6] * u1.z + m[7] * u1.w;
+        }
+        // unfold to 15x15 (DttRad2.java:443-489): CC[ay][ax] +- SC[ay][ax-1] +- CS[ay-1][ax] +- SS[ay-1][ax-1]
+        const float a = oq[0];
+        float b = __shfl(oq[1], (lane + 63) & 63);
+        float c = __shfl(oq[2], (lane + 56) & 63);
+        float d = __shfl(oq[3], (lane + 55) & 63);
+        if (xx == 0) { b = 0.f; d = 0.f; }
+        if (y == 0) { c = 0.f; d = 0.f; }
+        const float v_pp = 0.25f * (a + b + c + d), v_pm = 0.25f * (a - b + c - d);
+        const float v_mp = 0.25f * (a + b - c - d), v_mm = 0.25f * (a - b - c + d);
+        float* co = corr_t + p * CLT_CORR_LEN;
+        float* cs = s_corr + p * CLT_CORR_LEN;
+        co[o_pp] = v_pp;
+        cs[o_pp] = v_pp;
+        if (xx > 0) { co[o_pm] = v_pm; cs[o_pm] = v_pm; }
+        if (y > 0) { co[o_mp] = v_mp; cs[o_mp] = v_mp; }
+        if (xx > 0 && y > 0) { co[o_mm] = v_mm; cs[o_mm] = v_mm; }
+    }
+    wave_sync();
+    // ---- combined strip (C2:730-790) of the 6 rotated strips (C2:556-718)
+    {
+        const int ncomb = 6 * (A.twice_diagonal ? 2 : 1);
+        const float inv_n = 1.f / ncomb;
+        for (int e = lane; e < slen; e += 64) {
+            const int row = e / 15, scol = e - row * 15 - 7;
+            const StripAt<0, 1> sh(row, scol);
+            const StripAt<1, 1> sv(row, scol);
+            const StripAt<2, 0> sd(row, scol);
+            const StripAt<3, 0> sa(row, scol);
+            float st[6] = {sh(s_corr), sh(s_corr + CLT_CORR_LEN), sv(s_corr + 2 * CLT_CORR_LEN), sv(s_corr + 3 * CLT_CORR_LEN),
+                           sd(s_corr + 4 * CLT_CORR_LEN), sa(s_corr + 5 * CLT_CORR_LEN)};
+            const bool nan = sh.nan || sv.nan || sd.nan || sa.nan;
+            float v;
+            if (A.corr_offset >= 0.f) {
+                v = 1.f;
+#pragma unroll
+                for (int k = 0; k < 6; k++) {
+                    float dd = st[k] + A.corr_offset;
+                    if (dd < 0.f) dd = 0.f;   // (keeps a NaN)
+                    v *= A.twice_diagonal ? dd * dd : dd;
+                }
+                if (v > 0.f) v = fast_exp2(fast_log2(v) * inv_n) - A.corr_offset;
+            } else {
+                v = 0.f;
+#pragma unroll
+                for (int k = 0; k < 6; k++) v += st[k];
+                v *= (A.twice_diagonal ? 2.f : 1.f) * inv_n;
+            }
+            if (nan) v = qnan;
+            s_combo[e] = v;
+            if (combo_out) combo_out[(size_t)t * slen + e] = v;
+        }
+    }
+    // ---- getMaxXSOrtho 1-D profiles (C2:2502-2651) of the horizontal (pairs 0, 1: lanes 0..14)
+    // and vertical (pairs 2, 3: lanes 32..46) scale-1 pairs; terms i and 14 - i are equal
+    {
+        const int which = lane >> 5, jj = lane & 31;
+        if (jj < 15) {
+            const float* ca = s_corr + (which ? 2 : 0) * CLT_CORR_LEN;
+            const float* cb = ca + CLT_CORR_LEN;
+            const int st_i = which ? 1 : 15, st_j = which ? 15 : 1;   // element (i, jj) offsets
+            float v = 0.f;
+            if (A.corr_offset >= 0.f) {
+                for (int i = 0; i < 8; i++) {
+                    const float nw = (i < 7) ? A.notch[i] + A.notch[14 - i] : A.notch[7];
+                    if (nw == 0.f) continue;   // the combination is finite: adds exactly 0
+                    const int o1 = i * st_i + jj * st_j, o2 = (14 - i) * st_i + jj * st_j;
+                    const float da = 0.5f * (ca[o1] + ca[o2]) + A.corr_offset;
+                    const float db = 0.5f * (cb[o1] + cb[o2]) + A.corr_offset;
+                    const float cmb = da * db;
+                    v += nw * ((cmb > 0.f) ? fast_sqrt(cmb) - A.corr_offset : -A.corr_offset);
+                }
+            } else {
+                for (int i = 0; i < 15; i++) {
+                    const int o1 = i * st_i + jj * st_j;
+                    v += 0.5f * (ca[o1] + cb[o1]) * A.notch[i];
+                }
+            }
+            s_c1d[which * 16 + jj] = v;
+        }
+    }
+    wave_sync();
+    // ---- integer maximum of the strip's middle row (getMaxXYInt, C2:802-843), centre of mass
+    // (getMaxXCm, C2:936-1032) lane-parallel
+    const int imx = __shfl(argmax16(lane < 15 ? s_combo[lane & 15] : -INFINITY, lane & 15), 0);
+    const float vmax = s_combo[imx];
+    const bool have_max = vmax >= A.min_corr;
+    const int ix = imx - 7;
+    float s0 = 0.f, sx = 0.f;
+    if (have_max) {
+        // lanes 0..8: the 9 samples of row 0 (adx 0, then 2, 4, 6, 8 with dir -1 / +1), lanes
+        // 16..23: the 8 samples of row 1 (adx 1, 3, 5, 7 with dir -1 / +1)
+        const int dh = hw > 2 ? 2 : hw;
+        const float wy_scale = (hw < 2) ? 1.f / A.wndy[0] : 1.f;
+        const int dy = lane >> 4, k = lane & 15;
+        int adx = -1, dir = 1;
+        if (dy == 0 && k < 9) {
+            adx = (k + 1) & ~1;
+            dir = (k == 0 || (k & 1) == 0) ? 1 : -1;
+        } else if (dy == 1 && k < 8) {
+            adx = 2 * (k >> 1) + 1;
+            dir = (k & 1) ? 1 : -1;
+        }
+        if (dy < dh && adx >= 0) {
+            const int odd = dy & 1;
+            const float wy = ((dy == 0) ? wy_scale : 2.f * wy_scale) * A.wndy[dy];
+            const int idx = (adx * dir) >> 1;
+            const int xq = 2 * idx + odd, x1 = 7 + ix + idx;
+            if (x1 >= 0 && x1 < 15) {
+                float dv = s_combo[15 * dy + x1];
+                if (!isnan(dv) && dv > 0.f) {
+                    dv *= wy * A.wndx[adx];
+                    s0 = dv;
+                    sx = dv * xq;
+                }
+            }
+        }
+        s0 = wave_sum(s0);
+        sx = wave_sum(sx);
+    }
+    // ---- maxima of the ortho profiles with the 3-point parabola (PolynomialApproximation, degree 2)
+    float xo = 0.f, so = 0.f;
+    {
+        const int which = lane >> 5, jj = lane & 15;
+        const float* v = s_c1d + which * 16;
+        const int ic = argmax16((lane & 31) < 15 ? v[jj] : -INFINITY, jj);
+        float xc = (float)ic;
+        if (ic > 0 && ic < 14) {
+            const float a = v[ic - 1], b = v[ic], c = v[ic + 1];
+            const float den = a - 2.f * b + c;
+            if (den != 0.f) xc = ic + (a - c) / (2.f * den);
+        }
+        const int icr = (int)floorf(xc + 0.5f);
+        xo = -(xc - 7.f);
+        so = v[icr] * 0.125f;
+    }
+    const float hor = __shfl(xo, 0), hor_s = __shfl(so, 0), vert = __shfl(xo, 32), vert_s = __shfl(so, 32);
+    if (lane < CLT_DISP_LAYERS && disp_out) {
+        float r = qnan;
+        switch (lane) {
+        case CLT_DISP_INT: r = have_max ? (float)(-ix) : qnan; break;
+        case CLT_DISP_CM: r = (have_max && s0 != 0.f) ? -(ix + sx / s0 / 2.f) : qnan; break;
+        case CLT_DISP_STRENGTH: r = have_max ? vmax : 0.f; break;
+        case CLT_DISP_HOR: r = hor; break;
+        case CLT_DISP_HOR_STR: r = hor_s; break;
+        case CLT_DISP_VERT: r = vert; break;
+        case CLT_DISP_VERT_STR: r = vert_s; break;
+        default: r = have_max ? s0 : 0.f; break;
+        }
+        disp_out[(size_t)t * CLT_DISP_LAYERS + lane] = r;
+    }
+}
+
 // ====================================================================================
 // textures: per tile RGBA (+ port weights, rms) from all cameras (ImageDtt.java:2662-2812,
 // tile_debayer_shot_corr 3022-3147, tile_combine_rgba 2830-3000).  Block (256 threads) = tile,
@@ -694,9 +1046,6 @@ struct TexArgs {
     float port_offsets[CLT_MAX_CAMS][2];
 };
 
-__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 
 // Per-pixel port combination of one tile (tile_combine_rgba, ImageDtt.java:2830-3000, with the
 // final mix ImageDtt.java:2789-2801).  ALL: every port of the NC is used (no per-port masking).

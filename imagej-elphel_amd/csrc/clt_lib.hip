@@ -566,8 +566,9 @@ int clt_exec_corr2d(clt_handle* h) {
                    h->pairs[i].dir == dq[i].dir && h->pairs[i].scale == 1;
     }
     if (quad)
-        hipLaunchKernelGGL(k_corr2d<true>, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
-                           h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
+        hipLaunchKernelGGL(k_corr2d_quad, dim3(blocks), dim3(256), (size_t)4 * CQ_WAVE_FLOATS * 4, h->stream, h->d_clt,
+                           h->d_tasks, h->task_words, h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo,
+                           h->d_disp, h->gd);
     else
         hipLaunchKernelGGL(k_corr2d<false>, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
                            h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
