@@ -10,10 +10,15 @@ Prints one JSON line: tiles/s, HBM roofline of the dominant kernel (HIP events o
 stream) and the CPU baseline (the FP64 C restatement of the reference CPU path, all host cores).
 """
 import argparse
+import csv
+import glob
 import importlib
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -89,6 +94,81 @@ def copy_calibration(mib):
     print("bench: copied %d MiB device to device" % mib, file=sys.stderr, flush=True)
 
 
+KERNEL_NAMES = {"convert": "k_convert", "corr2d": "k_corr2d", "textures": "k_textures", "imclt": "k_imclt"}
+COPY_MIB = 256
+
+
+def pmc_traffic(seed):
+    """HBM bytes per launch of each kernel from two rocprofv3 counter runs (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch) of a short child run of this script.  Each counter is scaled by
+    what it reads for a device-to-device copy of COPY_MIB MiB in the same child run.  Returns
+    ({kernel: bytes}, {"fetch": factor, "write": factor}) or (None, reason)."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    vals, factors = {}, {}
+    out_dir = tempfile.mkdtemp(prefix="clt_pmc_")
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            name = counter.lower()
+            cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out_dir, "-o", name, "--",
+                   sys.executable, os.path.abspath(__file__), "--pmc-child", "--seed", str(seed)]
+            print("bench: counter run %s ..." % counter, file=sys.stderr, flush=True)
+            r = subprocess.run(cmd, cwd=tempfile.gettempdir(), stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                               timeout=150)
+            files = glob.glob(os.path.join(out_dir, "**", name + "_counter_collection.csv"), recursive=True)
+            if r.returncode != 0 and not files:
+                return None, "rocprofv3 %s exited with %d" % (counter, r.returncode)
+            per = {}
+            for row in csv.DictReader(open(files[0])):
+                per.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+            copy = sum(sum(v) for k, v in per.items() if "copyBuffer" in k)
+            if copy <= 0:
+                return None, "no copy dispatch in the %s run" % counter
+            factors[name.split("_")[0]] = (COPY_MIB << 20) / (copy * 1024.0)
+            for key, kname in KERNEL_NAMES.items():
+                v = [x for k, xs in per.items() if kname in k for x in xs]
+                if v:
+                    vals.setdefault(key, {})[name] = sum(v) / len(v) * 1024.0
+    except subprocess.TimeoutExpired:
+        return None, "rocprofv3 timed out"
+    finally:
+        shutil.rmtree(out_dir, ignore_errors=True)
+    traffic = {k: v.get("fetch_size", 0.0) * factors["fetch"] + v.get("write_size", 0.0) * factors["write"]
+               for k, v in vals.items()}
+    return traffic, factors
+
+
+def pmc_child(args):
+    """The profiled child: one copy of known size, then two frames."""
+    pk = importlib.import_module("imagej-elphel_amd")
+    copy_calibration(COPY_MIB)
+    sc = pk.synth.make_scene(W, H, seed=args.seed, disparity=("field", 0.0, 20.0))
+    tp = make_processor(pk, sc)
+    for _ in range(2):
+        run_step(tp)
+    tp.sync()
+    tp.close()
+
+
+def make_processor(pk, sc):
+    tp = pk.GPUTileProcessor(W, H, C, K)
+    tp.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
+    tp.setBayerImages(sc.images, force=True)
+    tp.setTasks(sc.tasks)
+    tp.setCorrSigma(0.8)
+    tp.setTexParams(keep_weights=0)
+    tp.setAsync(True)
+    return tp
+
+
+def run_step(tp):
+    tp.execConverCorrectTiles(0)
+    tp.execCorrelate2D()
+    tp.execTextures()
+    tp.execImcltRbg(True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,11 +178,22 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="run one device-to-device copy of MB MiB first (HBM counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the tile path has no multi-GPU form yet); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    # the counter runs are child processes started before this process touches the GPU
+    traffic, traffic_info = None, "skipped (--no-traffic)"
+    if not args.no_traffic:
+        traffic, traffic_info = pmc_traffic(args.seed)
+        if traffic is None:
+            print("bench: no HBM counters: %s" % traffic_info, file=sys.stderr, flush=True)
     pk = importlib.import_module("imagej-elphel_amd")
     if pk.load().clt_device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
@@ -114,19 +205,10 @@ def main():
     sc = pk.synth.make_scene(W, H, seed=args.seed, disparity=("field", 0.0, 20.0))
     print("bench: scene ready in %.1f s, %d tiles" % (time.perf_counter() - t, len(sc.tasks)), file=sys.stderr, flush=True)
 
-    tp = pk.GPUTileProcessor(W, H, C, K)
-    tp.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
-    tp.setBayerImages(sc.images, force=True)
-    tp.setTasks(sc.tasks)
-    tp.setCorrSigma(0.8)
-    tp.setTexParams(keep_weights=0)
-    tp.setAsync(True)
+    tp = make_processor(pk, sc)
 
     def step():
-        tp.execConverCorrectTiles(0)
-        tp.execCorrelate2D()
-        tp.execTextures()
-        tp.execImcltRbg(True)
+        run_step(tp)
 
     for _ in range(args.warmup):
         step()
@@ -158,6 +240,14 @@ def main():
             "frame_alg_MB": round(total_alg / 1e6, 1),
             "frame_frac": round(total_alg / (sum(k["avg_ms"] for k in kernels.values()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     tp.close()
+    if traffic is None:
+        roof["traffic_note"] = traffic_info
+    else:
+        for k, v in traffic.items():
+            kernels[k]["traffic_MB"] = round(v / 1e6, 1)
+        if dom in traffic:
+            roof["traffic"] = round(traffic[dom])
+        roof["counter_scale"] = {k: round(v, 3) for k, v in traffic_info.items()}
     cpu = None
     if not args.no_cpu:
         print("bench: timing the CPU baseline ...", file=sys.stderr, flush=True)

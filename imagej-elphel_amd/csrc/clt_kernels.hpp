@@ -40,11 +40,12 @@ __device__ __forceinline__ int clamp_parity(int p, int n) {   // ImageDtt.java:4
     return p;
 }
 
-// hardware sin/cos of pi*t (v_sin_f32 / v_cos_f32 take the angle in revolutions)
-__device__ __forceinline__ void sincos_pi(float t, float* s, float* c) {
-    *s = __builtin_amdgcn_sinf(0.5f * t);
-    *c = __builtin_amdgcn_cosf(0.5f * t);
-}
+
+// LDS of one 16-lane unit: 16 rows of CONV_ROW floats (16-byte row accesses of 8 consecutive lanes
+// cover all banks once), units CONV_UNIT floats apart (neighbouring units on the other half of the
+// banks).
+constexpr int CONV_ROW = 20;
+constexpr int CONV_UNIT = 336;
 
 // Block = `colors` wavefronts, wavefront w converts colour w of 4 (task, camera) units, 16 lanes
 // per unit.  Every row of a Bayer colour plane holds samples of one column parity only, so its
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
                                                  const float* __restrict__ koffsets, const int32_t* __restrict__ tasks,
                                                  int task_words, int num_tasks, float* __restrict__ clt,
                                                  const float* __restrict__ lpf, int lpf_mask, double inv_kstep, GeomDev g) {
-    __shared__ __attribute__((aligned(16))) float s_buf[12][256];
+    __shared__ __attribute__((aligned(16))) float s_buf[12][CONV_UNIT];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int color = threadIdx.x >> 6;                     // wave-uniform
     const int unit_in_block = threadIdx.x >> 4, l = threadIdx.x & 15;
@@ -89,7 +90,46 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     const int left = pos.left, top = pos.top;
     const float rx = pos.rx, ry = pos.ry;
 
-    // ---- window row r = l: 16 pixels of this colour (others 0), parity-preserving border clamp
+    // ---- 16x16 window: coalesced row loads (lane l = window column l, one row per load), staged
+    // through LDS so that lane r then holds window row r.  Pixels of other colours are 0; outside
+    // the frame the parity-preserving clamp of ImageDtt.java:4958-4969 applies.
+    float* sb = s_buf[unit_in_block];   // [16][CONV_ROW] floats, reused by the stage H output below
+    {
+        const float* img = images + (size_t)cam * g.height * g.width;
+        const int pc = left + l;
+        const int cp = pc & 1;
+        // keep pixel (row parity rp, column parity cp) of this colour
+        bool keep_e, keep_o;   // rows top + even / top + odd
+        {
+            const int rp_e = top & 1, rp_o = rp_e ^ 1;
+            auto colour_at = [&](int rp) {
+                const bool rr = rp == g.red_row, rc = cp == g.red_col;
+                return rr ? (rc ? 0 : 2) : (rc ? 2 : 1);   // 0 red, 1 blue, 2 green
+            };
+            keep_e = active && (!bayer || colour_at(rp_e) == color);
+            keep_o = active && (!bayer || colour_at(rp_o) == color);
+        }
+        const bool inside = (left >= 0) && (left + 16 <= g.width) && (top >= 0) && (top + 16 <= g.height);
+        if (__all(inside || !active)) {
+            // (an inactive lane reads pixel (0, 0) 16 times)
+            const float* src = img + (active ? (size_t)top * g.width + pc : 0);
+            const size_t stride = active ? (size_t)g.width : 0;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const float v = src[r * stride];
+                sb[r * CONV_ROW + l] = ((r & 1) ? keep_o : keep_e) ? v : 0.f;
+            }
+        } else {
+            const int pcc = clamp_parity(pc, g.width);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int pr = clamp_parity(top + r, g.height);
+                const bool keep = (r & 1) ? keep_o : keep_e;
+                sb[r * CONV_ROW + l] = keep ? img[(size_t)pr * g.width + pcc] : 0.f;
+            }
+        }
+    }
+    wave_sync();
     const int r = l;
     const int prow = top + r;
     const bool row_red = ((prow & 1) == g.red_row);
@@ -100,29 +140,15 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     else if (color == 1) cpar = row_red ? -1 : (g.red_col ^ 1);
     else cpar = row_red ? (g.red_col ^ 1) : g.red_col;
     float x[16];
-    const float* img = images + (size_t)cam * g.height * g.width;
-    const bool inside = (left >= 0) && (left + 16 <= g.width) && (top >= 0) && (top + 16 <= g.height);
-    if (active && inside && cpar >= 0) {
-        const float* rowp = img + (size_t)prow * g.width + left;
-        const int jpar = (cpar == 2) ? 2 : (cpar ^ (left & 1));        // window-relative parity
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const float v = rowp[j];
-            x[j] = (jpar == 2 || (j & 1) == jpar) ? v : 0.f;
-        }
-    } else {
-        const int pi = clamp_parity(prow, g.height);
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const int pc = left + j;
-            x[j] = 0.f;
-            if (active && cpar >= 0 && (cpar == 2 || (pc & 1) == cpar)) x[j] = img[(size_t)pi * g.width + clamp_parity(pc, g.width)];
-        }
+    for (int j = 0; j < 16; j += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(&sb[r * CONV_ROW + j]);
+        x[j] = v.x; x[j + 1] = v.y; x[j + 2] = v.z; x[j + 3] = v.w;
     }
+    wave_sync();
     float ahs, ahc, avs, avc;
     sincos_pi(rx * 0.0625f, &ahs, &ahc);
     sincos_pi(ry * 0.0625f, &avs, &avc);
-    float* sb = s_buf[unit_in_block];
     {   // ---- stage H: horizontal fold + DCT-IV (C) / DST-IV (S)
         float yc[8], ys[8], hc[8], hsn[8];
         fold16(x, ahc, ahs, yc, ys);
@@ -136,8 +162,8 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
         }
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            sb[k * 16 + r] = hc[k];
-            sb[(8 + k) * 16 + r] = hsn[k];
+            sb[k * CONV_ROW + r] = hc[k];
+            sb[(8 + k) * CONV_ROW + r] = hsn[k];
         }
     }
     wave_sync();
@@ -147,7 +173,7 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     float col[16];
 #pragma unroll
     for (int i = 0; i < 16; i += 4) {
-        float4 v = *reinterpret_cast<const float4*>(&sb[(hs * 8 + kh) * 16 + i]);
+        float4 v = *reinterpret_cast<const float4*>(&sb[(hs * 8 + kh) * CONV_ROW + i]);
         col[i] = v.x; col[i + 1] = v.y; col[i + 2] = v.z; col[i + 3] = v.w;
     }
     float a[8], b[8];
@@ -171,8 +197,8 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
         float pa[8], pb[8], k0[8], k1[8], k2[8], k3[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            pa[k] = __shfl_xor(a[k], 8, 16) * sigma;
-            pb[k] = __shfl_xor(b[k], 8, 16) * sigma;
+            pa[k] = lane_xor8(a[k]) * sigma;
+            pb[k] = lane_xor8(b[k]) * sigma;
         }
         const float* kq = kt + kh * 8;
 #pragma unroll
@@ -198,12 +224,11 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     {
         float sh, ch, cv[8], sv[8];
         sincos_pi((kh + 0.5f) * rx * 0.125f, &sh, &ch);
-#pragma unroll
-        for (int k = 0; k < 8; k++) sincos_pi((k + 0.5f) * ry * 0.125f, &sv[k], &cv[k]);
+        half_phase8(ry * 0.125f, cv, sv);
 #pragma unroll
         for (int k = 0; k < 8; k++) {
-            float pa = __shfl_xor(a[k], 8, 16) * sigma;
-            float pb = __shfl_xor(b[k], 8, 16) * sigma;
+            float pa = lane_xor8(a[k]) * sigma;
+            float pb = lane_xor8(b[k]) * sigma;
             float ta = a[k] * ch + pa * sh;
             float tb = b[k] * ch + pb * sh;
             a[k] = ta * cv[k] - tb * sv[k];
@@ -1271,12 +1296,12 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
                 const float syp = extra * A.port_offsets[cam][1] / A.corr_magic_scale;
                 const float sigma = hs ? 1.f : -1.f;
                 float sh, ch, cv[8], sv[8];
-                sincospif((kh + 0.5f) * sxp * 0.125f, &sh, &ch);
+                sincos_pi((kh + 0.5f) * sxp * 0.125f, &sh, &ch);
                 half_phase8(syp * 0.125f, cv, sv);
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    const float pa = __shfl_xor(p[k], 8, 16) * sigma;
-                    const float pb = __shfl_xor(q[k], 8, 16) * sigma;
+                    const float pa = lane_xor8(p[k]) * sigma;
+                    const float pb = lane_xor8(q[k]) * sigma;
                     const float ta = p[k] * ch + pa * sh;
                     const float tb = q[k] * ch + pb * sh;
                     p[k] = ta * cv[k] - tb * sv[k];

@@ -20,7 +20,7 @@ __constant__ Tables c_tab;
 using namespace clt;
 
 // ------------------------------------------------------------------------ host math helpers
-// Closed-form tables (DttRad2.java:671-694, 952-1012)
+// Closed-form tables (DttRad2.java:671-694, 952-1012; the DCT-IV twiddles of its FFT form)
 static void make_tables(Tables* t) {
     for (int i = 0; i < 8; i++) {
         const double w = std::sin(M_PI / 16.0 * (i + 0.5));
@@ -28,10 +28,15 @@ static void make_tables(Tables* t) {
         t->hw2[i] = (float)(w * w);
         t->ihw2[i] = (float)(1.0 / (w * w));
     }
+    for (int n = 0; n < 4; n++) {
+        t->d4pre[n][0] = (float)(0.5 * std::cos(M_PI * (n + 0.25) / 8.0));
+        t->d4pre[n][1] = (float)(-0.5 * std::sin(M_PI * (n + 0.25) / 8.0));
+        t->d4post[n][0] = (float)std::cos(M_PI * n / 8.0);
+        t->d4post[n][1] = (float)(-std::sin(M_PI * n / 8.0));
+    }
     const double sc = std::sqrt(2.0 / 8);
     for (int j = 0; j < 8; j++)
         for (int k = 0; k < 8; k++) {
-            t->civ[j][k] = (float)(sc * std::cos((2 * j + 1) * (2 * k + 1) * M_PI / 32.0));
             t->ciie[j][k] = (float)(sc * std::cos(j * (2 * k + 1) * M_PI / 16.0));
             t->siie[j][k] = (float)(sc * std::sin((j + 1) * (2 * k + 1) * M_PI / 16.0));
         }
