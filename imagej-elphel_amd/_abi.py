@@ -43,6 +43,14 @@ class clt_tex_params(ctypes.Structure):
                 ("port_offsets", (ctypes.c_float * 2) * 16)]
 
 
+class clt_port_geometry(ctypes.Structure):
+    _fields_ = [("num_sensors", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("focal_length", ctypes.c_double), ("pixel_size", ctypes.c_double),
+                ("distortion_radius", ctypes.c_double), ("distortion", ctypes.c_double * 7),
+                ("rxy", (ctypes.c_double * 2) * 4), ("pxy0", (ctypes.c_double * 2) * 4),
+                ("roll_deg", ctypes.c_double * 4), ("extrinsic", ctypes.c_double * 13)]
+
+
 _P = ctypes.c_void_p
 _H = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -81,6 +89,11 @@ SIGNATURES = {
     "clt_set_timing": (_I, [_H, _I]),
     "clt_get_timing": (_I, [_H, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I)]),
     "clt_reset_timing": (_I, [_H]),
+    "clt_default_port_geometry": (None, [ctypes.POINTER(clt_port_geometry), _I, _I]),
+    "clt_set_port_geometry": (_I, [_H, ctypes.POINTER(clt_port_geometry)]),
+    "clt_set_tasks_geometry": (_I, [_H, _P, _P, _P, _I]),
+    "clt_get_tasks": (_I, [_H, _P]),
+    "clt_get_port_derivatives": (_I, [_H, _P]),
 }
 
 _lib = None

@@ -254,6 +254,109 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
 }
 
 // ====================================================================================
+// Port geometry (GeometryCorrection.getPortsCoordinatesAndDerivatives, GC:2159-2314) in double
+// precision, one thread per (task, sensor); replaces the host loop of getTileSubcamOffsets
+// (GTP:686-778).  Contraction off: the same rounding as the double-precision reference.
+// ====================================================================================
+struct PortGeomDev {
+    double rots[4][9];          // per sensor rotation/zoom matrix (GC:1043-1076)
+    double drots[4][4][9];      // its derivatives by azimuth, tilt, roll, zoom (GC:1083-1147)
+    double rxy[4][2], pxy0[4][2];
+    double dist[7];             // C, B, A, A5, A6, A7, A8
+    double fl_pix, ri_scale, half_w, half_h, rd_scale, step_r;
+    int tab_len;
+};
+
+#pragma clang fp contract(off)
+__global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restrict__ pg, const double* __restrict__ rtab,
+                                                      const int32_t* __restrict__ words, const int32_t* __restrict__ txy,
+                                                      const float* __restrict__ disparity, int num_tasks, int task_words,
+                                                      int32_t* __restrict__ tasks, double* __restrict__ deriv) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gid >> 2, i = gid & 3;
+    if (t >= num_tasks) return;
+    int32_t* rec = tasks + (size_t)t * task_words;
+    const int word = words[t], xy16 = txy[t];
+    if (i == 0) {
+        rec[0] = word;
+        rec[1] = xy16;
+    }
+    double* dr = deriv ? deriv + (size_t)t * 8 * 14 + (size_t)(2 * i) * 14 : nullptr;
+    if (word == 0) {   // GTP:720-722: nothing to do for this tile
+        rec[2 + 2 * i] = 0;
+        rec[3 + 2 * i] = 0;
+        if (dr)
+            for (int k = 0; k < 28; k++) dr[k] = 0.0;
+        return;
+    }
+    const PortGeomDev& P = *pg;
+    const double px = (double)((xy16 & 0xffff) * 8 + 4), py = (double)(((xy16 >> 16) & 0xffff) * 8 + 4);   // GTP:724-725
+    const double d = (double)disparity[t];
+    // undo the common distortion of the tile centre (GC:2181-2187, getRByRDist GC:2802-2830)
+    const double pXcd = px - P.half_w, pYcd = py - P.half_h;
+    const double rrel = sqrt(pXcd * pXcd + pYcd * pYcd) * P.rd_scale;
+    double rND2R = __longlong_as_double(0x7ff8000000000000LL);
+    if (rrel >= 0.0) {
+        const int index = (int)floor(rrel / P.step_r);
+        if (index < P.tab_len - 1) rND2R = rtab[index] + (rtab[index + 1] - rtab[index]) * (rrel / P.step_r - index);
+    }
+    const double vi[3] = {pXcd * rND2R - d * P.rxy[i][0], pYcd * rND2R - d * P.rxy[i][1], P.fl_pix};
+    const double* R = P.rots[i];
+    const double rvi[3] = {R[0] * vi[0] + R[1] * vi[1] + R[2] * vi[2], R[3] * vi[0] + R[4] * vi[1] + R[5] * vi[2],
+                           R[6] * vi[0] + R[7] * vi[1] + R[8] * vi[2]};
+    const double norm_z = P.fl_pix / rvi[2];
+    const double pXci = rvi[0] * norm_z, pYci = rvi[1] * norm_z;
+    const double rNDi = sqrt(pXci * pXci + pYci * pYci);
+    const double ri = rNDi * P.ri_scale;
+    double k = 1.0, rri = 1.0, dk_dri = 0.0;
+#pragma unroll
+    for (int j = 0; j < 7; j++) {   // GC:2217-2222 and GC:2264-2268
+        dk_dri += P.dist[j] * (j + 1) * rri;
+        rri *= ri;
+        k += P.dist[j] * (rri - 1.0);
+    }
+    rec[2 + 2 * i] = __float_as_int((float)(pXci * k + P.pxy0[i][0]));
+    rec[3 + 2 * i] = __float_as_int((float)(pYci * k + P.pxy0[i][1]));
+    if (!dr) return;
+    double dx[5], dy[5];   // by azimuth, tilt, roll, zoom (GC:2235-2283), disparity
+#pragma unroll
+    for (int a = 0; a < 5; a++) {
+        double dv[3];
+        if (a < 4) {
+            const double* D = P.drots[i][a];
+#pragma unroll
+            for (int r = 0; r < 3; r++) dv[r] = D[r * 3] * vi[0] + D[r * 3 + 1] * vi[1] + D[r * 3 + 2] * vi[2];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 3; r++) dv[r] = -(R[r * 3] * P.rxy[i][0] + R[r * 3 + 1] * P.rxy[i][1]);
+        }
+        const double dxc = dv[0] * norm_z - pXci * dv[2] / rvi[2];
+        const double dyc = dv[1] * norm_z - pYci * dv[2] / rvi[2];
+        const double dri = (a == 2) ? 0.0 : P.ri_scale / rNDi * (pXci * dxc + pYci * dyc);   // roll keeps the radius
+        const double dkk = dk_dri * dri;
+        dx[a] = dxc * k + pXci * dkk;
+        dy[a] = dyc * k + pYci * dkk;
+    }
+    double* rx = dr;
+    double* ry = dr + 14;
+    for (int c = 0; c < 14; c++) rx[c] = ry[c] = 0.0;
+    if (i < 3) {   // sensor 3 tilt/azimuth/zoom are minus the sum of the others (GC:2285-2302)
+        rx[0 + i] = dx[1]; ry[0 + i] = dy[1];
+        rx[3 + i] = dx[0]; ry[3 + i] = dy[0];
+        rx[10 + i] = dx[3]; ry[10 + i] = dy[3];
+    } else {
+        for (int j = 0; j < 3; j++) {
+            rx[0 + j] = -dx[1]; ry[0 + j] = -dy[1];
+            rx[3 + j] = -dx[0]; ry[3 + j] = -dy[0];
+            rx[10 + j] = -dx[3]; ry[10 + j] = -dy[3];
+        }
+    }
+    rx[6 + i] = dx[2]; ry[6 + i] = dy[2];
+    rx[13] = dx[4]; ry[13] = dy[4];
+}
+#pragma clang fp contract(fast)
+
+// ====================================================================================
 // imclt_rbg: inverse CLT + overlap-add, gather form (no read-modify-write, one launch).
 // Block = strip of IMCLT_S output 8x8 blocks wide, IMCLT_R block rows tall; it sweeps down
 // the tile rows keeping the bottom halves of the previous tile row in LDS.  Output image of a

@@ -127,6 +127,13 @@ struct clt_handle {
     float* d_tex_stats = nullptr;  // [max_tasks][4C]
     float* d_tex_img = nullptr;    // [planes][H][W]
     float* d_extra = nullptr;      // [max_tasks]
+    PortGeomDev* d_port_geom = nullptr;
+    double* d_rtab = nullptr;      // reverse distortion table
+    int32_t* d_gwords = nullptr;   // [max_tasks] task words of clt_set_tasks_geometry
+    int32_t* d_gtxy = nullptr;     // [max_tasks]
+    float* d_gdisp = nullptr;      // [max_tasks]
+    double* d_gderiv = nullptr;    // [max_tasks][8][14]
+    bool port_geom_set = false, tasks_from_geom = false;
     CorrArgs* d_corr_args = nullptr;
     TexArgs* d_tex_args = nullptr;
     size_t corr_cap = 0, combo_cap = 0, tex_cap = 0, tex_img_cap = 0;
@@ -325,7 +332,8 @@ int clt_destroy(clt_handle* h) {
     if (!h) return CLT_E_ARG;
     void* bufs[] = {h->d_images, h->d_kernels, h->d_koffs, h->d_tasks, h->d_tile_map, h->d_clt, h->d_lpf_rbg,
                     h->d_lpf_corr, h->d_rbg, h->d_corr, h->d_combo, h->d_disp, h->d_tex, h->d_tex_stats,
-                    h->d_tex_img, h->d_extra, h->d_corr_args, h->d_tex_args};
+                    h->d_tex_img, h->d_extra, h->d_corr_args, h->d_tex_args, h->d_port_geom, h->d_rtab,
+                    h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv};
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void* p : bufs)
         if (p) (void)hipFree(p);
@@ -375,6 +383,7 @@ int clt_set_image(clt_handle* h, int32_t cam, const float* image, int32_t stride
     const clt_geom& g = h->g;
     if (cam < 0 || cam >= g.num_cams || !image || stride < g.width)
         return fail(h, CLT_E_ARG, "clt_set_image: bad camera %d, null image or stride %d < width %d", cam, stride, g.width);
+    HIPCHK(h, hipStreamSynchronize(h->stream));   // the stream may still read the previous frame
     HIPCHK(h, hipMemcpy2D(h->d_images + (size_t)cam * g.width * g.height, (size_t)g.width * 4, image, (size_t)stride * 4,
                           (size_t)g.width * 4, g.height, hipMemcpyHostToDevice));
     h->images_set[cam] = true;
@@ -386,6 +395,7 @@ int clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks) {
     if (num_tasks < 0 || num_tasks > h->max_tasks || (num_tasks > 0 && !tasks))
         return fail(h, CLT_E_ARG, "clt_set_tasks: %d tasks (capacity %d)", num_tasks, h->max_tasks);
     const int32_t* t = (const int32_t*)tasks;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
     std::fill(h->h_tile_map.begin(), h->h_tile_map.end(), -1);
     for (int i = 0; i < num_tasks; i++) {
         const int32_t* r = t + (size_t)i * h->task_words;
@@ -398,6 +408,182 @@ int clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks) {
     HIPCHK(h, hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice));
     h->num_tasks = num_tasks;
     h->converted = h->corr_done = h->tex_done = h->combined = false;
+    h->tasks_from_geom = false;
+    return CLT_OK;
+}
+
+// ------------------------------------------------------------------------ port geometry
+void clt_default_port_geometry(clt_port_geometry* p, int32_t width, int32_t height) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->num_sensors = 4;
+    p->width = width;
+    p->height = height;
+    p->focal_length = 4.5;         // GC:37
+    p->pixel_size = 2.2;           // GC:39
+    p->distortion_radius = 2.8512; // GC:38
+    static const double rxy[4][2] = {{-0.5, -0.5}, {0.5, -0.5}, {-0.5, 0.5}, {0.5, 0.5}};   // GC:78
+    for (int i = 0; i < 4; i++) {
+        p->rxy[i][0] = rxy[i][0];
+        p->rxy[i][1] = rxy[i][1];
+        p->pxy0[i][0] = 0.5 * width;
+        p->pxy0[i][1] = 0.5 * height;
+    }
+}
+
+static void mat3(const double* a, const double* b, double* c) {
+    double t[9];
+    for (int r = 0; r < 3; r++)
+        for (int k = 0; k < 3; k++) t[r * 3 + k] = a[r * 3] * b[k] + a[r * 3 + 1] * b[3 + k] + a[r * 3 + 2] * b[6 + k];
+    std::memcpy(c, t, sizeof(t));
+}
+
+int clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p) {
+    if (!h) return CLT_E_ARG;
+    if (!p || p->num_sensors != 4 || h->g.num_cams != 4 || p->width <= 0 || p->height <= 0 || p->pixel_size <= 0 ||
+        p->distortion_radius <= 0 || p->focal_length <= 0)
+        return fail(h, CLT_E_ARG, "clt_set_port_geometry: needs a quad camera handle and positive sizes");
+    PortGeomDev pg;
+    std::memset(&pg, 0, sizeof(pg));
+    const double* v = p->extrinsic;
+    for (int i = 0; i < 4; i++) {   // CorrVector angles (GC:1217-1269) and matrices (GC:1043-1147)
+        const double tl = (i < 3) ? v[i] : -(v[0] + v[1] + v[2]);
+        const double az = (i < 3) ? v[3 + i] : -(v[3] + v[4] + v[5]);
+        const double rl = v[6 + i] + p->roll_deg[i] * M_PI / 180.0;
+        const double zm = 1.0 + ((i < 3) ? v[10 + i] : -(v[10] + v[11] + v[12]));
+        const double ca = std::cos(az), sa = std::sin(az), ct = std::cos(tl), st = std::sin(tl);
+        const double cr = std::cos(rl), sr = std::sin(rl);
+        // azimuth (sign -1), tilt, roll with zoom; "inverted" rotations as in the reference
+        const double m_az[9] = {ca, 0, -sa, 0, 1, 0, sa, 0, ca};
+        const double m_t[9] = {1, 0, 0, 0, ct, st, 0, -st, ct};
+        const double m_r[9] = {cr * zm, sr * zm, 0, -sr * zm, cr * zm, 0, 0, 0, 1};
+        const double d_az[9] = {-sa, 0, -ca, 0, 0, 0, ca, 0, -sa};
+        const double d_t[9] = {0, 0, 0, 0, -st, ct, 0, -ct, -st};
+        const double d_r[9] = {-sr * zm, cr * zm, 0, -cr * zm, -sr * zm, 0, 0, 0, 0};
+        const double d_z[9] = {cr, sr, 0, -sr, cr, 0, 0, 0, 0};
+        double ta[9], tmp[9];
+        mat3(m_t, m_az, ta);
+        mat3(m_r, ta, pg.rots[i]);
+        mat3(m_t, d_az, tmp);
+        mat3(m_r, tmp, pg.drots[i][0]);
+        mat3(d_t, m_az, tmp);
+        mat3(m_r, tmp, pg.drots[i][1]);
+        mat3(d_r, ta, pg.drots[i][2]);
+        mat3(d_z, ta, pg.drots[i][3]);
+        pg.rxy[i][0] = p->rxy[i][0];
+        pg.rxy[i][1] = p->rxy[i][1];
+        pg.pxy0[i][0] = p->pxy0[i][0];
+        pg.pxy0[i][1] = p->pxy0[i][1];
+    }
+    for (int j = 0; j < 7; j++) pg.dist[j] = p->distortion[j];
+    pg.fl_pix = p->focal_length / (0.001 * p->pixel_size);
+    pg.ri_scale = 0.001 * p->pixel_size / p->distortion_radius;
+    pg.rd_scale = 0.001 * p->pixel_size / p->distortion_radius;
+    pg.half_w = 0.5 * p->width;
+    pg.half_h = 0.5 * p->height;
+    // reverse distortion table (calcReverseDistortionTable, GC:2707-2756): r / rD by Newton steps
+    const double step = 0.001, max_r = 2.0;
+    const int len = (int)std::ceil(max_r / step) + 1;
+    std::vector<double> tab(len, std::nan(""));
+    {
+        const double C = p->distortion[0], B = p->distortion[1], A = p->distortion[2], A5 = p->distortion[3];
+        const double A6 = p->distortion[4], A7 = p->distortion[5], A8 = p->distortion[6];
+        const double d0 = 1.0 - A8 - A7 - A6 - A5 - A - B - C;
+        auto k_of = [&](double r) { return (((((((A8 * r + A7) * r + A6) * r + A5) * r + A) * r + B) * r + C) * r + d0); };
+        auto dk_of = [&](double r) {
+            return (((((((8 * A8 * r + 7 * A7) * r + 6 * A6) * r + 5 * A5) * r + 4 * A) * r + 3 * B) * r + 2 * C) * r + d0);
+        };
+        tab[0] = 1.0 / d0;
+        double rprev = 0.0, slope = 1.0;
+        for (int i = 1; i < len; i++) {
+            const double rd_target = step * i;
+            double r = rprev + step / slope;
+            bool ok = false;
+            for (int it = 0; it < 1000; it++) {
+                slope = dk_of(r);
+                const double rd = r * k_of(r);
+                if (slope < 0.1) break;   // too much distortion: the rest of the table stays NaN
+                if (std::fabs(rd - rd_target) < 1e-8) { ok = true; break; }
+                r += (rd_target - rd) / slope;
+                ok = (it == 999);
+            }
+            if (slope < 0.1) break;
+            (void)ok;
+            rprev = r;
+            tab[i] = r / rd_target;
+        }
+    }
+    pg.step_r = step;
+    pg.tab_len = len;
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_port_geom, sizeof(PortGeomDev))) ||
+        (rc = dalloc(h, (void**)&h->d_rtab, sizeof(double) * len)))
+        return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(h->d_port_geom, &pg, sizeof(pg), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_rtab, tab.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+    h->port_geom_set = true;
+    return CLT_OK;
+}
+
+static int launch_port_centres(clt_handle* h, double* deriv) {
+    const int n = h->num_tasks;
+    if (n == 0) return CLT_OK;
+    hipLaunchKernelGGL(k_port_centres, dim3((4 * n + 255) / 256), dim3(256), 0, h->stream, h->d_port_geom, h->d_rtab,
+                       h->d_gwords, h->d_gtxy, h->d_gdisp, n, h->task_words, h->d_tasks, deriv);
+    HIPCHK(h, hipGetLastError());
+    return CLT_OK;
+}
+
+int clt_set_tasks_geometry(clt_handle* h, const int32_t* task, const int32_t* txy, const float* disparity,
+                           int32_t num_tasks) {
+    if (!h) return CLT_E_ARG;
+    if (!h->port_geom_set) return fail(h, CLT_E_STATE, "clt_set_tasks_geometry: no port geometry set");
+    if (num_tasks < 0 || num_tasks > h->max_tasks || (num_tasks > 0 && (!task || !txy || !disparity)))
+        return fail(h, CLT_E_ARG, "clt_set_tasks_geometry: %d tasks (capacity %d)", num_tasks, h->max_tasks);
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_gwords, (size_t)h->max_tasks * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_gtxy, (size_t)h->max_tasks * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_gdisp, (size_t)h->max_tasks * 4)))
+        return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    std::fill(h->h_tile_map.begin(), h->h_tile_map.end(), -1);
+    for (int i = 0; i < num_tasks; i++) {
+        const int tx = txy[i] & 0xffff, ty = (txy[i] >> 16) & 0xffff;
+        if (tx >= h->tiles_x || ty >= h->tiles_y)
+            return fail(h, CLT_E_ARG, "clt_set_tasks_geometry: task %d tile (%d,%d) outside %dx%d", i, tx, ty, h->tiles_x, h->tiles_y);
+        if (task[i] != 0) h->h_tile_map[(size_t)ty * h->tiles_x + tx] = i;
+    }
+    if (num_tasks > 0) {
+        HIPCHK(h, hipMemcpy(h->d_gwords, task, (size_t)num_tasks * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->d_gtxy, txy, (size_t)num_tasks * 4, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->d_gdisp, disparity, (size_t)num_tasks * 4, hipMemcpyHostToDevice));
+    }
+    HIPCHK(h, hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice));
+    h->num_tasks = num_tasks;
+    h->converted = h->corr_done = h->tex_done = h->combined = false;
+    h->tasks_from_geom = true;
+    if ((rc = launch_port_centres(h, nullptr))) return rc;
+    return finish(h);
+}
+
+int clt_get_tasks(clt_handle* h, void* tasks) {
+    if (!h || !tasks) return CLT_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->num_tasks > 0)
+        HIPCHK(h, hipMemcpy(tasks, h->d_tasks, (size_t)h->num_tasks * h->task_words * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+int clt_get_port_derivatives(clt_handle* h, double* out) {
+    if (!h || !out) return CLT_E_ARG;
+    if (!h->tasks_from_geom) return fail(h, CLT_E_STATE, "clt_get_port_derivatives: tasks were not set from the port geometry");
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_gderiv, (size_t)h->max_tasks * 8 * 14 * sizeof(double)))) return rc;
+    if ((rc = launch_port_centres(h, h->d_gderiv))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->num_tasks > 0)
+        HIPCHK(h, hipMemcpy(out, h->d_gderiv, (size_t)h->num_tasks * 8 * 14 * sizeof(double), hipMemcpyDeviceToHost));
     return CLT_OK;
 }
 

@@ -102,6 +102,56 @@ class GPUTileProcessor:
         self.num_task_tiles = recs.shape[0]
         self._tasks = recs
 
+    # ------------------------------------------------------------ port geometry (GTP:686-778)
+    @staticmethod
+    def defaultPortGeometry(width=2592, height=1936, **kw):
+        """clt_port_geometry with the reference defaults (zero distortion/extrinsics, GC:37-78);
+        keyword arguments set fields (distortion, extrinsic, roll_deg, rxy, pxy0 as sequences)."""
+        p = _abi.clt_port_geometry()
+        _abi.load().clt_default_port_geometry(ctypes.byref(p), width, height)
+        for k, v in kw.items():
+            if k in ("distortion", "extrinsic", "roll_deg"):
+                arr = getattr(p, k)
+                for i, x in enumerate(v):
+                    arr[i] = x
+            elif k in ("rxy", "pxy0"):
+                arr = getattr(p, k)
+                for i, (x, y) in enumerate(v):
+                    arr[i][0], arr[i][1] = x, y
+            else:
+                setattr(p, k, v)
+        return p
+
+    def setPortGeometry(self, geometry):
+        """The GeometryCorrection of the camera (rotation matrices, reverse distortion table)."""
+        self._chk(self.lib.clt_set_port_geometry(self.h, ctypes.byref(geometry)))
+
+    def getTileSubcamOffsets(self, task, tx, ty, target_disparity):
+        """GTP:686-778 followed by setTasks: the port centres of every task are computed on the
+        GPU from the port geometry.  task, tx, ty, target_disparity: per-task arrays."""
+        task = np.ascontiguousarray(task, np.int32)
+        txy = np.ascontiguousarray(np.asarray(tx, np.int32) | (np.asarray(ty, np.int32) << 16), np.int32)
+        disp = np.ascontiguousarray(target_disparity, np.float32)
+        n = len(task)
+        if txy.shape != (n,) or disp.shape != (n,):
+            raise ValueError("task, tx, ty and target_disparity must have one entry per task")
+        self._chk(self.lib.clt_set_tasks_geometry(self.h, _ptr(task), _ptr(txy), _ptr(disp), n))
+        self.num_task_tiles = n
+        self._tasks = None
+
+    def getTasks(self):
+        """The current task records [n][2+2C] (int32; port centres as float bits)."""
+        out = np.zeros((self.num_task_tiles, 2 + 2 * self.NUM_CAMS), np.int32)
+        self._chk(self.lib.clt_get_tasks(self.h, _ptr(out)))
+        return out
+
+    def getPortDerivatives(self):
+        """[n][8][14] derivatives of the port centres (x0, y0, ..., x3, y3) by the 13 extrinsic
+        parameters (GC:2245-2306) and by the disparity (last column)."""
+        out = np.zeros((self.num_task_tiles, 8, 14), np.float64)
+        self._chk(self.lib.clt_get_port_derivatives(self.h, _ptr(out)))
+        return out
+
     def setConvolutionKernel(self, kernel, kernel_offsets, ncam):
         """GTP:373-379: float kernels [KY][KX][K][4][8][8] ([v][h]) and offsets [KY][KX][K][8]."""
         k = np.ascontiguousarray(kernel, np.float32)

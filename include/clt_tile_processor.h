@@ -178,6 +178,45 @@ int  clt_get_textures(clt_handle* h, float* tiles, float* stats);
 /* [planes][height][width] overlap-added texture planes. */
 int  clt_get_combined_textures(clt_handle* h, float* out);
 
+/* ---- port geometry on the device ---------------------------------------------------------------
+ * The reference computes the port tile centres of every task on the host in double precision
+ * (getTileSubcamOffsets, GTP:686-778, calling GeometryCorrection.getPortsCoordinatesAndDerivatives,
+ * GC:2159-2314) before setTasks.  Here the same model runs on the GPU in double precision: undo the
+ * common radial distortion of the tile centre, subtract disparity * rXY, rotate/zoom per sensor
+ * (CorrVector, GC:1043-1147), project, re-apply the radial distortion, add the sensor centre. */
+#define CLT_EXTRINSICS 13   /* CorrVector: tilt0..2, azimuth0..2, roll0..3, zoom0..2 (GC:1004-1011) */
+typedef struct clt_port_geometry {
+    int32_t num_sensors;        /* 4: the CorrVector model is that of a quad camera               */
+    int32_t width, height;      /* pixelCorrectionWidth, pixelCorrectionHeight (GC:45-46)         */
+    int32_t reserved;
+    double  focal_length;       /* mm (GC:47)                                                     */
+    double  pixel_size;         /* um (GC:48)                                                     */
+    double  distortion_radius;  /* mm (GC:49)                                                     */
+    double  distortion[7];      /* distortionC, B, A, A5, A6, A7, A8: the r^2..r^8 terms (GC:2188) */
+    double  rxy[4][2];          /* port offsets in disparity units (GC:77 rXY)                    */
+    double  pxy0[4][2];         /* sensor centres, pixels (GC:72 pXY0)                            */
+    double  roll_deg[4];        /* factory rolls, degrees (GC:71 roll)                            */
+    double  extrinsic[CLT_EXTRINSICS];   /* CorrVector.vector                                     */
+} clt_port_geometry;
+
+/* Zero distortion, zero extrinsics, quad rXY {(-1/2,-1/2),(1/2,-1/2),(-1/2,1/2),(1/2,1/2)}
+ * (GC:78), sensor centres at the frame centre, reference focal length/pixel/radius (GC:37-39). */
+void clt_default_port_geometry(clt_port_geometry* p, int32_t width, int32_t height);
+/* Rotation matrices, their derivatives and the reverse distortion table (calcReverseDistortionTable,
+ * GC:2707-2756) are prepared on the host and uploaded; needs a handle of 4 cameras. */
+int  clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p);
+/* getTileSubcamOffsets + setTasks: task words, txy = tx | ty << 16 and target disparities of
+ * num_tasks tiles; the tile centre is (8 tx + 4, 8 ty + 4) (GTP:724-725).  Records of tasks with a
+ * zero task word keep zero centres. */
+int  clt_set_tasks_geometry(clt_handle* h, const int32_t* task, const int32_t* txy, const float* disparity,
+                            int32_t num_tasks);
+/* Copy of the current task records, [num_tasks][2 + 2*num_cams] 32-bit words. */
+int  clt_get_tasks(clt_handle* h, void* tasks);
+/* Derivatives of the port centres of the current tasks: [num_tasks][2*num_sensors][14] doubles,
+ * rows x0, y0, x1, y1, ...; columns 0..12 by the extrinsic parameters (pXYderiv, GC:2245-2306),
+ * column 13 by the disparity.  Computed for tasks set by clt_set_tasks_geometry. */
+int  clt_get_port_derivatives(clt_handle* h, double* out);
+
 /* ---- execution control and timing (no reference counterpart) --------------------------------- */
 int  clt_set_async(clt_handle* h, int32_t async);     /* 1: exec_* return after launch        */
 int  clt_sync(clt_handle* h);

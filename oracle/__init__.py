@@ -44,6 +44,38 @@ class TexParams(ctypes.Structure):
                 ("port_offsets", (ctypes.c_float * 2) * 16)]
 
 
+class PortGeometry(ctypes.Structure):
+    """clt_port_geometry: the GeometryCorrection parameters of a quad camera (GC:36-83)."""
+    _fields_ = [("num_sensors", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("focal_length", ctypes.c_double), ("pixel_size", ctypes.c_double),
+                ("distortion_radius", ctypes.c_double), ("distortion", ctypes.c_double * 7),
+                ("rxy", (ctypes.c_double * 2) * 4), ("pxy0", (ctypes.c_double * 2) * 4),
+                ("roll_deg", ctypes.c_double * 4), ("extrinsic", ctypes.c_double * 13)]
+
+
+def default_port_geometry(width=2592, height=1936, **kw):
+    """Zero distortion and extrinsics, rXY of GC:78, sensor centres at the frame centre,
+    focal length 4.5 mm, pixel 2.2 um, distortion radius 2.8512 mm (GC:37-39)."""
+    p = PortGeometry()
+    p.num_sensors, p.width, p.height = 4, width, height
+    p.focal_length, p.pixel_size, p.distortion_radius = 4.5, 2.2, 2.8512
+    for i, (x, y) in enumerate(QUAD_PORT_OFFSETS):
+        p.rxy[i][0], p.rxy[i][1] = x, y
+        p.pxy0[i][0], p.pxy0[i][1] = 0.5 * width, 0.5 * height
+    for k, v in kw.items():
+        if k in ("distortion", "extrinsic", "roll_deg"):
+            arr = getattr(p, k)
+            for i, x in enumerate(v):
+                arr[i] = x
+        elif k in ("rxy", "pxy0"):
+            arr = getattr(p, k)
+            for i, (x, y) in enumerate(v):
+                arr[i][0], arr[i][1] = x, y
+        else:
+            setattr(p, k, v)
+    return p
+
+
 # Reference quad pairs, Correlation2d.java:52-58 {first, second, orientation, scale}
 QUAD_PAIRS = [(0, 1, 0, 1), (2, 3, 0, 1), (0, 2, 1, 1), (1, 3, 1, 1), (0, 3, 2, 1), (2, 1, 3, 1)]
 # ImageDtt.java:1648-1652 port_offsets of a quad camera
@@ -124,6 +156,10 @@ def lib(precision=64):
                                      ctypes.POINTER(TexParams), ctypes.POINTER(CorrParams),
                                      ctypes.c_double, _VP, _D, _VP, ctypes.c_int]
     L.orc_combine_textures.argtypes = [ctypes.POINTER(Geom), _VP, ctypes.c_int, _D, ctypes.c_int, _D]
+    L.orc_rot_matrices.argtypes = [ctypes.POINTER(PortGeometry), _D, _VP]
+    L.orc_r_by_rdist.argtypes = [ctypes.POINTER(PortGeometry), ctypes.c_double]
+    L.orc_r_by_rdist.restype = ctypes.c_double
+    L.orc_port_coords.argtypes = [ctypes.POINTER(PortGeometry), ctypes.c_int, _D, _D, _D, _D, _VP]
     _LIBS[precision] = L
     return L
 
@@ -379,3 +415,31 @@ def combine_textures(geom, tasks, tiles):
     lib().orc_combine_textures(ctypes.byref(geom), tasks.ctypes.data_as(_VP), tasks.shape[0], _dp(tiles),
                                planes, _dp(out))
     return out
+
+
+# ------------------------------------------------------------------ port geometry (always FP64)
+def rot_matrices(pg):
+    """Per sensor rotation/zoom matrices [4][3][3] and their derivatives by azimuth, tilt, roll,
+    zoom [4][4][3][3] (GC:1043-1147)."""
+    r = np.zeros((4, 3, 3))
+    d = np.zeros((4, 4, 3, 3))
+    lib(64).orc_rot_matrices(ctypes.byref(pg), _dp(r), _vp(d))
+    return r, d
+
+
+def r_by_rdist(pg, rdist):
+    """Non-distorted / distorted radius ratio at relative distorted radius rdist (GC:2802-2830)."""
+    return lib(64).orc_r_by_rdist(ctypes.byref(pg), float(rdist))
+
+
+def port_coords(pg, px, py, disparity, deriv=False):
+    """Port centres [n][4][2] of tile centres (px, py) at disparity (GC:2159-2314); with deriv also
+    [n][8][14]: d/d 13 extrinsics (pXYderiv) and d/d disparity."""
+    px = np.ascontiguousarray(np.atleast_1d(px), np.float64)
+    py = np.ascontiguousarray(np.broadcast_to(py, px.shape), np.float64)
+    dd = np.ascontiguousarray(np.broadcast_to(disparity, px.shape), np.float64)
+    xy = np.zeros((len(px), 4, 2))
+    dv = np.zeros((len(px), 8, 14)) if deriv else None
+    lib(64).orc_port_coords(ctypes.byref(pg), len(px), _dp(px), _dp(py), _dp(dd), _dp(xy),
+                            _vp(dv) if deriv else None)
+    return (xy, dv) if deriv else xy

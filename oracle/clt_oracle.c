@@ -1135,8 +1135,194 @@ static void tex_task(void* vctx, int t) {
     free(deb);
 }
 
+/* ================================ port geometry (double) ================================ */
+/* CorrVector angles of sensor i (GC:1217-1269): sensor 3 tilt/azimuth/zoom are minus the sum of
+ * the other three; rolls add the factory roll in degrees. */
+static void corr_angles(const clt_port_geometry* p, int i, double* az, double* tl, double* rl, double* zm) {
+    const double* v = p->extrinsic;
+    *tl = (i < 3) ? v[0 + i] : -(v[0] + v[1] + v[2]);
+    *az = (i < 3) ? v[3 + i] : -(v[3] + v[4] + v[5]);
+    *rl = v[6 + i] + p->roll_deg[i] * M_PI / 180.0;
+    *zm = (i < 3) ? v[10 + i] : -(v[10] + v[11] + v[12]);
+}
+
+static void mat3_mul(const double* a, const double* b, double* c) {
+    double t[9];
+    for (int r = 0; r < 3; r++)
+        for (int k = 0; k < 3; k++) t[r * 3 + k] = a[r * 3] * b[k] + a[r * 3 + 1] * b[3 + k] + a[r * 3 + 2] * b[6 + k];
+    memcpy(c, t, sizeof(t));
+}
+
+/* GC:1043-1076 getRotMatrices (rots = a_r * (a_t * a_az)) and GC:1083-1147 getRotDeriveMatrices
+ * (d/d azimuth, tilt, roll, zoom), ROT_AZ_SGN = -1, ROT_TL_SGN = ROT_RL_SGN = 1 (GC:1012-1014). */
+static void rot_matrices(const clt_port_geometry* p, double rots[4][9], double drots[4][4][9]) {
+    for (int i = 0; i < 4; i++) {
+        double az, tl, rl, zm;
+        corr_angles(p, i, &az, &tl, &rl, &zm);
+        const double ca = cos(az), sa = sin(az), ct = cos(tl), st = sin(tl);
+        const double zoom = 1.0 + zm, cr = cos(rl), sr = sin(rl);
+        const double a_az[9] = {ca, 0, -sa, 0, 1, 0, sa, 0, ca};
+        const double a_t[9] = {1, 0, 0, 0, ct, st, 0, -st, ct};
+        const double a_r[9] = {cr * zoom, sr * zoom, 0, -sr * zoom, cr * zoom, 0, 0, 0, 1};
+        const double a_daz[9] = {-sa, 0, -ca, 0, 0, 0, ca, 0, -sa};
+        const double a_dt[9] = {0, 0, 0, 0, -st, ct, 0, -ct, -st};
+        const double a_dr[9] = {-sr * zoom, cr * zoom, 0, -cr * zoom, -sr * zoom, 0, 0, 0, 0};
+        const double a_dz[9] = {cr, sr, 0, -sr, cr, 0, 0, 0, 0};
+        double t[9];
+        mat3_mul(a_t, a_az, t);
+        mat3_mul(a_r, t, rots[i]);
+        if (!drots) continue;
+        double u[9];
+        mat3_mul(a_t, a_daz, u);
+        mat3_mul(a_r, u, drots[i][0]);
+        mat3_mul(a_dt, a_az, u);
+        mat3_mul(a_r, u, drots[i][1]);
+        mat3_mul(a_dr, t, drots[i][2]);
+        mat3_mul(a_dz, t, drots[i][3]);
+    }
+}
+
+/* GC:2707-2756 calcReverseDistortionTable: Newton solution of rD = r * k(r) per step 0.001 up to
+ * 2.0 distortion radii; table of r / rD (NaN past a failure). */
+#define RDIST_STEP 0.001
+#define RDIST_LEN 2001
+static void reverse_distortion_table(const clt_port_geometry* p, double* tab) {
+    const double C = p->distortion[0], B = p->distortion[1], A = p->distortion[2], A5 = p->distortion[3];
+    const double A6 = p->distortion[4], A7 = p->distortion[5], A8 = p->distortion[6];
+    const int use8 = (A8 != 0.0) || (A7 != 0.0) || (A6 != 0.0);
+    const double d = 1.0 - A8 - A7 - A6 - A5 - A - B - C;
+    for (int j = 1; j < RDIST_LEN; j++) tab[j] = NAN;
+    tab[0] = 1.0 / d;
+    double rprev = 0.0, drdr = 1.0;
+    for (int i = 1; i < RDIST_LEN; i++) {
+        const double rdist = RDIST_STEP * i;
+        double r = rprev + RDIST_STEP / drdr;
+        for (int it = 0; it < 1000; it++) {
+            double k;
+            if (use8) {
+                k = (((((((A8)*r + A7) * r + A6) * r + A5) * r + A) * r + B) * r + C) * r + d;
+                drdr = (((((((8 * A8) * r + 7 * A7) * r + 6 * A6) * r + 5 * A5) * r + 4 * A) * r + 3 * B) * r + 2 * C) * r + d;
+            } else {
+                k = (((A5 * r + A) * r + B) * r + C) * r + d;
+                drdr = (((5 * A5 * r + 4 * A) * r + 3 * B) * r + 2 * C) * r + d;
+            }
+            const double rd = r * k;
+            if (drdr < 0.1) return;   /* too much distortion: the rest stays NaN */
+            if (fabs(rd - rdist) < 1e-8) break;
+            r += (rdist - rd) / drdr;
+        }
+        rprev = r;
+        tab[i] = r / rdist;
+    }
+}
+
+/* GC:2802-2830 getRByRDist: linear interpolation in the table, NaN outside. */
+static double r_by_rdist(const double* tab, double rdist) {
+    if (rdist < 0) return NAN;
+    const int index = (int)floor(rdist / RDIST_STEP);
+    if (index >= RDIST_LEN - 1) return NAN;
+    return tab[index] + (tab[index + 1] - tab[index]) * (rdist / RDIST_STEP - index);
+}
+
+/* GC:2159-2314 getPortsCoordinatesAndDerivatives (main camera, no rig offsets) for tile centre
+ * (px, py) and disparity: xy[4][2]; deriv (optional) [8][14]: columns 0..12 the reference's
+ * pXYderiv, column 13 d/d disparity (not in the reference; from the same chain). */
+static void port_coords(const clt_port_geometry* p, const double rots[4][9], const double drots[4][4][9],
+                        const double* tab, double px, double py, double disparity, double* xy, double* deriv) {
+    const double pXcd = px - 0.5 * p->width, pYcd = py - 0.5 * p->height;
+    const double rD = sqrt(pXcd * pXcd + pYcd * pYcd) * 0.001 * p->pixel_size;
+    const double rND2R = r_by_rdist(tab, rD / p->distortion_radius);
+    const double pXc = pXcd * rND2R, pYc = pYcd * rND2R;
+    const double fl_pix = p->focal_length / (0.001 * p->pixel_size);
+    const double ri_scale = 0.001 * p->pixel_size / p->distortion_radius;
+    if (deriv) memset(deriv, 0, sizeof(double) * 8 * 14);
+    for (int i = 0; i < 4; i++) {
+        const double vi[3] = {pXc - disparity * p->rxy[i][0], pYc - disparity * p->rxy[i][1], fl_pix};
+        const double* R = rots[i];
+        const double rvi[3] = {R[0] * vi[0] + R[1] * vi[1] + R[2] * vi[2], R[3] * vi[0] + R[4] * vi[1] + R[5] * vi[2],
+                               R[6] * vi[0] + R[7] * vi[1] + R[8] * vi[2]};
+        const double norm_z = fl_pix / rvi[2];
+        const double pXci = rvi[0] * norm_z, pYci = rvi[1] * norm_z;
+        const double rNDi = sqrt(pXci * pXci + pYci * pYci);
+        const double ri = rNDi * ri_scale;
+        double rD2rND = 1.0, rri = 1.0;
+        for (int j = 0; j < 7; j++) {
+            rri *= ri;
+            rD2rND += p->distortion[j] * (rri - 1.0);
+        }
+        xy[2 * i] = pXci * rD2rND + p->pxy0[i][0];
+        xy[2 * i + 1] = pYci * rD2rND + p->pxy0[i][1];
+        if (!deriv) continue;
+        double drD2rND_dri = 0.0;
+        rri = 1.0;
+        for (int j = 0; j < 7; j++) {
+            drD2rND_dri += p->distortion[j] * (j + 1) * rri;
+            rri *= ri;
+        }
+        /* d(pXci, pYci) by a change dv of rvi: (dv0 * norm_z - pXci * dv2 / rvi2, ...) */
+        double dx[5], dy[5];   /* by azimuth, tilt, roll, zoom, disparity */
+        for (int a = 0; a < 5; a++) {
+            double dv[3];
+            if (a < 4) {
+                const double* D = drots[i][a];
+                for (int r = 0; r < 3; r++) dv[r] = D[r * 3] * vi[0] + D[r * 3 + 1] * vi[1] + D[r * 3 + 2] * vi[2];
+            } else {   /* d vi / d disparity = (-rx, -ry, 0) */
+                for (int r = 0; r < 3; r++) dv[r] = -(R[r * 3] * p->rxy[i][0] + R[r * 3 + 1] * p->rxy[i][1]);
+            }
+            const double dxc = dv[0] * norm_z - pXci * dv[2] / rvi[2];
+            const double dyc = dv[1] * norm_z - pYci * dv[2] / rvi[2];
+            /* the reference drops the radius change for roll (GC:2257-2260): it is 0 */
+            const double dri = (a == 2) ? 0.0 : ri_scale / rNDi * (pXci * dxc + pYci * dyc);
+            const double dk = drD2rND_dri * dri;
+            dx[a] = dxc * rD2rND + pXci * dk;
+            dy[a] = dyc * rD2rND + pYci * dk;
+        }
+        double* rx = deriv + (2 * i) * 14;
+        double* ry = deriv + (2 * i + 1) * 14;
+        if (i < 3) {   /* GC:2285-2302: sensor 3 depends on minus the sum of the others */
+            rx[0 + i] = dx[1]; ry[0 + i] = dy[1];
+            rx[3 + i] = dx[0]; ry[3 + i] = dy[0];
+            rx[10 + i] = dx[3]; ry[10 + i] = dy[3];
+        } else {
+            for (int j = 0; j < 3; j++) {
+                rx[0 + j] = -dx[1]; ry[0 + j] = -dy[1];
+                rx[3 + j] = -dx[0]; ry[3 + j] = -dy[0];
+                rx[10 + j] = -dx[3]; ry[10 + j] = -dy[3];
+            }
+        }
+        rx[6 + i] = dx[2]; ry[6 + i] = dy[2];
+        rx[13] = dx[4]; ry[13] = dy[4];
+    }
+}
+
 /* ================================ exported API ================================ */
 EXPORT int orc_real_size(void) { return (int)sizeof(real); }
+
+EXPORT void orc_rot_matrices(const clt_port_geometry* p, double* rots, double* drots) {
+    double r[4][9], d[4][4][9];
+    rot_matrices(p, r, d);
+    memcpy(rots, r, sizeof(r));
+    if (drots) memcpy(drots, d, sizeof(d));
+}
+
+EXPORT double orc_r_by_rdist(const clt_port_geometry* p, double rdist) {
+    double tab[RDIST_LEN];
+    reverse_distortion_table(p, tab);
+    return r_by_rdist(tab, rdist);
+}
+
+/* n tiles: px, py, disparity [n]; xy [n][8]; deriv [n][8][14] or NULL */
+EXPORT void orc_port_coords(const clt_port_geometry* p, int n, const double* px, const double* py,
+                            const double* disparity, double* xy, double* deriv) {
+    double r[4][9], d[4][4][9];
+    double* tab = malloc(sizeof(double) * RDIST_LEN);
+    rot_matrices(p, r, d);
+    reverse_distortion_table(p, tab);
+    for (int t = 0; t < n; t++)
+        port_coords(p, (const double(*)[9])r, (const double(*)[4][9])d, tab, px[t], py[t], disparity[t], xy + 8 * t,
+                    deriv ? deriv + 8 * 14 * t : NULL);
+    free(tab);
+}
 
 EXPORT void orc_hwindow(double* out8) {
     tables();
