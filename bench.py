@@ -169,6 +169,63 @@ def run_step(tp):
     tp.execImcltRbg(True)
 
 
+def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
+    """Batched form: `frames` quad frames, each with a random 5..100 % of the tiles as tasks (the
+    frames share the synthetic images), one graph launch for convert+correlate2D+textures+imclt of
+    all frames, against the same frames as sequential plain calls (setTasks + the four exec calls,
+    each returning after the GPU finished).  Both include the task upload."""
+    rng = np.random.default_rng(seed)
+    T = len(sc.tasks)
+    subsets = [np.sort(rng.choice(T, int(round(T * rng.uniform(0.05, 1.0))), replace=False)) for _ in range(frames)]
+    total = sum(len(x) for x in subsets)
+    # plain per-frame path
+    tp = make_processor(pk, sc)
+    tp.setAsync(False)
+    best_plain = 1e30
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        for sub in subsets:
+            tp.setTasks(sc.tasks[sub])
+            run_step(tp)
+        best_plain = min(best_plain, time.perf_counter() - t0)
+    tp.setTiming(True)
+    tp.resetTiming()
+    for sub in subsets:
+        tp.setTasks(sc.tasks[sub])
+        run_step(tp)
+    plain_gpu = sum(tp.getTiming(k)[0] for k in ("convert", "corr2d", "textures", "imclt"))
+    tp.close()
+    # batched
+    bt = pk.GPUTileBatch(frames, W, H, C, K)
+    bt.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
+    bt.setCorrSigma(0.8)
+    bt.setTexParams(keep_weights=0)
+    for f in range(frames):
+        bt.setBayerImagesFrame(f, sc.images)
+        bt.setTasksFrame(f, sc.tasks[subsets[f]])
+    tune = bt.autotune()
+    bt.execBatch()   # graph capture
+    bt.sync()
+    best_batch = 1e30
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        for f in range(frames):
+            bt.setTasksFrame(f, sc.tasks[subsets[f]])
+        bt.execBatch()
+        bt.sync()
+        best_batch = min(best_batch, time.perf_counter() - t0)
+    bt.setTiming(True)
+    bt.resetTiming()
+    bt.execBatch()
+    batch_gpu = bt.getTiming("batch")[0]
+    bt.close()
+    return {"frames": frames, "tasks": total, "coverage": "uniform 5..100 % per frame",
+            "batched_tiles_per_s": round(total / best_batch, 1), "plain_tiles_per_s": round(total / best_plain, 1),
+            "batched_ms": round(1e3 * best_batch, 3), "plain_ms": round(1e3 * best_plain, 3),
+            "speedup": round(best_plain / best_batch, 3), "batched_gpu_ms": round(batch_gpu, 3),
+            "plain_gpu_ms": round(plain_gpu, 3), "tune": tune}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +236,7 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MB",
                     help="run one device-to-device copy of MB MiB first (HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-frames comparison")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -240,6 +298,10 @@ def main():
             "frame_alg_MB": round(total_alg / 1e6, 1),
             "frame_frac": round(total_alg / (sum(k["avg_ms"] for k in kernels.values()) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     tp.close()
+    batch = None
+    if not args.no_batch:
+        print("bench: batched frames ...", file=sys.stderr, flush=True)
+        batch = batch_bench(pk, sc)
     if traffic is None:
         roof["traffic_note"] = traffic_info
     else:
@@ -259,7 +321,7 @@ def main():
         "config": {"workload": "quad 2592x1936 Bayer frame, 78408 tiles, dense tasks, disparity 0-20 px; "
                                "convert+correlate2D(6 pairs)+textures(RGBA)+imclt_rbg",
                    "tiles": T, "cameras": C, "colors": K, "pairs": 6},
-        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
+        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu, "batch": batch,
     }
     print(json.dumps(out))
 

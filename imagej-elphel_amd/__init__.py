@@ -6,6 +6,6 @@ The compute path is ``libclt_hip.so`` (hand-written gfx950 HIP kernels); there i
 """
 from . import geometry, synth  # noqa: F401
 from ._abi import CltError, load  # noqa: F401
-from .tile_processor import GPUTileProcessor, TpTask  # noqa: F401
+from .tile_processor import GPUTileBatch, GPUTileProcessor, TpTask  # noqa: F401
 
-__all__ = ["GPUTileProcessor", "TpTask", "CltError", "load", "geometry", "synth"]
+__all__ = ["GPUTileProcessor", "GPUTileBatch", "TpTask", "CltError", "load", "geometry", "synth"]

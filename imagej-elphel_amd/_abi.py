@@ -14,7 +14,7 @@ ERRORS = {-1: "CLT_E_ARG", -2: "CLT_E_STATE", -3: "CLT_E_HIP", -4: "CLT_E_NODEVI
           -5: "CLT_E_NOMEM", -6: "CLT_E_UNSUPPORTED"}
 DISP_LAYERS = 8
 DISP_NAMES = ["int", "cm", "strength", "hor", "hor_strength", "vert", "vert_strength", "cm_weight"]
-TIMING_KERNELS = {"convert": 0, "imclt": 1, "corr2d": 2, "textures": 3, "combine_textures": 4}
+TIMING_KERNELS = {"convert": 0, "imclt": 1, "corr2d": 2, "textures": 3, "combine_textures": 4, "batch": 5}
 
 
 class clt_geom(ctypes.Structure):
@@ -94,6 +94,15 @@ SIGNATURES = {
     "clt_set_tasks_geometry": (_I, [_H, _P, _P, _P, _I]),
     "clt_get_tasks": (_I, [_H, _P]),
     "clt_get_port_derivatives": (_I, [_H, _P]),
+    "clt_create_batch": (_I, [ctypes.POINTER(clt_geom), _I, ctypes.POINTER(_H)]),
+    "clt_set_image_frame": (_I, [_H, _I, _I, _P, _I]),
+    "clt_set_tasks_frame": (_I, [_H, _I, _P, _I]),
+    "clt_exec_batch": (_I, [_H, _I]),
+    "clt_autotune": (_I, [_H]),
+    "clt_get_tune": (_I, [_H, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "clt_get_corr2d_frame": (_I, [_H, _I, _P, _P, _P]),
+    "clt_get_textures_frame": (_I, [_H, _I, _P, _P]),
+    "clt_get_rbg_frame": (_I, [_H, _I, _I, _P]),
 }
 
 _lib = None

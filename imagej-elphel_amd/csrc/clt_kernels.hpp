@@ -54,7 +54,9 @@ constexpr int CONV_UNIT = 336;
 __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ images, const float* __restrict__ kernels,
                                                  const float* __restrict__ koffsets, const int32_t* __restrict__ tasks,
                                                  int task_words, int num_tasks, float* __restrict__ clt,
-                                                 const float* __restrict__ lpf, int lpf_mask, double inv_kstep, GeomDev g) {
+                                                 const float* __restrict__ lpf, int lpf_mask, double inv_kstep,
+                                                 const int32_t* __restrict__ task_frame, long long img_fstride,
+                                                 long long clt_fstride, GeomDev g) {
     __shared__ __attribute__((aligned(16))) float s_buf[12][CONV_UNIT];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int color = threadIdx.x >> 6;                     // wave-uniform
@@ -67,6 +69,7 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     const int32_t* rec = tasks + (size_t)t * task_words;
     active = active && (rec[0] != 0);
     if (!__any(active)) return;   // whole wave idle (no block-level barriers in this kernel)
+    const long long frame = (task_frame && active) ? task_frame[t] : 0;   // batched frames
     const bool bayer = g.colors == 3;
     const int txy = rec[1];
     const int tx = txy & 0xffff, ty = (txy >> 16) & 0xffff;
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     // the frame the parity-preserving clamp of ImageDtt.java:4958-4969 applies.
     float* sb = s_buf[unit_in_block];   // [16][CONV_ROW] floats, reused by the stage H output below
     {
-        const float* img = images + (size_t)cam * g.height * g.width;
+        const float* img = images + frame * img_fstride + (size_t)cam * g.height * g.width;
         const int pc = left + l;
         const int cp = pc & 1;
         // keep pixel (row parity rp, column parity cp) of this colour
@@ -245,11 +248,39 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     }
     if (active) {
         const size_t tile = (size_t)ty * g.tiles_x + tx;
-        float* o = clt + (((size_t)cam * g.tiles_x * g.tiles_y + tile) * g.colors + color) * 256 + kh * 8;
+        float* o = clt + frame * clt_fstride + (((size_t)cam * g.tiles_x * g.tiles_y + tile) * g.colors + color) * 256 + kh * 8;
         *reinterpret_cast<float4*>(o + hs * 64) = make_float4(a[0], a[1], a[2], a[3]);
         *reinterpret_cast<float4*>(o + hs * 64 + 4) = make_float4(a[4], a[5], a[6], a[7]);
         *reinterpret_cast<float4*>(o + (hs | 2) * 64) = make_float4(b[0], b[1], b[2], b[3]);
         *reinterpret_cast<float4*>(o + (hs | 2) * 64 + 4) = make_float4(b[4], b[5], b[6], b[7]);
+    }
+}
+
+// ====================================================================================
+// Batched frames: the task lists of the frames (staged frame-major, max_tasks records apart) are
+// packed into one list with the frame of each task, and the per-frame tile maps are rebuilt.
+// ====================================================================================
+__global__ __launch_bounds__(256) void k_fill_i32(int32_t* __restrict__ p, long long n, int32_t v) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_batch_assemble(const int32_t* __restrict__ staged, const int32_t* __restrict__ counts,
+                                                        const int32_t* __restrict__ starts, int num_frames, int max_tasks,
+                                                        int task_words, int32_t* __restrict__ tasks,
+                                                        int32_t* __restrict__ task_frame, int32_t* __restrict__ tile_map,
+                                                        GeomDev g) {
+    const long long slot = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (slot >= (long long)num_frames * max_tasks) return;
+    const int f = (int)(slot / max_tasks), i = (int)(slot % max_tasks);
+    if (i >= counts[f]) return;
+    const int dst = starts[f] + i;
+    const int32_t* src = staged + slot * task_words;
+    int32_t* d = tasks + (size_t)dst * task_words;
+    for (int w = 0; w < task_words; w++) d[w] = src[w];
+    task_frame[dst] = f;
+    if (src[0] != 0) {
+        const int tx = src[1] & 0xffff, ty = (src[1] >> 16) & 0xffff;
+        tile_map[(size_t)f * g.tiles_x * g.tiles_y + (size_t)ty * g.tiles_x + tx] = dst;
     }
 }
 
@@ -363,29 +394,47 @@ __global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restr
 // camera/colour is (H+8) x (W+8), uncropped (GPUTileProcessor.java:599-626).
 // ====================================================================================
 constexpr int IMCLT_S = 15;   // output blocks per strip (16 tiles incl. the left halo)
-constexpr int IMCLT_R = 16;   // output block rows per segment
 
+// grid: (strips, segments of seg_rows block rows, frames * cams * colours).  A segment without any
+// task tile only writes its zeros.
 __global__ __launch_bounds__(256) void k_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tile_map,
-                                               const float* __restrict__ lpf, float* __restrict__ rbg, GeomDev g) {
+                                               const float* __restrict__ lpf, float* __restrict__ rbg, int seg_rows,
+                                               long long clt_fstride, GeomDev g) {
     // per unit: IMCLT scratch, then (after the unit's last scratch read) its 16x16 tile, row stride 17
     __shared__ __attribute__((aligned(16))) float s_scr[16][IMCLT_SCR_UNIT];
     __shared__ float s_carry[8][IMCLT_S * 8];
     static_assert(16 * 17 <= IMCLT_SCR_UNIT, "tile must fit the unit scratch");
     auto s_tile = [&](int u, int r, int c) -> float& { return s_scr[u][r * 17 + c]; };
-    const int plane = blockIdx.z;               // cam * colors + color
+    const int planes = g.cams * g.colors;
+    const int frame = blockIdx.z / planes, plane = blockIdx.z % planes;   // plane = cam * colors + color
     const int cam = plane / g.colors, color = plane % g.colors;
-    const int bx0 = blockIdx.x * IMCLT_S, by0 = blockIdx.y * IMCLT_R;
+    const int bx0 = blockIdx.x * IMCLT_S, by0 = blockIdx.y * seg_rows;
     const int out_w = g.width + 8, out_h = g.height + 8;
-    const int nby = min(IMCLT_R, g.tiles_y + 1 - by0);
+    const int nby = min(seg_rows, g.tiles_y + 1 - by0);
     const int unit = threadIdx.x >> 4, l = threadIdx.x & 15;
     const int hs = l >> 3, kh = l & 7;
     const int tx = bx0 - 1 + unit;
     const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+    tile_map += (size_t)frame * ntiles;
+    clt += frame * clt_fstride;
+    float* out = rbg + ((size_t)frame * planes + plane) * out_w * out_h;
+    // any task tile in the (16 x (nby + 1)) tiles this segment reads?
+    bool any = false;
+    for (int k = threadIdx.x; k < 16 * (nby + 1); k += 256) {
+        const int ux = bx0 - 1 + (k & 15), uy = by0 - 1 + (k >> 4);
+        if (ux >= 0 && ux < g.tiles_x && uy >= 0 && uy < g.tiles_y && tile_map[(size_t)uy * g.tiles_x + ux] >= 0) any = true;
+    }
+    if (!__syncthreads_or(any)) {
+        for (int k = threadIdx.x; k < nby * 8 * IMCLT_S * 8; k += 256) {
+            const int row = by0 * 8 + k / (IMCLT_S * 8), x = bx0 * 8 + k % (IMCLT_S * 8);
+            if (x < out_w) out[(size_t)row * out_w + x] = 0.f;
+        }
+        return;
+    }
     float lp[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) lp[k] = 0.25f * (lpf ? lpf[kh * 8 + k] : 1.f);
     for (int i = threadIdx.x; i < 8 * IMCLT_S * 8; i += 256) (&s_carry[0][0])[i] = 0.f;
-    float* out = rbg + (size_t)plane * out_w * out_h;
     for (int ty = by0 - 1; ty < by0 + nby; ty++) {
         bool valid = (ty >= 0) && (ty < g.tiles_y) && (tx >= 0) && (tx < g.tiles_x) &&
                      (tile_map[(size_t)(ty < 0 ? 0 : ty) * g.tiles_x + (tx < 0 ? 0 : tx)] >= 0);
@@ -516,7 +565,8 @@ template <bool QUAD>
 __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                 int task_words, int num_tasks, const float* __restrict__ lpf,
                                                 const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
-                                                float* __restrict__ combo_out, float* __restrict__ disp_out, GeomDev g) {
+                                                float* __restrict__ combo_out, float* __restrict__ disp_out,
+                                                const int32_t* __restrict__ task_frame, long long clt_fstride, GeomDev g) {
     extern __shared__ __attribute__((aligned(16))) float s_dyn[];
     const CorrArgs& A = *args_p;
     const int K = QUAD ? 3 : g.colors;
@@ -549,6 +599,7 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
     const int txy = rec[1];
     const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
     const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
+    if (task_frame) clt += task_frame[t] * clt_fstride;   // batched frames
     // quad: the whole tile (4 cameras x 3 colours x 4 modes) for this lane's frequency bin
     float D[QUAD ? 4 : 1][3][4];
     if (QUAD) {
@@ -891,11 +942,13 @@ __device__ __forceinline__ int argmax16(float v, int idx) {
 __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                      int task_words, int num_tasks, const float* __restrict__ lpf,
                                                      const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
-                                                     float* __restrict__ combo_out, float* __restrict__ disp_out, GeomDev g) {
+                                                     float* __restrict__ combo_out, float* __restrict__ disp_out,
+                                                     const int32_t* __restrict__ task_frame, long long clt_fstride,
+                                                     GeomDev g) {
     extern __shared__ __attribute__((aligned(16))) float s_dyn[];
     const CorrArgs& A = *args_p;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wave;   // one tile per wavefront
     if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
     float* S = s_dyn + wave * CQ_WAVE_FLOATS;         // [4][64] transform staging
     float* s_corr = S + 256;                          // [6][225]
@@ -917,6 +970,7 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
     const int txy = rec[1];
     const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
     const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
+    if (task_frame) clt += task_frame[t] * clt_fstride;   // batched frames
     float D[4][3][4];
 #pragma unroll
     for (int c = 0; c < 4; c++)
@@ -1345,7 +1399,8 @@ template <int NC>
 __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                   int task_words, int num_tasks, const float* __restrict__ lpf,
                                                   const float* __restrict__ extra_disp, const TexArgs* __restrict__ args_p,
-                                                  float* __restrict__ tiles_out, float* __restrict__ stats_out, GeomDev g) {
+                                                  float* __restrict__ tiles_out, float* __restrict__ stats_out,
+                                                  const int32_t* __restrict__ task_frame, long long clt_fstride, GeomDev g) {
     constexpr int NU = NC * 3;   // camera/colour units
     __shared__ __attribute__((aligned(16))) float s_scr[16][IMCLT_SCR_UNIT];
     __shared__ float s_nw[NU][18 * 18];   // window-free tiles with a zero border for the 3x3 debayer
@@ -1367,6 +1422,7 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
     const int txy = rec[1];
     const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
     const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+    if (task_frame) clt += task_frame[t] * clt_fstride;   // batched frames
     float extra = extra_disp ? extra_disp[t] : 0.f;
     if (isnan(extra)) extra = 0.f;
     const bool do_shift = (extra != 0.f) && !((flags >> 24) & 1);

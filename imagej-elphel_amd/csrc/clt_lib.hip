@@ -4,6 +4,7 @@
 // buffers owned by a handle, H2D/D2H copies, kernel launches on the handle's stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -100,7 +101,7 @@ static void half_flat_top(int n, double hwidth, double blur, bool normalize, boo
 }
 
 // ------------------------------------------------------------------------ handle
-enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_NUM };
+enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_NUM };
 
 struct clt_handle {
     clt_geom g{};
@@ -110,6 +111,9 @@ struct clt_handle {
     std::string err;
     bool async = false, timing = false;
     int tiles_x = 0, tiles_y = 0, num_tiles = 0, task_words = 0, max_tasks = 0;
+    int num_frames = 1;            // frames (camera sets) held at once; 1 for the per-frame handle
+    int cap_tasks = 0;             // num_frames * max_tasks
+    long long img_fstride = 0, clt_fstride = 0;   // floats per frame of d_images / d_clt
     // device buffers
     float* d_images = nullptr;     // [C][H][W]
     float* d_kernels = nullptr;    // [C][KY][KX][K][4][h][v]
@@ -134,6 +138,22 @@ struct clt_handle {
     float* d_gdisp = nullptr;      // [max_tasks]
     double* d_gderiv = nullptr;    // [max_tasks][8][14]
     bool port_geom_set = false, tasks_from_geom = false;
+    // batched frames (clt_create_batch): host task lists per frame, flat device list with frames
+    int32_t* d_task_frame = nullptr;               // [cap_tasks]
+    int32_t* d_staged = nullptr;                   // [frames][max_tasks][task_words] as set per frame
+    int32_t* d_fcounts = nullptr;                  // [frames] tasks per frame, then [frames] first task
+    std::vector<int> frame_count;                  // tasks of each frame
+    std::vector<int> frame_start;                  // first task of each frame in the flat list, + total
+    int graph_tasks = -1;                          // task count the graph was captured for
+    bool batch_dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_stages = 0;
+    // launch shapes chosen by clt_autotune
+    int tune_corr_tpb = 4, tune_imclt_rows = 16;
+    bool tuned = false;
+    bool corr_quad = false;        // correlate2D runs the quad kernel
+    int corr_num_strip = 0;
     CorrArgs* d_corr_args = nullptr;
     TexArgs* d_tex_args = nullptr;
     size_t corr_cap = 0, combo_cap = 0, tex_cap = 0, tex_img_cap = 0;
@@ -205,10 +225,56 @@ static void tend(clt_handle* h, int which) {
     h->ev_used[which] = u + 1;
 }
 
+// ------------------------------------------------------------------------ kernel launches
+// n tasks of the device task list; task_frame: frame of each task (batches) or null (one frame).
+static void launch_convert(clt_handle* h, int n, const int32_t* task_frame, int lpf_mask) {
+    const long units = (long)n * h->g.num_cams;   // (task, camera); 4 per block
+    const int blocks = (int)((units + 3) / 4);
+    const double inv_kstep = 1.0 / h->g.kernel_step;   // exact: checked power of two
+    hipLaunchKernelGGL(k_convert, dim3(blocks), dim3(64 * h->g.num_colors), 0, h->stream, h->d_images,
+                       h->use_kernels ? h->d_kernels : nullptr, h->use_kernels ? h->d_koffs : nullptr, h->d_tasks,
+                       h->task_words, n, h->d_clt, lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, inv_kstep, task_frame,
+                       h->img_fstride, h->clt_fstride, h->gd);
+}
+
+static void launch_imclt(clt_handle* h, int frames, int apply_lpf) {
+    const int rows = h->tune_imclt_rows;
+    dim3 grid((h->tiles_x + 1 + IMCLT_S - 1) / IMCLT_S, (h->tiles_y + 1 + rows - 1) / rows,
+              frames * h->g.num_cams * h->g.num_colors);
+    hipLaunchKernelGGL(k_imclt, grid, dim3(256), 0, h->stream, h->d_clt, h->d_tile_map, apply_lpf ? h->d_lpf_corr : nullptr,
+                       h->d_rbg, rows, h->clt_fstride, h->gd);
+}
+
+static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
+    if (h->corr_quad) {
+        const int tpb = h->tune_corr_tpb;
+        hipLaunchKernelGGL(k_corr2d_quad, dim3((n + tpb - 1) / tpb), dim3(64 * tpb), (size_t)tpb * CQ_WAVE_FLOATS * 4,
+                           h->stream, h->d_clt, h->d_tasks, h->task_words, n, h->d_lpf_corr, h->d_corr_args, h->d_corr,
+                           h->d_combo, h->d_disp, task_frame, h->clt_fstride, h->gd);
+    } else {
+        const size_t lds = (size_t)4 * ((corr_wave_floats(h->corr_num_strip) + 3) & ~3) * 4;
+        hipLaunchKernelGGL(k_corr2d<false>, dim3((n + 3) / 4), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
+                           n, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, task_frame, h->clt_fstride,
+                           h->gd);
+    }
+}
+
 template <int NC>
-static void launch_tex(clt_handle* h, int blocks, const float* extra) {
-    hipLaunchKernelGGL(k_textures<NC>, dim3(blocks), dim3(256), 0, h->stream, h->d_clt, h->d_tasks, h->task_words, h->num_tasks,
-                       h->d_lpf_corr, extra, h->d_tex_args, h->d_tex, h->d_tex_stats, h->gd);
+static void launch_tex_nc(clt_handle* h, int n, const int32_t* task_frame, const float* extra) {
+    hipLaunchKernelGGL(k_textures<NC>, dim3(n), dim3(256), 0, h->stream, h->d_clt, h->d_tasks, h->task_words, n,
+                       h->d_lpf_corr, extra, h->d_tex_args, h->d_tex, h->d_tex_stats, task_frame, h->clt_fstride, h->gd);
+}
+
+static void launch_tex(clt_handle* h, int n, const int32_t* task_frame, const float* extra) {
+    switch (h->g.num_cams) {
+    case 2: launch_tex_nc<2>(h, n, task_frame, extra); break;
+    case 3: launch_tex_nc<3>(h, n, task_frame, extra); break;
+    case 4: launch_tex_nc<4>(h, n, task_frame, extra); break;
+    case 5: launch_tex_nc<5>(h, n, task_frame, extra); break;
+    case 6: launch_tex_nc<6>(h, n, task_frame, extra); break;
+    case 7: launch_tex_nc<7>(h, n, task_frame, extra); break;
+    default: launch_tex_nc<8>(h, n, task_frame, extra); break;
+    }
 }
 
 extern "C" {
@@ -264,8 +330,8 @@ int clt_lpf_coeffs(double sigma, float out[64]) {
 
 const char* clt_last_error(const clt_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
-int clt_create(const clt_geom* geom, clt_handle** out) {
-    if (!geom || !out) return CLT_E_ARG;
+static int create_handle(const clt_geom* geom, int frames, clt_handle** out) {
+    if (!geom || !out || frames < 1) return CLT_E_ARG;
     *out = nullptr;
     const clt_geom& g = *geom;
     if (g.width <= 16 || g.height <= 16 || (g.width & 7) || (g.height & 7) || g.num_cams < 1 ||
@@ -283,6 +349,12 @@ int clt_create(const clt_geom* geom, clt_handle** out) {
     h->num_tiles = h->tiles_x * h->tiles_y;
     h->task_words = 2 + 2 * g.num_cams;
     h->max_tasks = g.max_tasks > 0 ? g.max_tasks : h->num_tiles;
+    h->num_frames = frames;
+    h->cap_tasks = frames * h->max_tasks;
+    h->img_fstride = (long long)g.num_cams * g.width * g.height;
+    h->clt_fstride = (long long)g.num_cams * h->num_tiles * g.num_colors * 256;
+    h->frame_count.assign(frames, 0);
+    h->frame_start.assign(frames + 1, 0);
     h->gd = GeomDev{g.width, g.height, g.num_cams, g.num_colors, g.red_row, g.red_col,
                     g.kernels_hor, g.kernels_vert, g.kernel_step, h->tiles_x, h->tiles_y};
     clt_default_corr_params(&h->cp);
@@ -300,24 +372,27 @@ int clt_create(const clt_geom* geom, clt_handle** out) {
     }
     Tables tab;
     make_tables(&tab);
-    const size_t C = g.num_cams, K = g.num_colors;
-    const size_t img = (size_t)g.width * g.height;
-    if ((rc = dalloc(h, (void**)&h->d_images, C * img * 4)) || (rc = dalloc(h, (void**)&h->d_tasks, (size_t)h->max_tasks * h->task_words * 4)) ||
-        (rc = dalloc(h, (void**)&h->d_tile_map, (size_t)h->num_tiles * 4)) ||
-        (rc = dalloc(h, (void**)&h->d_clt, C * h->num_tiles * K * 256 * 4)) || (rc = dalloc(h, (void**)&h->d_lpf_rbg, 3 * 64 * 4)) ||
+    const size_t F = frames;
+    if ((rc = dalloc(h, (void**)&h->d_images, F * h->img_fstride * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_tasks, (size_t)h->cap_tasks * h->task_words * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_tile_map, F * h->num_tiles * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_clt, F * h->clt_fstride * 4)) || (rc = dalloc(h, (void**)&h->d_lpf_rbg, 3 * 64 * 4)) ||
         (rc = dalloc(h, (void**)&h->d_lpf_corr, 64 * 4)) || (rc = dalloc(h, (void**)&h->d_corr_args, sizeof(CorrArgs))) ||
-        (rc = dalloc(h, (void**)&h->d_tex_args, sizeof(TexArgs))) || (rc = dalloc(h, (void**)&h->d_extra, (size_t)h->max_tasks * 4))) {
+        (rc = dalloc(h, (void**)&h->d_tex_args, sizeof(TexArgs))) || (rc = dalloc(h, (void**)&h->d_extra, (size_t)h->cap_tasks * 4)) ||
+        (frames > 1 && ((rc = dalloc(h, (void**)&h->d_task_frame, (size_t)h->cap_tasks * 4)) ||
+                        (rc = dalloc(h, (void**)&h->d_staged, (size_t)h->cap_tasks * h->task_words * 4)) ||
+                        (rc = dalloc(h, (void**)&h->d_fcounts, (size_t)2 * frames * 4))))) {
         std::string e = h->err;
         clt_destroy(h);
         return rc;
     }
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &tab, sizeof(tab)) != hipSuccess ||
-        hipMemset(h->d_images, 0, C * img * 4) != hipSuccess) {
+        hipMemset(h->d_images, 0, F * h->img_fstride * 4) != hipSuccess) {
         clt_destroy(h);
         return CLT_E_HIP;
     }
-    h->h_tile_map.assign(h->num_tiles, -1);
-    if (hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    h->h_tile_map.assign(F * h->num_tiles, -1);
+    if (hipMemcpy(h->d_tile_map, h->h_tile_map.data(), F * h->num_tiles * 4, hipMemcpyHostToDevice) != hipSuccess) {
         clt_destroy(h);
         return CLT_E_HIP;
     }
@@ -328,13 +403,21 @@ int clt_create(const clt_geom* geom, clt_handle** out) {
     return CLT_OK;
 }
 
+int clt_create(const clt_geom* geom, clt_handle** out) { return create_handle(geom, 1, out); }
+
+int clt_create_batch(const clt_geom* geom, int32_t num_frames, clt_handle** out) {
+    return create_handle(geom, num_frames, out);
+}
+
 int clt_destroy(clt_handle* h) {
     if (!h) return CLT_E_ARG;
     void* bufs[] = {h->d_images, h->d_kernels, h->d_koffs, h->d_tasks, h->d_tile_map, h->d_clt, h->d_lpf_rbg,
                     h->d_lpf_corr, h->d_rbg, h->d_corr, h->d_combo, h->d_disp, h->d_tex, h->d_tex_stats,
                     h->d_tex_img, h->d_extra, h->d_corr_args, h->d_tex_args, h->d_port_geom, h->d_rtab,
-                    h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv};
+                    h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv, h->d_task_frame, h->d_staged, h->d_fcounts};
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     for (int i = 0; i < T_NUM; i++) {
@@ -392,6 +475,7 @@ int clt_set_image(clt_handle* h, int32_t cam, const float* image, int32_t stride
 
 int clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks) {
     if (!h) return CLT_E_ARG;
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_set_tasks: a batch handle takes clt_set_tasks_frame");
     if (num_tasks < 0 || num_tasks > h->max_tasks || (num_tasks > 0 && !tasks))
         return fail(h, CLT_E_ARG, "clt_set_tasks: %d tasks (capacity %d)", num_tasks, h->max_tasks);
     const int32_t* t = (const int32_t*)tasks;
@@ -406,6 +490,7 @@ int clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks) {
     }
     if (num_tasks > 0) HIPCHK(h, hipMemcpy(h->d_tasks, tasks, (size_t)num_tasks * h->task_words * 4, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice));
+    h->batch_dirty = true;   // the device task list now holds frame 0 only
     h->num_tasks = num_tasks;
     h->converted = h->corr_done = h->tex_done = h->combined = false;
     h->tasks_from_geom = false;
@@ -563,6 +648,7 @@ int clt_set_tasks_geometry(clt_handle* h, const int32_t* task, const int32_t* tx
     h->num_tasks = num_tasks;
     h->converted = h->corr_done = h->tex_done = h->combined = false;
     h->tasks_from_geom = true;
+    h->batch_dirty = true;
     if ((rc = launch_port_centres(h, nullptr))) return rc;
     return finish(h);
 }
@@ -645,13 +731,9 @@ int clt_exec_convert(clt_handle* h, int32_t lpf_mask) {
     for (int c = 0; c < h->g.num_cams; c++)
         if (!h->images_set[c]) return fail(h, CLT_E_STATE, "clt_exec_convert: image of camera %d not set", c);
     if (h->num_tasks <= 0) return fail(h, CLT_E_STATE, "clt_exec_convert: no tasks");
-    const long units = (long)h->num_tasks * h->g.num_cams;   // (task, camera); 4 per block
-    const int blocks = (int)((units + 3) / 4);
-    const double inv_kstep = 1.0 / h->g.kernel_step;            // exact: checked power of two
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_convert: a batch handle runs clt_exec_batch");
     tbegin(h, T_CONVERT);
-    hipLaunchKernelGGL(k_convert, dim3(blocks), dim3(64 * h->g.num_colors), 0, h->stream, h->d_images,
-                       h->use_kernels ? h->d_kernels : nullptr, h->use_kernels ? h->d_koffs : nullptr, h->d_tasks,
-                       h->task_words, h->num_tasks, h->d_clt, lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, inv_kstep, h->gd);
+    launch_convert(h, h->num_tasks, nullptr, lpf_mask);
     tend(h, T_CONVERT);
     HIPCHK(h, hipGetLastError());
     h->converted = true;
@@ -661,14 +743,15 @@ int clt_exec_convert(clt_handle* h, int32_t lpf_mask) {
 
 int clt_exec_imclt(clt_handle* h, int32_t apply_lpf) {
     if (!h) return CLT_E_ARG;
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_imclt: a batch handle runs clt_exec_batch");
     if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_imclt: no converted tiles");
     const clt_geom& g = h->g;
     int rc;
-    if ((rc = dalloc(h, (void**)&h->d_rbg, (size_t)g.num_cams * g.num_colors * (g.width + 8) * (g.height + 8) * 4))) return rc;
-    dim3 grid((h->tiles_x + 1 + IMCLT_S - 1) / IMCLT_S, (h->tiles_y + 1 + IMCLT_R - 1) / IMCLT_R, g.num_cams * g.num_colors);
+    if ((rc = dalloc(h, (void**)&h->d_rbg,
+                     (size_t)h->num_frames * g.num_cams * g.num_colors * (g.width + 8) * (g.height + 8) * 4)))
+        return rc;
     tbegin(h, T_IMCLT);
-    hipLaunchKernelGGL(k_imclt, grid, dim3(256), 0, h->stream, h->d_clt, h->d_tile_map, apply_lpf ? h->d_lpf_corr : nullptr,
-                       h->d_rbg, h->gd);
+    launch_imclt(h, 1, apply_lpf);
     tend(h, T_IMCLT);
     HIPCHK(h, hipGetLastError());
     return finish(h);
@@ -686,16 +769,15 @@ static int ensure_capacity(clt_handle* h, float** p, size_t* cap, size_t floats)
     return rc;
 }
 
-int clt_exec_corr2d(clt_handle* h) {
-    if (!h) return CLT_E_ARG;
-    if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_corr2d: no converted tiles");
+// Correlation arguments and output buffers for the current pairs/parameters (CorrArgs upload).
+static int prepare_corr(clt_handle* h) {
     if (h->pairs.empty()) return fail(h, CLT_E_STATE, "clt_exec_corr2d: no correlation pairs set");
     const clt_geom& g = h->g;
     const int P = (int)h->pairs.size(), hw = h->cp.strip_height;
     int rc;
-    if ((rc = ensure_capacity(h, &h->d_corr, &h->corr_cap, (size_t)h->max_tasks * P * CLT_CORR_LEN)) ||
-        (rc = ensure_capacity(h, &h->d_combo, &h->combo_cap, (size_t)h->max_tasks * 16 * 15)) ||
-        (rc = dalloc(h, (void**)&h->d_disp, (size_t)h->max_tasks * CLT_DISP_LAYERS * 4)))
+    if ((rc = ensure_capacity(h, &h->d_corr, &h->corr_cap, (size_t)h->cap_tasks * P * CLT_CORR_LEN)) ||
+        (rc = ensure_capacity(h, &h->d_combo, &h->combo_cap, (size_t)h->cap_tasks * 16 * 15)) ||
+        (rc = dalloc(h, (void**)&h->d_disp, (size_t)h->cap_tasks * CLT_DISP_LAYERS * 4)))
         return rc;
     CorrArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -744,9 +826,6 @@ int clt_exec_corr2d(clt_handle* h) {
         a.notch[i] = (float)(f * std::sin(M_PI * (i + 1.0) / 16));
     }
     HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
-    const size_t lds = (size_t)4 * ((corr_wave_floats(a.num_strip) + 3) & ~3) * 4;
-    const int blocks = (h->num_tasks + 3) / 4;
-    tbegin(h, T_CORR);
     // the reference quad configuration runs the specialised kernel (pairs as compile-time constants)
     bool quad = g.num_cams == 4 && g.num_colors == 3 && P == 6;
     if (quad) {
@@ -756,32 +835,36 @@ int clt_exec_corr2d(clt_handle* h) {
             quad = quad && h->pairs[i].cam1 == dq[i].cam1 && h->pairs[i].cam2 == dq[i].cam2 &&
                    h->pairs[i].dir == dq[i].dir && h->pairs[i].scale == 1;
     }
-    if (quad)
-        hipLaunchKernelGGL(k_corr2d_quad, dim3(blocks), dim3(256), (size_t)4 * CQ_WAVE_FLOATS * 4, h->stream, h->d_clt,
-                           h->d_tasks, h->task_words, h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo,
-                           h->d_disp, h->gd);
-    else
-        hipLaunchKernelGGL(k_corr2d<false>, dim3(blocks), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
-                           h->num_tasks, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, h->gd);
+    h->corr_quad = quad;
+    h->corr_num_strip = a.num_strip;
+    h->corr_pairs_at_exec = P;
+    h->corr_strip_h_at_exec = a.num_strip ? hw : 0;
+    return CLT_OK;
+}
+
+int clt_exec_corr2d(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_corr2d: a batch handle runs clt_exec_batch");
+    if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_corr2d: no converted tiles");
+    int rc;
+    if ((rc = prepare_corr(h))) return rc;
+    tbegin(h, T_CORR);
+    launch_corr(h, h->num_tasks, nullptr);
     tend(h, T_CORR);
     HIPCHK(h, hipGetLastError());
     h->corr_done = true;
-    h->corr_pairs_at_exec = P;
-    h->corr_strip_h_at_exec = a.num_strip ? hw : 0;
-    int rc2 = finish(h);
-    return rc2;
+    return finish(h);
 }
 
-int clt_exec_textures(clt_handle* h, const float* extra_disparity) {
-    if (!h) return CLT_E_ARG;
+// Texture arguments and output buffers for the current parameters (TexArgs upload).
+static int prepare_tex(clt_handle* h) {
     const clt_geom& g = h->g;
-    if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_textures: no converted tiles");
     if (g.num_colors != 3 || g.num_cams < 2 || g.num_cams > 8)
         return fail(h, CLT_E_UNSUPPORTED, "clt_exec_textures: needs a Bayer frame of 2..8 cameras");
     const int planes = h->tp.keep_weights ? 4 + g.num_cams + 4 : 4;
     int rc;
-    if ((rc = ensure_capacity(h, &h->d_tex, &h->tex_cap, (size_t)h->max_tasks * planes * 256)) ||
-        (rc = dalloc(h, (void**)&h->d_tex_stats, (size_t)h->max_tasks * 4 * CLT_MAX_CAMS * 4)))
+    if ((rc = ensure_capacity(h, &h->d_tex, &h->tex_cap, (size_t)h->cap_tasks * planes * 256)) ||
+        (rc = dalloc(h, (void**)&h->d_tex_stats, (size_t)h->cap_tasks * 4 * CLT_MAX_CAMS * 4)))
         return rc;
     TexArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -805,31 +888,33 @@ int clt_exec_textures(clt_handle* h, const float* extra_disparity) {
     a.planes = planes;
     std::memcpy(a.port_offsets, h->tp.port_offsets, sizeof(a.port_offsets));
     HIPCHK(h, hipMemcpyAsync(h->d_tex_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    h->tex_planes_at_exec = planes;
+    return CLT_OK;
+}
+
+int clt_exec_textures(clt_handle* h, const float* extra_disparity) {
+    if (!h) return CLT_E_ARG;
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_textures: a batch handle runs clt_exec_batch");
+    if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_textures: no converted tiles");
+    int rc;
+    if ((rc = prepare_tex(h))) return rc;
     const float* extra = nullptr;
     if (extra_disparity) {
         HIPCHK(h, hipMemcpyAsync(h->d_extra, extra_disparity, (size_t)h->num_tasks * 4, hipMemcpyHostToDevice, h->stream));
         extra = h->d_extra;
     }
     tbegin(h, T_TEX);
-    switch (g.num_cams) {
-    case 2: launch_tex<2>(h, h->num_tasks, extra); break;
-    case 3: launch_tex<3>(h, h->num_tasks, extra); break;
-    case 4: launch_tex<4>(h, h->num_tasks, extra); break;
-    case 5: launch_tex<5>(h, h->num_tasks, extra); break;
-    case 6: launch_tex<6>(h, h->num_tasks, extra); break;
-    case 7: launch_tex<7>(h, h->num_tasks, extra); break;
-    default: launch_tex<8>(h, h->num_tasks, extra); break;
-    }
+    launch_tex(h, h->num_tasks, nullptr, extra);
     tend(h, T_TEX);
     HIPCHK(h, hipGetLastError());
     h->tex_done = true;
-    h->tex_planes_at_exec = planes;
     h->combined = false;
     return finish(h);
 }
 
 int clt_exec_combine_textures(clt_handle* h) {
     if (!h) return CLT_E_ARG;
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_combine_textures: not available for a batch");
     if (!h->tex_done) return fail(h, CLT_E_STATE, "clt_exec_combine_textures: no textures");
     const clt_geom& g = h->g;
     const int planes = h->tex_planes_at_exec;
@@ -896,6 +981,236 @@ int clt_get_combined_textures(clt_handle* h, float* out) {
     if (!h->combined) return fail(h, CLT_E_STATE, "clt_get_combined_textures: combine not executed");
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(out, h->d_tex_img, (size_t)h->tex_planes_at_exec * h->g.width * h->g.height * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+// ------------------------------------------------------------------------ batched frames
+int clt_set_image_frame(clt_handle* h, int32_t frame, int32_t cam, const float* image, int32_t stride) {
+    if (!h) return CLT_E_ARG;
+    const clt_geom& g = h->g;
+    if (frame < 0 || frame >= h->num_frames || cam < 0 || cam >= g.num_cams || !image || stride < g.width)
+        return fail(h, CLT_E_ARG, "clt_set_image_frame: bad frame %d / camera %d, null image or stride %d", frame, cam, stride);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy2D(h->d_images + frame * h->img_fstride + (size_t)cam * g.width * g.height, (size_t)g.width * 4, image,
+                          (size_t)stride * 4, (size_t)g.width * 4, g.height, hipMemcpyHostToDevice));
+    if (frame == 0) h->images_set[cam] = true;
+    return CLT_OK;
+}
+
+int clt_set_tasks_frame(clt_handle* h, int32_t frame, const void* tasks, int32_t num_tasks) {
+    if (!h) return CLT_E_ARG;
+    if (frame < 0 || frame >= h->num_frames || num_tasks < 0 || num_tasks > h->max_tasks || (num_tasks > 0 && !tasks))
+        return fail(h, CLT_E_ARG, "clt_set_tasks_frame: frame %d, %d tasks (capacity %d)", frame, num_tasks, h->max_tasks);
+    const int32_t* t = (const int32_t*)tasks;
+    for (int i = 0; i < num_tasks; i++) {
+        const int32_t* r = t + (size_t)i * h->task_words;
+        const int tx = r[1] & 0xffff, ty = (r[1] >> 16) & 0xffff;
+        if (tx >= h->tiles_x || ty >= h->tiles_y)
+            return fail(h, CLT_E_ARG, "clt_set_tasks_frame: task %d tile (%d,%d) outside %dx%d", i, tx, ty, h->tiles_x, h->tiles_y);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));   // the staging area may still be read
+    if (num_tasks > 0)
+        HIPCHK(h, hipMemcpy(h->d_staged + (size_t)frame * h->max_tasks * h->task_words, tasks,
+                            (size_t)num_tasks * h->task_words * 4, hipMemcpyHostToDevice));
+    h->frame_count[frame] = num_tasks;
+    h->batch_dirty = true;
+    return CLT_OK;
+}
+
+// Flat task list of all frames, the frame of each task and the per-frame tile maps, packed on the
+// device from the staged per-frame lists.
+static int assemble_batch(clt_handle* h) {
+    if (!h->batch_dirty) return CLT_OK;
+    const int F = h->num_frames;
+    if (F == 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_batch: needs a handle of clt_create_batch");
+    std::vector<int32_t> cs(2 * F);
+    int total = 0;
+    for (int f = 0; f < F; f++) {
+        h->frame_start[f] = total;
+        cs[f] = h->frame_count[f];
+        cs[F + f] = total;
+        total += h->frame_count[f];
+    }
+    h->frame_start[F] = total;
+    HIPCHK(h, hipMemcpyAsync(h->d_fcounts, cs.data(), cs.size() * 4, hipMemcpyHostToDevice, h->stream));
+    const long long map_n = (long long)F * h->num_tiles;
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)std::min<long long>((map_n + 255) / 256, 4096)), dim3(256), 0, h->stream,
+                       h->d_tile_map, map_n, -1);
+    const long long slots = (long long)F * h->max_tasks;
+    hipLaunchKernelGGL(k_batch_assemble, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, h->stream, h->d_staged,
+                       h->d_fcounts, h->d_fcounts + F, F, h->max_tasks, h->task_words, h->d_tasks, h->d_task_frame,
+                       h->d_tile_map, h->gd);
+    HIPCHK(h, hipGetLastError());
+    h->num_tasks = total;
+    h->batch_dirty = false;
+    return CLT_OK;
+}
+
+enum { STAGE_CONVERT = 1, STAGE_CORR = 2, STAGE_TEX = 4, STAGE_IMCLT = 8 };
+
+static void launch_batch(clt_handle* h, int stages) {
+    const int32_t* tf = h->num_frames > 1 ? h->d_task_frame : nullptr;
+    if (stages & STAGE_CONVERT) launch_convert(h, h->num_tasks, tf, 0);
+    if (stages & STAGE_CORR) launch_corr(h, h->num_tasks, tf);
+    if (stages & STAGE_TEX) launch_tex(h, h->num_tasks, tf, nullptr);
+    if (stages & STAGE_IMCLT) launch_imclt(h, h->num_frames, 1);
+}
+
+static int prepare_batch(clt_handle* h, int stages) {
+    for (int c = 0; c < h->g.num_cams; c++)
+        if (!h->images_set[c]) return fail(h, CLT_E_STATE, "clt_exec_batch: image of camera %d of frame 0 not set", c);
+    int rc;
+    if ((rc = assemble_batch(h))) return rc;
+    if (h->num_tasks <= 0) return fail(h, CLT_E_STATE, "clt_exec_batch: no tasks");
+    if ((stages & STAGE_CORR) && (rc = prepare_corr(h))) return rc;
+    if ((stages & STAGE_TEX) && (rc = prepare_tex(h))) return rc;
+    const clt_geom& g = h->g;
+    if ((stages & STAGE_IMCLT) &&
+        (rc = dalloc(h, (void**)&h->d_rbg, (size_t)h->num_frames * g.num_cams * g.num_colors * (g.width + 8) * (g.height + 8) * 4)))
+        return rc;
+    return CLT_OK;
+}
+
+// Time one stage over the whole batch (median of 3 after one warm-up), ms.
+static float time_stage(clt_handle* h, int stage) {
+    hipEvent_t b, e;
+    if (hipEventCreate(&b) != hipSuccess) return 1e30f;
+    if (hipEventCreate(&e) != hipSuccess) {
+        (void)hipEventDestroy(b);
+        return 1e30f;
+    }
+    float t[3] = {1e30f, 1e30f, 1e30f};
+    launch_batch(h, stage);
+    for (int i = 0; i < 3; i++) {
+        (void)hipEventRecord(b, h->stream);
+        launch_batch(h, stage);
+        (void)hipEventRecord(e, h->stream);
+        if (hipEventSynchronize(e) != hipSuccess || hipEventElapsedTime(&t[i], b, e) != hipSuccess) t[i] = 1e30f;
+    }
+    (void)hipEventDestroy(b);
+    (void)hipEventDestroy(e);
+    std::sort(t, t + 3);
+    return t[1];
+}
+
+int clt_autotune(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    int rc;
+    if ((rc = prepare_batch(h, STAGE_CONVERT | STAGE_CORR | STAGE_IMCLT))) return rc;
+    launch_batch(h, STAGE_CONVERT);   // every candidate below reads this CLT
+    if (h->corr_quad) {   // tiles (wavefronts) per workgroup of correlate2D, and so its LDS per workgroup
+        const int cand[3] = {1, 2, 4};
+        float best = 1e30f;
+        int pick = h->tune_corr_tpb;
+        for (int c : cand) {
+            h->tune_corr_tpb = c;
+            const float ms = time_stage(h, STAGE_CORR);
+            if (ms < best) { best = ms; pick = c; }
+        }
+        h->tune_corr_tpb = pick;
+    }
+    {   // tile rows per imclt segment (LDS carry reuse against the number of workgroups)
+        const int cand[3] = {8, 16, 32};
+        float best = 1e30f;
+        int pick = h->tune_imclt_rows;
+        for (int c : cand) {
+            h->tune_imclt_rows = c;
+            const float ms = time_stage(h, STAGE_IMCLT);
+            if (ms < best) { best = ms; pick = c; }
+        }
+        h->tune_imclt_rows = pick;
+    }
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->tuned = true;
+    h->graph_stages = 0;
+    h->converted = h->corr_done = h->tex_done = false;
+    return CLT_OK;
+}
+
+int clt_get_tune(clt_handle* h, int32_t* corr_tiles_per_block, int32_t* imclt_rows) {
+    if (!h) return CLT_E_ARG;
+    if (corr_tiles_per_block) *corr_tiles_per_block = h->tune_corr_tpb;
+    if (imclt_rows) *imclt_rows = h->tune_imclt_rows;
+    return CLT_OK;
+}
+
+int clt_exec_batch(clt_handle* h, int32_t stages) {
+    if (!h) return CLT_E_ARG;
+    if (stages <= 0 || stages > 15 || ((stages & ~STAGE_CONVERT) && !(stages & STAGE_CONVERT) && !h->converted))
+        return fail(h, CLT_E_ARG, "clt_exec_batch: stages 0x%x", stages);
+    int rc;
+    if ((rc = prepare_batch(h, stages))) return rc;
+    if (h->graph_stages != stages || h->graph_tasks != h->num_tasks) {   // grid sizes follow the task count
+        if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+        if (h->graph) (void)hipGraphDestroy(h->graph);
+        h->graph_exec = nullptr;
+        h->graph = nullptr;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        launch_batch(h, stages);
+        HIPCHK(h, hipStreamEndCapture(h->stream, &h->graph));
+        HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+        h->graph_stages = stages;
+        h->graph_tasks = h->num_tasks;
+    }
+    tbegin(h, T_BATCH);
+    HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+    tend(h, T_BATCH);
+    if (stages & STAGE_CONVERT) h->converted = true;
+    if (stages & STAGE_CORR) h->corr_done = true;
+    if (stages & STAGE_TEX) h->tex_done = true;
+    return finish(h);
+}
+
+static int frame_range(clt_handle* h, int frame, size_t* first, size_t* count) {
+    if (frame < 0 || frame >= h->num_frames) return fail(h, CLT_E_ARG, "frame %d of %d", frame, h->num_frames);
+    if (h->num_frames == 1) {
+        *first = 0;
+        *count = h->num_tasks;
+    } else {
+        if (h->batch_dirty) return fail(h, CLT_E_STATE, "tasks changed since the last clt_exec_batch");
+        *first = h->frame_start[frame];
+        *count = h->frame_start[frame + 1] - h->frame_start[frame];
+    }
+    return CLT_OK;
+}
+
+int clt_get_corr2d_frame(clt_handle* h, int32_t frame, float* corr, float* combo, float* disparity) {
+    if (!h) return CLT_E_ARG;
+    if (!h->corr_done) return fail(h, CLT_E_STATE, "clt_get_corr2d_frame: corr2d not executed");
+    size_t f0, nt;
+    int rc;
+    if ((rc = frame_range(h, frame, &f0, &nt))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const size_t P = h->corr_pairs_at_exec, sh = h->corr_strip_h_at_exec;
+    if (corr && nt) HIPCHK(h, hipMemcpy(corr, h->d_corr + f0 * P * CLT_CORR_LEN, nt * P * CLT_CORR_LEN * 4, hipMemcpyDeviceToHost));
+    if (combo && sh && nt) HIPCHK(h, hipMemcpy(combo, h->d_combo + f0 * sh * 15, nt * sh * 15 * 4, hipMemcpyDeviceToHost));
+    if (disparity && sh && nt)
+        HIPCHK(h, hipMemcpy(disparity, h->d_disp + f0 * CLT_DISP_LAYERS, nt * CLT_DISP_LAYERS * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+int clt_get_textures_frame(clt_handle* h, int32_t frame, float* tiles, float* stats) {
+    if (!h) return CLT_E_ARG;
+    if (!h->tex_done) return fail(h, CLT_E_STATE, "clt_get_textures_frame: textures not executed");
+    size_t f0, nt;
+    int rc;
+    if ((rc = frame_range(h, frame, &f0, &nt))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const size_t pl = h->tex_planes_at_exec, C = h->g.num_cams;
+    if (tiles && nt) HIPCHK(h, hipMemcpy(tiles, h->d_tex + f0 * pl * 256, nt * pl * 256 * 4, hipMemcpyDeviceToHost));
+    if (stats && nt) HIPCHK(h, hipMemcpy(stats, h->d_tex_stats + f0 * 4 * C, nt * 4 * C * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+int clt_get_rbg_frame(clt_handle* h, int32_t frame, int32_t cam, float* out) {
+    if (!h || !out) return CLT_E_ARG;
+    if (frame < 0 || frame >= h->num_frames || cam < 0 || cam >= h->g.num_cams)
+        return fail(h, CLT_E_ARG, "clt_get_rbg_frame: frame %d / camera %d", frame, cam);
+    if (!h->d_rbg) return fail(h, CLT_E_STATE, "clt_get_rbg_frame: imclt not executed");
+    const size_t n = (size_t)h->g.num_colors * (h->g.width + 8) * (h->g.height + 8);
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(out, h->d_rbg + ((size_t)frame * h->g.num_cams + cam) * n, n * 4, hipMemcpyDeviceToHost));
     return CLT_OK;
 }
 

@@ -44,7 +44,7 @@ class GPUTileProcessor:
     DTT_SIZE = 8
 
     def __init__(self, width=2592, height=1936, num_cams=4, num_colors=3, red_row=0, red_col=1,
-                 kernels=True, kernel_step=16, max_tasks=0):
+                 kernels=True, kernel_step=16, max_tasks=0, _frames=0):
         self.lib = _abi.load()
         g = _abi.clt_geom(width=width, height=height, num_cams=num_cams, num_colors=num_colors,
                           red_row=red_row, red_col=red_col,
@@ -56,7 +56,10 @@ class GPUTileProcessor:
         self.KERNELS_HOR, self.KERNELS_VERT = g.kernels_hor, g.kernels_vert
         self.tiles_x, self.tiles_y = width // 8, height // 8
         h = ctypes.c_void_p()
-        rc = self.lib.clt_create(ctypes.byref(g), ctypes.byref(h))
+        if _frames:
+            rc = self.lib.clt_create_batch(ctypes.byref(g), _frames, ctypes.byref(h))
+        else:
+            rc = self.lib.clt_create(ctypes.byref(g), ctypes.byref(h))
         if rc != 0:
             raise CltError(rc, "clt_create failed (geometry %dx%d, %d cameras)" % (width, height, num_cams))
         self.h = h
@@ -306,3 +309,61 @@ class GPUTileProcessor:
         cnt = ctypes.c_int32()
         self._chk(self.lib.clt_get_timing(self.h, _abi.TIMING_KERNELS[kernel], ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+
+class GPUTileBatch(GPUTileProcessor):
+    """Many frames (camera sets of one geometry) with uneven task lists, processed by one
+    hipGraph launch per call (clt_exec_batch); parameters and kernels are shared."""
+
+    CONVERT, CORR2D, TEXTURES, IMCLT = 1, 2, 4, 8
+
+    def __init__(self, num_frames, width=2592, height=1936, num_cams=4, num_colors=3, red_row=0, red_col=1,
+                 kernels=True, kernel_step=16, max_tasks=0):
+        super().__init__(width, height, num_cams, num_colors, red_row, red_col, kernels, kernel_step,
+                         max_tasks, _frames=num_frames)
+        self.num_frames = num_frames
+        self.frame_tasks = [0] * num_frames
+
+    def setBayerImagesFrame(self, frame, images):
+        """images: [C][H][W] float mosaics of one frame."""
+        for c in range(self.NUM_CAMS):
+            img = np.ascontiguousarray(images[c], np.float32).reshape(self.IMG_HEIGHT, -1)
+            self._chk(self.lib.clt_set_image_frame(self.h, frame, c, _ptr(img), img.shape[1]))
+
+    def setTasksFrame(self, frame, recs):
+        recs = np.ascontiguousarray(recs, np.int32)
+        self._chk(self.lib.clt_set_tasks_frame(self.h, frame, _ptr(recs), recs.shape[0]))
+        self.frame_tasks[frame] = recs.shape[0]
+
+    def autotune(self):
+        self._chk(self.lib.clt_autotune(self.h))
+        return self.getTune()
+
+    def getTune(self):
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        self._chk(self.lib.clt_get_tune(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return {"corr_tiles_per_block": a.value, "imclt_rows": b.value}
+
+    def execBatch(self, stages=15):
+        self._chk(self.lib.clt_exec_batch(self.h, stages))
+
+    def getCorr2DFrame(self, frame):
+        n = self.frame_tasks[frame]
+        corr = np.empty((n, self.num_pairs, 15, 15), np.float32)
+        combo = np.empty((n, self.strip_height, 15), np.float32)
+        disp = np.empty((n, _abi.DISP_LAYERS), np.float32)
+        self._chk(self.lib.clt_get_corr2d_frame(self.h, frame, _ptr(corr), _ptr(combo), _ptr(disp)))
+        return corr, combo, disp
+
+    def getTexturesFrame(self, frame):
+        n = self.frame_tasks[frame]
+        planes = (4 + self.NUM_CAMS + 4) if self.keep_weights else 4
+        tiles = np.empty((n, planes, 16, 16), np.float32)
+        stats = np.empty((n, 4 * self.NUM_CAMS), np.float32)
+        self._chk(self.lib.clt_get_textures_frame(self.h, frame, _ptr(tiles), _ptr(stats)))
+        return tiles, stats
+
+    def getRBGFrame(self, frame, ncam):
+        out = np.empty((self.NUM_COLORS, (self.IMG_HEIGHT + 8) * (self.IMG_WIDTH + 8)), np.float32)
+        self._chk(self.lib.clt_get_rbg_frame(self.h, frame, ncam, _ptr(out)))
+        return out

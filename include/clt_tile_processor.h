@@ -217,12 +217,32 @@ int  clt_get_tasks(clt_handle* h, void* tasks);
  * column 13 by the disparity.  Computed for tasks set by clt_set_tasks_geometry. */
 int  clt_get_port_derivatives(clt_handle* h, double* out);
 
+/* ---- batched frames (no reference counterpart) -------------------------------------------------
+ * One handle holds num_frames frames (camera sets of the same geometry, sharing the kernels and
+ * parameters), each with its own images and uneven task list.  clt_exec_batch runs the selected
+ * stages for every task of every frame as one hipGraph launch (captured on first use, again when
+ * the task lists change); launch shapes come from clt_autotune.  The per-frame clt_exec_* and
+ * clt_set_tasks calls are not available on a batch handle. */
+int  clt_create_batch(const clt_geom* geom, int32_t num_frames, clt_handle** out);
+int  clt_set_image_frame(clt_handle* h, int32_t frame, int32_t cam, const float* image, int32_t stride);
+int  clt_set_tasks_frame(clt_handle* h, int32_t frame, const void* tasks, int32_t num_tasks);
+/* stages: 1 convert (no LPF), 2 correlate2D, 4 textures, 8 imclt_rbg with the corr_sigma LPF. */
+int  clt_exec_batch(clt_handle* h, int32_t stages);
+/* Times the candidate launch shapes on the current batch and keeps the fastest: tiles per
+ * workgroup of correlate2D (1, 2, 4) and tile rows per imclt segment (8, 16, 32). */
+int  clt_autotune(clt_handle* h);
+int  clt_get_tune(clt_handle* h, int32_t* corr_tiles_per_block, int32_t* imclt_rows);
+/* Results of one frame, laid out as the per-frame getters (tasks in the frame's own order). */
+int  clt_get_corr2d_frame(clt_handle* h, int32_t frame, float* corr, float* combo, float* disparity);
+int  clt_get_textures_frame(clt_handle* h, int32_t frame, float* tiles, float* stats);
+int  clt_get_rbg_frame(clt_handle* h, int32_t frame, int32_t cam, float* out);
+
 /* ---- execution control and timing (no reference counterpart) --------------------------------- */
 int  clt_set_async(clt_handle* h, int32_t async);     /* 1: exec_* return after launch        */
 int  clt_sync(clt_handle* h);
 int  clt_set_timing(clt_handle* h, int32_t enable);   /* record HIP events around each kernel  */
 /* Accumulated device time (ms) and launch count of a kernel since the last reset:
- * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures. */
+ * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures, 5 batch graph. */
 int  clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count);
 int  clt_reset_timing(clt_handle* h);
 

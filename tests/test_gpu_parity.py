@@ -261,3 +261,43 @@ def test_config0_pipeline_disparity(pk, orc):
     assert e_clt < CLT_TOL and e_img < IMG_TOL and e_disp < DISP_TOL
     assert np.array_equal(np.isfinite(o_disp[:, 1]), np.isfinite(g_disp[:, 1]))
     tp.close()
+
+
+def test_batch_matches_per_frame(pk, orc):
+    """Three frames with uneven sparse task lists through one batched graph launch give the same
+    results as each frame through the per-frame path; the autotuned shapes change nothing."""
+    scenes = [pk.synth.make_scene(128, 96, seed=30 + f, disparity=("field", 0.0, 6.0)) for f in range(3)]
+    rng = np.random.default_rng(5)
+    keep = [rng.random(len(sc.tasks)) < frac for sc, frac in zip(scenes, (0.1, 1.0, 0.5))]
+    bt = pk.GPUTileBatch(3, 128, 96)
+    bt.setConvolutionKernels(scenes[0].kernels, offsets=scenes[0].offsets, force=True)
+    bt.setCorrSigma(0.8)
+    for f, sc in enumerate(scenes):
+        bt.setBayerImagesFrame(f, sc.images)
+        bt.setTasksFrame(f, sc.tasks[keep[f]])
+    tune = bt.autotune()
+    log("case: batch of 3 frames (10%%, 100%%, 50%% tasks), tune %s" % tune)
+    bt.execBatch()
+    bt.execBatch()   # graph replay
+    for f, sc in enumerate(scenes):
+        tp = pk.GPUTileProcessor(128, 96)
+        tp.setConvolutionKernels(scenes[0].kernels, offsets=scenes[0].offsets, force=True)
+        tp.setBayerImages(sc.images, force=True)
+        tp.setTasks(sc.tasks[keep[f]])
+        tp.setCorrSigma(0.8)
+        tp.execConverCorrectTiles()
+        tp.execCorrelate2D()
+        tp.execTextures()
+        tp.execImcltRbg(True)
+        c1, s1, d1 = tp.getCorr2D()
+        c2, s2, d2 = bt.getCorr2DFrame(f)
+        t1, st1 = tp.getTextures()
+        t2, st2 = bt.getTexturesFrame(f)
+        for name, a, b in (("corr", c2, c1), ("combo", s2, s1), ("disp", d2, d1), ("tex", t2, t1), ("stats", st2, st1)):
+            assert np.array_equal(np.isnan(a), np.isnan(b)), name
+            m = ~np.isnan(b)
+            assert np.abs(a[m] - b[m]).max() <= 1e-6 * max(1.0, np.abs(b[m]).max()), (f, name)
+        for c in range(4):
+            assert np.abs(bt.getRBGFrame(f, c) - tp.getRBG(c)).max() <= 1e-6 * max(1.0, np.abs(tp.getRBG(c)).max())
+        tp.close()
+    bt.close()
