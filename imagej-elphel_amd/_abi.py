@@ -13,8 +13,10 @@ CLT_OK = 0
 ERRORS = {-1: "CLT_E_ARG", -2: "CLT_E_STATE", -3: "CLT_E_HIP", -4: "CLT_E_NODEVICE",
           -5: "CLT_E_NOMEM", -6: "CLT_E_UNSUPPORTED"}
 DISP_LAYERS = 8
+CORR_ADJ = 32
 DISP_NAMES = ["int", "cm", "strength", "hor", "hor_strength", "vert", "vert_strength", "cm_weight"]
-TIMING_KERNELS = {"convert": 0, "imclt": 1, "corr2d": 2, "textures": 3, "combine_textures": 4, "batch": 5}
+TIMING_KERNELS = {"convert": 0, "imclt": 1, "corr2d": 2, "textures": 3, "combine_textures": 4, "batch": 5,
+                  "corr_adjoint": 6, "geometry_adjoint": 7}
 
 
 class clt_geom(ctypes.Structure):
@@ -103,6 +105,8 @@ SIGNATURES = {
     "clt_get_corr2d_frame": (_I, [_H, _I, _P, _P, _P]),
     "clt_get_textures_frame": (_I, [_H, _I, _P, _P]),
     "clt_get_rbg_frame": (_I, [_H, _I, _I, _P]),
+    "clt_exec_corr_adjoint": (_I, [_H]),
+    "clt_get_corr_adjoint": (_I, [_H, _P]),
 }
 
 _lib = None

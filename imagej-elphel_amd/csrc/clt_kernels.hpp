@@ -1215,6 +1215,287 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
 }
 
 // ====================================================================================
+// Correlation-peak loss and its adjoint (quad camera, 3 colours, the 6 reference pairs).
+//
+// Per task t the loss is l_t = -sum_p C_p, C_p the zero-offset value of pair p's phase correlation,
+// i.e. the centre (7, 7) of its 15x15 tile.  With the DCT-IIe and unfold of correlate2D this is
+//   C_p = 1/16 sum_colours w_k sum_bins lpf_bin <D1, D2>_bin / (|D1|_bin |D2|_bin)   (fat zero 0)
+// over the 64 frequency bins of the 4-mode CLT vectors D of the two cameras.  One workgroup per
+// task does convert_direct for its 4 cameras x 3 colours together with the tangents of the CLT by
+// the residual shifts (rx, ry) of every unit (forward mode through the shifted-window fold, the
+// transforms, the kernel multiply and the phase rotation), then the reverse pass of the loss to
+// dl/dD in LDS, and contracts it with the tangents: dl/d(rx, ry) per unit, dl/d(x, y) of every port
+// centre through the kernel-offset chain of the window position (ImageDtt.java:4915-4943).
+// Also returned per pair: the curvatures of C_p under a relative shift along x and y
+// (-d2C/dtau2 = 1/16 sum w lpf omega^2 f, omega the phase rate of the bin), used as the
+// Gauss-Newton weights of the LMA.
+// Output per task (CORR_ADJ_STRIDE floats): loss, C[6], kx[6], ky[6], dl/dxy[4][2].
+// ====================================================================================
+constexpr int CORR_ADJ_STRIDE = 32;
+
+__global__ __launch_bounds__(192) void k_corr_adjoint_quad(const float* __restrict__ images, const float* __restrict__ kernels,
+                                                           const float* __restrict__ koffsets, const int32_t* __restrict__ tasks,
+                                                           int task_words, int num_tasks, const float* __restrict__ lpf,
+                                                           const CorrArgs* __restrict__ args_p, double inv_kstep,
+                                                           float* __restrict__ out, GeomDev g) {
+    __shared__ __attribute__((aligned(16))) float s_buf[24][CONV_UNIT];   // unit u: value rows u, tangent rows 12 + u
+    __shared__ float s_red[3][20];
+    __shared__ float s_gxy[4][3][2];
+    const CorrArgs& A = *args_p;
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    if (t >= num_tasks) return;                              // block-uniform
+    const int color = threadIdx.x >> 6;                     // wave = colour
+    const int unit_in_block = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int cam = (threadIdx.x >> 4) & 3;
+    const int32_t* rec = tasks + (size_t)t * task_words;
+    float* o = out + (size_t)t * CORR_ADJ_STRIDE;
+    if (rec[0] == 0) {
+        if (threadIdx.x < CORR_ADJ_STRIDE) o[threadIdx.x] = (threadIdx.x == 0) ? __int_as_float(0x7fc00000) : 0.f;
+        return;
+    }
+    const int txy = rec[1];
+    const double cx = (double)__int_as_float(rec[2 + 2 * cam]);
+    const double cy = (double)__int_as_float(rec[3 + 2 * cam]);
+    const float* ce = nullptr;
+    const float* kt = nullptr;
+    int ktx = 0, kty = 0;
+    if (kernels) {
+        ktx = (int)floor(cx * inv_kstep + 0.5) + 1;
+        kty = (int)floor(cy * inv_kstep + 0.5) + 1;
+        kty = kty < 0 ? 0 : (kty >= g.kv ? g.kv - 1 : kty);
+        ktx = ktx < 0 ? 0 : (ktx >= g.kh ? g.kh - 1 : ktx);
+        const size_t kidx = (((size_t)cam * g.kv + kty) * g.kh + ktx) * 3 + color;
+        ce = koffsets + kidx * 8;
+        kt = kernels + kidx * 256;
+    }
+    const TilePos pos = tile_position(cx, cy, ce, g.kstep, ktx, kty);
+    const int left = pos.left, top = pos.top;
+    const float rx = pos.rx, ry = pos.ry;
+    float* sb = s_buf[unit_in_block];
+    float* sbx = s_buf[12 + unit_in_block];
+    {   // window rows, coalesced (lane = column), this colour only, parity-preserving clamp
+        const float* img = images + (size_t)cam * g.height * g.width;
+        const int pc = left + l, cp = pc & 1;
+        auto colour_at = [&](int rp) {
+            const bool rr = rp == g.red_row, rc = cp == g.red_col;
+            return rr ? (rc ? 0 : 2) : (rc ? 2 : 1);
+        };
+        const bool keep_e = colour_at(top & 1) == color, keep_o = colour_at((top & 1) ^ 1) == color;
+        const int pcc = clamp_parity(pc, g.width);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int pr = clamp_parity(top + r, g.height);
+            sb[r * CONV_ROW + l] = ((r & 1) ? keep_o : keep_e) ? img[(size_t)pr * g.width + pcc] : 0.f;
+        }
+    }
+    wave_sync();
+    const int r = l;
+    const bool row_red = (((top + r) & 1) == g.red_row);
+    const int cpar = (color == 0) ? (row_red ? g.red_col : -1) : (color == 1) ? (row_red ? -1 : (g.red_col ^ 1))
+                                                                               : (row_red ? (g.red_col ^ 1) : g.red_col);
+    float x[16];
+#pragma unroll
+    for (int j = 0; j < 16; j += 4) {
+        const float4 v = *reinterpret_cast<const float4*>(&sb[r * CONV_ROW + j]);
+        x[j] = v.x; x[j + 1] = v.y; x[j + 2] = v.z; x[j + 3] = v.w;
+    }
+    wave_sync();
+    constexpr float kPi16 = 3.14159265358979f / 16.f;
+    float ahs, ahc, avs, avc;
+    sincos_pi(rx * 0.0625f, &ahs, &ahc);
+    sincos_pi(ry * 0.0625f, &avs, &avc);
+    {   // stage H: value with (ahc, ahs), tangent by rx with (d ahc, d ahs) = pi/16 (-ahs, ahc)
+        float yc[8], ys[8], hc[8], hsn[8];
+        const float sg = (((cpar ^ left) & 1) ? -1.f : 1.f);
+        fold16(x, ahc, ahs, yc, ys);
+        dct_iv8(yc, hc);
+#pragma unroll
+        for (int k = 0; k < 8; k++) hsn[k] = sg * hc[7 - k];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            sb[k * CONV_ROW + r] = hc[k];
+            sb[(8 + k) * CONV_ROW + r] = hsn[k];
+        }
+        fold16(x, -ahs * kPi16, ahc * kPi16, yc, ys);
+        dct_iv8(yc, hc);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            sbx[k * CONV_ROW + r] = hc[k];
+            sbx[(8 + k) * CONV_ROW + r] = sg * hc[7 - k];
+        }
+    }
+    wave_sync();
+    const int hs = l >> 3, kh = l & 7;
+    float av[8], bv[8], ax[8], bx[8], ay[8], by[8];
+    {   // stage V: value V(col), by rx V(col_x), by ry V'(col)
+        float col[16], colx[16];
+#pragma unroll
+        for (int i = 0; i < 16; i += 4) {
+            const float4 v = *reinterpret_cast<const float4*>(&sb[(hs * 8 + kh) * CONV_ROW + i]);
+            const float4 w = *reinterpret_cast<const float4*>(&sbx[(hs * 8 + kh) * CONV_ROW + i]);
+            col[i] = v.x; col[i + 1] = v.y; col[i + 2] = v.z; col[i + 3] = v.w;
+            colx[i] = w.x; colx[i + 1] = w.y; colx[i + 2] = w.z; colx[i + 3] = w.w;
+        }
+        const bool rb = color != 2;
+        const int rpar = (color == 0) ? g.red_row : (g.red_row ^ 1);
+        const float sgv = (((rpar ^ top) & 1) ? -1.f : 1.f);
+        auto vstage = [&](const float* c, float vc, float vs, float* oa, float* ob) {
+            float zc[8], zs[8];
+            fold16(c, vc, vs, zc, zs);
+            dct_iv8(zc, oa);
+            if (rb) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) ob[k] = sgv * oa[7 - k];
+            } else {
+                dst_iv8(zs, ob);
+            }
+        };
+        vstage(col, avc, avs, av, bv);
+        vstage(colx, avc, avs, ax, bx);
+        vstage(col, -avs * kPi16, avc * kPi16, ay, by);
+    }
+    const float sigma = hs ? 1.f : -1.f;
+    if (kt) {   // kernel multiply (linear: same for value and tangents)
+        float k0[8], k1[8], k2[8], k3[8];
+        const float* kq = kt + kh * 8;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            k0[k] = kq[k];
+            k1[k] = kq[64 + k];
+            k2[k] = kq[128 + k];
+            k3[k] = kq[192 + k];
+        }
+        auto kmul = [&](float* a, float* b) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float pa = lane_xor8(a[k]) * sigma, pb = lane_xor8(b[k]) * sigma;
+                const float oa = a[k] * k0[k] + pa * k1[k] - b[k] * k2[k] - pb * k3[k];
+                const float ob = b[k] * k0[k] + pb * k1[k] + a[k] * k2[k] + pa * k3[k];
+                a[k] = oa;
+                b[k] = ob;
+            }
+        };
+        kmul(av, bv);
+        kmul(ax, bx);
+        kmul(ay, by);
+    }
+    {   // phase rotation: value, and the product rule for both tangents
+        float sh, ch, cv[8], sv[8];
+        sincos_pi((kh + 0.5f) * rx * 0.125f, &sh, &ch);
+        half_phase8(ry * 0.125f, cv, sv);
+        const float wh = 3.14159265358979f * (kh + 0.5f) * 0.125f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float wv = 3.14159265358979f * (k + 0.5f) * 0.125f;
+            const float pa = lane_xor8(av[k]) * sigma, pb = lane_xor8(bv[k]) * sigma;
+            const float pax = lane_xor8(ax[k]) * sigma, pbx = lane_xor8(bx[k]) * sigma;
+            const float pay = lane_xor8(ay[k]) * sigma, pby = lane_xor8(by[k]) * sigma;
+            const float ta = av[k] * ch + pa * sh, tb = bv[k] * ch + pb * sh;                 // S_h Y
+            const float tax = ax[k] * ch + pax * sh + wh * (pa * ch - av[k] * sh);            // d/drx S_h Y
+            const float tbx = bx[k] * ch + pbx * sh + wh * (pb * ch - bv[k] * sh);
+            const float tay = ay[k] * ch + pay * sh, tby = by[k] * ch + pby * sh;             // S_h dY/dry
+            av[k] = ta * cv[k] - tb * sv[k];
+            bv[k] = tb * cv[k] + ta * sv[k];
+            ax[k] = tax * cv[k] - tbx * sv[k];
+            bx[k] = tbx * cv[k] + tax * sv[k];
+            ay[k] = tay * cv[k] - tby * sv[k] - wv * (ta * sv[k] + tb * cv[k]);
+            by[k] = tby * cv[k] + tay * sv[k] + wv * (ta * cv[k] - tb * sv[k]);
+        }
+    }
+    __syncthreads();   // every wavefront is done with its staging rows
+    float* V = &s_buf[0][0];   // [cam][colour][mode][kh][kv], then dl/dCLT in place
+    {
+        float* vu = V + (cam * 3 + color) * 256 + kh * 8;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            vu[hs * 64 + k] = av[k];
+            vu[(hs | 2) * 64 + k] = bv[k];
+        }
+    }
+    __syncthreads();
+    {   // reverse pass of the loss: thread = (colour, bin)
+        const int k = threadIdx.x >> 6, bin = threadIdx.x & 63;
+        const int bh = bin >> 3, bvv = bin & 7;   // bin = kh * 8 + kv
+        float D[4][4], rr[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            float n2 = 0.f;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                D[c][n] = V[(c * 3 + k) * 256 + n * 64 + bin];
+                n2 += D[c][n] * D[c][n];
+            }
+            rr[c] = __builtin_amdgcn_rsqf(n2);
+        }
+        const float lw = 0.0625f * lpf[bin] * A.col_w[k];
+        const float oh = 3.14159265358979f * (bh + 0.5f) * 0.125f, ov = 3.14159265358979f * (bvv + 0.5f) * 0.125f;
+        float G[4][4] = {};
+        float part[18];
+#pragma unroll
+        for (int p = 0; p < 6; p++) {
+            const int c1 = QP_C1[p], c2 = QP_C2[p];
+            float dot = 0.f;
+#pragma unroll
+            for (int n = 0; n < 4; n++) dot += D[c1][n] * D[c2][n];
+            const float r12 = rr[c1] * rr[c2], f = dot * r12;
+            part[p] = lw * f;
+            part[6 + p] = lw * oh * oh * f;
+            part[12 + p] = lw * ov * ov * f;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {   // d(-C_p)/dD
+                G[c1][n] -= lw * (D[c2][n] * r12 - f * D[c1][n] * rr[c1] * rr[c1]);
+                G[c2][n] -= lw * (D[c1][n] * r12 - f * D[c2][n] * rr[c2] * rr[c2]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int n = 0; n < 4; n++) V[(c * 3 + k) * 256 + n * 64 + bin] = G[c][n];
+#pragma unroll
+        for (int q = 0; q < 18; q++) {
+            const float s = wave_sum(part[q]);
+            if ((threadIdx.x & 63) == 0) s_red[k][q] = s;
+        }
+    }
+    __syncthreads();
+    {   // contraction with the tangents over this unit's 256 coefficients
+        const float* gu = V + (cam * 3 + color) * 256 + kh * 8;
+        float grx = 0.f, gry = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float ga = gu[hs * 64 + k], gb = gu[(hs | 2) * 64 + k];
+            grx += ga * ax[k] + gb * bx[k];
+            gry += ga * ay[k] + gb * by[k];
+        }
+#pragma unroll
+        for (int ofs = 8; ofs > 0; ofs >>= 1) {
+            grx += __shfl_xor(grx, ofs, 16);
+            gry += __shfl_xor(gry, ofs, 16);
+        }
+        if (l == 0) {   // rx = left - px, px = cx - 8 - (c0 + c4 kdx + c5 kdy), py likewise (ImageDtt.java:4915-4943)
+            const float e4 = ce ? ce[4] : 0.f, e5 = ce ? ce[5] : 0.f, e6 = ce ? ce[6] : 0.f, e7 = ce ? ce[7] : 0.f;
+            s_gxy[cam][color][0] = -grx * (1.f - e4) + gry * e6;
+            s_gxy[cam][color][1] = grx * e5 - gry * (1.f - e7);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 18) {
+        const int q = threadIdx.x;
+        const float v = s_red[0][q] + s_red[1][q] + s_red[2][q];
+        o[1 + q] = v;   // C[6], kx[6], ky[6]
+    } else if (threadIdx.x < 26) {
+        const int c = (threadIdx.x - 18) >> 1, xyi = (threadIdx.x - 18) & 1;
+        o[19 + (threadIdx.x - 18)] = s_gxy[c][0][xyi] + s_gxy[c][1][xyi] + s_gxy[c][2][xyi];
+    } else if (threadIdx.x == 26) {
+        float loss = 0.f;
+        for (int p = 0; p < 6; p++) loss -= s_red[0][p] + s_red[1][p] + s_red[2][p];
+        o[0] = loss;
+    } else if (threadIdx.x < CORR_ADJ_STRIDE) {
+        o[threadIdx.x] = 0.f;
+    }
+}
+
+// ====================================================================================
 // textures: per tile RGBA (+ port weights, rms) from all cameras (ImageDtt.java:2662-2812,
 // tile_debayer_shot_corr 3022-3147, tile_combine_rgba 2830-3000).  Block (256 threads) = tile,
 // thread = pixel of the 16x16 tile in the pixel stages.  NC = number of cameras.

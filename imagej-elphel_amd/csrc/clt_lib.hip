@@ -101,7 +101,7 @@ static void half_flat_top(int n, double hwidth, double blur, bool normalize, boo
 }
 
 // ------------------------------------------------------------------------ handle
-enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_NUM };
+enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_NUM };
 
 struct clt_handle {
     clt_geom g{};
@@ -137,6 +137,8 @@ struct clt_handle {
     int32_t* d_gtxy = nullptr;     // [max_tasks]
     float* d_gdisp = nullptr;      // [max_tasks]
     double* d_gderiv = nullptr;    // [max_tasks][8][14]
+    float* d_adj = nullptr;        // [cap_tasks][CORR_ADJ_STRIDE] correlation-peak loss and gradient
+    bool adj_done = false;
     bool port_geom_set = false, tasks_from_geom = false;
     // batched frames (clt_create_batch): host task lists per frame, flat device list with frames
     int32_t* d_task_frame = nullptr;               // [cap_tasks]
@@ -414,7 +416,8 @@ int clt_destroy(clt_handle* h) {
     void* bufs[] = {h->d_images, h->d_kernels, h->d_koffs, h->d_tasks, h->d_tile_map, h->d_clt, h->d_lpf_rbg,
                     h->d_lpf_corr, h->d_rbg, h->d_corr, h->d_combo, h->d_disp, h->d_tex, h->d_tex_stats,
                     h->d_tex_img, h->d_extra, h->d_corr_args, h->d_tex_args, h->d_port_geom, h->d_rtab,
-                    h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv, h->d_task_frame, h->d_staged, h->d_fcounts};
+                    h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv, h->d_task_frame, h->d_staged, h->d_fcounts,
+                    h->d_adj};
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -492,7 +495,7 @@ int clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks) {
     HIPCHK(h, hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice));
     h->batch_dirty = true;   // the device task list now holds frame 0 only
     h->num_tasks = num_tasks;
-    h->converted = h->corr_done = h->tex_done = h->combined = false;
+    h->converted = h->corr_done = h->tex_done = h->combined = h->adj_done = false;
     h->tasks_from_geom = false;
     return CLT_OK;
 }
@@ -646,7 +649,7 @@ int clt_set_tasks_geometry(clt_handle* h, const int32_t* task, const int32_t* tx
     }
     HIPCHK(h, hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice));
     h->num_tasks = num_tasks;
-    h->converted = h->corr_done = h->tex_done = h->combined = false;
+    h->converted = h->corr_done = h->tex_done = h->combined = h->adj_done = false;
     h->tasks_from_geom = true;
     h->batch_dirty = true;
     if ((rc = launch_port_centres(h, nullptr))) return rc;
@@ -981,6 +984,47 @@ int clt_get_combined_textures(clt_handle* h, float* out) {
     if (!h->combined) return fail(h, CLT_E_STATE, "clt_get_combined_textures: combine not executed");
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(out, h->d_tex_img, (size_t)h->tex_planes_at_exec * h->g.width * h->g.height * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+// ------------------------------------------------------------------------ correlation-peak loss adjoint
+static int check_adjoint_config(clt_handle* h, const char* who) {
+    const clt_geom& g = h->g;
+    if (h->num_frames > 1 || g.num_cams != 4 || g.num_colors != 3)
+        return fail(h, CLT_E_UNSUPPORTED, "%s: needs a single-frame quad Bayer handle", who);
+    if (h->cp.fat_zero != 0.f) return fail(h, CLT_E_UNSUPPORTED, "%s: the adjoint is written for fat_zero 0", who);
+    for (int c = 0; c < g.num_cams; c++)
+        if (!h->images_set[c]) return fail(h, CLT_E_STATE, "%s: image of camera %d not set", who, c);
+    if (h->num_tasks <= 0) return fail(h, CLT_E_STATE, "%s: no tasks", who);
+    return CLT_OK;
+}
+
+static void launch_corr_adjoint(clt_handle* h) {
+    hipLaunchKernelGGL(k_corr_adjoint_quad, dim3(h->num_tasks), dim3(192), 0, h->stream, h->d_images,
+                       h->use_kernels ? h->d_kernels : nullptr, h->use_kernels ? h->d_koffs : nullptr, h->d_tasks,
+                       h->task_words, h->num_tasks, h->d_lpf_corr, h->d_corr_args, 1.0 / h->g.kernel_step, h->d_adj, h->gd);
+}
+
+int clt_exec_corr_adjoint(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    int rc;
+    if ((rc = check_adjoint_config(h, "clt_exec_corr_adjoint"))) return rc;
+    if ((rc = prepare_corr(h))) return rc;   // colour weights
+    if (!h->corr_quad) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_corr_adjoint: needs the 6 reference quad pairs");
+    if ((rc = dalloc(h, (void**)&h->d_adj, (size_t)h->cap_tasks * CORR_ADJ_STRIDE * 4))) return rc;
+    tbegin(h, T_ADJ);
+    launch_corr_adjoint(h);
+    tend(h, T_ADJ);
+    HIPCHK(h, hipGetLastError());
+    h->adj_done = true;
+    return finish(h);
+}
+
+int clt_get_corr_adjoint(clt_handle* h, float* out) {
+    if (!h || !out) return CLT_E_ARG;
+    if (!h->adj_done) return fail(h, CLT_E_STATE, "clt_get_corr_adjoint: not executed");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, hipMemcpy(out, h->d_adj, (size_t)h->num_tasks * CORR_ADJ_STRIDE * 4, hipMemcpyDeviceToHost));
     return CLT_OK;
 }
 

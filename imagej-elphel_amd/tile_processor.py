@@ -291,6 +291,18 @@ class GPUTileProcessor:
         self._chk(self.lib.clt_get_combined_textures(self.h, _ptr(out)))
         return out
 
+    # ------------------------------------------------------------------ correlation-peak loss adjoint
+    def execCorrAdjoint(self):
+        """Loss -sum_p C_p(0) of every task and its gradient by the port centres (quad, fat zero 0)."""
+        self._chk(self.lib.clt_exec_corr_adjoint(self.h))
+
+    def getCorrAdjoint(self):
+        """dict of arrays: loss [n], c [n][6], kx [n][6], ky [n][6], dxy [n][4][2]."""
+        out = np.empty((self.num_task_tiles, _abi.CORR_ADJ), np.float32)
+        self._chk(self.lib.clt_get_corr_adjoint(self.h, _ptr(out)))
+        return {"loss": out[:, 0], "c": out[:, 1:7], "kx": out[:, 7:13], "ky": out[:, 13:19],
+                "dxy": out[:, 19:27].reshape(-1, 4, 2)}
+
     # ------------------------------------------------------------------ execution control
     def setAsync(self, on):
         self._chk(self.lib.clt_set_async(self.h, int(on)))

@@ -217,6 +217,18 @@ int  clt_get_tasks(clt_handle* h, void* tasks);
  * column 13 by the disparity.  Computed for tasks set by clt_set_tasks_geometry. */
 int  clt_get_port_derivatives(clt_handle* h, double* out);
 
+/* ---- correlation-peak loss and its adjoint (no reference counterpart) ---------------------------
+ * Loss of a task: l = -sum_p C_p, C_p the zero-offset (centre) value of pair p's 15x15 phase
+ * correlation (C2:275-389 with the unfold of DR:443-489).  Gradient by the port centres of the task
+ * through convert_direct (ID:4867-5584) by a hand-written forward/reverse pass, no CLT in HBM.
+ * Quad Bayer handle, the 6 reference pairs, fat_zero 0.  Per task CLT_CORR_ADJ floats:
+ * [0] loss, [1..6] C_p, [7..12] and [13..18] the curvatures -d2 C_p / d tau^2 under a relative
+ * shift of the pair along x and y, [19..26] dl/d(x0, y0, x1, y1, x2, y2, x3, y3), rest 0.
+ * A task with task word 0 gets NaN loss and zero gradient. */
+#define CLT_CORR_ADJ 32
+int  clt_exec_corr_adjoint(clt_handle* h);
+int  clt_get_corr_adjoint(clt_handle* h, float* out);
+
 /* ---- batched frames (no reference counterpart) -------------------------------------------------
  * One handle holds num_frames frames (camera sets of the same geometry, sharing the kernels and
  * parameters), each with its own images and uneven task list.  clt_exec_batch runs the selected
@@ -242,7 +254,8 @@ int  clt_set_async(clt_handle* h, int32_t async);     /* 1: exec_* return after 
 int  clt_sync(clt_handle* h);
 int  clt_set_timing(clt_handle* h, int32_t enable);   /* record HIP events around each kernel  */
 /* Accumulated device time (ms) and launch count of a kernel since the last reset:
- * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures, 5 batch graph. */
+ * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures, 5 batch graph,
+ * 6 correlation adjoint, 7 geometry adjoint. */
 int  clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count);
 int  clt_reset_timing(clt_handle* h);
 
