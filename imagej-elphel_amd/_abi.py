@@ -53,6 +53,12 @@ class clt_port_geometry(ctypes.Structure):
                 ("roll_deg", ctypes.c_double * 4), ("extrinsic", ctypes.c_double * 13)]
 
 
+class clt_lma_params(ctypes.Structure):
+    _fields_ = [("lambda_", ctypes.c_double), ("lambda_scale_good", ctypes.c_double),
+                ("lambda_scale_bad", ctypes.c_double), ("lambda_max", ctypes.c_double),
+                ("rms_diff", ctypes.c_double), ("num_iter", ctypes.c_int32), ("adjust_disparity", ctypes.c_int32)]
+
+
 _P = ctypes.c_void_p
 _H = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -107,6 +113,9 @@ SIGNATURES = {
     "clt_get_rbg_frame": (_I, [_H, _I, _I, _P]),
     "clt_exec_corr_adjoint": (_I, [_H]),
     "clt_get_corr_adjoint": (_I, [_H, _P]),
+    "clt_exec_geometry_adjoint": (_I, [_H, _P, _P, _P]),
+    "clt_default_lma_params": (None, [ctypes.POINTER(clt_lma_params)]),
+    "clt_lma_extrinsics": (_I, [_H, ctypes.POINTER(clt_lma_params), _P, _P, _P, ctypes.POINTER(_I)]),
 }
 
 _lib = None

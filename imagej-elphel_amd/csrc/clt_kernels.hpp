@@ -299,30 +299,10 @@ struct PortGeomDev {
 };
 
 #pragma clang fp contract(off)
-__global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restrict__ pg, const double* __restrict__ rtab,
-                                                      const int32_t* __restrict__ words, const int32_t* __restrict__ txy,
-                                                      const float* __restrict__ disparity, int num_tasks, int task_words,
-                                                      int32_t* __restrict__ tasks, double* __restrict__ deriv) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = gid >> 2, i = gid & 3;
-    if (t >= num_tasks) return;
-    int32_t* rec = tasks + (size_t)t * task_words;
-    const int word = words[t], xy16 = txy[t];
-    if (i == 0) {
-        rec[0] = word;
-        rec[1] = xy16;
-    }
-    double* dr = deriv ? deriv + (size_t)t * 8 * 14 + (size_t)(2 * i) * 14 : nullptr;
-    if (word == 0) {   // GTP:720-722: nothing to do for this tile
-        rec[2 + 2 * i] = 0;
-        rec[3 + 2 * i] = 0;
-        if (dr)
-            for (int k = 0; k < 28; k++) dr[k] = 0.0;
-        return;
-    }
-    const PortGeomDev& P = *pg;
-    const double px = (double)((xy16 & 0xffff) * 8 + 4), py = (double)(((xy16 >> 16) & 0xffff) * 8 + 4);   // GTP:724-725
-    const double d = (double)disparity[t];
+// Port centre (x, y) of sensor i for tile centre (px, py) at disparity d; with rx, ry non-null
+// also its derivatives: 14 values each, by the 13 extrinsic parameters and by the disparity.
+__device__ __forceinline__ void port_centre(const PortGeomDev& P, const double* __restrict__ rtab, double px, double py,
+                                            double d, int i, double* x, double* y, double* rx, double* ry) {
     // undo the common distortion of the tile centre (GC:2181-2187, getRByRDist GC:2802-2830)
     const double pXcd = px - P.half_w, pYcd = py - P.half_h;
     const double rrel = sqrt(pXcd * pXcd + pYcd * pYcd) * P.rd_scale;
@@ -346,9 +326,9 @@ __global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restr
         rri *= ri;
         k += P.dist[j] * (rri - 1.0);
     }
-    rec[2 + 2 * i] = __float_as_int((float)(pXci * k + P.pxy0[i][0]));
-    rec[3 + 2 * i] = __float_as_int((float)(pYci * k + P.pxy0[i][1]));
-    if (!dr) return;
+    *x = pXci * k + P.pxy0[i][0];
+    *y = pYci * k + P.pxy0[i][1];
+    if (!rx) return;
     double dx[5], dy[5];   // by azimuth, tilt, roll, zoom (GC:2235-2283), disparity
 #pragma unroll
     for (int a = 0; a < 5; a++) {
@@ -368,8 +348,6 @@ __global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restr
         dx[a] = dxc * k + pXci * dkk;
         dy[a] = dyc * k + pYci * dkk;
     }
-    double* rx = dr;
-    double* ry = dr + 14;
     for (int c = 0; c < 14; c++) rx[c] = ry[c] = 0.0;
     if (i < 3) {   // sensor 3 tilt/azimuth/zoom are minus the sum of the others (GC:2285-2302)
         rx[0 + i] = dx[1]; ry[0 + i] = dy[1];
@@ -385,6 +363,35 @@ __global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restr
     rx[6 + i] = dx[2]; ry[6 + i] = dy[2];
     rx[13] = dx[4]; ry[13] = dy[4];
 }
+
+__global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restrict__ pg, const double* __restrict__ rtab,
+                                                      const int32_t* __restrict__ words, const int32_t* __restrict__ txy,
+                                                      const float* __restrict__ disparity, int num_tasks, int task_words,
+                                                      int32_t* __restrict__ tasks, double* __restrict__ deriv) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gid >> 2, i = gid & 3;
+    if (t >= num_tasks) return;
+    int32_t* rec = tasks + (size_t)t * task_words;
+    const int word = words[t], xy16 = txy[t];
+    if (i == 0) {
+        rec[0] = word;
+        rec[1] = xy16;
+    }
+    double* dr = deriv ? deriv + (size_t)t * 8 * 14 + (size_t)(2 * i) * 14 : nullptr;
+    if (word == 0) {   // GTP:720-722: nothing to do for this tile
+        rec[2 + 2 * i] = 0;
+        rec[3 + 2 * i] = 0;
+        if (dr)
+            for (int k = 0; k < 28; k++) dr[k] = 0.0;
+        return;
+    }
+    const double px = (double)((xy16 & 0xffff) * 8 + 4), py = (double)(((xy16 >> 16) & 0xffff) * 8 + 4);   // GTP:724-725
+    double x, y;
+    port_centre(*pg, rtab, px, py, (double)disparity[t], i, &x, &y, dr, dr ? dr + 14 : nullptr);
+    rec[2 + 2 * i] = __float_as_int((float)x);
+    rec[3 + 2 * i] = __float_as_int((float)y);
+}
+
 #pragma clang fp contract(fast)
 
 // ====================================================================================
@@ -1271,6 +1278,13 @@ __global__ __launch_bounds__(192) void k_corr_adjoint_quad(const float* __restri
     const TilePos pos = tile_position(cx, cy, ce, g.kstep, ktx, kty);
     const int left = pos.left, top = pos.top;
     const float rx = pos.rx, ry = pos.ry;
+    // a task with any window outside the frame is left out (NaN loss, zero gradient): clamped
+    // border windows are nearly constant and would correlate perfectly
+    const bool inside = (left >= 0) && (left + 16 <= g.width) && (top >= 0) && (top + 16 <= g.height);
+    if (__syncthreads_or(!inside)) {
+        if (threadIdx.x < CORR_ADJ_STRIDE) o[threadIdx.x] = (threadIdx.x == 0) ? __int_as_float(0x7fc00000) : 0.f;
+        return;
+    }
     float* sb = s_buf[unit_in_block];
     float* sbx = s_buf[12 + unit_in_block];
     {   // window rows, coalesced (lane = column), this colour only, parity-preserving clamp
@@ -1493,6 +1507,128 @@ __global__ __launch_bounds__(192) void k_corr_adjoint_quad(const float* __restri
     } else if (threadIdx.x < CORR_ADJ_STRIDE) {
         o[threadIdx.x] = 0.f;
     }
+}
+
+// Gradient of the correlation-peak loss by the extrinsics and the disparities, and the
+// Gauss-Newton system of the LMA with the disparities eliminated (Schur complement).
+// Workgroup = 32 tasks, 128 threads.  Per task t, with J_c the 2 x 14 derivatives of port c's
+// centre (13 extrinsics, disparity), the loss gradient by the centres g_c and per pair p = (c1, c2)
+// the curvatures kx, ky: g_t = sum_c g_c J_c, H_t = sum_p (J_c1 - J_c2)^T diag(kx, ky) (J_c1 - J_c2).
+// With hd = H_t[13][13] (1 + lambda) + eps: A += H_ee - h h^T / hd, b += g_e - h g_d / hd,
+// h = H_t[0..12][13].  Per workgroup partial sums of A (91 entries, upper triangle row-major),
+// b (13), loss (1) and the plain extrinsic gradient sum_t g_e (13) go to part[block][118]; per task
+// {g_d, hd, h[13]} to tstate[t][15].
+constexpr int GEO_ADJ_TASKS = 32;
+constexpr int GEO_ADJ_SUMS = 118;
+__global__ __launch_bounds__(128) void k_geometry_adjoint(const PortGeomDev* __restrict__ pg, const double* __restrict__ rtab,
+                                                          const int32_t* __restrict__ words, const int32_t* __restrict__ txy,
+                                                          const float* __restrict__ disparity, const float* __restrict__ adj,
+                                                          int num_tasks, double lambda, double* __restrict__ part,
+                                                          double* __restrict__ tstate) {
+    __shared__ double sJ[GEO_ADJ_TASKS][4][2][14];
+    __shared__ double sG[GEO_ADJ_TASKS][4][14];
+    __shared__ double sH13[GEO_ADJ_TASKS][14];
+    __shared__ float sK[GEO_ADJ_TASKS][12];
+    const int t0 = blockIdx.x * GEO_ADJ_TASKS;
+    {   // derivatives of the 4 port centres of 32 tasks
+        const int tl = threadIdx.x >> 2, c = threadIdx.x & 3, t = t0 + tl;
+        double* rx = sJ[tl][c][0];
+        double* ry = sJ[tl][c][1];
+        const bool live = t < num_tasks && words[t] != 0 && !isnan(adj[(size_t)t * CORR_ADJ_STRIDE]);
+        if (live) {
+            const int xy16 = txy[t];
+            const double px = (double)((xy16 & 0xffff) * 8 + 4), py = (double)(((xy16 >> 16) & 0xffff) * 8 + 4);
+            double x, y;
+            port_centre(*pg, rtab, px, py, (double)disparity[t], c, &x, &y, rx, ry);
+            const float* a = adj + (size_t)t * CORR_ADJ_STRIDE;
+            const double gx = a[19 + 2 * c], gy = a[20 + 2 * c];
+            for (int k = 0; k < 14; k++) sG[tl][c][k] = gx * rx[k] + gy * ry[k];
+            if (c == 0)
+                for (int k = 0; k < 12; k++) sK[tl][k] = fmaxf(a[7 + k], 0.f);   // kx[6], ky[6]
+        } else {
+            for (int k = 0; k < 14; k++) rx[k] = ry[k] = sG[tl][c][k] = 0.0;
+            if (c == 0)
+                for (int k = 0; k < 12; k++) sK[tl][k] = 0.f;
+        }
+    }
+    __syncthreads();
+    auto hval = [&](int tl, int i, int j) {   // H_t[i][j]
+        double s = 0.0;
+#pragma unroll
+        for (int p = 0; p < 6; p++) {
+            const int c1 = QP_C1[p], c2 = QP_C2[p];
+            const double dxi = sJ[tl][c1][0][i] - sJ[tl][c2][0][i], dxj = sJ[tl][c1][0][j] - sJ[tl][c2][0][j];
+            const double dyi = sJ[tl][c1][1][i] - sJ[tl][c2][1][i], dyj = sJ[tl][c1][1][j] - sJ[tl][c2][1][j];
+            s += (double)sK[tl][p] * dxi * dxj + (double)sK[tl][6 + p] * dyi * dyj;
+        }
+        return s;
+    };
+    for (int k = threadIdx.x; k < GEO_ADJ_TASKS * 14; k += 128) {
+        const int tl = k / 14, i = k % 14;
+        sH13[tl][i] = hval(tl, i, 13);
+    }
+    __syncthreads();
+    const int e = threadIdx.x;
+    if (e < GEO_ADJ_SUMS) {
+        int ei = 0, ej = 0;
+        if (e < 91) {   // upper triangle index -> (i, j)
+            int r = e;
+            while (r >= 13 - ei) { r -= 13 - ei; ei++; }
+            ej = ei + r;
+        }
+        double acc = 0.0;
+        for (int tl = 0; tl < GEO_ADJ_TASKS; tl++) {
+            const int t = t0 + tl;
+            if (t >= num_tasks) break;
+            const double hd = sH13[tl][13] * (1.0 + lambda) + 1e-12;
+            const double gd = sG[tl][0][13] + sG[tl][1][13] + sG[tl][2][13] + sG[tl][3][13];
+            if (e < 91) {
+                acc += hval(tl, ei, ej) - sH13[tl][ei] * sH13[tl][ej] / hd;
+            } else if (e < 104) {
+                const int i = e - 91;
+                const double gi = sG[tl][0][i] + sG[tl][1][i] + sG[tl][2][i] + sG[tl][3][i];
+                acc += gi - sH13[tl][i] * gd / hd;
+            } else if (e == 104) {
+                const float l = adj[(size_t)t * CORR_ADJ_STRIDE];
+                if (!isnan(l)) acc += (double)l;
+            } else {
+                const int i = e - 105;
+                acc += sG[tl][0][i] + sG[tl][1][i] + sG[tl][2][i] + sG[tl][3][i];
+            }
+        }
+        part[(size_t)blockIdx.x * GEO_ADJ_SUMS + e] = acc;
+    }
+    for (int k = threadIdx.x; k < GEO_ADJ_TASKS; k += 128) {
+        const int t = t0 + k;
+        if (t >= num_tasks) break;
+        double* ts = tstate + (size_t)t * 15;
+        ts[0] = sG[k][0][13] + sG[k][1][13] + sG[k][2][13] + sG[k][3][13];
+        ts[1] = sH13[k][13] * (1.0 + lambda) + 1e-12;
+        for (int i = 0; i < 13; i++) ts[2 + i] = sH13[k][i];
+    }
+}
+
+// Column sums of rows x cols doubles (one thread per column).
+__global__ __launch_bounds__(128) void k_sum_rows(const double* __restrict__ a, int rows, int cols, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    double s = 0.0;
+    for (int r = 0; r < rows; r++) s += a[(size_t)r * cols + c];
+    out[c] = s;
+}
+
+// Disparity step of the LMA after the extrinsic step de: dd_t = -(g_d + h . de) / hd, limited to
+// max_step pixels (a tile whose correlations are flat along its disparity has a tiny hd).
+__global__ __launch_bounds__(256) void k_disparity_step(const double* __restrict__ tstate, const double* __restrict__ de,
+                                                        const float* __restrict__ d_in, float* __restrict__ d_out, int num_tasks,
+                                                        double max_step) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    const double* ts = tstate + (size_t)t * 15;
+    double s = ts[0];
+    for (int i = 0; i < 13; i++) s += ts[2 + i] * de[i];
+    const double dd = fmin(fmax(-s / ts[1], -max_step), max_step);
+    d_out[t] = (float)((double)d_in[t] + dd);
 }
 
 // ====================================================================================

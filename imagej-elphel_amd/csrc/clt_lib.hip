@@ -139,6 +139,11 @@ struct clt_handle {
     double* d_gderiv = nullptr;    // [max_tasks][8][14]
     float* d_adj = nullptr;        // [cap_tasks][CORR_ADJ_STRIDE] correlation-peak loss and gradient
     bool adj_done = false;
+    clt_port_geometry pgeom{};     // host copy of the current port geometry
+    double* d_tstate = nullptr;    // [max_tasks][15] per-task LMA state
+    double* d_gpart = nullptr;     // [blocks][GEO_ADJ_SUMS]
+    double* d_gsum = nullptr;      // [GEO_ADJ_SUMS + 13] sums, then the extrinsic step
+    float* d_gdisp2 = nullptr;     // [max_tasks] candidate disparities
     bool port_geom_set = false, tasks_from_geom = false;
     // batched frames (clt_create_batch): host task lists per frame, flat device list with frames
     int32_t* d_task_frame = nullptr;               // [cap_tasks]
@@ -417,7 +422,7 @@ int clt_destroy(clt_handle* h) {
                     h->d_lpf_corr, h->d_rbg, h->d_corr, h->d_combo, h->d_disp, h->d_tex, h->d_tex_stats,
                     h->d_tex_img, h->d_extra, h->d_corr_args, h->d_tex_args, h->d_port_geom, h->d_rtab,
                     h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv, h->d_task_frame, h->d_staged, h->d_fcounts,
-                    h->d_adj};
+                    h->d_adj, h->d_tstate, h->d_gpart, h->d_gsum, h->d_gdisp2};
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -526,12 +531,7 @@ static void mat3(const double* a, const double* b, double* c) {
     std::memcpy(c, t, sizeof(t));
 }
 
-int clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p) {
-    if (!h) return CLT_E_ARG;
-    if (!p || p->num_sensors != 4 || h->g.num_cams != 4 || p->width <= 0 || p->height <= 0 || p->pixel_size <= 0 ||
-        p->distortion_radius <= 0 || p->focal_length <= 0)
-        return fail(h, CLT_E_ARG, "clt_set_port_geometry: needs a quad camera handle and positive sizes");
-    PortGeomDev pg;
+static void port_geom_dev(const clt_port_geometry* p, PortGeomDev& pg) {
     std::memset(&pg, 0, sizeof(pg));
     const double* v = p->extrinsic;
     for (int i = 0; i < 4; i++) {   // CorrVector angles (GC:1217-1269) and matrices (GC:1043-1147)
@@ -569,6 +569,17 @@ int clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p) {
     pg.rd_scale = 0.001 * p->pixel_size / p->distortion_radius;
     pg.half_w = 0.5 * p->width;
     pg.half_h = 0.5 * p->height;
+    pg.step_r = 0.001;
+    pg.tab_len = (int)std::ceil(2.0 / pg.step_r) + 1;
+}
+
+int clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p) {
+    if (!h) return CLT_E_ARG;
+    if (!p || p->num_sensors != 4 || h->g.num_cams != 4 || p->width <= 0 || p->height <= 0 || p->pixel_size <= 0 ||
+        p->distortion_radius <= 0 || p->focal_length <= 0)
+        return fail(h, CLT_E_ARG, "clt_set_port_geometry: needs a quad camera handle and positive sizes");
+    PortGeomDev pg;
+    port_geom_dev(p, pg);
     // reverse distortion table (calcReverseDistortionTable, GC:2707-2756): r / rD by Newton steps
     const double step = 0.001, max_r = 2.0;
     const int len = (int)std::ceil(max_r / step) + 1;
@@ -601,8 +612,6 @@ int clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p) {
             tab[i] = r / rd_target;
         }
     }
-    pg.step_r = step;
-    pg.tab_len = len;
     int rc;
     if ((rc = dalloc(h, (void**)&h->d_port_geom, sizeof(PortGeomDev))) ||
         (rc = dalloc(h, (void**)&h->d_rtab, sizeof(double) * len)))
@@ -610,6 +619,7 @@ int clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(h->d_port_geom, &pg, sizeof(pg), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_rtab, tab.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+    h->pgeom = *p;
     h->port_geom_set = true;
     return CLT_OK;
 }
@@ -1025,6 +1035,177 @@ int clt_get_corr_adjoint(clt_handle* h, float* out) {
     if (!h->adj_done) return fail(h, CLT_E_STATE, "clt_get_corr_adjoint: not executed");
     HIPCHK(h, hipStreamSynchronize(h->stream));
     HIPCHK(h, hipMemcpy(out, h->d_adj, (size_t)h->num_tasks * CORR_ADJ_STRIDE * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
+// ------------------------------------------------------------------------ geometry adjoint and LMA
+void clt_default_lma_params(clt_lma_params* p) {
+    if (!p) return;
+    p->lambda = 0.1;               // IDP lma_lambda_initial
+    p->lambda_scale_good = 0.5;    // IDP lma_lambda_scale_good
+    p->lambda_scale_bad = 8.0;     // IDP lma_lambda_scale_bad
+    p->lambda_max = 100.0;         // IDP lma_lambda_max
+    p->rms_diff = 1e-6;
+    p->num_iter = 10;
+    p->adjust_disparity = 1;
+}
+
+static int geo_prepare(clt_handle* h, const char* who) {
+    int rc;
+    if ((rc = check_adjoint_config(h, who))) return rc;
+    if (!h->port_geom_set || !h->tasks_from_geom)
+        return fail(h, CLT_E_STATE, "%s: tasks must come from clt_set_tasks_geometry", who);
+    if ((rc = prepare_corr(h))) return rc;
+    if (!h->corr_quad) return fail(h, CLT_E_UNSUPPORTED, "%s: needs the 6 reference quad pairs", who);
+    const int blocks = (h->max_tasks + GEO_ADJ_TASKS - 1) / GEO_ADJ_TASKS;
+    if ((rc = dalloc(h, (void**)&h->d_adj, (size_t)h->cap_tasks * CORR_ADJ_STRIDE * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_tstate, (size_t)h->max_tasks * 15 * 8)) ||
+        (rc = dalloc(h, (void**)&h->d_gpart, (size_t)blocks * GEO_ADJ_SUMS * 8)) ||
+        (rc = dalloc(h, (void**)&h->d_gsum, (size_t)(GEO_ADJ_SUMS + 13) * 8)) ||
+        (rc = dalloc(h, (void**)&h->d_gdisp2, (size_t)h->max_tasks * 4)))
+        return rc;
+    return CLT_OK;
+}
+
+// Port centres of the tasks at disparities `disp` with the current device geometry, then the loss
+// adjoint; with `sums`, the geometry adjoint with damping lambda and its reduction to the host.
+static int geo_eval(clt_handle* h, const float* disp, double lambda, double* sums) {
+    const int n = h->num_tasks;
+    hipLaunchKernelGGL(k_port_centres, dim3((4 * n + 255) / 256), dim3(256), 0, h->stream, h->d_port_geom, h->d_rtab,
+                       h->d_gwords, h->d_gtxy, disp, n, h->task_words, h->d_tasks, nullptr);
+    launch_corr_adjoint(h);
+    if (!sums) return CLT_OK;
+    const int blocks = (n + GEO_ADJ_TASKS - 1) / GEO_ADJ_TASKS;
+    hipLaunchKernelGGL(k_geometry_adjoint, dim3(blocks), dim3(128), 0, h->stream, h->d_port_geom, h->d_rtab, h->d_gwords,
+                       h->d_gtxy, disp, h->d_adj, n, lambda, h->d_gpart, h->d_tstate);
+    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(128), 0, h->stream, h->d_gpart, blocks, GEO_ADJ_SUMS, h->d_gsum);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemcpyAsync(sums, h->d_gsum, GEO_ADJ_SUMS * 8, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return CLT_OK;
+}
+
+static int upload_geometry(clt_handle* h, const clt_port_geometry& p) {
+    PortGeomDev pg;
+    port_geom_dev(&p, pg);
+    HIPCHK(h, hipMemcpyAsync(h->d_port_geom, &pg, sizeof(pg), hipMemcpyHostToDevice, h->stream));
+    return CLT_OK;
+}
+
+int clt_exec_geometry_adjoint(clt_handle* h, double* grad_extrinsic, float* grad_disparity, double* loss) {
+    if (!h) return CLT_E_ARG;
+    int rc;
+    if ((rc = geo_prepare(h, "clt_exec_geometry_adjoint"))) return rc;
+    double sums[GEO_ADJ_SUMS];
+    tbegin(h, T_GEOADJ);
+    if ((rc = geo_eval(h, h->d_gdisp, 0.0, sums))) return rc;
+    tend(h, T_GEOADJ);
+    h->adj_done = true;
+    if (grad_extrinsic)
+        for (int i = 0; i < 13; i++) grad_extrinsic[i] = sums[105 + i];
+    if (loss) *loss = sums[104];
+    if (grad_disparity && h->num_tasks > 0) {
+        std::vector<double> ts((size_t)h->num_tasks * 15);
+        HIPCHK(h, hipMemcpy(ts.data(), h->d_tstate, ts.size() * 8, hipMemcpyDeviceToHost));
+        for (int t = 0; t < h->num_tasks; t++) grad_disparity[t] = (float)ts[(size_t)t * 15];
+    }
+    return CLT_OK;
+}
+
+// Cholesky solve of the 13 x 13 system (A + lambda diag(A)) x = -b; false if not positive definite.
+static bool solve13(const double* sums, double lambda, double* x) {
+    double a[13][13], l[13][13] = {};
+    for (int i = 0, e = 0; i < 13; i++)
+        for (int j = i; j < 13; j++, e++) a[i][j] = a[j][i] = sums[e];
+    double dmax = 0.0;
+    for (int i = 0; i < 13; i++) dmax = std::max(dmax, a[i][i]);
+    for (int i = 0; i < 13; i++) a[i][i] += lambda * a[i][i] + 1e-12 * dmax + 1e-300;   // damping, gauge freedoms
+    for (int i = 0; i < 13; i++) {
+        for (int j = 0; j <= i; j++) {
+            double s = a[i][j];
+            for (int k = 0; k < j; k++) s -= l[i][k] * l[j][k];
+            if (i == j) {
+                if (!(s > 0.0)) return false;
+                l[i][i] = std::sqrt(s);
+            } else {
+                l[i][j] = s / l[j][j];
+            }
+        }
+    }
+    double y[13];
+    for (int i = 0; i < 13; i++) {
+        double s = -sums[91 + i];
+        for (int k = 0; k < i; k++) s -= l[i][k] * y[k];
+        y[i] = s / l[i][i];
+    }
+    for (int i = 12; i >= 0; i--) {
+        double s = y[i];
+        for (int k = i + 1; k < 13; k++) s -= l[k][i] * x[k];
+        x[i] = s / l[i][i];
+    }
+    return true;
+}
+
+int clt_lma_extrinsics(clt_handle* h, const clt_lma_params* prm, double* extrinsic_out, float* disparity_out,
+                       double* history, int32_t* iterations) {
+    if (!h || !prm || prm->num_iter < 0) return CLT_E_ARG;
+    int rc;
+    if ((rc = geo_prepare(h, "clt_lma_extrinsics"))) return rc;
+    const int n = h->num_tasks;
+    double lambda = prm->lambda;
+    clt_port_geometry cur = h->pgeom, cand = h->pgeom;
+    float* d_cur = h->d_gdisp;
+    float* d_new = h->d_gdisp2;
+    double sums[GEO_ADJ_SUMS];
+    tbegin(h, T_GEOADJ);
+    if ((rc = geo_eval(h, d_cur, lambda, sums))) return rc;
+    double loss = sums[104];
+    if (history) history[0] = loss;
+    int it = 0, hist_n = 1;
+    for (; it < prm->num_iter; it++) {
+        double de[13];
+        while (!solve13(sums, lambda, de)) {   // make the damped system positive definite
+            lambda *= prm->lambda_scale_bad;
+            if (lambda > prm->lambda_max) break;
+            if ((rc = geo_eval(h, d_cur, lambda, sums))) return rc;
+        }
+        if (lambda > prm->lambda_max) break;
+        for (int i = 0; i < 13; i++) cand.extrinsic[i] = cur.extrinsic[i] + de[i];
+        HIPCHK(h, hipMemcpyAsync(h->d_gsum + GEO_ADJ_SUMS, de, sizeof(de), hipMemcpyHostToDevice, h->stream));
+        if (prm->adjust_disparity)
+            hipLaunchKernelGGL(k_disparity_step, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_tstate,
+                               h->d_gsum + GEO_ADJ_SUMS, d_cur, d_new, n, 0.5);
+        else
+            HIPCHK(h, hipMemcpyAsync(d_new, d_cur, (size_t)n * 4, hipMemcpyDeviceToDevice, h->stream));
+        if ((rc = upload_geometry(h, cand)) || (rc = geo_eval(h, d_new, lambda * prm->lambda_scale_good, sums))) return rc;
+        const double new_loss = sums[104];
+        if (new_loss < loss) {   // accept: the system just computed is the next one
+            const double gain = (loss - new_loss) / std::max(std::fabs(loss), 1e-300);
+            cur = cand;
+            std::swap(d_cur, d_new);
+            loss = new_loss;
+            lambda *= prm->lambda_scale_good;
+            if (history) history[hist_n++] = loss;
+            if (gain < prm->rms_diff) { it++; break; }
+        } else {                 // reject: back to the current point with more damping
+            lambda *= prm->lambda_scale_bad;
+            if (lambda > prm->lambda_max) { it++; break; }
+            if ((rc = upload_geometry(h, cur)) || (rc = geo_eval(h, d_cur, lambda, sums))) return rc;
+            if (history) history[hist_n++] = loss;
+        }
+    }
+    tend(h, T_GEOADJ);
+    // leave the handle at the solution: geometry, disparities and task records
+    if ((rc = upload_geometry(h, cur)) || (rc = geo_eval(h, d_cur, 0.0, nullptr))) return rc;
+    if (d_cur != h->d_gdisp) HIPCHK(h, hipMemcpyAsync(h->d_gdisp, d_cur, (size_t)n * 4, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->pgeom = cur;
+    h->converted = h->corr_done = h->tex_done = h->combined = false;
+    h->adj_done = true;
+    if (extrinsic_out)
+        for (int i = 0; i < 13; i++) extrinsic_out[i] = cur.extrinsic[i];
+    if (disparity_out && n > 0) HIPCHK(h, hipMemcpy(disparity_out, h->d_gdisp, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (iterations) *iterations = hist_n - 1;
     return CLT_OK;
 }
 

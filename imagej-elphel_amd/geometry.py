@@ -78,3 +78,44 @@ def full_frame_tasks(width, height, disparity, rxy, flags=0xff, mask=None, **kw)
     tx, ty, d = tx[sel], ty[sel], d[sel]
     centers = port_centers(width, height, tx, ty, d, rxy, **kw)
     return make_tasks(tx, ty, centers, flags)
+
+
+def corr_rot_matrices(extrinsic, roll_deg=(0.0, 0.0, 0.0, 0.0)):
+    """Per sensor rotation/zoom matrices [4][3][3] of a CorrVector (GC:1043-1076): tilt0..2,
+    azimuth0..2, roll0..3, zoom0..2; sensor 3 tilt, azimuth and zoom are minus the sum of the
+    others; factory rolls in degrees are added."""
+    v = np.asarray(extrinsic, np.float64)
+    out = np.zeros((4, 3, 3))
+    for i in range(4):
+        tl = v[i] if i < 3 else -(v[0] + v[1] + v[2])
+        az = v[3 + i] if i < 3 else -(v[3] + v[4] + v[5])
+        rl = v[6 + i] + np.deg2rad(roll_deg[i])
+        zm = 1.0 + (v[10 + i] if i < 3 else -(v[10] + v[11] + v[12]))
+        ca, sa, ct, st, cr, sr = np.cos(az), np.sin(az), np.cos(tl), np.sin(tl), np.cos(rl), np.sin(rl)
+        m_az = np.array([[ca, 0, -sa], [0, 1, 0], [sa, 0, ca]])
+        m_t = np.array([[1, 0, 0], [0, ct, st], [0, -st, ct]])
+        m_r = np.array([[cr * zm, sr * zm, 0], [-sr * zm, cr * zm, 0], [0, 0, 1]])
+        out[i] = m_r @ (m_t @ m_az)
+    return out
+
+
+def port_map(px, py, disparity, rxy, rots, focal_pix, width, height, pxy0=None):
+    """Undistorted port model at arbitrary points: pixel (px, py) of the virtual centre camera at
+    disparity d seen by sensor c at [..][C][2] (GC:2181-2231 without radial distortion)."""
+    px = np.asarray(px, np.float64)
+    d = np.broadcast_to(np.asarray(disparity, np.float64), px.shape)
+    rxy = np.asarray(rxy, np.float64)
+    C = rxy.shape[0]
+    if pxy0 is None:
+        pxy0 = np.tile([0.5 * width, 0.5 * height], (C, 1))
+    x = (px - 0.5 * width)[..., None] - d[..., None] * rxy[:, 0]
+    y = (np.asarray(py, np.float64) - 0.5 * height)[..., None] - d[..., None] * rxy[:, 1]
+    f = float(focal_pix)
+    r = np.asarray(rots, np.float64)
+    rx = r[:, 0, 0] * x + r[:, 0, 1] * y + r[:, 0, 2] * f
+    ry = r[:, 1, 0] * x + r[:, 1, 1] * y + r[:, 1, 2] * f
+    rz = r[:, 2, 0] * x + r[:, 2, 1] * y + r[:, 2, 2] * f
+    out = np.empty(x.shape + (2,))
+    out[..., 0] = rx * f / rz + pxy0[:, 0]
+    out[..., 1] = ry * f / rz + pxy0[:, 1]
+    return out

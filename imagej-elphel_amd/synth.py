@@ -255,3 +255,48 @@ def make_scene(width, height, num_cams=4, num_colors=3, rxy=None, disparity=3.0,
                  red_row=red_row, red_col=red_col, images=images, kernels=kern, offsets=offs,
                  tasks=tasks, disparity=d_tiles, kernels_hor=kh, kernels_vert=kv, seed=seed,
                  mask=mask)
+
+
+class GeometryScene:
+    """A quad scene rendered through the port geometry with given extrinsics (no aberrations)."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def make_geometry_scene(width, height, extrinsic, seed=0, disparity=("field", 1.0, 4.0), focal_pix=4.5 / 0.0022,
+                        red_row=0, red_col=1, noise=False):
+    """Camera c pixel q shows the master texture at the point p with port_map(p, d(p))_c = q
+    (solved by fixed-point steps); d(p) a smooth per-tile field, bilinear between tile centres."""
+    from scipy.ndimage import map_coordinates, zoom
+    rng = np.random.default_rng(seed)
+    tiles_y, tiles_x = height // 8, width // 8
+    if isinstance(disparity, tuple):
+        d_t = smooth_field(tiles_y, tiles_x, rng, disparity[1], disparity[2])
+    else:
+        d_t = np.full((tiles_y, tiles_x), float(disparity))
+    rxy = np.array(geometry.QUAD_RXY, np.float64)
+    rots = geometry.corr_rot_matrices(extrinsic)
+    m = 24
+    H2, W2 = height + 2 * m, width + 2 * m
+    base = texture_1f(H2, W2, rng)
+    gains = (0.85, 0.65, 1.0)
+    planes = [np.clip(g * base + 0.12 * (texture_1f(H2, W2, rng) - 2047.5), 0, 4095) for g in gains]
+    dpix = zoom(d_t, 8, order=1, mode="nearest")[:height, :width]
+    yy, xx = np.mgrid[0:height, 0:width].astype(np.float64)
+    colors = bayer_colors(height, width, red_row, red_col)
+    images = np.empty((4, height, width), np.float32)
+    for c in range(4):
+        px, py = xx + dpix * rxy[c, 0], yy + dpix * rxy[c, 1]
+        for _ in range(4):
+            dp = map_coordinates(d_t, [(py - 4) / 8, (px - 4) / 8], order=1, mode="nearest")
+            q = geometry.port_map(px, py, dp, rxy[c:c + 1], rots[c:c + 1], focal_pix, width, height)[..., 0, :]
+            px, py = px + (xx - q[..., 0]), py + (yy - q[..., 1])
+        samp = [map_coordinates(pl, [py + m, px + m], order=3, mode="nearest") for pl in planes]
+        img = np.clip(np.choose(colors, samp), 0, None)
+        if noise:
+            img = rng.poisson(img).astype(np.float64)
+        images[c] = np.where(colors == 2, img * 0.5, img)
+    ty, tx = np.mgrid[0:tiles_y, 0:tiles_x]
+    return GeometryScene(width=width, height=height, images=images, extrinsic=np.asarray(extrinsic, np.float64),
+                         disparity_tiles=d_t, tx=tx.ravel(), ty=ty.ravel(), focal_pix=focal_pix, rots=rots)

@@ -303,6 +303,32 @@ class GPUTileProcessor:
         return {"loss": out[:, 0], "c": out[:, 1:7], "kx": out[:, 7:13], "ky": out[:, 13:19],
                 "dxy": out[:, 19:27].reshape(-1, 4, 2)}
 
+    def getGeometryGradient(self):
+        """(dl/d extrinsics [13] float64, dl/d disparity [n] float32, loss) of the tasks set by
+        getTileSubcamOffsets."""
+        ge = np.zeros(13, np.float64)
+        gd = np.zeros(self.num_task_tiles, np.float32)
+        loss = ctypes.c_double()
+        self._chk(self.lib.clt_exec_geometry_adjoint(self.h, _ptr(ge), _ptr(gd), ctypes.byref(loss)))
+        return ge, gd, loss.value
+
+    def lmaExtrinsics(self, num_iter=10, adjust_disparity=True, **kw):
+        """LMA of the extrinsics (and disparities) on the correlation-peak loss; returns
+        (extrinsics [13], disparities [n], loss history)."""
+        p = _abi.clt_lma_params()
+        self.lib.clt_default_lma_params(ctypes.byref(p))
+        p.num_iter = num_iter
+        p.adjust_disparity = int(adjust_disparity)
+        for k, v in kw.items():
+            setattr(p, "lambda_" if k == "lambda" else k, v)
+        ext = np.zeros(13, np.float64)
+        disp = np.zeros(self.num_task_tiles, np.float32)
+        hist = np.full(num_iter + 1, np.nan)
+        its = ctypes.c_int32()
+        self._chk(self.lib.clt_lma_extrinsics(self.h, ctypes.byref(p), _ptr(ext), _ptr(disp), _ptr(hist),
+                                              ctypes.byref(its)))
+        return ext, disp, hist[:its.value + 1]
+
     # ------------------------------------------------------------------ execution control
     def setAsync(self, on):
         self._chk(self.lib.clt_set_async(self.h, int(on)))

@@ -224,10 +224,37 @@ int  clt_get_port_derivatives(clt_handle* h, double* out);
  * Quad Bayer handle, the 6 reference pairs, fat_zero 0.  Per task CLT_CORR_ADJ floats:
  * [0] loss, [1..6] C_p, [7..12] and [13..18] the curvatures -d2 C_p / d tau^2 under a relative
  * shift of the pair along x and y, [19..26] dl/d(x0, y0, x1, y1, x2, y2, x3, y3), rest 0.
- * A task with task word 0 gets NaN loss and zero gradient. */
+ * A task with task word 0, or with a window of any camera outside the frame, gets NaN loss and
+ * zero gradient (and is left out of the sums below). */
 #define CLT_CORR_ADJ 32
 int  clt_exec_corr_adjoint(clt_handle* h);
 int  clt_get_corr_adjoint(clt_handle* h, float* out);
+
+/* Gradient of the summed loss of the current tasks (set by clt_set_tasks_geometry) by the 13
+ * extrinsic parameters (CorrVector, GC:1004-1011) and by each task's disparity, through the port
+ * geometry derivatives (GC:2235-2306).  Any output may be NULL. */
+int  clt_exec_geometry_adjoint(clt_handle* h, double* grad_extrinsic, float* grad_disparity, double* loss);
+
+/* Levenberg-Marquardt over the extrinsics (and the task disparities) minimising the summed loss
+ * (the role of the reference's extrinsic LMA, AlignmentCorrection.java:3346-3452): gradient from
+ * the adjoint; Gauss-Newton matrix from the port-centre derivatives weighted by the per-pair
+ * curvatures; the disparities are eliminated per task (Schur complement) so the host solves one
+ * 13 x 13 system in double precision per step. */
+typedef struct clt_lma_params {
+    double  lambda;             /* initial damping 0.1           (IDP lma_lambda_initial)        */
+    double  lambda_scale_good;  /* 0.5                           (IDP lma_lambda_scale_good)     */
+    double  lambda_scale_bad;   /* 8.0                           (IDP lma_lambda_scale_bad)      */
+    double  lambda_max;         /* 100                           (IDP lma_lambda_max)            */
+    double  rms_diff;           /* stop below this relative loss improvement                     */
+    int32_t num_iter;           /* iterations                                                    */
+    int32_t adjust_disparity;   /* 1: the task disparities are parameters too                    */
+} clt_lma_params;
+void clt_default_lma_params(clt_lma_params* p);
+/* Runs from the handle's port geometry and task disparities and leaves the handle at the solution
+ * (geometry, disparities, task records).  extrinsic_out [13], disparity_out [num_tasks],
+ * history [num_iter + 1] losses (initial, then after each iteration); any may be NULL. */
+int  clt_lma_extrinsics(clt_handle* h, const clt_lma_params* p, double* extrinsic_out, float* disparity_out,
+                        double* history, int32_t* iterations);
 
 /* ---- batched frames (no reference counterpart) -------------------------------------------------
  * One handle holds num_frames frames (camera sets of the same geometry, sharing the kernels and
