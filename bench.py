@@ -226,6 +226,46 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
             "plain_gpu_ms": round(plain_gpu, 3), "tune": tune}
 
 
+def adjoint_bench(pk, sc, repeats=5, seed=11):
+    """Differentiable form on the same frame: forward (convert + correlate2D) and the adjoint
+    (port centres from the geometry, the fused convert/loss adjoint, the geometry adjoint and the
+    LMA system reduction) per pass, then 10 LMA iterations of the extrinsics and disparities
+    from extrinsics perturbed N(0, 3e-4)."""
+    T = len(sc.tasks)
+    tx, ty = sc.tasks[:, 1] & 0xFFFF, (sc.tasks[:, 1] >> 16) & 0xFFFF
+    d = sc.disparity[ty, tx].astype(np.float32)
+    words = np.full(T, 0xFF, np.int32)
+    tp = pk.GPUTileProcessor(W, H, C, K)
+    tp.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
+    tp.setBayerImages(sc.images, force=True)
+    tp.setCorrSigma(0.8)
+    tp.setPortGeometry(pk.GPUTileProcessor.defaultPortGeometry(W, H))
+    tp.getTileSubcamOffsets(words, tx, ty, d)
+    tp.execConverCorrectTiles(0)
+    tp.execCorrelate2D()
+    tp.getGeometryGradient()
+    tp.setTiming(True)
+    tp.resetTiming()
+    for _ in range(repeats):
+        tp.execConverCorrectTiles(0)
+        tp.execCorrelate2D()
+        tp.getGeometryGradient()
+    fwd = (tp.getTiming("convert")[0] + tp.getTiming("corr2d")[0]) / repeats
+    adj = tp.getTiming("geometry_adjoint")[0] / repeats
+    tp.setTiming(False)
+    rng = np.random.default_rng(seed)
+    tp.setPortGeometry(pk.GPUTileProcessor.defaultPortGeometry(W, H, extrinsic=list(rng.normal(0, 3e-4, 13))))
+    tp.getTileSubcamOffsets(words, tx, ty, d)
+    t0 = time.perf_counter()
+    _, _, hist = tp.lmaExtrinsics(10)
+    lma_s = time.perf_counter() - t0
+    tp.close()
+    return {"tasks": T, "forward_ms": round(fwd, 4), "adjoint_ms": round(adj, 4),
+            "forward_tiles_per_s": round(T / (fwd * 1e-3), 1), "adjoint_tiles_per_s": round(T / (adj * 1e-3), 1),
+            "lma_iterations": len(hist) - 1, "lma_ms": round(1e3 * lma_s, 2),
+            "lma_loss": [round(float(hist[0]), 2), round(float(hist[-1]), 2)]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +277,7 @@ def main():
                     help="run one device-to-device copy of MB MiB first (HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-frames comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/LMA timing")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -302,6 +343,10 @@ def main():
     if not args.no_batch:
         print("bench: batched frames ...", file=sys.stderr, flush=True)
         batch = batch_bench(pk, sc)
+    adjoint = None
+    if not args.no_adjoint:
+        print("bench: adjoint and LMA ...", file=sys.stderr, flush=True)
+        adjoint = adjoint_bench(pk, sc)
     if traffic is None:
         roof["traffic_note"] = traffic_info
     else:
@@ -321,7 +366,7 @@ def main():
         "config": {"workload": "quad 2592x1936 Bayer frame, 78408 tiles, dense tasks, disparity 0-20 px; "
                                "convert+correlate2D(6 pairs)+textures(RGBA)+imclt_rbg",
                    "tiles": T, "cameras": C, "colors": K, "pairs": 6},
-        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu, "batch": batch,
+        "roofline": roof, "kernels": kernels, "cpu_baseline": cpu, "batch": batch, "adjoint": adjoint,
     }
     print(json.dumps(out))
 
