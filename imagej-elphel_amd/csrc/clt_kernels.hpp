@@ -89,6 +89,23 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
         ce = koffsets + kidx * 8;
         kt = kernels + kidx * 256;
     }
+    // aberration kernel of this lane's coefficients, loaded early so the load overlaps the window
+    // loads and the transforms
+    float k0[8], k1[8], k2[8], k3[8];
+    if (kt) {
+        const float* kq = kt + (l & 7) * 8;
+#pragma unroll
+        for (int k = 0; k < 8; k += 4) {
+            float4 v0 = *reinterpret_cast<const float4*>(kq + k);
+            float4 v1 = *reinterpret_cast<const float4*>(kq + 64 + k);
+            float4 v2 = *reinterpret_cast<const float4*>(kq + 128 + k);
+            float4 v3 = *reinterpret_cast<const float4*>(kq + 192 + k);
+            k0[k] = v0.x; k0[k + 1] = v0.y; k0[k + 2] = v0.z; k0[k + 3] = v0.w;
+            k1[k] = v1.x; k1[k + 1] = v1.y; k1[k + 2] = v1.z; k1[k + 3] = v1.w;
+            k2[k] = v2.x; k2[k + 1] = v2.y; k2[k + 2] = v2.z; k2[k + 3] = v2.w;
+            k3[k] = v3.x; k3[k + 1] = v3.y; k3[k + 2] = v3.z; k3[k + 3] = v3.w;
+        }
+    }
     const TilePos pos = tile_position(cx, cy, ce, g.kstep, ktx, kty);
     const int left = pos.left, top = pos.top;
     const float rx = pos.rx, ry = pos.ry;
@@ -114,13 +131,25 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
         }
         const bool inside = (left >= 0) && (left + 16 <= g.width) && (top >= 0) && (top + 16 <= g.height);
         if (__all(inside || !active)) {
-            // (an inactive lane reads pixel (0, 0) 16 times)
+            // (an inactive lane reads pixel (0, 0))
             const float* src = img + (active ? (size_t)top * g.width + pc : 0);
             const size_t stride = active ? (size_t)g.width : 0;
+            if (bayer && color != 2) {   // red / blue: only the 8 rows of this colour's parity are read
+                const int rsel = (((color == 0) ? g.red_row : (g.red_row ^ 1)) ^ top) & 1;
+                const bool keep = rsel ? keep_o : keep_e;
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const float v = src[r * stride];
-                sb[r * CONV_ROW + l] = ((r & 1) ? keep_o : keep_e) ? v : 0.f;
+                for (int i = 0; i < 8; i++) {
+                    const int rr = 2 * i + rsel;
+                    const float v = src[rr * stride];
+                    sb[rr * CONV_ROW + l] = keep ? v : 0.f;
+                    sb[(rr ^ 1) * CONV_ROW + l] = 0.f;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const float v = src[r * stride];
+                    sb[r * CONV_ROW + l] = ((r & 1) ? keep_o : keep_e) ? v : 0.f;
+                }
             }
         } else {
             const int pcc = clamp_parity(pc, g.width);
@@ -197,23 +226,11 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
 
     // ---- aberration kernel multiply (ImageDtt.java:5429-5488), partner lane holds modes hs^1
     if (kt) {
-        float pa[8], pb[8], k0[8], k1[8], k2[8], k3[8];
+        float pa[8], pb[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             pa[k] = lane_xor8(a[k]) * sigma;
             pb[k] = lane_xor8(b[k]) * sigma;
-        }
-        const float* kq = kt + kh * 8;
-#pragma unroll
-        for (int k = 0; k < 8; k += 4) {
-            float4 v0 = *reinterpret_cast<const float4*>(kq + k);
-            float4 v1 = *reinterpret_cast<const float4*>(kq + 64 + k);
-            float4 v2 = *reinterpret_cast<const float4*>(kq + 128 + k);
-            float4 v3 = *reinterpret_cast<const float4*>(kq + 192 + k);
-            k0[k] = v0.x; k0[k + 1] = v0.y; k0[k + 2] = v0.z; k0[k + 3] = v0.w;
-            k1[k] = v1.x; k1[k + 1] = v1.y; k1[k + 2] = v1.z; k1[k + 3] = v1.w;
-            k2[k] = v2.x; k2[k + 1] = v2.y; k2[k + 2] = v2.z; k2[k + 3] = v2.w;
-            k3[k] = v3.x; k3[k + 1] = v3.y; k3[k + 2] = v3.z; k3[k + 3] = v3.w;
         }
 #pragma unroll
         for (int k = 0; k < 8; k++) {
