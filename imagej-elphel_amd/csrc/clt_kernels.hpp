@@ -963,6 +963,7 @@ __device__ __forceinline__ int argmax16(float v, int idx) {
     return idx;
 }
 
+template <bool FZ>   // FZ: fat zero in use (the per-bin normaliser needs the mean over the bins)
 __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                      int task_words, int num_tasks, const float* __restrict__ lpf,
                                                      const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
@@ -1002,7 +1003,7 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
         for (int col = 0; col < 3; col++)
 #pragma unroll
             for (int n = 0; n < 4; n++) D[c][col][n] = clt[(((size_t)c * ntiles + tile) * 3 + col) * 256 + n * 64 + lane];
-    const bool fz = A.fat_zero != 0.f;
+    constexpr bool fz = FZ;
     float nrm[4][3];   // per bin sum over the 4 modes of the squares (C2:205-212)
 #pragma unroll
     for (int c = 0; c < 4; c++)

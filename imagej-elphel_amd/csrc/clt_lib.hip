@@ -255,7 +255,8 @@ static void launch_imclt(clt_handle* h, int frames, int apply_lpf) {
 static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
     if (h->corr_quad) {
         const int tpb = h->tune_corr_tpb;
-        hipLaunchKernelGGL(k_corr2d_quad, dim3((n + tpb - 1) / tpb), dim3(64 * tpb), (size_t)tpb * CQ_WAVE_FLOATS * 4,
+        auto kern = (h->cp.fat_zero != 0.f) ? k_corr2d_quad<true> : k_corr2d_quad<false>;
+        hipLaunchKernelGGL(kern, dim3((n + tpb - 1) / tpb), dim3(64 * tpb), (size_t)tpb * CQ_WAVE_FLOATS * 4,
                            h->stream, h->d_clt, h->d_tasks, h->task_words, n, h->d_lpf_corr, h->d_corr_args, h->d_corr,
                            h->d_combo, h->d_disp, task_frame, h->clt_fstride, h->gd);
     } else {
