@@ -12,11 +12,12 @@ LIB_PATH = os.path.join(HERE, "libclt_hip.so")
 CLT_OK = 0
 ERRORS = {-1: "CLT_E_ARG", -2: "CLT_E_STATE", -3: "CLT_E_HIP", -4: "CLT_E_NODEVICE",
           -5: "CLT_E_NOMEM", -6: "CLT_E_UNSUPPORTED"}
-DISP_LAYERS = 8
+DISP_LAYERS = 12
 CORR_ADJ = 32
-DISP_NAMES = ["int", "cm", "strength", "hor", "hor_strength", "vert", "vert_strength", "cm_weight"]
+DISP_NAMES = ["int", "cm", "strength", "hor", "hor_strength", "vert", "vert_strength", "cm_weight",
+              "poly", "poly_strength", "cm_raw", "strength_raw"]
 TIMING_KERNELS = {"convert": 0, "imclt": 1, "corr2d": 2, "textures": 3, "combine_textures": 4, "batch": 5,
-                  "corr_adjoint": 6, "geometry_adjoint": 7}
+                  "corr_adjoint": 6, "geometry_adjoint": 7, "corr_lma": 8}
 
 
 class clt_geom(ctypes.Structure):
@@ -36,6 +37,25 @@ class clt_corr_params(ctypes.Structure):
                 ("twice_diagonal", ctypes.c_int32), ("min_corr", ctypes.c_float),
                 ("strip_height", ctypes.c_int32), ("enhortho_width", ctypes.c_float),
                 ("enhortho_scale", ctypes.c_float)]
+
+
+class clt_corr_lma_params(ctypes.Structure):
+    _fields_ = [("enable", ctypes.c_int32), ("mix_corr_poly", ctypes.c_int32),
+                ("min_poly_strength", ctypes.c_double), ("ortho_vasw_pwr", ctypes.c_double),
+                ("cnvx_hwnd_size", ctypes.c_int32), ("cnvx_add3x3", ctypes.c_int32),
+                ("cnvx_weight", ctypes.c_double),
+                ("lma_adjust_wm", ctypes.c_int32), ("lma_adjust_wy", ctypes.c_int32),
+                ("lma_adjust_wxy", ctypes.c_int32), ("lma_adjust_ag", ctypes.c_int32),
+                ("lma_half_width", ctypes.c_double), ("lma_cost_wy", ctypes.c_double),
+                ("lma_cost_wxy", ctypes.c_double), ("lma_lambda_initial", ctypes.c_double),
+                ("lma_lambda_scale_good", ctypes.c_double), ("lma_lambda_scale_bad", ctypes.c_double),
+                ("lma_lambda_max", ctypes.c_double), ("lma_rms_diff", ctypes.c_double),
+                ("lma_num_iter", ctypes.c_int32), ("fo_correct", ctypes.c_int32),
+                ("fo_far", ctypes.c_int32), ("fo_ortho", ctypes.c_int32),
+                ("fo_min_strength", ctypes.c_double), ("fo_min_eff", ctypes.c_double),
+                ("fo_min_eff_ratio", ctypes.c_double), ("fo_max_hwidth", ctypes.c_double),
+                ("fo_min_diff", ctypes.c_double), ("fo_overcorrection", ctypes.c_double),
+                ("fo_lim_overcorr", ctypes.c_double)]
 
 
 class clt_tex_params(ctypes.Structure):
@@ -78,9 +98,11 @@ SIGNATURES = {
     "clt_set_pairs": (_I, [_H, _P, _I]),
     "clt_set_corr_params": (_I, [_H, ctypes.POINTER(clt_corr_params)]),
     "clt_set_tex_params": (_I, [_H, ctypes.POINTER(clt_tex_params)]),
+    "clt_set_corr_lma_params": (_I, [_H, ctypes.POINTER(clt_corr_lma_params)]),
     "clt_lpf_coeffs": (_I, [ctypes.c_double, _P]),
     "clt_default_corr_params": (None, [ctypes.POINTER(clt_corr_params)]),
     "clt_default_tex_params": (None, [ctypes.POINTER(clt_tex_params), _I]),
+    "clt_default_corr_lma_params": (None, [ctypes.POINTER(clt_corr_lma_params)]),
     "clt_default_quad_pairs": (_I, [_P]),
     "clt_exec_convert": (_I, [_H, _I]),
     "clt_exec_imclt": (_I, [_H, _I]),

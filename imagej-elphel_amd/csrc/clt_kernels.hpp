@@ -616,7 +616,7 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
             if (lane < CLT_DISP_LAYERS && disp_out)
                 disp_out[(size_t)t * CLT_DISP_LAYERS + lane] =
                     (lane == CLT_DISP_STRENGTH || lane == CLT_DISP_HOR_STR || lane == CLT_DISP_VERT_STR ||
-                     lane == CLT_DISP_CM_WEIGHT) ? 0.f : qnan;
+                     lane == CLT_DISP_CM_WEIGHT || lane == CLT_DISP_STR_RAW) ? 0.f : qnan;
         }
         return;
     }
@@ -856,6 +856,7 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
         float res[CLT_DISP_LAYERS];
         for (int i = 0; i < CLT_DISP_LAYERS; i++) res[i] = qnan;
         res[CLT_DISP_STRENGTH] = res[CLT_DISP_HOR_STR] = res[CLT_DISP_VERT_STR] = res[CLT_DISP_CM_WEIGHT] = 0.f;
+        res[CLT_DISP_STR_RAW] = 0.f;
         if (have_max) {
             res[CLT_DISP_INT] = (float)(-ix);
             res[CLT_DISP_STRENGTH] = s_combo[ix + 7];
@@ -880,6 +881,8 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
             res[which ? CLT_DISP_VERT : CLT_DISP_HOR] = -(xc - 7.f);
             res[which ? CLT_DISP_VERT_STR : CLT_DISP_HOR_STR] = v[icr] / 8.f;
         }
+        res[CLT_DISP_CM_RAW] = res[CLT_DISP_CM];   // kept for the LMA stage (k_corr_lma)
+        res[CLT_DISP_STR_RAW] = res[CLT_DISP_STRENGTH];
         float* dout = disp_out + (size_t)t * CLT_DISP_LAYERS;
         for (int i = 0; i < CLT_DISP_LAYERS; i++) dout[i] = res[i];
     }
@@ -989,7 +992,7 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
         if (lane < CLT_DISP_LAYERS && disp_out)
             disp_out[(size_t)t * CLT_DISP_LAYERS + lane] =
                 (lane == CLT_DISP_STRENGTH || lane == CLT_DISP_HOR_STR || lane == CLT_DISP_VERT_STR ||
-                 lane == CLT_DISP_CM_WEIGHT) ? 0.f : qnan;
+                 lane == CLT_DISP_CM_WEIGHT || lane == CLT_DISP_STR_RAW) ? 0.f : qnan;
         return;
     }
     const int txy = rec[1];
@@ -1227,13 +1230,17 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
         float r = qnan;
         switch (lane) {
         case CLT_DISP_INT: r = have_max ? (float)(-ix) : qnan; break;
-        case CLT_DISP_CM: r = (have_max && s0 != 0.f) ? -(ix + sx / s0 / 2.f) : qnan; break;
-        case CLT_DISP_STRENGTH: r = have_max ? vmax : 0.f; break;
+        case CLT_DISP_CM:
+        case CLT_DISP_CM_RAW: r = (have_max && s0 != 0.f) ? -(ix + sx / s0 / 2.f) : qnan; break;
+        case CLT_DISP_STRENGTH:
+        case CLT_DISP_STR_RAW: r = have_max ? vmax : 0.f; break;
+        case CLT_DISP_POLY:
+        case CLT_DISP_POLY_STR: r = qnan; break;
         case CLT_DISP_HOR: r = hor; break;
         case CLT_DISP_HOR_STR: r = hor_s; break;
         case CLT_DISP_VERT: r = vert; break;
         case CLT_DISP_VERT_STR: r = vert_s; break;
-        default: r = have_max ? s0 : 0.f; break;
+        default: r = have_max ? s0 : 0.f; break;   // CLT_DISP_CM_WEIGHT
         }
         disp_out[(size_t)t * CLT_DISP_LAYERS + lane] = r;
     }

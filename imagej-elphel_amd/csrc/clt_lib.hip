@@ -14,6 +14,7 @@
 
 #include "../../include/clt_tile_processor.h"
 #include "clt_kernels.hpp"
+#include "clt_corr_lma.hpp"
 
 namespace clt {
 __constant__ Tables c_tab;
@@ -101,7 +102,7 @@ static void half_flat_top(int n, double hwidth, double blur, bool normalize, boo
 }
 
 // ------------------------------------------------------------------------ handle
-enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_NUM };
+enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_CORR_LMA, T_NUM };
 
 struct clt_handle {
     clt_geom g{};
@@ -161,6 +162,11 @@ struct clt_handle {
     bool tuned = false;
     bool corr_quad = false;        // correlate2D runs the quad kernel
     int corr_num_strip = 0;
+    clt_corr_lma_params lp{};      // per-tile LMA chain parameters
+    LmaArgs lma_args{};            // ... as launched (built by prepare_corr)
+    bool lma_run = false;          // correlate2D is followed by k_corr_lma
+    int graph_epoch = 0;           // bumped whenever a captured launch would go stale
+    int graph_epoch_cap = -1;
     CorrArgs* d_corr_args = nullptr;
     TexArgs* d_tex_args = nullptr;
     size_t corr_cap = 0, combo_cap = 0, tex_cap = 0, tex_img_cap = 0;
@@ -267,6 +273,14 @@ static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
     }
 }
 
+// Per-tile LMA chain over the records just written by correlate2D (16 tiles per 256-thread block).
+static void launch_corr_lma(clt_handle* h, int n) {
+    if (!h->lma_run) return;
+    const int slots = LMA_SLOTS;
+    hipLaunchKernelGGL(k_corr_lma, dim3((n + slots - 1) / slots), dim3(64 * LMA_WAVES), 0, h->stream, h->d_corr,
+                       h->d_disp, n, h->lma_args);
+}
+
 template <int NC>
 static void launch_tex_nc(clt_handle* h, int n, const int32_t* task_frame, const float* extra) {
     hipLaunchKernelGGL(k_textures<NC>, dim3(n), dim3(256), 0, h->stream, h->d_clt, h->d_tasks, h->task_words, n,
@@ -303,6 +317,40 @@ void clt_default_corr_params(clt_corr_params* p) {
     p->strip_height = 9;
     p->enhortho_width = 2.0f;
     p->enhortho_scale = 0.0f;
+}
+
+void clt_default_corr_lma_params(clt_corr_lma_params* p) {   // ImageDttParameters IDP:32-110
+    std::memset(p, 0, sizeof(*p));
+    p->enable = 1;
+    p->mix_corr_poly = 1;
+    p->min_poly_strength = 0.2;
+    p->ortho_vasw_pwr = 2.0;
+    p->cnvx_hwnd_size = 4;
+    p->cnvx_add3x3 = 1;
+    p->cnvx_weight = 0.5;
+    p->lma_adjust_wm = 1;
+    p->lma_adjust_wy = 0;
+    p->lma_adjust_wxy = 1;
+    p->lma_adjust_ag = 1;
+    p->lma_half_width = 2.0;
+    p->lma_cost_wy = 0.1;
+    p->lma_cost_wxy = 0.1;
+    p->lma_lambda_initial = 0.1;
+    p->lma_lambda_scale_good = 0.5;
+    p->lma_lambda_scale_bad = 8.0;
+    p->lma_lambda_max = 100.0;
+    p->lma_rms_diff = 0.001;
+    p->lma_num_iter = 20;
+    p->fo_correct = 1;
+    p->fo_far = 0;
+    p->fo_ortho = 1;
+    p->fo_min_strength = 0.1;
+    p->fo_min_eff = 0.25;
+    p->fo_min_eff_ratio = 0.65;
+    p->fo_max_hwidth = 3.0;
+    p->fo_min_diff = 0.2;
+    p->fo_overcorrection = 0.0;
+    p->fo_lim_overcorr = 2.0;
 }
 
 void clt_default_tex_params(clt_tex_params* p, int32_t num_cams) {
@@ -366,6 +414,7 @@ static int create_handle(const clt_geom* geom, int frames, clt_handle** out) {
     h->gd = GeomDev{g.width, g.height, g.num_cams, g.num_colors, g.red_row, g.red_col,
                     g.kernels_hor, g.kernels_vert, g.kernel_step, h->tiles_x, h->tiles_y};
     clt_default_corr_params(&h->cp);
+    clt_default_corr_lma_params(&h->lp);
     clt_default_tex_params(&h->tp, g.num_cams);
     h->pairs.resize(0);
     if (g.num_cams == 4) {
@@ -722,6 +771,7 @@ int clt_set_pairs(clt_handle* h, const clt_pair* pairs, int32_t num_pairs) {
     if (nstrip > CORR_MAX_STRIP_PAIRS) return fail(h, CLT_E_UNSUPPORTED, "clt_set_pairs: %d oriented pairs > %d", nstrip, CORR_MAX_STRIP_PAIRS);
     h->pairs.assign(pairs, pairs + num_pairs);
     h->corr_done = false;
+    h->graph_epoch++;
     return CLT_OK;
 }
 
@@ -729,6 +779,18 @@ int clt_set_corr_params(clt_handle* h, const clt_corr_params* p) {
     if (!h || !p) return CLT_E_ARG;
     if (p->strip_height < 1 || p->strip_height > 16) return fail(h, CLT_E_ARG, "strip_height %d", p->strip_height);
     h->cp = *p;
+    h->graph_epoch++;   // the correlation kernel variant follows fat_zero
+    return CLT_OK;
+}
+
+int clt_set_corr_lma_params(clt_handle* h, const clt_corr_lma_params* p) {
+    if (!h || !p) return CLT_E_ARG;
+    if (p->cnvx_hwnd_size < 1 || p->cnvx_hwnd_size > 4 || p->lma_num_iter < 0 || !(p->lma_lambda_initial >= 0) ||
+        !(p->lma_half_width > 0))
+        return fail(h, CLT_E_ARG, "clt_set_corr_lma_params: half window %d (1..4), %d iterations", p->cnvx_hwnd_size,
+                    p->lma_num_iter);
+    h->lp = *p;
+    h->graph_epoch++;   // the LMA arguments are launch arguments
     return CLT_OK;
 }
 
@@ -778,9 +840,82 @@ static int ensure_capacity(clt_handle* h, float** p, size_t* cap, size_t floats)
         (void)hipFree(*p);
         *p = nullptr;
     }
+    h->graph_epoch++;   // a captured graph holds the old address
     int rc = dalloc(h, (void**)p, floats * 4);
     if (rc == CLT_OK) *cap = floats;
     return rc;
+}
+
+// LMA launch arguments for the current pairs: fit 0 over every oriented pair, fits 1..4 over the
+// horizontal, vertical, main- and other-diagonal pairs (corr4dirsLMA, C2:1591-1630); each fit's pairs
+// grouped by {diagonal, scale} in the order of GROUPS (C2:72-78) and combined as
+// combineCompatiblePairs does (C2:462-488).
+static int build_lma_args(clt_handle* h, bool have_records) {
+    LmaArgs& a = h->lma_args;
+    std::memset(&a, 0, sizeof(a));
+    h->lma_run = false;
+    const clt_corr_lma_params& lp = h->lp;
+    if (!lp.enable || !have_records) return CLT_OK;
+    static const int groups[6][2] = {{0, 1}, {1, 1}, {0, 2}, {1, 2}, {0, 4}, {1, 4}};
+    const int P = (int)h->pairs.size();
+    for (int f = 0; f < LMA_NFITS; f++) {
+        LmaFit& F = a.fit[f];
+        for (int gi = 0; gi < 6; gi++) {
+            const int diag = groups[gi][0], scale = groups[gi][1];
+            int members[CLT_MAX_PAIRS], n = 0;
+            for (int p = 0; p < P; p++) {
+                const clt_pair& q = h->pairs[p];
+                const bool sel = (f == 0) ? q.dir >= 0 : q.dir == f - 1;
+                if (sel && q.dir >= 0 && ((q.dir >= 2) == (diag != 0)) && q.scale == scale) members[n++] = p;
+            }
+            if (n == 0) continue;
+            if (F.ngroups == LMA_MAXG || n > LMA_GPAIRS)
+                return fail(h, CLT_E_UNSUPPORTED,
+                            "per-tile LMA: at most %d pair groups of at most %d pairs (disable it with "
+                            "clt_set_corr_lma_params)", LMA_MAXG, LMA_GPAIRS);
+            LmaGroup& G = F.g[F.ngroups++];
+            G.npairs = n;
+            G.diag = diag;
+            G.scale = scale;
+            for (int i = 0; i < n; i++) {
+                const int d = h->pairs[members[i]].dir;
+                G.pair[i] = members[i];
+                G.xform[i] = (d == 1) ? 1 : ((d == 3) ? 2 : 0);
+            }
+        }
+    }
+    a.min_poly_strength = lp.min_poly_strength;
+    a.vasw_pwr = lp.ortho_vasw_pwr;
+    a.nc_cost = lp.cnvx_weight;
+    a.half_width = lp.lma_half_width;
+    a.cost_wm = lp.lma_cost_wy;
+    a.cost_wxy = lp.lma_cost_wxy;
+    a.lambda0 = lp.lma_lambda_initial;
+    a.l_good = lp.lma_lambda_scale_good;
+    a.l_bad = lp.lma_lambda_scale_bad;
+    a.l_max = lp.lma_lambda_max;
+    a.rms_diff = lp.lma_rms_diff;
+    a.fo_min_strength = lp.fo_min_strength;
+    a.fo_min_eff = lp.fo_min_eff;
+    a.fo_min_eff_ratio = lp.fo_min_eff_ratio;
+    a.fo_max_hwidth = lp.fo_max_hwidth;
+    a.fo_min_diff = lp.fo_min_diff;
+    a.fo_overcorr = lp.fo_overcorrection;
+    a.fo_lim_overcorr = lp.fo_lim_overcorr;
+    a.hwin = lp.cnvx_hwnd_size;
+    a.add3x3 = lp.cnvx_add3x3;
+    a.adj_wm = lp.lma_adjust_wm;
+    a.adj_wyd = lp.lma_adjust_wy;
+    a.adj_wxy = lp.lma_adjust_wxy;
+    a.adj_ag = lp.lma_adjust_ag;
+    a.num_iter = lp.lma_num_iter;
+    a.mix = lp.mix_corr_poly;
+    a.fo_correct = lp.fo_correct;
+    a.fo_far = lp.fo_far;
+    a.fo_ortho = lp.fo_ortho;
+    a.npairs = P;
+    h->lma_run = a.fit[0].ngroups > 0;
+    return CLT_OK;
 }
 
 // Correlation arguments and output buffers for the current pairs/parameters (CorrArgs upload).
@@ -840,6 +975,7 @@ static int prepare_corr(clt_handle* h) {
         a.notch[i] = (float)(f * std::sin(M_PI * (i + 1.0) / 16));
     }
     HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    if ((rc = build_lma_args(h, a.num_strip > 0))) return rc;
     // the reference quad configuration runs the specialised kernel (pairs as compile-time constants)
     bool quad = g.num_cams == 4 && g.num_colors == 3 && P == 6;
     if (quad) {
@@ -865,6 +1001,9 @@ int clt_exec_corr2d(clt_handle* h) {
     tbegin(h, T_CORR);
     launch_corr(h, h->num_tasks, nullptr);
     tend(h, T_CORR);
+    tbegin(h, T_CORR_LMA);
+    launch_corr_lma(h, h->num_tasks);
+    tend(h, T_CORR_LMA);
     HIPCHK(h, hipGetLastError());
     h->corr_done = true;
     return finish(h);
@@ -1277,7 +1416,10 @@ enum { STAGE_CONVERT = 1, STAGE_CORR = 2, STAGE_TEX = 4, STAGE_IMCLT = 8 };
 static void launch_batch(clt_handle* h, int stages) {
     const int32_t* tf = h->num_frames > 1 ? h->d_task_frame : nullptr;
     if (stages & STAGE_CONVERT) launch_convert(h, h->num_tasks, tf, 0);
-    if (stages & STAGE_CORR) launch_corr(h, h->num_tasks, tf);
+    if (stages & STAGE_CORR) {
+        launch_corr(h, h->num_tasks, tf);
+        launch_corr_lma(h, h->num_tasks);
+    }
     if (stages & STAGE_TEX) launch_tex(h, h->num_tasks, tf, nullptr);
     if (stages & STAGE_IMCLT) launch_imclt(h, h->num_frames, 1);
 }
@@ -1367,7 +1509,8 @@ int clt_exec_batch(clt_handle* h, int32_t stages) {
         return fail(h, CLT_E_ARG, "clt_exec_batch: stages 0x%x", stages);
     int rc;
     if ((rc = prepare_batch(h, stages))) return rc;
-    if (h->graph_stages != stages || h->graph_tasks != h->num_tasks) {   // grid sizes follow the task count
+    if (h->graph_stages != stages || h->graph_tasks != h->num_tasks ||   // grid sizes follow the task count
+        h->graph_epoch_cap != h->graph_epoch) {                             // launch arguments changed
         if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
         if (h->graph) (void)hipGraphDestroy(h->graph);
         h->graph_exec = nullptr;
@@ -1378,6 +1521,7 @@ int clt_exec_batch(clt_handle* h, int32_t stages) {
         HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
         h->graph_stages = stages;
         h->graph_tasks = h->num_tasks;
+        h->graph_epoch_cap = h->graph_epoch;
     }
     tbegin(h, T_BATCH);
     HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));

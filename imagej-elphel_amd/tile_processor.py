@@ -224,6 +224,15 @@ class GPUTileProcessor:
         self._chk(self.lib.clt_set_corr_params(self.h, ctypes.byref(p)))
         self.strip_height = p.strip_height
 
+    def setCorrLmaParams(self, **kw):
+        """Per-tile correlation LMA and foreground correction (ImageDttParameters IDP:32-110);
+        reference defaults (enabled) updated by the keywords, e.g. enable=0."""
+        p = _abi.clt_corr_lma_params()
+        self.lib.clt_default_corr_lma_params(ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._chk(self.lib.clt_set_corr_lma_params(self.h, ctypes.byref(p)))
+
     def setTexParams(self, port_offsets=None, **kw):
         p = _abi.clt_tex_params()
         self.lib.clt_default_tex_params(ctypes.byref(p), self.NUM_CAMS)
@@ -268,7 +277,7 @@ class GPUTileProcessor:
         return out
 
     def getCorr2D(self):
-        """(corr [n][P][15][15], combined strip [n][h][15], disparity [n][8])."""
+        """(corr [n][P][15][15], combined strip [n][h][15], disparity [n][DISP_LAYERS])."""
         n = self.num_task_tiles
         corr = np.empty((n, self.num_pairs, 15, 15), np.float32)
         combo = np.empty((n, self.strip_height, 15), np.float32)

@@ -36,7 +36,7 @@ extern "C" {
 #define CLT_CORR_LEN      225
 #define CLT_MAX_CAMS       16
 #define CLT_MAX_PAIRS     120
-#define CLT_DISP_LAYERS     8    /* per-tile disparity record, see CLT_DISP_*          */
+#define CLT_DISP_LAYERS    12    /* per-tile disparity record, see CLT_DISP_*          */
 #define CLT_TEX_STATS      16    /* per-tile texture statistics of a quad (4*num_cams)  */
 
 enum {
@@ -58,7 +58,15 @@ enum {
     CLT_DISP_HOR_STR    = 4,
     CLT_DISP_VERT       = 5, /* vertical pairs, parabola                    (ID:2227-2237)      */
     CLT_DISP_VERT_STR   = 6,
-    CLT_DISP_CM_WEIGHT  = 7  /* total weight of the centre-of-mass window   (C2:1025)           */
+    CLT_DISP_CM_WEIGHT  = 7, /* total weight of the centre-of-mass window   (C2:1025)           */
+    /* With the per-tile correlation LMA enabled (clt_corr_lma_params.enable, the reference
+     * default), CLT_DISP_CM and CLT_DISP_STRENGTH hold the final values of ID:2254-2330: the LMA
+     * disparity/strength when mix_corr_poly, then the foreground correction.  Without it they
+     * hold the centre of mass and the integer-maximum strength. */
+    CLT_DISP_POLY       = 8, /* LMA disparity (ID:2266 DISPARITY_INDEX_POLY), NaN if not fitted */
+    CLT_DISP_POLY_STR   = 9, /* LMA strength (Correlations2dLMA.getDisparityStrength)           */
+    CLT_DISP_CM_RAW     = 10,/* centre-of-mass disparity before the LMA                         */
+    CLT_DISP_STR_RAW    = 11 /* integer-maximum strength before the LMA                         */
 };
 
 /* Frame geometry: fixed for the life of a handle (the reference injects the same values as
@@ -108,6 +116,43 @@ typedef struct clt_tex_params {
     float   port_offsets[CLT_MAX_CAMS][2]; /* per camera {x,y} offset (ID:1648-1652)           */
 } clt_tex_params;
 
+/* Per-tile correlation LMA and foreground correction (ImageDttParameters, IDP:32-110): the fit of
+ * Correlations2dLMA (C2:1754-1900, C2:2029-2338) run on the combined pairs when the strength
+ * exceeds min_poly_strength (ID:2254-2288), then corr4dirsLMA + foregroundCorrect
+ * (C2:1591-1750, ID:2288-2330).  Doubles, as in the reference. */
+typedef struct clt_corr_lma_params {
+    int32_t enable;              /* 1: clt_exec_corr2d runs the chain (the reference always does)  */
+    int32_t mix_corr_poly;       /* 1: LMA result replaces CM and strength        (IDP:32)         */
+    double  min_poly_strength;   /* 0.2                                           (IDP:33)         */
+    double  ortho_vasw_pwr;      /* 2.0: sample weight *= |value|^pwr             (IDP:46)         */
+    int32_t cnvx_hwnd_size;      /* 4 (at most 4): half window of the convex cells (IDP:76)        */
+    int32_t cnvx_add3x3;         /* 1                                             (IDP:78)         */
+    double  cnvx_weight;         /* 0.5: weight of non-convex selected cells      (IDP:77)         */
+    int32_t lma_adjust_wm;       /* 1                                             (IDP:95)         */
+    int32_t lma_adjust_wy;       /* 0                                             (IDP:96)         */
+    int32_t lma_adjust_wxy;      /* 1                                             (IDP:97)         */
+    int32_t lma_adjust_ag;       /* 1                                             (IDP:98)         */
+    double  lma_half_width;      /* 2.0                                           (IDP:100)        */
+    double  lma_cost_wy;         /* 0.1                                           (IDP:101)        */
+    double  lma_cost_wxy;        /* 0.1                                           (IDP:102)        */
+    double  lma_lambda_initial;  /* 0.1                                           (IDP:104)        */
+    double  lma_lambda_scale_good; /* 0.5                                         (IDP:105)        */
+    double  lma_lambda_scale_bad;  /* 8.0                                         (IDP:106)        */
+    double  lma_lambda_max;      /* 100.0                                         (IDP:107)        */
+    double  lma_rms_diff;        /* 0.001                                         (IDP:108)        */
+    int32_t lma_num_iter;        /* 20                                            (IDP:109)        */
+    int32_t fo_correct;          /* 1                                             (IDP:54)         */
+    int32_t fo_far;              /* 0                                             (IDP:55)         */
+    int32_t fo_ortho;            /* 1                                             (IDP:61)         */
+    double  fo_min_strength;     /* 0.1                                           (IDP:56)         */
+    double  fo_min_eff;          /* 0.25                                          (IDP:57)         */
+    double  fo_min_eff_ratio;    /* 0.65                                          (IDP:58)         */
+    double  fo_max_hwidth;       /* 3.0                                           (IDP:59)         */
+    double  fo_min_diff;         /* 0.2                                           (IDP:60)         */
+    double  fo_overcorrection;   /* 0.0                                           (IDP:62)         */
+    double  fo_lim_overcorr;     /* 2.0                                           (IDP:63)         */
+} clt_corr_lma_params;
+
 typedef struct clt_handle clt_handle;
 
 /* ---- life cycle: GPUTileProcessor() (GTP:243-360); the reference never frees ---------------- */
@@ -139,11 +184,13 @@ int  clt_set_corr_sigma(clt_handle* h, float sigma);
 int  clt_set_pairs(clt_handle* h, const clt_pair* pairs, int32_t num_pairs);
 int  clt_set_corr_params(clt_handle* h, const clt_corr_params* p);
 int  clt_set_tex_params(clt_handle* h, const clt_tex_params* p);
+int  clt_set_corr_lma_params(clt_handle* h, const clt_corr_lma_params* p);
 /* 64 LPF multipliers for sigma, [h][v] order (GTP:814-848 setCltLpf, then dttt_iiie * 2n). */
 int  clt_lpf_coeffs(double sigma, float out[64]);
 /* Defaults of the reference parameter classes (IDP, ECP); quad camera pairs C2:52-58. */
 void clt_default_corr_params(clt_corr_params* p);
 void clt_default_tex_params(clt_tex_params* p, int32_t num_cams);
+void clt_default_corr_lma_params(clt_corr_lma_params* p);
 int  clt_default_quad_pairs(clt_pair out[6]);
 
 /* ---- kernels ---------------------------------------------------------------------------------- */
@@ -152,8 +199,9 @@ int  clt_exec_convert(clt_handle* h, int32_t lpf_mask);
 /* execImcltRbg() (GTP:560-597): inverse CLT with overlap-add of every converted tile of the last
  * convert; apply_lpf != 0 multiplies by the corr_sigma LPF first (clt_lpf, ID:4178-4271). */
 int  clt_exec_imclt(clt_handle* h, int32_t apply_lpf);
-/* Phase correlation of every pair for every task (C2:197-389), combined strip (C2:556-790) and
- * per-tile disparity (ID:2186-2238). */
+/* Phase correlation of every pair for every task (C2:197-389), combined strip (C2:556-790),
+ * per-tile disparity (ID:2186-2238) and, when enabled, the per-tile LMA chain (ID:2254-2330) over
+ * the pairs with a known orientation (at most 2 compatible groups, C2:72-78). */
 int  clt_exec_corr2d(clt_handle* h);
 /* Per-tile textures (ID:2662-2812).  extra_disparity: num_tasks floats or NULL (ID:2663-2677). */
 int  clt_exec_textures(clt_handle* h, const float* extra_disparity);
@@ -282,7 +330,7 @@ int  clt_sync(clt_handle* h);
 int  clt_set_timing(clt_handle* h, int32_t enable);   /* record HIP events around each kernel  */
 /* Accumulated device time (ms) and launch count of a kernel since the last reset:
  * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures, 5 batch graph,
- * 6 correlation adjoint, 7 geometry adjoint. */
+ * 6 correlation adjoint, 7 geometry adjoint, 8 per-tile correlation LMA. */
 int  clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count);
 int  clt_reset_timing(clt_handle* h);
 

@@ -37,6 +37,26 @@ class CorrParams(ctypes.Structure):
                 ("enhortho_scale", ctypes.c_float)]
 
 
+class CorrLmaParams(ctypes.Structure):
+    """Mirror of ``clt_corr_lma_params`` (ImageDttParameters IDP:32-110)."""
+    _fields_ = [("enable", ctypes.c_int32), ("mix_corr_poly", ctypes.c_int32),
+                ("min_poly_strength", ctypes.c_double), ("ortho_vasw_pwr", ctypes.c_double),
+                ("cnvx_hwnd_size", ctypes.c_int32), ("cnvx_add3x3", ctypes.c_int32),
+                ("cnvx_weight", ctypes.c_double),
+                ("lma_adjust_wm", ctypes.c_int32), ("lma_adjust_wy", ctypes.c_int32),
+                ("lma_adjust_wxy", ctypes.c_int32), ("lma_adjust_ag", ctypes.c_int32),
+                ("lma_half_width", ctypes.c_double), ("lma_cost_wy", ctypes.c_double),
+                ("lma_cost_wxy", ctypes.c_double), ("lma_lambda_initial", ctypes.c_double),
+                ("lma_lambda_scale_good", ctypes.c_double), ("lma_lambda_scale_bad", ctypes.c_double),
+                ("lma_lambda_max", ctypes.c_double), ("lma_rms_diff", ctypes.c_double),
+                ("lma_num_iter", ctypes.c_int32), ("fo_correct", ctypes.c_int32),
+                ("fo_far", ctypes.c_int32), ("fo_ortho", ctypes.c_int32),
+                ("fo_min_strength", ctypes.c_double), ("fo_min_eff", ctypes.c_double),
+                ("fo_min_eff_ratio", ctypes.c_double), ("fo_max_hwidth", ctypes.c_double),
+                ("fo_min_diff", ctypes.c_double), ("fo_overcorrection", ctypes.c_double),
+                ("fo_lim_overcorr", ctypes.c_double)]
+
+
 class TexParams(ctypes.Structure):
     _fields_ = [("min_shot", ctypes.c_float), ("scale_shot", ctypes.c_float),
                 ("diff_sigma", ctypes.c_float), ("dust_remove", ctypes.c_int32),
@@ -89,6 +109,26 @@ def default_corr_params(**kw):
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def default_corr_lma_params(**kw):
+    """ImageDttParameters defaults of the per-tile LMA and foreground correction (IDP:32-110)."""
+    p = CorrLmaParams(enable=1, mix_corr_poly=1, min_poly_strength=0.2, ortho_vasw_pwr=2.0,
+                      cnvx_hwnd_size=4, cnvx_add3x3=1, cnvx_weight=0.5,
+                      lma_adjust_wm=1, lma_adjust_wy=0, lma_adjust_wxy=1, lma_adjust_ag=1,
+                      lma_half_width=2.0, lma_cost_wy=0.1, lma_cost_wxy=0.1, lma_lambda_initial=0.1,
+                      lma_lambda_scale_good=0.5, lma_lambda_scale_bad=8.0, lma_lambda_max=100.0,
+                      lma_rms_diff=0.001, lma_num_iter=20, fo_correct=1, fo_far=0, fo_ortho=1,
+                      fo_min_strength=0.1, fo_min_eff=0.25, fo_min_eff_ratio=0.65, fo_max_hwidth=3.0,
+                      fo_min_diff=0.2, fo_overcorrection=0.0, fo_lim_overcorr=2.0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+DISP_LAYERS = 12
+(DISP_INT, DISP_CM, DISP_STRENGTH, DISP_HOR, DISP_HOR_STR, DISP_VERT, DISP_VERT_STR, DISP_CM_WEIGHT,
+ DISP_POLY, DISP_POLY_STR, DISP_CM_RAW, DISP_STR_RAW) = range(DISP_LAYERS)
 
 
 def default_tex_params(num_cams=4, port_offsets=None, **kw):
@@ -150,8 +190,16 @@ def lib(precision=64):
                                     ctypes.c_int, ctypes.c_int, _VP, ctypes.c_int, _D, _VP, ctypes.c_int]
     L.orc_imclt_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, _VP, _D, ctypes.c_int]
     L.orc_corr_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, ctypes.c_int, ctypes.POINTER(Pair),
-                                 ctypes.c_int, ctypes.POINTER(CorrParams), ctypes.c_double, _D, _VP,
-                                 _VP, ctypes.c_int]
+                                 ctypes.c_int, ctypes.POINTER(CorrParams), ctypes.POINTER(CorrLmaParams),
+                                 ctypes.c_double, _D, _VP, _VP, ctypes.c_int]
+    L.orc_corr_lma_frame.argtypes = [ctypes.POINTER(Pair), ctypes.c_int, ctypes.POINTER(CorrLmaParams), _F,
+                                     ctypes.c_int, _D, ctypes.c_int]
+    L.orc_corr_lma.argtypes = [_D, ctypes.POINTER(Pair), ctypes.c_int, ctypes.c_char_p, ctypes.c_double,
+                               ctypes.POINTER(CorrLmaParams), _D]
+    L.orc_filter_convex.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_double, _D]
+    L.orc_foreground_correct.argtypes = [ctypes.POINTER(CorrLmaParams), _I, _D, ctypes.c_double]
+    L.orc_foreground_correct.restype = ctypes.c_double
     L.orc_textures_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, ctypes.c_int,
                                      ctypes.POINTER(TexParams), ctypes.POINTER(CorrParams),
                                      ctypes.c_double, _VP, _D, _VP, ctypes.c_int]
@@ -366,24 +414,74 @@ def imclt_frame(geom, clt, valid, lpf64=None, nthreads=None, precision=64):
     return out
 
 
-def corr_frame(geom, clt, tasks, pairs=None, params=None, corr_sigma=0.8, nthreads=None, precision=64):
-    """Pair correlations [task][pair][15][15], combined strip [task][h][15], disparity [task][8]."""
+def corr_frame(geom, clt, tasks, pairs=None, params=None, corr_sigma=0.8, nthreads=None, precision=64,
+               lma=None):
+    """Pair correlations [task][pair][15][15], combined strip [task][h][15], disparity
+    [task][DISP_LAYERS].  lma: CorrLmaParams (default: the reference defaults, enabled)."""
     nthreads = nthreads or os.cpu_count()
     pairs = pairs if pairs is not None else QUAD_PAIRS
     params = params or default_corr_params()
+    lma = lma if lma is not None else default_corr_lma_params()
     parr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
     tasks = np.ascontiguousarray(tasks, np.int32)
     ntasks = tasks.shape[0]
     clt = _f64(clt)
     corr = np.zeros((ntasks, len(pairs), 15, 15))
     combo = np.zeros((ntasks, params.strip_height, 15))
-    disp = np.zeros((ntasks, 8))
+    disp = np.zeros((ntasks, DISP_LAYERS))
     rc = lib(precision).orc_corr_frame(ctypes.byref(geom), _dp(clt), tasks.ctypes.data_as(_VP), ntasks,
-                                       parr, len(pairs), ctypes.byref(params), corr_sigma, _dp(corr),
-                                       _vp(combo), _vp(disp), nthreads)
+                                       parr, len(pairs), ctypes.byref(params), ctypes.byref(lma), corr_sigma,
+                                       _dp(corr), _vp(combo), _vp(disp), nthreads)
     if rc:
         raise ValueError("orc_corr_frame failed: %d" % rc)
     return corr, combo, disp
+
+
+def corr_lma_frame(corrs, disp, pairs=None, lma=None, nthreads=None):
+    """The per-tile LMA chain (ID:2254-2330) alone on given pair correlations [T][P][15][15]
+    (float32) and disparity records [T][DISP_LAYERS]; returns the updated records (float64)."""
+    nthreads = nthreads or os.cpu_count()
+    pairs = pairs if pairs is not None else QUAD_PAIRS
+    lma = lma if lma is not None else default_corr_lma_params()
+    parr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+    corrs = np.ascontiguousarray(corrs, np.float32)
+    out = np.array(disp, np.float64, copy=True, order="C")
+    assert corrs.shape[0] == out.shape[0] and out.shape[1] == DISP_LAYERS
+    rc = lib(64).orc_corr_lma_frame(parr, len(pairs), ctypes.byref(lma), corrs.ctypes.data_as(_F),
+                                    out.shape[0], _dp(out), nthreads)
+    if rc:
+        raise ValueError("orc_corr_lma_frame failed: %d" % rc)
+    return out
+
+
+def corr_lma(corrs, xcenter, sel=None, pairs=None, lma=None):
+    """One Correlations2dLMA fit (C2:1754-1900) over the selected pairs of one tile
+    (corrs [P][15][15]); returns (disparity, strength, Wm, Wxy) or None when the fit fails."""
+    pairs = pairs if pairs is not None else QUAD_PAIRS
+    lma = lma if lma is not None else default_corr_lma_params()
+    sel = [1] * len(pairs) if sel is None else sel
+    parr = (Pair * len(pairs))(*[Pair(*p) for p in pairs])
+    c = _f64(np.asarray(corrs, np.float64).reshape(len(pairs), 225))
+    out = np.zeros(4)
+    ok = lib(64).orc_corr_lma(_dp(c), parr, len(pairs), bytes(bytearray(int(bool(x)) for x in sel)),
+                              float(xcenter), ctypes.byref(lma), _dp(out))
+    return tuple(out) if ok == 1 else None
+
+
+def filter_convex(data15, hwin, x0c, y0c, add3x3=True, nc_cost=0.5):
+    """filterConvex (C2:2029-2207): [15][15] weights of the cells selected around (x0c, y0c)."""
+    d = _f64(np.asarray(data15, np.float64).reshape(225))
+    w = np.zeros(225)
+    lib(64).orc_filter_convex(_dp(d), hwin, x0c, y0c, int(add3x3), nc_cost, _dp(w))
+    return w.reshape(15, 15)
+
+
+def foreground_correct(dirs, full_disp, lma=None):
+    """foregroundCorrect (C2:1663-1750); dirs: 4 entries (disparity, strength, Wm, Wxy) or None."""
+    lma = lma if lma is not None else default_corr_lma_params()
+    have = np.array([d is not None for d in dirs], np.int32)
+    dds = np.array([d if d is not None else (0, 0, 0, 0) for d in dirs], np.float64)
+    return lib(64).orc_foreground_correct(ctypes.byref(lma), have.ctypes.data_as(_I), _dp(dds), full_disp)
 
 
 def textures_frame(geom, clt, tasks, tex_params=None, corr_params=None, corr_sigma=0.8,

@@ -747,6 +747,533 @@ static int max_xs_ortho(real** corrs, int npairs, real offset, int is_vert, cons
     return 1;
 }
 
+/* ============ per-tile correlation LMA (Correlation2d.java, Correlations2dLMA.java) ============ */
+/* The fit always runs in double (the reference's arithmetic), also in the -DORC_FP32 build. */
+#define LMA_HWIN_MAX 4
+#define LMA_MAX_GROUPS 6
+#define LMA_BOX ((2 * LMA_HWIN_MAX + 1) * (2 * LMA_HWIN_MAX + 1))
+#define LMA_MAX_SAMPLES (LMA_MAX_GROUPS * LMA_BOX)
+#define LMA_MAX_PARS (4 + LMA_MAX_GROUPS)
+#define LMA_MAX_POINTS (LMA_MAX_SAMPLES + 2)
+enum { LMA_X0 = 0, LMA_WM = 1, LMA_WYD = 2, LMA_WXY = 3, LMA_AG = 4 };   /* Correlations2dLMA:61-65 */
+
+/* C2:72-78 GROUPS: {diagonal, baseline scale} in the reference order */
+static const int LMA_GROUPS[6][2] = {{0, 1}, {1, 1}, {0, 2}, {1, 2}, {0, 4}, {1, 4}};
+
+static int pair_diag(const clt_pair* p) { return p->dir == 2 || p->dir == 3; }
+static int pair_compatible(const clt_pair* p, int diag, int scale) {
+    return p->dir >= 0 && pair_diag(p) == diag && p->scale == scale;
+}
+
+/* C2:462-488 combineCompatiblePairs: horizontal and main-diagonal pairs as they are, vertical pairs
+ * transposed (getTransposeAllOrtho, C2:168-177), other-diagonal pairs flipped upside down
+ * (getTransposeAllDiagonal, C2:179-188); the sum is divided by the number of pairs. */
+static int combine_compatible(const double (*corrs)[225], const clt_pair* pairs, int npairs, const char* sel,
+                              int diag, int scale, double* combo) {
+    int n = 0;
+    for (int i = 0; i < 225; i++) combo[i] = 0;
+    for (int p = 0; p < npairs; p++) {
+        if (!sel[p] || !pair_compatible(&pairs[p], diag, scale)) continue;
+        for (int r = 0; r < 15; r++)
+            for (int c = 0; c < 15; c++) {
+                int src = r * 15 + c;
+                if (pairs[p].dir == 1) src = c * 15 + r;
+                else if (pairs[p].dir == 3) src = (14 - r) * 15 + c;
+                combo[r * 15 + c] += corrs[p][src];
+            }
+        n++;
+    }
+    if (n > 1)
+        for (int i = 0; i < 225; i++) combo[i] /= n;
+    return n;
+}
+
+/* C2:2029-2207 filterConvex: grow the selection from (x0c, y0c) ring by ring through bi-convex
+ * cells; selected cells weigh 1 (convex) or nc_cost, the 3x3 around the start at least nc_cost. */
+static void filter_convex(const double* d, int hwin, int x0c, int y0c, int add3x3, double nc_cost, double* weights) {
+    const int center = 7, width = 15;
+    const int x0 = x0c + center, y0 = y0c + center;
+    char convex[225] = {0}, sel[225] = {0}, sel_new[225];
+    const int min_row = (y0 - hwin < 0) ? 0 : y0 - hwin, max_row = (y0 + hwin >= width) ? width - 1 : y0 + hwin;
+    const int min_col = (x0 - hwin < 0) ? 0 : x0 - hwin, max_col = (x0 + hwin >= width) ? width - 1 : x0 + hwin;
+    for (int row = min_row + 1; row < max_row; row++)
+        for (int col = min_col + 1; col < max_col; col++) {
+            int i = row * width + col;
+            convex[i] = (2 * d[i] > (d[i - 1] + d[i + 1])) && (2 * d[i] > (d[i - width] + d[i + width]));
+        }
+    if (y0 >= min_row && y0 <= max_row && x0 >= min_col && x0 <= max_col) sel[y0 * width + x0] = 1;
+#define IN225(i) ((i) >= 0 && (i) < 225)
+#define MARK(i) do { if (!sel_new[i]) { sel_new[i] = 1; num_new++; } } while (0)
+    for (int istep = 0; istep < hwin; istep++) {
+        memcpy(sel_new, sel, sizeof(sel));
+        const int first = (istep == 0);
+        int num_new = 0;
+        const int smin_row = (y0 - istep < 0) ? 0 : y0 - istep, smax_row = (y0 + istep >= width) ? width - 1 : y0 + istep;
+        const int smin_col = (x0 - istep < 0) ? 0 : x0 - istep, smax_col = (x0 + istep >= width) ? width - 1 : x0 + istep;
+        if (istep < y0) {   /* up */
+            const int row = y0 - istep;
+            for (int col = smin_col; col <= smax_col; col++) {
+                int i = row * width + col;
+                if (!IN225(i) || !(sel[i] && (convex[i] || first))) continue;
+                MARK(i - width);
+                int j = i - width;
+                if (convex[j]) {
+                    if (col <= x0 && col > min_col) MARK(j - 1);
+                    if (col >= x0 && col < max_col) MARK(j + 1);
+                }
+            }
+        }
+        if (y0 + istep < max_row) {   /* down */
+            const int row = y0 + istep;
+            for (int col = smin_col; col <= smax_col; col++) {
+                int i = row * width + col;
+                if (!IN225(i) || !(sel[i] && (convex[i] || first))) continue;
+                MARK(i + width);
+                int j = i + width;
+                if (convex[j]) {
+                    if (col <= x0 && col > min_col) MARK(j - 1);
+                    if (col >= x0 && col < max_col) MARK(j + 1);
+                }
+            }
+        }
+        if (istep < x0) {   /* left */
+            const int col = x0 - istep;
+            for (int row = smin_row; row <= smax_row; row++) {
+                int i = row * width + col;
+                if (!IN225(i) || !(sel[i] && (convex[i] || first))) continue;
+                MARK(i - 1);
+                int j = i - 1;
+                if (convex[j]) {
+                    if (row <= y0 && row > min_row) MARK(j - width);
+                    if (row >= y0 && row < max_row) MARK(j + width);
+                }
+            }
+        }
+        if (x0 + istep < max_col) {   /* right */
+            const int col = x0 + istep;
+            for (int row = smin_row; row <= smax_row; row++) {
+                int i = row * width + col;
+                if (!IN225(i) || !(sel[i] && (convex[i] || first))) continue;
+                MARK(i + 1);
+                int j = i + 1;
+                if (convex[j]) {
+                    if (row <= y0 && row > min_row) MARK(j - width);
+                    if (row >= y0 && row < max_row) MARK(j + width);
+                }
+            }
+        }
+        if (num_new == 0) break;
+        memcpy(sel, sel_new, sizeof(sel));
+    }
+#undef MARK
+#undef IN225
+    for (int i = 0; i < 225; i++) weights[i] = sel[i] ? (convex[i] ? 1.0 : nc_cost) : 0.0;
+    if (add3x3)
+        for (int row = y0 - 1; row <= y0 + 1; row++)
+            if (row >= 0 && row < width)
+                for (int col = x0 - 1; col <= x0 + 1; col++)
+                    if (col >= 0 && col < width && weights[row * width + col] == 0.0) weights[row * width + col] = nc_cost;
+}
+
+typedef struct {
+    int ns;
+    double x[LMA_MAX_SAMPLES], y[LMA_MAX_SAMPLES], v[LMA_MAX_SAMPLES], w[LMA_MAX_SAMPLES];
+    int si[LMA_MAX_SAMPLES], gi[LMA_MAX_SAMPLES];
+    int ngroups, gnum[LMA_MAX_GROUPS], gdiag[LMA_MAX_GROUPS];   /* numbering by first sample */
+    int input_diag;
+    double scales[3];
+    double all_pars[LMA_MAX_PARS];
+    int par_mask[LMA_MAX_PARS], npars;
+    double vector[LMA_MAX_PARS];
+    int nvec;
+    double group_weights[LMA_MAX_GROUPS], weights[LMA_MAX_POINTS], values[LMA_MAX_POINTS], pure_weight;
+    int have_last;
+    double last_rms[2], initial_rms[2];
+    double last_ymfx[LMA_MAX_POINTS], last_jt[LMA_MAX_PARS][LMA_MAX_POINTS];
+} lma_t;
+
+/* Correlations2dLMA:196-210 addSample */
+static void lma_add_sample(lma_t* L, double x, double y, double v, double w, int si, int gi) {
+    int n = L->ns++;
+    L->x[n] = x; L->y[n] = y; L->v[n] = v; L->w[n] = w; L->si[n] = si; L->gi[n] = gi;
+    if (L->gnum[gi] < 0) {
+        L->gdiag[L->ngroups] = L->input_diag;
+        L->gnum[gi] = L->ngroups++;
+    }
+}
+
+/* Correlations2dLMA:245-278 setWeightsValues */
+static void lma_set_weights_values(lma_t* L, double half_width, double cost_wm, double cost_wxy) {
+    const int np = L->ns;
+    for (int g = 0; g < L->ngroups; g++) L->group_weights[g] = 0;
+    L->weights[np] = cost_wm;
+    L->weights[np + 1] = cost_wxy;
+    L->values[np] = half_width;
+    L->values[np + 1] = 0.0;
+    double sw = 0;
+    for (int i = 0; i < np; i++) {
+        L->weights[i] = L->w[i];
+        L->values[i] = L->v[i];
+        L->group_weights[L->gnum[L->gi[i]]] += L->w[i];
+        if (isnan(L->values[i]) || isnan(L->weights[i])) {
+            L->weights[i] = 0.0;
+            L->values[i] = 0.0;
+        }
+        sw += L->weights[i];
+    }
+    L->pure_weight = sw;
+    sw += L->weights[np] + L->weights[np + 1];
+    if (sw != 0.0) {
+        sw = 1.0 / sw;
+        for (int i = 0; i < np + 2; i++) L->weights[i] *= sw;
+    }
+    if (L->pure_weight > 0.0)
+        for (int g = 0; g < L->ngroups; g++) L->group_weights[g] /= L->pure_weight;
+    L->pure_weight *= sw;
+}
+
+/* Correlations2dLMA:212-243 initVector (+ toVector, 280-286) */
+static void lma_init_vector(lma_t* L, const clt_corr_lma_params* lp, double x0) {
+    L->npars = LMA_AG + L->ngroups;
+    L->all_pars[LMA_X0] = x0;
+    L->all_pars[LMA_WM] = lp->lma_half_width;
+    L->all_pars[LMA_WYD] = 0.0;
+    L->all_pars[LMA_WXY] = 0.0;
+    L->par_mask[LMA_X0] = 1;
+    L->par_mask[LMA_WM] = lp->lma_adjust_wm != 0;
+    L->par_mask[LMA_WYD] = lp->lma_adjust_wy != 0;
+    L->par_mask[LMA_WXY] = lp->lma_adjust_wxy != 0;
+    for (int g = 0; g < L->ngroups; g++) L->par_mask[LMA_AG + g] = lp->lma_adjust_ag != 0;
+    lma_set_weights_values(L, lp->lma_half_width, lp->lma_cost_wy, lp->lma_cost_wxy);
+    int imx = 0;
+    for (int i = 1; i < L->ns; i++)
+        if (L->values[i] > L->values[imx]) imx = i;
+    for (int g = 0; g < L->ngroups; g++) L->all_pars[LMA_AG + g] = L->values[imx];
+    L->nvec = 0;
+    for (int i = 0; i < L->npars; i++)
+        if (L->par_mask[i]) L->vector[L->nvec++] = L->all_pars[i];
+}
+
+/* Correlations2dLMA:368-416 getFxJt: model Ag * (1 - (dx/Wx)^2 - (y/Wy)^2) (x sqrt2 on diagonals),
+ * Wy = Wm * scale + Wyd, Wx = Wy + Wxy; two regularisation points Wm -> half_width, Wxy -> 0. */
+static void lma_fx_jt(const lma_t* L, const double* vec, double* fx, double (*jt)[LMA_MAX_POINTS]) {
+    double av[LMA_MAX_PARS];
+    for (int i = 0, k = 0; i < L->npars; i++) av[i] = L->par_mask[i] ? vec[k++] : L->all_pars[i];
+    const double sqrt2 = sqrt(2.0);
+    const int ns = L->ns;
+    for (int n = 0; n < ns; n++) {
+        const int grp = L->gnum[L->gi[n]];
+        const double wscale = L->gdiag[grp] ? sqrt2 : 1.0;
+        const double Wy = av[LMA_WM] * L->scales[L->si[n]] + av[LMA_WYD];
+        const double Wx = Wy + av[LMA_WXY];
+        const double dx = L->x[n] - av[LMA_X0];
+        const double Ag = av[LMA_AG + grp];
+        const double dxw = wscale * dx / Wx;
+        const double dyw = wscale * L->y[n] / Wy;
+        const double d = (1.0 - dxw * dxw - dyw * dyw);
+        fx[n] = d * Ag;
+        if (jt) {
+            int np = 0;
+            if (L->par_mask[LMA_X0]) jt[np++][n] = wscale * Ag * 2 * dxw / Wx;
+            const double dfdWx = Ag * 2 * dxw * dxw / Wx;
+            const double dfdWy = Ag * 2 * dyw * dyw / Wy;
+            if (L->par_mask[LMA_WM]) jt[np++][n] = L->scales[L->si[n]] * (dfdWx + dfdWy);
+            if (L->par_mask[LMA_WYD]) jt[np++][n] = (dfdWx + dfdWy);
+            if (L->par_mask[LMA_WXY]) jt[np++][n] = dfdWx;
+            for (int g = 0; g < L->ngroups; g++)
+                if (L->par_mask[LMA_AG + g]) jt[np++][n] = (g == grp) ? d : 0.0;
+        }
+    }
+    fx[ns] = av[LMA_WM];
+    fx[ns + 1] = av[LMA_WXY];
+    if (jt)
+        for (int i = 0, np = 0; i < L->npars; i++)
+            if (L->par_mask[i]) {
+                jt[np][ns] = (i == LMA_WM) ? 1.0 : 0.0;
+                jt[np++][ns + 1] = (i == LMA_WXY) ? 1.0 : 0.0;
+            }
+}
+
+/* Correlations2dLMA:442-461 getWYmFxRms: fx <- w * (y - fx), returns {rms, rms_pure} */
+static void lma_ymfx_rms(const lma_t* L, double* fx, double rms_out[2]) {
+    double rms = 0;
+    for (int i = 0; i < L->ns; i++) {
+        double d = L->values[i] - fx[i];
+        fx[i] = L->weights[i] * d;
+        rms += fx[i] * d;
+    }
+    const double rms_pure = sqrt(rms) / L->pure_weight;
+    for (int i = L->ns; i < L->ns + 2; i++) {
+        double d = L->values[i] - fx[i];
+        fx[i] = L->weights[i] * d;
+        rms += fx[i] * d;
+    }
+    rms_out[0] = sqrt(rms);
+    rms_out[1] = rms_pure;
+}
+
+/* Inverse of a small dense matrix by LU decomposition with partial pivoting (the Jama
+ * Matrix.inverse() of Correlations2dLMA:556); returns 0 when a pivot is exactly zero. */
+static int small_inverse(int n, double a[LMA_MAX_PARS][LMA_MAX_PARS], double inv[LMA_MAX_PARS][LMA_MAX_PARS]) {
+    int piv[LMA_MAX_PARS];
+    for (int i = 0; i < n; i++) piv[i] = i;
+    for (int j = 0; j < n; j++) {
+        double col[LMA_MAX_PARS];
+        for (int i = 0; i < n; i++) col[i] = a[i][j];
+        for (int i = 0; i < n; i++) {
+            int kmax = (i < j) ? i : j;
+            double s = 0.0;
+            for (int k = 0; k < kmax; k++) s += a[i][k] * col[k];
+            a[i][j] = col[i] -= s;
+        }
+        int p = j;
+        for (int i = j + 1; i < n; i++)
+            if (fabs(col[i]) > fabs(col[p])) p = i;
+        if (p != j) {
+            for (int k = 0; k < n; k++) {
+                double t = a[p][k];
+                a[p][k] = a[j][k];
+                a[j][k] = t;
+            }
+            int t = piv[p];
+            piv[p] = piv[j];
+            piv[j] = t;
+        }
+        if (a[j][j] != 0.0)
+            for (int i = j + 1; i < n; i++) a[i][j] /= a[j][j];
+    }
+    for (int j = 0; j < n; j++)
+        if (a[j][j] == 0.0) return 0;
+    for (int c = 0; c < n; c++) {   /* solve for column c of the identity */
+        double x[LMA_MAX_PARS];
+        for (int i = 0; i < n; i++) x[i] = (piv[i] == c) ? 1.0 : 0.0;
+        for (int k = 0; k < n; k++)
+            for (int i = k + 1; i < n; i++) x[i] -= x[k] * a[i][k];
+        for (int k = n - 1; k >= 0; k--) {
+            x[k] /= a[k][k];
+            for (int i = 0; i < k; i++) x[i] -= x[k] * a[i][k];
+        }
+        for (int i = 0; i < n; i++) inv[i][c] = x[i];
+    }
+    return 1;
+}
+
+/* Correlations2dLMA:533-640 lmaStep; returns success in ok[0], done in ok[1] */
+static void lma_step(lma_t* L, double lambda, double rms_diff, int ok[2]) {
+    const int npts = L->ns + 2, nv = L->nvec;
+    ok[0] = ok[1] = 0;
+    if (!L->have_last) {
+        lma_fx_jt(L, L->vector, L->last_ymfx, L->last_jt);
+        lma_ymfx_rms(L, L->last_ymfx, L->last_rms);
+        L->initial_rms[0] = L->last_rms[0];
+        L->initial_rms[1] = L->last_rms[1];
+        L->have_last = 1;
+    }
+    double m[LMA_MAX_PARS][LMA_MAX_PARS], inv[LMA_MAX_PARS][LMA_MAX_PARS];
+    for (int i = 0; i < nv; i++)   /* Correlations2dLMA:418-440 getWJtJlambda */
+        for (int j = i; j < nv; j++) {
+            double d = 0.0;
+            for (int k = 0; k < npts; k++) d += L->weights[k] * L->last_jt[i][k] * L->last_jt[j][k];
+            m[i][j] = d;
+            if (i == j) m[i][j] += d * lambda;
+            else m[j][i] = d;
+        }
+    if (!small_inverse(nv, m, inv)) {
+        ok[1] = 1;
+        return;
+    }
+    double jty[LMA_MAX_PARS], nvec[LMA_MAX_PARS];
+    for (int i = 0; i < nv; i++) {
+        double s = 0;
+        for (int k = 0; k < npts; k++) s += L->last_jt[i][k] * L->last_ymfx[k];
+        jty[i] = s;
+    }
+    for (int i = 0; i < nv; i++) {
+        double s = 0;
+        for (int k = 0; k < nv; k++) s += inv[i][k] * jty[k];
+        nvec[i] = L->vector[i] + s;
+    }
+    lma_fx_jt(L, nvec, L->last_ymfx, L->last_jt);
+    double rms[2];
+    lma_ymfx_rms(L, L->last_ymfx, rms);
+    if (rms[0] < L->last_rms[0]) {
+        ok[0] = 1;
+        ok[1] = rms[0] >= (L->last_rms[0] * (1.0 - rms_diff));
+        L->last_rms[0] = rms[0];
+        L->last_rms[1] = rms[1];
+        for (int i = 0; i < nv; i++) L->vector[i] = nvec[i];
+    } else {
+        lma_fx_jt(L, L->vector, L->last_ymfx, L->last_jt);
+        lma_ymfx_rms(L, L->last_ymfx, L->last_rms);
+    }
+}
+
+/* Correlations2dLMA:477-530 runLma */
+static int lma_run(lma_t* L, const clt_corr_lma_params* lp) {
+    double lambda = lp->lma_lambda_initial;
+    int ok[2] = {0, 0};
+    for (int iter = 0; iter < lp->lma_num_iter; iter++) {
+        lma_step(L, lambda, lp->lma_rms_diff, ok);
+        if (ok[1]) break;
+        if (ok[0]) lambda *= lp->lma_lambda_scale_good;
+        else {
+            lambda *= lp->lma_lambda_scale_bad;
+            if (lambda > lp->lma_lambda_max) break;
+        }
+    }
+    if (ok[0]) return 1;
+    return L->last_rms[0] < L->initial_rms[0];
+}
+
+/* C2:1754-1900 corrLMA (run_poly_instead = false) with C2:2230-2338 addSamples; on success
+ * out = {disparity, strength, Wm, Wxy} (getDisparityStrengthWidth, Correlations2dLMA:165-185). */
+static int corr_lma(const double (*corrs)[225], const clt_pair* pairs, int npairs, const char* sel, double xcenter,
+                    const clt_corr_lma_params* lp, double out[4]) {
+    lma_t* L = malloc(sizeof(lma_t));
+    L->ns = L->ngroups = L->have_last = L->input_diag = 0;
+    for (int g = 0; g < LMA_MAX_GROUPS; g++) L->gnum[g] = -1;
+    int sl[3], nsl = 0, g_num[6], g_scale_ind[6];
+    for (int i = 0; i < 6; i++) {   /* getNumberBaseOfCompatiblePairs (C2:502-519) */
+        g_num[i] = 0;
+        for (int p = 0; p < npairs; p++)
+            if (sel[p] && pair_compatible(&pairs[p], LMA_GROUPS[i][0], LMA_GROUPS[i][1])) g_num[i]++;
+        if (g_num[i] > 0) {
+            int k = 0;
+            while (k < nsl && sl[k] != LMA_GROUPS[i][1]) k++;
+            if (k == nsl) sl[nsl++] = LMA_GROUPS[i][1];
+            g_scale_ind[i] = k;
+        }
+    }
+    for (int k = 0; k < nsl; k++) L->scales[k] = sl[k];
+    const int hwin = lp->cnvx_hwnd_size, hw2 = 2 * hwin + 1, center = 7, width = 15;
+    const int center_index = (width + 1) * center;
+    static const int quad_signs[4][2] = {{-1, -1}, {1, -1}, {-1, 1}, {1, 1}};
+    int ig = 0;
+    for (int i = 0; i < 6; i++) {
+        if (g_num[i] == 0) continue;
+        const int diag = LMA_GROUPS[i][0];
+        const double scale = sl[g_scale_ind[i]], scale05 = scale / 2.0;
+        double combo[225], filt[225];
+        combine_compatible(corrs, pairs, npairs, sel, diag, LMA_GROUPS[i][1], combo);
+        const int ixc = (int)jround(xcenter / scale);
+        const double xcs = ixc * scale;
+        filter_convex(combo, hwin, ixc, diag ? ixc : 0, lp->cnvx_add3x3, lp->cnvx_weight, filt);
+        L->input_diag = diag;
+        for (int arow = 0; arow < hw2; arow += diag ? 1 : 2) {
+            for (int acol = diag ? (arow & 1) : 0; acol < hw2; acol += 2) {
+                for (int quad = 0; quad < 4; quad++) {
+                    if (!((arow > 0) || (quad & 2)) || !((acol > 0) || (quad & 1))) continue;
+                    const int sx = quad_signs[quad][0], sy = quad_signs[quad][1];
+                    const int cx = diag ? (sx * acol - sy * arow) / 2 + ixc : (sx * acol) / 2 + ixc;
+                    const int cy = diag ? (sx * acol + sy * arow) / 2 + ixc : (sy * arow) / 2;
+                    if (cx < -center || cx > center || cy < -center || cy > center) continue;
+                    double w = filt[center_index + width * cy + cx] * g_num[i];
+                    if (!(w > 0.0)) continue;
+                    const double v = combo[center_index + width * cy + cx];
+                    if (lp->ortho_vasw_pwr != 0) w *= pow(fabs(v), lp->ortho_vasw_pwr);
+                    lma_add_sample(L, sx * acol * scale05 + xcs, sy * arow * scale05, v, w, g_scale_ind[i], ig);
+                }
+            }
+        }
+        ig++;
+    }
+    lma_init_vector(L, lp, xcenter);
+    const int ok = lma_run(L, lp);
+    for (int i = 0, k = 0; i < L->npars; i++)   /* updateFromVector */
+        if (L->par_mask[i]) L->all_pars[i] = L->vector[k++];
+    if (ok) {
+        double sum_amp = 0.0, max_amp = 0.0;
+        for (int g = 0; g < L->ngroups; g++) sum_amp += L->group_weights[g] * L->all_pars[LMA_AG + g];
+        for (int n = 0; n < L->ns; n++)
+            if (L->v[n] > max_amp) max_amp = L->v[n];
+        if (sum_amp > 1.25 * max_amp) sum_amp = max_amp;
+        out[0] = -L->all_pars[LMA_X0];
+        out[1] = sum_amp;
+        out[2] = L->all_pars[LMA_WM];
+        out[3] = L->all_pars[LMA_WXY];
+    }
+    free(L);
+    return ok;
+}
+
+/* C2:1663-1750 foregroundCorrect: the disparity to use (full_disp unless corrected) */
+static double foreground_correct(const clt_corr_lma_params* lp, const int have[4], double dds[4][4], double full_disp) {
+    const int bg = lp->fo_far != 0, ortho = lp->fo_ortho != 0;
+    double mn = NAN, mx = NAN;
+    int imn = -1, imx = -1;
+    double eff[4] = {0, 0, 0, 0}, width_d[4] = {0, 0, 0, 0};
+    int strong[4] = {0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+        if (have[i]) {
+            if (isnan(mn) || dds[i][0] < mn) { mn = dds[i][0]; imn = i; }
+            if (isnan(mx) || dds[i][0] > mx) { mx = dds[i][0]; imx = i; }
+            eff[i] = dds[i][1] * (dds[i][2] - dds[i][3]) / (dds[i][2] + dds[i][3]);
+            width_d[i] = dds[i][2] + dds[i][3];
+            strong[i] = (dds[i][1] >= lp->fo_min_strength) && (eff[i] >= lp->fo_min_eff);
+        } else {
+            eff[i] = NAN;
+        }
+    }
+    const int are_ortho = (imn ^ imx) == 1;
+    double corr = lp->fo_overcorrection * (mx - mn);
+    const int isel = bg ? imn : imx, iother = bg ? imx : imn;
+    if (isel < 0 || !strong[isel]) corr = NAN;
+    else if (width_d[isel] > lp->fo_max_hwidth) corr = NAN;
+    else if (eff[isel] / eff[isel ^ 1] < lp->fo_min_eff_ratio) corr = NAN;
+    else {
+        if (ortho && !are_ortho) corr = 0.0;
+        else if (!strong[iother]) corr = 0.0;
+        if (!(mx >= (mn + lp->fo_min_diff))) corr = NAN;
+    }
+    double disp = full_disp;
+    if (!isnan(corr)) {
+        if (bg) {
+            double lim = full_disp - (full_disp - mn) * lp->fo_lim_overcorr;
+            disp = fmax(mn - corr, lim);
+        } else {
+            double lim = full_disp + (mx - full_disp) * lp->fo_lim_overcorr;
+            disp = fmin(mx + corr, lim);
+        }
+    }
+    return disp;
+}
+
+/* ID:2254-2330 for one tile: the LMA over all oriented pairs (C2:1754), mixing into CM/strength
+ * (ID:2273-2281), the four one-direction fits and the foreground correction (ID:2288-2330).
+ * disp: the tile's record; reads CLT_DISP_CM_RAW / CLT_DISP_STR_RAW, writes CM, STRENGTH, POLY. */
+static void lma_chain(const double (*corrs)[225], const clt_pair* pairs, int npairs, const clt_corr_lma_params* lp,
+                      double* disp) {
+    disp[CLT_DISP_POLY] = disp[CLT_DISP_POLY_STR] = NAN;
+    const double cm = disp[CLT_DISP_CM_RAW];
+    double strength = disp[CLT_DISP_STR_RAW];
+    disp[CLT_DISP_CM] = cm;
+    disp[CLT_DISP_STRENGTH] = strength;
+    if (!lp->enable || isnan(cm) || !(strength > lp->min_poly_strength)) return;
+    char sel[CLT_MAX_PAIRS];
+    for (int p = 0; p < npairs; p++) sel[p] = pairs[p].dir >= 0;
+    double r[4];
+    if (!corr_lma(corrs, pairs, npairs, sel, -cm, lp, r)) return;
+    double disparity = cm;
+    disp[CLT_DISP_POLY] = r[0];
+    disp[CLT_DISP_POLY_STR] = r[1];
+    if (lp->mix_corr_poly) {
+        disparity = r[0];
+        strength = r[1];
+        disp[CLT_DISP_CM] = disparity;
+        disp[CLT_DISP_STRENGTH] = strength;
+    }
+    if (!(lp->fo_correct && strength > 0)) return;
+    double dds[4][4];
+    int have[4];
+    for (int dir = 0; dir < 4; dir++) {   /* C2:1591-1630 corr4dirsLMA */
+        int any = 0;
+        for (int p = 0; p < npairs; p++) any |= (sel[p] = (pairs[p].dir == dir));
+        have[dir] = any && corr_lma(corrs, pairs, npairs, sel, -disparity, lp, dds[dir]);
+    }
+    const double mod = foreground_correct(lp, have, dds, disparity);
+    if (mod != disparity && lp->fo_correct && strength > lp->fo_min_strength) disp[CLT_DISP_CM] = mod;
+}
+
 typedef struct {
     frame_t f;
     const double* clt;
@@ -754,13 +1281,14 @@ typedef struct {
     const clt_pair* pairs;
     int npairs;
     clt_corr_params prm;
+    clt_corr_lma_params lma;
     real lpf[64];
     real col_weights[3];
     double wndy[16], wndx[16], notch[15];
     int wndy_len, wndx_len;
     double* corr_out;   /* [task][pair][225] */
     double* combo_out;  /* [task][strip_height*15] or NULL */
-    double* disp_out;   /* [task][8] or NULL */
+    double* disp_out;   /* [task][CLT_DISP_LAYERS] or NULL */
 } corr_ctx;
 
 /* ID:2101-2238 for one task tile */
@@ -770,11 +1298,11 @@ static void corr_task(void* vctx, int t) {
     const int K = g->num_colors, hw = c->prm.strip_height, slen = hw * 15;
     double* corr_o = c->corr_out + (size_t)t * c->npairs * 225;
     double* combo_o = c->combo_out ? c->combo_out + (size_t)t * slen : NULL;
-    double* disp_o = c->disp_out ? c->disp_out + (size_t)t * 8 : NULL;
+    double* disp_o = c->disp_out ? c->disp_out + (size_t)t * CLT_DISP_LAYERS : NULL;
     if (disp_o) {   /* ID:1713-1719 init: NaN, strengths 0 */
-        for (int i = 0; i < 8; i++) disp_o[i] = NAN;
+        for (int i = 0; i < CLT_DISP_LAYERS; i++) disp_o[i] = NAN;
         disp_o[CLT_DISP_STRENGTH] = disp_o[CLT_DISP_HOR_STR] = disp_o[CLT_DISP_VERT_STR] = 0;
-        disp_o[CLT_DISP_CM_WEIGHT] = 0;
+        disp_o[CLT_DISP_CM_WEIGHT] = disp_o[CLT_DISP_STR_RAW] = 0;
     }
     if (combo_o)
         for (int i = 0; i < slen; i++) combo_o[i] = NAN;
@@ -839,6 +1367,16 @@ static void corr_task(void* vctx, int t) {
             disp_o[vert ? CLT_DISP_VERT_STR : CLT_DISP_HOR_STR] = r[1];
         }
     }
+    disp_o[CLT_DISP_CM_RAW] = disp_o[CLT_DISP_CM];
+    disp_o[CLT_DISP_STR_RAW] = disp_o[CLT_DISP_STRENGTH];
+#ifdef ORC_FP32
+    static __thread double dcorrs[CLT_MAX_PAIRS][225];
+    for (int p = 0; p < c->npairs; p++)
+        for (int i = 0; i < 225; i++) dcorrs[p][i] = corrs[p][i];
+    lma_chain((const double (*)[225])dcorrs, c->pairs, c->npairs, &c->lma, disp_o);
+#else
+    lma_chain((const double (*)[225])corrs, c->pairs, c->npairs, &c->lma, disp_o);
+#endif
 }
 
 static void color_weights(const clt_geom* g, double red, double blue, real* w) {
@@ -1536,10 +2074,12 @@ EXPORT int orc_imclt_frame(const clt_geom* g, const double* clt, const uint8_t* 
 }
 
 EXPORT int orc_corr_frame(const clt_geom* g, const double* clt, const int32_t* tasks, int ntasks,
-                          const clt_pair* pairs, int npairs, const clt_corr_params* prm, double corr_sigma,
+                          const clt_pair* pairs, int npairs, const clt_corr_params* prm,
+                          const clt_corr_lma_params* lma, double corr_sigma,
                           double* corr_out, double* combo_out, double* disp_out, int nthreads) {
     tables();
     if (npairs > CLT_MAX_PAIRS || prm->strip_height > 16) return CLT_E_ARG;
+    if (lma && lma->enable && (lma->cnvx_hwnd_size < 1 || lma->cnvx_hwnd_size > LMA_HWIN_MAX)) return CLT_E_ARG;
     corr_ctx* c = calloc(1, sizeof(corr_ctx));
     frame_init(&c->f, g);
     c->clt = clt;
@@ -1547,6 +2087,8 @@ EXPORT int orc_corr_frame(const clt_geom* g, const double* clt, const int32_t* t
     c->pairs = pairs;
     c->npairs = npairs;
     c->prm = *prm;
+    if (lma) c->lma = *lma;
+    else c->lma.enable = 0;
     make_lpf(corr_sigma, c->lpf);
     color_weights(g, prm->corr_red, prm->corr_blue, c->col_weights);
     half_flat_top_window(2, 1.0, 1.0, 1, 0, 1.0, c->wndy);
@@ -1560,6 +2102,52 @@ EXPORT int orc_corr_frame(const clt_geom* g, const double* clt, const int32_t* t
     run_pool(ntasks, nthreads, corr_task, c);
     free(c);
     return 0;
+}
+
+typedef struct {
+    const clt_pair* pairs;
+    int npairs;
+    const clt_corr_lma_params* lp;
+    const float* corrs;
+    double* disp;
+} lma_ctx;
+
+static void lma_task(void* vctx, int t) {
+    lma_ctx* c = (lma_ctx*)vctx;
+    double (*dc)[225] = malloc(sizeof(double) * 225 * c->npairs);
+    const float* src = c->corrs + (size_t)t * c->npairs * 225;
+    for (int p = 0; p < c->npairs; p++)
+        for (int i = 0; i < 225; i++) dc[p][i] = src[p * 225 + i];
+    lma_chain((const double (*)[225])dc, c->pairs, c->npairs, c->lp, c->disp + (size_t)t * CLT_DISP_LAYERS);
+    free(dc);
+}
+
+/* The LMA chain alone (ID:2254-2330) on given pair correlations [T][npairs][225] and disparity
+ * records [T][CLT_DISP_LAYERS] (reads CM_RAW / STR_RAW, rewrites CM, STRENGTH, POLY, POLY_STR). */
+EXPORT int orc_corr_lma_frame(const clt_pair* pairs, int npairs, const clt_corr_lma_params* lp, const float* corrs,
+                              int ntasks, double* disp, int nthreads) {
+    if (npairs > CLT_MAX_PAIRS || lp->cnvx_hwnd_size < 1 || lp->cnvx_hwnd_size > LMA_HWIN_MAX) return CLT_E_ARG;
+    lma_ctx c = {pairs, npairs, lp, corrs, disp};
+    run_pool(ntasks, nthreads, lma_task, &c);
+    return 0;
+}
+
+/* Single fits for the oracle checks: corr_lma over the pairs flagged in sel (C2:1754-1900),
+ * out = {disparity, strength, Wm, Wxy}; filter_convex (C2:2029-2207). */
+EXPORT int orc_corr_lma(const double* corrs, const clt_pair* pairs, int npairs, const char* sel, double xcenter,
+                        const clt_corr_lma_params* lp, double* out) {
+    if (npairs > CLT_MAX_PAIRS || lp->cnvx_hwnd_size < 1 || lp->cnvx_hwnd_size > LMA_HWIN_MAX) return CLT_E_ARG;
+    return corr_lma((const double (*)[225])corrs, pairs, npairs, sel, xcenter, lp, out);
+}
+
+EXPORT void orc_filter_convex(const double* d, int hwin, int x0c, int y0c, int add3x3, double nc_cost, double* w) {
+    filter_convex(d, hwin, x0c, y0c, add3x3, nc_cost, w);
+}
+
+EXPORT double orc_foreground_correct(const clt_corr_lma_params* lp, const int* have, const double* dds, double full) {
+    double d[4][4];
+    memcpy(d, dds, sizeof(d));
+    return foreground_correct(lp, have, d, full);
 }
 
 EXPORT int orc_textures_frame(const clt_geom* g, const double* clt, const int32_t* tasks, int ntasks,

@@ -149,12 +149,12 @@ def _corr_compare(pk, orc, sc, fat_zero=0.0, pairs=None, strip=True):
     if strip:
         assert np.array_equal(np.isnan(g_combo), np.isnan(o_combo)), "NaN pattern of the combined strip"
         res["combo"] = rel_rms(g_combo, o_combo)
-        for name, k in (("int", 0), ("cm", 1), ("hor", 3), ("vert", 5)):
+        for name, k in (("int", 0), ("cm", 10), ("hor", 3), ("vert", 5)):   # cm: before the LMA
             for ref, tag in ((o_disp, ""), (e_disp, "_e2e")):
                 both = np.isfinite(ref[:, k]) & np.isfinite(g_disp[:, k])
                 assert np.array_equal(np.isfinite(ref[:, k]), np.isfinite(g_disp[:, k])), name + tag
                 res[name + tag] = float(np.abs(g_disp[both, k] - ref[both, k]).max()) if both.any() else 0.0
-        res["strength"] = rel_rms(g_disp[:, 2], o_disp[:, 2])
+        res["strength"] = rel_rms(g_disp[:, 11], o_disp[:, 11])
     tp.close()
     return res
 
@@ -255,11 +255,11 @@ def test_config0_pipeline_disparity(pk, orc):
     g_clt = gpu_clt_cpu_layout(tp, sc)
     e_clt = rel_rms(g_clt, o_clt)
     e_img = rel_rms(g_img, o_img)
-    fin = np.isfinite(o_disp[:, 1]) & np.isfinite(g_disp[:, 1])
-    e_disp = float(np.abs(g_disp[fin, 1] - o_disp[fin, 1]).max())
+    fin = np.isfinite(o_disp[:, 10]) & np.isfinite(g_disp[:, 10])
+    e_disp = float(np.abs(g_disp[fin, 10] - o_disp[fin, 10]).max())
     log("  CLT=%.3g image=%.3g cm-disparity max|d|=%.3g px over %d tiles" % (e_clt, e_img, e_disp, fin.sum()))
     assert e_clt < CLT_TOL and e_img < IMG_TOL and e_disp < DISP_TOL
-    assert np.array_equal(np.isfinite(o_disp[:, 1]), np.isfinite(g_disp[:, 1]))
+    assert np.array_equal(np.isfinite(o_disp[:, 10]), np.isfinite(g_disp[:, 10]))
     tp.close()
 
 

@@ -24,6 +24,15 @@ Each oracle function is pinned by what the reference itself holds or what mathem
   combine (C2:730-790)                equal strips reproduce themselves; negative -> 0
   maxima (C2:802-1032, 2502-2651)     argmax, symmetric peaks, parabola vertex closed form
   halfFlatTopWindow (C2:1047-1081)    hand-computed values
+  correlation LMA (C2:1754-2338,      a paraboloid of known centre/widths sampled on the pair grids
+   Correlations2dLMA)                 (ortho, diagonal x sqrt2, transposed vertical and flipped
+                                      other-diagonal pairs) is recovered exactly without the width
+                                      priors and within 1e-3 px with them; a symmetric peak stays on
+                                      its integer; noise-free scenes of known residual give it within
+                                      10% where the centre of mass is biased
+  filterConvex (C2:2029-2207)         everywhere-convex data: 7x7 interior 1, ring without corners
+                                      nc_cost, corners 0; a pit: only the 3x3 at nc_cost
+  foregroundCorrect (C2:1663-1750)    hand-worked cases for each branch
   textures (ID:2830-3147)             identical ports: RGB = input, weights 1, alpha = window^2;
                                       single port copies; debayer keeps constants; overlap-add of
                                       window^2 alpha is 1 in the interior
@@ -330,7 +339,10 @@ def test_disparity_sign_all_orientations(orc, pk, delta):
             ((sc.tasks[:, 1] >> 16) > 1) & ((sc.tasks[:, 1] >> 16) < 10)
     # the sign is exact; the magnitude carries the centre-of-mass bias towards the integer maximum
     # that the reference compensates with corr_magic_scale (about 0.85, ECP corr_magic_scale)
-    assert 0.7 < np.nanmedian(disp[inner, 1]) / delta < 1.1          # CM
+    assert 0.7 < np.nanmedian(disp[inner, orc.DISP_CM_RAW]) / delta < 1.1          # CM
+    # the paraboloid LMA (C2:1754-1900) removes that bias on this noise-free scene
+    assert np.isfinite(disp[inner, orc.DISP_POLY]).all()
+    assert abs(np.nanmedian(disp[inner, orc.DISP_POLY]) / delta - 1) < 0.1
     if abs(delta) <= 1:   # the notched ortho profiles (C2:2502) only resolve small residuals
         assert 0.3 < np.nanmedian(disp[inner, 3]) / delta < 1.5      # horizontal pairs
         assert 0.3 < np.nanmedian(disp[inner, 5]) / delta < 1.5      # vertical pairs
@@ -450,3 +462,95 @@ def test_fp32_port_close_to_fp64(orc, pk):
     a = orc.convert_frame(g, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
     b = orc.convert_frame(g, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets, precision=32)
     assert rel_rms(b, a) < 1e-5
+
+
+# ---------------------------------------------------------------- per-tile correlation LMA
+def _paraboloid_pairs(X0, Wm, Wxy, Ag_o, Ag_d):
+    """The six quad pair correlations of the Correlations2dLMA model with centre X0 (pixels along
+    the disparity axis) as the reference lays them out: horizontal pairs with the disparity along
+    x, vertical pairs transposed, the main diagonal along (x, y), the other diagonal flipped
+    upside down; diagonal distances scaled by sqrt2 (C2:462-488, Correlations2dLMA:378-393)."""
+    r, c = np.mgrid[0:15, 0:15]
+    cx, cy = c - 7.0, r - 7.0
+    Wy, Wx = Wm, Wm + Wxy
+    fo = Ag_o * (1 - ((cx - X0) / Wx) ** 2 - (cy / Wy) ** 2)
+    xd, yd = (cx + cy) / 2, (cy - cx) / 2
+    fd = Ag_d * (1 - (np.sqrt(2) * (xd - X0) / Wx) ** 2 - (np.sqrt(2) * yd / Wy) ** 2)
+    return np.stack([fo, fo, fo.T, fo.T, fd, fd[::-1, :]])
+
+
+@pytest.mark.parametrize("xc", [1.3, 0.8, 2.2])
+def test_lma_recovers_paraboloid(orc, xc):
+    corrs = _paraboloid_pairs(1.3, 2.5, 0.3, 0.8, 0.7)
+    free = orc.default_corr_lma_params(lma_cost_wy=0.0, lma_cost_wxy=0.0)
+    d, strength, wm, wxy = orc.corr_lma(corrs, xc, lma=free)
+    assert abs(d + 1.3) < 1e-9 and abs(wm - 2.5) < 1e-9 and abs(wxy - 0.3) < 1e-9
+    assert 0.7 < strength < 0.8          # group-weighted mean of the two amplitudes
+    # with the reference priors (Wm -> 2.0, Wxy -> 0 at cost 0.1) the centre barely moves
+    d2, _, wm2, wxy2 = orc.corr_lma(corrs, xc)
+    assert abs(d2 + 1.3) < 1e-3 and 2.0 < wm2 < 2.5 and 0.0 <= wxy2 < 0.3
+
+
+def test_lma_single_groups_and_directions(orc):
+    corrs = _paraboloid_pairs(-0.6, 2.2, 0.0, 0.5, 0.5)
+    free = orc.default_corr_lma_params(lma_cost_wy=0.0, lma_cost_wxy=0.0)
+    for d in range(4):   # one orientation at a time, as corr4dirsLMA fits them (C2:1591-1630)
+        sel = [1 if p[2] == d else 0 for p in orc.QUAD_PAIRS]
+        r = orc.corr_lma(corrs, -0.5, sel=sel, lma=free)
+        assert r is not None and abs(r[0] - 0.6) < 1e-9 and abs(r[2] - 2.2) < 1e-9, (d, r)
+
+
+def test_lma_symmetric_peak_stays_on_integer(orc):
+    corrs = _paraboloid_pairs(2.0, 2.0, 0.0, 0.6, 0.6)
+    corrs = np.where(corrs > 0, np.abs(corrs) ** 1.5, corrs)   # not the model, symmetric about x = 2
+    d, _, _, wxy = orc.corr_lma(corrs, 2.3)
+    assert abs(d + 2.0) < 1e-12
+
+
+def test_filter_convex_paraboloid(orc):
+    fo = _paraboloid_pairs(0.0, 3.0, 0.0, 1.0, 1.0)[0]
+    w = orc.filter_convex(fo, 4, 1, 0, add3x3=True, nc_cost=0.5)
+    expect = np.zeros((15, 15))
+    y0, x0 = 7, 8
+    expect[y0 - 4:y0 + 5, x0 - 4:x0 + 5] = 0.5
+    for yy, xx in ((-4, -4), (-4, 4), (4, -4), (4, 4)):
+        expect[y0 + yy, x0 + xx] = 0.0
+    expect[y0 - 3:y0 + 4, x0 - 3:x0 + 4] = 1.0
+    assert np.array_equal(w, expect)
+    # clamped at the border: start at column 13, the window ends at column 14
+    w2 = orc.filter_convex(fo, 4, 6, 0)
+    assert w2[:, :9].max() == 0 and w2[7, 13] == 1.0 and w2[7, 14] == 0.5
+
+
+def test_filter_convex_pit(orc):
+    r, c = np.mgrid[0:15, 0:15]
+    pit = (r - 7.0) ** 2 + (c - 7.0) ** 2   # nothing is bi-convex
+    w = orc.filter_convex(pit, 4, 0, 0, add3x3=True, nc_cost=0.5)
+    expect = np.zeros((15, 15))
+    expect[6:9, 6:9] = 0.5
+    assert np.array_equal(w, expect)
+    w = orc.filter_convex(pit, 4, 0, 0, add3x3=False, nc_cost=0.5)
+    expect[6, 6] = expect[6, 8] = expect[8, 6] = expect[8, 8] = 0   # the first ring is the cross
+    assert np.array_equal(w, expect)
+
+
+def test_foreground_correct_branches(orc):
+    full = 1.2
+    # dirs: (disparity, strength, Wm, Wxy); eff = strength * (Wm - Wxy) / (Wm + Wxy)
+    strong = lambda d: (d, 0.6, 2.0, 0.2)
+    # max (vertical, 1.5) strong, orthogonal to the min (horizontal): no overcorrection -> max
+    assert orc.foreground_correct([strong(1.0), strong(1.5), strong(1.2), strong(1.1)], full) == 1.5
+    # max - min below fo_min_diff (0.2): unchanged
+    assert orc.foreground_correct([strong(1.1), strong(1.25), strong(1.2), strong(1.15)], full) == full
+    # the max direction is weak: unchanged
+    assert orc.foreground_correct([strong(1.0), (1.5, 0.05, 2.0, 0.2), strong(1.2), None], full) == full
+    # too wide in the disparity direction (Wm + Wxy > 3): unchanged
+    assert orc.foreground_correct([strong(1.0), (1.5, 0.6, 2.9, 0.2), strong(1.2), None], full) == full
+    # overcorrection 0.5 of (max - min), limited to full + 2 (max - full)
+    lp = orc.default_corr_lma_params(fo_overcorrection=0.5)
+    assert abs(orc.foreground_correct([strong(1.0), strong(1.5), None, None], full, lma=lp) - 1.75) < 1e-12
+    lp = orc.default_corr_lma_params(fo_overcorrection=5.0)
+    assert abs(orc.foreground_correct([strong(1.0), strong(1.5), None, None], full, lma=lp) - 1.8) < 1e-12
+    # far objects (fo_far): the minimum
+    lp = orc.default_corr_lma_params(fo_far=1)
+    assert orc.foreground_correct([strong(1.0), strong(1.5), strong(1.2), strong(1.1)], full, lma=lp) == 1.0
