@@ -6,13 +6,16 @@
 // C2:2230-2338; the fit Correlations2dLMA.java), four one-direction fits (corr4dirsLMA,
 // C2:1591-1630) and the foreground correction (foregroundCorrect, C2:1663-1750).
 //
-// Mapping: 16 lanes (one DPP row) per tile, 4 tiles per wavefront.  A fit's samples are the cells
-// of the 9x9 box around the start point: each lane owns box cells l, l+16, ... (6 per group), the
-// combined correlation values and sample weights of the tile sit in LDS, the selection grows in
-// LDS byte flags, and every normal-equation sum is an all-reduce over the row (DPP row rotates on
-// the two halves of each double; every lane ends with bit-identical sums, so the fit's decisions
-// are uniform within the tile).  Arithmetic is FP64 like the reference; the 6x6 normal system is
-// solved by LDL^T in registers (each lane redundantly).
+// Mapping: 16 lanes (one DPP row) per tile, 4 tiles per wavefront.  The fit over all pairs uses
+// the tile's 16 lanes; the four one-direction fits run two at a time on 8 lanes each.  A fit's
+// candidate samples are the 9x9 box around its start point: the lanes own box cells fl, fl+W, ...,
+// the combined correlation values sit in LDS as doubles, the convexity flags are gathered into an
+// 81-bit mask with ballots and the selection grows by shifts of that mask (no memory round trips),
+// and the selected cells are compacted into a per-group sample list, so an evaluation of the
+// model touches only the samples (typically 15-30 per group of the 81 cells).  Every
+// normal-equation sum is an all-reduce over the fit's lanes (DPP on the two halves of each
+// double; all lanes end with bit-identical sums, so the fit's decisions are uniform).  Arithmetic
+// is FP64 like the reference; the damped normal system is solved by LDL^T in registers.
 #pragma once
 
 namespace clt {
@@ -21,11 +24,10 @@ constexpr int LMA_TILES_PER_WAVE = 4;
 constexpr int LMA_WAVES = 4;                 // 256-thread blocks
 constexpr int LMA_SLOTS = LMA_TILES_PER_WAVE * LMA_WAVES;
 constexpr int LMA_CELLS = 81;                // 9x9 box: half window at most 4
-constexpr int LMA_CPL = 6;                   // box cells per lane (16 * 6 >= 81)
-constexpr int LMA_MAXG = 2;                  // compatible pair groups per fit
-constexpr int LMA_GPAIRS = 8;                // pairs per group
-constexpr int LMA_NACC = 28;                 // 21 (upper triangle of 6x6) + 6 + rms
+constexpr int LMA_MAXG = 2;                  // compatible pair groups of the fit over all pairs
+constexpr int LMA_GPAIRS = 4;                // pairs per group (the 8-camera table has at most 4)
 constexpr int LMA_NFITS = 5;                 // all pairs, then horizontal, vertical, diag main, diag other
+constexpr int LMA_ACC = 28;                  // LDS doubles per fit for the accepted sums
 
 struct LmaGroup {
     int32_t npairs, diag;
@@ -47,7 +49,7 @@ struct LmaArgs {
     int32_t hwin, add3x3, adj_wm, adj_wyd, adj_wxy, adj_ag, num_iter, mix, fo_correct, fo_far, fo_ortho, npairs;
 };
 
-// ---- row-of-16 reductions on doubles (DPP row rotates; identical result in all 16 lanes) ----------
+// ---- reductions over the W lanes of one fit (DPP; identical result in all W lanes) ---------------
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
     const long long b = __double_as_longlong(v);
@@ -55,344 +57,430 @@ __device__ __forceinline__ double dpp_f64(double v) {
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-__device__ __forceinline__ double row_sum(double v) {
-    v += dpp_f64<0x128>(v);   // row_ror:8
-    v += dpp_f64<0x124>(v);   // row_ror:4
-    v += dpp_f64<0x122>(v);   // row_ror:2
-    v += dpp_f64<0x121>(v);   // row_ror:1
+template <int W>
+__device__ __forceinline__ double fit_sum(double v) {
+    if constexpr (W == 16) {   // row_ror 8, 4, 2, 1
+        v += dpp_f64<0x128>(v);
+        v += dpp_f64<0x124>(v);
+        v += dpp_f64<0x122>(v);
+        v += dpp_f64<0x121>(v);
+    } else {                   // quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror
+        v += dpp_f64<0xb1>(v);
+        v += dpp_f64<0x4e>(v);
+        v += dpp_f64<0x141>(v);
+    }
     return v;
 }
-__device__ __forceinline__ double row_max(double v) {
-    v = fmax(v, dpp_f64<0x128>(v));
-    v = fmax(v, dpp_f64<0x124>(v));
-    v = fmax(v, dpp_f64<0x122>(v));
-    v = fmax(v, dpp_f64<0x121>(v));
+template <int W>
+__device__ __forceinline__ double fit_max(double v) {
+    if constexpr (W == 16) {
+        v = fmax(v, dpp_f64<0x128>(v));
+        v = fmax(v, dpp_f64<0x124>(v));
+        v = fmax(v, dpp_f64<0x122>(v));
+        v = fmax(v, dpp_f64<0x121>(v));
+    } else {
+        v = fmax(v, dpp_f64<0xb1>(v));
+        v = fmax(v, dpp_f64<0x4e>(v));
+        v = fmax(v, dpp_f64<0x141>(v));
+    }
     return v;
-}
-// any lane of this tile's row
-__device__ __forceinline__ bool row_any(bool p, int sub) {
-    return ((__ballot(p) >> (16 * sub)) & 0xffffull) != 0;
 }
 
-// index of (i, j), i <= j, in the packed upper triangle of the 6x6 normal matrix
-__host__ __device__ constexpr int lma_tri(int i, int j) { return i * 6 - i * (i - 1) / 2 + (j - i); }
+// 1/x to about 1 ulp: hardware estimate and two Newton steps
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// |v|^e for a value-as-weight power other than 2 (out of line: its constants would otherwise stay
+// in registers through the whole fit)
+__device__ __noinline__ double lma_pow(double v, double e) { return pow(fabs(v), e); }
+
+// index of (i, j), i <= j, in a packed upper triangle of n x n
+__host__ __device__ constexpr int lma_tri(int n, int i, int j) { return i * n - i * (i - 1) / 2 + (j - i); }
+
+// ---- the 9x9 box as an 81-bit mask (bit br * 9 + bc) -------------------------------------------
+struct Box81 {
+    uint64_t lo, hi;   // bits 0..63, 64..80
+};
+__device__ __forceinline__ Box81 b_and(Box81 a, Box81 b) { return {a.lo & b.lo, a.hi & b.hi}; }
+__device__ __forceinline__ Box81 b_or(Box81 a, Box81 b) { return {a.lo | b.lo, a.hi | b.hi}; }
+__device__ __forceinline__ Box81 b_andn(Box81 a, Box81 b) { return {a.lo & ~b.lo, a.hi & ~b.hi}; }
+__device__ __forceinline__ bool b_any(Box81 a) { return (a.lo | a.hi) != 0; }
+template <int N> __device__ __forceinline__ Box81 b_shl(Box81 a) { return {a.lo << N, ((a.hi << N) | (a.lo >> (64 - N))) & 0x1ffffull}; }
+template <int N> __device__ __forceinline__ Box81 b_shr(Box81 a) { return {(a.lo >> N) | (a.hi << (64 - N)), a.hi >> N}; }
+__device__ __forceinline__ bool b_bit(Box81 a, int c) { return ((c < 64 ? a.lo >> c : a.hi >> (c - 64)) & 1) != 0; }
+// number of set bits below bit c
+__device__ __forceinline__ int b_rank(Box81 a, int c) {
+    return c < 64 ? __popcll(a.lo & ((1ull << c) - 1)) : __popcll(a.lo) + __popcll(a.hi & ((1ull << (c - 64)) - 1));
+}
+__device__ __forceinline__ int b_count(Box81 a) { return __popcll(a.lo) + __popcll(a.hi); }
+// rows a..b (empty when a > b)
+__device__ __forceinline__ Box81 b_rows(int a, int b) {
+    Box81 m = {0, 0};
+    for (int r = a; r <= b; r++) {
+        const int s = 9 * r;
+        if (s < 64) m.lo |= 0x1ffull << s;
+        if (s > 55) m.hi |= (s >= 64) ? (0x1ffull << (s - 64)) : (0x1ffull >> (64 - s));
+    }
+    return m;
+}
+// columns a..b of every row (empty when a > b)
+__device__ __forceinline__ Box81 b_cols(int a, int b) {
+    const uint64_t m9 = (a > b) ? 0 : (((1ull << (b + 1)) - 1) & ~((1ull << a) - 1));
+    Box81 m = {0, 0};
+#pragma unroll
+    for (int r = 0; r < 9; r++) {
+        const int s = 9 * r;
+        if (s < 64) m.lo |= m9 << s;
+        if (s > 55) m.hi |= (s >= 64) ? (m9 << (s - 64)) : (m9 >> (64 - s));
+    }
+    return m;
+}
 
 struct LmaOut {
     bool ok;
     double disp, strength, wm, wxy;
 };
 
-// One Correlations2dLMA fit of tile t (its 16 lanes) over the groups of F, started at xcenter
-// (pixels along the disparity axis, = -disparity).  C2:1754-1900.
-__device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit& F, const float* __restrict__ tcorr,
-                                       double xcenter, int l, int sub, double2* s_vw, uint8_t* s_new,
-                                       uint8_t* s_cvx, double* s_acc) {
+// Per-fit LDS of one tile: values of the box cells (group g at +81 g), compacted sample lists, and
+// the sums of the accepted parameter vector.
+struct LmaLds {
+    double* v;
+    uint8_t* idx;
+    double* acc;
+};
+
+// One Correlations2dLMA fit (C2:1754-1900) over the groups of *F, started at xcenter (pixels along
+// the disparity axis, = -disparity), on W lanes (fl = lane within the fit, fmask = the fit's lanes
+// in the wavefront).  NC core parameters: X0, Wm, Wxy (3) or X0, Wm, Wyd, Wxy (4, when Wyd is
+// adjusted); then one amplitude per group.
+template <int W, int NC>
+__device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, const float* __restrict__ tcorr,
+                                          double xcenter, int fl, int lane, uint64_t fmask, const LmaLds& S) {
+    constexpr int CPL = (LMA_CELLS + W - 1) / W;   // box cells per lane
+    constexpr int NP = NC + 2;
+    // sums: core triangle, per group {core x amplitude, amplitude^2, J^T r of the amplitude},
+    // J^T r of the core, weighted residual square sum
+    constexpr int I_CROSS = NC * (NC + 1) / 2, I_DIAG = I_CROSS + 2 * NC, I_BAG = I_DIAG + 2;
+    constexpr int I_BC = I_BAG + 2, I_RMS = I_BC + NC, NV = I_RMS + 1;
+    constexpr int P_WXY = NC - 1;
+    static_assert(NV <= LMA_ACC, "LDS room for the accepted sums");
     LmaOut out = {false, 0.0, 0.0, 0.0, 0.0};
-    const int G = F.ngroups;
+    const int G = F->ngroups;
     if (G == 0) return out;
     const int hwin = A.hwin;
-    unsigned smp_bits = 0;                       // [group][k] own cells that are samples
-    double gw_raw[LMA_MAXG] = {0.0, 0.0};        // per group: sum of raw sample weights (group_weights)
-    double sw_fit = 0.0, vmax = -INFINITY;
-    int x0s[LMA_MAXG], y0s[LMA_MAXG];
+    int ns[LMA_MAXG] = {0, 0}, x0s[LMA_MAXG] = {0, 0}, y0s[LMA_MAXG] = {0, 0};
+    double gw_raw[LMA_MAXG] = {0.0, 0.0}, sw_fit = 0.0, vmax = -INFINITY;
+    Box81 cvs[LMA_MAXG] = {{0, 0}, {0, 0}};   // selected convex cells (weight class 1)
+    Box81 bad[LMA_MAXG] = {{0, 0}, {0, 0}};   // samples whose value or weight is NaN (weight 0)
 #pragma unroll
     for (int g = 0; g < LMA_MAXG; g++) {
         if (g >= G) break;
-        const LmaGroup& grp = F.g[g];
+        const LmaGroup& grp = F->g[g];
+        const int npairs = grp.npairs;
         const int ixc = (int)floor(xcenter / grp.scale + 0.5);   // Math.round (C2:2256)
         const int x0 = ixc + 7, y0 = grp.diag ? ixc + 7 : 7;
         x0s[g] = x0;
         y0s[g] = y0;
-        double2* vw = s_vw + g * LMA_CELLS;
-        // combined correlation of the group's pairs over the box (combineCompatiblePairs, C2:462-488)
-#pragma unroll 1
-        for (int k = 0; k < LMA_CPL; k++) {
-            const int c = l + 16 * k;
-            if (c >= LMA_CELLS) break;
-            const int row = y0 - 4 + c / 9, col = x0 - 4 + c % 9;
-            double v = 0.0;
-            if (row >= 0 && row < 15 && col >= 0 && col < 15) {
-                for (int p = 0; p < grp.npairs; p++) {
-                    const int xf = grp.xform[p];
-                    const int src = (xf == 1) ? col * 15 + row : ((xf == 2) ? (14 - row) * 15 + col : row * 15 + col);
-                    v += (double)tcorr[grp.pair[p] * CLT_CORR_LEN + src];
-                }
-                if (grp.npairs > 1) v /= grp.npairs;
+        double* v = S.v + g * LMA_CELLS;
+        uint8_t* idx = S.idx + g * LMA_CELLS;
+        // combined correlation of the group's pairs over the box (combineCompatiblePairs, C2:462-488):
+        // every load of the lane's cells is issued before the first is used
+        const float* pp[LMA_GPAIRS];
+        int xf[LMA_GPAIRS];
+#pragma unroll
+        for (int q = 0; q < LMA_GPAIRS; q++) {
+            pp[q] = tcorr + grp.pair[q < npairs ? q : 0] * CLT_CORR_LEN;
+            xf[q] = grp.xform[q < npairs ? q : 0];
+        }
+        float raw[CPL][LMA_GPAIRS];
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            const int c = fl + W * k;
+            const int br = (c * 57) >> 9, bc = c - 9 * br;   // c / 9, c % 9 for c < 81
+            const int row = y0 - 4 + br, col = x0 - 4 + bc;
+            const bool ing = c < LMA_CELLS && row >= 0 && row < 15 && col >= 0 && col < 15;
+            const int r0 = ing ? row : 0, c0 = ing ? col : 0;   // loads stay unconditional and in bounds
+#pragma unroll
+            for (int q = 0; q < LMA_GPAIRS; q++) {
+                const int rr = (xf[q] == 2) ? 14 - r0 : r0;
+                const int src = (xf[q] == 1) ? c0 * 15 + r0 : rr * 15 + c0;
+                raw[k][q] = pp[q][src];
             }
-            vw[c].x = v;
+        }
+        __asm__ volatile("" ::: "memory");   // all loads in flight before the first use
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            const int c = fl + W * k;
+            const int br = (c * 57) >> 9, bc = c - 9 * br;
+            const int row = y0 - 4 + br, col = x0 - 4 + bc;
+            const bool ing = c < LMA_CELLS && row >= 0 && row < 15 && col >= 0 && col < 15;
+#pragma unroll
+            for (int q = 0; q < LMA_GPAIRS; q++) raw[k][q] = (ing && q < npairs) ? raw[k][q] : 0.f;
+        }
+        // the reference divides the sum by the pair count; for 2 and 4 a product is the same number
+        const bool pow2 = (npairs & (npairs - 1)) == 0;
+        const double rn = 1.0 / npairs;
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            const int c = fl + W * k;
+            double val = 0.0;
+#pragma unroll
+            for (int q = 0; q < LMA_GPAIRS; q++) val += (double)raw[k][q];
+            if (npairs > 1) val = pow2 ? val * rn : val / npairs;
+            if (c < LMA_CELLS) v[c] = val;
         }
         wave_sync();
-        // bi-convex cells inside the clamped window (C2:2046-2064)
-        const int min_row = max(y0 - hwin, 0), max_row = min(y0 + hwin, 14);
-        const int min_col = max(x0 - hwin, 0), max_col = min(x0 + hwin, 14);
-        unsigned own_cvx = 0, sel = 0;
-        const bool start_in = (y0 >= min_row && y0 <= max_row && x0 >= min_col && x0 <= max_col);
-#pragma unroll 1
-        for (int k = 0; k < LMA_CPL; k++) {
-            const int c = l + 16 * k;
-            if (c >= LMA_CELLS) break;
-            const int row = y0 - 4 + c / 9, col = x0 - 4 + c % 9;
-            bool cv = false;
-            if (row > min_row && row < max_row && col > min_col && col < max_col) {
-                const double d = vw[c].x;
-                cv = (2 * d > (vw[c - 1].x + vw[c + 1].x)) && (2 * d > (vw[c - 9].x + vw[c + 9].x));
+        // bi-convex cells inside the clamped window (C2:2046-2064), in box coordinates (box row br is
+        // frame row y0 - 4 + br); the 81-bit masks are the same in all lanes of the fit
+        const int gr0 = max(4 - y0, 0), gr1 = min(18 - y0, 8), gc0 = max(4 - x0, 0), gc1 = min(18 - x0, 8);
+        const int wr0 = max(4 - hwin, gr0), wr1 = min(4 + hwin, gr1), wc0 = max(4 - hwin, gc0), wc1 = min(4 + hwin, gc1);
+        Box81 cv = {0, 0};
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            const int c = fl + W * k;
+            const int br = (c * 57) >> 9, bc = c - 9 * br;
+            bool conv = false;
+            if (c < LMA_CELLS && br > wr0 && br < wr1 && bc > wc0 && bc < wc1) {
+                const double d = v[c];
+                conv = (2 * d > (v[c - 1] + v[c + 1])) && (2 * d > (v[c - 9] + v[c + 9]));
             }
-            own_cvx |= (unsigned)cv << k;
-            s_cvx[c] = cv;
-            const bool st = start_in && c == 40;
-            sel |= (unsigned)st << k;
-            s_new[c] = st;
+            const uint64_t bits = (__ballot(conv) & fmask) >> (lane - fl);   // cells W k .. W k + W - 1
+            if (W * k < 64) cv.lo |= bits << (W * k);
+            if (W * k + W > 64) cv.hi |= (W * k >= 64) ? (bits << (W * k - 64)) : (bits >> (64 - W * k));
         }
-        wave_sync();
-        // grow the selection ring by ring (C2:2073-2195); sel / s_new hold the same set between steps
+        cv.hi &= 0x1ffffull;
+        // the selection grows ring by ring from the start cell (C2:2066-2195): the sources of step
+        // istep are the selected cells of ring istep, convex ones (any on the first step)
+        Box81 sel = {0, 0};
+        if (4 >= wr0 && 4 <= wr1 && 4 >= wc0 && 4 <= wc1) sel.lo = 1ull << 40;
+        const Box81 dl = b_cols(wc0 + 1, 4), dr = b_cols(4, wc1 - 1);   // diagonal steps of vertical moves
+        const Box81 du = b_rows(wr0 + 1, 4), dd = b_rows(4, wr1 - 1);   // ... of horizontal moves
         for (int istep = 0; istep < hwin; istep++) {
-            const bool first = istep == 0;
-            const int smin_row = max(y0 - istep, 0), smax_row = min(y0 + istep, 14);
-            const int smin_col = max(x0 - istep, 0), smax_col = min(x0 + istep, 14);
-#pragma unroll 1
-            for (int k = 0; k < LMA_CPL; k++) {
-                const int c = l + 16 * k;
-                if (c >= LMA_CELLS) break;
-                if (!((sel >> k) & 1) || !(((own_cvx >> k) & 1) || first)) continue;
-                const int row = y0 - 4 + c / 9, col = x0 - 4 + c % 9;
-                if (istep < y0 && row == y0 - istep && col >= smin_col && col <= smax_col) {        // up
-                    const int j = c - 9;
-                    s_new[j] = 1;
-                    if (s_cvx[j]) {
-                        if (col <= x0 && col > min_col) s_new[j - 1] = 1;
-                        if (col >= x0 && col < max_col) s_new[j + 1] = 1;
-                    }
-                }
-                if (y0 + istep < max_row && row == y0 + istep && col >= smin_col && col <= smax_col) {   // down
-                    const int j = c + 9;
-                    s_new[j] = 1;
-                    if (s_cvx[j]) {
-                        if (col <= x0 && col > min_col) s_new[j - 1] = 1;
-                        if (col >= x0 && col < max_col) s_new[j + 1] = 1;
-                    }
-                }
-                if (istep < x0 && col == x0 - istep && row >= smin_row && row <= smax_row) {        // left
-                    const int j = c - 1;
-                    s_new[j] = 1;
-                    if (s_cvx[j]) {
-                        if (row <= y0 && row > min_row) s_new[j - 9] = 1;
-                        if (row >= y0 && row < max_row) s_new[j + 9] = 1;
-                    }
-                }
-                if (x0 + istep < max_col && col == x0 + istep && row >= smin_row && row <= smax_row) {   // right
-                    const int j = c + 1;
-                    s_new[j] = 1;
-                    if (s_cvx[j]) {
-                        if (row <= y0 && row > min_row) s_new[j - 9] = 1;
-                        if (row >= y0 && row < max_row) s_new[j + 9] = 1;
-                    }
-                }
+            const Box81 src = (istep == 0) ? sel : b_and(sel, cv);
+            const Box81 scols = b_cols(max(4 - istep, gc0), min(4 + istep, gc1));
+            const Box81 srows = b_rows(max(4 - istep, gr0), min(4 + istep, gr1));
+            Box81 nw = sel;
+            if (istep < y0) {   // up
+                const Box81 t = b_shr<9>(b_and(src, b_and(b_rows(4 - istep, 4 - istep), scols)));
+                const Box81 d = b_and(t, cv);
+                nw = b_or(nw, b_or(t, b_or(b_shr<1>(b_and(d, dl)), b_shl<1>(b_and(d, dr)))));
             }
-            wave_sync();
-            unsigned nsel = 0;
-#pragma unroll 1
-            for (int k = 0; k < LMA_CPL; k++) {
-                const int c = l + 16 * k;
-                if (c >= LMA_CELLS) break;
-                nsel |= (unsigned)(s_new[c] != 0) << k;
+            if (4 + istep < wr1) {   // down
+                const Box81 t = b_shl<9>(b_and(src, b_and(b_rows(4 + istep, 4 + istep), scols)));
+                const Box81 d = b_and(t, cv);
+                nw = b_or(nw, b_or(t, b_or(b_shr<1>(b_and(d, dl)), b_shl<1>(b_and(d, dr)))));
             }
-            const bool grew = row_any((nsel & ~sel) != 0, sub);
-            sel = nsel;
+            if (istep < x0) {   // left
+                const Box81 t = b_shr<1>(b_and(src, b_and(b_cols(4 - istep, 4 - istep), srows)));
+                const Box81 d = b_and(t, cv);
+                nw = b_or(nw, b_or(t, b_or(b_shr<9>(b_and(d, du)), b_shl<9>(b_and(d, dd)))));
+            }
+            if (4 + istep < wc1) {   // right
+                const Box81 t = b_shl<1>(b_and(src, b_and(b_cols(4 + istep, 4 + istep), srows)));
+                const Box81 d = b_and(t, cv);
+                nw = b_or(nw, b_or(t, b_or(b_shr<9>(b_and(d, du)), b_shl<9>(b_and(d, dd)))));
+            }
+            const bool grew = b_any(b_andn(nw, sel));
+            sel = nw;
             if (!grew) break;
         }
-        wave_sync();
-        // cell weights (C2:2197-2206) and samples (addSamples, C2:2230-2338)
+        // cell weights (C2:2197-2206): selected convex 1, other selected nc_cost, with add3x3 the
+        // 3x3 around the start at least nc_cost; the samples (addSamples, C2:2230-2338) in cell
+        // order; NaN values or weights enter the fit as 0 (Correlations2dLMA:262-266)
+        const Box81 grid = b_and(b_rows(gr0, gr1), b_cols(gc0, gc1));
+        Box81 smp = sel;
+        if (A.add3x3) smp = b_or(smp, b_and(b_rows(3, 5), b_cols(3, 5)));
+        smp = b_and(smp, grid);
+        if (!(A.nc_cost > 0.0)) smp = b_and(sel, cv);   // the non-convex cells weigh nothing
+        cvs[g] = b_and(sel, cv);                      // weight class 1
         double gsum = 0.0;
-#pragma unroll 1
-        for (int k = 0; k < LMA_CPL; k++) {
-            const int c = l + 16 * k;
-            if (c >= LMA_CELLS) break;
-            const int br = c / 9, bc = c % 9;
-            const int row = y0 - 4 + br, col = x0 - 4 + bc;
-            const bool ing = row >= 0 && row < 15 && col >= 0 && col < 15;
-            double w = ((sel >> k) & 1) ? (((own_cvx >> k) & 1) ? 1.0 : A.nc_cost) : 0.0;
-            if (A.add3x3 && ing && br >= 3 && br <= 5 && bc >= 3 && bc <= 5 && w == 0.0) w = A.nc_cost;
-            w *= grp.npairs;
-            double wv = 0.0, vv = 0.0;
-            if (ing && w > 0.0) {
-                smp_bits |= 1u << (g * LMA_CPL + k);
-                const double v = vw[c].x;
-                if (A.vasw_pwr != 0.0) w *= (A.vasw_pwr == 2.0) ? v * v : pow(fabs(v), A.vasw_pwr);
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            const int c = fl + W * k;
+            bool nan = false;
+            if (c < LMA_CELLS && b_bit(smp, c)) {
+                const double val = v[c];
+                double w = (b_bit(cvs[g], c) ? 1.0 : A.nc_cost) * npairs;
+                if (A.vasw_pwr != 0.0) w *= (A.vasw_pwr == 2.0) ? val * val : lma_pow(val, A.vasw_pwr);
                 gsum += w;
-                if (!isnan(v) && !isnan(w)) {
-                    wv = w;
-                    vv = v;
-                }
-                sw_fit += wv;
-                vmax = fmax(vmax, vv);
+                nan = isnan(val) || isnan(w);
+                if (nan) v[c] = 0.0;
+                sw_fit += nan ? 0.0 : w;
+                vmax = fmax(vmax, nan ? 0.0 : val);
+                idx[b_rank(smp, c)] = (uint8_t)c;
             }
-            vw[c] = make_double2(vv, wv);
+            const uint64_t bits = (__ballot(nan) & fmask) >> (lane - fl);
+            if (W * k < 64) bad[g].lo |= bits << (W * k);
+            if (W * k + W > 64) bad[g].hi |= (W * k >= 64) ? (bits << (W * k - 64)) : (bits >> (64 - W * k));
         }
-        gw_raw[g] = row_sum(gsum);
+        ns[g] = b_count(smp);
+        gw_raw[g] = fit_sum<W>(gsum);
     }
     // groups are numbered in the order of their first sample (Correlations2dLMA:196-210)
-    bool gex[LMA_MAXG];
-    int ag_of[LMA_MAXG], ng = 0;
-#pragma unroll
-    for (int g = 0; g < LMA_MAXG; g++) {
-        gex[g] = g < G && row_any(((smp_bits >> (g * LMA_CPL)) & 0x3fu) != 0, sub);
-        ag_of[g] = gex[g] ? 4 + ng++ : -1;
-    }
+    const bool ex0 = ns[0] > 0, ex1 = G > 1 && ns[1] > 0;
+    const int ng = (int)ex0 + (int)ex1;
     if (ng == 0) return out;
-    sw_fit = row_sum(sw_fit);
-    vmax = row_max(vmax);
-    // setWeightsValues (Correlations2dLMA:245-278): weights normalised to a unit sum with the two
-    // regularisation points (Wm -> half_width at cost_wm, Wxy -> 0 at cost_wxy)
+    sw_fit = fit_sum<W>(sw_fit);
+    vmax = fit_max<W>(vmax);
+    // setWeightsValues (Correlations2dLMA:245-278): unit total weight with the two regularisation
+    // points (Wm -> half_width at cost_wm, Wxy -> 0 at cost_wxy)
     const double tot = sw_fit + A.cost_wm + A.cost_wxy;
     const double inv = (tot != 0.0) ? 1.0 / tot : 1.0;
     const double wr0 = A.cost_wm * inv, wr1 = A.cost_wxy * inv;
     double gwn[LMA_MAXG];
 #pragma unroll
     for (int g = 0; g < LMA_MAXG; g++) gwn[g] = (sw_fit > 0.0) ? gw_raw[g] / sw_fit : gw_raw[g];
-#pragma unroll
-    for (int g = 0; g < LMA_MAXG; g++)
-#pragma unroll 1
-        for (int k = 0; k < LMA_CPL; k++) {
-            if (g >= G) break;
-            const int c = l + 16 * k;
-            if (c >= LMA_CELLS) break;
-            s_vw[g * LMA_CELLS + c].y *= inv;
-        }
     wave_sync();
-    // parameters: X0, Wm, Wyd, Wxy, Ag0, Ag1 (initVector, Correlations2dLMA:212-243)
-    bool msk[6] = {true, A.adj_wm != 0, A.adj_wyd != 0, A.adj_wxy != 0, ng > 0 && A.adj_ag != 0,
-                   ng > 1 && A.adj_ag != 0};
-    double p[6] = {xcenter, A.half_width, 0.0, 0.0, vmax, vmax};
-    double acc[LMA_NACC];
+    // parameters (initVector, Correlations2dLMA:212-243): X0, Wm, [Wyd,] Wxy, Ag of each group
+    bool msk[NP];
+    double p[NP], pn[NP];
+    msk[0] = true;
+    msk[1] = A.adj_wm != 0;
+    if constexpr (NC == 4) msk[2] = A.adj_wyd != 0;
+    msk[P_WXY] = A.adj_wxy != 0;
+    msk[NC] = A.adj_ag != 0;
+    msk[NC + 1] = A.adj_ag != 0 && ng > 1;
+#pragma unroll
+    for (int i = 0; i < NP; i++) p[i] = 0.0;
+    p[0] = xcenter;
+    p[1] = A.half_width;
+    p[NC] = p[NC + 1] = vmax;
     const double sqrt2 = 1.4142135623730951;
+    double acc[NV];
 
     // one pass over the samples at parameters q: normal matrix, J^T W (y - f) and the weighted
     // residual sum (getFxJt + getWJtJlambda + getWYmFxRms, Correlations2dLMA:368-461)
     auto evaluate = [&](const double* q) {
 #pragma unroll
-        for (int i = 0; i < LMA_NACC; i++) acc[i] = 0.0;
+        for (int i = 0; i < NV; i++) acc[i] = 0.0;
 #pragma unroll
         for (int g = 0; g < LMA_MAXG; g++) {
-            if (!gex[g]) continue;
-            const LmaGroup& grp = F.g[g];
-            const double ws = grp.diag ? sqrt2 : 1.0;
-            const double Wy = q[1] * grp.scale + q[2], Wx = Wy + q[3];
-            const double iWx = 1.0 / Wx, iWy = 1.0 / Wy;
-            const double Ag = (ag_of[g] == 4) ? q[4] : q[5];
-            const bool a0 = ag_of[g] == 4;
-            const int x0 = x0s[g], y0 = y0s[g];
-#pragma unroll 1
-            for (int k = 0; k < LMA_CPL; k++) {
-                const int c = l + 16 * k;
-                if (c >= LMA_CELLS) break;
-                if (!((smp_bits >> (g * LMA_CPL + k)) & 1)) continue;
-                const double2 s = s_vw[g * LMA_CELLS + c];
-                const int cx = x0 - 11 + c % 9, cy = y0 - 11 + c / 9;   // relative to the correlation centre
-                const double x = grp.diag ? 0.5 * (cx + cy) * grp.scale : cx * grp.scale;
-                const double y = grp.diag ? 0.5 * (cy - cx) * grp.scale : cy * grp.scale;
+            if (g >= G || ns[g] == 0) continue;
+            const int gs = (g == 1 && ex0) ? 1 : 0;   // amplitude slot of this group
+            const LmaGroup& grp = F->g[g];
+            const double ws = grp.diag ? sqrt2 : 1.0, scale = grp.scale;
+            const double Wy = q[1] * scale + ((NC == 4) ? q[2] : 0.0), Wx = Wy + q[P_WXY];
+            const double iWx = rcp64(Wx), iWy = rcp64(Wy);
+            const double Ag = gs ? q[NC + 1] : q[NC];
+            const double hx = grp.diag ? 0.5 * scale : scale;
+            const double wc1 = 1.0 * grp.npairs, wc2 = A.nc_cost * grp.npairs;
+            const int x0 = x0s[g], y0 = y0s[g], diag = grp.diag;
+            for (int j = fl; j < ns[g]; j += W) {
+                const int c = S.idx[g * LMA_CELLS + j];
+                const double sv = S.v[g * LMA_CELLS + c];
+                double sw = b_bit(bad[g], c) ? 0.0 : (b_bit(cvs[g], c) ? wc1 : wc2);
+                if (A.vasw_pwr != 0.0) sw *= (A.vasw_pwr == 2.0) ? sv * sv : lma_pow(sv, A.vasw_pwr);
+                sw *= inv;
+                const int br = (c * 57) >> 9, bc = c - 9 * br;
+                const int cx = x0 - 11 + bc, cy = y0 - 11 + br;   // relative to the correlation centre
+                const double x = (double)(diag ? cx + cy : cx) * hx;
+                const double y = (double)(diag ? cy - cx : cy) * hx;
                 const double dxw = ws * (x - q[0]) * iWx, dyw = ws * y * iWy;
                 const double d = 1.0 - dxw * dxw - dyw * dyw;
-                const double f = d * Ag;
+                const double r = sv - d * Ag, wr = sw * r;
                 const double dfdWx = Ag * 2 * dxw * dxw * iWx, dfdWy = Ag * 2 * dyw * dyw * iWy;
-                double j[6];
-                j[0] = ws * Ag * 2 * dxw * iWx;
-                j[1] = grp.scale * (dfdWx + dfdWy);
-                j[2] = dfdWx + dfdWy;
-                j[3] = dfdWx;
-                j[4] = a0 ? d : 0.0;
-                j[5] = a0 ? 0.0 : d;
-                const double r = s.x - f, wr = s.y * r;
-                acc[27] += wr * r;
+                double jc[NC];
+                jc[0] = ws * Ag * 2 * dxw * iWx;
+                jc[1] = scale * (dfdWx + dfdWy);
+                if constexpr (NC == 4) jc[2] = dfdWx + dfdWy;
+                jc[P_WXY] = dfdWx;
+                const double wd = sw * d;
 #pragma unroll
-                for (int a = 0; a < 6; a++) {
-                    acc[21 + a] += j[a] * wr;
-                    const double wj = s.y * j[a];
+                for (int a = 0; a < NC; a++) {
+                    const double wj = sw * jc[a];
 #pragma unroll
-                    for (int b = a; b < 6; b++) acc[lma_tri(a, b)] += wj * j[b];
+                    for (int b = a; b < NC; b++) acc[lma_tri(NC, a, b)] += wj * jc[b];
+                    acc[I_CROSS + gs * NC + a] += wd * jc[a];
+                    acc[I_BC + a] += jc[a] * wr;
                 }
+                acc[I_DIAG + gs] += wd * d;
+                acc[I_BAG + gs] += d * wr;
+                acc[I_RMS] += wr * r;
             }
         }
 #pragma unroll
-        for (int i = 0; i < LMA_NACC; i++) acc[i] = row_sum(acc[i]);
+        for (int i = 0; i < NV; i++) {
+            const bool slot1 = (i >= I_CROSS + NC && i < I_DIAG) || i == I_DIAG + 1 || i == I_BAG + 1;
+            if (!slot1 || ng > 1) acc[i] = fit_sum<W>(acc[i]);
+        }
         // the two regularisation points
-        const double r0 = A.half_width - q[1], r1 = -q[3];
-        acc[lma_tri(1, 1)] += wr0;
-        acc[lma_tri(3, 3)] += wr1;
-        acc[22] += wr0 * r0;
-        acc[24] += wr1 * r1;
-        acc[27] += wr0 * r0 * r0 + wr1 * r1 * r1;
+        const double r0 = A.half_width - q[1], r1 = -q[P_WXY];
+        acc[lma_tri(NC, 1, 1)] += wr0;
+        acc[lma_tri(NC, P_WXY, P_WXY)] += wr1;
+        acc[I_BC + 1] += wr0 * r0;
+        acc[I_BC + P_WXY] += wr1 * r1;
+        acc[I_RMS] += wr0 * r0 * r0 + wr1 * r1 * r1;
+    };
+    auto keep = [&]() {   // the accepted point's sums, read by the next solve
+        if (fl == 0)
+#pragma unroll
+            for (int i = 0; i < NV; i++) S.acc[i] = acc[i];
+        wave_sync();
     };
 
-    // Iteration -1 evaluates the start point; every later one solves the damped normal equations
-    // of the accepted point (its sums in LDS, ping-pong buffers cur / 1 - cur) and evaluates the
-    // trial point (lmaStep, Correlations2dLMA:533-640; runLma 477-530).
-    double pn[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) pn[i] = p[i];
-    double rms0 = 0.0, rms = 0.0, lambda = A.lambda0;
+    evaluate(p);
+    const double rms0 = sqrt(acc[I_RMS]);
+    double rms = rms0, lambda = A.lambda0;
+    keep();
     bool good = false;
-    int cur = 0;
-    for (int iter = -1; iter < A.num_iter; iter++) {
-        if (iter >= 0) {
-            // (J^T W J + lambda diag(J^T W J)) delta = J^T W (y - f) over the adjusted parameters, LDL^T
-            const double* sa = s_acc + cur * 32;
-            double L[6][6], dd[6], x[6];
-            bool singular = false;
+    for (int iter = 0; iter < A.num_iter; iter++) {
+        // (J^T W J + lambda diag(J^T W J)) delta = J^T W (y - f) over the adjusted parameters
+        // (lmaStep, Correlations2dLMA:533-640), LDL^T; a zero pivot is the singular matrix
+        double L[NP][NP], dd[NP], x[NP];
+        bool singular = false;
 #pragma unroll
-            for (int k = 0; k < 6; k++) {
-                const double v = sa[lma_tri(k, k)];
-                double dk = msk[k] ? v + v * lambda : 1.0;
+        for (int k = 0; k < NP; k++) {
+            const double v0 = (k < NC) ? S.acc[lma_tri(NC, k, k)] : S.acc[I_DIAG + k - NC];
+            double dk = msk[k] ? v0 + v0 * lambda : 1.0;
 #pragma unroll
-                for (int s2 = 0; s2 < k; s2++) dk -= L[k][s2] * L[k][s2] * dd[s2];
-                dd[k] = dk;
-                singular |= (dk == 0.0);
-                const double idk = 1.0 / dk;
+            for (int s = 0; s < k; s++) dk -= L[k][s] * L[k][s] * dd[s];
+            dd[k] = dk;
+            singular |= (dk == 0.0);
+            const double idk = rcp64(dk);
 #pragma unroll
-                for (int i = k + 1; i < 6; i++) {
-                    double w = (msk[i] && msk[k]) ? sa[lma_tri(k, i)] : 0.0;
-#pragma unroll
-                    for (int s2 = 0; s2 < k; s2++) w -= L[i][s2] * L[k][s2] * dd[s2];
-                    L[i][k] = w * idk;
+            for (int i = k + 1; i < NP; i++) {
+                double m = 0.0;
+                if (msk[i] && msk[k]) {
+                    if (i < NC) m = S.acc[lma_tri(NC, k, i)];
+                    else if (k < NC) m = S.acc[I_CROSS + (i - NC) * NC + k];
                 }
+#pragma unroll
+                for (int s = 0; s < k; s++) m -= L[i][s] * L[k][s] * dd[s];
+                L[i][k] = m * idk;
             }
-            if (singular) break;   // lmaStep returns {false, true}
-#pragma unroll
-            for (int i = 0; i < 6; i++) {
-                double w = msk[i] ? sa[21 + i] : 0.0;
-#pragma unroll
-                for (int s2 = 0; s2 < i; s2++) w -= L[i][s2] * x[s2];
-                x[i] = w;
-            }
-#pragma unroll
-            for (int i = 5; i >= 0; i--) {
-                double w = x[i] / dd[i];
-#pragma unroll
-                for (int s2 = i + 1; s2 < 6; s2++) w -= L[s2][i] * x[s2];
-                x[i] = w;
-            }
-#pragma unroll
-            for (int i = 0; i < 6; i++) pn[i] = msk[i] ? p[i] + x[i] : p[i];
         }
+        if (singular) break;   // lmaStep returns {false, true}
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+            double m = msk[i] ? ((i < NC) ? S.acc[I_BC + i] : S.acc[I_BAG + i - NC]) : 0.0;
+#pragma unroll
+            for (int s = 0; s < i; s++) m -= L[i][s] * x[s];
+            x[i] = m;
+        }
+#pragma unroll
+        for (int i = NP - 1; i >= 0; i--) {
+            double m = x[i] * rcp64(dd[i]);
+#pragma unroll
+            for (int s = i + 1; s < NP; s++) m -= L[s][i] * x[s];
+            x[i] = m;
+        }
+#pragma unroll
+        for (int i = 0; i < NP; i++) pn[i] = msk[i] ? p[i] + x[i] : p[i];
         evaluate(pn);
-        const double rms_n = sqrt(acc[27]);
-        const int trial = iter < 0 ? 0 : 1 - cur;
-        if (l == 0)
-#pragma unroll
-            for (int i = 0; i < LMA_NACC; i++) s_acc[trial * 32 + i] = acc[i];
-        wave_sync();
-        if (iter < 0) {
-            rms0 = rms = rms_n;
-            cur = trial;
-            continue;
-        }
+        const double rms_n = sqrt(acc[I_RMS]);
         bool done = false;
         if (rms_n < rms) {
             good = true;
             done = rms_n >= rms * (1.0 - A.rms_diff);
             rms = rms_n;
-            cur = trial;
 #pragma unroll
-            for (int i = 0; i < 6; i++) p[i] = pn[i];
+            for (int i = 0; i < NP; i++) p[i] = pn[i];
+            keep();
         } else {
             good = false;
         }
@@ -405,19 +493,18 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit& F, con
         }
     }
     // runLma (Correlations2dLMA:510-524): success when the last step improved or the final rms is
-    // below the initial one; an improving step always lowers it, so the second test covers both
+    // below the initial one (an improving step always lowers it)
     out.ok = good || (rms < rms0);
     if (out.ok) {   // getDisparityStrength (Correlations2dLMA:165-185)
         double sum_amp = 0.0;
-#pragma unroll
-        for (int g = 0; g < LMA_MAXG; g++)
-            if (gex[g]) sum_amp += gwn[g] * ((ag_of[g] == 4) ? p[4] : p[5]);
+        if (ex0) sum_amp += gwn[0] * p[NC];
+        if (ex1) sum_amp += gwn[1] * (ex0 ? p[NC + 1] : p[NC]);
         const double max_amp = fmax(vmax, 0.0);
         if (sum_amp > 1.25 * max_amp) sum_amp = max_amp;
         out.disp = -p[0];
         out.strength = sum_amp;
         out.wm = p[1];
-        out.wxy = p[3];
+        out.wxy = p[P_WXY];
     }
     return out;
 }
@@ -460,30 +547,29 @@ __device__ double lma_foreground(const LmaArgs& A, const bool* have, const doubl
 
 // ID:2254-2330 over the task list: reads the raw centre of mass and strength of each record,
 // writes CM, STRENGTH (final), POLY, POLY_STR.  16 lanes per tile, 4 tiles per wavefront.
-__global__ __launch_bounds__(256, 2) void k_corr_lma(const float* __restrict__ corr, float* __restrict__ disp,
-                                                  int num_tasks, LmaArgs A) {
-    __shared__ double2 s_vw[LMA_SLOTS][LMA_MAXG * LMA_CELLS];
-    __shared__ uint8_t s_new[LMA_SLOTS][LMA_CELLS + 15];
-    __shared__ uint8_t s_cvx[LMA_SLOTS][LMA_CELLS + 15];
-    __shared__ double s_acc[LMA_SLOTS][64];
+// NC = 4 when the extra vertical width Wyd is adjusted, else 3.
+template <int NC>
+__global__ __launch_bounds__(256) void k_corr_lma(const float* __restrict__ corr, float* __restrict__ disp,
+                                                  int num_tasks, const LmaArgs* __restrict__ Ap) {
+    const LmaArgs& A = *Ap;   // in device memory: the fits index it per lane
+    __shared__ double s_v[LMA_SLOTS][2 * LMA_CELLS];
+    __shared__ uint8_t s_idx[LMA_SLOTS][2 * LMA_CELLS];
+    __shared__ double s_acc[LMA_SLOTS][2 * LMA_ACC];
     __shared__ double s_dds[LMA_SLOTS][4][4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int sub = lane >> 4, l = lane & 15, slot = wave * LMA_TILES_PER_WAVE + sub;
     const int t = (xcd_block(blockIdx.x, gridDim.x) * LMA_WAVES + wave) * LMA_TILES_PER_WAVE + sub;
-    if (t >= num_tasks) return;   // whole rows: the row reductions never see an exited lane
+    if (t >= num_tasks) return;   // whole rows: the reductions never see an exited lane
     float* rec = disp + (size_t)t * CLT_DISP_LAYERS;
     const float cm = rec[CLT_DISP_CM_RAW], st = rec[CLT_DISP_STR_RAW];
     const float* tcorr = corr + (size_t)t * A.npairs * CLT_CORR_LEN;
     double out_cm = cm, out_st = st, poly = NAN, poly_s = NAN;
-    double disparity = cm, strength = st, xc = -(double)cm;
-    bool run = !isnan(cm) && (double)st > A.min_poly_strength, dirs = false;
-    double (*dds)[4] = s_dds[slot];
-    // fit 0 over all oriented pairs, then (when it succeeded) the four one-direction fits
-    for (int j = 0; j < LMA_NFITS; j++) {
-        LmaOut r = {false, 0.0, 0.0, 0.0, 0.0};
-        if (run) r = lma_fit(A, A.fit[j], tcorr, xc, l, sub, s_vw[slot], s_new[slot], s_cvx[slot], s_acc[slot]);
-        if (j == 0) {
-            if (!r.ok) break;
+    double disparity = cm, strength = st;
+    bool dirs = false;
+    if (!isnan(cm) && (double)st > A.min_poly_strength) {   // the fit over all oriented pairs
+        const LmaLds S = {s_v[slot], s_idx[slot], s_acc[slot]};
+        const LmaOut r = lma_fit<16, NC>(A, &A.fit[0], tcorr, -(double)cm, l, lane, 0xffffull << (16 * sub), S);
+        if (r.ok) {
             poly = r.disp;
             poly_s = r.strength;
             if (A.mix) {
@@ -492,26 +578,33 @@ __global__ __launch_bounds__(256, 2) void k_corr_lma(const float* __restrict__ c
                 out_cm = disparity;
                 out_st = strength;
             }
-            run = dirs = A.fo_correct && strength > 0.0;
-            xc = -disparity;
-        } else if (l == 0) {
-            dds[j - 1][0] = r.ok ? r.disp : NAN;
-            dds[j - 1][1] = r.strength;
-            dds[j - 1][2] = r.wm;
-            dds[j - 1][3] = r.wxy;
+            dirs = A.fo_correct && strength > 0.0;
         }
     }
-    if (dirs) {   // foregroundCorrect (ID:2288-2330)
-        wave_sync();
+    if (dirs) {   // corr4dirsLMA: two one-direction fits at a time, 8 lanes each
+        const int half = l >> 3, hl = l & 7;
+        const LmaLds S = {s_v[slot] + half * LMA_CELLS, s_idx[slot] + half * LMA_CELLS, s_acc[slot] + half * LMA_ACC};
+        for (int pass = 0; pass < 2; pass++) {
+            const int f = 1 + 2 * pass + half;
+            const LmaOut q = lma_fit<8, NC>(A, &A.fit[f], tcorr, -disparity, hl, lane,
+                                            0xffull << (16 * sub + 8 * half), S);
+            if (hl == 0) {
+                s_dds[slot][f - 1][0] = q.ok ? q.disp : NAN;
+                s_dds[slot][f - 1][1] = q.strength;
+                s_dds[slot][f - 1][2] = q.wm;
+                s_dds[slot][f - 1][3] = q.wxy;
+            }
+            wave_sync();
+        }
         bool have[4];
         double dv[4][4];
 #pragma unroll
         for (int d = 0; d < 4; d++) {
 #pragma unroll
-            for (int e = 0; e < 4; e++) dv[d][e] = dds[d][e];
+            for (int e = 0; e < 4; e++) dv[d][e] = s_dds[slot][d][e];
             have[d] = !isnan(dv[d][0]);
         }
-        const double mod = lma_foreground(A, have, dv, disparity);
+        const double mod = lma_foreground(A, have, dv, disparity);   // ID:2288-2330
         if (mod != disparity && strength > A.fo_min_strength) out_cm = mod;
     }
     if (l == 0) {

@@ -164,6 +164,7 @@ struct clt_handle {
     int corr_num_strip = 0;
     clt_corr_lma_params lp{};      // per-tile LMA chain parameters
     LmaArgs lma_args{};            // ... as launched (built by prepare_corr)
+    LmaArgs* d_lma_args = nullptr; // device copy read by k_corr_lma
     bool lma_run = false;          // correlate2D is followed by k_corr_lma
     int graph_epoch = 0;           // bumped whenever a captured launch would go stale
     int graph_epoch_cap = -1;
@@ -277,8 +278,9 @@ static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
 static void launch_corr_lma(clt_handle* h, int n) {
     if (!h->lma_run) return;
     const int slots = LMA_SLOTS;
-    hipLaunchKernelGGL(k_corr_lma, dim3((n + slots - 1) / slots), dim3(64 * LMA_WAVES), 0, h->stream, h->d_corr,
-                       h->d_disp, n, h->lma_args);
+    auto kern = h->lma_args.adj_wyd ? k_corr_lma<4> : k_corr_lma<3>;
+    hipLaunchKernelGGL(kern, dim3((n + slots - 1) / slots), dim3(64 * LMA_WAVES), 0, h->stream, h->d_corr, h->d_disp, n,
+                       h->d_lma_args);
 }
 
 template <int NC>
@@ -472,7 +474,7 @@ int clt_destroy(clt_handle* h) {
                     h->d_lpf_corr, h->d_rbg, h->d_corr, h->d_combo, h->d_disp, h->d_tex, h->d_tex_stats,
                     h->d_tex_img, h->d_extra, h->d_corr_args, h->d_tex_args, h->d_port_geom, h->d_rtab,
                     h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv, h->d_task_frame, h->d_staged, h->d_fcounts,
-                    h->d_adj, h->d_tstate, h->d_gpart, h->d_gsum, h->d_gdisp2};
+                    h->d_adj, h->d_tstate, h->d_gpart, h->d_gsum, h->d_gdisp2, h->d_lma_args};
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -869,10 +871,10 @@ static int build_lma_args(clt_handle* h, bool have_records) {
                 if (sel && q.dir >= 0 && ((q.dir >= 2) == (diag != 0)) && q.scale == scale) members[n++] = p;
             }
             if (n == 0) continue;
-            if (F.ngroups == LMA_MAXG || n > LMA_GPAIRS)
+            if (F.ngroups == ((f == 0) ? LMA_MAXG : 1) || n > LMA_GPAIRS)
                 return fail(h, CLT_E_UNSUPPORTED,
-                            "per-tile LMA: at most %d pair groups of at most %d pairs (disable it with "
-                            "clt_set_corr_lma_params)", LMA_MAXG, LMA_GPAIRS);
+                            "per-tile LMA: at most %d pair groups over all pairs, one per direction, of at most %d "
+                            "pairs (disable it with clt_set_corr_lma_params)", LMA_MAXG, LMA_GPAIRS);
             LmaGroup& G = F.g[F.ngroups++];
             G.npairs = n;
             G.diag = diag;
@@ -915,6 +917,9 @@ static int build_lma_args(clt_handle* h, bool have_records) {
     a.fo_ortho = lp.fo_ortho;
     a.npairs = P;
     h->lma_run = a.fit[0].ngroups > 0;
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_lma_args, sizeof(LmaArgs)))) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->d_lma_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
     return CLT_OK;
 }
 
