@@ -26,6 +26,7 @@ constexpr int LMA_SLOTS = LMA_TILES_PER_WAVE * LMA_WAVES;
 constexpr int LMA_CELLS = 81;                // 9x9 box: half window at most 4
 constexpr int LMA_MAXG = 2;                  // compatible pair groups of the fit over all pairs
 constexpr int LMA_GPAIRS = 4;                // pairs per group (the 8-camera table has at most 4)
+constexpr int LMA_DPAIRS = 2;                // pairs per group of a one-direction fit
 constexpr int LMA_NFITS = 5;                 // all pairs, then horizontal, vertical, diag main, diag other
 constexpr int LMA_ACC = 28;                  // LDS doubles per fit for the accepted sums
 
@@ -118,27 +119,22 @@ __device__ __forceinline__ int b_rank(Box81 a, int c) {
     return c < 64 ? __popcll(a.lo & ((1ull << c) - 1)) : __popcll(a.lo) + __popcll(a.hi & ((1ull << (c - 64)) - 1));
 }
 __device__ __forceinline__ int b_count(Box81 a) { return __popcll(a.lo) + __popcll(a.hi); }
-// rows a..b (empty when a > b)
-__device__ __forceinline__ Box81 b_rows(int a, int b) {
-    Box81 m = {0, 0};
-    for (int r = a; r <= b; r++) {
-        const int s = 9 * r;
-        if (s < 64) m.lo |= 0x1ffull << s;
-        if (s > 55) m.hi |= (s >= 64) ? (0x1ffull << (s - 64)) : (0x1ffull >> (64 - s));
-    }
-    return m;
+// bits [s, e) of a 64-bit word, 0 <= s <= e <= 64
+__device__ __forceinline__ uint64_t bits64(int s, int e) {
+    const uint64_t hi = (e >= 64) ? ~0ull : ((1ull << e) - 1);
+    return (s >= e) ? 0ull : (hi & ~((1ull << s) - 1));
 }
-// columns a..b of every row (empty when a > b)
+// rows a..b (empty when a > b): bits [9a, 9b + 9)
+__device__ __forceinline__ Box81 b_rows(int a, int b) {
+    if (a > b) return {0, 0};
+    const int s = 9 * a, e = 9 * b + 9;
+    return {bits64(min(s, 64), min(e, 64)), bits64(max(s, 64) - 64, max(e, 64) - 64)};
+}
+// columns a..b of every row (empty when a > b): the 9-bit row pattern times sum_r 2^(9 r)
 __device__ __forceinline__ Box81 b_cols(int a, int b) {
-    const uint64_t m9 = (a > b) ? 0 : (((1ull << (b + 1)) - 1) & ~((1ull << a) - 1));
-    Box81 m = {0, 0};
-#pragma unroll
-    for (int r = 0; r < 9; r++) {
-        const int s = 9 * r;
-        if (s < 64) m.lo |= m9 << s;
-        if (s > 55) m.hi |= (s >= 64) ? (m9 << (s - 64)) : (m9 >> (64 - s));
-    }
-    return m;
+    const uint64_t m9 = (a > b) ? 0 : bits64(a, b + 1);
+    constexpr uint64_t COL0_LO = 0x8040201008040201ull;   // bits 0, 9, ..., 63
+    return {COL0_LO * m9, __umul64hi(COL0_LO, m9) + (m9 << 8)};   // row 8 starts at bit 72 = 64 + 8
 }
 
 struct LmaOut {
@@ -157,9 +153,10 @@ struct LmaLds {
 // One Correlations2dLMA fit (C2:1754-1900) over the groups of *F, started at xcenter (pixels along
 // the disparity axis, = -disparity), on W lanes (fl = lane within the fit, fmask = the fit's lanes
 // in the wavefront).  NC core parameters: X0, Wm, Wxy (3) or X0, Wm, Wyd, Wxy (4, when Wyd is
-// adjusted); then one amplitude per group.
-template <int W, int NC>
-__device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, const float* __restrict__ tcorr,
+// adjusted); then one amplitude per group of at most NPM pairs.  The tile's correlations start at
+// corr[toff] (a wave-uniform base and 32-bit offsets keep one register per load address).
+template <int W, int NC, int NPM>
+__device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, const float* __restrict__ corr, int toff,
                                           double xcenter, int fl, int lane, uint64_t fmask, const LmaLds& S) {
     constexpr int CPL = (LMA_CELLS + W - 1) / W;   // box cells per lane
     constexpr int NP = NC + 2;
@@ -190,14 +187,13 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         uint8_t* idx = S.idx + g * LMA_CELLS;
         // combined correlation of the group's pairs over the box (combineCompatiblePairs, C2:462-488):
         // every load of the lane's cells is issued before the first is used
-        const float* pp[LMA_GPAIRS];
-        int xf[LMA_GPAIRS];
+        int pp[NPM], xf[NPM];
 #pragma unroll
-        for (int q = 0; q < LMA_GPAIRS; q++) {
-            pp[q] = tcorr + grp.pair[q < npairs ? q : 0] * CLT_CORR_LEN;
+        for (int q = 0; q < NPM; q++) {
+            pp[q] = toff + grp.pair[q < npairs ? q : 0] * CLT_CORR_LEN;
             xf[q] = grp.xform[q < npairs ? q : 0];
         }
-        float raw[CPL][LMA_GPAIRS];
+        float raw[CPL][NPM];
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
             const int c = fl + W * k;
@@ -206,10 +202,10 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             const bool ing = c < LMA_CELLS && row >= 0 && row < 15 && col >= 0 && col < 15;
             const int r0 = ing ? row : 0, c0 = ing ? col : 0;   // loads stay unconditional and in bounds
 #pragma unroll
-            for (int q = 0; q < LMA_GPAIRS; q++) {
+            for (int q = 0; q < NPM; q++) {
                 const int rr = (xf[q] == 2) ? 14 - r0 : r0;
                 const int src = (xf[q] == 1) ? c0 * 15 + r0 : rr * 15 + c0;
-                raw[k][q] = pp[q][src];
+                raw[k][q] = corr[pp[q] + src];
             }
         }
         __asm__ volatile("" ::: "memory");   // all loads in flight before the first use
@@ -220,7 +216,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             const int row = y0 - 4 + br, col = x0 - 4 + bc;
             const bool ing = c < LMA_CELLS && row >= 0 && row < 15 && col >= 0 && col < 15;
 #pragma unroll
-            for (int q = 0; q < LMA_GPAIRS; q++) raw[k][q] = (ing && q < npairs) ? raw[k][q] : 0.f;
+            for (int q = 0; q < NPM; q++) raw[k][q] = (ing && q < npairs) ? raw[k][q] : 0.f;
         }
         // the reference divides the sum by the pair count; for 2 and 4 a product is the same number
         const bool pow2 = (npairs & (npairs - 1)) == 0;
@@ -230,7 +226,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             const int c = fl + W * k;
             double val = 0.0;
 #pragma unroll
-            for (int q = 0; q < LMA_GPAIRS; q++) val += (double)raw[k][q];
+            for (int q = 0; q < NPM; q++) val += (double)raw[k][q];
             if (npairs > 1) val = pow2 ? val * rn : val / npairs;
             if (c < LMA_CELLS) v[c] = val;
         }
@@ -240,7 +236,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         const int gr0 = max(4 - y0, 0), gr1 = min(18 - y0, 8), gc0 = max(4 - x0, 0), gc1 = min(18 - x0, 8);
         const int wr0 = max(4 - hwin, gr0), wr1 = min(4 + hwin, gr1), wc0 = max(4 - hwin, gc0), wc1 = min(4 + hwin, gc1);
         Box81 cv = {0, 0};
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < CPL; k++) {
             const int c = fl + W * k;
             const int br = (c * 57) >> 9, bc = c - 9 * br;
@@ -299,7 +295,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         if (!(A.nc_cost > 0.0)) smp = b_and(sel, cv);   // the non-convex cells weigh nothing
         cvs[g] = b_and(sel, cv);                      // weight class 1
         double gsum = 0.0;
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < CPL; k++) {
             const int c = fl + W * k;
             bool nan = false;
@@ -549,7 +545,7 @@ __device__ double lma_foreground(const LmaArgs& A, const bool* have, const doubl
 // writes CM, STRENGTH (final), POLY, POLY_STR.  16 lanes per tile, 4 tiles per wavefront.
 // NC = 4 when the extra vertical width Wyd is adjusted, else 3.
 template <int NC>
-__global__ __launch_bounds__(256) void k_corr_lma(const float* __restrict__ corr, float* __restrict__ disp,
+__global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ corr, float* __restrict__ disp,
                                                   int num_tasks, const LmaArgs* __restrict__ Ap) {
     const LmaArgs& A = *Ap;   // in device memory: the fits index it per lane
     __shared__ double s_v[LMA_SLOTS][2 * LMA_CELLS];
@@ -562,13 +558,14 @@ __global__ __launch_bounds__(256) void k_corr_lma(const float* __restrict__ corr
     if (t >= num_tasks) return;   // whole rows: the reductions never see an exited lane
     float* rec = disp + (size_t)t * CLT_DISP_LAYERS;
     const float cm = rec[CLT_DISP_CM_RAW], st = rec[CLT_DISP_STR_RAW];
-    const float* tcorr = corr + (size_t)t * A.npairs * CLT_CORR_LEN;
+    const int toff = t * A.npairs * CLT_CORR_LEN;
     double out_cm = cm, out_st = st, poly = NAN, poly_s = NAN;
     double disparity = cm, strength = st;
     bool dirs = false;
     if (!isnan(cm) && (double)st > A.min_poly_strength) {   // the fit over all oriented pairs
         const LmaLds S = {s_v[slot], s_idx[slot], s_acc[slot]};
-        const LmaOut r = lma_fit<16, NC>(A, &A.fit[0], tcorr, -(double)cm, l, lane, 0xffffull << (16 * sub), S);
+        const LmaOut r = lma_fit<16, NC, LMA_GPAIRS>(A, &A.fit[0], corr, toff, -(double)cm, l, lane,
+                                                     0xffffull << (16 * sub), S);
         if (r.ok) {
             poly = r.disp;
             poly_s = r.strength;
@@ -586,8 +583,8 @@ __global__ __launch_bounds__(256) void k_corr_lma(const float* __restrict__ corr
         const LmaLds S = {s_v[slot] + half * LMA_CELLS, s_idx[slot] + half * LMA_CELLS, s_acc[slot] + half * LMA_ACC};
         for (int pass = 0; pass < 2; pass++) {
             const int f = 1 + 2 * pass + half;
-            const LmaOut q = lma_fit<8, NC>(A, &A.fit[f], tcorr, -disparity, hl, lane,
-                                            0xffull << (16 * sub + 8 * half), S);
+            const LmaOut q = lma_fit<8, NC, LMA_DPAIRS>(A, &A.fit[f], corr, toff, -disparity, hl, lane,
+                                                        0xffull << (16 * sub + 8 * half), S);
             if (hl == 0) {
                 s_dds[slot][f - 1][0] = q.ok ? q.disp : NAN;
                 s_dds[slot][f - 1][1] = q.strength;

@@ -166,6 +166,12 @@ struct clt_handle {
     LmaArgs lma_args{};            // ... as launched (built by prepare_corr)
     LmaArgs* d_lma_args = nullptr; // device copy read by k_corr_lma
     bool lma_run = false;          // correlate2D is followed by k_corr_lma
+    // k_corr_lma runs on its own stream, concurrently with the HBM-bound kernels that follow
+    // correlate2D (textures, imclt, the next convert); ev_lma joins it back before its outputs are
+    // read or correlate2D overwrites its inputs
+    hipStream_t stream_lma = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_lma = nullptr;
+    bool lma_pending = false;
     int graph_epoch = 0;           // bumped whenever a captured launch would go stale
     int graph_epoch_cap = -1;
     CorrArgs* d_corr_args = nullptr;
@@ -214,13 +220,24 @@ static int dalloc(clt_handle* h, void** p, size_t bytes) {
     return CLT_OK;
 }
 
+// The main stream waits for the last per-tile LMA (its outputs are read or its inputs rewritten).
+static void join_lma(clt_handle* h) {
+    if (!h->lma_pending) return;
+    (void)hipStreamWaitEvent(h->stream, h->ev_lma, 0);
+    h->lma_pending = false;
+}
+static hipError_t sync_streams(clt_handle* h) {
+    join_lma(h);
+    return hipStreamSynchronize(h->stream);
+}
+
 static int finish(clt_handle* h) {
     if (h->async) return CLT_OK;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     return CLT_OK;
 }
 
-static void tbegin(clt_handle* h, int which) {
+static void tbegin(clt_handle* h, int which, hipStream_t st = nullptr) {
     if (!h->timing) return;
     const int u = h->ev_used[which];
     if (u == (int)h->ev_beg[which].size()) {
@@ -229,13 +246,13 @@ static void tbegin(clt_handle* h, int which) {
         h->ev_beg[which].push_back(b);
         h->ev_end[which].push_back(e);
     }
-    (void)hipEventRecord(h->ev_beg[which][u], h->stream);
+    (void)hipEventRecord(h->ev_beg[which][u], st ? st : h->stream);
 }
-static void tend(clt_handle* h, int which) {
+static void tend(clt_handle* h, int which, hipStream_t st = nullptr) {
     if (!h->timing) return;
     const int u = h->ev_used[which];
     if (u >= (int)h->ev_end[which].size()) return;
-    (void)hipEventRecord(h->ev_end[which][u], h->stream);
+    (void)hipEventRecord(h->ev_end[which][u], st ? st : h->stream);
     h->ev_used[which] = u + 1;
 }
 
@@ -260,6 +277,7 @@ static void launch_imclt(clt_handle* h, int frames, int apply_lpf) {
 }
 
 static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
+    join_lma(h);   // the previous LMA still reads d_corr and d_disp
     if (h->corr_quad) {
         const int tpb = h->tune_corr_tpb;
         auto kern = (h->cp.fat_zero != 0.f) ? k_corr2d_quad<true> : k_corr2d_quad<false>;
@@ -274,13 +292,20 @@ static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
     }
 }
 
-// Per-tile LMA chain over the records just written by correlate2D (16 tiles per 256-thread block).
-static void launch_corr_lma(clt_handle* h, int n) {
+// Per-tile LMA chain over the records just written by correlate2D (16 tiles per 256-thread block),
+// forked onto its own stream; the caller joins it (join_lma, or the end of a captured batch).
+static void launch_corr_lma(clt_handle* h, int n, bool timed) {
     if (!h->lma_run) return;
     const int slots = LMA_SLOTS;
     auto kern = h->lma_args.adj_wyd ? k_corr_lma<4> : k_corr_lma<3>;
-    hipLaunchKernelGGL(kern, dim3((n + slots - 1) / slots), dim3(64 * LMA_WAVES), 0, h->stream, h->d_corr, h->d_disp, n,
-                       h->d_lma_args);
+    (void)hipEventRecord(h->ev_fork, h->stream);
+    (void)hipStreamWaitEvent(h->stream_lma, h->ev_fork, 0);
+    if (timed) tbegin(h, T_CORR_LMA, h->stream_lma);
+    hipLaunchKernelGGL(kern, dim3((n + slots - 1) / slots), dim3(64 * LMA_WAVES), 0, h->stream_lma, h->d_corr, h->d_disp,
+                       n, h->d_lma_args);
+    if (timed) tend(h, T_CORR_LMA, h->stream_lma);
+    (void)hipEventRecord(h->ev_lma, h->stream_lma);
+    h->lma_pending = true;
 }
 
 template <int NC>
@@ -429,6 +454,12 @@ static int create_handle(const clt_geom* geom, int frames, clt_handle** out) {
         delete h;
         return CLT_E_HIP;
     }
+    if (hipStreamCreateWithFlags(&h->stream_lma, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_lma, hipEventDisableTiming) != hipSuccess) {
+        (void)clt_destroy(h);
+        return CLT_E_HIP;
+    }
     Tables tab;
     make_tables(&tab);
     const size_t F = frames;
@@ -475,7 +506,7 @@ int clt_destroy(clt_handle* h) {
                     h->d_tex_img, h->d_extra, h->d_corr_args, h->d_tex_args, h->d_port_geom, h->d_rtab,
                     h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv, h->d_task_frame, h->d_staged, h->d_fcounts,
                     h->d_adj, h->d_tstate, h->d_gpart, h->d_gsum, h->d_gdisp2, h->d_lma_args};
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->stream) (void)sync_streams(h);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
     for (void* p : bufs)
@@ -485,6 +516,9 @@ int clt_destroy(clt_handle* h) {
         for (hipEvent_t e : h->ev_end[i]) (void)hipEventDestroy(e);
     }
     if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream_lma) (void)hipStreamDestroy(h->stream_lma);
+    if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+    if (h->ev_lma) (void)hipEventDestroy(h->ev_lma);
     delete h;
     return CLT_OK;
 }
@@ -526,7 +560,7 @@ int clt_set_image(clt_handle* h, int32_t cam, const float* image, int32_t stride
     const clt_geom& g = h->g;
     if (cam < 0 || cam >= g.num_cams || !image || stride < g.width)
         return fail(h, CLT_E_ARG, "clt_set_image: bad camera %d, null image or stride %d < width %d", cam, stride, g.width);
-    HIPCHK(h, hipStreamSynchronize(h->stream));   // the stream may still read the previous frame
+    HIPCHK(h, sync_streams(h));   // the stream may still read the previous frame
     HIPCHK(h, hipMemcpy2D(h->d_images + (size_t)cam * g.width * g.height, (size_t)g.width * 4, image, (size_t)stride * 4,
                           (size_t)g.width * 4, g.height, hipMemcpyHostToDevice));
     h->images_set[cam] = true;
@@ -539,7 +573,7 @@ int clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks) {
     if (num_tasks < 0 || num_tasks > h->max_tasks || (num_tasks > 0 && !tasks))
         return fail(h, CLT_E_ARG, "clt_set_tasks: %d tasks (capacity %d)", num_tasks, h->max_tasks);
     const int32_t* t = (const int32_t*)tasks;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     std::fill(h->h_tile_map.begin(), h->h_tile_map.end(), -1);
     for (int i = 0; i < num_tasks; i++) {
         const int32_t* r = t + (size_t)i * h->task_words;
@@ -668,7 +702,7 @@ int clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p) {
     if ((rc = dalloc(h, (void**)&h->d_port_geom, sizeof(PortGeomDev))) ||
         (rc = dalloc(h, (void**)&h->d_rtab, sizeof(double) * len)))
         return rc;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     HIPCHK(h, hipMemcpy(h->d_port_geom, &pg, sizeof(pg), hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_rtab, tab.data(), sizeof(double) * len, hipMemcpyHostToDevice));
     h->pgeom = *p;
@@ -696,7 +730,7 @@ int clt_set_tasks_geometry(clt_handle* h, const int32_t* task, const int32_t* tx
         (rc = dalloc(h, (void**)&h->d_gtxy, (size_t)h->max_tasks * 4)) ||
         (rc = dalloc(h, (void**)&h->d_gdisp, (size_t)h->max_tasks * 4)))
         return rc;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     std::fill(h->h_tile_map.begin(), h->h_tile_map.end(), -1);
     for (int i = 0; i < num_tasks; i++) {
         const int tx = txy[i] & 0xffff, ty = (txy[i] >> 16) & 0xffff;
@@ -720,7 +754,7 @@ int clt_set_tasks_geometry(clt_handle* h, const int32_t* task, const int32_t* tx
 
 int clt_get_tasks(clt_handle* h, void* tasks) {
     if (!h || !tasks) return CLT_E_ARG;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     if (h->num_tasks > 0)
         HIPCHK(h, hipMemcpy(tasks, h->d_tasks, (size_t)h->num_tasks * h->task_words * 4, hipMemcpyDeviceToHost));
     return CLT_OK;
@@ -732,7 +766,7 @@ int clt_get_port_derivatives(clt_handle* h, double* out) {
     int rc;
     if ((rc = dalloc(h, (void**)&h->d_gderiv, (size_t)h->max_tasks * 8 * 14 * sizeof(double)))) return rc;
     if ((rc = launch_port_centres(h, h->d_gderiv))) return rc;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     if (h->num_tasks > 0)
         HIPCHK(h, hipMemcpy(out, h->d_gderiv, (size_t)h->num_tasks * 8 * 14 * sizeof(double), hipMemcpyDeviceToHost));
     return CLT_OK;
@@ -838,7 +872,7 @@ int clt_exec_imclt(clt_handle* h, int32_t apply_lpf) {
 static int ensure_capacity(clt_handle* h, float** p, size_t* cap, size_t floats) {
     if (*p && *cap >= floats) return CLT_OK;
     if (*p) {
-        (void)hipStreamSynchronize(h->stream);
+        (void)sync_streams(h);
         (void)hipFree(*p);
         *p = nullptr;
     }
@@ -871,10 +905,10 @@ static int build_lma_args(clt_handle* h, bool have_records) {
                 if (sel && q.dir >= 0 && ((q.dir >= 2) == (diag != 0)) && q.scale == scale) members[n++] = p;
             }
             if (n == 0) continue;
-            if (F.ngroups == ((f == 0) ? LMA_MAXG : 1) || n > LMA_GPAIRS)
+            if (F.ngroups == ((f == 0) ? LMA_MAXG : 1) || n > ((f == 0) ? LMA_GPAIRS : LMA_DPAIRS))
                 return fail(h, CLT_E_UNSUPPORTED,
-                            "per-tile LMA: at most %d pair groups over all pairs, one per direction, of at most %d "
-                            "pairs (disable it with clt_set_corr_lma_params)", LMA_MAXG, LMA_GPAIRS);
+                            "per-tile LMA: over all pairs at most %d groups of %d pairs, per direction one group of "
+                            "%d pairs (disable it with clt_set_corr_lma_params)", LMA_MAXG, LMA_GPAIRS, LMA_DPAIRS);
             LmaGroup& G = F.g[F.ngroups++];
             G.npairs = n;
             G.diag = diag;
@@ -919,6 +953,7 @@ static int build_lma_args(clt_handle* h, bool have_records) {
     h->lma_run = a.fit[0].ngroups > 0;
     int rc;
     if ((rc = dalloc(h, (void**)&h->d_lma_args, sizeof(LmaArgs)))) return rc;
+    join_lma(h);   // a running LMA still reads the previous arguments
     HIPCHK(h, hipMemcpyAsync(h->d_lma_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
     return CLT_OK;
 }
@@ -1006,9 +1041,7 @@ int clt_exec_corr2d(clt_handle* h) {
     tbegin(h, T_CORR);
     launch_corr(h, h->num_tasks, nullptr);
     tend(h, T_CORR);
-    tbegin(h, T_CORR_LMA);
-    launch_corr_lma(h, h->num_tasks);
-    tend(h, T_CORR_LMA);
+    launch_corr_lma(h, h->num_tasks, true);
     HIPCHK(h, hipGetLastError());
     h->corr_done = true;
     return finish(h);
@@ -1093,7 +1126,7 @@ int clt_get_rbg(clt_handle* h, int32_t cam, float* out) {
     if (cam < 0 || cam >= h->g.num_cams) return fail(h, CLT_E_ARG, "clt_get_rbg: camera %d", cam);
     if (!h->d_rbg) return fail(h, CLT_E_STATE, "clt_get_rbg: imclt not executed");
     const size_t n = (size_t)h->g.num_colors * (h->g.width + 8) * (h->g.height + 8);
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     HIPCHK(h, hipMemcpy(out, h->d_rbg + cam * n, n * 4, hipMemcpyDeviceToHost));
     return CLT_OK;
 }
@@ -1103,7 +1136,7 @@ int clt_get_clt(clt_handle* h, int32_t cam, float* out) {
     if (cam < 0 || cam >= h->g.num_cams) return fail(h, CLT_E_ARG, "clt_get_clt: camera %d", cam);
     const size_t n = (size_t)h->num_tiles * h->g.num_colors * 256;
     std::vector<float> tmp(n);
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     HIPCHK(h, hipMemcpy(tmp.data(), h->d_clt + cam * n, n * 4, hipMemcpyDeviceToHost));
     for (size_t b = 0; b < n / 64; b++)   // device [h][v] -> [v][h] (TileProcessor.cuh:1065-1095 layout)
         for (int hh = 0; hh < 8; hh++)
@@ -1114,7 +1147,7 @@ int clt_get_clt(clt_handle* h, int32_t cam, float* out) {
 int clt_get_corr2d(clt_handle* h, float* corr, float* combo, float* disparity) {
     if (!h) return CLT_E_ARG;
     if (!h->corr_done) return fail(h, CLT_E_STATE, "clt_get_corr2d: corr2d not executed");
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     const size_t nt = h->num_tasks;
     if (corr) HIPCHK(h, hipMemcpy(corr, h->d_corr, nt * h->corr_pairs_at_exec * CLT_CORR_LEN * 4, hipMemcpyDeviceToHost));
     if (combo && h->corr_strip_h_at_exec)
@@ -1127,7 +1160,7 @@ int clt_get_corr2d(clt_handle* h, float* corr, float* combo, float* disparity) {
 int clt_get_textures(clt_handle* h, float* tiles, float* stats) {
     if (!h) return CLT_E_ARG;
     if (!h->tex_done) return fail(h, CLT_E_STATE, "clt_get_textures: textures not executed");
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     const size_t nt = h->num_tasks;
     if (tiles) HIPCHK(h, hipMemcpy(tiles, h->d_tex, nt * h->tex_planes_at_exec * 256 * 4, hipMemcpyDeviceToHost));
     if (stats) HIPCHK(h, hipMemcpy(stats, h->d_tex_stats, nt * 4 * h->g.num_cams * 4, hipMemcpyDeviceToHost));
@@ -1137,7 +1170,7 @@ int clt_get_textures(clt_handle* h, float* tiles, float* stats) {
 int clt_get_combined_textures(clt_handle* h, float* out) {
     if (!h || !out) return CLT_E_ARG;
     if (!h->combined) return fail(h, CLT_E_STATE, "clt_get_combined_textures: combine not executed");
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     HIPCHK(h, hipMemcpy(out, h->d_tex_img, (size_t)h->tex_planes_at_exec * h->g.width * h->g.height * 4, hipMemcpyDeviceToHost));
     return CLT_OK;
 }
@@ -1178,7 +1211,7 @@ int clt_exec_corr_adjoint(clt_handle* h) {
 int clt_get_corr_adjoint(clt_handle* h, float* out) {
     if (!h || !out) return CLT_E_ARG;
     if (!h->adj_done) return fail(h, CLT_E_STATE, "clt_get_corr_adjoint: not executed");
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     HIPCHK(h, hipMemcpy(out, h->d_adj, (size_t)h->num_tasks * CORR_ADJ_STRIDE * 4, hipMemcpyDeviceToHost));
     return CLT_OK;
 }
@@ -1226,7 +1259,7 @@ static int geo_eval(clt_handle* h, const float* disp, double lambda, double* sum
     hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(128), 0, h->stream, h->d_gpart, blocks, GEO_ADJ_SUMS, h->d_gsum);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemcpyAsync(sums, h->d_gsum, GEO_ADJ_SUMS * 8, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     return CLT_OK;
 }
 
@@ -1343,7 +1376,7 @@ int clt_lma_extrinsics(clt_handle* h, const clt_lma_params* prm, double* extrins
     // leave the handle at the solution: geometry, disparities and task records
     if ((rc = upload_geometry(h, cur)) || (rc = geo_eval(h, d_cur, 0.0, nullptr))) return rc;
     if (d_cur != h->d_gdisp) HIPCHK(h, hipMemcpyAsync(h->d_gdisp, d_cur, (size_t)n * 4, hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     h->pgeom = cur;
     h->converted = h->corr_done = h->tex_done = h->combined = false;
     h->adj_done = true;
@@ -1360,7 +1393,7 @@ int clt_set_image_frame(clt_handle* h, int32_t frame, int32_t cam, const float* 
     const clt_geom& g = h->g;
     if (frame < 0 || frame >= h->num_frames || cam < 0 || cam >= g.num_cams || !image || stride < g.width)
         return fail(h, CLT_E_ARG, "clt_set_image_frame: bad frame %d / camera %d, null image or stride %d", frame, cam, stride);
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     HIPCHK(h, hipMemcpy2D(h->d_images + frame * h->img_fstride + (size_t)cam * g.width * g.height, (size_t)g.width * 4, image,
                           (size_t)stride * 4, (size_t)g.width * 4, g.height, hipMemcpyHostToDevice));
     if (frame == 0) h->images_set[cam] = true;
@@ -1378,7 +1411,7 @@ int clt_set_tasks_frame(clt_handle* h, int32_t frame, const void* tasks, int32_t
         if (tx >= h->tiles_x || ty >= h->tiles_y)
             return fail(h, CLT_E_ARG, "clt_set_tasks_frame: task %d tile (%d,%d) outside %dx%d", i, tx, ty, h->tiles_x, h->tiles_y);
     }
-    HIPCHK(h, hipStreamSynchronize(h->stream));   // the staging area may still be read
+    HIPCHK(h, sync_streams(h));   // the staging area may still be read
     if (num_tasks > 0)
         HIPCHK(h, hipMemcpy(h->d_staged + (size_t)frame * h->max_tasks * h->task_words, tasks,
                             (size_t)num_tasks * h->task_words * 4, hipMemcpyHostToDevice));
@@ -1423,10 +1456,11 @@ static void launch_batch(clt_handle* h, int stages) {
     if (stages & STAGE_CONVERT) launch_convert(h, h->num_tasks, tf, 0);
     if (stages & STAGE_CORR) {
         launch_corr(h, h->num_tasks, tf);
-        launch_corr_lma(h, h->num_tasks);
+        launch_corr_lma(h, h->num_tasks, false);
     }
     if (stages & STAGE_TEX) launch_tex(h, h->num_tasks, tf, nullptr);
     if (stages & STAGE_IMCLT) launch_imclt(h, h->num_frames, 1);
+    join_lma(h);   // a captured graph must end on the origin stream
 }
 
 static int prepare_batch(clt_handle* h, int stages) {
@@ -1494,7 +1528,7 @@ int clt_autotune(clt_handle* h) {
         h->tune_imclt_rows = pick;
     }
     HIPCHK(h, hipGetLastError());
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     h->tuned = true;
     h->graph_stages = 0;
     h->converted = h->corr_done = h->tex_done = false;
@@ -1556,7 +1590,7 @@ int clt_get_corr2d_frame(clt_handle* h, int32_t frame, float* corr, float* combo
     size_t f0, nt;
     int rc;
     if ((rc = frame_range(h, frame, &f0, &nt))) return rc;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     const size_t P = h->corr_pairs_at_exec, sh = h->corr_strip_h_at_exec;
     if (corr && nt) HIPCHK(h, hipMemcpy(corr, h->d_corr + f0 * P * CLT_CORR_LEN, nt * P * CLT_CORR_LEN * 4, hipMemcpyDeviceToHost));
     if (combo && sh && nt) HIPCHK(h, hipMemcpy(combo, h->d_combo + f0 * sh * 15, nt * sh * 15 * 4, hipMemcpyDeviceToHost));
@@ -1571,7 +1605,7 @@ int clt_get_textures_frame(clt_handle* h, int32_t frame, float* tiles, float* st
     size_t f0, nt;
     int rc;
     if ((rc = frame_range(h, frame, &f0, &nt))) return rc;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     const size_t pl = h->tex_planes_at_exec, C = h->g.num_cams;
     if (tiles && nt) HIPCHK(h, hipMemcpy(tiles, h->d_tex + f0 * pl * 256, nt * pl * 256 * 4, hipMemcpyDeviceToHost));
     if (stats && nt) HIPCHK(h, hipMemcpy(stats, h->d_tex_stats + f0 * 4 * C, nt * 4 * C * 4, hipMemcpyDeviceToHost));
@@ -1584,7 +1618,7 @@ int clt_get_rbg_frame(clt_handle* h, int32_t frame, int32_t cam, float* out) {
         return fail(h, CLT_E_ARG, "clt_get_rbg_frame: frame %d / camera %d", frame, cam);
     if (!h->d_rbg) return fail(h, CLT_E_STATE, "clt_get_rbg_frame: imclt not executed");
     const size_t n = (size_t)h->g.num_colors * (h->g.width + 8) * (h->g.height + 8);
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     HIPCHK(h, hipMemcpy(out, h->d_rbg + ((size_t)frame * h->g.num_cams + cam) * n, n * 4, hipMemcpyDeviceToHost));
     return CLT_OK;
 }
@@ -1598,7 +1632,7 @@ int clt_set_async(clt_handle* h, int32_t async) {
 
 int clt_sync(clt_handle* h) {
     if (!h) return CLT_E_ARG;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     return CLT_OK;
 }
 
@@ -1610,7 +1644,7 @@ int clt_set_timing(clt_handle* h, int32_t enable) {
 
 int clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count) {
     if (!h || which < 0 || which >= T_NUM) return CLT_E_ARG;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     double tot = 0;
     for (int u = 0; u < h->ev_used[which]; u++) {
         float ms = 0.f;
@@ -1624,7 +1658,7 @@ int clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* coun
 
 int clt_reset_timing(clt_handle* h) {
     if (!h) return CLT_E_ARG;
-    HIPCHK(h, hipStreamSynchronize(h->stream));
+    HIPCHK(h, sync_streams(h));
     for (int i = 0; i < T_NUM; i++) h->ev_used[i] = 0;
     return CLT_OK;
 }
