@@ -146,7 +146,7 @@ struct LmaOut {
 // the sums of the accepted parameter vector.
 struct LmaLds {
     double* v;
-    uint8_t* idx;
+    uint16_t* idx;   // cell | weight class << 8 (1 convex, 2 other selected, 0 NaN sample)
     double* acc;
 };
 
@@ -173,7 +173,6 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
     int ns[LMA_MAXG] = {0, 0}, x0s[LMA_MAXG] = {0, 0}, y0s[LMA_MAXG] = {0, 0};
     double gw_raw[LMA_MAXG] = {0.0, 0.0}, sw_fit = 0.0, vmax = -INFINITY;
     Box81 cvs[LMA_MAXG] = {{0, 0}, {0, 0}};   // selected convex cells (weight class 1)
-    Box81 bad[LMA_MAXG] = {{0, 0}, {0, 0}};   // samples whose value or weight is NaN (weight 0)
 #pragma unroll
     for (int g = 0; g < LMA_MAXG; g++) {
         if (g >= G) break;
@@ -184,7 +183,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         x0s[g] = x0;
         y0s[g] = y0;
         double* v = S.v + g * LMA_CELLS;
-        uint8_t* idx = S.idx + g * LMA_CELLS;
+        uint16_t* idx = S.idx + g * LMA_CELLS;
         // combined correlation of the group's pairs over the box (combineCompatiblePairs, C2:462-488):
         // every load of the lane's cells is issued before the first is used
         int pp[NPM], xf[NPM];
@@ -308,11 +307,9 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
                 if (nan) v[c] = 0.0;
                 sw_fit += nan ? 0.0 : w;
                 vmax = fmax(vmax, nan ? 0.0 : val);
-                idx[b_rank(smp, c)] = (uint8_t)c;
+                const int cls = nan ? 0 : (b_bit(cvs[g], c) ? 1 : 2);
+                idx[b_rank(smp, c)] = (uint16_t)(c | cls << 8);
             }
-            const uint64_t bits = (__ballot(nan) & fmask) >> (lane - fl);
-            if (W * k < 64) bad[g].lo |= bits << (W * k);
-            if (W * k + W > 64) bad[g].hi |= (W * k >= 64) ? (bits << (W * k - 64)) : (bits >> (64 - W * k));
         }
         ns[g] = b_count(smp);
         gw_raw[g] = fit_sum<W>(gsum);
@@ -348,12 +345,57 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
     p[NC] = p[NC + 1] = vmax;
     const double sqrt2 = 1.4142135623730951;
     double acc[NV];
-
-    // one pass over the samples at parameters q: normal matrix, J^T W (y - f) and the weighted
-    // residual sum (getFxJt + getWJtJlambda + getWYmFxRms, Correlations2dLMA:368-461)
-    auto evaluate = [&](const double* q) {
+    double wc[LMA_MAXG][2];   // weight of a class-1 / class-2 sample before |v|^pwr, normalised
 #pragma unroll
-        for (int i = 0; i < NV; i++) acc[i] = 0.0;
+    for (int g = 0; g < LMA_MAXG; g++) {
+        const int np = (g < G) ? F->g[g].npairs : 0;
+        wc[g][0] = (1.0 * np) * inv;
+        wc[g][1] = (A.nc_cost * np) * inv;
+    }
+
+    // Passes over the samples at parameters q (getFxJt, getWJtJlambda, getWYmFxRms,
+    // Correlations2dLMA:368-461): `rms_at` the weighted residual square sum only, `normal_at` the
+    // normal matrix and J^T W (y - f).  The LMA needs the normal equations only at accepted points
+    // it continues from, so a rejected or final trial point costs one reduction instead of twenty.
+    auto sample = [&](int g, int j, double& sv, double& sw, double& x, double& y) {
+        const LmaGroup& grp = F->g[g];
+        const int e = S.idx[g * LMA_CELLS + j];
+        const int c = e & 0xff, cls = e >> 8;
+        sv = S.v[g * LMA_CELLS + c];
+        sw = (cls == 1) ? wc[g][0] : ((cls == 2) ? wc[g][1] : 0.0);
+        if (A.vasw_pwr != 0.0) sw *= (A.vasw_pwr == 2.0) ? sv * sv : lma_pow(sv, A.vasw_pwr);
+        const int br = (c * 57) >> 9, bc = c - 9 * br;
+        const int cx = x0s[g] - 11 + bc, cy = y0s[g] - 11 + br;   // relative to the correlation centre
+        const double hx = grp.diag ? 0.5 * grp.scale : grp.scale;
+        x = (double)(grp.diag ? cx + cy : cx) * hx;
+        y = (double)(grp.diag ? cy - cx : cy) * hx;
+    };
+    auto rms_at = [&](const double* q) {
+        double s2 = 0.0;
+#pragma unroll
+        for (int g = 0; g < LMA_MAXG; g++) {
+            if (g >= G || ns[g] == 0) continue;
+            const int gs = (g == 1 && ex0) ? 1 : 0;
+            const LmaGroup& grp = F->g[g];
+            const double ws = grp.diag ? sqrt2 : 1.0;
+            const double Wy = q[1] * grp.scale + ((NC == 4) ? q[2] : 0.0), Wx = Wy + q[P_WXY];
+            const double iWx = rcp64(Wx), iWy = rcp64(Wy);
+            const double Ag = gs ? q[NC + 1] : q[NC];
+            for (int j = fl; j < ns[g]; j += W) {
+                double sv, sw, x, y;
+                sample(g, j, sv, sw, x, y);
+                const double dxw = ws * (x - q[0]) * iWx, dyw = ws * y * iWy;
+                const double r = sv - (1.0 - dxw * dxw - dyw * dyw) * Ag;
+                s2 += sw * r * r;
+            }
+        }
+        s2 = fit_sum<W>(s2);
+        const double r0 = A.half_width - q[1], r1 = -q[P_WXY];
+        return s2 + wr0 * r0 * r0 + wr1 * r1 * r1;
+    };
+    auto normal_at = [&](const double* q) {
+#pragma unroll
+        for (int i = 0; i < I_RMS; i++) acc[i] = 0.0;
 #pragma unroll
         for (int g = 0; g < LMA_MAXG; g++) {
             if (g >= G || ns[g] == 0) continue;
@@ -363,22 +405,12 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             const double Wy = q[1] * scale + ((NC == 4) ? q[2] : 0.0), Wx = Wy + q[P_WXY];
             const double iWx = rcp64(Wx), iWy = rcp64(Wy);
             const double Ag = gs ? q[NC + 1] : q[NC];
-            const double hx = grp.diag ? 0.5 * scale : scale;
-            const double wc1 = 1.0 * grp.npairs, wc2 = A.nc_cost * grp.npairs;
-            const int x0 = x0s[g], y0 = y0s[g], diag = grp.diag;
             for (int j = fl; j < ns[g]; j += W) {
-                const int c = S.idx[g * LMA_CELLS + j];
-                const double sv = S.v[g * LMA_CELLS + c];
-                double sw = b_bit(bad[g], c) ? 0.0 : (b_bit(cvs[g], c) ? wc1 : wc2);
-                if (A.vasw_pwr != 0.0) sw *= (A.vasw_pwr == 2.0) ? sv * sv : lma_pow(sv, A.vasw_pwr);
-                sw *= inv;
-                const int br = (c * 57) >> 9, bc = c - 9 * br;
-                const int cx = x0 - 11 + bc, cy = y0 - 11 + br;   // relative to the correlation centre
-                const double x = (double)(diag ? cx + cy : cx) * hx;
-                const double y = (double)(diag ? cy - cx : cy) * hx;
+                double sv, sw, x, y;
+                sample(g, j, sv, sw, x, y);
                 const double dxw = ws * (x - q[0]) * iWx, dyw = ws * y * iWy;
                 const double d = 1.0 - dxw * dxw - dyw * dyw;
-                const double r = sv - d * Ag, wr = sw * r;
+                const double wr = sw * (sv - d * Ag);
                 const double dfdWx = Ag * 2 * dxw * dxw * iWx, dfdWy = Ag * 2 * dyw * dyw * iWy;
                 double jc[NC];
                 jc[0] = ws * Ag * 2 * dxw * iWx;
@@ -396,11 +428,10 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
                 }
                 acc[I_DIAG + gs] += wd * d;
                 acc[I_BAG + gs] += d * wr;
-                acc[I_RMS] += wr * r;
             }
         }
 #pragma unroll
-        for (int i = 0; i < NV; i++) {
+        for (int i = 0; i < I_RMS; i++) {
             const bool slot1 = (i >= I_CROSS + NC && i < I_DIAG) || i == I_DIAG + 1 || i == I_BAG + 1;
             if (!slot1 || ng > 1) acc[i] = fit_sum<W>(acc[i]);
         }
@@ -410,18 +441,17 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         acc[lma_tri(NC, P_WXY, P_WXY)] += wr1;
         acc[I_BC + 1] += wr0 * r0;
         acc[I_BC + P_WXY] += wr1 * r1;
-        acc[I_RMS] += wr0 * r0 * r0 + wr1 * r1 * r1;
     };
-    auto keep = [&]() {   // the accepted point's sums, read by the next solve
+    auto keep = [&]() {   // the accepted point's normal equations, read by the next solve
         if (fl == 0)
 #pragma unroll
-            for (int i = 0; i < NV; i++) S.acc[i] = acc[i];
+            for (int i = 0; i < I_RMS; i++) S.acc[i] = acc[i];
         wave_sync();
     };
 
-    evaluate(p);
-    const double rms0 = sqrt(acc[I_RMS]);
+    const double rms0 = sqrt(rms_at(p));
     double rms = rms0, lambda = A.lambda0;
+    normal_at(p);
     keep();
     bool good = false;
     for (int iter = 0; iter < A.num_iter; iter++) {
@@ -467,8 +497,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         }
 #pragma unroll
         for (int i = 0; i < NP; i++) pn[i] = msk[i] ? p[i] + x[i] : p[i];
-        evaluate(pn);
-        const double rms_n = sqrt(acc[I_RMS]);
+        const double rms_n = sqrt(rms_at(pn));
         bool done = false;
         if (rms_n < rms) {
             good = true;
@@ -476,7 +505,10 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             rms = rms_n;
 #pragma unroll
             for (int i = 0; i < NP; i++) p[i] = pn[i];
-            keep();
+            if (!done && iter + 1 < A.num_iter) {   // the next step solves from here
+                normal_at(p);
+                keep();
+            }
         } else {
             good = false;
         }
@@ -549,7 +581,7 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
                                                   int num_tasks, const LmaArgs* __restrict__ Ap) {
     const LmaArgs& A = *Ap;   // in device memory: the fits index it per lane
     __shared__ double s_v[LMA_SLOTS][2 * LMA_CELLS];
-    __shared__ uint8_t s_idx[LMA_SLOTS][2 * LMA_CELLS];
+    __shared__ uint16_t s_idx[LMA_SLOTS][2 * LMA_CELLS];
     __shared__ double s_acc[LMA_SLOTS][2 * LMA_ACC];
     __shared__ double s_dds[LMA_SLOTS][4][4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
