@@ -1,7 +1,7 @@
 // Per-tile correlation LMA and foreground correction (gfx950).
 //
 // The reference refines every tile's disparity after the centre-of-mass step (ImageDtt.java
-// ID:2254-2330): a Levenberg-Marquardt fit of a paraboloid to the combined pair correlations
+// ID:2254-2336): a Levenberg-Marquardt fit of a paraboloid to the combined pair correlations
 // (Correlation2d.corrLMA, C2:1754-1900; sample selection filterConvex C2:2029-2207 and addSamples
 // C2:2230-2338; the fit Correlations2dLMA.java), four one-direction fits (corr4dirsLMA,
 // C2:1591-1630) and the foreground correction (foregroundCorrect, C2:1663-1750).
@@ -573,7 +573,7 @@ __device__ double lma_foreground(const LmaArgs& A, const bool* have, const doubl
     return disp;
 }
 
-// ID:2254-2330 over the task list: reads the raw centre of mass and strength of each record,
+// ID:2254-2336 over the task list: reads the raw centre of mass and strength of each record,
 // writes CM, STRENGTH (final), POLY, POLY_STR.  16 lanes per tile, 4 tiles per wavefront.
 // NC = 4 when the extra vertical width Wyd is adjusted, else 3.
 template <int NC>
@@ -633,7 +633,7 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
             for (int e = 0; e < 4; e++) dv[d][e] = s_dds[slot][d][e];
             have[d] = !isnan(dv[d][0]);
         }
-        const double mod = lma_foreground(A, have, dv, disparity);   // ID:2288-2330
+        const double mod = lma_foreground(A, have, dv, disparity);   // ID:2288-2336
         if (mod != disparity && strength > A.fo_min_strength) out_cm = mod;
     }
     if (l == 0) {

@@ -60,7 +60,7 @@ enum {
     CLT_DISP_VERT_STR   = 6,
     CLT_DISP_CM_WEIGHT  = 7, /* total weight of the centre-of-mass window   (C2:1025)           */
     /* With the per-tile correlation LMA enabled (clt_corr_lma_params.enable, the reference
-     * default), CLT_DISP_CM and CLT_DISP_STRENGTH hold the final values of ID:2254-2330: the LMA
+     * default), CLT_DISP_CM and CLT_DISP_STRENGTH hold the final values of ID:2254-2336: the LMA
      * disparity/strength when mix_corr_poly, then the foreground correction.  Without it they
      * hold the centre of mass and the integer-maximum strength. */
     CLT_DISP_POLY       = 8, /* LMA disparity (ID:2266 DISPARITY_INDEX_POLY), NaN if not fitted */
@@ -119,7 +119,7 @@ typedef struct clt_tex_params {
 /* Per-tile correlation LMA and foreground correction (ImageDttParameters, IDP:32-110): the fit of
  * Correlations2dLMA (C2:1754-1900, C2:2029-2338) run on the combined pairs when the strength
  * exceeds min_poly_strength (ID:2254-2288), then corr4dirsLMA + foregroundCorrect
- * (C2:1591-1750, ID:2288-2330).  Doubles, as in the reference. */
+ * (C2:1591-1750, ID:2288-2336).  Doubles, as in the reference. */
 typedef struct clt_corr_lma_params {
     int32_t enable;              /* 1: clt_exec_corr2d runs the chain (the reference always does)  */
     int32_t mix_corr_poly;       /* 1: LMA result replaces CM and strength        (IDP:32)         */
@@ -200,7 +200,7 @@ int  clt_exec_convert(clt_handle* h, int32_t lpf_mask);
  * convert; apply_lpf != 0 multiplies by the corr_sigma LPF first (clt_lpf, ID:4178-4271). */
 int  clt_exec_imclt(clt_handle* h, int32_t apply_lpf);
 /* Phase correlation of every pair for every task (C2:197-389), combined strip (C2:556-790),
- * per-tile disparity (ID:2186-2238) and, when enabled, the per-tile LMA chain (ID:2254-2330) over
+ * per-tile disparity (ID:2186-2238) and, when enabled, the per-tile LMA chain (ID:2254-2336) over
  * the pairs with a known orientation (at most 2 compatible groups, C2:72-78). */
 int  clt_exec_corr2d(clt_handle* h);
 /* Per-tile textures (ID:2662-2812).  extra_disparity: num_tasks floats or NULL (ID:2663-2677). */

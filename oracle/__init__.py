@@ -438,7 +438,7 @@ def corr_frame(geom, clt, tasks, pairs=None, params=None, corr_sigma=0.8, nthrea
 
 
 def corr_lma_frame(corrs, disp, pairs=None, lma=None, nthreads=None):
-    """The per-tile LMA chain (ID:2254-2330) alone on given pair correlations [T][P][15][15]
+    """The per-tile LMA chain (ID:2254-2336) alone on given pair correlations [T][P][15][15]
     (float32) and disparity records [T][DISP_LAYERS]; returns the updated records (float64)."""
     nthreads = nthreads or os.cpu_count()
     pairs = pairs if pairs is not None else QUAD_PAIRS

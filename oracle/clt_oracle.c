@@ -1238,8 +1238,8 @@ static double foreground_correct(const clt_corr_lma_params* lp, const int have[4
     return disp;
 }
 
-/* ID:2254-2330 for one tile: the LMA over all oriented pairs (C2:1754), mixing into CM/strength
- * (ID:2273-2281), the four one-direction fits and the foreground correction (ID:2288-2330).
+/* ID:2254-2336 for one tile: the LMA over all oriented pairs (C2:1754), mixing into CM/strength
+ * (ID:2273-2281), the four one-direction fits and the foreground correction (ID:2288-2336).
  * disp: the tile's record; reads CLT_DISP_CM_RAW / CLT_DISP_STR_RAW, writes CM, STRENGTH, POLY. */
 static void lma_chain(const double (*corrs)[225], const clt_pair* pairs, int npairs, const clt_corr_lma_params* lp,
                       double* disp) {
@@ -2122,7 +2122,7 @@ static void lma_task(void* vctx, int t) {
     free(dc);
 }
 
-/* The LMA chain alone (ID:2254-2330) on given pair correlations [T][npairs][225] and disparity
+/* The LMA chain alone (ID:2254-2336) on given pair correlations [T][npairs][225] and disparity
  * records [T][CLT_DISP_LAYERS] (reads CM_RAW / STR_RAW, rewrites CM, STRENGTH, POLY, POLY_STR). */
 EXPORT int orc_corr_lma_frame(const clt_pair* pairs, int npairs, const clt_corr_lma_params* lp, const float* corrs,
                               int ntasks, double* disp, int nthreads) {

@@ -1,4 +1,4 @@
-"""GPU parity of the per-tile correlation LMA chain (ImageDtt ID:2254-2330: Correlations2dLMA fit
+"""GPU parity of the per-tile correlation LMA chain (ImageDtt ID:2254-2336: Correlations2dLMA fit
 over the combined pairs, the four one-direction fits and the foreground correction).
 
 Same input: the oracle's FP64 chain runs on the GPU's own pair correlations and pre-LMA records, so
@@ -55,11 +55,13 @@ def _run(pk, orc, sc, **lma):
     return g_corr, g_disp
 
 
-@pytest.mark.parametrize("name,kw", [("field 0..6 px", dict(disparity=("field", 0.0, 6.0))),
-                                     ("constant 3 px, residual 1 px", dict(disparity=3.0, target_disparity=2.0))])
-def test_corr_lma_quad(pk, orc, name, kw):
-    log("case: per-tile LMA chain, quad 128x96, %s" % name)
-    sc = pk.synth.make_scene(128, 96, seed=21, **kw)
+@pytest.mark.parametrize("name,size,kw", [
+    ("field 0..6 px", (128, 96), dict(disparity=("field", 0.0, 6.0))),
+    ("constant 3 px, residual 1 px", (128, 96), dict(disparity=3.0, target_disparity=2.0)),
+    ("512x512 frame, field 0..20 px", (512, 512), dict(disparity=("field", 0.0, 20.0)))])
+def test_corr_lma_quad(pk, orc, name, size, kw):
+    log("case: per-tile LMA chain, quad %dx%d, %s" % (size[0], size[1], name))
+    sc = pk.synth.make_scene(size[0], size[1], seed=21, **kw)
     g_corr, g_disp = _run(pk, orc, sc)
     r = _same_input(orc, g_corr, g_disp)
     log("  same input: " + ", ".join("%s=%.3g" % kv for kv in r.items()))
