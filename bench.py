@@ -266,6 +266,39 @@ def adjoint_bench(pk, sc, repeats=5, seed=11):
             "lma_loss": [round(float(hist[0]), 2), round(float(hist[-1]), 2)]}
 
 
+def kernel_prep_bench(pk, seed=13, cams=4):
+    """calculateCLTKernel on the device for the 2592x1936 kernel grid (164x123 kernels x 3 colours
+    per camera, a 991 MB raw stack of 64x64 double-resolution PSFs): wall time per camera with the
+    host-to-device upload, device time of the preparation kernels, and the FP64 oracle's time on
+    the same stack on one host core."""
+    import oracle
+    tp = pk.GPUTileProcessor(W, H, C, K)
+    kx, ky = tp.KERNELS_HOR, tp.KERNELS_VERT
+    rng = np.random.default_rng(seed)
+    raw = pk.synth.raw_kernel_stack(K, ky, kx, rng)
+    sxy = np.stack([pk.synth.smooth_field(ky, kx, rng, -1.0, 1.0) for _ in range(2)], axis=-1).astype(np.float32)
+    tp.calculateCLTKernel(0, raw, 64, sensor_xy=sxy)   # allocations
+    tp.setTiming(True)
+    tp.resetTiming()
+    t0 = time.perf_counter()
+    for cam in range(cams):
+        tp.calculateCLTKernel(cam, raw, 64, sensor_xy=sxy)
+    wall = (time.perf_counter() - t0) / cams
+    dev_ms, launches = tp.getTiming("kernel_prep")
+    dev_ms /= cams
+    tp.close()
+    n = kx * ky * K
+    alg = raw.nbytes + n * (256 + 8) * 4 + n * 16 * 10   # raw read, kernels + extras written, centres
+    t0 = time.perf_counter()
+    oracle.prepare_kernels(raw, kx, ky, sensor_xy=sxy)
+    cpu = time.perf_counter() - t0
+    gbs = alg / (dev_ms * 1e-3) / 1e9
+    return {"kernels_per_camera": n, "raw_MB": round(raw.nbytes / 1e6, 1), "wall_ms_per_camera": round(1e3 * wall, 2),
+            "device_ms_per_camera": round(dev_ms, 4), "launches_per_camera": launches // cams,
+            "alg_MB": round(alg / 1e6, 1), "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "cpu_port_ms_per_camera": round(1e3 * cpu, 1), "cpu_cores": 1}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +311,7 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM counter runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-frames comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/LMA timing")
+    ap.add_argument("--no-kprep", action="store_true", help="skip the kernel preparation timing")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -368,6 +402,10 @@ def main():
     if not args.no_adjoint:
         print("bench: adjoint and LMA ...", file=sys.stderr, flush=True)
         adjoint = adjoint_bench(pk, sc)
+    kprep = None
+    if not args.no_kprep:
+        print("bench: kernel preparation ...", file=sys.stderr, flush=True)
+        kprep = kernel_prep_bench(pk)
     if traffic is None:
         roof["traffic_note"] = traffic_info
     else:
@@ -388,6 +426,7 @@ def main():
                                "convert+correlate2D(6 pairs, per-tile LMA chain)+textures(RGBA)+imclt_rbg",
                    "tiles": T, "cameras": C, "colors": K, "pairs": 6},
         "roofline": roof, "kernels": kernels, "kernels_isolated": isolated, "cpu_baseline": cpu, "batch": batch, "adjoint": adjoint,
+        "kernel_prep": kprep,
     }
     print(json.dumps(out))
 

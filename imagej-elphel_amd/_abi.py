@@ -17,7 +17,8 @@ CORR_ADJ = 32
 DISP_NAMES = ["int", "cm", "strength", "hor", "hor_strength", "vert", "vert_strength", "cm_weight",
               "poly", "poly_strength", "cm_raw", "strength_raw"]
 TIMING_KERNELS = {"convert": 0, "imclt": 1, "corr2d": 2, "textures": 3, "combine_textures": 4, "batch": 5,
-                  "corr_adjoint": 6, "geometry_adjoint": 7, "corr_lma": 8}
+                  "corr_adjoint": 6, "geometry_adjoint": 7, "corr_lma": 8,
+                  "kernel_prep": 9}
 
 
 class clt_geom(ctypes.Structure):
@@ -91,6 +92,10 @@ SIGNATURES = {
     "clt_device_count": (_I, []),
     "clt_set_kernels": (_I, [_H, _I, _P, _P]),
     "clt_clear_kernels": (_I, [_H]),
+    "clt_prepare_kernels": (_I, [_H, _I, _P, _I, _P, _I]),
+    "clt_get_kernels": (_I, [_H, _I, _P, _P]),
+    "clt_kernels_to_flat": (_I, [_P, _P, _I, _I, _I, _P]),
+    "clt_kernels_from_flat": (_I, [_P, _I, _I, _I, _P, _P]),
     "clt_set_image": (_I, [_H, _I, _P, _I]),
     "clt_set_tasks": (_I, [_H, _P, _I]),
     "clt_set_lpf_rbg": (_I, [_H, ctypes.c_float, ctypes.c_float, ctypes.c_float]),

@@ -15,6 +15,7 @@
 #include "../../include/clt_tile_processor.h"
 #include "clt_kernels.hpp"
 #include "clt_corr_lma.hpp"
+#include "clt_kernel_prep.hpp"
 
 namespace clt {
 __constant__ Tables c_tab;
@@ -102,7 +103,7 @@ static void half_flat_top(int n, double hwidth, double blur, bool normalize, boo
 }
 
 // ------------------------------------------------------------------------ handle
-enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_CORR_LMA, T_NUM };
+enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_CORR_LMA, T_KPREP, T_NUM };
 
 struct clt_handle {
     clt_geom g{};
@@ -545,6 +546,134 @@ int clt_set_kernels(clt_handle* h, int32_t cam, const float* kernels, const floa
     h->kernels_set[cam] = true;
     h->use_kernels = true;
     for (int c = 0; c < g.num_cams; c++) h->use_kernels = h->use_kernels && h->kernels_set[c];
+    return CLT_OK;
+}
+
+// calculateCLTKernel (QuadCLT.java:433-668) on the device for one camera; the raw kernel stack is
+// uploaded in bands of kernel rows (at most 256 MB at a time).
+int clt_prepare_kernels(clt_handle* h, int32_t cam, const float* raw, int32_t ks, const float* sensor_xy,
+                        int32_t norm_kern) {
+    if (!h) return CLT_E_ARG;
+    const clt_geom& g = h->g;
+    if (cam < 0 || cam >= g.num_cams || !raw) return fail(h, CLT_E_ARG, "clt_prepare_kernels: bad camera %d or null buffer", cam);
+    if (ks < 32 || ks > 256 || (ks & 1)) return fail(h, CLT_E_ARG, "clt_prepare_kernels: raw kernel size %d (even, 32..256)", ks);
+    if (g.kernels_hor <= 0 || g.kernels_vert <= 0) return fail(h, CLT_E_ARG, "clt_prepare_kernels: geometry has no kernel grid");
+    const int kx = g.kernels_hor, ky = g.kernels_vert, K = g.num_colors;
+    const size_t nk = (size_t)ky * kx * K, C = g.num_cams, W = (size_t)kx * ks;
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_kernels, C * nk * 256 * 4)) || (rc = dalloc(h, (void**)&h->d_koffs, C * nk * 8 * 4))) return rc;
+    double tab[192];
+    const double sc = std::sqrt(2.0 / 8);
+    for (int k = 0; k < 8; k++)   // DCT-IIIe / DST-IIIe [k][j] (DttRad2.java:970-991, 1032-1052)
+        for (int j = 0; j < 8; j++) {
+            tab[k * 8 + j] = sc * ((j == 0 || j == 7) ? 0.5 : 1.0) * std::cos(j * (2 * k + 1) * M_PI / 16.0);
+            tab[64 + k * 8 + j] = sc * ((j == 7) ? 0.5 : 1.0) * std::sin((j + 1) * (2 * k + 1) * M_PI / 16.0);
+        }
+    for (int i = 0; i < 8; i++)   // norm_sym_weights (QuadCLT.java:466-474)
+        for (int j = 0; j < 8; j++) tab[128 + i * 8 + j] = std::cos(M_PI * i / 16) * std::cos(M_PI * j / 16);
+    const size_t row_bytes = (size_t)K * ks * W * 4;
+    const int rows = (int)std::max<size_t>(1, std::min<size_t>(ky, ((size_t)256 << 20) / row_bytes));
+    float *d_band = nullptr, *d_sxy = nullptr;
+    double *d_tab = nullptr, *d_ctr = nullptr;
+    auto cleanup = [&]() {
+        (void)sync_streams(h);
+        for (void* p : {(void*)d_band, (void*)d_sxy, (void*)d_tab, (void*)d_ctr})
+            if (p) (void)hipFree(p);
+    };
+    if ((rc = dalloc(h, (void**)&d_band, (size_t)rows * row_bytes)) || (rc = dalloc(h, (void**)&d_tab, sizeof(tab))) ||
+        (rc = dalloc(h, (void**)&d_ctr, nk * 2 * sizeof(double))) ||
+        (sensor_xy && (rc = dalloc(h, (void**)&d_sxy, (size_t)ky * kx * 2 * 4)))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMemcpyAsync(d_tab, tab, sizeof(tab), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess && sensor_xy)
+        e = hipMemcpyAsync(d_sxy, sensor_xy, (size_t)ky * kx * 2 * 4, hipMemcpyHostToDevice, h->stream);
+    float* kern = h->d_kernels + cam * nk * 256;
+    float* offs = h->d_koffs + cam * nk * 8;
+    for (int row0 = 0; row0 < ky && e == hipSuccess; row0 += rows) {
+        const int nr = std::min(rows, ky - row0);
+        for (int col = 0; col < K && e == hipSuccess; col++)
+            e = hipMemcpyAsync(d_band + (size_t)col * nr * ks * W, raw + ((size_t)col * ky + row0) * ks * W,
+                               (size_t)nr * ks * W * 4, hipMemcpyHostToDevice, h->stream);
+        if (e != hipSuccess) break;
+        const KPrepArgs a = {ks, kx, ky, K, row0, nr, norm_kern != 0};
+        const int tiles = K * nr * kx;
+        tbegin(h, T_KPREP);
+        hipLaunchKernelGGL(k_prepare_kernels, dim3((tiles + 3) / 4), dim3(256), 0, h->stream, d_band, d_tab, d_sxy, kern,
+                           offs, d_ctr, a);
+        tend(h, T_KPREP);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        tbegin(h, T_KPREP);
+        hipLaunchKernelGGL(k_fill_coord_corr, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, h->stream, d_ctr, offs, kx, ky,
+                           K, g.kernel_step);
+        tend(h, T_KPREP);
+        e = hipGetLastError();
+    }
+    cleanup();
+    if (e != hipSuccess) return fail(h, CLT_E_HIP, "clt_prepare_kernels: %s", hipGetErrorString(e));
+    h->kernels_set[cam] = true;
+    h->use_kernels = true;
+    for (int c = 0; c < g.num_cams; c++) h->use_kernels = h->use_kernels && h->kernels_set[c];
+    return CLT_OK;
+}
+
+// The kernels of one camera as clt_set_kernels takes them (upload layout [v][h]).
+int clt_get_kernels(clt_handle* h, int32_t cam, float* kernels, float* offsets) {
+    if (!h) return CLT_E_ARG;
+    const clt_geom& g = h->g;
+    if (cam < 0 || cam >= g.num_cams || !kernels || !offsets) return fail(h, CLT_E_ARG, "clt_get_kernels: bad camera %d or null buffer", cam);
+    if (!h->kernels_set[cam]) return fail(h, CLT_E_STATE, "clt_get_kernels: kernels of camera %d not set", cam);
+    const size_t nk = (size_t)g.kernels_vert * g.kernels_hor * g.num_colors;
+    HIPCHK(h, sync_streams(h));
+    std::vector<float> tr(nk * 256);
+    HIPCHK(h, hipMemcpy(tr.data(), h->d_kernels + cam * nk * 256, nk * 256 * 4, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(offsets, h->d_koffs + cam * nk * 8, nk * 8 * 4, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < nk; k++)
+        for (int q = 0; q < 4; q++)
+            for (int v = 0; v < 8; v++)
+                for (int hh = 0; hh < 8; hh++) kernels[k * 256 + q * 64 + v * 8 + hh] = tr[k * 256 + q * 64 + hh * 8 + v];
+    return CLT_OK;
+}
+
+// flattenCLTKernels (QuadCLT.java:670-734): per colour a (16 kx) x (17 ky) image; the tile of
+// kernel (tx, ty) holds CC | SC over CS | SS in the CPU orientation (the transpose of the upload
+// [v][h] blocks, GPUTileProcessor.java:406-408) and the 8 CltExtra values in its last row.
+int clt_kernels_to_flat(const float* kernels, const float* offsets, int32_t kx, int32_t ky, int32_t colors, float* flat) {
+    if (!kernels || !offsets || !flat || kx < 1 || ky < 1 || colors < 1) return CLT_E_ARG;
+    const size_t Wf = (size_t)16 * kx, Hf = (size_t)17 * ky;
+    std::memset(flat, 0, (size_t)colors * Wf * Hf * 4);
+    for (int ty = 0; ty < ky; ty++)
+        for (int tx = 0; tx < kx; tx++)
+            for (int col = 0; col < colors; col++) {
+                const size_t k = ((size_t)ty * kx + tx) * colors + col;
+                float* base = flat + (size_t)col * Wf * Hf + (size_t)ty * 17 * Wf + (size_t)tx * 16;
+                for (int q = 0; q < 4; q++)
+                    for (int i = 0; i < 8; i++)
+                        for (int j = 0; j < 8; j++)
+                            base[(i + ((q & 2) ? 8 : 0)) * Wf + j + ((q & 1) ? 8 : 0)] = kernels[k * 256 + q * 64 + j * 8 + i];
+                for (int i = 0; i < 8; i++) base[16 * Wf + i] = offsets[k * 8 + i];
+            }
+    return CLT_OK;
+}
+
+// extractCLTKernels (QuadCLT.java:790-857): the inverse of clt_kernels_to_flat.
+int clt_kernels_from_flat(const float* flat, int32_t kx, int32_t ky, int32_t colors, float* kernels, float* offsets) {
+    if (!kernels || !offsets || !flat || kx < 1 || ky < 1 || colors < 1) return CLT_E_ARG;
+    const size_t Wf = (size_t)16 * kx, Hf = (size_t)17 * ky;
+    for (int ty = 0; ty < ky; ty++)
+        for (int tx = 0; tx < kx; tx++)
+            for (int col = 0; col < colors; col++) {
+                const size_t k = ((size_t)ty * kx + tx) * colors + col;
+                const float* base = flat + (size_t)col * Wf * Hf + (size_t)ty * 17 * Wf + (size_t)tx * 16;
+                for (int q = 0; q < 4; q++)
+                    for (int i = 0; i < 8; i++)
+                        for (int j = 0; j < 8; j++)
+                            kernels[k * 256 + q * 64 + j * 8 + i] = base[(i + ((q & 2) ? 8 : 0)) * Wf + j + ((q & 1) ? 8 : 0)];
+                for (int i = 0; i < 8; i++) offsets[k * 8 + i] = base[16 * Wf + i];
+            }
     return CLT_OK;
 }
 

@@ -185,6 +185,35 @@ def clt_kernels(num_cams, num_colors, kernels_vert, kernels_hor, rng, sigma=(0.6
     return (kern.reshape(C, KY, KX, K, 4, 8, 8).astype(np.float32), extra.astype(np.float32))
 
 
+def raw_kernel_stack(num_colors, kernels_vert, kernels_hor, rng, ks=64, sigma=(0.7, 1.4)):
+    """Double-resolution aberration kernels as calculateCLTKernel (QC:433-668) reads them: the
+    stack [K][KY*ks][KX*ks] float32, kernel (tx, ty) of colour k in rows ty*ks.., columns tx*ks..
+    Each PSF is a sum of two anisotropic rotated Gaussians (a core and a weaker coma-like lobe)
+    with a field-dependent centre, so the kernels are neither separable nor symmetric."""
+    K, KY, KX = num_colors, kernels_vert, kernels_hor
+    n = K * KY * KX
+    g = (np.arange(ks) - ks // 2) * 0.5   # sensor pixels
+    drift = np.stack([smooth_field(KY, KX, rng, -0.8, 0.8) for _ in range(2)])
+    cx = np.broadcast_to(drift[0][None], (K, KY, KX)).reshape(n) + rng.uniform(-0.3, 0.3, n)
+    cy = np.broadcast_to(drift[1][None], (K, KY, KX)).reshape(n) + rng.uniform(-0.3, 0.3, n)
+    out = np.zeros((n, ks, ks), np.float64)
+    yy, xx = g[:, None], g[None, :]
+    for lobe, amp in ((0, 1.0), (1, 0.35)):
+        sa = rng.uniform(sigma[0], sigma[1], n) * (1.0 + lobe)
+        sb = sa * rng.uniform(0.6, 1.0, n)
+        th = rng.uniform(0, np.pi, n)
+        ox = cx + lobe * rng.uniform(-1.0, 1.0, n)
+        oy = cy + lobe * rng.uniform(-1.0, 1.0, n)
+        for c0 in range(0, n, 512):
+            c = slice(c0, min(n, c0 + 512))
+            dx, dy = xx[None] - ox[c, None, None], yy[None] - oy[c, None, None]
+            ct, st = np.cos(th[c])[:, None, None], np.sin(th[c])[:, None, None]
+            u = (dx * ct + dy * st) / sa[c, None, None]
+            v = (dy * ct - dx * st) / sb[c, None, None]
+            out[c] += amp * np.exp(-0.5 * (u * u + v * v))
+    return out.reshape(K, KY, KX, ks, ks).transpose(0, 1, 3, 2, 4).reshape(K, KY * ks, KX * ks).astype(np.float32)
+
+
 def _fill_coord_corr(cx, cy, kstep):
     """clt_fill_coord_corr (ImageDtt.java:4792-4851): least-squares slopes of the kernel centres
     over the 3x3 neighbourhood -> dxc_dx, dxc_dy, dyc_dx, dyc_dy."""

@@ -173,6 +173,32 @@ class GPUTileProcessor:
             self.setConvolutionKernel(clt_kernels[ncam], offsets[ncam], ncam)
         self.kernels_set = True
 
+    def calculateCLTKernel(self, ncam, kernel_stack, kernel_size=64, sensor_xy=None, norm_kern=True):
+        """QC:433-668 for one camera, on the device, installing the result as setConvolutionKernel
+        does.  kernel_stack: [NUM_COLORS][KERNELS_VERT*kernel_size][KERNELS_HOR*kernel_size] raw
+        double-resolution kernels; sensor_xy: [KERNELS_VERT][KERNELS_HOR][2] sensor correction at
+        the kernel centres (QC:603-610) or None."""
+        raw = np.ascontiguousarray(kernel_stack, np.float32)
+        shape = (self.NUM_COLORS, self.KERNELS_VERT * kernel_size, self.KERNELS_HOR * kernel_size)
+        if raw.shape != shape:
+            raise ValueError("kernel stack %s does not match %s" % (raw.shape, shape))
+        sxy = None
+        if sensor_xy is not None:
+            sxy = np.ascontiguousarray(sensor_xy, np.float32)
+            if sxy.size != self.KERNELS_VERT * self.KERNELS_HOR * 2:
+                raise ValueError("sensor_xy does not match the %dx%d kernel grid" % (self.KERNELS_HOR, self.KERNELS_VERT))
+        self._chk(self.lib.clt_prepare_kernels(self.h, ncam, _ptr(raw), kernel_size,
+                                               None if sxy is None else _ptr(sxy), 1 if norm_kern else 0))
+
+    def getConvolutionKernel(self, ncam):
+        """The installed kernels of one camera: (kernels [KY][KX][K][4][8][8] ([v][h]),
+        offsets [KY][KX][K][8]), the layout setConvolutionKernel takes."""
+        shp = (self.KERNELS_VERT, self.KERNELS_HOR, self.NUM_COLORS)
+        k = np.empty(shp + (4, 8, 8), np.float32)
+        o = np.empty(shp + (8,), np.float32)
+        self._chk(self.lib.clt_get_kernels(self.h, ncam, _ptr(k), _ptr(o)))
+        return k, o
+
     def clearConvolutionKernels(self):
         self._chk(self.lib.clt_clear_kernels(self.h))
         self.kernels_set = False

@@ -168,6 +168,26 @@ int  clt_device_count(void);                              /* 0 when no GPU is vi
 int  clt_set_kernels(clt_handle* h, int32_t cam, const float* kernels, const float* offsets);
 /* Drop the kernels: convert without deconvolution (ID:4896 use_kernels == false). */
 int  clt_clear_kernels(clt_handle* h);
+/* calculateCLTKernel(sensor, kernelStack, kernelSize, clt_parameters, ...) (QC:433-668) for one
+ * camera, computed on the device and installed as by clt_set_kernels.  raw: the kernel stack
+ * [num_colors][kernels_vert * ks][kernels_hor * ks] floats (one slice per colour, kernel (tx, ty)
+ * at rows ty*ks.., columns tx*ks.., extractOneKernel QC:1071-1089); ks: raw kernel size (64,
+ * even, 32..256); sensor_xy: [kernels_vert][kernels_hor][2] floats, the sensor correction vector
+ * at each kernel centre (interpolateCorrectionVector at ((tx - 0.5) kernel_step, (ty - 0.5)
+ * kernel_step), QC:603-610) or NULL for none; norm_kern: clt_parameters.norm_kern (QC:464). */
+int  clt_prepare_kernels(clt_handle* h, int32_t cam, const float* raw, int32_t ks,
+                         const float* sensor_xy, int32_t norm_kern);
+/* The installed kernels of one camera in the layout clt_set_kernels takes (GTP:381-435), e.g. to
+ * save them (saveFloatKernels TQ:1076-1177). */
+int  clt_get_kernels(clt_handle* h, int32_t cam, float* kernels, float* offsets);
+/* flattenCLTKernels (QC:670-745) / extractCLTKernels (QC:790-857) between the clt_set_kernels
+ * layout and the flat per-colour image [num_colors][17 ky][16 kx] (each kernel tile: CC | SC over
+ * CS | SS as 8x8 blocks in the reference's CPU orientation, then the 8 CltExtra values in the
+ * 17th row).  Host only; no handle. */
+int  clt_kernels_to_flat(const float* kernels, const float* offsets, int32_t kx, int32_t ky,
+                         int32_t colors, float* flat);
+int  clt_kernels_from_flat(const float* flat, int32_t kx, int32_t ky, int32_t colors,
+                           float* kernels, float* offsets);
 /* setBayerImage(float[] bayer_image, int ncam) (GTP:437-450).  image: [height][stride] floats,
  * colours of a Bayer frame interleaved as a mosaic (green already scaled by the caller,
  * TQ:2062-2067), stride >= width. */
@@ -330,7 +350,8 @@ int  clt_sync(clt_handle* h);
 int  clt_set_timing(clt_handle* h, int32_t enable);   /* record HIP events around each kernel  */
 /* Accumulated device time (ms) and launch count of a kernel since the last reset:
  * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures, 5 batch graph,
- * 6 correlation adjoint, 7 geometry adjoint, 8 per-tile correlation LMA. */
+ * 6 correlation adjoint, 7 geometry adjoint, 8 per-tile correlation LMA, 9 kernel preparation
+ * (clt_prepare_kernels: one count per band of kernel rows, plus one for the centre slopes). */
 int  clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count);
 int  clt_reset_timing(clt_handle* h);
 

@@ -199,6 +199,8 @@ def lib(precision=64):
     L.orc_filter_convex.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_double, _D]
     L.orc_foreground_correct.argtypes = [ctypes.POINTER(CorrLmaParams), _I, _D, ctypes.c_double]
+    L.orc_prepare_kernels.argtypes = [_F, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      _VP, ctypes.c_int, _F, _F]
     L.orc_foreground_correct.restype = ctypes.c_double
     L.orc_textures_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, ctypes.c_int,
                                      ctypes.POINTER(TexParams), ctypes.POINTER(CorrParams),
@@ -541,3 +543,20 @@ def port_coords(pg, px, py, disparity, deriv=False):
     lib(64).orc_port_coords(ctypes.byref(pg), len(px), _dp(px), _dp(py), _dp(dd), _dp(xy),
                             _vp(dv) if deriv else None)
     return (xy, dv) if deriv else xy
+
+
+def prepare_kernels(raw, kernels_hor, kernels_vert, ks=64, kernel_step=16, sensor_xy=None, norm_kern=True):
+    """calculateCLTKernel (QC:433-668) of one camera: raw kernel stack [colors][KY*ks][KX*ks] ->
+    (kernels [KY][KX][colors][4][8][8] upload layout [v][h], offsets [KY][KX][colors][8]), float32."""
+    raw = np.ascontiguousarray(raw, np.float32)
+    colors = raw.shape[0]
+    assert raw.shape == (colors, kernels_vert * ks, kernels_hor * ks)
+    kern = np.zeros((kernels_vert, kernels_hor, colors, 4, 8, 8), np.float32)
+    offs = np.zeros((kernels_vert, kernels_hor, colors, 8), np.float32)
+    sxy = None if sensor_xy is None else np.ascontiguousarray(sensor_xy, np.float32)
+    rc = lib(64).orc_prepare_kernels(raw.ctypes.data_as(_F), ks, kernels_hor, kernels_vert, colors, kernel_step,
+                                     None if sxy is None else sxy.ctypes.data_as(_VP), int(bool(norm_kern)),
+                                     kern.ctypes.data_as(_F), offs.ctypes.data_as(_F))
+    if rc:
+        raise ValueError("orc_prepare_kernels failed: %d" % rc)
+    return kern, offs

@@ -2200,3 +2200,167 @@ EXPORT int orc_combine_textures(const clt_geom* g, const int32_t* tasks, int nta
     }
     return 0;
 }
+
+/* ================= CLT kernel preparation (QuadCLT.calculateCLTKernel, QC:433-668) ================= */
+/* ID:4625-4672 clt_convert_double_kernel: centroid of the double-resolution kernel, 15x15 downsample
+ * around it with the 3x3 weights {1, 1/2 (sides), 1/4 (corners)}, then data_x, data_y (the
+ * chosen centre, half-pixel steps) and center_x, center_y (the exact centroid) */
+static void convert_double_kernel(const double* src, int src_size, double* dst) {
+    const int dtt_size = 8;
+    const int indices[9] = {0, -src_size, -1, 1, src_size, -src_size - 1, -src_size + 1, src_size - 1, src_size + 1};
+    const double weights[9] = {1.0, 0.5, 0.5, 0.5, 0.5, 0.25, 0.25, 0.25, 0.25};
+    const int src_center = src_size / 2;
+    double sx = 0.0, sy = 0.0, s = 0.0;
+    int indx = 0;
+    for (int i = -src_center; i < src_center; i++)
+        for (int j = -src_center; j < src_center; j++) {
+            double d = src[indx++];
+            sx += j * d;
+            sy += i * d;
+            s += d;
+        }
+    int src_x = (int)jround(sx / s) + src_center;
+    int src_y = (int)jround(sy / s) + src_center;
+    if (src_x < 2 * dtt_size) src_x = 2 * dtt_size - 1;
+    else if (src_x > (src_size - 2 * dtt_size)) src_x = src_size - 2 * dtt_size;
+    if (src_y < 2 * dtt_size) src_y = 2 * dtt_size - 1;
+    else if (src_y > (src_size - 2 * dtt_size)) src_y = src_size - 2 * dtt_size;
+    indx = 0;
+    for (int i = -dtt_size + 1; i < dtt_size; i++) {
+        int src_i = (src_y + 2 * i) * src_size + src_x;
+        for (int j = -dtt_size + 1; j < dtt_size; j++) {
+            double d = 0.0;
+            for (int k = 0; k < 9; k++) d += weights[k] * src[src_i + 2 * j + indices[k]];
+            dst[indx++] = d;
+        }
+    }
+    dst[indx++] = 0.5 * (src_x - src_center);
+    dst[indx++] = 0.5 * (src_y - src_center);
+    dst[indx++] = 0.5 * (sx / s);
+    dst[indx++] = 0.5 * (sy / s);
+}
+
+/* ID:4674-4696 clt_normalize_kernel: sum(kernel * window[|i|][|j|]) -> 16 */
+static void normalize_kernel(double* k, const double* window) {
+    double s = 0.0;
+    int indx = 0;
+    for (int i = -7; i < 8; i++)
+        for (int j = -7; j < 8; j++) s += k[indx++] * window[abs(i) * 8 + abs(j)];
+    s = 1.0 / s;
+    for (int i = 0; i < 225; i++) k[i] *= 16 * s;
+}
+
+/* ID:4698-4723 clt_symmetrize_kernel (the fourth quadrant takes the second's signs, as there) */
+static void symmetrize_kernel(const double* k, double sym[4][64]) {
+    const int in_size = 15, center = 7 * in_size + 7;
+    for (int i = 0; i < 8; i++) {
+        for (int j = 0; j < 8; j++) {
+            int i0 = center - i * in_size - j, i1 = center - i * in_size + j;
+            int i2 = center + i * in_size - j, i3 = center + i * in_size + j;
+            sym[0][i * 8 + j] = 0.25 * (k[i0] + k[i1] + k[i2] + k[i3]);
+            if (j > 0) sym[1][i * 8 + j - 1] = 0.25 * (-k[i0] + k[i1] - k[i2] + k[i3]);
+            if (i > 0) sym[2][(i - 1) * 8 + j] = 0.25 * (-k[i0] - k[i1] + k[i2] + k[i3]);
+            if (i > 0 && j > 0) sym[3][(i - 1) * 8 + (j - 1)] = 0.25 * (-k[i0] + k[i1] - k[i2] + k[i3]);
+        }
+        sym[1][i * 8 + 7] = 0.0;
+        sym[2][7 * 8 + i] = 0.0;
+        sym[3][i * 8 + 7] = 0.0;
+        sym[3][7 * 8 + i] = 0.0;
+    }
+}
+
+/* ID:4792-4851 clt_fill_coord_corr over one colour's kernel grid: least-squares slopes of the
+ * centres over the 3x3 neighbourhood; ext: [ky][kx][8] */
+static void fill_coord_corr(int kern_step, int kx, int ky, double* ext) {
+    double* dst = malloc(sizeof(double) * kx * ky * 4);
+    for (int ty = 0; ty < ky; ty++)
+        for (int tx = 0; tx < kx; tx++) {
+            double s0 = 0, sx = 0, sx2 = 0, sy = 0, sy2 = 0, sz = 0, sxz = 0, syz = 0, sw = 0, sxw = 0, syw = 0;
+            for (int dty = -1; dty < 2; dty++) {
+                int y = ty + dty;
+                if (y < 0 || y >= ky) continue;
+                for (int dtx = -1; dtx < 2; dtx++) {
+                    int x = tx + dtx;
+                    if (x < 0 || x >= kx) continue;
+                    const double* e = ext + (y * kx + x) * 8;
+                    s0 += 1;
+                    sx += dtx;
+                    sx2 += dtx * dtx;
+                    sy += dty;
+                    sy2 += dty * dty;
+                    sz += e[2];
+                    sxz += dtx * e[2];
+                    syz += dty * e[2];
+                    sw += e[3];
+                    sxw += dtx * e[3];
+                    syw += dty * e[3];
+                }
+            }
+            double denom_x = (sx2 * s0 - sx * sx) * kern_step;
+            double denom_y = (sy2 * s0 - sy * sy) * kern_step;
+            double* d = dst + (ty * kx + tx) * 4;
+            d[0] = (sxz * s0 - sz * sx) / denom_x;
+            d[1] = (syz * s0 - sz * sy) / denom_y;
+            d[2] = (sxw * s0 - sw * sx) / denom_x;
+            d[3] = (syw * s0 - sw * sy) / denom_y;
+        }
+    for (int t = 0; t < kx * ky; t++)
+        for (int i = 0; i < 4; i++) ext[t * 8 + 4 + i] = dst[t * 4 + i];
+    free(dst);
+}
+
+/* calculateCLTKernel (QC:433-668) for one camera.  raw: the kernel stack [colors][ky*ks][kx*ks]
+ * (extractOneKernel QC:1071-1089 takes kernel (tx, ty) at ((ty*kx*ks + tx)*ks)); sensor_xy:
+ * [ky][kx][2] sensor correction at the kernel centres (QC:603-610, offsetKernelSensor ID:4781-4791
+ * by its negative) or NULL.  Outputs in the upload layout of setConvolutionKernels (GTP:381-435):
+ * kern [ky][kx][colors][4][v][h], offs [ky][kx][colors][8], rounded to float. */
+EXPORT int orc_prepare_kernels(const float* raw, int ks, int kx, int ky, int colors, int kern_step,
+                               const float* sensor_xy, int norm_kern, float* kern_out, float* offs_out) {
+    tables();
+    if (ks < 32 || (ks & 1) || kx < 1 || ky < 1 || colors < 1) return CLT_E_ARG;
+    double window[64];
+    for (int i = 0; i < 8; i++)   /* QC:466-474 norm_sym_weights */
+        for (int j = 0; j < 8; j++) window[i * 8 + j] = cos(M_PI * i / 16) * cos(M_PI * j / 16);
+    double* k = malloc(sizeof(double) * ks * ks);
+    double* ext = malloc(sizeof(double) * ky * kx * 8);
+    const size_t W = (size_t)kx * ks;
+    for (int col = 0; col < colors; col++) {
+        const float* plane = raw + (size_t)col * ky * ks * W;
+        for (int ty = 0; ty < ky; ty++)
+            for (int tx = 0; tx < kx; tx++) {
+                const size_t base = ((size_t)ty * W + tx) * ks;
+                for (int i = 0; i < ks; i++)
+                    for (int j = 0; j < ks; j++) k[i * ks + j] = plane[base + i * W + j];
+                double kc[225 + 4], sym[4][64];
+                convert_double_kernel(k, ks, kc);
+                if (norm_kern) normalize_kernel(kc, window);
+                symmetrize_kernel(kc, sym);
+                float* ko = kern_out + (((size_t)ty * kx + tx) * colors + col) * 256;
+                for (int q = 0; q < 4; q++) {   /* ID:4725-4734 clt_dtt3_kernel, output [h][v] */
+                    double y[64];
+                    real xr[64], yr[64];
+                    for (int i = 0; i < 64; i++) xr[i] = (real)sym[q][i];
+                    dttt_iiie(xr, q, yr);
+                    for (int i = 0; i < 64; i++) y[i] = yr[i];
+                    for (int v = 0; v < 8; v++)
+                        for (int hh = 0; hh < 8; hh++) ko[q * 64 + v * 8 + hh] = (float)y[hh * 8 + v];
+                }
+                double* e = ext + (ty * kx + tx) * 8;
+                for (int i = 0; i < 4; i++) e[i] = kc[225 + i];
+                for (int i = 4; i < 8; i++) e[i] = 0.0;
+                if (sensor_xy) {
+                    const double dx = -(double)sensor_xy[(ty * kx + tx) * 2], dy = -(double)sensor_xy[(ty * kx + tx) * 2 + 1];
+                    e[2] += dx;
+                    e[3] += dy;
+                    e[0] += dx;
+                    e[1] += dy;
+                }
+            }
+        fill_coord_corr(kern_step, kx, ky, ext);
+        for (int t = 0; t < kx * ky; t++)
+            for (int i = 0; i < 8; i++) offs_out[((size_t)t * colors + col) * 8 + i] = (float)ext[t * 8 + i];
+    }
+    free(k);
+    free(ext);
+    return 0;
+}
