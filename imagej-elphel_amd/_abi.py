@@ -18,7 +18,7 @@ DISP_NAMES = ["int", "cm", "strength", "hor", "hor_strength", "vert", "vert_stre
               "poly", "poly_strength", "cm_raw", "strength_raw"]
 TIMING_KERNELS = {"convert": 0, "imclt": 1, "corr2d": 2, "textures": 3, "combine_textures": 4, "batch": 5,
                   "corr_adjoint": 6, "geometry_adjoint": 7, "corr_lma": 8,
-                  "kernel_prep": 9}
+                  "kernel_prep": 9, "rig": 10}
 
 
 class clt_geom(ctypes.Structure):
@@ -38,6 +38,21 @@ class clt_corr_params(ctypes.Structure):
                 ("twice_diagonal", ctypes.c_int32), ("min_corr", ctypes.c_float),
                 ("strip_height", ctypes.c_int32), ("enhortho_width", ctypes.c_float),
                 ("enhortho_scale", ctypes.c_float)]
+
+
+class clt_rig_params(ctypes.Structure):
+    _fields_ = [("use_poly", ctypes.c_int32), ("use_xy_poly", ctypes.c_int32), ("no_int_x0", ctypes.c_int32),
+                ("notch_mode", ctypes.c_int32), ("lt_rad", ctypes.c_int32), ("corr_strip_notch", ctypes.c_int32),
+                ("corr_notch_hwidth", ctypes.c_double), ("corr_notch_blur", ctypes.c_double),
+                ("cm_max_normalization", ctypes.c_double), ("max_corr_radius", ctypes.c_double)]
+
+
+BI_NAMES = ["disp_full", "disp_hor", "disp_vert", "disp_diagm", "disp_diago",
+            "adisp_full", "adisp_hor", "adisp_vert", "adisp_diagm", "adisp_diago",
+            "disp_cross", "disp_cross_dx", "disp_cross_dy",
+            "str_full", "str_hor", "str_vert", "str_diagm", "str_diago",
+            "astr_full", "astr_hor", "astr_vert", "astr_diagm", "astr_diago",
+            "str_cross", "str_all", "target"]
 
 
 class clt_corr_lma_params(ctypes.Structure):
@@ -108,6 +123,10 @@ SIGNATURES = {
     "clt_default_corr_params": (None, [ctypes.POINTER(clt_corr_params)]),
     "clt_default_tex_params": (None, [ctypes.POINTER(clt_tex_params), _I]),
     "clt_default_corr_lma_params": (None, [ctypes.POINTER(clt_corr_lma_params)]),
+    "clt_default_rig_params": (None, [ctypes.POINTER(clt_rig_params)]),
+    "clt_set_rig_params": (_I, [_H, ctypes.POINTER(clt_rig_params)]),
+    "clt_exec_corr2d_rig": (_I, [_H, _P]),
+    "clt_get_corr2d_rig": (_I, [_H, _P, _P, _P, _P, _P, _P]),
     "clt_default_quad_pairs": (_I, [_P]),
     "clt_exec_convert": (_I, [_H, _I]),
     "clt_exec_imclt": (_I, [_H, _I]),

@@ -27,7 +27,8 @@ constexpr int LMA_CELLS = 81;                // 9x9 box: half window at most 4
 constexpr int LMA_MAXG = 2;                  // compatible pair groups of the fit over all pairs
 constexpr int LMA_GPAIRS = 4;                // pairs per group (the 8-camera table has at most 4)
 constexpr int LMA_DPAIRS = 2;                // pairs per group of a one-direction fit
-constexpr int LMA_NFITS = 5;                 // all pairs, then horizontal, vertical, diag main, diag other
+constexpr int LMA_NFITS = 6;                 // all pairs, horizontal, vertical, diag main, diag other, one pair
+constexpr int LMA_FIT_SINGLE = 5;            // a single correlation as a horizontal pair (C2:1903-1935)
 constexpr int LMA_ACC = 28;                  // LDS doubles per fit for the accepted sums
 
 struct LmaGroup {
@@ -150,12 +151,70 @@ struct LmaLds {
     double* acc;
 };
 
+// LDL^T of a symmetric n x n matrix (packed upper triangle a, constant indices only) and the
+// solution of a x = b; returns the product of the pivots (the determinant).  A zero pivot leaves x
+// undefined.
+template <int N>
+__device__ __forceinline__ double ldl_solve(const double* a, const double* b, double* x) {
+    double L[N][N], d[N], det = 1.0;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        double dk = a[lma_tri(N, k, k)];
+#pragma unroll
+        for (int s = 0; s < k; s++) dk -= L[k][s] * L[k][s] * d[s];
+        d[k] = dk;
+        det *= dk;
+        const double idk = 1.0 / dk;
+#pragma unroll
+        for (int i = k + 1; i < N; i++) {
+            double m = a[lma_tri(N, k, i)];
+#pragma unroll
+            for (int s = 0; s < k; s++) m -= L[i][s] * L[k][s] * d[s];
+            L[i][k] = m * idk;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double m = b[i];
+#pragma unroll
+        for (int s = 0; s < i; s++) m -= L[i][s] * x[s];
+        x[i] = m;
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--) {
+        double m = x[i] / d[i];
+#pragma unroll
+        for (int s = i + 1; s < N; s++) m -= L[s][i] * x[s];
+        x[i] = m;
+    }
+    return det;
+}
+
+// product of the row norms of a symmetric matrix given as a packed upper triangle (PA:590-598)
+template <int N>
+__device__ __forceinline__ double tri_norm(const double* a) {
+    double norm = 1.0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double r = 0.0;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const double v = a[i <= j ? lma_tri(N, i, j) : lma_tri(N, j, i)];
+            r += v * v;
+        }
+        norm *= sqrt(r);
+    }
+    return norm;
+}
+
 // One Correlations2dLMA fit (C2:1754-1900) over the groups of *F, started at xcenter (pixels along
 // the disparity axis, = -disparity), on W lanes (fl = lane within the fit, fmask = the fit's lanes
 // in the wavefront).  NC core parameters: X0, Wm, Wxy (3) or X0, Wm, Wyd, Wxy (4, when Wyd is
 // adjusted); then one amplitude per group of at most NPM pairs.  The tile's correlations start at
 // corr[toff] (a wave-uniform base and 32-bit offsets keep one register per load address).
-template <int W, int NC, int NPM>
+// POLY: instead of the LMA, the 2-D quadratic maximum of the same samples (run_poly_instead,
+// getMaxXYPoly Correlations2dLMA:643-705); then disp = x, wm = y, strength = the value.
+template <int W, int NC, int NPM, bool POLY = false>
 __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, const float* __restrict__ corr, int toff,
                                           double xcenter, int fl, int lane, uint64_t fmask, const LmaLds& S) {
     constexpr int CPL = (LMA_CELLS + W - 1) / W;   // box cells per lane
@@ -370,6 +429,61 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         x = (double)(grp.diag ? cx + cy : cx) * hx;
         y = (double)(grp.diag ? cy - cx : cy) * hx;
     };
+    if constexpr (POLY) {
+        // PolynomialApproximation.quadraticApproximation (PA:350-580) and quadraticMaxV2dX2Y2XY
+        // (PA:201-224): weighted least squares z ~ A x^2 + B y^2 + C xy + D x + E y + F over the
+        // samples with w > 0 (the weights' common scale cancels in the solution and in the
+        // determinant / row-norm tests), then the stationary point
+        double m[21];
+#pragma unroll
+        for (int i = 0; i < 21; i++) m[i] = 0.0;
+#pragma unroll
+        for (int g = 0; g < LMA_MAXG; g++) {
+            if (g >= G || ns[g] == 0) continue;
+            for (int j = fl; j < ns[g]; j += W) {
+                double sv, sw, x, y;
+                sample(g, j, sv, sw, x, y);
+                if (!(sw > 0.0)) continue;
+                const double x2 = x * x, y2 = y * y, wz = sw * sv;
+                m[0] += sw;            m[1] += sw * x;        m[2] += sw * y;
+                m[3] += sw * x2;       m[4] += sw * x * y;    m[5] += sw * y2;
+                m[6] += sw * x2 * x;   m[7] += sw * x2 * y;   m[8] += sw * x * y2;   m[9] += sw * y2 * y;
+                m[10] += sw * x2 * x2; m[11] += sw * x2 * x * y; m[12] += sw * x2 * y2;
+                m[13] += sw * x * y2 * y; m[14] += sw * y2 * y2;
+                m[15] += wz;           m[16] += wz * x;       m[17] += wz * y;
+                m[18] += wz * x2;      m[19] += wz * x * y;   m[20] += wz * y2;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 21; i++) m[i] = fit_sum<W>(m[i]);
+        const double S00 = m[0], S10 = m[1], S01 = m[2], S20 = m[3], S11 = m[4], S02 = m[5], S30 = m[6], S21 = m[7];
+        const double S12 = m[8], S03 = m[9], S40 = m[10], S31 = m[11], S22 = m[12], S13 = m[13], S04 = m[14];
+        const double lin[6] = {S20, S11, S10, S02, S01, S00};   // packed upper triangle, 3 x 3
+        const double zl[3] = {0.0, 0.0, 0.0};
+        double xl[3];
+        const double nml = tri_norm<3>(lin);
+        const double detl = ldl_solve<3>(lin, zl, xl);
+        if (nml == 0.0 || fabs(detl) / nml < 1e-20) return out;   // linear only: null
+        const double quad[21] = {S40, S22, S31, S30, S21, S20, S04, S13, S12, S03, S02, S22, S21, S12, S11,
+                                 S20, S11, S10, S02, S01, S00};
+        const double zq[6] = {m[18], m[20], m[19], m[16], m[17], m[15]};
+        double c[6];
+        const double nmq = tri_norm<6>(quad);
+        const double detq = ldl_solve<6>(quad, zq, c);
+        if (nmq == 0.0 || fabs(detq) / nmq < 1e-30) return out;
+        const double m00 = 2 * c[0], m01 = c[2], m11 = 2 * c[1];
+        const double det2 = m00 * m11 - m01 * m01;
+        const double nm2 = sqrt(m00 * m00 + m01 * m01) * sqrt(m01 * m01 + m11 * m11);
+        if (nm2 == 0.0 || fabs(det2) / nm2 < 1e-30) return out;
+        const double px = (-c[3] * m11 + c[4] * m01) / det2, py = (-c[4] * m00 + c[3] * m01) / det2;
+        const double v = c[0] * px * px + c[1] * py * py + c[2] * px * py + c[3] * px + c[4] * py + c[5];
+        if (v < 0.0 || c[0] >= 0.0 || c[1] >= 0.0) return out;   // getMaxXYPoly: a maximum only
+        out.ok = true;
+        out.disp = px;
+        out.wm = py;
+        out.strength = v;
+        return out;
+    }
     auto rms_at = [&](const double* q) {
         double s2 = 0.0;
 #pragma unroll
@@ -576,9 +690,13 @@ __device__ double lma_foreground(const LmaArgs& A, const bool* have, const doubl
 // ID:2254-2336 over the task list: reads the raw centre of mass and strength of each record,
 // writes CM, STRENGTH (final), POLY, POLY_STR.  16 lanes per tile, 4 tiles per wavefront.
 // NC = 4 when the extra vertical width Wyd is adjusted, else 3.
+// dir_out (the rig's tileCorrs, ID:7581-7740): when not null, the four one-direction fits run on
+// every tile whose fit over all pairs succeeded and [T][8] receives their disparities (NaN when a
+// fit fails or does not run) and strengths (0 then); no foreground correction.
 template <int NC>
 __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ corr, float* __restrict__ disp,
-                                                  int num_tasks, const LmaArgs* __restrict__ Ap) {
+                                                  int num_tasks, const LmaArgs* __restrict__ Ap,
+                                                  float* __restrict__ dir_out) {
     const LmaArgs& A = *Ap;   // in device memory: the fits index it per lane
     __shared__ double s_v[LMA_SLOTS][2 * LMA_CELLS];
     __shared__ uint16_t s_idx[LMA_SLOTS][2 * LMA_CELLS];
@@ -607,7 +725,7 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
                 out_cm = disparity;
                 out_st = strength;
             }
-            dirs = A.fo_correct && strength > 0.0;
+            dirs = dir_out ? true : (A.fo_correct && strength > 0.0);
         }
     }
     if (dirs) {   // corr4dirsLMA: two one-direction fits at a time, 8 lanes each
@@ -633,14 +751,81 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
             for (int e = 0; e < 4; e++) dv[d][e] = s_dds[slot][d][e];
             have[d] = !isnan(dv[d][0]);
         }
-        const double mod = lma_foreground(A, have, dv, disparity);   // ID:2288-2336
-        if (mod != disparity && strength > A.fo_min_strength) out_cm = mod;
+        if (dir_out) {
+            if (l == 0)
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    dir_out[(size_t)t * 8 + d] = (float)dv[d][0];
+                    dir_out[(size_t)t * 8 + 4 + d] = have[d] ? (float)dv[d][1] : 0.f;
+                }
+        } else {
+            const double mod = lma_foreground(A, have, dv, disparity);   // ID:2288-2336
+            if (mod != disparity && strength > A.fo_min_strength) out_cm = mod;
+        }
+    } else if (dir_out && l == 0) {
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            dir_out[(size_t)t * 8 + d] = NAN;
+            dir_out[(size_t)t * 8 + 4 + d] = 0.f;
+        }
     }
     if (l == 0) {
         rec[CLT_DISP_CM] = (float)out_cm;
         rec[CLT_DISP_STRENGTH] = (float)out_st;
         rec[CLT_DISP_POLY] = (float)poly;
         rec[CLT_DISP_POLY_STR] = (float)poly_s;
+    }
+}
+
+// tileInterCamCorrs (ID:7370-7560) after the centre of mass: the LMA of the inter-camera
+// correlation as one horizontal pair (C2:1903-1935) and its dx, dy.  rec_in [T][8] from
+// k_corr_inter: {disparity, strength, LMA start (NaN: no centre of mass), -, CM dx, CM dy, -, -};
+// res_out [T][4] = {disparity, strength, dx, dy}.  16 lanes per tile as k_corr_lma.
+template <int NC>
+__global__ __launch_bounds__(256, 3) void k_corr_lma_inter(const float* __restrict__ corr, const float* __restrict__ rec_in,
+                                                        float* __restrict__ res_out, int num_tasks,
+                                                        const LmaArgs* __restrict__ Ap, int use_poly, int use_xy_poly,
+                                                        int notch) {
+    const LmaArgs& A = *Ap;
+    __shared__ double s_v[LMA_SLOTS][LMA_CELLS];
+    __shared__ uint16_t s_idx[LMA_SLOTS][LMA_CELLS];
+    __shared__ double s_acc[LMA_SLOTS][LMA_ACC];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int sub = lane >> 4, l = lane & 15, slot = wave * LMA_TILES_PER_WAVE + sub;
+    const int t = (xcd_block(blockIdx.x, gridDim.x) * LMA_WAVES + wave) * LMA_TILES_PER_WAVE + sub;
+    if (t >= num_tasks) return;   // whole rows: the reductions never see an exited lane
+    const float* r = rec_in + (size_t)t * 8;
+    double disparity = r[0], strength = r[1], dx = NAN, dy = NAN;
+    const float x0 = r[2];
+    if (!isnan(x0) && strength > A.min_poly_strength && !notch) {
+        const LmaLds S = {s_v[slot], s_idx[slot], s_acc[slot]};
+        const uint64_t fmask = 0xffffull << (16 * sub);
+        const LmaOut q = lma_fit<16, NC, 1>(A, &A.fit[LMA_FIT_SINGLE], corr, t * CLT_CORR_LEN, (double)x0, l, lane,
+                                            fmask, S);
+        if (q.ok) {
+            if (use_poly) {
+                disparity = q.disp;
+                strength = q.strength;
+            }
+            if (use_xy_poly) {   // single2dPoly (C2:1509-1536) around 0
+                const LmaOut pq = lma_fit<16, NC, 1, true>(A, &A.fit[LMA_FIT_SINGLE], corr, t * CLT_CORR_LEN, 0.0, l,
+                                                           lane, fmask, S);
+                if (pq.ok) {
+                    dx = -pq.disp;
+                    dy = -pq.wm;
+                }
+            } else {   // single2dCM (C2:1538-1586), computed with the centre of mass
+                dx = r[4];
+                dy = r[5];
+            }
+        }
+    }
+    if (l == 0) {
+        float* o = res_out + (size_t)t * 4;
+        o[0] = (float)disparity;
+        o[1] = (float)strength;
+        o[2] = (float)dx;
+        o[3] = (float)dy;
     }
 }
 

@@ -966,13 +966,65 @@ __device__ __forceinline__ int argmax16(float v, int idx) {
     return idx;
 }
 
+// One pair's correlation from its frequency-domain product (one bin per lane, the four modes in
+// acc): LPF, DCT-IIe/DST-IIe of each quadrant (C2:316-319, DR:595-614) through the wavefront's 256
+// floats of LDS at S, unfold to the 15x15 tile (DttRad2.java:443-489), written to co (global) and
+// cs (LDS).  cy_/sy_: DCT-IIe/DST-IIe rows of the lane's y = lane >> 3, cx_/sx_ of x = lane & 7.
+__device__ __forceinline__ void corr_fd_to_tile(const float* acc, float lp, float* S, const float* cy_, const float* sy_,
+                                                const float* cx_, const float* sx_, int lane, float* co, float* cs) {
+    const int y = lane >> 3, xx = lane & 7;
+    wave_sync();
+#pragma unroll
+    for (int n = 0; n < 4; n++) S[n * 64 + lane] = acc[n] * lp;
+    wave_sync();
+    // pass 1 along kv: lane (y, kh); quadrant q uses DST-IIe if q bit 1
+    float tq[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float4 u0 = *reinterpret_cast<const float4*>(S + q * 64 + xx * 8);
+        const float4 u1 = *reinterpret_cast<const float4*>(S + q * 64 + xx * 8 + 4);
+        const float* m = (q & 2) ? sy_ : cy_;
+        tq[q] = m[0] * u0.x + m[1] * u0.y + m[2] * u0.z + m[3] * u0.w + m[4] * u1.x + m[5] * u1.y + m[6] * u1.z + m[7] * u1.w;
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 4; q++) S[q * 64 + y * 8 + xx] = tq[q];
+    wave_sync();
+    // pass 2 along kh: lane (y, x); quadrant q uses DST-IIe if q bit 0
+    float oq[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float4 u0 = *reinterpret_cast<const float4*>(S + q * 64 + y * 8);
+        const float4 u1 = *reinterpret_cast<const float4*>(S + q * 64 + y * 8 + 4);
+        const float* m = (q & 1) ? sx_ : cx_;
+        oq[q] = m[0] * u0.x + m[1] * u0.y + m[2] * u0.z + m[3] * u0.w + m[4] * u1.x + m[5] * u1.y + m[6] * u1.z + m[7] * u1.w;
+    }
+    // unfold to 15x15: CC[ay][ax] +- SC[ay][ax-1] +- CS[ay-1][ax] +- SS[ay-1][ax-1]; the lane (ay, ax) =
+    // (y, xx) writes (7 +- y, 7 +- xx)
+    const float a = oq[0];
+    float b = __shfl(oq[1], (lane + 63) & 63);
+    float c = __shfl(oq[2], (lane + 56) & 63);
+    float d = __shfl(oq[3], (lane + 55) & 63);
+    if (xx == 0) { b = 0.f; d = 0.f; }
+    if (y == 0) { c = 0.f; d = 0.f; }
+    const float v_pp = 0.25f * (a + b + c + d), v_pm = 0.25f * (a - b + c - d);
+    const float v_mp = 0.25f * (a + b - c - d), v_mm = 0.25f * (a - b - c + d);
+    const int o_pp = (7 + y) * 15 + 7 + xx, o_pm = (7 + y) * 15 + 7 - xx;
+    const int o_mp = (7 - y) * 15 + 7 + xx, o_mm = (7 - y) * 15 + 7 - xx;
+    co[o_pp] = v_pp;
+    cs[o_pp] = v_pp;
+    if (xx > 0) { co[o_pm] = v_pm; cs[o_pm] = v_pm; }
+    if (y > 0) { co[o_mp] = v_mp; cs[o_mp] = v_mp; }
+    if (xx > 0 && y > 0) { co[o_mm] = v_mm; cs[o_mm] = v_mm; }
+}
+
 template <bool FZ>   // FZ: fat zero in use (the per-bin normaliser needs the mean over the bins)
 __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                      int task_words, int num_tasks, const float* __restrict__ lpf,
                                                      const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
                                                      float* __restrict__ combo_out, float* __restrict__ disp_out,
                                                      const int32_t* __restrict__ task_frame, long long clt_fstride,
-                                                     GeomDev g) {
+                                                     GeomDev g, float* __restrict__ mix_out) {
     extern __shared__ __attribute__((aligned(16))) float s_dyn[];
     const CorrArgs& A = *args_p;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1006,6 +1058,14 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
         for (int col = 0; col < 3; col++)
 #pragma unroll
             for (int n = 0; n < 4; n++) D[c][col][n] = clt[(((size_t)c * ntiles + tile) * 3 + col) * 256 + n * 64 + lane];
+    if (mix_out) {   // cltMixCameras (C2:425-447): the mean of the 4 cameras, for the rig's inter-camera pair
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+                mix_out[((size_t)t * 3 + col) * 256 + n * 64 + lane] =
+                    0.25f * (D[0][col][n] + D[1][col][n] + D[2][col][n] + D[3][col][n]);
+    }
     constexpr bool fz = FZ;
     float nrm[4][3];   // per bin sum over the 4 modes of the squares (C2:205-212)
 #pragma unroll
@@ -1036,9 +1096,6 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
         sx_[k] = c_tab.siie[xx][k];
     }
     const float lp = lpf[lane];
-    // unfold targets of this lane (ay, ax) = (y, xx): (7 +- y, 7 +- xx)
-    const int o_pp = (7 + y) * 15 + 7 + xx, o_pm = (7 + y) * 15 + 7 - xx;
-    const int o_mp = (7 - y) * 15 + 7 + xx, o_mm = (7 - y) * 15 + 7 - xx;
     float* corr_t = corr_out + (size_t)t * 6 * CLT_CORR_LEN;
 #pragma unroll
     for (int p = 0; p < 6; p++) {
@@ -1064,48 +1121,7 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
             acc[2] = fmaf(t2, scale, acc[2]);
             acc[3] = fmaf(t3, scale, acc[3]);
         }
-        wave_sync();
-#pragma unroll
-        for (int n = 0; n < 4; n++) S[n * 64 + lane] = acc[n] * lp;
-        wave_sync();
-        // pass 1 along kv: lane (y, kh); quadrant q uses DST-IIe if q bit 1 (C2:316-319, DR:595-614)
-        float tq[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const float4 u0 = *reinterpret_cast<const float4*>(S + q * 64 + xx * 8);
-            const float4 u1 = *reinterpret_cast<const float4*>(S + q * 64 + xx * 8 + 4);
-            const float* m = (q & 2) ? sy_ : cy_;
-            tq[q] = m[0] * u0.x + m[1] * u0.y + m[2] * u0.z + m[3] * u0.w + m[4] * u1.x + m[5] * u1.y + m[6] * u1.z + m[7] * u1.w;
-        }
-        wave_sync();
-#pragma unroll
-        for (int q = 0; q < 4; q++) S[q * 64 + y * 8 + xx] = tq[q];
-        wave_sync();
-        // pass 2 along kh: lane (y, x); quadrant q uses DST-IIe if q bit 0
-        float oq[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const float4 u0 = *reinterpret_cast<const float4*>(S + q * 64 + y * 8);
-            const float4 u1 = *reinterpret_cast<const float4*>(S + q * 64 + y * 8 + 4);
-            const float* m = (q & 1) ? sx_ : cx_;
-            oq[q] = m[0] * u0.x + m[1] * u0.y + m[2] * u0.z + m[3] * u0.w + m[4] * u1.x + m[5] * u1.y + m[6] * u1.z + m[7] * u1.w;
-        }
-        // unfold to 15x15 (DttRad2.java:443-489): CC[ay][ax] +- SC[ay][ax-1] +- CS[ay-1][ax] +- SS[ay-1][ax-1]
-        const float a = oq[0];
-        float b = __shfl(oq[1], (lane + 63) & 63);
-        float c = __shfl(oq[2], (lane + 56) & 63);
-        float d = __shfl(oq[3], (lane + 55) & 63);
-        if (xx == 0) { b = 0.f; d = 0.f; }
-        if (y == 0) { c = 0.f; d = 0.f; }
-        const float v_pp = 0.25f * (a + b + c + d), v_pm = 0.25f * (a - b + c - d);
-        const float v_mp = 0.25f * (a + b - c - d), v_mm = 0.25f * (a - b - c + d);
-        float* co = corr_t + p * CLT_CORR_LEN;
-        float* cs = s_corr + p * CLT_CORR_LEN;
-        co[o_pp] = v_pp;
-        cs[o_pp] = v_pp;
-        if (xx > 0) { co[o_pm] = v_pm; cs[o_pm] = v_pm; }
-        if (y > 0) { co[o_mp] = v_mp; cs[o_mp] = v_mp; }
-        if (xx > 0 && y > 0) { co[o_mm] = v_mm; cs[o_mm] = v_mm; }
+        corr_fd_to_tile(acc, lp, S, cy_, sy_, cx_, sx_, lane, corr_t + p * CLT_CORR_LEN, s_corr + p * CLT_CORR_LEN);
     }
     wave_sync();
     // ---- combined strip (C2:730-790) of the 6 rotated strips (C2:556-718)

@@ -16,6 +16,7 @@
 #include "clt_kernels.hpp"
 #include "clt_corr_lma.hpp"
 #include "clt_kernel_prep.hpp"
+#include "clt_rig.hpp"
 
 namespace clt {
 __constant__ Tables c_tab;
@@ -103,7 +104,7 @@ static void half_flat_top(int n, double hwidth, double blur, bool normalize, boo
 }
 
 // ------------------------------------------------------------------------ handle
-enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_CORR_LMA, T_KPREP, T_NUM };
+enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_CORR_LMA, T_KPREP, T_RIG, T_NUM };
 
 struct clt_handle {
     clt_geom g{};
@@ -177,6 +178,21 @@ struct clt_handle {
     int graph_epoch_cap = -1;
     CorrArgs* d_corr_args = nullptr;
     TexArgs* d_tex_args = nullptr;
+    // two-quad rig (clt_exec_corr2d_rig): cameras 0..3 the main quad, 4..7 the auxiliary one
+    clt_rig_params rp{};
+    float* d_rig_corr = nullptr;   // [2][max_tasks][6][225] pair correlations of the main / aux quad
+    float* d_rig_disp = nullptr;   // [2][max_tasks][CLT_DISP_LAYERS]
+    float* d_rig_mix = nullptr;    // [2][max_tasks][3][256] mean CLT of each quad
+    float* d_rig_dirs = nullptr;   // [2][max_tasks][8] one-direction fits
+    float* d_inter = nullptr;      // [max_tasks][225] inter-camera correlations
+    float* d_inter_avg = nullptr;  // [max_tasks][225] their low-texture averages
+    float* d_inter_rec = nullptr;  // [max_tasks][RIG_REC]
+    float* d_inter_res = nullptr;  // [max_tasks][4]
+    float* d_bimap = nullptr;      // [max_tasks][CLT_BI_LAYERS]
+    float* d_rig_target = nullptr; // [max_tasks]
+    RigArgs* d_rig_args = nullptr; // [2]: as set; with no_int_x0 for the low-texture pass
+    LmaArgs* d_lma_rig = nullptr;
+    bool rig_done = false, rig_target = false;
     size_t corr_cap = 0, combo_cap = 0, tex_cap = 0, tex_img_cap = 0;
     // state
     bool kernels_set[CLT_MAX_CAMS] = {};
@@ -284,7 +300,7 @@ static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
         auto kern = (h->cp.fat_zero != 0.f) ? k_corr2d_quad<true> : k_corr2d_quad<false>;
         hipLaunchKernelGGL(kern, dim3((n + tpb - 1) / tpb), dim3(64 * tpb), (size_t)tpb * CQ_WAVE_FLOATS * 4,
                            h->stream, h->d_clt, h->d_tasks, h->task_words, n, h->d_lpf_corr, h->d_corr_args, h->d_corr,
-                           h->d_combo, h->d_disp, task_frame, h->clt_fstride, h->gd);
+                           h->d_combo, h->d_disp, task_frame, h->clt_fstride, h->gd, nullptr);
     } else {
         const size_t lds = (size_t)4 * ((corr_wave_floats(h->corr_num_strip) + 3) & ~3) * 4;
         hipLaunchKernelGGL(k_corr2d<false>, dim3((n + 3) / 4), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
@@ -303,7 +319,7 @@ static void launch_corr_lma(clt_handle* h, int n, bool timed) {
     (void)hipStreamWaitEvent(h->stream_lma, h->ev_fork, 0);
     if (timed) tbegin(h, T_CORR_LMA, h->stream_lma);
     hipLaunchKernelGGL(kern, dim3((n + slots - 1) / slots), dim3(64 * LMA_WAVES), 0, h->stream_lma, h->d_corr, h->d_disp,
-                       n, h->d_lma_args);
+                       n, h->d_lma_args, nullptr);
     if (timed) tend(h, T_CORR_LMA, h->stream_lma);
     (void)hipEventRecord(h->ev_lma, h->stream_lma);
     h->lma_pending = true;
@@ -443,6 +459,7 @@ static int create_handle(const clt_geom* geom, int frames, clt_handle** out) {
                     g.kernels_hor, g.kernels_vert, g.kernel_step, h->tiles_x, h->tiles_y};
     clt_default_corr_params(&h->cp);
     clt_default_corr_lma_params(&h->lp);
+    clt_default_rig_params(&h->rp);
     clt_default_tex_params(&h->tp, g.num_cams);
     h->pairs.resize(0);
     if (g.num_cams == 4) {
@@ -506,7 +523,9 @@ int clt_destroy(clt_handle* h) {
                     h->d_lpf_corr, h->d_rbg, h->d_corr, h->d_combo, h->d_disp, h->d_tex, h->d_tex_stats,
                     h->d_tex_img, h->d_extra, h->d_corr_args, h->d_tex_args, h->d_port_geom, h->d_rtab,
                     h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv, h->d_task_frame, h->d_staged, h->d_fcounts,
-                    h->d_adj, h->d_tstate, h->d_gpart, h->d_gsum, h->d_gdisp2, h->d_lma_args};
+                    h->d_adj, h->d_tstate, h->d_gpart, h->d_gsum, h->d_gdisp2, h->d_lma_args, h->d_rig_corr,
+                    h->d_rig_disp, h->d_rig_mix, h->d_rig_dirs, h->d_inter, h->d_inter_avg, h->d_inter_rec,
+                    h->d_inter_res, h->d_bimap, h->d_rig_target, h->d_rig_args, h->d_lma_rig};
     if (h->stream) (void)sync_streams(h);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -715,7 +734,7 @@ int clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks) {
     HIPCHK(h, hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice));
     h->batch_dirty = true;   // the device task list now holds frame 0 only
     h->num_tasks = num_tasks;
-    h->converted = h->corr_done = h->tex_done = h->combined = h->adj_done = false;
+    h->converted = h->corr_done = h->tex_done = h->combined = h->adj_done = h->rig_done = false;
     h->tasks_from_geom = false;
     return CLT_OK;
 }
@@ -874,7 +893,7 @@ int clt_set_tasks_geometry(clt_handle* h, const int32_t* task, const int32_t* tx
     }
     HIPCHK(h, hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice));
     h->num_tasks = num_tasks;
-    h->converted = h->corr_done = h->tex_done = h->combined = h->adj_done = false;
+    h->converted = h->corr_done = h->tex_done = h->combined = h->adj_done = h->rig_done = false;
     h->tasks_from_geom = true;
     h->batch_dirty = true;
     if ((rc = launch_port_centres(h, nullptr))) return rc;
@@ -978,7 +997,7 @@ int clt_exec_convert(clt_handle* h, int32_t lpf_mask) {
     tend(h, T_CONVERT);
     HIPCHK(h, hipGetLastError());
     h->converted = true;
-    h->corr_done = h->tex_done = h->combined = false;
+    h->corr_done = h->tex_done = h->combined = h->rig_done = false;
     return finish(h);
 }
 
@@ -1015,21 +1034,19 @@ static int ensure_capacity(clt_handle* h, float** p, size_t* cap, size_t floats)
 // horizontal, vertical, main- and other-diagonal pairs (corr4dirsLMA, C2:1591-1630); each fit's pairs
 // grouped by {diagonal, scale} in the order of GROUPS (C2:72-78) and combined as
 // combineCompatiblePairs does (C2:462-488).
-static int build_lma_args(clt_handle* h, bool have_records) {
-    LmaArgs& a = h->lma_args;
+// The LMA fits of a pair list (the fit over all oriented pairs, then one per direction) and the
+// parameters of lp into a.
+static int fill_lma_args(clt_handle* h, const std::vector<clt_pair>& pairs, const clt_corr_lma_params& lp, LmaArgs& a) {
     std::memset(&a, 0, sizeof(a));
-    h->lma_run = false;
-    const clt_corr_lma_params& lp = h->lp;
-    if (!lp.enable || !have_records) return CLT_OK;
     static const int groups[6][2] = {{0, 1}, {1, 1}, {0, 2}, {1, 2}, {0, 4}, {1, 4}};
-    const int P = (int)h->pairs.size();
-    for (int f = 0; f < LMA_NFITS; f++) {
+    const int P = (int)pairs.size();
+    for (int f = 0; f < LMA_FIT_SINGLE; f++) {
         LmaFit& F = a.fit[f];
         for (int gi = 0; gi < 6; gi++) {
             const int diag = groups[gi][0], scale = groups[gi][1];
             int members[CLT_MAX_PAIRS], n = 0;
             for (int p = 0; p < P; p++) {
-                const clt_pair& q = h->pairs[p];
+                const clt_pair& q = pairs[p];
                 const bool sel = (f == 0) ? q.dir >= 0 : q.dir == f - 1;
                 if (sel && q.dir >= 0 && ((q.dir >= 2) == (diag != 0)) && q.scale == scale) members[n++] = p;
             }
@@ -1043,7 +1060,7 @@ static int build_lma_args(clt_handle* h, bool have_records) {
             G.diag = diag;
             G.scale = scale;
             for (int i = 0; i < n; i++) {
-                const int d = h->pairs[members[i]].dir;
+                const int d = pairs[members[i]].dir;
                 G.pair[i] = members[i];
                 G.xform[i] = (d == 1) ? 1 : ((d == 3) ? 2 : 0);
             }
@@ -1079,8 +1096,17 @@ static int build_lma_args(clt_handle* h, bool have_records) {
     a.fo_far = lp.fo_far;
     a.fo_ortho = lp.fo_ortho;
     a.npairs = P;
-    h->lma_run = a.fit[0].ngroups > 0;
+    return CLT_OK;
+}
+
+static int build_lma_args(clt_handle* h, bool have_records) {
+    LmaArgs& a = h->lma_args;
+    std::memset(&a, 0, sizeof(a));
+    h->lma_run = false;
+    if (!h->lp.enable || !have_records) return CLT_OK;
     int rc;
+    if ((rc = fill_lma_args(h, h->pairs, h->lp, a))) return rc;
+    h->lma_run = a.fit[0].ngroups > 0;
     if ((rc = dalloc(h, (void**)&h->d_lma_args, sizeof(LmaArgs)))) return rc;
     join_lma(h);   // a running LMA still reads the previous arguments
     HIPCHK(h, hipMemcpyAsync(h->d_lma_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
@@ -1174,6 +1200,205 @@ int clt_exec_corr2d(clt_handle* h) {
     HIPCHK(h, hipGetLastError());
     h->corr_done = true;
     return finish(h);
+}
+
+// ------------------------------------------------------------------------ two-quad rig
+void clt_default_rig_params(clt_rig_params* p) {   // BiQuadParameters BQP:32-83, IDP:81-112, ECP:2566
+    p->use_poly = 1;
+    p->use_xy_poly = 1;
+    p->no_int_x0 = 1;
+    p->notch_mode = 0;
+    p->lt_rad = 0;
+    p->corr_strip_notch = 11;
+    p->corr_notch_hwidth = 2.0;
+    p->corr_notch_blur = 2.0;
+    p->cm_max_normalization = 0.55;
+    p->max_corr_radius = 3.9;
+}
+
+int clt_set_rig_params(clt_handle* h, const clt_rig_params* p) {
+    if (!h || !p) return CLT_E_ARG;
+    if (p->lt_rad < 0 || p->lt_rad > 8 || p->corr_strip_notch < 1 || p->corr_strip_notch > 64 ||
+        !(fabs(p->max_corr_radius) <= 7.0) || !(p->cm_max_normalization > 0))
+        return fail(h, CLT_E_ARG, "clt_set_rig_params: lt_rad %d (0..8), notch rows %d (1..64), radius %g (|r| <= 7)",
+                    p->lt_rad, p->corr_strip_notch, p->max_corr_radius);
+    h->rp = *p;
+    return CLT_OK;
+}
+
+// Buffers, CorrArgs of the quad pairs, the rig's LmaArgs and RigArgs, the target disparities.
+static int prepare_rig(clt_handle* h, const float* target) {
+    const clt_geom& g = h->g;
+    if (g.num_cams != 8 || g.num_colors != 3)
+        return fail(h, CLT_E_UNSUPPORTED, "clt_exec_corr2d_rig: needs 8 cameras (two quads) of Bayer frames");
+    const size_t T = h->cap_tasks;
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_rig_corr, 2 * T * 6 * CLT_CORR_LEN * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_rig_disp, 2 * T * CLT_DISP_LAYERS * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_rig_mix, 2 * T * 3 * 256 * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_rig_dirs, 2 * T * 8 * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_inter, T * CLT_CORR_LEN * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_inter_rec, T * RIG_REC * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_inter_res, T * 4 * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_bimap, T * CLT_BI_LAYERS * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_rig_args, 2 * sizeof(RigArgs))) ||
+        (rc = dalloc(h, (void**)&h->d_lma_rig, sizeof(LmaArgs))) ||
+        (h->rp.lt_rad > 0 && (rc = dalloc(h, (void**)&h->d_inter_avg, T * CLT_CORR_LEN * 4))) ||
+        (target && (rc = dalloc(h, (void**)&h->d_rig_target, T * 4))))
+        return rc;
+    const clt_rig_params& rp = h->rp;
+    const int hw = h->cp.strip_height;
+    // CorrArgs: the quad pair table on cameras 0..3 of each quad's CLT base
+    std::vector<clt_pair> quad(6);
+    clt_default_quad_pairs(quad.data());
+    CorrArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.num_pairs = 6;
+    for (int i = 0; i < 6; i++) {
+        a.cam1[i] = quad[i].cam1;
+        a.cam2[i] = quad[i].cam2;
+        a.strip_slot[i] = i;
+        a.strip_dir[i] = quad[i].dir;
+        a.strip_lg[i] = quad[i].dir > 1 ? 0 : 1;
+        a.strip_is_hor1[i] = quad[i].dir == 0;
+        a.strip_is_vert1[i] = quad[i].dir == 1;
+    }
+    a.num_strip = 6;
+    a.n_hor1 = a.n_vert1 = 2;
+    const double gw = 1.0 / (1.0 + h->cp.corr_red + h->cp.corr_blue);   // ImageDtt.java:8793-8796
+    a.col_w[2] = (float)gw;
+    a.col_w[0] = (float)(h->cp.corr_red * gw);
+    a.col_w[1] = (float)(h->cp.corr_blue * gw);
+    a.fat_zero = h->cp.fat_zero;
+    a.corr_offset = h->cp.corr_offset;
+    a.min_corr = h->cp.min_corr;
+    a.twice_diagonal = h->cp.twice_diagonal;
+    a.strip_height = hw;
+    half_flat_top(2, 1.0, 1.0, true, false, 1.0, a.wndy);
+    half_flat_top(9, 6.0, 5.0, true, false, 2.0, a.wndx);
+    for (int i = 0; i < 15; i++) {
+        double f = ((i < (8 - h->cp.enhortho_width)) || (i > (8 - 2 + h->cp.enhortho_width))) ? 1.0 : h->cp.enhortho_scale;
+        if (i == 7) f = 0.0;
+        a.notch[i] = (float)(f * std::sin(M_PI * (i + 1.0) / 16));
+    }
+    // the tileCorrs LMA: mixing by rig.use_poly, the four directions on every fitted tile (dir_out),
+    // no foreground correction; fit LMA_FIT_SINGLE is one correlation as a horizontal pair
+    LmaArgs la;
+    if ((rc = fill_lma_args(h, quad, h->lp, la))) return rc;
+    la.mix = rp.use_poly;
+    la.fo_correct = 0;
+    la.fit[LMA_FIT_SINGLE].ngroups = 1;
+    la.fit[LMA_FIT_SINGLE].g[0].npairs = 1;
+    la.fit[LMA_FIT_SINGLE].g[0].diag = 0;
+    la.fit[LMA_FIT_SINGLE].g[0].scale = 1.0;
+    // RigArgs: tileInterCamCorrs as set, and its low-texture call with no_int_x0 (ID:9455-9466)
+    RigArgs ra[2];
+    for (int k = 0; k < 2; k++) {
+        RigArgs& r = ra[k];
+        std::memset(&r, 0, sizeof(r));
+        const bool nix = k ? true : rp.no_int_x0 != 0;
+        r.min_corr = (float)((nix ? 0.5 : 1.0) * h->cp.min_corr);
+        r.cm_at_zero = nix || rp.notch_mode;
+        // getMaxXCm (C2:936-1032) / getMaxXCmNotch (C2:925-934) windows of the Correlation2d
+        // constructor (C2:139-159)
+        const int wlen = rp.notch_mode ? rp.corr_strip_notch : 2;
+        std::vector<float> wy(wlen);
+        if (rp.notch_mode) half_flat_top(wlen, rp.corr_notch_hwidth, rp.corr_notch_blur, true, true, 2.0, wy.data());
+        else half_flat_top(2, 1.0, 1.0, true, false, 1.0, wy.data());
+        int rows = hw;
+        double wy_scale = 1.0;
+        if (rows > wlen) {
+            rows = wlen;
+        } else if (rows < wlen) {
+            double swy = wy[0];
+            for (int i = 1; i < rows; i++) swy += wy[i];
+            wy_scale = 1.0 / swy;
+        }
+        r.cm_rows = rows;
+        for (int i = 0; i < rows; i++) r.cm_wy[i] = (float)(((i == 0) ? wy_scale : 2.0 * wy_scale) * wy[i]);
+        half_flat_top(9, 6.0, 5.0, true, false, 2.0, r.wndx);
+        r.cm_max_norm = (float)rp.cm_max_normalization;
+        r.iradius = (int)std::fabs(rp.max_corr_radius);
+        r.ir2 = (int)(rp.max_corr_radius * rp.max_corr_radius);
+        r.square = rp.max_corr_radius < 0;
+    }
+    join_lma(h);
+    HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_lma_rig, &la, sizeof(la), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->d_rig_args, ra, sizeof(ra), hipMemcpyHostToDevice, h->stream));
+    h->rig_target = target != nullptr;
+    if (target) HIPCHK(h, hipMemcpyAsync(h->d_rig_target, target, (size_t)h->num_tasks * 4, hipMemcpyHostToDevice, h->stream));
+    h->graph_epoch++;   // the correlation arguments now hold the quad table
+    return CLT_OK;
+}
+
+int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
+    if (!h) return CLT_E_ARG;
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_corr2d_rig: a batch handle runs clt_exec_batch");
+    if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_corr2d_rig: no converted tiles");
+    int rc;
+    if ((rc = prepare_rig(h, target_disparity))) return rc;
+    const int n = h->num_tasks;
+    const size_t T = h->cap_tasks, ntiles = (size_t)h->tiles_x * h->tiles_y;
+    const bool fz = h->cp.fat_zero != 0.f;
+    const int nc = h->lp.lma_adjust_wy ? 4 : 3;
+    tbegin(h, T_RIG);
+    for (int q = 0; q < 2; q++) {   // each quad's correlate2D, with its mean CLT
+        const int tpb = h->tune_corr_tpb;
+        auto kern = fz ? k_corr2d_quad<true> : k_corr2d_quad<false>;
+        hipLaunchKernelGGL(kern, dim3((n + tpb - 1) / tpb), dim3(64 * tpb), (size_t)tpb * CQ_WAVE_FLOATS * 4, h->stream,
+                           h->d_clt + (size_t)q * 4 * ntiles * 3 * 256, h->d_tasks, h->task_words, n, h->d_lpf_corr,
+                           h->d_corr_args, h->d_rig_corr + q * T * 6 * CLT_CORR_LEN, nullptr,
+                           h->d_rig_disp + q * T * CLT_DISP_LAYERS, nullptr, 0LL, h->gd, h->d_rig_mix + q * T * 3 * 256);
+    }
+    const dim3 g4((n + 3) / 4), g_lma((n + LMA_SLOTS - 1) / LMA_SLOTS);
+    auto inter = fz ? k_corr_inter<true, true> : k_corr_inter<false, true>;
+    hipLaunchKernelGGL(inter, g4, dim3(256), 0, h->stream, h->d_rig_mix, h->d_rig_mix + T * 3 * 256, nullptr, h->d_tasks,
+                       h->task_words, n, h->d_lpf_corr, h->d_corr_args, h->d_rig_args, h->d_inter, h->d_inter_rec);
+    auto lma = (nc == 4) ? k_corr_lma<4> : k_corr_lma<3>;
+    for (int q = 0; q < 2; q++)
+        hipLaunchKernelGGL(lma, g_lma, dim3(64 * LMA_WAVES), 0, h->stream, h->d_rig_corr + q * T * 6 * CLT_CORR_LEN,
+                           h->d_rig_disp + q * T * CLT_DISP_LAYERS, n, h->d_lma_rig, h->d_rig_dirs + q * T * 8);
+    auto lma_x = (nc == 4) ? k_corr_lma_inter<4> : k_corr_lma_inter<3>;
+    const clt_rig_params& rp = h->rp;
+    hipLaunchKernelGGL(lma_x, g_lma, dim3(64 * LMA_WAVES), 0, h->stream, h->d_inter, h->d_inter_rec, h->d_inter_res, n,
+                       h->d_lma_rig, rp.use_poly, rp.use_xy_poly, rp.notch_mode);
+    if (rp.lt_rad > 0) {   // low-texture pass (ID:9404-9480)
+        hipLaunchKernelGGL(k_inter_lt_avg, g4, dim3(256), 0, h->stream, h->d_inter, h->d_tasks, h->task_words, n,
+                           h->d_tile_map, rp.lt_rad, rp.notch_mode ? 0 : rp.lt_rad, h->gd, h->d_inter_avg);
+        auto inter_lt = k_corr_inter<false, false>;   // no mix input: the fat zero is not read
+        hipLaunchKernelGGL(inter_lt, g4, dim3(256), 0, h->stream, nullptr, nullptr, h->d_inter_avg, h->d_tasks,
+                           h->task_words, n, h->d_lpf_corr, h->d_corr_args, h->d_rig_args + 1, nullptr, h->d_inter_rec);
+        hipLaunchKernelGGL(lma_x, g_lma, dim3(64 * LMA_WAVES), 0, h->stream, h->d_inter_avg, h->d_inter_rec,
+                           h->d_inter_res, n, h->d_lma_rig, rp.use_poly, rp.use_xy_poly, rp.notch_mode);
+    }
+    hipLaunchKernelGGL(k_rig_bimap, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_tasks, h->task_words, n,
+                       h->d_rig_disp, h->d_rig_dirs, h->d_rig_disp + T * CLT_DISP_LAYERS, h->d_rig_dirs + T * 8,
+                       h->d_inter_res, h->rig_target ? h->d_rig_target : nullptr, h->d_bimap);
+    tend(h, T_RIG);
+    HIPCHK(h, hipGetLastError());
+    h->rig_done = true;
+    return finish(h);
+}
+
+int clt_get_corr2d_rig(clt_handle* h, float* bimap, float* inter, float* corr_main, float* corr_aux, float* disp_main,
+                       float* disp_aux) {
+    if (!h) return CLT_E_ARG;
+    if (!h->rig_done) return fail(h, CLT_E_STATE, "clt_get_corr2d_rig: rig correlation not executed");
+    HIPCHK(h, sync_streams(h));
+    const size_t nt = h->num_tasks, T = h->cap_tasks;
+    if (bimap) HIPCHK(h, hipMemcpy(bimap, h->d_bimap, nt * CLT_BI_LAYERS * 4, hipMemcpyDeviceToHost));
+    if (inter) HIPCHK(h, hipMemcpy(inter, h->d_inter, nt * CLT_CORR_LEN * 4, hipMemcpyDeviceToHost));
+    float* dst[4] = {corr_main, corr_aux, disp_main, disp_aux};
+    for (int q = 0; q < 2; q++) {
+        if (dst[q])
+            HIPCHK(h, hipMemcpy(dst[q], h->d_rig_corr + q * T * 6 * CLT_CORR_LEN, nt * 6 * CLT_CORR_LEN * 4,
+                                hipMemcpyDeviceToHost));
+        if (dst[2 + q])
+            HIPCHK(h, hipMemcpy(dst[2 + q], h->d_rig_disp + q * T * CLT_DISP_LAYERS, nt * CLT_DISP_LAYERS * 4,
+                                hipMemcpyDeviceToHost));
+    }
+    return CLT_OK;
 }
 
 // Texture arguments and output buffers for the current parameters (TexArgs upload).

@@ -259,6 +259,15 @@ class GPUTileProcessor:
             setattr(p, k, v)
         self._chk(self.lib.clt_set_corr_lma_params(self.h, ctypes.byref(p)))
 
+    def setRigParams(self, **kw):
+        """Two-quad rig (BiQuadParameters BQP:32-83, IDP:81-112); reference defaults updated by
+        the keywords, e.g. lt_rad=1, notch_mode=1, use_xy_poly=0."""
+        p = _abi.clt_rig_params()
+        self.lib.clt_default_rig_params(ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._chk(self.lib.clt_set_rig_params(self.h, ctypes.byref(p)))
+
     def setTexParams(self, port_offsets=None, **kw):
         p = _abi.clt_tex_params()
         self.lib.clt_default_tex_params(ctypes.byref(p), self.NUM_CAMS)
@@ -310,6 +319,26 @@ class GPUTileProcessor:
         disp = np.empty((n, _abi.DISP_LAYERS), np.float32)
         self._chk(self.lib.clt_get_corr2d(self.h, _ptr(corr), _ptr(combo), _ptr(disp)))
         return corr, combo, disp
+
+    def execCorrelate2DRig(self, target_disparity=None):
+        """clt_bi_quad (ID:8731-9504) correlation part on an 8-camera handle (main quad 0..3,
+        auxiliary 4..7); target_disparity: per task (the BI_TARGET layer) or None."""
+        tg = None
+        if target_disparity is not None:
+            tg = np.ascontiguousarray(np.broadcast_to(np.asarray(target_disparity, np.float32), (self.num_task_tiles,)))
+        self._chk(self.lib.clt_exec_corr2d_rig(self.h, None if tg is None else _ptr(tg)))
+
+    def getCorr2DRig(self):
+        """dict: bimap [n][BI_LAYERS] (layers named by _abi.BI_NAMES), inter [n][15][15],
+        corr_main / corr_aux [n][6][15][15], disp_main / disp_aux [n][DISP_LAYERS]."""
+        n = self.num_task_tiles
+        out = {"bimap": np.empty((n, len(_abi.BI_NAMES)), np.float32), "inter": np.empty((n, 15, 15), np.float32),
+               "corr_main": np.empty((n, 6, 15, 15), np.float32), "corr_aux": np.empty((n, 6, 15, 15), np.float32),
+               "disp_main": np.empty((n, _abi.DISP_LAYERS), np.float32),
+               "disp_aux": np.empty((n, _abi.DISP_LAYERS), np.float32)}
+        self._chk(self.lib.clt_get_corr2d_rig(self.h, *[_ptr(out[k]) for k in ("bimap", "inter", "corr_main", "corr_aux",
+                                                                             "disp_main", "disp_aux")]))
+        return out
 
     def getTextures(self):
         """(tiles [n][planes][16][16], stats [n][4C])."""

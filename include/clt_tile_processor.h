@@ -153,6 +153,33 @@ typedef struct clt_corr_lma_params {
     double  fo_lim_overcorr;     /* 2.0                                           (IDP:63)         */
 } clt_corr_lma_params;
 
+/* Two-quad rig (ImageDtt.clt_bi_quad, ID:8731-9504): BiQuadParameters (BQP) and the
+ * ImageDttParameters / CLTParameters fields the inter-camera stage reads beyond the ones above. */
+typedef struct clt_rig_params {
+    int32_t use_poly;             /* LMA replaces the centre-of-mass disparity, 1    (BQP:33)     */
+    int32_t use_xy_poly;          /* dx, dy by the 2-D polynomial (else CM), 1       (BQP:34)     */
+    int32_t no_int_x0;            /* inter CM around 0, not the integer max, 1       (BQP:32)     */
+    int32_t notch_mode;           /* pole detection of the inter correlation, 0      (TQ:312)     */
+    int32_t lt_rad;               /* low-texture averaging radius in tiles, 0        (BQP:83)     */
+    int32_t corr_strip_notch;     /* rows of the notch window, 11                    (IDP:81)     */
+    double  corr_notch_hwidth;    /* 2.0                                             (IDP:82)     */
+    double  corr_notch_blur;      /* 2.0                                             (IDP:83)     */
+    double  cm_max_normalization; /* 0.55                                            (IDP:112)    */
+    double  max_corr_radius;      /* single2dCM radius, 3.9                          (ECP:2566)   */
+} clt_rig_params;
+
+/* Per-task rig record written by clt_exec_corr2d_rig: the BIDISPARITY layers of clt_bi_quad
+ * (ID:89-114); the main quad is cameras 0..3, the auxiliary quad cameras 4..7. */
+enum {
+    CLT_BI_DISP_FULL = 0, CLT_BI_DISP_HOR, CLT_BI_DISP_VERT, CLT_BI_DISP_DIAGM, CLT_BI_DISP_DIAGO,
+    CLT_BI_ADISP_FULL = 5, CLT_BI_ADISP_HOR, CLT_BI_ADISP_VERT, CLT_BI_ADISP_DIAGM, CLT_BI_ADISP_DIAGO,
+    CLT_BI_DISP_CROSS = 10, CLT_BI_DISP_CROSS_DX, CLT_BI_DISP_CROSS_DY,
+    CLT_BI_STR_FULL = 13, CLT_BI_STR_HOR, CLT_BI_STR_VERT, CLT_BI_STR_DIAGM, CLT_BI_STR_DIAGO,
+    CLT_BI_ASTR_FULL = 18, CLT_BI_ASTR_HOR, CLT_BI_ASTR_VERT, CLT_BI_ASTR_DIAGM, CLT_BI_ASTR_DIAGO,
+    CLT_BI_STR_CROSS = 23, CLT_BI_STR_ALL = 24, CLT_BI_TARGET = 25,
+    CLT_BI_LAYERS = 26
+};
+
 typedef struct clt_handle clt_handle;
 
 /* ---- life cycle: GPUTileProcessor() (GTP:243-360); the reference never frees ---------------- */
@@ -211,6 +238,9 @@ int  clt_lpf_coeffs(double sigma, float out[64]);
 void clt_default_corr_params(clt_corr_params* p);
 void clt_default_tex_params(clt_tex_params* p, int32_t num_cams);
 void clt_default_corr_lma_params(clt_corr_lma_params* p);
+void clt_default_rig_params(clt_rig_params* p);
+/* BiQuadParameters of the two-quad rig path (clt_exec_corr2d_rig). */
+int  clt_set_rig_params(clt_handle* h, const clt_rig_params* p);
 int  clt_default_quad_pairs(clt_pair out[6]);
 
 /* ---- kernels ---------------------------------------------------------------------------------- */
@@ -223,6 +253,15 @@ int  clt_exec_imclt(clt_handle* h, int32_t apply_lpf);
  * per-tile disparity (ID:2186-2238) and, when enabled, the per-tile LMA chain (ID:2254-2336) over
  * the pairs with a known orientation (at most 2 compatible groups, C2:72-78). */
 int  clt_exec_corr2d(clt_handle* h);
+/* The correlation part of clt_bi_quad(...) (ID:8731-9504) for a two-quad rig held as one 8-camera
+ * frame: cameras 0..3 the main quad, 4..7 the auxiliary quad (tasks carry TpTask.xy then xy_aux,
+ * GTP:132-166).  Per task: tileCorrs (ID:7581-7740) of each quad over the quad pair table
+ * (correlate2D, the LMA over all pairs replacing the centre of mass when use_poly, the four
+ * one-direction LMA fits), correlateInterCamerasFD (C2:401-448) of the two quads' mean CLT,
+ * tileInterCamCorrs (ID:7370-7560), with lt_rad > 0 the low-texture pass (ID:9404-9480), and the
+ * BIDISPARITY record (CLT_BI_*).  Uses the correlation, LMA and rig parameters and corr_sigma;
+ * target_disparity: num_tasks floats for CLT_BI_TARGET or NULL (NaN). */
+int  clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity);
 /* Per-tile textures (ID:2662-2812).  extra_disparity: num_tasks floats or NULL (ID:2663-2677). */
 int  clt_exec_textures(clt_handle* h, const float* extra_disparity);
 /* Overlap-add of the texture tiles into full-frame planes (combineRGBATiles, ID:3858-3989,
@@ -238,6 +277,13 @@ int  clt_get_clt(clt_handle* h, int32_t cam, float* out);
 /* corr: [num_tasks][num_pairs][225]; combo: [num_tasks][strip_height*15] (may be NULL);
  * disparity: [num_tasks][CLT_DISP_LAYERS] (may be NULL). */
 int  clt_get_corr2d(clt_handle* h, float* corr, float* combo, float* disparity);
+/* Results of clt_exec_corr2d_rig, each may be NULL: bimap [num_tasks][CLT_BI_LAYERS] (the
+ * disparity_bimap layers of clt_bi_quad); inter [num_tasks][225] inter-camera correlations
+ * (clt_corr_combo[0], before the low-texture averaging); corr_main / corr_aux [num_tasks][6][225]
+ * pair correlations; disp_main / disp_aux [num_tasks][CLT_DISP_LAYERS] correlate2D records of
+ * each quad after its LMA (CM = the tileCorrs disparity, STRENGTH its strength). */
+int  clt_get_corr2d_rig(clt_handle* h, float* bimap, float* inter, float* corr_main, float* corr_aux,
+                        float* disp_main, float* disp_aux);
 /* tiles: [num_tasks][planes][256] with planes = 4 + num_cams + 4 when keep_weights (RGBA, one
  * weight per port, rms r,b,g,combined), else 4 (RGBA); stats:
  * [num_tasks][4*num_cams] = max_diff[num_cams] (ID:2802-2806) then the mean tones
@@ -351,7 +397,8 @@ int  clt_set_timing(clt_handle* h, int32_t enable);   /* record HIP events aroun
 /* Accumulated device time (ms) and launch count of a kernel since the last reset:
  * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures, 5 batch graph,
  * 6 correlation adjoint, 7 geometry adjoint, 8 per-tile correlation LMA, 9 kernel preparation
- * (clt_prepare_kernels: one count per band of kernel rows, plus one for the centre slopes). */
+ * (clt_prepare_kernels: one count per band of kernel rows, plus one for the centre slopes),
+ * 10 the whole of clt_exec_corr2d_rig. */
 int  clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count);
 int  clt_reset_timing(clt_handle* h);
 

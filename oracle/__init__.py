@@ -57,6 +57,14 @@ class CorrLmaParams(ctypes.Structure):
                 ("fo_lim_overcorr", ctypes.c_double)]
 
 
+class RigParams(ctypes.Structure):
+    """Mirror of ``clt_rig_params`` (BiQuadParameters BQP:32-83, IDP:81-112, ECP:2566)."""
+    _fields_ = [("use_poly", ctypes.c_int32), ("use_xy_poly", ctypes.c_int32), ("no_int_x0", ctypes.c_int32),
+                ("notch_mode", ctypes.c_int32), ("lt_rad", ctypes.c_int32), ("corr_strip_notch", ctypes.c_int32),
+                ("corr_notch_hwidth", ctypes.c_double), ("corr_notch_blur", ctypes.c_double),
+                ("cm_max_normalization", ctypes.c_double), ("max_corr_radius", ctypes.c_double)]
+
+
 class TexParams(ctypes.Structure):
     _fields_ = [("min_shot", ctypes.c_float), ("scale_shot", ctypes.c_float),
                 ("diff_sigma", ctypes.c_float), ("dust_remove", ctypes.c_int32),
@@ -127,6 +135,22 @@ def default_corr_lma_params(**kw):
 
 
 DISP_LAYERS = 12
+BI_LAYERS = 26
+(BI_DISP_FULL, BI_DISP_HOR, BI_DISP_VERT, BI_DISP_DIAGM, BI_DISP_DIAGO,
+ BI_ADISP_FULL, BI_ADISP_HOR, BI_ADISP_VERT, BI_ADISP_DIAGM, BI_ADISP_DIAGO,
+ BI_DISP_CROSS, BI_DISP_CROSS_DX, BI_DISP_CROSS_DY,
+ BI_STR_FULL, BI_STR_HOR, BI_STR_VERT, BI_STR_DIAGM, BI_STR_DIAGO,
+ BI_ASTR_FULL, BI_ASTR_HOR, BI_ASTR_VERT, BI_ASTR_DIAGM, BI_ASTR_DIAGO,
+ BI_STR_CROSS, BI_STR_ALL, BI_TARGET) = range(BI_LAYERS)
+
+
+def default_rig_params(**kw):
+    """BiQuadParameters and ImageDttParameters defaults of the rig path (BQP:32-83, IDP:81-112)."""
+    p = RigParams(use_poly=1, use_xy_poly=1, no_int_x0=1, notch_mode=0, lt_rad=0, corr_strip_notch=11,
+                  corr_notch_hwidth=2.0, corr_notch_blur=2.0, cm_max_normalization=0.55, max_corr_radius=3.9)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
 (DISP_INT, DISP_CM, DISP_STRENGTH, DISP_HOR, DISP_HOR_STR, DISP_VERT, DISP_VERT_STR, DISP_CM_WEIGHT,
  DISP_POLY, DISP_POLY_STR, DISP_CM_RAW, DISP_STR_RAW) = range(DISP_LAYERS)
 
@@ -210,6 +234,15 @@ def lib(precision=64):
     L.orc_r_by_rdist.argtypes = [ctypes.POINTER(PortGeometry), ctypes.c_double]
     L.orc_r_by_rdist.restype = ctypes.c_double
     L.orc_port_coords.argtypes = [ctypes.POINTER(PortGeometry), ctypes.c_int, _D, _D, _D, _D, _VP]
+    L.orc_rig_analysis.argtypes = [ctypes.POINTER(Geom), _VP, ctypes.c_int, ctypes.POINTER(CorrParams),
+                                   ctypes.POINTER(CorrLmaParams), ctypes.POINTER(RigParams), _D, _D, _D, _D, _D,
+                                   _VP, _D, ctypes.c_int]
+    L.orc_rig_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, ctypes.c_int, ctypes.POINTER(CorrParams),
+                                ctypes.POINTER(CorrLmaParams), ctypes.POINTER(RigParams), ctypes.c_double, _VP,
+                                _D, _D, _D, _D, _D, _D, ctypes.c_int]
+    L.orc_quadratic_max.argtypes = [_D, _D, _D, _D, ctypes.c_int, _D]
+    L.orc_single2d_cm.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D]
+    L.orc_single2d_poly.argtypes = [_D, ctypes.c_double, ctypes.POINTER(CorrLmaParams), _D]
     _LIBS[precision] = L
     return L
 
@@ -543,6 +576,72 @@ def port_coords(pg, px, py, disparity, deriv=False):
     lib(64).orc_port_coords(ctypes.byref(pg), len(px), _dp(px), _dp(py), _dp(dd), _dp(xy),
                             _vp(dv) if deriv else None)
     return (xy, dv) if deriv else xy
+
+
+def rig_frame(geom, clt, tasks, target=None, params=None, lma=None, rig=None, corr_sigma=0.8, nthreads=None):
+    """clt_bi_quad (ID:8731-9504) correlation part over a converted 8-camera frame (cameras 0..3 the
+    main quad, 4..7 the auxiliary one).  Returns a dict: corr_main / corr_aux [T][6][15][15],
+    disp_main / disp_aux [T][DISP_LAYERS] (correlate2D records, no LMA), inter [T][15][15],
+    bimap [T][BI_LAYERS]."""
+    nthreads = nthreads or os.cpu_count()
+    params = params or default_corr_params()
+    lma = lma if lma is not None else default_corr_lma_params()
+    rig = rig if rig is not None else default_rig_params()
+    tasks = np.ascontiguousarray(tasks, np.int32)
+    T = tasks.shape[0]
+    out = {"corr_main": np.zeros((T, 6, 15, 15)), "disp_main": np.zeros((T, DISP_LAYERS)),
+           "corr_aux": np.zeros((T, 6, 15, 15)), "disp_aux": np.zeros((T, DISP_LAYERS)),
+           "inter": np.zeros((T, 15, 15)), "bimap": np.zeros((T, BI_LAYERS))}
+    tg = None if target is None else np.ascontiguousarray(target, np.float32)
+    rc = lib(64).orc_rig_frame(ctypes.byref(geom), _dp(_f64(clt)), tasks.ctypes.data_as(_VP), T, ctypes.byref(params),
+                               ctypes.byref(lma), ctypes.byref(rig), corr_sigma, _vp(tg),
+                               *[_dp(out[k]) for k in ("corr_main", "disp_main", "corr_aux", "disp_aux", "inter",
+                                                       "bimap")], nthreads)
+    if rc:
+        raise ValueError("orc_rig_frame failed: %d" % rc)
+    return out
+
+
+def rig_analysis(geom, tasks, corr_main, disp_main, corr_aux, disp_aux, inter, target=None, params=None, lma=None,
+                 rig=None, nthreads=None):
+    """The rig analysis alone (tileCorrs of both quads, tileInterCamCorrs, the low-texture pass) on
+    given correlations and correlate2D records; returns bimap [T][BI_LAYERS]."""
+    nthreads = nthreads or os.cpu_count()
+    params = params or default_corr_params()
+    lma = lma if lma is not None else default_corr_lma_params()
+    rig = rig if rig is not None else default_rig_params()
+    tasks = np.ascontiguousarray(tasks, np.int32)
+    T = tasks.shape[0]
+    arrs = [_f64(a) for a in (corr_main, disp_main, corr_aux, disp_aux, inter)]
+    bimap = np.zeros((T, BI_LAYERS))
+    tg = None if target is None else np.ascontiguousarray(target, np.float32)
+    rc = lib(64).orc_rig_analysis(ctypes.byref(geom), tasks.ctypes.data_as(_VP), T, ctypes.byref(params),
+                                  ctypes.byref(lma), ctypes.byref(rig), *[_dp(a) for a in arrs], _vp(tg), _dp(bimap),
+                                  nthreads)
+    if rc:
+        raise ValueError("orc_rig_analysis failed: %d" % rc)
+    return bimap
+
+
+def quadratic_max(x, y, z, w):
+    """PA:201-224 over PA:350-580: (x, y, f, A, B) of the weighted quadratic fit, or None."""
+    x, y, z, w = (_f64(a) for a in (x, y, z, w))
+    out = np.zeros(5)
+    ok = lib(64).orc_quadratic_max(_dp(x), _dp(y), _dp(z), _dp(w), len(x), _dp(out))
+    return out if ok else None
+
+
+def single2d_cm(corr225, xcenter=0.0, radius=3.9):
+    out = np.zeros(3)
+    lib(64).orc_single2d_cm(_dp(_f64(corr225).ravel()), xcenter, radius, _dp(out))
+    return out
+
+
+def single2d_poly(corr225, xcenter=0.0, lma=None):
+    lma = lma if lma is not None else default_corr_lma_params()
+    out = np.zeros(3)
+    lib(64).orc_single2d_poly(_dp(_f64(corr225).ravel()), xcenter, ctypes.byref(lma), _dp(out))
+    return out
 
 
 def prepare_kernels(raw, kernels_hor, kernels_vert, ks=64, kernel_step=16, sensor_xy=None, norm_kern=True):

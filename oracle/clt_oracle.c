@@ -1125,11 +1125,10 @@ static int lma_run(lma_t* L, const clt_corr_lma_params* lp) {
     return L->last_rms[0] < L->initial_rms[0];
 }
 
-/* C2:1754-1900 corrLMA (run_poly_instead = false) with C2:2230-2338 addSamples; on success
- * out = {disparity, strength, Wm, Wxy} (getDisparityStrengthWidth, Correlations2dLMA:165-185). */
-static int corr_lma(const double (*corrs)[225], const clt_pair* pairs, int npairs, const char* sel, double xcenter,
-                    const clt_corr_lma_params* lp, double out[4]) {
-    lma_t* L = malloc(sizeof(lma_t));
+/* C2:1754-1820 / C2:1903-1935 the samples of corrLMA: the compatible pair groups combined
+ * (C2:462-488), filterConvex around the start point and addSamples (C2:2230-2338). */
+static void lma_samples(lma_t* L, const double (*corrs)[225], const clt_pair* pairs, int npairs, const char* sel,
+                        double xcenter, const clt_corr_lma_params* lp) {
     L->ns = L->ngroups = L->have_last = L->input_diag = 0;
     for (int g = 0; g < LMA_MAX_GROUPS; g++) L->gnum[g] = -1;
     int sl[3], nsl = 0, g_num[6], g_scale_ind[6];
@@ -1177,6 +1176,14 @@ static int corr_lma(const double (*corrs)[225], const clt_pair* pairs, int npair
         }
         ig++;
     }
+}
+
+/* C2:1754-1900 corrLMA (run_poly_instead = false); on success out = {disparity, strength, Wm, Wxy}
+ * (getDisparityStrengthWidth, Correlations2dLMA:165-185). */
+static int corr_lma(const double (*corrs)[225], const clt_pair* pairs, int npairs, const char* sel, double xcenter,
+                    const clt_corr_lma_params* lp, double out[4]) {
+    lma_t* L = malloc(sizeof(lma_t));
+    lma_samples(L, corrs, pairs, npairs, sel, xcenter, lp);
     lma_init_vector(L, lp, xcenter);
     const int ok = lma_run(L, lp);
     for (int i = 0, k = 0; i < L->npars; i++)   /* updateFromVector */
@@ -2199,6 +2206,442 @@ EXPORT int orc_combine_textures(const clt_geom* g, const int32_t* tasks, int nta
         }
     }
     return 0;
+}
+
+/* ================= two-quad rig (ImageDtt.clt_bi_quad, ID:8731-9504) ================= */
+
+/* Jama 1.0.3 LUDecomposition (Matrix.det / Matrix.solve of PolynomialApproximation; Jama is not in
+ * the reference) restated as textbook LU with partial pivoting.  Returns the determinant; b (may
+ * be NULL) is replaced by A^-1 b when the determinant is not 0.  a (n x n, n <= 6) is destroyed. */
+static double lu_det_solve(int n, double* a, double* b) {
+    double det = 1.0;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(a[i * n + k]) > fabs(a[p * n + k])) p = i;
+        if (p != k) {
+            for (int j = 0; j < n; j++) {
+                double t = a[k * n + j];
+                a[k * n + j] = a[p * n + j];
+                a[p * n + j] = t;
+            }
+            if (b) {
+                double t = b[k];
+                b[k] = b[p];
+                b[p] = t;
+            }
+            det = -det;
+        }
+        const double d = a[k * n + k];
+        det *= d;
+        if (d == 0.0) continue;
+        for (int i = k + 1; i < n; i++) {
+            const double f = a[i * n + k] / d;
+            for (int j = k + 1; j < n; j++) a[i * n + j] -= f * a[k * n + j];
+            if (b) b[i] -= f * b[k];
+        }
+    }
+    if (b && det != 0.0)
+        for (int k = n - 1; k >= 0; k--) {
+            double x = b[k];
+            for (int j = k + 1; j < n; j++) x -= a[k * n + j] * b[j];
+            b[k] = x / a[k * n + k];
+        }
+    return det;
+}
+
+/* PA:590-598 normMatix: the product of the row norms */
+static double norm_matrix(int n, const double* a) {
+    double norm = 1.0;
+    for (int i = 0; i < n; i++) {
+        double d = 0.0;
+        for (int j = 0; j < n; j++) d += a[i * n + j] * a[i * n + j];
+        norm *= sqrt(d);
+    }
+    return norm;
+}
+
+/* PA:201-224 quadraticMaxV2dX2Y2XY over PA:350-580 quadraticApproximation (forceLinear false,
+ * no damping, thresholdLin 1e-20): weighted least squares z ~ A x^2 + B y^2 + C xy + D x + E y + F
+ * over the samples with w > 0, then the stationary point.  Returns 0 where the reference returns
+ * null; out = {x, y, f(x, y), A, B}. */
+static int quadratic_max(const double* x, const double* y, const double* z, const double* w, int n,
+                         double thr_quad, double out[5]) {
+    if (n == 0) return 0;
+    double S00 = 0, S10 = 0, S01 = 0, S20 = 0, S11 = 0, S02 = 0, S30 = 0, S21 = 0, S12 = 0, S03 = 0;
+    double S40 = 0, S31 = 0, S22 = 0, S13 = 0, S04 = 0, Z00 = 0, Z10 = 0, Z01 = 0, Z20 = 0, Z11 = 0, Z02 = 0;
+    for (int i = 0; i < n; i++) {
+        const double wi = w[i];
+        if (!(wi > 0)) continue;
+        const double xi = x[i], yi = y[i], x2 = xi * xi, y2 = yi * yi;
+        S00 += wi;
+        S10 += wi * xi;
+        S01 += wi * yi;
+        S11 += wi * xi * yi;
+        S20 += wi * x2;
+        S02 += wi * y2;
+        S30 += wi * x2 * xi;
+        S21 += wi * x2 * yi;
+        S12 += wi * xi * y2;
+        S03 += wi * y2 * yi;
+        S40 += wi * x2 * x2;
+        S31 += wi * x2 * xi * yi;
+        S22 += wi * x2 * y2;
+        S13 += wi * xi * y2 * yi;
+        S04 += wi * y2 * y2;
+        const double wz = wi * z[i];
+        Z00 += wz;
+        Z10 += wz * xi;
+        Z01 += wz * yi;
+        Z20 += wz * x2;
+        Z11 += wz * xi * yi;
+        Z02 += wz * y2;
+    }
+    double ml[9] = {S20, S11, S10, S11, S02, S01, S10, S01, S00};
+    const double nml = norm_matrix(3, ml);
+    if (nml == 0.0 || fabs(lu_det_solve(3, ml, NULL)) / nml < 1e-20) return 0;   /* linear only: null */
+    double mq[36] = {S40, S22, S31, S30, S21, S20, S22, S04, S13, S12, S03, S02, S31, S13, S22, S21, S12, S11,
+                     S30, S12, S21, S20, S11, S10, S21, S03, S12, S11, S02, S01, S20, S02, S11, S10, S01, S00};
+    const double nmq = norm_matrix(6, mq);
+    double c[6] = {Z20, Z02, Z11, Z10, Z01, Z00};
+    if (nmq == 0.0 || fabs(lu_det_solve(6, mq, c)) / nmq < thr_quad) return 0;
+    double m[4] = {2 * c[0], c[2], c[2], 2 * c[1]};
+    const double nm = norm_matrix(2, m);
+    double xy[2] = {-c[3], -c[4]};
+    if (nm == 0.0 || fabs(lu_det_solve(2, m, xy)) / nm < thr_quad) return 0;
+    out[0] = xy[0];
+    out[1] = xy[1];
+    out[2] = c[0] * xy[0] * xy[0] + c[1] * xy[1] * xy[1] + c[2] * xy[0] * xy[1] + c[3] * xy[0] + c[4] * xy[1] + c[5];
+    out[3] = c[0];
+    out[4] = c[1];
+    return 1;
+}
+
+/* C2:1509-1536 single2dPoly: corrLMA(run_poly_instead = true) of one correlation as a horizontal
+ * pair (C2:1903-1935), getMaxXYPoly (Correlations2dLMA:643-705) over its samples; out = {xcenter -
+ * x, -y, value} or {NaN, NaN, 0}. */
+static void single2d_poly(const double* corr, double xcenter, const clt_corr_lma_params* lp, double out[3]) {
+    static const clt_pair hor = {0, 1, 0, 1};
+    const char sel = 1;
+    out[0] = out[1] = NAN;
+    out[2] = 0.0;
+    lma_t* L = malloc(sizeof(lma_t));
+    lma_samples(L, (const double (*)[225])corr, &hor, 1, &sel, xcenter, lp);
+    double q[5];
+    if (quadratic_max(L->x, L->y, L->v, L->w, L->ns, 1.0e-30, q) && !(q[2] < 0.0) && !(q[3] >= 0.0) && !(q[4] >= 0.0)) {
+        out[0] = xcenter - q[0];
+        out[1] = -q[1];
+        out[2] = isnan(q[2]) ? 0.0 : q[2];
+    }
+    free(L);
+}
+
+/* C2:1538-1586 single2dCM: centre of mass of the positive values within radius of (round(xcenter), 0);
+ * out = {xcenter - xm, -ym, value at the rounded centre} */
+static void single2d_cm(const double* corr, double xcenter, double radius, double out[3]) {
+    const int width = 15, center = 7, center_index = (width + 1) * center;
+    const int ixcenter = (int)jround(xcenter);
+    const int iradius = (int)fabs(radius), ir2 = (int)(radius * radius), square = radius < 0;
+    double s0 = 0, sx = 0, sy = 0;
+    for (int y = -iradius; y <= iradius; y++) {
+        const int dy = y;
+        if (dy < -center || dy > center) continue;
+        for (int x = -iradius; x <= iradius; x++) {
+            const int dx = ixcenter + x;
+            const double r2 = y * y + x * x;
+            if (dx >= -center && dx <= center && (square || r2 <= ir2)) {
+                const double d = corr[dy * width + dx + center_index];
+                if (d > 0.0) {
+                    s0 += d;
+                    sx += d * dx;
+                    sy += d * dy;
+                }
+            }
+        }
+    }
+    const double xm = sx / s0, ym = sy / s0;
+    const int ixm = isnan(xm) ? 0 : (int)jround(xm), iym = isnan(ym) ? 0 : (int)jround(ym);   /* Math.round(NaN) = 0 */
+    out[0] = xcenter - xm;
+    out[1] = -ym;
+    out[2] = corr[iym * width + ixm + center_index];
+}
+
+typedef struct {
+    frame_t f;
+    const int32_t* tasks;
+    int ntasks;
+    clt_corr_params prm;
+    clt_corr_lma_params lp;   /* mix_corr_poly = use_poly, no foreground correction */
+    clt_rig_params rig;
+    double wndy[16], wndx[16], wndy_notch[32];
+    const double *corr_main, *disp_main, *corr_aux, *disp_aux, *inter;
+    const float* target;
+    const int32_t* tile_task;
+    double* bimap;
+} rig_ctx;
+
+/* ID:7370-7560 tileInterCamCorrs on one inter-camera correlation: the single correlation as a
+ * horizontal pair's strip, the integer maximum, the centre of mass (or its notch form), the LMA
+ * (C2:1903-1935) and dx, dy; res = {disparity, strength, dx, dy} */
+static void inter_cam_corrs(const rig_ctx* c, const double* corr, int no_int_x0, double res[4]) {
+    const int hw = c->prm.strip_height;
+    res[0] = NAN;
+    res[1] = 0.0;
+    res[2] = res[3] = NAN;
+    real rc[225], strip[16 * 15], combo[16 * 15];
+    for (int i = 0; i < 225; i++) rc[i] = (real)corr[i];
+    strip_single(rc, hw, 0, 1, strip);
+    real* strips[1] = {strip};
+    combine_strips(strips, 1, hw * 15, (real)c->prm.corr_offset, c->prm.twice_diagonal, combo);
+    int ix;
+    if (!max_xy_int_axis(combo, (real)((no_int_x0 ? 0.5 : 1.0) * c->prm.min_corr), &ix)) return;
+    res[0] = -ix;
+    res[1] = combo[ix + 7];
+    real cs[3];
+    const int notch = c->rig.notch_mode != 0;
+    const int ok = notch ? max_x_cm(combo, hw, 0, c->wndy_notch, c->rig.corr_strip_notch, c->wndx, 9, cs)
+                         : max_x_cm(combo, hw, no_int_x0 ? 0 : ix, c->wndy, 2, c->wndx, 9, cs);
+    if (!ok) return;
+    res[0] = -cs[0];
+    const double eff_radius = cs[2] * c->rig.cm_max_normalization;
+    const double strength = cs[1] / (eff_radius * eff_radius);
+    res[1] = strength;
+    if (!(strength > c->lp.min_poly_strength) || notch) return;
+    static const clt_pair hor = {0, 1, 0, 1};
+    const char sel = 1;
+    double r[4];
+    if (!corr_lma((const double (*)[225])corr, &hor, 1, &sel, cs[0], &c->lp, r)) return;
+    if (c->rig.use_poly) {
+        res[0] = r[0];
+        res[1] = r[1];
+    }
+    double xy[3];
+    if (c->rig.use_xy_poly) single2d_poly(corr, 0.0, &c->lp, xy);
+    else single2d_cm(corr, 0.0, c->rig.max_corr_radius, xy);
+    res[2] = xy[0];
+    res[3] = xy[1];
+}
+
+/* ID:7581-7740 tileCorrs of one quad from its pair correlations and its correlate2D record
+ * (CM_RAW = the centre of mass, STR_RAW = the strength at the integer maximum): the LMA over all
+ * pairs (replacing disparity/strength when use_poly) and the four one-direction LMA fits;
+ * res = {disparity, hor, vert, diag main, diag other, strength, ... the same strengths} */
+static void tile_corrs(const rig_ctx* c, const double (*corrs)[225], const double* rec, double res[10]) {
+    static const clt_pair quad[6] = {{0, 1, 0, 1}, {2, 3, 0, 1}, {0, 2, 1, 1}, {1, 3, 1, 1}, {0, 3, 2, 1}, {2, 1, 3, 1}};
+    for (int i = 0; i < 5; i++) res[i] = NAN;
+    for (int i = 5; i < 10; i++) res[i] = 0.0;
+    const double cm = rec[CLT_DISP_CM_RAW], str = rec[CLT_DISP_STR_RAW];
+    res[0] = isnan(cm) ? rec[CLT_DISP_INT] : cm;   /* -ixy when the centre of mass has no weight */
+    res[5] = str;
+    if (isnan(cm) || !(str > c->lp.min_poly_strength)) return;
+    char sel[6] = {1, 1, 1, 1, 1, 1};
+    double r[4];
+    if (!corr_lma(corrs, quad, 6, sel, -cm, &c->lp, r)) return;
+    double disparity = cm;
+    if (c->rig.use_poly) {
+        disparity = r[0];
+        res[0] = r[0];
+        res[5] = r[1];
+    }
+    for (int dir = 0; dir < 4; dir++) {   /* C2:1591-1630 corr4dirsLMA */
+        for (int p = 0; p < 6; p++) sel[p] = quad[p].dir == dir;
+        double dd[4];
+        if (corr_lma(corrs, quad, 6, sel, -disparity, &c->lp, dd)) {
+            res[1 + dir] = dd[0];
+            res[6 + dir] = dd[1];
+        }
+    }
+}
+
+static const int SNGL_TO_BI[2][10] = {   /* ID:190-194 */
+    {CLT_BI_DISP_FULL, CLT_BI_DISP_HOR, CLT_BI_DISP_VERT, CLT_BI_DISP_DIAGM, CLT_BI_DISP_DIAGO, CLT_BI_STR_FULL,
+     CLT_BI_STR_HOR, CLT_BI_STR_VERT, CLT_BI_STR_DIAGM, CLT_BI_STR_DIAGO},
+    {CLT_BI_ADISP_FULL, CLT_BI_ADISP_HOR, CLT_BI_ADISP_VERT, CLT_BI_ADISP_DIAGM, CLT_BI_ADISP_DIAGO, CLT_BI_ASTR_FULL,
+     CLT_BI_ASTR_HOR, CLT_BI_ASTR_VERT, CLT_BI_ASTR_DIAGM, CLT_BI_ASTR_DIAGO}};
+
+static void put_inter(double* b, const double r[4]) {   /* ID:9238-9248 */
+    b[CLT_BI_DISP_CROSS] = r[0];
+    b[CLT_BI_STR_CROSS] = r[1];
+    b[CLT_BI_DISP_CROSS_DX] = r[2];
+    b[CLT_BI_DISP_CROSS_DY] = r[3];
+    b[CLT_BI_STR_ALL] = pow(r[1] * b[CLT_BI_STR_FULL] * b[CLT_BI_ASTR_FULL], 1.0 / 3);
+}
+
+/* ID:8946-9248 for one task: both quads' tileCorrs, the inter-camera result, the record */
+static void rig_task(void* vctx, int t) {
+    rig_ctx* c = (rig_ctx*)vctx;
+    double* b = c->bimap + (size_t)t * CLT_BI_LAYERS;
+    for (int i = 0; i < CLT_BI_LAYERS; i++) b[i] = 0.0;   /* the reference's zeroed arrays */
+    if (task_flags(c->f.g, c->tasks, t) == 0) {
+        b[CLT_BI_TARGET] = NAN;
+        return;
+    }
+    b[CLT_BI_TARGET] = c->target ? c->target[t] : NAN;
+    for (int q = 0; q < 2; q++) {
+        const double* corrs = (q ? c->corr_aux : c->corr_main) + (size_t)t * 6 * 225;
+        const double* rec = (q ? c->disp_aux : c->disp_main) + (size_t)t * CLT_DISP_LAYERS;
+        double res[10];
+        tile_corrs(c, (const double (*)[225])corrs, rec, res);
+        for (int i = 0; i < 10; i++) b[SNGL_TO_BI[q][i]] = res[i];
+    }
+    double r[4];
+    inter_cam_corrs(c, c->inter + (size_t)t * 225, c->rig.no_int_x0, r);
+    put_inter(b, r);
+}
+
+/* ID:9404-9480: the inter-camera correlations averaged over the (2 lt_rad + 1)^2 neighbours
+ * (vertical neighbours only in notch mode) with cos weights, then tileInterCamCorrs (no_int_x0) */
+static void rig_lt_task(void* vctx, int t) {
+    rig_ctx* c = (rig_ctx*)vctx;
+    if (task_flags(c->f.g, c->tasks, t) == 0) return;
+    int tx, ty;
+    task_txy(c->f.g, c->tasks, t, &tx, &ty);
+    const int lr = c->rig.lt_rad, lrx = c->rig.notch_mode ? 0 : lr;
+    double avg[225], sw = 0.0;
+    for (int i = 0; i < 225; i++) avg[i] = 0.0;
+    for (int dy = -lr; dy <= lr; dy++)
+        for (int dx = -lrx; dx <= lrx; dx++) {
+            const int x = tx + dx, y = ty + dy;
+            if (x < 0 || y < 0 || x >= c->f.tiles_x || y >= c->f.tiles_y) continue;
+            const int o = c->tile_task[y * c->f.tiles_x + x];
+            if (o < 0) continue;
+            const double nw = cos(M_PI * abs(dy) / (2 * lr + 1)) * cos(M_PI * abs(dx) / (2 * lr + 1));
+            const double* oc = c->inter + (size_t)o * 225;
+            for (int i = 0; i < 225; i++) avg[i] += nw * oc[i];
+            sw += nw;
+        }
+    if (!(sw > 0.0)) return;
+    const double s = 1.0 / sw;
+    for (int i = 0; i < 225; i++) avg[i] *= s;
+    double r[4];
+    inter_cam_corrs(c, avg, 1, r);
+    put_inter(c->bimap + (size_t)t * CLT_BI_LAYERS, r);
+}
+
+/* The rig analysis from the pair correlations and correlate2D records of both quads
+ * ([T][6][225], [T][CLT_DISP_LAYERS]) and the inter-camera correlations [T][225] -> bimap
+ * [T][CLT_BI_LAYERS].  g: the frame (8 cameras); target: [T] or NULL. */
+EXPORT int orc_rig_analysis(const clt_geom* g, const int32_t* tasks, int ntasks, const clt_corr_params* prm,
+                            const clt_corr_lma_params* lma, const clt_rig_params* rig, const double* corr_main,
+                            const double* disp_main, const double* corr_aux, const double* disp_aux,
+                            const double* inter, const float* target, double* bimap, int nthreads) {
+    if (g->num_cams != 8 || prm->strip_height > 16 || rig->corr_strip_notch > 32 || rig->corr_strip_notch < 1 ||
+        lma->cnvx_hwnd_size < 1 || lma->cnvx_hwnd_size > LMA_HWIN_MAX)
+        return CLT_E_ARG;
+    rig_ctx* c = calloc(1, sizeof(rig_ctx));
+    frame_init(&c->f, g);
+    c->tasks = tasks;
+    c->ntasks = ntasks;
+    c->prm = *prm;
+    c->lp = *lma;
+    c->lp.mix_corr_poly = rig->use_poly;
+    c->lp.fo_correct = 0;
+    c->rig = *rig;
+    half_flat_top_window(2, 1.0, 1.0, 1, 0, 1.0, c->wndy);   /* C2:139-159 with IDP defaults */
+    half_flat_top_window(9, 6.0, 5.0, 1, 0, 2.0, c->wndx);
+    half_flat_top_window(rig->corr_strip_notch, rig->corr_notch_hwidth, rig->corr_notch_blur, 1, 1, 2.0, c->wndy_notch);
+    c->corr_main = corr_main;
+    c->disp_main = disp_main;
+    c->corr_aux = corr_aux;
+    c->disp_aux = disp_aux;
+    c->inter = inter;
+    c->target = target;
+    c->bimap = bimap;
+    run_pool(ntasks, nthreads, rig_task, c);
+    if (rig->lt_rad > 0) {
+        int32_t* tt = malloc(sizeof(int32_t) * c->f.num_tiles);
+        for (int i = 0; i < c->f.num_tiles; i++) tt[i] = -1;
+        for (int t = 0; t < ntasks; t++)
+            if (task_flags(g, tasks, t) != 0) {
+                int tx, ty;
+                task_txy(g, tasks, t, &tx, &ty);
+                tt[ty * c->f.tiles_x + tx] = t;
+            }
+        c->tile_task = tt;
+        run_pool(ntasks, nthreads, rig_lt_task, c);
+        free(tt);
+    }
+    free(c);
+    return 0;
+}
+
+typedef struct {
+    frame_t f;
+    const double* clt;
+    const int32_t* tasks;
+    real lpf[64], col_weights[3];
+    real fat_zero;
+    double* inter;
+} inter_ctx;
+
+/* C2:401-448 correlateInterCamerasFD: the mean CLT of each quad (cltMixCameras), then
+ * correlateCompositeFD of the two means */
+static void inter_task(void* vctx, int t) {
+    inter_ctx* c = (inter_ctx*)vctx;
+    double* o = c->inter + (size_t)t * 225;
+    if (task_flags(c->f.g, c->tasks, t) == 0) {
+        for (int i = 0; i < 225; i++) o[i] = NAN;
+        return;
+    }
+    int tx, ty;
+    task_txy(c->f.g, c->tasks, t, &tx, &ty);
+    const size_t tile = (size_t)ty * c->f.tiles_x + tx;
+    const int K = c->f.g->num_colors;
+    real mix[2][3][4][64], blk[4][64];
+    for (int q = 0; q < 2; q++)
+        for (int col = 0; col < K; col++) {
+            double acc[4][64] = {{0}};
+            for (int cam = 0; cam < 4; cam++) {
+                load_clt(&c->f, c->clt, 4 * q + cam, tile, col, blk);
+                for (int n = 0; n < 4; n++)
+                    for (int i = 0; i < 64; i++) acc[n][i] += blk[n][i];
+            }
+            for (int n = 0; n < 4; n++)
+                for (int i = 0; i < 64; i++) mix[q][col][n][i] = (real)(acc[n][i] * 0.25);
+        }
+    real out[225];
+    correlate_composite_fd(K, mix[0], mix[1], c->lpf, c->col_weights, c->fat_zero, out);
+    for (int i = 0; i < 225; i++) o[i] = out[i];
+}
+
+/* clt_bi_quad (ID:8731-9504) correlation part over a converted 8-camera frame: both quads'
+ * correlate2D (cameras 0..3 and 4..7, the quad pair table), the inter-camera correlations and the
+ * rig analysis.  Outputs may not be NULL: corr_main/aux [T][6][225], disp_main/aux
+ * [T][CLT_DISP_LAYERS], inter [T][225], bimap [T][CLT_BI_LAYERS]. */
+EXPORT int orc_rig_frame(const clt_geom* g, const double* clt, const int32_t* tasks, int ntasks,
+                         const clt_corr_params* prm, const clt_corr_lma_params* lma, const clt_rig_params* rig,
+                         double corr_sigma, const float* target, double* corr_main, double* disp_main,
+                         double* corr_aux, double* disp_aux, double* inter, double* bimap, int nthreads) {
+    tables();
+    if (g->num_cams != 8 || g->num_colors != 3) return CLT_E_ARG;
+    const clt_pair qp[2][6] = {{{0, 1, 0, 1}, {2, 3, 0, 1}, {0, 2, 1, 1}, {1, 3, 1, 1}, {0, 3, 2, 1}, {2, 1, 3, 1}},
+                               {{4, 5, 0, 1}, {6, 7, 0, 1}, {4, 6, 1, 1}, {5, 7, 1, 1}, {4, 7, 2, 1}, {6, 5, 3, 1}}};
+    int rc = orc_corr_frame(g, clt, tasks, ntasks, qp[0], 6, prm, NULL, corr_sigma, corr_main, NULL, disp_main, nthreads);
+    if (!rc) rc = orc_corr_frame(g, clt, tasks, ntasks, qp[1], 6, prm, NULL, corr_sigma, corr_aux, NULL, disp_aux, nthreads);
+    if (rc) return rc;
+    inter_ctx* ic = calloc(1, sizeof(inter_ctx));
+    frame_init(&ic->f, g);
+    ic->clt = clt;
+    ic->tasks = tasks;
+    make_lpf(corr_sigma, ic->lpf);
+    color_weights(g, prm->corr_red, prm->corr_blue, ic->col_weights);
+    ic->fat_zero = (real)prm->fat_zero;
+    ic->inter = inter;
+    run_pool(ntasks, nthreads, inter_task, ic);
+    free(ic);
+    return orc_rig_analysis(g, tasks, ntasks, prm, lma, rig, corr_main, disp_main, corr_aux, disp_aux, inter, target,
+                            bimap, nthreads);
+}
+
+/* single checks for the tests */
+EXPORT int orc_quadratic_max(const double* x, const double* y, const double* z, const double* w, int n, double* out5) {
+    return quadratic_max(x, y, z, w, n, 1.0e-30, out5);
+}
+EXPORT void orc_single2d_cm(const double* corr, double xcenter, double radius, double* out3) {
+    single2d_cm(corr, xcenter, radius, out3);
+}
+EXPORT void orc_single2d_poly(const double* corr, double xcenter, const clt_corr_lma_params* lp, double* out3) {
+    single2d_poly(corr, xcenter, lp, out3);
 }
 
 /* ================= CLT kernel preparation (QuadCLT.calculateCLTKernel, QC:433-668) ================= */

@@ -1,0 +1,246 @@
+"""Two-quad rig (ImageDtt.clt_bi_quad ID:8731-9504: tileCorrs ID:7581-7740 of each quad,
+correlateInterCamerasFD C2:401-448, tileInterCamCorrs ID:7370-7560, the low-texture pass
+ID:9404-9480).
+
+Oracle pins (CPU):
+* quadraticMaxV2dX2Y2XY (PA:201-224 over quadraticApproximation PA:350-580): an exact quadratic
+  surface gives its stationary point (numpy solve of the 2x2 gradient system); degenerate sample sets
+  (collinear samples) give none;
+* single2dCM (C2:1538-1586): a symmetric positive blob gives its centre exactly; negatives ignored;
+* single2dPoly (C2:1509-1536): a sampled paraboloid gives its apex exactly, a minimum none;
+* the inter-camera correlation of two quads whose four ports are identical copies of cameras A and B
+  equals the pair correlation (A, B) of correlate2D, and its centre of mass / LMA disparity equal
+  that pair's (so the inter path inherits the pair-sign pins of the quad tests);
+* on synthetic rigs with a known residual disparity r, the inter disparity is r times the rig
+  baseline (2 quad baselines here) within the centre-of-mass bias, with the sign of r;
+* the low-texture pass equals the analysis of the cos-weighted neighbour averages;
+* the notch form ignores the strip's axis row; skipped tasks give zeros and a NaN target.
+GPU parity: the device rig against the oracle's analysis of the device's own correlations (1e-4,
+same discrete decisions) and against the oracle end to end (at most 1 % of the tiles off by more
+than 0.01 px).
+"""
+import numpy as np
+import pytest
+
+from helpers import gpu_processor, log, oracle_geom
+
+RIG_OFF = np.array([2.0, 0.0])   # aux quad centre, in main-quad disparity units
+AUX_SCALE = 0.75                 # aux baseline / main baseline
+
+
+def rig_rxy(pk):
+    q = pk.geometry.QUAD_RXY
+    return np.concatenate([q, RIG_OFF + AUX_SCALE * q])
+
+
+def rig_scene(pk, w, h, seed, **kw):
+    return pk.synth.make_scene(w, h, num_cams=8, rxy=rig_rxy(pk), seed=seed, **kw)
+
+
+# ------------------------------------------------------------------------------------- oracle pins
+def test_quadratic_max_closed_form(orc):
+    rng = np.random.default_rng(0)
+    x, y = rng.uniform(-3, 3, 50), rng.uniform(-2, 2, 50)
+    A, B, C, D, E, F = -0.5, -0.3, 0.1, 0.4, -0.2, 2.0
+    z = A * x * x + B * y * y + C * x * y + D * x + E * y + F
+    w = rng.uniform(0.1, 2.0, 50)
+    q = orc.quadratic_max(x, y, z, w)
+    xy = np.linalg.solve([[2 * A, C], [C, 2 * B]], [-D, -E])
+    v = A * xy[0] ** 2 + B * xy[1] ** 2 + C * xy[0] * xy[1] + D * xy[0] + E * xy[1] + F
+    assert np.allclose(q, [xy[0], xy[1], v, A, B], atol=1e-10)
+    assert orc.quadratic_max(x, np.zeros(50), z, w) is None              # collinear samples
+    assert orc.quadratic_max(x, 0.5 * x + 1, z, w) is None
+    w0 = w.copy()
+    w0[:] = 0
+    assert orc.quadratic_max(x, y, z, w0) is None                        # no weight
+
+
+def test_single2d_cm_blob(orc):
+    c = -0.05 * np.ones((15, 15))
+    for (yy, xx), v in (((-1, 1), 1.0), ((-1, 0), 0.4), ((-1, 2), 0.4), ((0, 1), 0.3), ((-2, 1), 0.3)):
+        c[7 + yy, 7 + xx] = v
+    out = orc.single2d_cm(c, 0.0, 3.9)
+    assert np.allclose(out, [-1.0, 1.0, 1.0], atol=1e-12)
+
+
+@pytest.mark.parametrize("x0,y0", [(0.3, -0.4), (-1.2, 0.7), (0.0, 0.0)])
+def test_single2d_poly_paraboloid(orc, x0, y0):
+    yy, xx = np.mgrid[-7:8, -7:8].astype(np.float64)
+    c = 2.0 - 0.11 * (xx - x0) ** 2 - 0.07 * (yy - y0) ** 2 - 0.02 * (xx - x0) * (yy - y0)
+    out = orc.single2d_poly(c, 0.0)
+    assert np.allclose(out[:2], [-x0, -y0], atol=1e-9) and abs(out[2] - 2.0) < 1e-9
+    out = orc.single2d_poly(-c, 0.0)                                      # a minimum: no maximum
+    assert np.isnan(out[0]) and np.isnan(out[1]) and out[2] == 0.0
+
+
+def _copies_clt(orc, pk, seed):
+    """An 8-camera CLT whose main ports are all camera A of a horizontal pair and whose aux
+    ports are all camera B, plus the pair's own correlate2D."""
+    sc = pk.synth.make_scene(64, 48, num_cams=2, rxy=np.array([[-0.5, 0.0], [0.5, 0.0]]), seed=seed,
+                             disparity=("field", 0.5, 2.5), target_disparity=0.0)
+    g2 = oracle_geom(orc, sc)
+    clt2 = orc.convert_frame(g2, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+    clt8 = np.concatenate([np.repeat(clt2[:1], 4, axis=0), np.repeat(clt2[1:], 4, axis=0)])
+    tasks8 = np.zeros((len(sc.tasks), 2 + 16), np.int32)
+    tasks8[:, :2] = sc.tasks[:, :2]
+    return sc, g2, clt2, clt8, tasks8
+
+
+def test_inter_is_the_pair_correlation_of_the_means(orc, pk):
+    sc, g2, clt2, clt8, tasks8 = _copies_clt(orc, pk, 3)
+    g8 = orc.make_geom(64, 48, 8, 3, kernels=False)
+    lp = orc.default_corr_lma_params(fo_correct=0)
+    r = orc.rig_frame(g8, clt8, tasks8, rig=orc.default_rig_params(use_poly=0, no_int_x0=0), lma=lp)
+    corr, _, disp = orc.corr_frame(g2, clt2, sc.tasks, pairs=[(0, 1, 0, 1)], lma=lp)
+    assert np.abs(r["inter"] - corr[:, 0]).max() < 1e-12 * np.abs(corr).max()
+    cm = disp[:, orc.DISP_CM_RAW]
+    fin = np.isfinite(cm)
+    assert fin.mean() > 0.9
+    assert np.abs(r["bimap"][fin, orc.BI_DISP_CROSS] - cm[fin]).max() < 1e-9
+    r2 = orc.rig_frame(g8, clt8, tasks8, rig=orc.default_rig_params(no_int_x0=0), lma=lp)
+    both = np.isfinite(disp[:, orc.DISP_POLY]) & np.isfinite(r2["bimap"][:, orc.BI_DISP_CROSS_DX])
+    assert both.sum() > 0.5 * len(cm)
+    assert np.abs(r2["bimap"][both, orc.BI_DISP_CROSS] - disp[both, orc.DISP_POLY]).max() < 1e-9
+
+
+def test_tile_corrs_is_the_lma_chain_without_foreground(orc, pk):
+    sc = rig_scene(pk, 64, 48, 4, disparity=("field", 1.0, 3.0), target_disparity=1.5)
+    g8 = orc.make_geom(64, 48, 8, 3)
+    clt = orc.convert_frame(g8, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+    r = orc.rig_frame(g8, clt, sc.tasks)
+    g4 = orc.make_geom(64, 48, 4, 3)
+    for q, (bd, bs) in enumerate(((orc.BI_DISP_FULL, orc.BI_STR_FULL), (orc.BI_ADISP_FULL, orc.BI_ASTR_FULL))):
+        tasks4 = np.concatenate([sc.tasks[:, :2], sc.tasks[:, 2 + 8 * q:10 + 8 * q]], axis=1)
+        _, _, d = orc.corr_frame(g4, clt[4 * q:4 * q + 4], tasks4, lma=orc.default_corr_lma_params(fo_correct=0))
+        assert np.array_equal(np.isnan(d[:, orc.DISP_CM]), np.isnan(r["bimap"][:, bd]))
+        f = np.isfinite(d[:, orc.DISP_CM])
+        assert np.abs(d[f, orc.DISP_CM] - r["bimap"][f, bd]).max() < 1e-12
+        assert np.abs(d[:, orc.DISP_STRENGTH] - r["bimap"][:, bs]).max() < 1e-12
+    # one-direction fits exist exactly where the fit over all pairs succeeded
+    fitted = np.isfinite(r["bimap"][:, orc.BI_DISP_HOR]) | np.isfinite(r["bimap"][:, orc.BI_DISP_VERT])
+    assert fitted.mean() > 0.5
+    assert (r["bimap"][~np.isfinite(r["bimap"][:, orc.BI_DISP_HOR]), orc.BI_STR_HOR] == 0).all()
+    b = r["bimap"]
+    assert np.allclose(b[:, orc.BI_STR_ALL], np.cbrt(b[:, orc.BI_STR_CROSS] * b[:, orc.BI_STR_FULL] *
+                                                       b[:, orc.BI_ASTR_FULL]), equal_nan=True)
+
+
+@pytest.mark.parametrize("residual", [0.5, -0.5])
+def test_rig_residual_disparity(orc, pk, residual):
+    sc = rig_scene(pk, 128, 96, 5, disparity=3.0, target_disparity=3.0 - residual)
+    g8 = orc.make_geom(128, 96, 8, 3)
+    clt = orc.convert_frame(g8, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+    b = orc.rig_frame(g8, clt, sc.tasks)["bimap"]
+    cross = np.nanmedian(b[:, orc.BI_DISP_CROSS])
+    main = np.nanmedian(b[:, orc.BI_DISP_FULL])
+    log("  residual %.2f: main %.3f, aux %.3f, cross %.3f (expected %.2f)" %
+        (residual, main, np.nanmedian(b[:, orc.BI_ADISP_FULL]), cross, residual * RIG_OFF[0]))
+    assert np.sign(main) == np.sign(residual) and np.sign(cross) == np.sign(residual)
+    assert abs(cross - residual * RIG_OFF[0]) < 0.3 * abs(residual * RIG_OFF[0])
+    assert abs(np.nanmedian(b[:, orc.BI_DISP_CROSS_DX]) - residual * RIG_OFF[0]) < 0.3 * abs(residual * RIG_OFF[0])
+
+
+def test_low_texture_pass(orc, pk):
+    sc = rig_scene(pk, 64, 48, 6, disparity=("field", 1.0, 3.0), target_disparity=2.0, task_fraction=0.8)
+    g8 = orc.make_geom(64, 48, 8, 3)
+    clt = orc.convert_frame(g8, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+    r = orc.rig_frame(g8, clt, sc.tasks)
+    lt = orc.rig_analysis(g8, sc.tasks, r["corr_main"], r["disp_main"], r["corr_aux"], r["disp_aux"], r["inter"],
+                          rig=orc.default_rig_params(lt_rad=1))
+    # the cos-weighted average of the neighbours' inter correlations (ID:9404-9454)
+    tx, ty = sc.tasks[:, 1] & 0xFFFF, sc.tasks[:, 1] >> 16
+    where = {(x, y): i for i, (x, y) in enumerate(zip(tx, ty))}
+    avg = np.zeros_like(r["inter"])
+    for i, (x, y) in enumerate(zip(tx, ty)):
+        acc, sw = 0.0, 0.0
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                j = where.get((x + dx, y + dy))
+                if j is None:
+                    continue
+                wgt = np.cos(np.pi * abs(dy) / 3) * np.cos(np.pi * abs(dx) / 3)
+                acc = acc + wgt * r["inter"][j]
+                sw += wgt
+        avg[i] = acc / sw
+    ref = orc.rig_analysis(g8, sc.tasks, r["corr_main"], r["disp_main"], r["corr_aux"], r["disp_aux"], avg,
+                           rig=orc.default_rig_params(no_int_x0=1))
+    for k in (orc.BI_DISP_CROSS, orc.BI_STR_CROSS, orc.BI_DISP_CROSS_DX, orc.BI_DISP_CROSS_DY, orc.BI_STR_ALL):
+        assert np.allclose(lt[:, k], ref[:, k], atol=1e-9, equal_nan=True), k
+
+
+def test_notch_ignores_the_axis_row_and_skipped_tasks(orc, pk):
+    sc = rig_scene(pk, 32, 24, 7, disparity=2.0)
+    g8 = orc.make_geom(32, 24, 8, 3)
+    T = len(sc.tasks)
+    inter = np.zeros((T, 15, 15))
+    inter[:, 7, :] = np.exp(-0.5 * ((np.arange(15) - 9) / 1.0) ** 2)   # the axis row only, maximum at x = +2
+    cm = np.zeros((T, 6, 15, 15))
+    d = np.full((T, orc.DISP_LAYERS), np.nan)
+    d[:, orc.DISP_STR_RAW] = 0.0
+    tasks = sc.tasks.copy()
+    tasks[0, 0] = 0
+    b = orc.rig_analysis(g8, tasks, cm, d, cm, d, inter, rig=orc.default_rig_params(notch_mode=1))
+    assert np.allclose(b[1:, orc.BI_DISP_CROSS], -2.0) and np.allclose(b[1:, orc.BI_STR_CROSS], 1.0)
+    assert np.isnan(b[1:, orc.BI_DISP_CROSS_DX]).all()
+    assert np.isnan(b[0, orc.BI_TARGET]) and (b[0, :orc.BI_TARGET] == 0).all()
+    b = orc.rig_analysis(g8, tasks, cm, d, cm, d, inter, rig=orc.default_rig_params(no_int_x0=0))
+    assert np.allclose(b[1:, orc.BI_DISP_CROSS], -2.0, atol=1e-9)   # symmetric row: the centre of mass is on +2
+
+
+# ------------------------------------------------------------------------------------- GPU
+LAYERS = {"disp_full": 0, "disp_hor": 1, "disp_vert": 2, "adisp_full": 5, "cross": 10, "cross_dx": 11, "cross_dy": 12,
+          "str_full": 13, "astr_full": 18, "str_cross": 23, "str_all": 24}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,rig,fz", [("defaults", {}, 0.0),
+                                         ("lt_rad 1, CM dx/dy", dict(lt_rad=1, use_xy_poly=0), 0.0),
+                                         ("notch", dict(notch_mode=1), 0.0),
+                                         ("fat zero 0.05, no_int_x0 off", dict(no_int_x0=0), 0.05)])
+def test_gpu_rig_parity(pk, orc, name, rig, fz):
+    log("case: two-quad rig 128x96, %s" % name)
+    sc = rig_scene(pk, 128, 96, 8, disparity=("field", 1.0, 4.0), target_disparity=2.0)
+    tp = gpu_processor(pk, sc)
+    tp.setCorrSigma(0.8)
+    if fz:
+        tp.setCorrParams(fat_zero=fz)
+    if rig:
+        tp.setRigParams(**rig)
+    tp.execConverCorrectTiles()
+    tp.execCorrelate2DRig(np.full(len(sc.tasks), 2.0, np.float32))
+    g = tp.getCorr2DRig()
+    tp.close()
+    rp = orc.default_rig_params(**rig)
+    prm = orc.default_corr_params(fat_zero=fz)
+    g8 = oracle_geom(orc, sc)
+    same = orc.rig_analysis(g8, sc.tasks, g["corr_main"], g["disp_main"], g["corr_aux"], g["disp_aux"], g["inter"],
+                            target=np.full(len(sc.tasks), 2.0, np.float32), params=prm, rig=rp)
+    gb = g["bimap"].astype(np.float64)
+    for nm, k in LAYERS.items():
+        fo, fg = np.isfinite(same[:, k]), np.isfinite(gb[:, k])
+        both = fo & fg
+        d = float(np.abs(gb[both, k] - same[both, k]).max()) if both.any() else 0.0
+        log("  same input %-10s nan mismatch %d, max|d| %.3g" % (nm, int((fo != fg).sum()), d))
+        assert (fo != fg).sum() == 0 and d < 1e-4, nm
+    clt = orc.convert_frame(g8, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+    e = orc.rig_frame(g8, clt, sc.tasks, params=prm, rig=rp)["bimap"]
+    for nm in ("disp_full", "adisp_full", "cross", "cross_dx"):
+        k = LAYERS[nm]
+        if nm == "cross_dx" and rig.get("notch_mode"):   # the notch form runs no LMA: no dx, dy
+            assert np.isnan(e[:, k]).all() and np.isnan(gb[:, k]).all()
+            continue
+        both = np.isfinite(e[:, k]) & np.isfinite(gb[:, k])
+        off = float((np.abs(gb[both, k] - e[both, k]) > 0.01).mean())
+        med = float(np.median(np.abs(gb[both, k] - e[both, k]))) if both.any() else 0.0
+        log("  end to end %-10s %d both, %.4f off by > 0.01 px, median %.2g" % (nm, both.sum(), off, med))
+        assert both.sum() > 0.5 * len(sc.tasks) and off <= 0.01, nm
+
+
+@pytest.mark.gpu
+def test_gpu_rig_rejects_other_camera_counts(pk):
+    sc = pk.synth.make_scene(64, 48, seed=2)
+    tp = gpu_processor(pk, sc)
+    tp.execConverCorrectTiles()
+    with pytest.raises(pk.CltError):
+        tp.execCorrelate2DRig()
+    tp.close()
