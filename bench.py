@@ -299,6 +299,57 @@ def kernel_prep_bench(pk, seed=13, cams=4):
             "cpu_port_ms_per_camera": round(1e3 * cpu, 1), "cpu_cores": 1}
 
 
+def rig_bench(pk, seed=17, steps=5, cpu_size=(512, 384)):
+    """Two-quad rig (the correlation part of clt_bi_quad) on an 8-camera 2592x1936 frame: main
+    quad rXY as the quad, the auxiliary quad at 2 baselines to the right with 0.75 of the baseline;
+    per step convert (8 cameras) and clt_exec_corr2d_rig (both quads' correlate2D and tileCorrs
+    LMA, the inter-camera correlation, its LMA and 2-D polynomial).  The FP64 oracle's rig stage is
+    timed on all host cores on a smaller frame of the same kind."""
+    import oracle as orc
+    q = pk.geometry.QUAD_RXY
+    rxy = np.concatenate([q, np.array([2.0, 0.0]) + 0.75 * q])
+    sc = pk.synth.make_scene(W, H, num_cams=8, rxy=rxy, seed=seed, disparity=("field", 0.0, 10.0))
+    tx, ty = sc.tasks[:, 1] & 0xFFFF, sc.tasks[:, 1] >> 16
+    target = sc.disparity[ty, tx].astype(np.float32)
+    tp = pk.GPUTileProcessor(W, H, 8, K)
+    tp.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
+    tp.setBayerImages(sc.images, force=True)
+    tp.setTasks(sc.tasks)
+    tp.setCorrSigma(0.8)
+    tp.execConverCorrectTiles(0)
+    tp.execCorrelate2DRig(target)
+    tp.setTiming(True)
+    tp.resetTiming()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        tp.execConverCorrectTiles(0)
+        tp.execCorrelate2DRig(target)
+    wall = (time.perf_counter() - t0) / steps
+    rig_ms = tp.getTiming("rig")[0] / steps
+    conv_ms = tp.getTiming("convert")[0] / steps
+    b = tp.getCorr2DRig()["bimap"]
+    tp.close()
+    T = len(sc.tasks)
+    cw, ch = cpu_size
+    cs = pk.synth.make_scene(cw, ch, num_cams=8, rxy=rxy, seed=seed, disparity=("field", 0.0, 10.0))
+    g = orc.make_geom(cw, ch, 8, K)
+    clt = orc.convert_frame(g, cs.images, cs.tasks, kernels=cs.kernels, offsets=cs.offsets)
+    cores = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        orc.rig_frame(g, clt, cs.tasks, nthreads=cores)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= 3.0 or n >= 20:
+            break
+    return {"tasks": T, "cameras": 8, "ms_per_step": round(1e3 * wall, 3), "rig_ms": round(rig_ms, 4),
+            "convert_ms": round(conv_ms, 4), "rig_tiles_per_s": round(T / (rig_ms * 1e-3), 1),
+            "cross_fitted": int(np.isfinite(b[:, 11]).sum()),
+            "cpu_port_rig_tiles_per_s": round(n * len(cs.tasks) / dt, 1), "cpu_cores": cores,
+            "cpu_sample": "%dx%d 8-camera frame, %d tiles, the rig stage only, %d pass(es)" % (cw, ch, len(cs.tasks), n)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -312,6 +363,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-frames comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/LMA timing")
     ap.add_argument("--no-kprep", action="store_true", help="skip the kernel preparation timing")
+    ap.add_argument("--no-rig", action="store_true", help="skip the two-quad rig timing")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -406,6 +458,10 @@ def main():
     if not args.no_kprep:
         print("bench: kernel preparation ...", file=sys.stderr, flush=True)
         kprep = kernel_prep_bench(pk)
+    rig = None
+    if not args.no_rig:
+        print("bench: two-quad rig ...", file=sys.stderr, flush=True)
+        rig = rig_bench(pk)
     if traffic is None:
         roof["traffic_note"] = traffic_info
     else:
@@ -426,7 +482,7 @@ def main():
                                "convert+correlate2D(6 pairs, per-tile LMA chain)+textures(RGBA)+imclt_rbg",
                    "tiles": T, "cameras": C, "colors": K, "pairs": 6},
         "roofline": roof, "kernels": kernels, "kernels_isolated": isolated, "cpu_baseline": cpu, "batch": batch, "adjoint": adjoint,
-        "kernel_prep": kprep,
+        "kernel_prep": kprep, "rig": rig,
     }
     print(json.dumps(out))
 

@@ -1343,6 +1343,8 @@ int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
     const bool fz = h->cp.fat_zero != 0.f;
     const int nc = h->lp.lma_adjust_wy ? 4 : 3;
     tbegin(h, T_RIG);
+    const dim3 g4((n + 3) / 4), g_lma((n + LMA_SLOTS - 1) / LMA_SLOTS);
+    auto lma = (nc == 4) ? k_corr_lma<4> : k_corr_lma<3>;
     for (int q = 0; q < 2; q++) {   // each quad's correlate2D, with its mean CLT
         const int tpb = h->tune_corr_tpb;
         auto kern = fz ? k_corr2d_quad<true> : k_corr2d_quad<false>;
@@ -1350,15 +1352,19 @@ int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
                            h->d_clt + (size_t)q * 4 * ntiles * 3 * 256, h->d_tasks, h->task_words, n, h->d_lpf_corr,
                            h->d_corr_args, h->d_rig_corr + q * T * 6 * CLT_CORR_LEN, nullptr,
                            h->d_rig_disp + q * T * CLT_DISP_LAYERS, nullptr, 0LL, h->gd, h->d_rig_mix + q * T * 3 * 256);
+        if (q == 0) {   // the main quad's LMA on the second stream, beside the aux quad's correlate2D
+            HIPCHK(h, hipEventRecord(h->ev_fork, h->stream));
+            HIPCHK(h, hipStreamWaitEvent(h->stream_lma, h->ev_fork, 0));
+            hipLaunchKernelGGL(lma, g_lma, dim3(64 * LMA_WAVES), 0, h->stream_lma, h->d_rig_corr, h->d_rig_disp, n,
+                               h->d_lma_rig, h->d_rig_dirs);
+            HIPCHK(h, hipEventRecord(h->ev_lma, h->stream_lma));
+        }
     }
-    const dim3 g4((n + 3) / 4), g_lma((n + LMA_SLOTS - 1) / LMA_SLOTS);
     auto inter = fz ? k_corr_inter<true, true> : k_corr_inter<false, true>;
     hipLaunchKernelGGL(inter, g4, dim3(256), 0, h->stream, h->d_rig_mix, h->d_rig_mix + T * 3 * 256, nullptr, h->d_tasks,
                        h->task_words, n, h->d_lpf_corr, h->d_corr_args, h->d_rig_args, h->d_inter, h->d_inter_rec);
-    auto lma = (nc == 4) ? k_corr_lma<4> : k_corr_lma<3>;
-    for (int q = 0; q < 2; q++)
-        hipLaunchKernelGGL(lma, g_lma, dim3(64 * LMA_WAVES), 0, h->stream, h->d_rig_corr + q * T * 6 * CLT_CORR_LEN,
-                           h->d_rig_disp + q * T * CLT_DISP_LAYERS, n, h->d_lma_rig, h->d_rig_dirs + q * T * 8);
+    hipLaunchKernelGGL(lma, g_lma, dim3(64 * LMA_WAVES), 0, h->stream, h->d_rig_corr + T * 6 * CLT_CORR_LEN,
+                       h->d_rig_disp + T * CLT_DISP_LAYERS, n, h->d_lma_rig, h->d_rig_dirs + T * 8);
     auto lma_x = (nc == 4) ? k_corr_lma_inter<4> : k_corr_lma_inter<3>;
     const clt_rig_params& rp = h->rp;
     hipLaunchKernelGGL(lma_x, g_lma, dim3(64 * LMA_WAVES), 0, h->stream, h->d_inter, h->d_inter_rec, h->d_inter_res, n,
@@ -1372,6 +1378,7 @@ int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
         hipLaunchKernelGGL(lma_x, g_lma, dim3(64 * LMA_WAVES), 0, h->stream, h->d_inter_avg, h->d_inter_rec,
                            h->d_inter_res, n, h->d_lma_rig, rp.use_poly, rp.use_xy_poly, rp.notch_mode);
     }
+    HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_lma, 0));   // the main quad's LMA
     hipLaunchKernelGGL(k_rig_bimap, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_tasks, h->task_words, n,
                        h->d_rig_disp, h->d_rig_dirs, h->d_rig_disp + T * CLT_DISP_LAYERS, h->d_rig_dirs + T * 8,
                        h->d_inter_res, h->rig_target ? h->d_rig_target : nullptr, h->d_bimap);
