@@ -127,6 +127,8 @@ SIGNATURES = {
     "clt_set_rig_params": (_I, [_H, ctypes.POINTER(clt_rig_params)]),
     "clt_exec_corr2d_rig": (_I, [_H, _P]),
     "clt_get_corr2d_rig": (_I, [_H, _P, _P, _P, _P, _P, _P]),
+    "clt_exec_textures_rig": (_I, [_H]),
+    "clt_get_textures_rig": (_I, [_H, _P, _P, _P, _P]),
     "clt_default_quad_pairs": (_I, [_P]),
     "clt_exec_convert": (_I, [_H, _I]),
     "clt_exec_imclt": (_I, [_H, _I]),

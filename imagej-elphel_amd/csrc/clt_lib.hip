@@ -190,6 +190,13 @@ struct clt_handle {
     float* d_inter_res = nullptr;  // [max_tasks][4]
     float* d_bimap = nullptr;      // [max_tasks][CLT_BI_LAYERS]
     float* d_rig_target = nullptr; // [max_tasks]
+    float* d_rig_extra = nullptr;  // [2][max_tasks] each quad's disparity, for its textures
+    float* d_rig_tex = nullptr;    // [2][max_tasks][planes][256] textures of the main / aux quad
+    float* d_rig_tex_stats = nullptr;   // [2][max_tasks][16]
+    TexArgs* d_rig_tex_args = nullptr;
+    bool rig_tex_done = false;
+    int rig_tex_planes = 0;        // planes the rig texture buffer holds
+    int tex_planes_rig = 0;        // planes of the last rig textures
     RigArgs* d_rig_args = nullptr; // [2]: as set; with no_int_x0 for the low-texture pass
     LmaArgs* d_lma_rig = nullptr;
     bool rig_done = false, rig_target = false;
@@ -525,7 +532,8 @@ int clt_destroy(clt_handle* h) {
                     h->d_gwords, h->d_gtxy, h->d_gdisp, h->d_gderiv, h->d_task_frame, h->d_staged, h->d_fcounts,
                     h->d_adj, h->d_tstate, h->d_gpart, h->d_gsum, h->d_gdisp2, h->d_lma_args, h->d_rig_corr,
                     h->d_rig_disp, h->d_rig_mix, h->d_rig_dirs, h->d_inter, h->d_inter_avg, h->d_inter_rec,
-                    h->d_inter_res, h->d_bimap, h->d_rig_target, h->d_rig_args, h->d_lma_rig};
+                    h->d_inter_res, h->d_bimap, h->d_rig_target, h->d_rig_args, h->d_lma_rig, h->d_rig_extra,
+                    h->d_rig_tex, h->d_rig_tex_stats, h->d_rig_tex_args};
     if (h->stream) (void)sync_streams(h);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -1241,6 +1249,7 @@ static int prepare_rig(clt_handle* h, const float* target) {
         (rc = dalloc(h, (void**)&h->d_inter_rec, T * RIG_REC * 4)) ||
         (rc = dalloc(h, (void**)&h->d_inter_res, T * 4 * 4)) ||
         (rc = dalloc(h, (void**)&h->d_bimap, T * CLT_BI_LAYERS * 4)) ||
+        (rc = dalloc(h, (void**)&h->d_rig_extra, 2 * T * 4)) ||
         (rc = dalloc(h, (void**)&h->d_rig_args, 2 * sizeof(RigArgs))) ||
         (rc = dalloc(h, (void**)&h->d_lma_rig, sizeof(LmaArgs))) ||
         (h->rp.lt_rad > 0 && (rc = dalloc(h, (void**)&h->d_inter_avg, T * CLT_CORR_LEN * 4))) ||
@@ -1381,10 +1390,12 @@ int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
     HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_lma, 0));   // the main quad's LMA
     hipLaunchKernelGGL(k_rig_bimap, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_tasks, h->task_words, n,
                        h->d_rig_disp, h->d_rig_dirs, h->d_rig_disp + T * CLT_DISP_LAYERS, h->d_rig_dirs + T * 8,
-                       h->d_inter_res, h->rig_target ? h->d_rig_target : nullptr, h->d_bimap);
+                       h->d_inter_res, h->rig_target ? h->d_rig_target : nullptr, h->d_bimap, h->d_rig_extra,
+                       h->d_rig_extra + T);
     tend(h, T_RIG);
     HIPCHK(h, hipGetLastError());
     h->rig_done = true;
+    h->rig_tex_done = false;
     return finish(h);
 }
 
@@ -1409,16 +1420,8 @@ int clt_get_corr2d_rig(clt_handle* h, float* bimap, float* inter, float* corr_ma
 }
 
 // Texture arguments and output buffers for the current parameters (TexArgs upload).
-static int prepare_tex(clt_handle* h) {
-    const clt_geom& g = h->g;
-    if (g.num_colors != 3 || g.num_cams < 2 || g.num_cams > 8)
-        return fail(h, CLT_E_UNSUPPORTED, "clt_exec_textures: needs a Bayer frame of 2..8 cameras");
-    const int planes = h->tp.keep_weights ? 4 + g.num_cams + 4 : 4;
-    int rc;
-    if ((rc = ensure_capacity(h, &h->d_tex, &h->tex_cap, (size_t)h->cap_tasks * planes * 256)) ||
-        (rc = dalloc(h, (void**)&h->d_tex_stats, (size_t)h->cap_tasks * 4 * CLT_MAX_CAMS * 4)))
-        return rc;
-    TexArgs a;
+// TexArgs of ncams ports with the given port offsets and the handle's texture parameters.
+static void fill_tex_args(const clt_handle* h, int ncams, const float (*port_offsets)[2], TexArgs& a) {
     std::memset(&a, 0, sizeof(a));
     const double gw = 1.0 / (1.0 + h->cp.corr_red + h->cp.corr_blue);
     a.col_w[2] = (float)gw;
@@ -1429,16 +1432,32 @@ static int prepare_tex(clt_handle* h) {
     const double ksigma = 1.0 / (2.0 * h->tp.diff_sigma * h->tp.diff_sigma);
     a.ksigma = (float)ksigma;
     a.ksigma_l2 = (float)(ksigma * M_LOG2E);
-    for (int i = 0, np = 0; i < g.num_cams; i++)
-        for (int j = i + 1; j < g.num_cams; j++, np++) {   // ImageDtt.java:2881-2891 pair weights
-            const double dx = (double)h->tp.port_offsets[j][0] - h->tp.port_offsets[i][0];
-            const double dy = (double)h->tp.port_offsets[j][1] - h->tp.port_offsets[i][1];
+    for (int i = 0, np = 0; i < ncams; i++)
+        for (int j = i + 1; j < ncams; j++, np++) {   // ImageDtt.java:2881-2891 pair weights
+            const double dx = (double)port_offsets[j][0] - port_offsets[i][0];
+            const double dy = (double)port_offsets[j][1] - port_offsets[i][1];
             a.pair_k_l2[np] = (float)(ksigma / (dx * dx + dy * dy) * M_LOG2E);
         }
     a.corr_magic_scale = h->tp.corr_magic_scale;
     a.dust_remove = h->tp.dust_remove;
-    a.planes = planes;
-    std::memcpy(a.port_offsets, h->tp.port_offsets, sizeof(a.port_offsets));
+    a.planes = h->tp.keep_weights ? 4 + ncams + 4 : 4;
+    for (int i = 0; i < ncams; i++) {
+        a.port_offsets[i][0] = port_offsets[i][0];
+        a.port_offsets[i][1] = port_offsets[i][1];
+    }
+}
+
+static int prepare_tex(clt_handle* h) {
+    const clt_geom& g = h->g;
+    if (g.num_colors != 3 || g.num_cams < 2 || g.num_cams > 8)
+        return fail(h, CLT_E_UNSUPPORTED, "clt_exec_textures: needs a Bayer frame of 2..8 cameras");
+    const int planes = h->tp.keep_weights ? 4 + g.num_cams + 4 : 4;
+    int rc;
+    if ((rc = ensure_capacity(h, &h->d_tex, &h->tex_cap, (size_t)h->cap_tasks * planes * 256)) ||
+        (rc = dalloc(h, (void**)&h->d_tex_stats, (size_t)h->cap_tasks * 4 * CLT_MAX_CAMS * 4)))
+        return rc;
+    TexArgs a;
+    fill_tex_args(h, g.num_cams, h->tp.port_offsets, a);
     HIPCHK(h, hipMemcpyAsync(h->d_tex_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
     h->tex_planes_at_exec = planes;
     return CLT_OK;
@@ -1462,6 +1481,55 @@ int clt_exec_textures(clt_handle* h, const float* extra_disparity) {
     h->tex_done = true;
     h->combined = false;
     return finish(h);
+}
+
+// generateTextureTiles of both quads of the rig (ID:9360-9398): each quad's 4 ports with the quad
+// port offsets (ID:8832-8836) and its tileCorrs disparity as the extra disparity.
+int clt_exec_textures_rig(clt_handle* h) {
+    if (!h) return CLT_E_ARG;
+    if (!h->rig_done) return fail(h, CLT_E_STATE, "clt_exec_textures_rig: rig correlation not executed");
+    const size_t T = h->cap_tasks, ntiles = (size_t)h->tiles_x * h->tiles_y;
+    static const float quad_offsets[4][2] = {{-0.5f, -0.5f}, {0.5f, -0.5f}, {-0.5f, 0.5f}, {0.5f, 0.5f}};
+    TexArgs a;
+    fill_tex_args(h, 4, quad_offsets, a);
+    const int planes = a.planes;
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_rig_tex_args, sizeof(TexArgs))) ||
+        (rc = dalloc(h, (void**)&h->d_rig_tex_stats, 2 * T * 16 * 4)))
+        return rc;
+    if (planes > h->rig_tex_planes) {   // the buffer grows when keep_weights is switched on
+        if (h->d_rig_tex) HIPCHK(h, hipFree(h->d_rig_tex));
+        h->d_rig_tex = nullptr;
+        if ((rc = dalloc(h, (void**)&h->d_rig_tex, 2 * T * planes * 256 * 4))) return rc;
+        h->rig_tex_planes = planes;
+    }
+    HIPCHK(h, hipMemcpyAsync(h->d_rig_tex_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    const int n = h->num_tasks;
+    tbegin(h, T_TEX);
+    for (int q = 0; q < 2; q++)
+        hipLaunchKernelGGL(k_textures<4>, dim3(n), dim3(256), 0, h->stream, h->d_clt + (size_t)q * 4 * ntiles * 3 * 256,
+                           h->d_tasks, h->task_words, n, h->d_lpf_corr, h->d_rig_extra + q * T, h->d_rig_tex_args,
+                           h->d_rig_tex + q * T * planes * 256, h->d_rig_tex_stats + q * T * 16, nullptr, 0LL, h->gd);
+    tend(h, T_TEX);
+    HIPCHK(h, hipGetLastError());
+    h->rig_tex_done = true;
+    h->tex_planes_rig = planes;
+    return finish(h);
+}
+
+int clt_get_textures_rig(clt_handle* h, float* tiles_main, float* tiles_aux, float* stats_main, float* stats_aux) {
+    if (!h) return CLT_E_ARG;
+    if (!h->rig_tex_done) return fail(h, CLT_E_STATE, "clt_get_textures_rig: rig textures not executed");
+    HIPCHK(h, sync_streams(h));
+    const size_t nt = h->num_tasks, T = h->cap_tasks, planes = h->tex_planes_rig;
+    float* tl[2] = {tiles_main, tiles_aux};
+    float* st[2] = {stats_main, stats_aux};
+    for (int q = 0; q < 2; q++) {
+        if (tl[q])
+            HIPCHK(h, hipMemcpy(tl[q], h->d_rig_tex + q * T * planes * 256, nt * planes * 256 * 4, hipMemcpyDeviceToHost));
+        if (st[q]) HIPCHK(h, hipMemcpy(st[q], h->d_rig_tex_stats + q * T * 16, nt * 16 * 4, hipMemcpyDeviceToHost));
+    }
+    return CLT_OK;
 }
 
 int clt_exec_combine_textures(clt_handle* h) {

@@ -246,12 +246,14 @@ __global__ __launch_bounds__(256) void k_inter_lt_avg(const float* __restrict__ 
 
 // The BIDISPARITY record (ID:89-114, SNGL_TO_BI ID:190-194, ID:9186-9248) of every task from the
 // two quads' records and one-direction fits and the inter result; skipped tasks: zeros and a NaN
-// target (the reference's zero-initialised arrays, ID:8964-8969).
+// target (the reference's zero-initialised arrays, ID:8964-8969).  extra_main / extra_aux: each
+// quad's disparity, the extra disparity of its textures (ID:9194-9197).
 __global__ __launch_bounds__(256) void k_rig_bimap(const int32_t* __restrict__ tasks, int task_words, int num_tasks,
                                                    const float* __restrict__ disp_main, const float* __restrict__ dir_main,
                                                    const float* __restrict__ disp_aux, const float* __restrict__ dir_aux,
                                                    const float* __restrict__ inter, const float* __restrict__ target,
-                                                   float* __restrict__ bimap) {
+                                                   float* __restrict__ bimap, float* __restrict__ extra_main,
+                                                   float* __restrict__ extra_aux) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= num_tasks) return;
     float* b = bimap + (size_t)t * CLT_BI_LAYERS;
@@ -286,6 +288,8 @@ __global__ __launch_bounds__(256) void k_rig_bimap(const int32_t* __restrict__ t
     }
 #pragma unroll
     for (int i = 0; i < CLT_BI_LAYERS; i++) b[i] = v[i];
+    extra_main[t] = v[CLT_BI_DISP_FULL];
+    extra_aux[t] = v[CLT_BI_ADISP_FULL];
 }
 
 }  // namespace clt

@@ -340,6 +340,20 @@ class GPUTileProcessor:
                                                                              "disp_main", "disp_aux")]))
         return out
 
+    def execTexturesRig(self):
+        """Both quads' textures after execCorrelate2DRig (ID:9360-9398)."""
+        self._chk(self.lib.clt_exec_textures_rig(self.h))
+
+    def getTexturesRig(self):
+        """((tiles_main, tiles_aux) [n][planes][16][16], (stats_main, stats_aux) [n][16]); planes
+        12 with keep_weights, else 4."""
+        n = self.num_task_tiles
+        planes = 12 if self.keep_weights else 4
+        t = [np.empty((n, planes, 16, 16), np.float32) for _ in range(2)]
+        st = [np.empty((n, 16), np.float32) for _ in range(2)]
+        self._chk(self.lib.clt_get_textures_rig(self.h, _ptr(t[0]), _ptr(t[1]), _ptr(st[0]), _ptr(st[1])))
+        return tuple(t), tuple(st)
+
     def getTextures(self):
         """(tiles [n][planes][16][16], stats [n][4C])."""
         n = self.num_task_tiles

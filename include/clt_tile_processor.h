@@ -262,6 +262,11 @@ int  clt_exec_corr2d(clt_handle* h);
  * BIDISPARITY record (CLT_BI_*).  Uses the correlation, LMA and rig parameters and corr_sigma;
  * target_disparity: num_tasks floats for CLT_BI_TARGET or NULL (NaN). */
 int  clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity);
+/* generateTextureTiles of both quads after clt_exec_corr2d_rig (ID:9360-9398): the 4 ports of each
+ * quad with the quad port offsets (ID:8832-8836) and the quad's tileCorrs disparity
+ * (CLT_BI_DISP_FULL / CLT_BI_ADISP_FULL) as the extra disparity; the texture parameters as set
+ * (the port offsets excepted). */
+int  clt_exec_textures_rig(clt_handle* h);
 /* Per-tile textures (ID:2662-2812).  extra_disparity: num_tasks floats or NULL (ID:2663-2677). */
 int  clt_exec_textures(clt_handle* h, const float* extra_disparity);
 /* Overlap-add of the texture tiles into full-frame planes (combineRGBATiles, ID:3858-3989,
@@ -284,6 +289,11 @@ int  clt_get_corr2d(clt_handle* h, float* corr, float* combo, float* disparity);
  * each quad after its LMA (CM = the tileCorrs disparity, STRENGTH its strength). */
 int  clt_get_corr2d_rig(clt_handle* h, float* bimap, float* inter, float* corr_main, float* corr_aux,
                         float* disp_main, float* disp_aux);
+/* Results of clt_exec_textures_rig, each may be NULL: tiles [num_tasks][planes][256] with planes =
+ * 12 when keep_weights (RGBA, 4 port weights, rms r,b,g,combined), else 4; stats [num_tasks][16]
+ * as clt_get_textures for 4 cameras. */
+int  clt_get_textures_rig(clt_handle* h, float* tiles_main, float* tiles_aux, float* stats_main,
+                          float* stats_aux);
 /* tiles: [num_tasks][planes][256] with planes = 4 + num_cams + 4 when keep_weights (RGBA, one
  * weight per port, rms r,b,g,combined), else 4 (RGBA); stats:
  * [num_tasks][4*num_cams] = max_diff[num_cams] (ID:2802-2806) then the mean tones

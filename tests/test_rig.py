@@ -22,7 +22,7 @@ than 0.01 px).
 import numpy as np
 import pytest
 
-from helpers import gpu_processor, log, oracle_geom
+from helpers import gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, robust_rel_rms
 
 RIG_OFF = np.array([2.0, 0.0])   # aux quad centre, in main-quad disparity units
 AUX_SCALE = 0.75                 # aux baseline / main baseline
@@ -244,3 +244,34 @@ def test_gpu_rig_rejects_other_camera_counts(pk):
     with pytest.raises(pk.CltError):
         tp.execCorrelate2DRig()
     tp.close()
+
+
+@pytest.mark.gpu
+def test_gpu_rig_textures(pk, orc):
+    """generateTextureTiles of each quad (ID:9360-9398) against the oracle's textures of the same
+    cameras, on the device CLT, with the quad's device disparity as the extra disparity."""
+    log("case: two-quad rig 128x96 textures of both quads")
+    sc = rig_scene(pk, 128, 96, 9, disparity=("field", 1.0, 4.0), target_disparity=2.0)
+    tp = gpu_processor(pk, sc)
+    tp.setCorrSigma(0.8)
+    tp.execConverCorrectTiles()
+    tp.execCorrelate2DRig()
+    b = tp.getCorr2DRig()["bimap"]
+    tp.execTexturesRig()
+    (tm, ta), (sm, sa) = tp.getTexturesRig()
+    clt = gpu_clt_cpu_layout(tp, sc).astype(np.float64)
+    tp.close()
+    g4 = orc.make_geom(128, 96, 4, 3)
+    for q, (tiles, stats, k) in enumerate(((tm, sm, orc.BI_DISP_FULL), (ta, sa, orc.BI_ADISP_FULL))):
+        extra = np.nan_to_num(b[:, k].astype(np.float64))
+        tasks4 = np.ascontiguousarray(np.concatenate([sc.tasks[:, :2], sc.tasks[:, 2 + 8 * q:10 + 8 * q]], axis=1))
+        o_tiles, o_stats = orc.textures_frame(g4, clt[4 * q:4 * q + 4], tasks4, extra_disparity=extra)
+        res = {}
+        for name, sl in (("rgb", slice(0, 3)), ("alpha", slice(3, 4)), ("weights", slice(4, 8)), ("rms", slice(8, None))):
+            res[name], res[name + "_flips"] = robust_rel_rms(tiles[:, sl], o_tiles[:, sl])
+        res["stats"] = rel_rms(stats, o_stats)
+        log("  quad %d: " % q + ", ".join("%s=%.3g" % kv for kv in res.items()))
+        for name in ("rgb", "alpha", "weights", "rms"):
+            assert res[name] < 1e-4 and res[name + "_flips"] <= 1e-3, name
+        assert res["stats"] < 1e-4
+
