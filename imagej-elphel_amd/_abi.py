@@ -89,6 +89,13 @@ class clt_port_geometry(ctypes.Structure):
                 ("roll_deg", ctypes.c_double * 4), ("extrinsic", ctypes.c_double * 13)]
 
 
+class clt_rig_offset(ctypes.Structure):
+    _fields_ = [("baseline", ctypes.c_double), ("aux_angle", ctypes.c_double), ("aux_z", ctypes.c_double),
+                ("aux_azimuth", ctypes.c_double), ("aux_tilt", ctypes.c_double), ("aux_roll", ctypes.c_double),
+                ("aux_zoom", ctypes.c_double), ("main_disparity_radius", ctypes.c_double),
+                ("aux_disparity_radius", ctypes.c_double)]
+
+
 class clt_lma_params(ctypes.Structure):
     _fields_ = [("lambda_", ctypes.c_double), ("lambda_scale_good", ctypes.c_double),
                 ("lambda_scale_bad", ctypes.c_double), ("lambda_max", ctypes.c_double),
@@ -147,6 +154,8 @@ SIGNATURES = {
     "clt_reset_timing": (_I, [_H]),
     "clt_default_port_geometry": (None, [ctypes.POINTER(clt_port_geometry), _I, _I]),
     "clt_set_port_geometry": (_I, [_H, ctypes.POINTER(clt_port_geometry)]),
+    "clt_set_port_geometry_rig": (_I, [_H, ctypes.POINTER(clt_port_geometry), ctypes.POINTER(clt_port_geometry),
+                                       ctypes.POINTER(clt_rig_offset)]),
     "clt_set_tasks_geometry": (_I, [_H, _P, _P, _P, _I]),
     "clt_get_tasks": (_I, [_H, _P]),
     "clt_get_port_derivatives": (_I, [_H, _P]),

@@ -381,12 +381,17 @@ __device__ __forceinline__ void port_centre(const PortGeomDev& P, const double* 
     rx[13] = dx[4]; ry[13] = dy[4];
 }
 
+// rig: pg[0] the main quad (sensors 0..3), pg[1] the auxiliary quad (sensors 4..7, clt_bi_quad
+// ID:8970-8993) at disparity * dscale (R_aux / R_main); derivatives only for a single quad.
 __global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restrict__ pg, const double* __restrict__ rtab,
                                                       const int32_t* __restrict__ words, const int32_t* __restrict__ txy,
                                                       const float* __restrict__ disparity, int num_tasks, int task_words,
-                                                      int32_t* __restrict__ tasks, double* __restrict__ deriv) {
+                                                      int32_t* __restrict__ tasks, double* __restrict__ deriv, int rig,
+                                                      double dscale) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = gid >> 2, i = gid & 3;
+    const int shift = rig ? 3 : 2;
+    const int t = gid >> shift, i = gid & ((1 << shift) - 1);
+    const int q = i >> 2, s = i & 3;
     if (t >= num_tasks) return;
     int32_t* rec = tasks + (size_t)t * task_words;
     const int word = words[t], xy16 = txy[t];
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restr
         rec[0] = word;
         rec[1] = xy16;
     }
-    double* dr = deriv ? deriv + (size_t)t * 8 * 14 + (size_t)(2 * i) * 14 : nullptr;
+    double* dr = (deriv && !rig) ? deriv + (size_t)t * 8 * 14 + (size_t)(2 * i) * 14 : nullptr;
     if (word == 0) {   // GTP:720-722: nothing to do for this tile
         rec[2 + 2 * i] = 0;
         rec[3 + 2 * i] = 0;
@@ -404,7 +409,7 @@ __global__ __launch_bounds__(256) void k_port_centres(const PortGeomDev* __restr
     }
     const double px = (double)((xy16 & 0xffff) * 8 + 4), py = (double)(((xy16 >> 16) & 0xffff) * 8 + 4);   // GTP:724-725
     double x, y;
-    port_centre(*pg, rtab, px, py, (double)disparity[t], i, &x, &y, dr, dr ? dr + 14 : nullptr);
+    port_centre(pg[q], rtab, px, py, (double)disparity[t] * (q ? dscale : 1.0), s, &x, &y, dr, dr ? dr + 14 : nullptr);
     rec[2 + 2 * i] = __float_as_int((float)x);
     rec[3 + 2 * i] = __float_as_int((float)y);
 }

@@ -148,6 +148,8 @@ struct clt_handle {
     double* d_gsum = nullptr;      // [GEO_ADJ_SUMS + 13] sums, then the extrinsic step
     float* d_gdisp2 = nullptr;     // [max_tasks] candidate disparities
     bool port_geom_set = false, tasks_from_geom = false;
+    bool rig_geom = false;         // d_port_geom holds the main and the auxiliary quad of a rig
+    double rig_dscale = 1.0;       // auxiliary disparity per main disparity (R_aux / R_main)
     // batched frames (clt_create_batch): host task lists per frame, flat device list with frames
     int32_t* d_task_frame = nullptr;               // [cap_tasks]
     int32_t* d_staged = nullptr;                   // [frames][max_tasks][task_words] as set per frame
@@ -815,62 +817,119 @@ static void port_geom_dev(const clt_port_geometry* p, PortGeomDev& pg) {
     pg.tab_len = (int)std::ceil(2.0 / pg.step_r) + 1;
 }
 
-int clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p) {
-    if (!h) return CLT_E_ARG;
-    if (!p || p->num_sensors != 4 || h->g.num_cams != 4 || p->width <= 0 || p->height <= 0 || p->pixel_size <= 0 ||
-        p->distortion_radius <= 0 || p->focal_length <= 0)
-        return fail(h, CLT_E_ARG, "clt_set_port_geometry: needs a quad camera handle and positive sizes");
-    PortGeomDev pg;
-    port_geom_dev(p, pg);
-    // reverse distortion table (calcReverseDistortionTable, GC:2707-2756): r / rD by Newton steps
+// reverse distortion table (calcReverseDistortionTable, GC:2707-2756): r / rD by Newton steps
+static std::vector<double> reverse_distortion_tab(const clt_port_geometry* p) {
     const double step = 0.001, max_r = 2.0;
     const int len = (int)std::ceil(max_r / step) + 1;
     std::vector<double> tab(len, std::nan(""));
-    {
-        const double C = p->distortion[0], B = p->distortion[1], A = p->distortion[2], A5 = p->distortion[3];
-        const double A6 = p->distortion[4], A7 = p->distortion[5], A8 = p->distortion[6];
-        const double d0 = 1.0 - A8 - A7 - A6 - A5 - A - B - C;
-        auto k_of = [&](double r) { return (((((((A8 * r + A7) * r + A6) * r + A5) * r + A) * r + B) * r + C) * r + d0); };
-        auto dk_of = [&](double r) {
-            return (((((((8 * A8 * r + 7 * A7) * r + 6 * A6) * r + 5 * A5) * r + 4 * A) * r + 3 * B) * r + 2 * C) * r + d0);
-        };
-        tab[0] = 1.0 / d0;
-        double rprev = 0.0, slope = 1.0;
-        for (int i = 1; i < len; i++) {
-            const double rd_target = step * i;
-            double r = rprev + step / slope;
-            bool ok = false;
-            for (int it = 0; it < 1000; it++) {
-                slope = dk_of(r);
-                const double rd = r * k_of(r);
-                if (slope < 0.1) break;   // too much distortion: the rest of the table stays NaN
-                if (std::fabs(rd - rd_target) < 1e-8) { ok = true; break; }
-                r += (rd_target - rd) / slope;
-                ok = (it == 999);
-            }
-            if (slope < 0.1) break;
-            (void)ok;
-            rprev = r;
-            tab[i] = r / rd_target;
+    const double C = p->distortion[0], B = p->distortion[1], A = p->distortion[2], A5 = p->distortion[3];
+    const double A6 = p->distortion[4], A7 = p->distortion[5], A8 = p->distortion[6];
+    const double d0 = 1.0 - A8 - A7 - A6 - A5 - A - B - C;
+    auto k_of = [&](double r) { return (((((((A8 * r + A7) * r + A6) * r + A5) * r + A) * r + B) * r + C) * r + d0); };
+    auto dk_of = [&](double r) {
+        return (((((((8 * A8 * r + 7 * A7) * r + 6 * A6) * r + 5 * A5) * r + 4 * A) * r + 3 * B) * r + 2 * C) * r + d0);
+    };
+    tab[0] = 1.0 / d0;
+    double rprev = 0.0, slope = 1.0;
+    for (int i = 1; i < len; i++) {
+        const double rd_target = step * i;
+        double r = rprev + step / slope;
+        for (int it = 0; it < 1000; it++) {
+            slope = dk_of(r);
+            const double rd = r * k_of(r);
+            if (slope < 0.1) break;   // too much distortion: the rest of the table stays NaN
+            if (std::fabs(rd - rd_target) < 1e-8) break;
+            r += (rd_target - rd) / slope;
         }
+        if (slope < 0.1) break;
+        rprev = r;
+        tab[i] = r / rd_target;
     }
+    return tab;
+}
+
+// ns PortGeomDev records (1: a quad, 2: main and auxiliary quad) and the reverse distortion table
+static int upload_port_geometry(clt_handle* h, const PortGeomDev* pg, int ns, const std::vector<double>& tab) {
     int rc;
-    if ((rc = dalloc(h, (void**)&h->d_port_geom, sizeof(PortGeomDev))) ||
-        (rc = dalloc(h, (void**)&h->d_rtab, sizeof(double) * len)))
+    if ((rc = dalloc(h, (void**)&h->d_port_geom, 2 * sizeof(PortGeomDev))) ||
+        (rc = dalloc(h, (void**)&h->d_rtab, sizeof(double) * tab.size())))
         return rc;
     HIPCHK(h, sync_streams(h));
-    HIPCHK(h, hipMemcpy(h->d_port_geom, &pg, sizeof(pg), hipMemcpyHostToDevice));
-    HIPCHK(h, hipMemcpy(h->d_rtab, tab.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_port_geom, pg, ns * sizeof(PortGeomDev), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->d_rtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    return CLT_OK;
+}
+
+static bool geometry_ok(const clt_port_geometry* p) {
+    return p && p->num_sensors == 4 && p->width > 0 && p->height > 0 && p->pixel_size > 0 && p->distortion_radius > 0 &&
+           p->focal_length > 0;
+}
+
+int clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p) {
+    if (!h) return CLT_E_ARG;
+    if (!geometry_ok(p) || h->g.num_cams != 4)
+        return fail(h, CLT_E_ARG, "clt_set_port_geometry: needs a quad camera handle and positive sizes");
+    PortGeomDev pg;
+    port_geom_dev(p, pg);
+    int rc;
+    if ((rc = upload_port_geometry(h, &pg, 1, reverse_distortion_tab(p)))) return rc;
     h->pgeom = *p;
     h->port_geom_set = true;
+    h->rig_geom = false;
+    return CLT_OK;
+}
+
+// clt_bi_quad's port centres (ID:8970-8993): the auxiliary quad undistorts the tile centre with the
+// main camera's model (gc_main, GC:2179-2184), uses rXY_aux (RigOffset.recalcRXY GC:728-745) and
+// its CorrVector rotations times the rig matrix (getRotMatrices(rigMatrix) GC:1016-1026,
+// RigOffset.getRotMatrix GC:770-803), at disparity * R_aux / R_main (ID:8968).
+int clt_set_port_geometry_rig(clt_handle* h, const clt_port_geometry* main, const clt_port_geometry* aux,
+                              const clt_rig_offset* rig) {
+    if (!h) return CLT_E_ARG;
+    if (!geometry_ok(main) || !geometry_ok(aux) || !rig || h->g.num_cams != 8 || !(rig->main_disparity_radius > 0) ||
+        !(rig->aux_disparity_radius > 0))
+        return fail(h, CLT_E_ARG, "clt_set_port_geometry_rig: needs an 8-camera handle, positive sizes and radii");
+    PortGeomDev pg[2];
+    port_geom_dev(main, pg[0]);
+    port_geom_dev(aux, pg[1]);
+    const double ca = std::cos(rig->aux_azimuth), sa = std::sin(rig->aux_azimuth);
+    const double ct = std::cos(rig->aux_tilt), st = std::sin(rig->aux_tilt);
+    const double zoom = 1.0 + rig->aux_zoom, cr = std::cos(rig->aux_roll) * zoom, sr = std::sin(rig->aux_roll) * zoom;
+    const double m_az[9] = {ca, 0, -sa, 0, 1, 0, sa, 0, ca};   // ROT_AZ_SGN = -1
+    const double m_t[9] = {1, 0, 0, 0, ct, st, 0, -st, ct};
+    const double m_r[9] = {cr, sr, 0, -sr, cr, 0, 0, 0, 1};
+    double ta[9], rm[9];
+    mat3(m_t, m_az, ta);
+    mat3(m_r, ta, rm);
+    const double xc = rig->baseline * std::cos(rig->aux_angle) / rig->aux_disparity_radius;
+    const double yc = rig->baseline * std::sin(rig->aux_angle) / rig->aux_disparity_radius;
+    const double c = std::cos(rig->aux_roll), sn = std::sin(rig->aux_roll);
+    for (int i = 0; i < 4; i++) {
+        double r[9];
+        mat3(pg[1].rots[i], rm, r);
+        std::memcpy(pg[1].rots[i], r, sizeof(r));
+        pg[1].rxy[i][0] = xc + c * aux->rxy[i][0] + sn * aux->rxy[i][1];
+        pg[1].rxy[i][1] = yc - sn * aux->rxy[i][0] + c * aux->rxy[i][1];
+    }
+    pg[1].half_w = pg[0].half_w;   // the tile centre through the main camera's distortion
+    pg[1].half_h = pg[0].half_h;
+    pg[1].rd_scale = pg[0].rd_scale;
+    int rc;
+    if ((rc = upload_port_geometry(h, pg, 2, reverse_distortion_tab(main)))) return rc;
+    h->pgeom = *main;
+    h->port_geom_set = true;
+    h->rig_geom = true;
+    h->rig_dscale = rig->aux_disparity_radius / rig->main_disparity_radius;
     return CLT_OK;
 }
 
 static int launch_port_centres(clt_handle* h, double* deriv) {
     const int n = h->num_tasks;
     if (n == 0) return CLT_OK;
-    hipLaunchKernelGGL(k_port_centres, dim3((4 * n + 255) / 256), dim3(256), 0, h->stream, h->d_port_geom, h->d_rtab,
-                       h->d_gwords, h->d_gtxy, h->d_gdisp, n, h->task_words, h->d_tasks, deriv);
+    const int ns = h->rig_geom ? 8 : 4;
+    hipLaunchKernelGGL(k_port_centres, dim3((ns * n + 255) / 256), dim3(256), 0, h->stream, h->d_port_geom, h->d_rtab,
+                       h->d_gwords, h->d_gtxy, h->d_gdisp, n, h->task_words, h->d_tasks, deriv, h->rig_geom ? 1 : 0,
+                       h->rig_dscale);
     HIPCHK(h, hipGetLastError());
     return CLT_OK;
 }
@@ -918,6 +977,7 @@ int clt_get_tasks(clt_handle* h, void* tasks) {
 
 int clt_get_port_derivatives(clt_handle* h, double* out) {
     if (!h || !out) return CLT_E_ARG;
+    if (h->rig_geom) return fail(h, CLT_E_UNSUPPORTED, "clt_get_port_derivatives: a single quad only");
     if (!h->tasks_from_geom) return fail(h, CLT_E_STATE, "clt_get_port_derivatives: tasks were not set from the port geometry");
     int rc;
     if ((rc = dalloc(h, (void**)&h->d_gderiv, (size_t)h->max_tasks * 8 * 14 * sizeof(double)))) return rc;
@@ -1679,7 +1739,7 @@ static int geo_prepare(clt_handle* h, const char* who) {
 static int geo_eval(clt_handle* h, const float* disp, double lambda, double* sums) {
     const int n = h->num_tasks;
     hipLaunchKernelGGL(k_port_centres, dim3((4 * n + 255) / 256), dim3(256), 0, h->stream, h->d_port_geom, h->d_rtab,
-                       h->d_gwords, h->d_gtxy, disp, n, h->task_words, h->d_tasks, nullptr);
+                       h->d_gwords, h->d_gtxy, disp, n, h->task_words, h->d_tasks, nullptr, 0, 1.0);
     launch_corr_adjoint(h);
     if (!sums) return CLT_OK;
     const int blocks = (n + GEO_ADJ_TASKS - 1) / GEO_ADJ_TASKS;

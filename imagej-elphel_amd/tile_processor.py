@@ -129,6 +129,24 @@ class GPUTileProcessor:
         """The GeometryCorrection of the camera (rotation matrices, reverse distortion table)."""
         self._chk(self.lib.clt_set_port_geometry(self.h, ctypes.byref(geometry)))
 
+    @staticmethod
+    def rigOffset(baseline=1256.0, main_disparity_radius=None, aux_disparity_radius=None, **kw):
+        """clt_rig_offset (GeometryCorrection.RigOffset GC:256-290); the disparity radii default to
+        sqrt(2) * 105 mm (GC:81, GC:1757)."""
+        r = _abi.clt_rig_offset(baseline=baseline)
+        dflt = 2 ** 0.5 * 105.0
+        r.main_disparity_radius = dflt if main_disparity_radius is None else main_disparity_radius
+        r.aux_disparity_radius = dflt if aux_disparity_radius is None else aux_disparity_radius
+        for k, v in kw.items():
+            setattr(r, k, v)
+        return r
+
+    def setPortGeometryRig(self, main, aux, rig_offset):
+        """The two quads of a rig (an 8-camera processor): main ports 0..3, auxiliary 4..7 with the
+        rig offsets, as clt_bi_quad computes them (ID:8970-8993)."""
+        self._chk(self.lib.clt_set_port_geometry_rig(self.h, ctypes.byref(main), ctypes.byref(aux),
+                                                     ctypes.byref(rig_offset)))
+
     def getTileSubcamOffsets(self, task, tx, ty, target_disparity):
         """GTP:686-778 followed by setTasks: the port centres of every task are computed on the
         GPU from the port geometry.  task, tx, ty, target_disparity: per-task arrays."""

@@ -323,12 +323,32 @@ typedef struct clt_port_geometry {
     double  extrinsic[CLT_EXTRINSICS];   /* CorrVector.vector                                     */
 } clt_port_geometry;
 
+/* GeometryCorrection.RigOffset (GC:256-290) of the auxiliary quad of a two-quad rig, and the
+ * disparityRadius of each camera (GC:81, GC:1757: sqrt(2) * cameraRadius, mm). */
+typedef struct clt_rig_offset {
+    double  baseline;           /* mm between the camera centres, 1256 (GC:268)                   */
+    double  aux_angle;          /* radians, 0: the auxiliary camera to the right (GC:269)         */
+    double  aux_z;              /* mm towards the scene; not used by the port centres (GC:271)   */
+    double  aux_azimuth;        /* radians, positive looks right (GC:272)                         */
+    double  aux_tilt;           /* radians, positive looks up (GC:273)                            */
+    double  aux_roll;           /* radians, positive clockwise (GC:274)                           */
+    double  aux_zoom;           /* relative zoom - 1 (GC:275)                                     */
+    double  main_disparity_radius, aux_disparity_radius;   /* mm                                 */
+} clt_rig_offset;
+
 /* Zero distortion, zero extrinsics, quad rXY {(-1/2,-1/2),(1/2,-1/2),(-1/2,1/2),(1/2,1/2)}
  * (GC:78), sensor centres at the frame centre, reference focal length/pixel/radius (GC:37-39). */
 void clt_default_port_geometry(clt_port_geometry* p, int32_t width, int32_t height);
 /* Rotation matrices, their derivatives and the reverse distortion table (calcReverseDistortionTable,
  * GC:2707-2756) are prepared on the host and uploaded; needs a handle of 4 cameras. */
 int  clt_set_port_geometry(clt_handle* h, const clt_port_geometry* p);
+/* The port geometry of a two-quad rig on an 8-camera handle, for clt_set_tasks_geometry: the main
+ * quad (sensors 0..3) as clt_set_port_geometry; the auxiliary quad (sensors 4..7) as clt_bi_quad
+ * computes it (ID:8970-8993): getPortsCoordinatesAndDerivatives(gc_main, use_rig_offsets = true,
+ * aux CorrVector rotations times RigOffset.getRotMatrix, ...) at the disparity scaled by
+ * aux_disparity_radius / main_disparity_radius.  No derivatives (clt_get_port_derivatives). */
+int  clt_set_port_geometry_rig(clt_handle* h, const clt_port_geometry* main, const clt_port_geometry* aux,
+                               const clt_rig_offset* rig);
 /* getTileSubcamOffsets + setTasks: task words, txy = tx | ty << 16 and target disparities of
  * num_tasks tiles; the tile centre is (8 tx + 4, 8 ty + 4) (GTP:724-725).  Records of tasks with a
  * zero task word keep zero centres. */

@@ -65,6 +65,14 @@ class RigParams(ctypes.Structure):
                 ("cm_max_normalization", ctypes.c_double), ("max_corr_radius", ctypes.c_double)]
 
 
+class RigOffset(ctypes.Structure):
+    """Mirror of ``clt_rig_offset`` (GeometryCorrection.RigOffset GC:256-290)."""
+    _fields_ = [("baseline", ctypes.c_double), ("aux_angle", ctypes.c_double), ("aux_z", ctypes.c_double),
+                ("aux_azimuth", ctypes.c_double), ("aux_tilt", ctypes.c_double), ("aux_roll", ctypes.c_double),
+                ("aux_zoom", ctypes.c_double), ("main_disparity_radius", ctypes.c_double),
+                ("aux_disparity_radius", ctypes.c_double)]
+
+
 class TexParams(ctypes.Structure):
     _fields_ = [("min_shot", ctypes.c_float), ("scale_shot", ctypes.c_float),
                 ("diff_sigma", ctypes.c_float), ("dust_remove", ctypes.c_int32),
@@ -240,6 +248,8 @@ def lib(precision=64):
     L.orc_rig_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, ctypes.c_int, ctypes.POINTER(CorrParams),
                                 ctypes.POINTER(CorrLmaParams), ctypes.POINTER(RigParams), ctypes.c_double, _VP,
                                 _D, _D, _D, _D, _D, _D, ctypes.c_int]
+    L.orc_port_coords_rig.argtypes = [ctypes.POINTER(PortGeometry), ctypes.POINTER(PortGeometry),
+                                      ctypes.POINTER(RigOffset), ctypes.c_int, _D, _D, _D, _D]
     L.orc_quadratic_max.argtypes = [_D, _D, _D, _D, ctypes.c_int, _D]
     L.orc_single2d_cm.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D]
     L.orc_single2d_poly.argtypes = [_D, ctypes.c_double, ctypes.POINTER(CorrLmaParams), _D]
@@ -621,6 +631,15 @@ def rig_analysis(geom, tasks, corr_main, disp_main, corr_aux, disp_aux, inter, t
     if rc:
         raise ValueError("orc_rig_analysis failed: %d" % rc)
     return bimap
+
+
+def port_coords_rig(main, aux, rig, px, py, disparity):
+    """Port centres of both quads of a rig (ID:8970-8993): [n][8][2], main ports then auxiliary."""
+    px, py, d = (_f64(np.atleast_1d(a)) for a in (px, py, disparity))
+    xy = np.zeros((len(px), 8, 2))
+    lib(64).orc_port_coords_rig(ctypes.byref(main), ctypes.byref(aux), ctypes.byref(rig), len(px), _dp(px), _dp(py),
+                                _dp(d), _dp(xy))
+    return xy
 
 
 def quadratic_max(x, y, z, w):

@@ -1772,17 +1772,20 @@ static double r_by_rdist(const double* tab, double rdist) {
 /* GC:2159-2314 getPortsCoordinatesAndDerivatives (main camera, no rig offsets) for tile centre
  * (px, py) and disparity: xy[4][2]; deriv (optional) [8][14]: columns 0..12 the reference's
  * pXYderiv, column 13 d/d disparity (not in the reference; from the same chain). */
-static void port_coords(const clt_port_geometry* p, const double rots[4][9], const double drots[4][4][9],
-                        const double* tab, double px, double py, double disparity, double* xy, double* deriv) {
-    const double pXcd = px - 0.5 * p->width, pYcd = py - 0.5 * p->height;
-    const double rD = sqrt(pXcd * pXcd + pYcd * pYcd) * 0.001 * p->pixel_size;
-    const double rND2R = r_by_rdist(tab, rD / p->distortion_radius);
+static void port_coords(const clt_port_geometry* pm, const double* tab, const clt_port_geometry* p,
+                        const double (*rxy)[2], const double rots[4][9], const double drots[4][4][9], double px,
+                        double py, double disparity, double* xy, double* deriv) {
+    /* the tile centre is undistorted by the main camera's model (gc_main, GC:2179-2184); rxy: the
+     * port offsets of this camera (with rig offsets for the auxiliary one, GC:2177) */
+    const double pXcd = px - 0.5 * pm->width, pYcd = py - 0.5 * pm->height;
+    const double rD = sqrt(pXcd * pXcd + pYcd * pYcd) * 0.001 * pm->pixel_size;
+    const double rND2R = r_by_rdist(tab, rD / pm->distortion_radius);
     const double pXc = pXcd * rND2R, pYc = pYcd * rND2R;
     const double fl_pix = p->focal_length / (0.001 * p->pixel_size);
     const double ri_scale = 0.001 * p->pixel_size / p->distortion_radius;
     if (deriv) memset(deriv, 0, sizeof(double) * 8 * 14);
     for (int i = 0; i < 4; i++) {
-        const double vi[3] = {pXc - disparity * p->rxy[i][0], pYc - disparity * p->rxy[i][1], fl_pix};
+        const double vi[3] = {pXc - disparity * rxy[i][0], pYc - disparity * rxy[i][1], fl_pix};
         const double* R = rots[i];
         const double rvi[3] = {R[0] * vi[0] + R[1] * vi[1] + R[2] * vi[2], R[3] * vi[0] + R[4] * vi[1] + R[5] * vi[2],
                                R[6] * vi[0] + R[7] * vi[1] + R[8] * vi[2]};
@@ -1812,7 +1815,7 @@ static void port_coords(const clt_port_geometry* p, const double rots[4][9], con
                 const double* D = drots[i][a];
                 for (int r = 0; r < 3; r++) dv[r] = D[r * 3] * vi[0] + D[r * 3 + 1] * vi[1] + D[r * 3 + 2] * vi[2];
             } else {   /* d vi / d disparity = (-rx, -ry, 0) */
-                for (int r = 0; r < 3; r++) dv[r] = -(R[r * 3] * p->rxy[i][0] + R[r * 3 + 1] * p->rxy[i][1]);
+                for (int r = 0; r < 3; r++) dv[r] = -(R[r * 3] * rxy[i][0] + R[r * 3 + 1] * rxy[i][1]);
             }
             const double dxc = dv[0] * norm_z - pXci * dv[2] / rvi[2];
             const double dyc = dv[1] * norm_z - pYci * dv[2] / rvi[2];
@@ -1864,8 +1867,57 @@ EXPORT void orc_port_coords(const clt_port_geometry* p, int n, const double* px,
     rot_matrices(p, r, d);
     reverse_distortion_table(p, tab);
     for (int t = 0; t < n; t++)
-        port_coords(p, (const double(*)[9])r, (const double(*)[4][9])d, tab, px[t], py[t], disparity[t], xy + 8 * t,
-                    deriv ? deriv + 8 * 14 * t : NULL);
+        port_coords(p, tab, p, (const double(*)[2])p->rxy, (const double(*)[9])r, (const double(*)[4][9])d, px[t], py[t],
+                    disparity[t], xy + 8 * t, deriv ? deriv + 8 * 14 * t : NULL);
+    free(tab);
+}
+
+/* GC:770-803 RigOffset.getRotMatrix: roll (with zoom) * tilt * azimuth of the auxiliary camera */
+static void rig_matrix(const clt_rig_offset* ro, double* m) {
+    const double ca = cos(ro->aux_azimuth), sa = sin(ro->aux_azimuth);
+    const double ct = cos(ro->aux_tilt), st = sin(ro->aux_tilt);
+    const double zoom = 1.0 + ro->aux_zoom, cr = cos(ro->aux_roll) * zoom, sr = sin(ro->aux_roll) * zoom;
+    const double a_az[9] = {ca, 0, -sa, 0, 1, 0, sa, 0, ca};   /* ROT_AZ_SGN = -1 */
+    const double a_t[9] = {1, 0, 0, 0, ct, st, 0, -st, ct};
+    const double a_r[9] = {cr, sr, 0, -sr, cr, 0, 0, 0, 1};
+    double t[9];
+    mat3_mul(a_t, a_az, t);
+    mat3_mul(a_r, t, m);
+}
+
+/* GC:728-745 RigOffset.recalcRXY: the auxiliary ports per nominal disparity pixel of the auxiliary
+ * camera: its own rXY rolled by aux_roll, plus the baseline vector over its disparity radius */
+static void rig_rxy(const clt_port_geometry* aux, const clt_rig_offset* ro, double rxy[4][2]) {
+    const double xc = ro->baseline * cos(ro->aux_angle) / ro->aux_disparity_radius;
+    const double yc = ro->baseline * sin(ro->aux_angle) / ro->aux_disparity_radius;
+    const double c = cos(ro->aux_roll), sn = sin(ro->aux_roll);
+    for (int i = 0; i < 4; i++) {
+        rxy[i][0] = xc + c * aux->rxy[i][0] + sn * aux->rxy[i][1];
+        rxy[i][1] = yc - sn * aux->rxy[i][0] + c * aux->rxy[i][1];
+    }
+}
+
+/* clt_bi_quad's port centres (ID:8970-8993): the main quad as getPortsCoordinatesAndDerivatives
+ * of the main camera at the target disparity; the auxiliary quad with use_rig_offsets (GC:2159-2230
+ * with the main camera's undistortion, rXY_aux, CorrVector rotations times the rig matrix,
+ * getRotMatrices(rigMatrix) GC:1016-1026) at disparity * R_aux / R_main.  xy: [n][16]. */
+EXPORT void orc_port_coords_rig(const clt_port_geometry* main, const clt_port_geometry* aux, const clt_rig_offset* ro,
+                                int n, const double* px, const double* py, const double* disparity, double* xy) {
+    double rm[4][9], ra[4][9], rig[9], rxy_aux[4][2];
+    double* tab = malloc(sizeof(double) * RDIST_LEN);
+    rot_matrices(main, rm, NULL);
+    rot_matrices(aux, ra, NULL);
+    rig_matrix(ro, rig);
+    for (int i = 0; i < 4; i++) mat3_mul(ra[i], rig, ra[i]);
+    rig_rxy(aux, ro, rxy_aux);
+    reverse_distortion_table(main, tab);
+    const double dscale = ro->aux_disparity_radius / ro->main_disparity_radius;
+    for (int t = 0; t < n; t++) {
+        port_coords(main, tab, main, (const double(*)[2])main->rxy, (const double(*)[9])rm, NULL, px[t], py[t],
+                    disparity[t], xy + 16 * t, NULL);
+        port_coords(main, tab, aux, (const double(*)[2])rxy_aux, (const double(*)[9])ra, NULL, px[t], py[t],
+                    disparity[t] * dscale, xy + 16 * t + 8, NULL);
+    }
     free(tab);
 }
 

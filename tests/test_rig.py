@@ -275,3 +275,99 @@ def test_gpu_rig_textures(pk, orc):
             assert res[name] < 1e-4 and res[name + "_flips"] <= 1e-3, name
         assert res["stats"] < 1e-4
 
+
+
+# ------------------------------------------------------------------------------------- rig geometry
+def _geoms(orc, **aux_kw):
+    main = orc.default_port_geometry(640, 480)
+    aux = orc.default_port_geometry(640, 480, **aux_kw)
+    return main, aux
+
+
+def test_rig_ports_identity(orc):
+    """An auxiliary camera equal to the main one, no baseline, no rig rotation: the same ports."""
+    main, aux = _geoms(orc)
+    main.distortion[1] = aux.distortion[1] = 0.02
+    main.extrinsic[3] = aux.extrinsic[3] = 1e-3
+    rig = orc.RigOffset(baseline=0.0, main_disparity_radius=150.0, aux_disparity_radius=150.0)
+    px, py, d = np.array([100.0, 333.0]), np.array([50.0, 401.0]), np.array([0.0, 7.5])
+    xy = orc.port_coords_rig(main, aux, rig, px, py, d)
+    assert np.array_equal(xy[:, :4], xy[:, 4:])
+    single = orc.port_coords(main, px, py, d)
+    assert np.allclose(xy[:, :4].reshape(2, 8), np.asarray(single).reshape(2, 8), rtol=0, atol=1e-12)
+
+
+def test_rig_ports_closed_form(orc):
+    """No distortion, no CorrVector rotations: the auxiliary port of sensor i is
+    R_roll(zoom) . (p - d R_a/R_m rXY_aux[i]) + pXY0_aux with rXY_aux the rolled quad offsets plus the
+    baseline vector over R_a (GC:728-745, 770-803, 2159-2230); an aux azimuth alpha projects
+    (x, y, f) through the rotation about the vertical axis."""
+    B, th, rho, zm, Rm, Ra = 1256.0, 0.3, 0.02, 0.01, 150.0, 120.0
+    main, aux = _geoms(orc, focal_length=5.0, pxy0=[(300.0, 250.0)] * 4)
+    rig = orc.RigOffset(baseline=B, aux_angle=th, aux_roll=rho, aux_zoom=zm, main_disparity_radius=Rm,
+                        aux_disparity_radius=Ra)
+    px, py, d = np.array([400.0, 123.0]), np.array([200.0, 377.0]), np.array([2.0, 0.5])
+    xy = orc.port_coords_rig(main, aux, rig, px, py, d)
+    q = np.array(orc.QUAD_PORT_OFFSETS)
+    c, s = np.cos(rho), np.sin(rho)
+    rxy = np.stack([B * np.cos(th) / Ra + c * q[:, 0] + s * q[:, 1], B * np.sin(th) / Ra - s * q[:, 0] + c * q[:, 1]], 1)
+    for t in range(2):
+        v = np.array([px[t] - 320.0, py[t] - 240.0])[None, :] - d[t] * Ra / Rm * rxy
+        cz, sz = c * (1 + zm), s * (1 + zm)
+        exp = np.stack([cz * v[:, 0] + sz * v[:, 1], -sz * v[:, 0] + cz * v[:, 1]], 1) + 300.0 * np.array([1.0, 250 / 300])
+        assert np.allclose(xy[t, 4:], exp, rtol=0, atol=1e-9)
+    al = 0.01
+    rig = orc.RigOffset(baseline=B, aux_azimuth=al, main_disparity_radius=Rm, aux_disparity_radius=Ra)
+    xy = orc.port_coords_rig(main, aux, rig, px, py, d)
+    fl = 5.0 / (0.001 * 2.2)
+    rxy = np.stack([B / Ra + q[:, 0], q[:, 1]], 1)
+    for t in range(2):
+        v = np.array([px[t] - 320.0, py[t] - 240.0])[None, :] - d[t] * Ra / Rm * rxy
+        rz = np.sin(al) * v[:, 0] + np.cos(al) * fl
+        ex = (np.cos(al) * v[:, 0] - np.sin(al) * fl) * fl / rz + 300.0
+        ey = v[:, 1] * fl / rz + 250.0
+        assert np.allclose(xy[t, 4:, 0], ex, atol=1e-9) and np.allclose(xy[t, 4:, 1], ey, atol=1e-9)
+
+
+@pytest.mark.gpu
+def test_gpu_rig_port_centres(pk, orc):
+    log("case: rig port centres on the device, 640x480, distortion, extrinsics, rig offsets")
+    main, aux = _geoms(orc, focal_length=4.6, pxy0=[(322.0, 238.5), (318.2, 241.0), (320.5, 239.9), (321.1, 240.2)])
+    for g in (main, aux):
+        g.distortion[0], g.distortion[1], g.distortion[2] = 0.01, -0.03, 0.015
+    main.extrinsic[0], main.extrinsic[4], main.extrinsic[7], main.extrinsic[11] = 2e-4, -3e-4, 1e-3, 5e-4
+    aux.extrinsic[1], aux.extrinsic[3], aux.extrinsic[8], aux.extrinsic[10] = -1e-4, 2e-4, -5e-4, 3e-4
+    kw = dict(baseline=1256.0, aux_angle=0.05, aux_azimuth=2e-3, aux_tilt=-1e-3, aux_roll=4e-3, aux_zoom=1e-3,
+              main_disparity_radius=150.0, aux_disparity_radius=120.0)
+    rig = orc.RigOffset(**kw)
+    tp = pk.GPUTileProcessor(640, 480, 8, 3, kernels=False)
+    gm = pk.GPUTileProcessor.defaultPortGeometry(640, 480)
+    ga = pk.GPUTileProcessor.defaultPortGeometry(640, 480)
+    for dst, src in ((gm, main), (ga, aux)):
+        for f, _ in src._fields_:
+            v = getattr(src, f)
+            if hasattr(v, "__len__"):
+                for i in range(len(v)):
+                    if hasattr(v[i], "__len__"):
+                        for j in range(len(v[i])):
+                            getattr(dst, f)[i][j] = v[i][j]
+                    else:
+                        getattr(dst, f)[i] = v[i]
+            else:
+                setattr(dst, f, v)
+    tp.setPortGeometryRig(gm, ga, pk.GPUTileProcessor.rigOffset(**kw))
+    rng = np.random.default_rng(3)
+    n = 600
+    tx, ty = rng.integers(0, 80, n), rng.integers(0, 60, n)
+    d = rng.uniform(0, 20, n).astype(np.float32)
+    words = np.full(n, 0xff, np.int32)
+    words[::7] = 0
+    tp.getTileSubcamOffsets(words, tx, ty, d)
+    got = tp.getTasks()[:, 2:].view(np.float32).reshape(n, 8, 2)
+    tp.close()
+    exp = orc.port_coords_rig(main, aux, rig, tx * 8 + 4.0, ty * 8 + 4.0, d.astype(np.float64)).astype(np.float32)
+    live = words != 0
+    log("  max |d| %.3g px, identical %d of %d" % (np.abs(got[live] - exp[live]).max(),
+                                                    int((got[live] == exp[live]).all(axis=(1, 2)).sum()), live.sum()))
+    assert np.array_equal(got[live], exp[live])
+    assert (got[~live] == 0).all()
