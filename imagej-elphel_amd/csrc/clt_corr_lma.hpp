@@ -277,16 +277,25 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             for (int q = 0; q < NPM; q++) raw[k][q] = (ing && q < npairs) ? raw[k][q] : 0.f;
         }
         // the reference divides the sum by the pair count; for 2 and 4 a product is the same number
-        const bool pow2 = (npairs & (npairs - 1)) == 0;
-        const double rn = 1.0 / npairs;
+        if ((npairs & (npairs - 1)) == 0) {   // n = 1, 2, 4: the product by 1/n is the division exactly
+            const double rn = (npairs == 4) ? 0.25 : ((npairs == 2) ? 0.5 : 1.0);
 #pragma unroll
-        for (int k = 0; k < CPL; k++) {
-            const int c = fl + W * k;
-            double val = 0.0;
+            for (int k = 0; k < CPL; k++) {
+                const int c = fl + W * k;
+                double val = 0.0;
 #pragma unroll
-            for (int q = 0; q < NPM; q++) val += (double)raw[k][q];
-            if (npairs > 1) val = pow2 ? val * rn : val / npairs;
-            if (c < LMA_CELLS) v[c] = val;
+                for (int q = 0; q < NPM; q++) val += (double)raw[k][q];
+                if (c < LMA_CELLS) v[c] = val * rn;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < CPL; k++) {
+                const int c = fl + W * k;
+                double val = 0.0;
+#pragma unroll
+                for (int q = 0; q < NPM; q++) val += (double)raw[k][q];
+                if (c < LMA_CELLS) v[c] = val / npairs;
+            }
         }
         wave_sync();
         // bi-convex cells inside the clamped window (C2:2046-2064), in box coordinates (box row br is
@@ -708,13 +717,15 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
     if (t >= num_tasks) return;   // whole rows: the reductions never see an exited lane
     float* rec = disp + (size_t)t * CLT_DISP_LAYERS;
     const float cm = rec[CLT_DISP_CM_RAW], st = rec[CLT_DISP_STR_RAW];
-    const int toff = t * A.npairs * CLT_CORR_LEN;
+    // the wavefront's first tile as a 64-bit base, the tile within it as a small 32-bit offset
+    const float* corr_w = corr + (size_t)__builtin_amdgcn_readfirstlane(t - sub) * A.npairs * CLT_CORR_LEN;
+    const int toff = sub * A.npairs * CLT_CORR_LEN;
     double out_cm = cm, out_st = st, poly = NAN, poly_s = NAN;
     double disparity = cm, strength = st;
     bool dirs = false;
     if (!isnan(cm) && (double)st > A.min_poly_strength) {   // the fit over all oriented pairs
         const LmaLds S = {s_v[slot], s_idx[slot], s_acc[slot]};
-        const LmaOut r = lma_fit<16, NC, LMA_GPAIRS>(A, &A.fit[0], corr, toff, -(double)cm, l, lane,
+        const LmaOut r = lma_fit<16, NC, LMA_GPAIRS>(A, &A.fit[0], corr_w, toff, -(double)cm, l, lane,
                                                      0xffffull << (16 * sub), S);
         if (r.ok) {
             poly = r.disp;
@@ -733,7 +744,7 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
         const LmaLds S = {s_v[slot] + half * LMA_CELLS, s_idx[slot] + half * LMA_CELLS, s_acc[slot] + half * LMA_ACC};
         for (int pass = 0; pass < 2; pass++) {
             const int f = 1 + 2 * pass + half;
-            const LmaOut q = lma_fit<8, NC, LMA_DPAIRS>(A, &A.fit[f], corr, toff, -disparity, hl, lane,
+            const LmaOut q = lma_fit<8, NC, LMA_DPAIRS>(A, &A.fit[f], corr_w, toff, -disparity, hl, lane,
                                                         0xffull << (16 * sub + 8 * half), S);
             if (hl == 0) {
                 s_dds[slot][f - 1][0] = q.ok ? q.disp : NAN;
@@ -800,7 +811,8 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma_inter(const float* __restri
     if (!isnan(x0) && strength > A.min_poly_strength && !notch) {
         const LmaLds S = {s_v[slot], s_idx[slot], s_acc[slot]};
         const uint64_t fmask = 0xffffull << (16 * sub);
-        const LmaOut q = lma_fit<16, NC, 1>(A, &A.fit[LMA_FIT_SINGLE], corr, t * CLT_CORR_LEN, (double)x0, l, lane,
+        const float* corr_w = corr + (size_t)__builtin_amdgcn_readfirstlane(t - sub) * CLT_CORR_LEN;   // wave base
+        const LmaOut q = lma_fit<16, NC, 1>(A, &A.fit[LMA_FIT_SINGLE], corr_w, sub * CLT_CORR_LEN, (double)x0, l, lane,
                                             fmask, S);
         if (q.ok) {
             if (use_poly) {
@@ -808,8 +820,8 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma_inter(const float* __restri
                 strength = q.strength;
             }
             if (use_xy_poly) {   // single2dPoly (C2:1509-1536) around 0
-                const LmaOut pq = lma_fit<16, NC, 1, true>(A, &A.fit[LMA_FIT_SINGLE], corr, t * CLT_CORR_LEN, 0.0, l,
-                                                           lane, fmask, S);
+                const LmaOut pq = lma_fit<16, NC, 1, true>(A, &A.fit[LMA_FIT_SINGLE], corr_w, sub * CLT_CORR_LEN, 0.0,
+                                                           l, lane, fmask, S);
                 if (pq.ok) {
                     dx = -pq.disp;
                     dy = -pq.wm;
