@@ -160,6 +160,19 @@ class GPUTileProcessor:
         self.num_task_tiles = n
         self._tasks = None
 
+    def setFullFrameImages(self, target_disparity, task=0xFF):
+        """GTP:472-527: one task per tile of the frame at the given disparity (a scalar or
+        [tilesY * tilesX] values), port centres from the port geometry (setPortGeometry or
+        setPortGeometryRig) on the device, then set as the task list.  The reference writes the task
+        word 1 (any non-zero word converts a tile); here bits 0..7 also select the ports the textures
+        combine, so the default uses all of them.  Returns the task records."""
+        tiles_x, tiles_y = self.IMG_WIDTH // self.DTT_SIZE, self.IMG_HEIGHT // self.DTT_SIZE
+        ty, tx = np.mgrid[0:tiles_y, 0:tiles_x]
+        d = np.broadcast_to(np.asarray(target_disparity, np.float32), (tiles_y * tiles_x,))
+        words = np.full(tiles_y * tiles_x, task, np.int32)
+        self.getTileSubcamOffsets(words, tx.ravel(), ty.ravel(), d)
+        return self.getTasks()
+
     def getTasks(self):
         """The current task records [n][2+2C] (int32; port centres as float bits)."""
         out = np.zeros((self.num_task_tiles, 2 + 2 * self.NUM_CAMS), np.int32)

@@ -127,3 +127,28 @@ def test_device_port_centres_match_oracle(pk, orc):
     print("port derivatives: max relative error %.3g" % rel)
     assert rel < 1e-9
     tp.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("per_tile", [False, True])
+def test_set_full_frame_images(pk, orc, per_tile):
+    """setFullFrameImages (GTP:472-527): every tile of the frame in row-major order, at one
+    disparity or one per tile, centres on the device equal to the oracle's port coordinates."""
+    w, h = 320, 240
+    tiles_x, tiles_y = w // 8, h // 8
+    n = tiles_x * tiles_y
+    d = np.linspace(0.0, 12.0, n).astype(np.float32) if per_tile else 2.5
+    tp = pk.GPUTileProcessor(w, h, 4, 3, kernels=False)
+    kw = dict(distortion=DIST, extrinsic=RNG_EXTR)
+    tp.setPortGeometry(pk.GPUTileProcessor.defaultPortGeometry(w, h, **kw))
+    rec = tp.setFullFrameImages(d)
+    tp.close()
+    assert rec.shape == (n, 10) and (rec[:, 0] == 0xFF).all()
+    tx, ty = rec[:, 1] & 0xFFFF, rec[:, 1] >> 16
+    assert np.array_equal(ty * tiles_x + tx, np.arange(n))
+    o_xy = orc.port_coords(orc.default_port_geometry(w, h, **kw), tx * 8 + 4.0, ty * 8 + 4.0,
+                           np.broadcast_to(np.asarray(d, np.float64), (n,)))
+    g_xy = rec[:, 2:].view(np.float32).reshape(-1, 4, 2)
+    err = np.abs(g_xy - o_xy.astype(np.float32)).max()
+    print("full frame port centres: max |gpu - oracle| = %.3g px over %d tiles" % (err, n))
+    assert err <= 1e-3
