@@ -1696,7 +1696,7 @@ struct TexArgs {
 // final mix ImageDtt.java:2789-2801).  ALL: every port of the NC is used (no per-port masking).
 template <int NC, bool ALL>
 __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, const float (&deb)[NC][3], float w2,
-                                              float iw2, const float* s_nw, int c0, float* out, int i, int planes,
+                                              float iw2, const float* s_nw, float* out, int i, int planes,
                                               float (&avg)[3]) {
     auto on = [&](int ip) { return ALL || ((port_mask >> ip) & 1); };
     int used = 0;
@@ -1832,7 +1832,7 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
     for (int chn = 0; chn < 3; chn++) {
         float s = 0.f;
 #pragma unroll
-        for (int ip = 0; ip < NC; ip++) s += pw[ip] * s_nw[(ip * 3 + chn) * 324 + c0];
+        for (int ip = 0; ip < NC; ip++) s += pw[ip] * s_nw[(ip * 3 + chn) * 256 + i];
         out[chn * 256 + i] = sw * s;
     }
     if (planes > 4) {   // keep_weights: weights, then the rms of the ports (ImageDtt.java:2858-2874)
@@ -1858,132 +1858,148 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
     }
 }
 
+// LDS of one texture tile.  scr: the IMCLT scratch of each 16-lane unit; afterwards, when every
+// unit fits one pass (NU <= 16), unit u's debayered tile.  nw: the window-free tiles (the IMCLT
+// tiles divided by the squared window), pixel-major [unit][16 * row + col].
 template <int NC>
-__global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
-                                                  int task_words, int num_tasks, const float* __restrict__ lpf,
-                                                  const float* __restrict__ extra_disp, const TexArgs* __restrict__ args_p,
-                                                  float* __restrict__ tiles_out, float* __restrict__ stats_out,
-                                                  const int32_t* __restrict__ task_frame, long long clt_fstride, GeomDev g) {
-    constexpr int NU = NC * 3;   // camera/colour units
-    __shared__ __attribute__((aligned(16))) float s_scr[16][IMCLT_SCR_UNIT];
-    __shared__ float s_nw[NU][18 * 18];   // window-free tiles with a zero border for the 3x3 debayer
-    const TexArgs& A = *args_p;
-    const int t = xcd_block(blockIdx.x, gridDim.x);
-    if (t >= num_tasks) return;
-    const int32_t* rec = tasks + (size_t)t * task_words;
-    const int flags = rec[0];
-    const int planes = A.planes;
-    float* out = tiles_out + (size_t)t * planes * 256;
-    const int i = threadIdx.x;   // pixel index in the 16x16 tile
-    if (flags == 0) {
-        const float qnan = __int_as_float(0x7fc00000);
-        for (int p = 0; p < planes; p++) out[p * 256 + i] = qnan;
-        if (stats_out && i < 4 * NC) stats_out[(size_t)t * 4 * NC + i] = qnan;
-        return;
-    }
-    const int port_mask = flags & 0xff & ((1 << NC) - 1);
-    const int txy = rec[1];
-    const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
-    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
-    if (task_frame) clt += task_frame[t] * clt_fstride;   // batched frames
-    float extra = extra_disp ? extra_disp[t] : 0.f;
-    if (isnan(extra)) extra = 0.f;
-    const bool do_shift = (extra != 0.f) && !((flags >> 24) & 1);
-    // zero the border ring of the padded tiles
-    for (int k = i; k < NU * 68; k += 256) {
-        const int uu = k / 68, b = k % 68;
-        const int pos = (b < 18) ? b : (b < 36) ? (17 * 18 + b - 18) : (b < 52) ? ((b - 36 + 1) * 18) : ((b - 52 + 1) * 18 + 17);
-        s_nw[uu][pos] = 0.f;
-    }
-    // ---- IMCLT of the NC*3 units (16 lanes each) after LPF * 0.25 (ImageDtt.java:2684-2716)
-    {
-        const int unit = threadIdx.x >> 4, l = threadIdx.x & 15, hs = l >> 3, kh = l & 7;
-        for (int ub = 0; ub < NU; ub += 16) {
-            if (ub + (threadIdx.x >> 6) * 4 >= NU) continue;   // wavefront without a unit
-            const int uu = ub + unit;
-            const bool act = uu < NU;
-            const int cam = act ? uu / 3 : 0, color = act ? uu % 3 : 0;
-            const float* src = clt + (((size_t)cam * ntiles + tile) * 3 + color) * 256 + kh * 8;
-            float p[8], q[8];
-            {
-                float4 v0 = *reinterpret_cast<const float4*>(src + hs * 64);
-                float4 v1 = *reinterpret_cast<const float4*>(src + hs * 64 + 4);
-                float4 v2 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64);
-                float4 v3 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64 + 4);
-                p[0] = v0.x; p[1] = v0.y; p[2] = v0.z; p[3] = v0.w; p[4] = v1.x; p[5] = v1.y; p[6] = v1.z; p[7] = v1.w;
-                q[0] = v2.x; q[1] = v2.y; q[2] = v2.z; q[3] = v2.w; q[4] = v3.x; q[5] = v3.y; q[6] = v3.z; q[7] = v3.w;
-            }
-            if (do_shift) {   // fract_shift by extra * port_offset / corr_magic_scale (ID:2663-2677)
-                const float sxp = extra * A.port_offsets[cam][0] / A.corr_magic_scale;
-                const float syp = extra * A.port_offsets[cam][1] / A.corr_magic_scale;
-                const float sigma = hs ? 1.f : -1.f;
-                float sh, ch, cv[8], sv[8];
-                sincos_pi((kh + 0.5f) * sxp * 0.125f, &sh, &ch);
-                half_phase8(syp * 0.125f, cv, sv);
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const float pa = lane_xor8(p[k]) * sigma;
-                    const float pb = lane_xor8(q[k]) * sigma;
-                    const float ta = p[k] * ch + pa * sh;
-                    const float tb = q[k] * ch + pb * sh;
-                    p[k] = ta * cv[k] - tb * sv[k];
-                    q[k] = tb * cv[k] + ta * sv[k];
-                }
-            }
+struct TexShared {
+    static constexpr int NU = NC * 3;
+    static constexpr bool DEB_IN_SCR = NU <= 16;
+    float scr[16][IMCLT_SCR_UNIT];
+    float nw[NU][256];
+    float deb[DEB_IN_SCR ? 1 : NU][256];
+    __device__ __forceinline__ float* deb_of(int u) { return DEB_IN_SCR ? &scr[u][0] : &deb[u][0]; }
+};
+
+// Value of the lane one row above / below within the 16-lane unit (DPP row shifts; 0 past the
+// tile's first / last row, which is the zero border of the 3x3 debayer).  Every lane active.
+__device__ __forceinline__ float unit_row_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float unit_row_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf, false));
+}
+
+// IMCLT of the NC*3 camera/colour units of one tile after LPF * 0.25 (ImageDtt.java:2684-2716),
+// with the optional fract_shift by the extra disparity (ID:2663-2677), then the debayer with
+// shot-noise correction and the re-applied window (tile_debayer_shot_corr, ID:3022-3147) on the
+// unit's lanes: lane l holds output row l, the vertical taps come from the neighbouring lanes, the
+// horizontal ones from the lane's own row; taps outside the tile are dropped and the kernel sum
+// renormalised (corner / side factors).  Leaves S.nw and the debayered tiles in LDS; sink(uu, l,
+// row) receives each unit row of the windowed IMCLT tile (the imclt_rbg tile) while it is in
+// registers.  All 256 threads call this.
+template <int NC, class Sink>
+__device__ __forceinline__ void tex_units(const float* __restrict__ clt, size_t tile, size_t ntiles,
+                                          const float* __restrict__ lpf, const TexArgs& A, float extra, bool do_shift,
+                                          TexShared<NC>& S, Sink sink) {
+    constexpr int NU = NC * 3;
+    const int unit = threadIdx.x >> 4, l = threadIdx.x & 15, hs = l >> 3, kh = l & 7;
+    // 3x3 kernels (ImageDtt.java:43-50) with the renormalisation of the dropped taps folded in:
+    // kern_rb = (1 2 1) x (1 2 1) / 16 (tap sums 9/16 corner, 12/16 side), kern_g = cross (0.75,
+    // 0.875); per lane the factors of an inner (k_in) and an edge column (k_out) of its row
+    const bool row_edge = (l == 0) || (l == 15);
+    const float kshot = fast_rcp(fast_sqrt(A.min_shot));
+    const float wy = c_tab.hw2[l < 8 ? l : 15 - l], iwy = c_tab.ihw2[l < 8 ? l : 15 - l];
+    for (int ub = 0; ub < NU; ub += 16) {
+        if (ub + (threadIdx.x >> 6) * 4 >= NU) continue;   // wavefront without a unit
+        const int uu = ub + unit;
+        const bool act = uu < NU;
+        const int cam = act ? uu / 3 : 0, color = act ? uu % 3 : 0;
+        const float* src = clt + (((size_t)cam * ntiles + tile) * 3 + color) * 256 + kh * 8;
+        float p[8], q[8];
+        {
+            float4 v0 = *reinterpret_cast<const float4*>(src + hs * 64);
+            float4 v1 = *reinterpret_cast<const float4*>(src + hs * 64 + 4);
+            float4 v2 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64);
+            float4 v3 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64 + 4);
+            p[0] = v0.x; p[1] = v0.y; p[2] = v0.z; p[3] = v0.w; p[4] = v1.x; p[5] = v1.y; p[6] = v1.z; p[7] = v1.w;
+            q[0] = v2.x; q[1] = v2.y; q[2] = v2.z; q[3] = v2.w; q[4] = v3.x; q[5] = v3.y; q[6] = v3.z; q[7] = v3.w;
+        }
+        if (do_shift) {   // fract_shift by extra * port_offset / corr_magic_scale (ID:2663-2677)
+            const float sxp = extra * A.port_offsets[cam][0] / A.corr_magic_scale;
+            const float syp = extra * A.port_offsets[cam][1] / A.corr_magic_scale;
+            const float sigma = hs ? 1.f : -1.f;
+            float sh, ch, cv[8], sv[8];
+            sincos_pi((kh + 0.5f) * sxp * 0.125f, &sh, &ch);
+            half_phase8(syp * 0.125f, cv, sv);
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const float f = 0.25f * lpf[kh * 8 + k];
-                p[k] *= f;
-                q[k] *= f;
-            }
-            float row[16];
-            imclt_unit(l, p, q, s_scr[unit], row);
-            if (act) {
-                const float iwy = c_tab.ihw2[l < 8 ? l : 15 - l];
-#pragma unroll
-                for (int c = 0; c < 16; c++)   // un-apply the squared window
-                    s_nw[uu][(l + 1) * 18 + c + 1] = row[c] * (iwy * c_tab.ihw2[c < 8 ? c : 15 - c]);
+                const float pa = lane_xor8(p[k]) * sigma;
+                const float pb = lane_xor8(q[k]) * sigma;
+                const float ta = p[k] * ch + pa * sh;
+                const float tb = q[k] * ch + pb * sh;
+                p[k] = ta * cv[k] - tb * sv[k];
+                q[k] = tb * cv[k] + ta * sv[k];
             }
         }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float f = 0.25f * lpf[kh * 8 + k];
+            p[k] *= f;
+            q[k] *= f;
+        }
+        float row[16];
+        imclt_unit(l, p, q, S.scr[unit], row);   // the unit's scratch is free again on return
+        if (act) sink(uu, l, row);
+        float nw[16];
+#pragma unroll
+        for (int c = 0; c < 16; c++) nw[c] = row[c] * (iwy * c_tab.ihw2[c < 8 ? c : 15 - c]);
+        if (act) {
+#pragma unroll
+            for (int c = 0; c < 16; c += 4)
+                *reinterpret_cast<float4*>(&S.nw[uu][l * 16 + c]) = make_float4(nw[c], nw[c + 1], nw[c + 2], nw[c + 3]);
+        }
+        const bool green = color == 2;
+        const float f_in = green ? (row_edge ? 8.f / 7.f : 1.f) : (row_edge ? 4.f / 3.f : 1.f);
+        const float f_out = green ? (row_edge ? 4.f / 3.f : 8.f / 7.f) : (row_edge ? 16.f / 9.f : 4.f / 3.f);
+        const float k_corner = green ? 0.f : 0.0625f, k_side = 0.125f, k_centre = green ? 0.5f : 0.25f;
+        const float kc_in = k_centre * f_in, ks_in = k_side * f_in, kk_in = k_corner * f_in;
+        const float kc_out = k_centre * f_out, ks_out = k_side * f_out, kk_out = k_corner * f_out;
+        float vs[16];   // sum of the pixels above and below
+#pragma unroll
+        for (int c = 0; c < 16; c++) vs[c] = unit_row_prev(nw[c]) + unit_row_next(nw[c]);
+        float deb[16];
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+            float side = vs[c], corner = 0.f;
+            if (c > 0) { side += nw[c - 1]; corner += vs[c - 1]; }
+            if (c < 15) { side += nw[c + 1]; corner += vs[c + 1]; }
+            const bool edge = (c == 0 || c == 15);
+            float s = (edge ? kc_out : kc_in) * nw[c] + (edge ? ks_out : ks_in) * side + (edge ? kk_out : kk_in) * corner;
+            if (A.scale_shot > 0.f) s = A.scale_shot * ((s > A.min_shot) ? fast_sqrt(s) : (kshot * s));
+            deb[c] = s * (wy * c_tab.hw2[c < 8 ? c : 15 - c]);
+        }
+        if (act) {
+            float* d = S.deb_of(uu);
+#pragma unroll
+            for (int c = 0; c < 16; c += 4)
+                *reinterpret_cast<float4*>(&d[l * 16 + c]) = make_float4(deb[c], deb[c + 1], deb[c + 2], deb[c + 3]);
+        }
     }
-    __syncthreads();
-    // ---- debayer with shot-noise correction and re-applied window (ID:3022-3147); the zero border
-    // plays the role of the dropped taps, the corner/side factors renormalise the kernel sums
+}
+
+// Port combination (pixel per thread) and the tile statistics of one texture tile, from the LDS
+// that tex_units left (the caller's barrier between them).  All 256 threads call this.
+template <int NC>
+__device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, TexShared<NC>& S, float* __restrict__ out,
+                                            float* __restrict__ stats_t) {
+    const int i = threadIdx.x;   // pixel index in the 16x16 tile
+    const int planes = A.planes;
     const int py = i >> 4, px = i & 15;
     const int qy = py < 8 ? py : 15 - py, qx = px < 8 ? px : 15 - px;
     const float w2 = c_tab.hw2[qy] * c_tab.hw2[qx];
     const float iw2 = c_tab.ihw2[qy] * c_tab.ihw2[qx];
-    const int nb = (py == 0 || py == 15) + (px == 0 || px == 15);   // 0 inside, 1 side, 2 corner
-    const float k_rb = (nb == 2) ? (16.f / 9.f) : (nb == 1) ? (4.f / 3.f) : 1.f;
-    const float k_g = (nb == 2) ? (4.f / 3.f) : (nb == 1) ? (8.f / 7.f) : 1.f;
     float deb[NC][3];
-    const int c0 = (py + 1) * 18 + px + 1;
-    {
-        const float kshot = fast_rcp(fast_sqrt(A.min_shot));
 #pragma unroll
-        for (int uu = 0; uu < NU; uu++) {
-            const float* nw = s_nw[uu];
-            float s;
-            if (uu % 3 != 2) {   // kern_rb = (1 2 1)x(1 2 1)/16 (ImageDtt.java:47-50)
-                s = 0.0625f * (nw[c0 - 19] + nw[c0 - 17] + nw[c0 + 17] + nw[c0 + 19]) +
-                    0.125f * (nw[c0 - 18] + nw[c0 - 1] + nw[c0 + 1] + nw[c0 + 18]) + 0.25f * nw[c0];
-                s *= k_rb;
-            } else {             // kern_g = cross (ImageDtt.java:43-46)
-                s = 0.125f * (nw[c0 - 18] + nw[c0 - 1] + nw[c0 + 1] + nw[c0 + 18]) + 0.5f * nw[c0];
-                s *= k_g;
-            }
-            if (A.scale_shot > 0.f) s = A.scale_shot * ((s > A.min_shot) ? fast_sqrt(s) : (kshot * s));
-            deb[uu / 3][uu % 3] = s * w2;
-        }
-    }
+    for (int uu = 0; uu < NC * 3; uu++) deb[uu / 3][uu % 3] = S.deb_of(uu)[i];
     float avg[3];
-    if (port_mask == (1 << NC) - 1) combine_pixel<NC, true>(A, port_mask, deb, w2, iw2, &s_nw[0][0], c0, out, i, planes, avg);
-    else combine_pixel<NC, false>(A, port_mask, deb, w2, iw2, &s_nw[0][0], c0, out, i, planes, avg);
-    if (stats_out) {   // max_diff (ImageDtt.java:2965-2981) and ports_rgb (ImageDtt.java:2982-2996)
-        // transpose through LDS: value k of pixel i at [k][i] (the NC max_diff rows reuse s_scr, the
-        // 3*NC colour rows reuse s_nw); 16 threads then reduce each row
-        float* red_max = &s_scr[0][0];
-        float* red_rgb = &s_nw[0][0];
+    if (port_mask == (1 << NC) - 1) combine_pixel<NC, true>(A, port_mask, deb, w2, iw2, &S.nw[0][0], out, i, planes, avg);
+    else combine_pixel<NC, false>(A, port_mask, deb, w2, iw2, &S.nw[0][0], out, i, planes, avg);
+    if (stats_t) {   // max_diff (ImageDtt.java:2965-2981) and ports_rgb (ImageDtt.java:2982-2996)
+        // transpose through LDS: value k of pixel i at [k][i] (the NC max_diff rows reuse scr, the
+        // 3*NC colour rows reuse nw); 16 threads then reduce each row
+        float* red_max = &S.scr[0][0];
+        float* red_rgb = &S.nw[0][0];
         __syncthreads();
 #pragma unroll
         for (int ip = 0; ip < NC; ip++) {
@@ -2016,10 +2032,53 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
                     const float u = __shfl_xor(v, o, 16);
                     v = is_max ? fmaxf(v, u) : v + u;
                 }
-                if (j == 0) stats_out[(size_t)t * 4 * NC + k] = is_max ? sqrtf(v) : v / 256.f;
+                if (j == 0) stats_t[k] = is_max ? sqrtf(v) : v / 256.f;
             }
         }
     }
+}
+
+// NaN texture tile and statistics of a task without work (flags 0)
+template <int NC>
+__device__ __forceinline__ void tex_nan(float* __restrict__ out, float* __restrict__ stats_t, int planes) {
+    const float qnan = __int_as_float(0x7fc00000);
+    for (int p = 0; p < planes; p++) out[p * 256 + threadIdx.x] = qnan;
+    if (stats_t && threadIdx.x < 4 * NC) stats_t[threadIdx.x] = qnan;
+}
+
+struct NoRowSink {
+    __device__ __forceinline__ void operator()(int, int, const float*) const {}
+};
+
+template <int NC>
+__global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+                                                  int task_words, int num_tasks, const float* __restrict__ lpf,
+                                                  const float* __restrict__ extra_disp, const TexArgs* __restrict__ args_p,
+                                                  float* __restrict__ tiles_out, float* __restrict__ stats_out,
+                                                  const int32_t* __restrict__ task_frame, long long clt_fstride, GeomDev g) {
+    __shared__ __attribute__((aligned(16))) TexShared<NC> S;
+    const TexArgs& A = *args_p;
+    const int t = xcd_block(blockIdx.x, gridDim.x);
+    if (t >= num_tasks) return;
+    const int32_t* rec = tasks + (size_t)t * task_words;
+    const int flags = rec[0];
+    float* out = tiles_out + (size_t)t * A.planes * 256;
+    float* stats_t = stats_out ? stats_out + (size_t)t * 4 * NC : nullptr;
+    if (flags == 0) {
+        tex_nan<NC>(out, stats_t, A.planes);
+        return;
+    }
+    const int port_mask = flags & 0xff & ((1 << NC) - 1);
+    const int txy = rec[1];
+    const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
+    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+    if (task_frame) clt += task_frame[t] * clt_fstride;   // batched frames
+    float extra = extra_disp ? extra_disp[t] : 0.f;
+    if (isnan(extra)) extra = 0.f;
+    const bool do_shift = (extra != 0.f) && !((flags >> 24) & 1);
+    tex_units<NC>(clt, tile, ntiles, lpf, A, extra, do_shift, S, NoRowSink{});
+    __syncthreads();
+    tex_combine<NC>(A, port_mask, S, out, stats_t);
 }
 
 // ====================================================================================
