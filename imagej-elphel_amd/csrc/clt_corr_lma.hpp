@@ -138,6 +138,55 @@ __device__ __forceinline__ Box81 b_cols(int a, int b) {
     return {COL0_LO * m9, __umul64hi(COL0_LO, m9) + (m9 << 8)};   // row 8 starts at bit 72 = 64 + 8
 }
 
+// ---- the same box as rows on the lanes of a fit (the ring growth of the selection) ------------
+// Lane j = fl & 7 of a fit holds box row j in bits 0..8, lane j = 0 also row 8 in bits 16..24; a
+// 16-lane fit keeps two identical copies (lanes 0..7 and 8..15).  A mask operation is one 32-bit
+// operation and a move by a row is a DPP row shift.  Every lane of the 16-lane DPP row must be
+// active when r9_up, r9_down or r9_to_box run, or at least every lane of the caller's group.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t r9_rows(int j, int a, int b) {   // rows a..b (empty when a > b)
+    uint32_t m = (j >= a && j <= b) ? 0x1ffu : 0u;
+    if (j == 0 && a <= 8 && b >= 8) m |= 0x1ff0000u;
+    return m;
+}
+__device__ __forceinline__ uint32_t r9_cols(int a, int b) {          // columns a..b of every row
+    const uint32_t m = (a > b) ? 0u : (((2u << b) - 1u) & ~((1u << a) - 1u));
+    return m | (m << 16);
+}
+// row r + 1 -> row r (the linear mask shifted right by 9)
+__device__ __forceinline__ uint32_t r9_up(uint32_t x, int j) {
+    const uint32_t nxt = dpp_u32<0x101>(x);   // row_shl:1, lane + 1
+    const uint32_t r8 = dpp_u32<0x117>(x);    // row_shr:7, lane j = 7 reads lane j = 0 of its group
+    return (j == 7) ? (r8 >> 16) : (nxt & 0x1ffu);
+}
+// row r -> row r + 1 (shifted left by 9; row 8 leaves the box)
+__device__ __forceinline__ uint32_t r9_down(uint32_t x, int j) {
+    const uint32_t prv = dpp_u32<0x111>(x);   // row_shr:1, lane - 1
+    const uint32_t r7 = dpp_u32<0x107>(x);    // row_shl:7, lane j = 0 reads lane j = 7 of its group
+    return (j == 0) ? ((r7 & 0x1ffu) << 16) : (prv & 0x1ffu);
+}
+__device__ __forceinline__ uint32_t r9_from_box(Box81 a, int j) {
+    uint32_t r = (j < 7) ? ((uint32_t)(a.lo >> (9 * j)) & 0x1ffu)
+                         : (((uint32_t)(a.lo >> 63) | ((uint32_t)a.hi << 1)) & 0x1ffu);   // bits 63..71
+    if (j == 0) r |= ((uint32_t)(a.hi >> 8) & 0x1ffu) << 16;                              // bits 72..80
+    return r;
+}
+// the linear mask in every lane: OR of the 8 rows of the group (quad_perm [1,0,3,2], [2,3,0,1],
+// row_half_mirror)
+__device__ __forceinline__ Box81 r9_to_box(uint32_t x, int j) {
+    const uint64_t lo = (uint64_t)(x & 0x1ffu) << (9 * j);   // row 7 keeps only bit 63 here
+    uint32_t l0 = (uint32_t)lo, l1 = (uint32_t)(lo >> 32);
+    uint32_t h = (j == 7) ? ((x & 0x1ffu) >> 1) : 0u;
+    if (j == 0) h |= ((x >> 16) & 0x1ffu) << 8;
+    l0 |= dpp_u32<0xb1>(l0); l1 |= dpp_u32<0xb1>(l1); h |= dpp_u32<0xb1>(h);
+    l0 |= dpp_u32<0x4e>(l0); l1 |= dpp_u32<0x4e>(l1); h |= dpp_u32<0x4e>(h);
+    l0 |= dpp_u32<0x141>(l0); l1 |= dpp_u32<0x141>(l1); h |= dpp_u32<0x141>(h);
+    return {((uint64_t)l1 << 32) | l0, (uint64_t)h};
+}
+
 struct LmaOut {
     bool ok;
     double disp, strength, wm, wxy;
@@ -318,49 +367,40 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         }
         cv.hi &= 0x1ffffull;
         // the selection grows ring by ring from the start cell (C2:2066-2195): the sources of step
-        // istep are the selected cells of ring istep, convex ones (any on the first step)
-        Box81 sel = {0, 0};
-        if (4 >= wr0 && 4 <= wr1 && 4 >= wc0 && 4 <= wc1) sel.lo = 1ull << 40;
-        const Box81 dl = b_cols(wc0 + 1, 4), dr = b_cols(4, wc1 - 1);   // diagonal steps of vertical moves
-        const Box81 du = b_rows(wr0 + 1, 4), dd = b_rows(4, wr1 - 1);   // ... of horizontal moves
+        // istep are the selected cells of ring istep, convex ones (any on the first step).  On the
+        // rows form; the four moves are masked, not branched, so every lane takes part in the DPP
+        const int j = fl & 7;
+        const uint32_t cvr = r9_from_box(cv, j);
+        uint32_t sel = (4 >= wr0 && 4 <= wr1 && 4 >= wc0 && 4 <= wc1) ? (r9_rows(j, 4, 4) & r9_cols(4, 4)) : 0u;
+        const uint32_t dl = r9_cols(wc0 + 1, 4), dr = r9_cols(4, wc1 - 1);         // diagonal steps of vertical moves
+        const uint32_t du = r9_rows(j, wr0 + 1, 4), dd = r9_rows(j, 4, wr1 - 1);   // ... of horizontal moves
         for (int istep = 0; istep < hwin; istep++) {
-            const Box81 src = (istep == 0) ? sel : b_and(sel, cv);
-            const Box81 scols = b_cols(max(4 - istep, gc0), min(4 + istep, gc1));
-            const Box81 srows = b_rows(max(4 - istep, gr0), min(4 + istep, gr1));
-            Box81 nw = sel;
-            if (istep < y0) {   // up
-                const Box81 t = b_shr<9>(b_and(src, b_and(b_rows(4 - istep, 4 - istep), scols)));
-                const Box81 d = b_and(t, cv);
-                nw = b_or(nw, b_or(t, b_or(b_shr<1>(b_and(d, dl)), b_shl<1>(b_and(d, dr)))));
-            }
-            if (4 + istep < wr1) {   // down
-                const Box81 t = b_shl<9>(b_and(src, b_and(b_rows(4 + istep, 4 + istep), scols)));
-                const Box81 d = b_and(t, cv);
-                nw = b_or(nw, b_or(t, b_or(b_shr<1>(b_and(d, dl)), b_shl<1>(b_and(d, dr)))));
-            }
-            if (istep < x0) {   // left
-                const Box81 t = b_shr<1>(b_and(src, b_and(b_cols(4 - istep, 4 - istep), srows)));
-                const Box81 d = b_and(t, cv);
-                nw = b_or(nw, b_or(t, b_or(b_shr<9>(b_and(d, du)), b_shl<9>(b_and(d, dd)))));
-            }
-            if (4 + istep < wc1) {   // right
-                const Box81 t = b_shl<1>(b_and(src, b_and(b_cols(4 + istep, 4 + istep), srows)));
-                const Box81 d = b_and(t, cv);
-                nw = b_or(nw, b_or(t, b_or(b_shr<9>(b_and(d, du)), b_shl<9>(b_and(d, dd)))));
-            }
-            const bool grew = b_any(b_andn(nw, sel));
+            const uint32_t src = (istep == 0) ? sel : (sel & cvr);
+            const uint32_t scols = r9_cols(max(4 - istep, gc0), min(4 + istep, gc1));
+            const uint32_t srows = r9_rows(j, max(4 - istep, gr0), min(4 + istep, gr1));
+            const uint32_t m_up = (istep < y0) ? ~0u : 0u, m_dn = (4 + istep < wr1) ? ~0u : 0u;
+            const uint32_t m_lf = (istep < x0) ? ~0u : 0u, m_rt = (4 + istep < wc1) ? ~0u : 0u;
+            const uint32_t tu = r9_up(src & r9_rows(j, 4 - istep, 4 - istep) & scols & m_up, j);
+            const uint32_t td = r9_down(src & r9_rows(j, 4 + istep, 4 + istep) & scols & m_dn, j);
+            const uint32_t tl = (src & r9_cols(4 - istep, 4 - istep) & srows & m_lf) >> 1;
+            const uint32_t tr = (src & r9_cols(4 + istep, 4 + istep) & srows & m_rt) << 1;
+            const uint32_t dv = (tu | td) & cvr, dh = (tl | tr) & cvr;
+            const uint32_t nw = sel | tu | td | tl | tr | ((dv & dl) >> 1) | ((dv & dr) << 1) | r9_up(dh & du, j) |
+                                r9_down(dh & dd, j);
+            const bool grew = (__ballot((nw & ~sel) != 0u) & fmask) != 0;
             sel = nw;
             if (!grew) break;
         }
+        const Box81 selb = r9_to_box(sel, j);
         // cell weights (C2:2197-2206): selected convex 1, other selected nc_cost, with add3x3 the
         // 3x3 around the start at least nc_cost; the samples (addSamples, C2:2230-2338) in cell
         // order; NaN values or weights enter the fit as 0 (Correlations2dLMA:262-266)
         const Box81 grid = b_and(b_rows(gr0, gr1), b_cols(gc0, gc1));
-        Box81 smp = sel;
+        Box81 smp = selb;
         if (A.add3x3) smp = b_or(smp, b_and(b_rows(3, 5), b_cols(3, 5)));
         smp = b_and(smp, grid);
-        if (!(A.nc_cost > 0.0)) smp = b_and(sel, cv);   // the non-convex cells weigh nothing
-        cvs[g] = b_and(sel, cv);                      // weight class 1
+        if (!(A.nc_cost > 0.0)) smp = b_and(selb, cv);   // the non-convex cells weigh nothing
+        cvs[g] = b_and(selb, cv);                      // weight class 1
         double gsum = 0.0;
 #pragma unroll 1
         for (int k = 0; k < CPL; k++) {
