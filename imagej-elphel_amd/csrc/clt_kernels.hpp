@@ -2082,6 +2082,111 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
 }
 
 // ====================================================================================
+// textures + imclt_rbg in one pass over the converted tiles: each tile's NC*3 units go through
+// the IMCLT once; the windowed IMCLT rows feed both the texture tile (tex_units / tex_combine) and
+// the overlap-add of the (W+8) x (H+8) image planes (imclt_rbg, same LPF * 0.25).
+// A block sweeps one tile column down a segment of seg_rows output block rows, starting one tile
+// above the segment (its bottom half only); the lower half of each tile row stays in the lanes
+// (lane l >= 8 of a unit) until the next tile adds its upper half.  The two halves of a tile's
+// 16 columns overlap the neighbouring columns, so the columns run in two launches: parity 0
+// (even tx) stores its 16 output columns, parity 1 (odd tx) adds to them; when tiles_x is even the
+// last column's right half has no even neighbour and is stored.  Requires NC * 3 <= 16, no extra
+// disparity, and at most one task per tile with every task's flags non-zero (tile_map then names
+// every task exactly once).
+// ====================================================================================
+template <int NC>
+__global__ __launch_bounds__(256) void k_tex_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+                                                   int task_words, const int32_t* __restrict__ tile_map,
+                                                   const float* __restrict__ lpf, const TexArgs* __restrict__ args_p,
+                                                   float* __restrict__ tiles_out, float* __restrict__ stats_out,
+                                                   float* __restrict__ rbg, int seg_rows, int parity, GeomDev g) {
+    static_assert(NC * 3 <= 16, "one pass of units");
+    __shared__ __attribute__((aligned(16))) TexShared<NC> S;
+    const TexArgs& A = *args_p;
+    const int tx = 2 * (int)blockIdx.x + parity;
+    if (tx >= g.tiles_x) return;   // whole block
+    const int by0 = blockIdx.y * seg_rows;
+    const int nby = min(seg_rows, g.tiles_y + 1 - by0);
+    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+    const int out_w = g.width + 8, out_h = g.height + 8;
+    const size_t plane_sz = (size_t)out_w * out_h;
+    const int l = threadIdx.x & 15;
+    // right half stored, not added: the last column when no even column follows it
+    const bool add_left = parity == 1;
+    const bool add_right = parity == 1 && !(tx == g.tiles_x - 1 && (g.tiles_x & 1) == 0);
+    float carry[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) carry[c] = 0.f;
+    for (int ty = by0 - 1; ty < by0 + nby; ty++) {
+        const bool in_frame = ty >= 0 && ty < g.tiles_y;
+        const int t = in_frame ? tile_map[(size_t)ty * g.tiles_x + tx] : -1;
+        const bool own = ty >= by0;   // the segment writes output block row ty
+        // the odd columns' read of the even columns' sums, issued before this tile's transforms
+        const int unit = threadIdx.x >> 4;
+        float* o = rbg + (size_t)min(unit, NC * 3 - 1) * plane_sz + (size_t)(max(ty, 0) * 8 + (l & 7)) * out_w + tx * 8;
+        float4 pre[4];
+#pragma unroll
+        for (int c = 0; c < 4; c++) pre[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (own && l < 8 && unit < NC * 3) {
+            if (add_left) {
+                pre[0] = *reinterpret_cast<const float4*>(o);
+                pre[1] = *reinterpret_cast<const float4*>(o + 4);
+            }
+            if (add_right) {
+                pre[2] = *reinterpret_cast<const float4*>(o + 8);
+                pre[3] = *reinterpret_cast<const float4*>(o + 12);
+            }
+        }
+        auto sink = [&](int uu, int ll, const float* row) {
+            if (own && ll < 8) {
+#pragma unroll
+                for (int c = 0; c < 16; c += 4) {
+                    const float4 u = pre[c >> 2];
+                    const float4 w = make_float4(row[c] + carry[c] + u.x, row[c + 1] + carry[c + 1] + u.y,
+                                                 row[c + 2] + carry[c + 2] + u.z, row[c + 3] + carry[c + 3] + u.w);
+                    *reinterpret_cast<float4*>(o + c) = w;
+                }
+            }
+        };
+        if (t >= 0) {
+            const int32_t* rec = tasks + (size_t)t * task_words;
+            const int port_mask = rec[0] & 0xff & ((1 << NC) - 1);
+            tex_units<NC>(clt, (size_t)ty * g.tiles_x + tx, ntiles, lpf, A, 0.f, false, S,
+                          [&](int uu, int ll, const float* row) {
+                              // lane l < 8 adds the lower half of the tile above (lane l + 8)
+                              float up[16];
+#pragma unroll
+                              for (int c = 0; c < 16; c++) up[c] = lane_xor8(carry[c]);
+#pragma unroll
+                              for (int c = 0; c < 16; c++) carry[c] = up[c];
+                              sink(uu, ll, row);
+#pragma unroll
+                              for (int c = 0; c < 16; c++) carry[c] = row[c];
+                          });
+            __syncthreads();
+            if (own && ty < g.tiles_y)
+                tex_combine<NC>(A, port_mask, S, tiles_out + (size_t)t * A.planes * 256,
+                                stats_out ? stats_out + (size_t)t * 4 * NC : nullptr);
+            __syncthreads();   // the next tile reuses the LDS
+        } else {   // no task tile (or outside the frame): nothing to add but the lower half above
+            if (unit < NC * 3) {
+                float up[16], zero[16];
+#pragma unroll
+                for (int c = 0; c < 16; c++) {
+                    up[c] = lane_xor8(carry[c]);
+                    zero[c] = 0.f;
+                }
+#pragma unroll
+                for (int c = 0; c < 16; c++) carry[c] = up[c];
+                sink(unit, l, zero);
+#pragma unroll
+                for (int c = 0; c < 16; c++) carry[c] = 0.f;
+            }
+        }
+    }
+}
+
+// ====================================================================================
 // combine textures: overlap-add of the texture tiles (ImageDtt.java:3858-3989, overlap,
 // sharp_alpha = false), gather form: thread = output pixel.
 // ====================================================================================

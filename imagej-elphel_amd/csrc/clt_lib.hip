@@ -104,7 +104,7 @@ static void half_flat_top(int n, double hwidth, double blur, bool normalize, boo
 }
 
 // ------------------------------------------------------------------------ handle
-enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_CORR_LMA, T_KPREP, T_RIG, T_NUM };
+enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_CORR_LMA, T_KPREP, T_RIG, T_TEX_IMCLT, T_NUM };
 
 struct clt_handle {
     clt_geom g{};
@@ -163,6 +163,7 @@ struct clt_handle {
     int graph_stages = 0;
     // launch shapes chosen by clt_autotune
     int tune_corr_tpb = 4, tune_imclt_rows = 16;
+    int tex_imclt_rows = 8;        // output block rows per segment of k_tex_imclt
     bool tuned = false;
     bool corr_quad = false;        // correlate2D runs the quad kernel
     int corr_num_strip = 0;
@@ -176,6 +177,9 @@ struct clt_handle {
     hipStream_t stream_lma = nullptr;
     hipEvent_t ev_fork = nullptr, ev_lma = nullptr;
     bool lma_pending = false;
+    // every task has non-zero flags and a tile of its own: tile_map names each task once, so
+    // textures and imclt_rbg can run as one pass over the tile columns (k_tex_imclt)
+    bool tex_fusable = false;
     int graph_epoch = 0;           // bumped whenever a captured launch would go stale
     int graph_epoch_cap = -1;
     CorrArgs* d_corr_args = nullptr;
@@ -338,6 +342,16 @@ template <int NC>
 static void launch_tex_nc(clt_handle* h, int n, const int32_t* task_frame, const float* extra) {
     hipLaunchKernelGGL(k_textures<NC>, dim3(n), dim3(256), 0, h->stream, h->d_clt, h->d_tasks, h->task_words, n,
                        h->d_lpf_corr, extra, h->d_tex_args, h->d_tex, h->d_tex_stats, task_frame, h->clt_fstride, h->gd);
+}
+
+template <int NC>
+static void launch_tex_imclt_nc(clt_handle* h) {
+    const int rows = h->tex_imclt_rows;
+    const dim3 grid((h->tiles_x + 1) / 2, (h->tiles_y + 1 + rows - 1) / rows);
+    for (int parity = 0; parity < 2; parity++)   // even columns store, odd columns add
+        hipLaunchKernelGGL(k_tex_imclt<NC>, grid, dim3(256), 0, h->stream, h->d_clt, h->d_tasks, h->task_words,
+                           h->d_tile_map, h->d_lpf_corr, h->d_tex_args, h->d_tex, h->d_tex_stats, h->d_rbg, rows, parity,
+                           h->gd);
 }
 
 static void launch_tex(clt_handle* h, int n, const int32_t* task_frame, const float* extra) {
@@ -733,13 +747,17 @@ int clt_set_tasks(clt_handle* h, const void* tasks, int32_t num_tasks) {
     const int32_t* t = (const int32_t*)tasks;
     HIPCHK(h, sync_streams(h));
     std::fill(h->h_tile_map.begin(), h->h_tile_map.end(), -1);
+    bool fusable = true;
     for (int i = 0; i < num_tasks; i++) {
         const int32_t* r = t + (size_t)i * h->task_words;
         const int tx = r[1] & 0xffff, ty = (r[1] >> 16) & 0xffff;
         if (tx >= h->tiles_x || ty >= h->tiles_y)
             return fail(h, CLT_E_ARG, "clt_set_tasks: task %d tile (%d,%d) outside %dx%d", i, tx, ty, h->tiles_x, h->tiles_y);
-        if (r[0] != 0) h->h_tile_map[(size_t)ty * h->tiles_x + tx] = i;
+        int32_t& m = h->h_tile_map[(size_t)ty * h->tiles_x + tx];
+        fusable = fusable && r[0] != 0 && m < 0;
+        if (r[0] != 0) m = i;
     }
+    h->tex_fusable = fusable;
     if (num_tasks > 0) HIPCHK(h, hipMemcpy(h->d_tasks, tasks, (size_t)num_tasks * h->task_words * 4, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_tile_map, h->h_tile_map.data(), (size_t)h->num_tiles * 4, hipMemcpyHostToDevice));
     h->batch_dirty = true;   // the device task list now holds frame 0 only
@@ -947,12 +965,16 @@ int clt_set_tasks_geometry(clt_handle* h, const int32_t* task, const int32_t* tx
         return rc;
     HIPCHK(h, sync_streams(h));
     std::fill(h->h_tile_map.begin(), h->h_tile_map.end(), -1);
+    bool fusable = true;
     for (int i = 0; i < num_tasks; i++) {
         const int tx = txy[i] & 0xffff, ty = (txy[i] >> 16) & 0xffff;
         if (tx >= h->tiles_x || ty >= h->tiles_y)
             return fail(h, CLT_E_ARG, "clt_set_tasks_geometry: task %d tile (%d,%d) outside %dx%d", i, tx, ty, h->tiles_x, h->tiles_y);
-        if (task[i] != 0) h->h_tile_map[(size_t)ty * h->tiles_x + tx] = i;
+        int32_t& m = h->h_tile_map[(size_t)ty * h->tiles_x + tx];
+        fusable = fusable && task[i] != 0 && m < 0;
+        if (task[i] != 0) m = i;
     }
+    h->tex_fusable = fusable;
     if (num_tasks > 0) {
         HIPCHK(h, hipMemcpy(h->d_gwords, task, (size_t)num_tasks * 4, hipMemcpyHostToDevice));
         HIPCHK(h, hipMemcpy(h->d_gtxy, txy, (size_t)num_tasks * 4, hipMemcpyHostToDevice));
@@ -1537,6 +1559,35 @@ int clt_exec_textures(clt_handle* h, const float* extra_disparity) {
     tbegin(h, T_TEX);
     launch_tex(h, h->num_tasks, nullptr, extra);
     tend(h, T_TEX);
+    HIPCHK(h, hipGetLastError());
+    h->tex_done = true;
+    h->combined = false;
+    return finish(h);
+}
+
+int clt_exec_textures_imclt(clt_handle* h, const float* extra_disparity, int32_t apply_lpf) {
+    if (!h) return CLT_E_ARG;
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_textures_imclt: a batch handle runs clt_exec_batch");
+    if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_textures_imclt: no converted tiles");
+    const clt_geom& g = h->g;
+    const bool fuse = !extra_disparity && apply_lpf && h->tex_fusable && g.num_colors == 3 && g.num_cams >= 2 &&
+                      g.num_cams <= 5;
+    if (!fuse) {   // the two passes
+        int rc = clt_exec_textures(h, extra_disparity);
+        return rc ? rc : clt_exec_imclt(h, apply_lpf);
+    }
+    int rc;
+    if ((rc = prepare_tex(h))) return rc;
+    if ((rc = dalloc(h, (void**)&h->d_rbg, (size_t)g.num_cams * g.num_colors * (g.width + 8) * (g.height + 8) * 4)))
+        return rc;
+    tbegin(h, T_TEX_IMCLT);
+    switch (g.num_cams) {
+    case 2: launch_tex_imclt_nc<2>(h); break;
+    case 3: launch_tex_imclt_nc<3>(h); break;
+    case 4: launch_tex_imclt_nc<4>(h); break;
+    default: launch_tex_imclt_nc<5>(h); break;
+    }
+    tend(h, T_TEX_IMCLT);
     HIPCHK(h, hipGetLastError());
     h->tex_done = true;
     h->combined = false;

@@ -326,6 +326,13 @@ class GPUTileProcessor:
         ed = None if extra_disparity is None else np.ascontiguousarray(extra_disparity, np.float32)
         self._chk(self.lib.clt_exec_textures(self.h, _ptr(ed)))
 
+    def execTexturesImclt(self, apply_lpf=True, extra_disparity=None):
+        """execTextures then execImcltRbg (ID:2662-2812, GTP:560-597) with the same outputs; one pass
+        over the converted tiles when there is no extra disparity, apply_lpf is set and every task
+        has its own tile and non-zero flags."""
+        ed = None if extra_disparity is None else np.ascontiguousarray(extra_disparity, np.float32)
+        self._chk(self.lib.clt_exec_textures_imclt(self.h, _ptr(ed), int(apply_lpf)))
+
     def execCombineTextures(self):
         self._chk(self.lib.clt_exec_combine_textures(self.h))
 

@@ -269,6 +269,12 @@ int  clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity);
 int  clt_exec_textures_rig(clt_handle* h);
 /* Per-tile textures (ID:2662-2812).  extra_disparity: num_tasks floats or NULL (ID:2663-2677). */
 int  clt_exec_textures(clt_handle* h, const float* extra_disparity);
+/* No single reference counterpart: clt_exec_textures(h, extra_disparity) followed by
+ * clt_exec_imclt(h, apply_lpf) (ID:2662-2812 and execImcltRbg GTP:560-597), with the same outputs.
+ * When extra_disparity is NULL, apply_lpf != 0, there are at most 5 cameras and every task has
+ * non-zero flags and a tile of its own, one pass computes each tile's inverse CLT once for both
+ * outputs; otherwise the two calls run. */
+int  clt_exec_textures_imclt(clt_handle* h, const float* extra_disparity, int32_t apply_lpf);
 /* Overlap-add of the texture tiles into full-frame planes (combineRGBATiles, ID:3858-3989,
  * overlap = true, sharp_alpha = false). */
 int  clt_exec_combine_textures(clt_handle* h);
@@ -428,7 +434,8 @@ int  clt_set_timing(clt_handle* h, int32_t enable);   /* record HIP events aroun
  * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures, 5 batch graph,
  * 6 correlation adjoint, 7 geometry adjoint, 8 per-tile correlation LMA, 9 kernel preparation
  * (clt_prepare_kernels: one count per band of kernel rows, plus one for the centre slopes),
- * 10 the whole of clt_exec_corr2d_rig. */
+ * 10 the whole of clt_exec_corr2d_rig, 11 the one-pass textures + imclt of
+ * clt_exec_textures_imclt (both column launches). */
 int  clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count);
 int  clt_reset_timing(clt_handle* h);
 

@@ -1,0 +1,88 @@
+"""clt_exec_textures_imclt: textures and imclt_rbg in one call must give the outputs of
+clt_exec_textures followed by clt_exec_imclt (both checked against the oracle in
+test_gpu_parity.py).  The texture tiles and statistics run the same per-tile code on the same
+inputs and must be bit-identical; the image planes are the same four overlapping tile halves summed
+in another order (two column launches instead of a strip sweep), so they agree to float rounding.
+Cases: dense and sparse task lists, even and odd tile columns, segment boundaries, and the inputs
+that make the call fall back to the two passes (extra disparity, no LPF, a zero-flag task, two
+tasks on one tile)."""
+import numpy as np
+import pytest
+
+from helpers import gpu_processor, log
+
+pytestmark = pytest.mark.gpu
+
+
+def _outputs(tp, sc):
+    tiles, stats = tp.getTextures()
+    img = np.stack([tp.getRBG(c) for c in range(sc.num_cams)])
+    return tiles, stats, img
+
+
+def _compare(pk, sc, tasks=None, extra=None, apply_lpf=True, keep_weights=1):
+    def run(fused):
+        tp = gpu_processor(pk, sc)
+        if tasks is not None:
+            tp.setTasks(tasks)
+        tp.setCorrSigma(0.8)
+        tp.setTexParams(keep_weights=keep_weights)
+        tp.execConverCorrectTiles()
+        tp.setTiming(True)
+        if fused:
+            tp.execTexturesImclt(apply_lpf, extra)
+        else:
+            tp.execTextures(extra)
+            tp.execImcltRbg(apply_lpf)
+        out = _outputs(tp, sc) + (tp.getTiming("tex_imclt")[1],)
+        tp.close()
+        return out
+
+    (t1, s1, i1, _), (t2, s2, i2, one_pass) = run(False), run(True)
+    log("  one pass: %s" % bool(one_pass))
+    tiles_same = np.array_equal(t1.view(np.int32), t2.view(np.int32))
+    stats_same = np.array_equal(s1.view(np.int32), s2.view(np.int32))
+    scale = np.abs(i1).max()
+    img_err = float(np.abs(i1.astype(np.float64) - i2).max() / scale)
+    log("  tiles bit-identical=%s stats bit-identical=%s image max |diff| / max = %.3g" %
+        (tiles_same, stats_same, img_err))
+    return tiles_same, stats_same, img_err, bool(one_pass), (i1, i2)
+
+
+def test_dense_quad(pk):
+    log("case: dense quad 128x96 (16 x 12 tiles, even column count)")
+    sc = pk.synth.make_scene(128, 96, seed=21, disparity=("field", 0.0, 6.0))
+    ts, ss, ie, one, _ = _compare(pk, sc)
+    assert one and ts and ss and ie < 1e-6
+
+
+def test_sparse_odd_columns_segments(pk):
+    log("case: sparse tasks, 136x152 (17 x 19 tiles: odd columns, several 8-row segments)")
+    sc = pk.synth.make_scene(136, 152, seed=22, disparity=("field", 0.0, 5.0), task_fraction=0.6)
+    ts, ss, ie, one, (i1, _) = _compare(pk, sc)
+    assert one and ts and ss and ie < 1e-6
+    assert np.abs(i1).max() > 0
+
+
+def test_rgba_only(pk):
+    log("case: keep_weights off (4 texture planes)")
+    sc = pk.synth.make_scene(128, 96, seed=23, disparity=2.0)
+    ts, ss, ie, one, _ = _compare(pk, sc, keep_weights=0)
+    assert one and ts and ss and ie < 1e-6
+
+
+@pytest.mark.parametrize("case", ["extra", "no_lpf", "zero_flag", "duplicate"])
+def test_fallbacks_match_two_passes(pk, case):
+    log("case: fallback to the two passes (%s)" % case)
+    sc = pk.synth.make_scene(128, 96, seed=24, disparity=2.0)
+    tasks, extra, lpf = sc.tasks.copy(), None, True
+    if case == "extra":
+        extra = np.random.default_rng(5).uniform(-1.0, 1.0, len(tasks)).astype(np.float32)
+    elif case == "no_lpf":
+        lpf = False
+    elif case == "zero_flag":
+        tasks[7, 0] = 0
+    else:
+        tasks[5] = tasks[3]   # two tasks on one tile (the same record: convert writes the same CLT)
+    ts, ss, ie, one, _ = _compare(pk, sc, tasks=tasks, extra=extra, apply_lpf=lpf)
+    assert not one and ts and ss and ie == 0.0
