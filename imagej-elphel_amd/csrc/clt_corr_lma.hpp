@@ -294,34 +294,33 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         uint16_t* idx = S.idx + g * LMA_CELLS;
         // combined correlation of the group's pairs over the box (combineCompatiblePairs, C2:462-488):
         // every load of the lane's cells is issued before the first is used
-        int pp[NPM], xf[NPM];
+        // element (row, col) of pair q's correlation as seen by the group: base + a row + b col
+        // (as is, transposed for a vertical pair, upside down for the other diagonal)
+        int base[NPM], ra[NPM], cb[NPM];
 #pragma unroll
         for (int q = 0; q < NPM; q++) {
-            pp[q] = toff + grp.pair[q < npairs ? q : 0] * CLT_CORR_LEN;
-            xf[q] = grp.xform[q < npairs ? q : 0];
+            const int xf = grp.xform[q < npairs ? q : 0];
+            base[q] = toff + grp.pair[q < npairs ? q : 0] * CLT_CORR_LEN + ((xf == 2) ? 210 : 0);
+            ra[q] = (xf == 1) ? 1 : ((xf == 2) ? -15 : 15);
+            cb[q] = (xf == 1) ? 15 : 1;
         }
         float raw[CPL][NPM];
+        uint32_t in_grid = 0;   // bit k: cell fl + W k lies on the 15 x 15 correlation
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
             const int c = fl + W * k;
             const int br = (c * 57) >> 9, bc = c - 9 * br;   // c / 9, c % 9 for c < 81
             const int row = y0 - 4 + br, col = x0 - 4 + bc;
             const bool ing = c < LMA_CELLS && row >= 0 && row < 15 && col >= 0 && col < 15;
+            in_grid |= ing ? (1u << k) : 0u;
             const int r0 = ing ? row : 0, c0 = ing ? col : 0;   // loads stay unconditional and in bounds
 #pragma unroll
-            for (int q = 0; q < NPM; q++) {
-                const int rr = (xf[q] == 2) ? 14 - r0 : r0;
-                const int src = (xf[q] == 1) ? c0 * 15 + r0 : rr * 15 + c0;
-                raw[k][q] = corr[pp[q] + src];
-            }
+            for (int q = 0; q < NPM; q++) raw[k][q] = corr[base[q] + ra[q] * r0 + cb[q] * c0];
         }
         __asm__ volatile("" ::: "memory");   // all loads in flight before the first use
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
-            const int c = fl + W * k;
-            const int br = (c * 57) >> 9, bc = c - 9 * br;
-            const int row = y0 - 4 + br, col = x0 - 4 + bc;
-            const bool ing = c < LMA_CELLS && row >= 0 && row < 15 && col >= 0 && col < 15;
+            const bool ing = (in_grid >> k) & 1;
 #pragma unroll
             for (int q = 0; q < NPM; q++) raw[k][q] = (ing && q < npairs) ? raw[k][q] : 0.f;
         }
