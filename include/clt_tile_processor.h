@@ -113,7 +113,8 @@ typedef struct clt_tex_params {
     float   corr_magic_scale;  /* 0.85   (ECP corr_magic_scale)                               */
     int32_t keep_weights;      /* 1: 12 planes per tile (R,B,G,A,w0..w3,rms_r,rms_b,rms_g,rms),
                                   0: 4 planes (R,B,G,A)                                        */
-    float   port_offsets[CLT_MAX_CAMS][2]; /* per camera {x,y} offset (ID:1648-1652)           */
+    float   port_offsets[CLT_MAX_CAMS][2]; /* per camera {x,y} offset (ID:1648-1652); default
+                                            * for other than 4 cameras: a circle of radius 0.5 */
 } clt_tex_params;
 
 /* Per-tile correlation LMA and foreground correction (ImageDttParameters, IDP:32-110): the fit of

@@ -9,6 +9,7 @@ Parity pinning of each oracle function (golden tables held by the reference, clo
 invariants) is documented and tested in ``tests/test_oracle.py``.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -164,10 +165,18 @@ def default_rig_params(**kw):
 
 
 def default_tex_params(num_cams=4, port_offsets=None, **kw):
-    """CLTParameters defaults (ECP:2584-2594) and the quad port offsets (ID:1648-1652)."""
+    """CLTParameters defaults (ECP:2584-2594) and the quad port offsets (ID:1648-1652); other camera
+    counts default to a circle of radius 0.5 as clt_default_tex_params does (the reference has only
+    the quad's)."""
     p = TexParams(min_shot=10.0, scale_shot=3.0, diff_sigma=5.0, dust_remove=1,
                   corr_magic_scale=0.85, keep_weights=1)
-    offs = port_offsets if port_offsets is not None else QUAD_PORT_OFFSETS
+    if port_offsets is not None:
+        offs = port_offsets
+    elif num_cams == 4:
+        offs = QUAD_PORT_OFFSETS
+    else:
+        offs = [(0.5 * math.cos(2 * math.pi * i / num_cams), 0.5 * math.sin(2 * math.pi * i / num_cams))
+                for i in range(num_cams)]
     for i, (x, y) in enumerate(offs):
         p.port_offsets[i][0] = x
         p.port_offsets[i][1] = y

@@ -200,7 +200,9 @@ def _tex_compare(pk, orc, sc, extra=None, tasks=None):
     o_img = orc.combine_textures(oracle_geom(orc, sc), tasks, o_tiles)
     live = tasks[:, 0] != 0
     res = {}
-    for name, sl in (("rgb", slice(0, 3)), ("alpha", slice(3, 4)), ("weights", slice(4, 8)), ("rms", slice(8, None))):
+    nc = sc.num_cams
+    for name, sl in (("rgb", slice(0, 3)), ("alpha", slice(3, 4)), ("weights", slice(4, 4 + nc)),
+                     ("rms", slice(4 + nc, None))):
         res[name], res[name + "_flips"] = robust_rel_rms(g_tiles[live, sl], o_tiles[live, sl])
     res["stats"] = rel_rms(g_stats[live], o_stats[live])
     res["image"], res["image_flips"] = robust_rel_rms(g_img[:4], o_img[:4])
@@ -220,6 +222,17 @@ def _assert_textures(r):
 def test_textures_quad(pk, orc, quad_scene):
     log("case: textures quad 128x96, all ports")
     r = _tex_compare(pk, orc, quad_scene)
+    log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
+    _assert_textures(r)
+
+
+@pytest.mark.parametrize("cams", [2, 3, 6, 8])
+def test_textures_camera_counts(pk, orc, cams):
+    """Other port counts (cameras on a circle): the units of a tile take one pass of 16 lanes up to
+    5 cameras and two passes from 6 on (the debayered tiles then get their own LDS)."""
+    log("case: textures, %d cameras on a circle, 96x64" % cams)
+    sc = pk.synth.make_scene(96, 64, num_cams=cams, seed=40 + cams, disparity=("field", 0.0, 4.0))
+    r = _tex_compare(pk, orc, sc)
     log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
     _assert_textures(r)
 

@@ -64,6 +64,14 @@ def test_sparse_odd_columns_segments(pk):
     assert np.abs(i1).max() > 0
 
 
+@pytest.mark.parametrize("cams", [2, 3, 5])
+def test_camera_counts(pk, cams):
+    log("case: %d cameras on a circle, 96x64" % cams)
+    sc = pk.synth.make_scene(96, 64, num_cams=cams, seed=50 + cams, disparity=("field", 0.0, 4.0))
+    ts, ss, ie, one, _ = _compare(pk, sc)
+    assert one and ts and ss and ie < 1e-6
+
+
 def test_rgba_only(pk):
     log("case: keep_weights off (4 texture planes)")
     sc = pk.synth.make_scene(128, 96, seed=23, disparity=2.0)
