@@ -19,8 +19,8 @@ DISP_TOL = 0.01
 LAYERS = (("final", 1), ("strength", 2), ("poly", 8), ("poly_strength", 9))
 
 
-def _same_input(orc, g_corr, g_disp):
-    o = orc.corr_lma_frame(g_corr, g_disp.astype(np.float64))
+def _same_input(orc, g_corr, g_disp, lma=None):
+    o = orc.corr_lma_frame(g_corr, g_disp.astype(np.float64), lma=lma)
     res = {}
     for name, k in LAYERS:
         fo, fg = np.isfinite(o[:, k]), np.isfinite(g_disp[:, k])
@@ -92,3 +92,23 @@ def test_corr_lma_disabled_and_parameters(pk, orc):
     log("  non-default: %d fitted, poly max|d|=%.3g px" % (both.sum(), d))
     assert np.array_equal(np.isfinite(o[:, 8]), np.isfinite(g_disp[:, 8])) and d < 1e-4
     assert np.array_equal(g_disp[same, 1], g_disp[same, 10])   # mix off: CM unchanged
+
+
+@pytest.mark.parametrize("name,kw", [
+    ("Wyd adjusted (6 parameters, 4-core direction fits)", dict(lma_adjust_wy=1)),
+    ("background correction, any direction, over-correction",
+     dict(fo_far=1, fo_ortho=0, fo_overcorrection=0.5, fo_min_diff=0.05))])
+def test_corr_lma_foreground_variants(pk, orc, name, kw):
+    """The chain with the one-direction fits and the foreground correction on, under parameters the
+    default run does not take: the NC = 4 fits (Wyd adjusted) and the far/any-direction/over-
+    correction branches of foregroundCorrect (C2:1663-1750)."""
+    log("case: per-tile LMA chain, %s" % name)
+    sc = pk.synth.make_scene(128, 96, seed=23, disparity=("field", 0.0, 6.0))
+    g_corr, g_disp = _run(pk, orc, sc, **kw)
+    r = _same_input(orc, g_corr, g_disp, lma=orc.default_corr_lma_params(**kw))
+    log("  same input: " + ", ".join("%s=%.3g" % kv for kv in r.items()))
+    assert r["fitted"] > 0.5 * len(sc.tasks)
+    for nm, _ in LAYERS:
+        assert r[nm + "_nan"] == 0 and r[nm] < 1e-4, nm
+    moved = int(np.sum(np.isfinite(g_disp[:, 1]) & (g_disp[:, 1] != g_disp[:, 8])))
+    log("  tiles whose final disparity differs from the LMA's (foreground correction): %d" % moved)
