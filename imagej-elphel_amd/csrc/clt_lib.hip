@@ -177,6 +177,12 @@ struct clt_handle {
     hipStream_t stream_lma = nullptr;
     hipEvent_t ev_fork = nullptr, ev_lma = nullptr;
     bool lma_pending = false;
+    // imclt_rbg of a single frame runs on a side stream from the point of its call, so it overlaps
+    // what the caller launches next (correlate2D reads the same CLT); ev_side joins it back before
+    // the CLT, tile map, LPF or image planes are rewritten or the images are read
+    hipStream_t stream_side = nullptr;
+    hipEvent_t ev_side_fork = nullptr, ev_side = nullptr;
+    bool side_pending = false;
     // every task has non-zero flags and a tile of its own: tile_map names each task once, so
     // textures and imclt_rbg can run as one pass over the tile columns (k_tex_imclt)
     bool tex_fusable = false;
@@ -256,8 +262,14 @@ static void join_lma(clt_handle* h) {
     (void)hipStreamWaitEvent(h->stream, h->ev_lma, 0);
     h->lma_pending = false;
 }
+static void join_side(clt_handle* h) {
+    if (!h->side_pending) return;
+    (void)hipStreamWaitEvent(h->stream, h->ev_side, 0);
+    h->side_pending = false;
+}
 static hipError_t sync_streams(clt_handle* h) {
     join_lma(h);
+    join_side(h);
     return hipStreamSynchronize(h->stream);
 }
 
@@ -289,6 +301,7 @@ static void tend(clt_handle* h, int which, hipStream_t st = nullptr) {
 // ------------------------------------------------------------------------ kernel launches
 // n tasks of the device task list; task_frame: frame of each task (batches) or null (one frame).
 static void launch_convert(clt_handle* h, int n, const int32_t* task_frame, int lpf_mask) {
+    join_side(h);   // a running imclt_rbg still reads the CLT
     const long units = (long)n * h->g.num_cams;   // (task, camera); 4 per block
     const int blocks = (int)((units + 3) / 4);
     const double inv_kstep = 1.0 / h->g.kernel_step;   // exact: checked power of two
@@ -298,12 +311,12 @@ static void launch_convert(clt_handle* h, int n, const int32_t* task_frame, int 
                        h->img_fstride, h->clt_fstride, h->gd);
 }
 
-static void launch_imclt(clt_handle* h, int frames, int apply_lpf) {
+static void launch_imclt(clt_handle* h, int frames, int apply_lpf, hipStream_t st = nullptr) {
     const int rows = h->tune_imclt_rows;
     dim3 grid((h->tiles_x + 1 + IMCLT_S - 1) / IMCLT_S, (h->tiles_y + 1 + rows - 1) / rows,
               frames * h->g.num_cams * h->g.num_colors);
-    hipLaunchKernelGGL(k_imclt, grid, dim3(256), 0, h->stream, h->d_clt, h->d_tile_map, apply_lpf ? h->d_lpf_corr : nullptr,
-                       h->d_rbg, rows, h->clt_fstride, h->gd);
+    hipLaunchKernelGGL(k_imclt, grid, dim3(256), 0, st ? st : h->stream, h->d_clt, h->d_tile_map,
+                       apply_lpf ? h->d_lpf_corr : nullptr, h->d_rbg, rows, h->clt_fstride, h->gd);
 }
 
 static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
@@ -497,7 +510,10 @@ static int create_handle(const clt_geom* geom, int frames, clt_handle** out) {
     }
     if (hipStreamCreateWithFlags(&h->stream_lma, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&h->ev_lma, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&h->ev_lma, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream_side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_side_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_side, hipEventDisableTiming) != hipSuccess) {
         (void)clt_destroy(h);
         return CLT_E_HIP;
     }
@@ -561,6 +577,9 @@ int clt_destroy(clt_handle* h) {
     }
     if (h->stream) (void)hipStreamDestroy(h->stream);
     if (h->stream_lma) (void)hipStreamDestroy(h->stream_lma);
+    if (h->ev_side_fork) (void)hipEventDestroy(h->ev_side_fork);
+    if (h->ev_side) (void)hipEventDestroy(h->ev_side);
+    if (h->stream_side) (void)hipStreamDestroy(h->stream_side);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_lma) (void)hipEventDestroy(h->ev_lma);
     delete h;
@@ -1017,6 +1036,7 @@ int clt_set_lpf_rbg(clt_handle* h, float sigma_r, float sigma_b, float sigma_g) 
     lpf_coeffs(sigma_r, l[0]);
     lpf_coeffs(sigma_b, l[1]);
     lpf_coeffs(sigma_g, l[2]);
+    HIPCHK(h, sync_streams(h));   // kernels in flight may read the table
     HIPCHK(h, hipMemcpy(h->d_lpf_rbg, l, sizeof(l), hipMemcpyHostToDevice));
     return CLT_OK;
 }
@@ -1026,6 +1046,7 @@ int clt_set_corr_sigma(clt_handle* h, float sigma) {
     if (sigma < 0) return fail(h, CLT_E_ARG, "clt_set_corr_sigma: negative sigma");
     float l[64];
     lpf_coeffs(sigma, l);
+    HIPCHK(h, sync_streams(h));   // kernels in flight may read the table
     HIPCHK(h, hipMemcpy(h->d_lpf_corr, l, sizeof(l), hipMemcpyHostToDevice));
     h->corr_sigma = sigma;
     return CLT_OK;
@@ -1100,10 +1121,15 @@ int clt_exec_imclt(clt_handle* h, int32_t apply_lpf) {
     if ((rc = dalloc(h, (void**)&h->d_rbg,
                      (size_t)h->num_frames * g.num_cams * g.num_colors * (g.width + 8) * (g.height + 8) * 4)))
         return rc;
-    tbegin(h, T_IMCLT);
-    launch_imclt(h, 1, apply_lpf);
-    tend(h, T_IMCLT);
+    join_side(h);   // the previous imclt_rbg writes the same planes
+    HIPCHK(h, hipEventRecord(h->ev_side_fork, h->stream));
+    HIPCHK(h, hipStreamWaitEvent(h->stream_side, h->ev_side_fork, 0));
+    tbegin(h, T_IMCLT, h->stream_side);
+    launch_imclt(h, 1, apply_lpf, h->stream_side);
+    tend(h, T_IMCLT, h->stream_side);
     HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipEventRecord(h->ev_side, h->stream_side));
+    h->side_pending = true;
     return finish(h);
 }
 
@@ -1580,6 +1606,7 @@ int clt_exec_textures_imclt(clt_handle* h, const float* extra_disparity, int32_t
     if ((rc = prepare_tex(h))) return rc;
     if ((rc = dalloc(h, (void**)&h->d_rbg, (size_t)g.num_cams * g.num_colors * (g.width + 8) * (g.height + 8) * 4)))
         return rc;
+    join_side(h);   // an imclt_rbg still writing the planes
     tbegin(h, T_TEX_IMCLT);
     switch (g.num_cams) {
     case 2: launch_tex_imclt_nc<2>(h); break;
