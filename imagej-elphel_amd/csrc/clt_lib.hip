@@ -603,6 +603,7 @@ int clt_set_kernels(clt_handle* h, int32_t cam, const float* kernels, const floa
         for (int q = 0; q < 4; q++)
             for (int v = 0; v < 8; v++)
                 for (int hh = 0; hh < 8; hh++) tr[k * 256 + q * 64 + hh * 8 + v] = kernels[k * 256 + q * 64 + v * 8 + hh];
+    HIPCHK(h, sync_streams(h));   // a convert in flight may read the previous kernels
     HIPCHK(h, hipMemcpy(h->d_kernels + cam * nk * 256, tr.data(), nk * 256 * 4, hipMemcpyHostToDevice));
     HIPCHK(h, hipMemcpy(h->d_koffs + cam * nk * 8, offsets, nk * 8 * 4, hipMemcpyHostToDevice));
     h->kernels_set[cam] = true;

@@ -94,3 +94,31 @@ def test_fallbacks_match_two_passes(pk, case):
         tasks[5] = tasks[3]   # two tasks on one tile (the same record: convert writes the same CLT)
     ts, ss, ie, one, _ = _compare(pk, sc, tasks=tasks, extra=extra, apply_lpf=lpf)
     assert not one and ts and ss and ie == 0.0
+
+
+def test_async_side_stream_ordering(pk):
+    """imclt_rbg runs on a side stream in async mode: a convert issued right after it (no host
+    synchronisation in between) must not change the CLT it reads.  Reference: the same calls in
+    synchronous mode."""
+    log("case: async imclt_rbg on its side stream, then a convert with another LPF mask")
+    sc = pk.synth.make_scene(128, 96, seed=25, disparity=("field", 0.0, 6.0))
+
+    def run(asynchronous):
+        tp = gpu_processor(pk, sc)
+        tp.setCorrSigma(0.8)
+        tp.setLpfRbg(0.7, 0.7, 0.7)
+        tp.setAsync(asynchronous)
+        tp.execConverCorrectTiles(0)
+        tp.execImcltRbg(True)
+        tp.execConverCorrectTiles(7)   # rewrites the CLT while imclt_rbg may still run
+        img = np.stack([tp.getRBG(c) for c in range(sc.num_cams)])
+        clt = tp.getCLT(0)
+        tp.close()
+        return img, clt
+
+    i0, c0 = run(False)
+    i1, c1 = run(True)
+    same_img = np.array_equal(i0.view(np.int32), i1.view(np.int32))
+    same_clt = np.array_equal(c0.view(np.int32), c1.view(np.int32))
+    log("  bit-identical: images=%s, second CLT=%s" % (same_img, same_clt))
+    assert same_img and same_clt
