@@ -1654,13 +1654,20 @@ __global__ __launch_bounds__(128) void k_geometry_adjoint(const PortGeomDev* __r
     }
 }
 
-// Column sums of rows x cols doubles (one thread per column).
-__global__ __launch_bounds__(128) void k_sum_rows(const double* __restrict__ a, int rows, int cols, double* __restrict__ out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+// Column sums of rows x cols doubles: one 256-thread workgroup per column (grid = cols), a strided
+// partial sum per thread, then a tree in LDS (a fixed order: the same sums on every run).
+__global__ __launch_bounds__(256) void k_sum_rows(const double* __restrict__ a, int rows, int cols, double* __restrict__ out) {
+    __shared__ double s_part[256];
+    const int c = blockIdx.x;
     double s = 0.0;
-    for (int r = 0; r < rows; r++) s += a[(size_t)r * cols + c];
-    out[c] = s;
+    for (int r = threadIdx.x; r < rows; r += 256) s += a[(size_t)r * cols + c];
+    s_part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s_part[threadIdx.x] += s_part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = s_part[0];
 }
 
 // Disparity step of the LMA after the extrinsic step de: dd_t = -(g_d + h . de) / hd, limited to

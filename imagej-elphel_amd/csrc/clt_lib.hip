@@ -1824,7 +1824,7 @@ static int geo_eval(clt_handle* h, const float* disp, double lambda, double* sum
     const int blocks = (n + GEO_ADJ_TASKS - 1) / GEO_ADJ_TASKS;
     hipLaunchKernelGGL(k_geometry_adjoint, dim3(blocks), dim3(128), 0, h->stream, h->d_port_geom, h->d_rtab, h->d_gwords,
                        h->d_gtxy, disp, h->d_adj, n, lambda, h->d_gpart, h->d_tstate);
-    hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(128), 0, h->stream, h->d_gpart, blocks, GEO_ADJ_SUMS, h->d_gsum);
+    hipLaunchKernelGGL(k_sum_rows, dim3(GEO_ADJ_SUMS), dim3(256), 0, h->stream, h->d_gpart, blocks, GEO_ADJ_SUMS, h->d_gsum);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemcpyAsync(sums, h->d_gsum, GEO_ADJ_SUMS * 8, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(h, sync_streams(h));
