@@ -179,20 +179,22 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
     T = len(sc.tasks)
     subsets = [np.sort(rng.choice(T, int(round(T * rng.uniform(0.05, 1.0))), replace=False)) for _ in range(frames)]
     total = sum(len(x) for x in subsets)
+    # the frames' task lists are the caller's data, built before the timed regions of both paths
+    frame_tasks = [np.ascontiguousarray(sc.tasks[sub]) for sub in subsets]
     # plain per-frame path
     tp = make_processor(pk, sc)
     tp.setAsync(False)
     best_plain = 1e30
     for _ in range(repeats):
         t0 = time.perf_counter()
-        for sub in subsets:
-            tp.setTasks(sc.tasks[sub])
+        for ft in frame_tasks:
+            tp.setTasks(ft)
             run_step(tp)
         best_plain = min(best_plain, time.perf_counter() - t0)
     tp.setTiming(True)
     tp.resetTiming()
-    for sub in subsets:
-        tp.setTasks(sc.tasks[sub])
+    for ft in frame_tasks:
+        tp.setTasks(ft)
         run_step(tp)
     plain_gpu = sum(tp.getTiming(k)[0] for k in ("convert", "corr2d", "corr_lma", "textures", "imclt"))
     tp.close()
@@ -203,7 +205,7 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
     bt.setTexParams(keep_weights=0)
     for f in range(frames):
         bt.setBayerImagesFrame(f, sc.images)
-        bt.setTasksFrame(f, sc.tasks[subsets[f]])
+        bt.setTasksFrame(f, frame_tasks[f])
     tune = bt.autotune()
     bt.execBatch()   # graph capture
     bt.sync()
@@ -211,7 +213,7 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
     for _ in range(repeats):
         t0 = time.perf_counter()
         for f in range(frames):
-            bt.setTasksFrame(f, sc.tasks[subsets[f]])
+            bt.setTasksFrame(f, frame_tasks[f])
         bt.execBatch()
         bt.sync()
         best_batch = min(best_batch, time.perf_counter() - t0)
