@@ -1481,8 +1481,15 @@ int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
     auto inter = fz ? k_corr_inter<true, true> : k_corr_inter<false, true>;
     hipLaunchKernelGGL(inter, g4, dim3(256), 0, h->stream, h->d_rig_mix, h->d_rig_mix + T * 3 * 256, nullptr, h->d_tasks,
                        h->task_words, n, h->d_lpf_corr, h->d_corr_args, h->d_rig_args, h->d_inter, h->d_inter_rec);
-    hipLaunchKernelGGL(lma, g_lma, dim3(64 * LMA_WAVES), 0, h->stream, h->d_rig_corr + T * 6 * CLT_CORR_LEN,
+    // the aux quad's LMA on the side stream, beside the main quad's and the inter-camera kernels
+    // (they read only the two quads' mean CLT and their own buffers)
+    join_side(h);
+    HIPCHK(h, hipEventRecord(h->ev_side_fork, h->stream));
+    HIPCHK(h, hipStreamWaitEvent(h->stream_side, h->ev_side_fork, 0));
+    hipLaunchKernelGGL(lma, g_lma, dim3(64 * LMA_WAVES), 0, h->stream_side, h->d_rig_corr + T * 6 * CLT_CORR_LEN,
                        h->d_rig_disp + T * CLT_DISP_LAYERS, n, h->d_lma_rig, h->d_rig_dirs + T * 8);
+    HIPCHK(h, hipEventRecord(h->ev_side, h->stream_side));
+    h->side_pending = true;
     auto lma_x = (nc == 4) ? k_corr_lma_inter<4> : k_corr_lma_inter<3>;
     const clt_rig_params& rp = h->rp;
     hipLaunchKernelGGL(lma_x, g_lma, dim3(64 * LMA_WAVES), 0, h->stream, h->d_inter, h->d_inter_rec, h->d_inter_res, n,
@@ -1497,6 +1504,7 @@ int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
                            h->d_inter_res, n, h->d_lma_rig, rp.use_poly, rp.use_xy_poly, rp.notch_mode);
     }
     HIPCHK(h, hipStreamWaitEvent(h->stream, h->ev_lma, 0));   // the main quad's LMA
+    join_side(h);                                              // the aux quad's LMA
     hipLaunchKernelGGL(k_rig_bimap, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_tasks, h->task_words, n,
                        h->d_rig_disp, h->d_rig_dirs, h->d_rig_disp + T * CLT_DISP_LAYERS, h->d_rig_dirs + T * 8,
                        h->d_inter_res, h->rig_target ? h->d_rig_target : nullptr, h->d_bimap, h->d_rig_extra,
