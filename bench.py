@@ -1,14 +1,15 @@
 """Benchmark of the CLT tile processor on one MI355X.
 
 Workload (one "step" = one frame through the plain per-frame GPU path): 4 x 2592x1936 Bayer
-(GR/BG) float32 mosaics, 78,408 tiles, dense task list, per-tile disparity a smooth random field
+(RGGB) float32 mosaics, 78,408 tiles, dense task list, per-tile disparity a smooth random field
 0..20 px, random per-tile aberration kernels; convert_direct -> correlate2D (6 pairs + combined
 strip + disparity, per-tile LMA chain) -> imclt_rbg (with LPF) -> textures (RGBA).  imclt_rbg and
 textures read only the converted tiles, so the LMA chain (on its own stream) overlaps them and the
 next convert.  Inputs are resident in HBM before the timed region; outputs stay on the device.
 
 Prints one JSON line: tiles/s, HBM roofline of the dominant kernel (HIP events on the library's
-stream) and the CPU baseline (the FP64 C restatement of the reference CPU path, all host cores).
+stream), the CPU baseline (the C restatement of the reference CPU path in FP64 and FP32 on the
+host cores) and the error of each output against that FP64 run.
 """
 import argparse
 import csv
@@ -30,6 +31,7 @@ sys.path.insert(0, ROOT)
 METRIC = "CLT tiles/s per frame (convert+corr+imclt), % HBM roofline; x vs Java CPU"
 HBM_PEAK_GBS = 8000.0
 W, H, C, K = 2592, 1936, 4, 3
+RED = (0, 0)   # Bayer RGGB: red at even rows and even columns
 
 
 def algorithmic_bytes(sc, planes, n_pairs, strip_h):
@@ -50,31 +52,107 @@ def algorithmic_bytes(sc, planes, n_pairs, strip_h):
     }
 
 
-def cpu_baseline(sc, min_s=10.0, max_passes=20):
-    """FP64 oracle (C restatement of the reference CPU path, one worker per host core, atomic tile
-    counter as ImageDtt.java:5847-5872) on the same frame, repeated for at least min_s seconds."""
-    import oracle as orc
-    cores = os.cpu_count() or 1
-    g = orc.make_geom(W, H)
+def host_cores():
+    """Host threads the CPU baseline uses: the cores this process may run on, at most 16 (one
+    GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def java_probe():
+    """The reference CPU path is Java (ImageDtt) and needs a JVM plus the ij 1.52e and Jama 1.0.3
+    jars, which the reference does not ship; the run records what it found."""
+    java = shutil.which("java")
+    return {"java": java, "reference_jars": None,
+            "ran": False, "note": "no JVM" if java is None else "JVM present, ij/Jama jars absent"}
+
+
+def oracle_frame(orc, sc, precision, cores):
+    """The FP64 (or FP32) oracle of one frame: convert, correlate2D with the LMA chain, textures
+    (RGBA), imclt_rbg with the LPF, as the GPU step computes them."""
+    g = orc.make_geom(W, H, red_row=RED[0], red_col=RED[1])
     valid = np.zeros((H // 8) * (W // 8), np.uint8)
     valid[((sc.tasks[:, 1] >> 16) * (W // 8) + (sc.tasks[:, 1] & 0xFFFF))] = 1
-    tp = orc.default_tex_params(keep_weights=0)
-    lpf = orc.lpf(0.8)
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        clt = orc.convert_frame(g, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets, nthreads=cores)
-        orc.corr_frame(g, clt, sc.tasks, nthreads=cores)
-        orc.textures_frame(g, clt, sc.tasks, tex_params=tp, nthreads=cores)
-        orc.imclt_frame(g, clt, valid, lpf64=lpf, nthreads=cores)
-        del clt
-        n += 1
-        dt = time.perf_counter() - t0
-        if dt >= min_s or n >= max_passes:
-            break
-    return {"value": round(n * len(sc.tasks) / dt, 1), "unit": "tiles/s", "cores": cores, "kind": "port",
-            "sample": "the same 78408-tile frame, convert+corr+textures+imclt in FP64, %d pass(es) in %.1f s"
-                      % (n, dt)}
+    clt = orc.convert_frame(g, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets, nthreads=cores,
+                            precision=precision)
+    corr, combo, disp = orc.corr_frame(g, clt, sc.tasks, nthreads=cores, precision=precision)
+    tex, _ = orc.textures_frame(g, clt, sc.tasks, tex_params=orc.default_tex_params(keep_weights=0), nthreads=cores,
+                                precision=precision)
+    img = orc.imclt_frame(g, clt, valid, lpf64=orc.lpf(0.8), nthreads=cores, precision=precision)
+    return {"clt": clt, "corr": corr, "combo": combo, "disp": disp, "tex": tex, "img": img}
+
+
+def cpu_baseline(sc, passes=5):
+    """The oracle (C restatement of the reference CPU path, one worker per host core pulling tiles
+    from an atomic counter as ImageDtt.java:5847-5872) on the same frame: FP64 and FP32 builds,
+    one warm-up then `passes` timed passes each, the median reported.  Returns the baseline and
+    the FP64 and FP32 outputs of the last passes."""
+    import oracle as orc
+    cores = host_cores()
+    legs, outs = {}, {}
+    for prec in (64, 32):
+        oracle_frame(orc, sc, prec, cores)   # warm-up
+        times = []
+        for _ in range(passes):
+            t0 = time.perf_counter()
+            out = oracle_frame(orc, sc, prec, cores)
+            times.append(time.perf_counter() - t0)
+        outs[prec] = out
+        med = float(np.median(times))
+        legs["fp%d" % prec] = {"tiles_per_s": round(len(sc.tasks) / med, 1), "median_s": round(med, 4),
+                               "passes_s": [round(t, 4) for t in times]}
+    return ({"value": legs["fp64"]["tiles_per_s"], "unit": "tiles/s", "cores": cores, "kind": "port",
+             "cpu_model": cpu_model(), "legs": legs, "reference_probe": java_probe(),
+             "sample": "the same 78408-tile frame, convert+correlate2D(LMA)+textures+imclt; per build 1 warm-up "
+                       "and the median of %d passes; value = the FP64 build" % passes}, outs)
+
+
+def _rel_rms(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    m = np.isfinite(b)
+    den = np.sqrt(np.mean(b[m] ** 2)) or 1.0
+    return float(np.sqrt(np.mean((a[m] - b[m]) ** 2)) / den)
+
+
+def output_errors(gpu, ref):
+    """Error of each output against a reference run of the oracle: relative RMS of the CLT, pair
+    correlations, combined strips, texture tiles and images; max |difference| (px) and the count
+    of tiles past 0.01 px for the disparities (integer maximum, centre of mass before the LMA, the
+    final disparity after the LMA chain, the LMA disparity)."""
+    e = {}
+    for k in ("clt", "corr", "combo", "tex", "img"):
+        e[k + "_rel_rms"] = float("%.3g" % _rel_rms(np.asarray(gpu[k]).reshape(ref[k].shape), ref[k]))
+    for name, k in (("int", 0), ("cm", 10), ("final", 1), ("lma", 8)):
+        g, o = gpu["disp"][:, k].astype(np.float64), ref["disp"][:, k]
+        both = np.isfinite(g) & np.isfinite(o)
+        d = np.abs(g[both] - o[both])
+        e["disp_" + name] = {"max_px": float("%.3g" % (d.max() if d.size else 0.0)),
+                             "over_0.01": int((d > 0.01).sum()),
+                             "nan_mismatch": int((np.isfinite(g) != np.isfinite(o)).sum())}
+    return e
+
+
+def gpu_outputs(tp, sc):
+    """The outputs of the last step, fetched from the device (outside the timed region)."""
+    clt = np.stack([tp.getCLT(c) for c in range(C)])
+    corr, combo, disp = tp.getCorr2D()
+    tex, _ = tp.getTextures()
+    img = np.stack([tp.getRBG(c) for c in range(C)]).reshape(C, K, H + 8, W + 8)
+    return {"clt": clt, "corr": corr, "combo": combo, "disp": disp, "tex": tex, "img": img}
 
 
 def copy_calibration(mib):
@@ -144,7 +222,7 @@ def pmc_child(args):
     """The profiled child: one copy of known size, then two frames."""
     pk = importlib.import_module("imagej-elphel_amd")
     copy_calibration(COPY_MIB)
-    sc = pk.synth.make_scene(W, H, seed=args.seed, disparity=("field", 0.0, 20.0))
+    sc = pk.synth.make_scene(W, H, seed=args.seed, disparity=("field", 0.0, 20.0), red_row=RED[0], red_col=RED[1])
     tp = make_processor(pk, sc)
     for _ in range(2):
         run_step(tp)
@@ -153,7 +231,7 @@ def pmc_child(args):
 
 
 def make_processor(pk, sc):
-    tp = pk.GPUTileProcessor(W, H, C, K)
+    tp = pk.GPUTileProcessor(W, H, C, K, red_row=RED[0], red_col=RED[1])
     tp.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
     tp.setBayerImages(sc.images, force=True)
     tp.setTasks(sc.tasks)
@@ -199,7 +277,7 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
     plain_gpu = sum(tp.getTiming(k)[0] for k in ("convert", "corr2d", "corr_lma", "textures", "imclt"))
     tp.close()
     # batched
-    bt = pk.GPUTileBatch(frames, W, H, C, K)
+    bt = pk.GPUTileBatch(frames, W, H, C, K, red_row=RED[0], red_col=RED[1])
     bt.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
     bt.setCorrSigma(0.8)
     bt.setTexParams(keep_weights=0)
@@ -238,7 +316,7 @@ def adjoint_bench(pk, sc, repeats=5, seed=11):
     tx, ty = sc.tasks[:, 1] & 0xFFFF, (sc.tasks[:, 1] >> 16) & 0xFFFF
     d = sc.disparity[ty, tx].astype(np.float32)
     words = np.full(T, 0xFF, np.int32)
-    tp = pk.GPUTileProcessor(W, H, C, K)
+    tp = pk.GPUTileProcessor(W, H, C, K, red_row=RED[0], red_col=RED[1])
     tp.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
     tp.setBayerImages(sc.images, force=True)
     tp.setCorrSigma(0.8)
@@ -390,7 +468,7 @@ def main():
         copy_calibration(args.calibrate_copy)
     print("bench: generating the 2592x1936 quad frame ...", file=sys.stderr, flush=True)
     t = time.perf_counter()
-    sc = pk.synth.make_scene(W, H, seed=args.seed, disparity=("field", 0.0, 20.0))
+    sc = pk.synth.make_scene(W, H, seed=args.seed, disparity=("field", 0.0, 20.0), red_row=RED[0], red_col=RED[1])
     print("bench: scene ready in %.1f s, %d tiles" % (time.perf_counter() - t, len(sc.tasks)), file=sys.stderr, flush=True)
 
     tp = make_processor(pk, sc)
@@ -448,6 +526,7 @@ def main():
             gbs = ab[name] / (avg * 1e-3) / 1e9
             isolated[name].update({"GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)})
     roof["isolated_frac"] = isolated[dom].get("frac")
+    gpu_out = gpu_outputs(tp, sc)
     tp.close()
     batch = None
     if not args.no_batch:
@@ -473,18 +552,23 @@ def main():
         if dom in traffic:
             roof["traffic"] = round(traffic[dom])
         roof["counter_scale"] = {k: round(v, 3) for k, v in traffic_info.items()}
-    cpu = None
+    cpu, errors = None, None
     if not args.no_cpu:
         print("bench: timing the CPU baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(sc)
+        cpu, ref = cpu_baseline(sc)
+        errors = {"vs": "the FP64 oracle on the same frame", "gpu": output_errors(gpu_out, ref[64]),
+                  "oracle_fp32": output_errors(ref[32], ref[64])}
+        del ref
+    del gpu_out
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "tiles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "quad 2592x1936 Bayer frame, 78408 tiles, dense tasks, disparity 0-20 px; "
+        "config": {"workload": "quad 2592x1936 Bayer RGGB frame, 78408 tiles, dense tasks, disparity 0-20 px; "
                                "convert+correlate2D(6 pairs, per-tile LMA chain)+textures(RGBA)+imclt_rbg",
                    "tiles": T, "cameras": C, "colors": K, "pairs": 6},
-        "roofline": roof, "kernels": kernels, "kernels_isolated": isolated, "cpu_baseline": cpu, "batch": batch, "adjoint": adjoint,
+        "roofline": roof, "kernels": kernels, "kernels_isolated": isolated, "cpu_baseline": cpu, "errors": errors,
+        "batch": batch, "adjoint": adjoint,
         "kernel_prep": kprep, "rig": rig,
     }
     print(json.dumps(out))

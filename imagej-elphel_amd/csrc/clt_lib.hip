@@ -609,6 +609,7 @@ int clt_set_kernels(clt_handle* h, int32_t cam, const float* kernels, const floa
     h->kernels_set[cam] = true;
     h->use_kernels = true;
     for (int c = 0; c < g.num_cams; c++) h->use_kernels = h->use_kernels && h->kernels_set[c];
+    h->graph_epoch++;   // a captured convert holds the kernel pointers of its capture
     return CLT_OK;
 }
 
@@ -680,6 +681,7 @@ int clt_prepare_kernels(clt_handle* h, int32_t cam, const float* raw, int32_t ks
     h->kernels_set[cam] = true;
     h->use_kernels = true;
     for (int c = 0; c < g.num_cams; c++) h->use_kernels = h->use_kernels && h->kernels_set[c];
+    h->graph_epoch++;   // a captured convert holds the kernel pointers of its capture
     return CLT_OK;
 }
 
@@ -744,6 +746,7 @@ int clt_clear_kernels(clt_handle* h) {
     if (!h) return CLT_E_ARG;
     for (int c = 0; c < CLT_MAX_CAMS; c++) h->kernels_set[c] = false;
     h->use_kernels = false;
+    h->graph_epoch++;   // a captured convert holds the kernel pointers of its capture
     return CLT_OK;
 }
 
@@ -1645,7 +1648,10 @@ int clt_exec_textures_rig(clt_handle* h) {
         (rc = dalloc(h, (void**)&h->d_rig_tex_stats, 2 * T * 16 * 4)))
         return rc;
     if (planes > h->rig_tex_planes) {   // the buffer grows when keep_weights is switched on
-        if (h->d_rig_tex) HIPCHK(h, hipFree(h->d_rig_tex));
+        if (h->d_rig_tex) {
+            HIPCHK(h, sync_streams(h));   // the previous rig textures may still be written
+            HIPCHK(h, hipFree(h->d_rig_tex));
+        }
         h->d_rig_tex = nullptr;
         if ((rc = dalloc(h, (void**)&h->d_rig_tex, 2 * T * planes * 256 * 4))) return rc;
         h->rig_tex_planes = planes;

@@ -143,32 +143,36 @@ def clt_kernels(num_cams, num_colors, kernels_vert, kernels_hor, rng, sigma=(0.6
         r = np.arange(g.shape[0])[:, None]
         return 0.5 * g[r, idx - 1] + g[r, idx] + 0.5 * g[r, idx + 1]
 
-    dx, dy = down(gx, sx), down(gy, sy)                      # separable 15x15 = dy (x) dx
-    k15 = dy[:, :, None] * dx[:, None, :]
+    # The 15x15 kernel is separable, k15[y][x] = dy[y] * dx[x], so every step below is applied to
+    # the two 15-vectors and the 8x8 quadrants are outer products.
+    dx, dy = down(gx, sx), down(gy, sy)
     # clt_normalize_kernel (ID:4674-4696) with cos(pi i/16) cos(pi j/16) weights (QC:466-474)
     cw = np.cos(np.pi * np.abs(np.arange(-7, 8)) / 16)
-    s = (k15 * cw[None, :, None] * cw[None, None, :]).sum((1, 2))
-    k15 *= (16.0 / s)[:, None, None]
-    # clt_symmetrize_kernel (ID:4698-4723)
-    i = np.arange(8)[:, None]
-    j = np.arange(8)[None, :]
-    k0 = k15[:, 7 - i, 7 - j]
-    k1 = k15[:, 7 - i, 7 + j]
-    k2 = k15[:, 7 + i, 7 - j]
-    k3 = k15[:, 7 + i, 7 + j]
-    sym = np.zeros((n, 4, 8, 8))
-    sym[:, 0] = 0.25 * (k0 + k1 + k2 + k3)
-    sym[:, 1, :, 0:7] = (0.25 * (-k0 + k1 - k2 + k3))[:, :, 1:8]
-    sym[:, 2, 0:7, :] = (0.25 * (-k0 - k1 + k2 + k3))[:, 1:8, :]
-    sym[:, 3, 0:7, 0:7] = (0.25 * (-k0 + k1 - k2 + k3))[:, 1:8, 1:8]   # sign pattern as in ID:4717
+    dy = dy * (16.0 / ((dy @ cw) * (dx @ cw)))[:, None]
+    # clt_symmetrize_kernel (ID:4698-4723): with k0..k3 = k15[7 -+ i][7 -+ j],
+    #   CC = (k0 + k1 + k2 + k3) / 4             = yp[i] xp[j] / 4
+    #   SC = (-k0 + k1 - k2 + k3) / 4, j >= 1     = yp[i] xm[j] / 4
+    #   CS = (-k0 - k1 + k2 + k3) / 4, i >= 1     = ym[i] xp[j] / 4
+    #   SS = (-k0 + k1 - k2 + k3) / 4, i, j >= 1  = yp[i] xm[j] / 4 (the sign pattern of ID:4717)
+    # the sine quadrants shifted by one (index 1..7 stored at 0..6, the last row/column zero)
+    ii = np.arange(8)
+    yp, ym = dy[:, 7 - ii] + dy[:, 7 + ii], dy[:, 7 + ii] - dy[:, 7 - ii]
+    xp, xm = dx[:, 7 - ii] + dx[:, 7 + ii], dx[:, 7 + ii] - dx[:, 7 - ii]
+
+    def shift1(v):
+        return np.concatenate([v[:, 1:], np.zeros((v.shape[0], 1))], axis=1)
+
+    vy = (yp, yp, shift1(ym), shift1(yp))
+    vx = (xp, shift1(xm), xp, shift1(xm))
     # clt_dtt3_kernel (ID:4725-4734): dttt_iiie(quadrant q, mode q); upload layout [v][h] =
-    # M_v . X . M_h^T with M_h = (q & 1 ? SIIIe : CIIIe), M_v = (q & 2 ? SIIIe : CIIIe)
+    # M_v . X . M_h^T with M_h = (q & 1 ? SIIIe : CIIIe), M_v = (q & 2 ? SIIIe : CIIIe); for
+    # X = outer(vy, vx) / 4 that is outer(M_v vy, M_h vx) / 4
     c3, s3 = _dct3e_mats()
     kern = np.empty((n, 4, 8, 8))
     for q in range(4):
         mh = s3 if q & 1 else c3
         mv = s3 if q & 2 else c3
-        kern[:, q] = np.einsum("ai,nij,bj->nab", mv, sym[:, q], mh)
+        kern[:, q] = 0.25 * (vy[q] @ mv.T)[:, :, None] * (vx[q] @ mh.T)[:, None, :]
     # extras: data_x, data_y, center_x, center_y (ID:4668-4671), derivatives (ID:4792-4851)
     extra = np.zeros((n, 8))
     extra[:, 0] = 0.5 * (sx - 32)
@@ -259,7 +263,9 @@ def make_scene(width, height, num_cams=4, num_colors=3, rxy=None, disparity=3.0,
                red_row=0, red_col=1, kernels=True, task_fraction=1.0, target_disparity=None,
                noise=True, flags=0xff):
     """One frame.  disparity: scalar, [TY][TX] field, or ('field', lo, hi) for a smooth random
-    field; target_disparity: disparity used to build the tasks (default: the ground truth)."""
+    field; target_disparity: disparity used to build the tasks (default: the ground truth);
+    kernels: True (random kernels of this camera set), False (none) or a (kernels, offsets) pair
+    from clt_kernels (frames of one camera set share its kernels)."""
     rng = np.random.default_rng(seed)
     tiles_x, tiles_y = width // 8, height // 8
     if rxy is None:
@@ -273,7 +279,9 @@ def make_scene(width, height, num_cams=4, num_colors=3, rxy=None, disparity=3.0,
     images = camera_images(width, height, rxy, d_tiles, rng, num_colors, red_row, red_col, noise=noise)
     kh, kv = width // 16 + 2, height // 16 + 2
     kern = offs = None
-    if kernels:
+    if isinstance(kernels, tuple):
+        kern, offs = kernels
+    elif kernels:
         kern, offs = clt_kernels(num_cams, num_colors, kv, kh, rng)
     mask = None
     if task_fraction < 1.0:

@@ -36,6 +36,10 @@ typedef float real;
 #else
 typedef double real;
 #endif
+/* Port weights of the texture combiner are double in both builds: in float, exp(-k d) underflows
+ * to 0 for every port of a pixel where all ports disagree, and the reference's double arithmetic
+ * (ID:2880-2935) keeps their ratios there. */
+typedef double wreal;
 
 #define EXPORT __attribute__((visibility("default")))
 #define N8 8
@@ -484,7 +488,7 @@ static void correlate_single_color_fd(real d1[4][64], real d2[4][64], real fat_z
             s1 += d1[n][i] * d1[n][i];
             s2 += d2[n][i] * d2[n][i];
         }
-        a2[i] = (real)sqrt((double)(s1 * s2));
+        a2[i] = (real)sqrt((double)s1 * (double)s2);   /* in double as C2:213 (no float underflow) */
         sa2 += a2[i];
     }
     real fz2 = sa2 / 64 * fat_zero * fat_zero;
@@ -1481,16 +1485,16 @@ static void tile_debayer_shot_corr(int rb, const real* tile, const real* window2
 }
 
 /* ID:3002-3019 dust_remove */
-static void dust_remove(real pw[][256], int np) {
+static void dust_remove(wreal pw[][256], int np) {
     for (int i = 0; i < 256; i++) {
         int wi = 0;
         for (int ip = 1; ip < np; ip++)
             if (pw[ip][i] < pw[wi][i]) wi = ip;
-        real avg = 0;
+        wreal avg = 0;
         for (int ip = 1; ip < np; ip++)
             if (ip != wi) avg += pw[ip][i];
         avg /= (np - 1);
-        real scale = 1 + (avg - pw[wi][i]) / (avg * (np - 1));
+        wreal scale = 1 + (avg - pw[wi][i]) / (avg * (np - 1));
         for (int ip = 1; ip < np; ip++)
             if (ip != wi) pw[ip][i] *= scale;
         pw[wi][i] *= pw[wi][i] / avg;
@@ -1551,7 +1555,7 @@ static void tex_task(void* vctx, int t) {
     const real* chw = c->col_weights;
     int used = 0;
     for (int ip = 0; ip < ports; ip++) used += (port_mask >> ip) & 1;
-    real (*pw)[256] = calloc((size_t)ports, sizeof(real) * 256);
+    wreal (*pw)[256] = calloc((size_t)ports, sizeof(wreal) * 256);
     real avg[3][256], alpha[256], rms[4][256];
     for (int i = 0; i < 256; i++) {   /* rms for keep_weights (ID:2858-2874) */
         real sw = 0;
@@ -1563,7 +1567,16 @@ static void tex_task(void* vctx, int t) {
                     s1 += deb[ip][chn][i];
                     s2 += deb[ip][chn][i] * deb[ip][chn][i];
                 }
+#ifdef ORC_FP32
+            /* the same standard deviation in two passes: s0 s2 - s1^2 cancels in float */
+            real m = s1 / s0, v = 0;
+            for (int ip = 0; ip < ports; ip++)
+                if ((port_mask >> ip) & 1) v += (deb[ip][chn][i] - m) * (deb[ip][chn][i] - m);
+            rms[chn][i] = (real)sqrt((double)(v / s0));
+            (void)s2;
+#else
             rms[chn][i] = (real)sqrt((double)(s0 * s2 - s1 * s1)) / s0;
+#endif
             sw += chw[chn] * rms[chn][i] * rms[chn][i];
         }
         rms[3][i] = (real)sqrt((double)sw);
@@ -1591,9 +1604,9 @@ static void tex_task(void* vctx, int t) {
                     dc /= lt_window[i];
                     d += chw[chn] * dc * dc;
                 }
-                d = (real)exp((double)(-pd2r[ip] * d));
-                pw[pp[ip][0]][i] += d;
-                pw[pp[ip][1]][i] += d;
+                const wreal e = exp((double)(-pd2r[ip] * d));
+                pw[pp[ip][0]][i] += e;
+                pw[pp[ip][1]][i] += e;
             }
             int b1 = 0;
             for (int ip = b1 + 1; ip < ports; ip++)
@@ -1601,7 +1614,7 @@ static void tex_task(void* vctx, int t) {
             int b2 = (b1 == 0) ? 1 : 0;
             for (int ip = b2 + 1; ip < ports; ip++)
                 if ((ip != b1) && (pw[ip][i] > pw[b2][i])) b2 = ip;
-            real w1 = pw[b1][i] / (pw[b1][i] + pw[b2][i]);
+            real w1 = (real)(pw[b1][i] / (pw[b1][i] + pw[b2][i]));
             real w2 = 1 - w1;
             for (int chn = 0; chn < numcol; chn++) avg[chn][i] = w1 * deb[b1][chn][i] + w2 * deb[b2][chn][i];
             for (int ip = 0; ip < ports; ip++)
@@ -1612,14 +1625,14 @@ static void tex_task(void* vctx, int t) {
                         dc /= lt_window[i];
                         d2 += chw[chn] * dc * dc;
                     }
-                    pw[ip][i] = (real)exp((double)(-ksigma * d2));
+                    pw[ip][i] = exp((double)(-ksigma * d2));
                 }
-            real k = 0;
+            wreal k = 0;
             for (int ip = 0; ip < ports; ip++) k += pw[ip][i];
             k = 1 / k;
             for (int chn = 0; chn < numcol; chn++) {
                 avg[chn][i] = 0;
-                for (int ip = 0; ip < ports; ip++) avg[chn][i] += k * pw[ip][i] * deb[ip][chn][i];
+                for (int ip = 0; ip < ports; ip++) avg[chn][i] += (real)(k * pw[ip][i]) * deb[ip][chn][i];
             }
         }
     } else if (used > 0) {
@@ -1632,10 +1645,10 @@ static void tex_task(void* vctx, int t) {
     }
     if (c->prm.dust_remove && (used == 4)) dust_remove(pw, ports);
     for (int i = 0; i < 256; i++) {
-        alpha[i] = 0;
+        wreal a = 0;
         for (int ip = 0; ip < ports; ip++)
-            if ((port_mask >> ip) & 1) alpha[i] += pw[ip][i];
-        alpha[i] *= lt_window[i] / used;
+            if ((port_mask >> ip) & 1) a += pw[ip][i];
+        alpha[i] = (real)(a * (lt_window[i] / used));   /* ID:2958: alpha *= lt_window / used */
     }
     if (stats) {   /* max_diff (ID:2965-2981), ports_rgb (ID:2982-2996) */
         for (int ip = 0; ip < ports; ip++) {
@@ -1661,12 +1674,12 @@ static void tex_task(void* vctx, int t) {
     }
     /* ID:2789-2801 final mix of the non-debayered tiles with the port weights */
     for (int i = 0; i < 256; i++) {
-        real sw = 0;
+        wreal sw = 0;
         for (int ip = 0; ip < ports; ip++) sw += pw[ip][i];
         if (sw != 0) sw = 1 / sw;
         for (int chn = 0; chn < numcol; chn++) {
             real s = 0;
-            for (int ip = 0; ip < ports; ip++) s += sw * pw[ip][i] * iclt[ip][chn][i];
+            for (int ip = 0; ip < ports; ip++) s += (real)(sw * pw[ip][i]) * iclt[ip][chn][i];
             out[chn * 256 + i] = s;
         }
         out[3 * 256 + i] = alpha[i];

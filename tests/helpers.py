@@ -71,3 +71,57 @@ def robust_rel_rms(a, b, frac=1e-3, tol=1e-3):
     keep = np.argsort(err)[: max(1, int(np.ceil(len(err) * (1 - frac))))]
     r = float(np.sqrt(np.mean(err[keep] ** 2)) / scale)
     return r, float(np.mean(err > tol * scale))
+
+
+def fp32_sensitive(run, clt64, moved, draws=3, seed=1):
+    """Tasks whose FP64 outputs are not determined at float32 precision.
+
+    run(clt) returns the FP64 oracle's outputs of a stage on the CLT clt [C][tiles][K][4][8][8];
+    moved(out, base) returns a bool mask over the tasks of outputs moved past the bars.  The CLT is
+    perturbed `draws` times by Gaussian noise of half a float32 ulp of the largest coefficient of
+    each (camera, tile, colour) block (2^-24 of it: one rounding of a float32 result) and the
+    stage rerun.  Returns (mask of the tasks moved by any draw, the unperturbed outputs).
+
+    Such tasks occur where a camera's window lies mostly outside the frame: the border clamp
+    leaves many bins with rounding noise only, which the phase correlation normalises to unit
+    weight; and where a discrete decision of the LMA chain (a convexity test, the rounding of the
+    start point, an accepted step) sits within rounding of its threshold.  No float32 computation,
+    the reference's own GPU path included, can be held to the bars there."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    base = run(clt64)
+    C, T, K = clt64.shape[:3]
+    blk = clt64.reshape(C, T, K, 256)
+    scale = np.abs(blk).max(-1, keepdims=True) * 2.0 ** -24
+    bad = None
+    for _ in range(draws):
+        noisy = (blk + rng.standard_normal(blk.shape) * scale).reshape(clt64.shape)
+        m = moved(run(noisy), base)
+        del noisy
+        bad = m if bad is None else (bad | m)
+    return bad, base
+
+
+def corr_moved(corr_tol=1e-3, disp_tol=0.01, layers=(0, 10, 1, 8, 3, 5)):
+    """moved() for correlate2D outputs (corr, combo, disp): a pair correlation value moved by more
+    than corr_tol, or a disparity layer (integer maximum, centre of mass, final, LMA, horizontal,
+    vertical) by more than disp_tol or between NaN and finite."""
+    import numpy as np
+
+    def moved(out, base):
+        n = len(base[0])
+        bad = np.nan_to_num(np.abs(out[0] - base[0]).reshape(n, -1), nan=np.inf).max(1) > corr_tol
+        for k in layers:
+            a, b = out[2][:, k], base[2][:, k]
+            fa, fb = np.isfinite(a), np.isfinite(b)
+            bad |= fa != fb
+            both = fa & fb
+            bad[both] |= np.abs(a[both] - b[both]) > disp_tol
+        return bad
+    return moved
+
+
+def fp32_ill_conditioned(orc, g, clt64, tasks, draws=3, seed=1, pairs=None, nthreads=None, lma=None):
+    """fp32_sensitive for the oracle's correlate2D with the LMA chain: (mask, (corr, combo, disp))."""
+    return fp32_sensitive(lambda c: orc.corr_frame(g, c, tasks, pairs=pairs, nthreads=nthreads, lma=lma), clt64,
+                          corr_moved(), draws=draws, seed=seed)

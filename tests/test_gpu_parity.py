@@ -250,10 +250,12 @@ def test_textures_extra_disparity_port_masks(pk, orc, quad_scene):
     _assert_textures(r)
 
 
-def test_config0_pipeline_disparity(pk, orc):
-    """4 x 512x512 Bayer, constant 3.0 px disparity, all tiles: convert + corr + imclt."""
-    log("case: 512x512 quad frame, disparity 3.0 px, full pipeline")
-    sc = pk.synth.make_scene(512, 512, seed=20261203, disparity=3.0)
+@pytest.mark.parametrize("phase,red", [("GR/BG", (0, 1)), ("RGGB", (0, 0))])
+def test_config0_pipeline_disparity(pk, orc, phase, red):
+    """4 x 512x512 Bayer, constant 3.0 px disparity, all tiles: convert + corr + imclt, in the
+    reference's Bayer phase and in RGGB."""
+    log("case: 512x512 quad frame %s, disparity 3.0 px, full pipeline" % phase)
+    sc = pk.synth.make_scene(512, 512, seed=20261203, disparity=3.0, red_row=red[0], red_col=red[1])
     tp = gpu_processor(pk, sc)
     tp.setCorrSigma(0.8)
     tp.execConverCorrectTiles()
@@ -313,4 +315,44 @@ def test_batch_matches_per_frame(pk, orc):
         for c in range(4):
             assert np.abs(bt.getRBGFrame(f, c) - tp.getRBG(c)).max() <= 1e-6 * max(1.0, np.abs(tp.getRBG(c)).max())
         tp.close()
+    bt.close()
+
+
+def test_batch_follows_kernel_changes(pk):
+    """A captured batch graph holds the kernel pointers of its capture: after setting or clearing
+    the kernels the next clt_exec_batch must use the new ones.  Without kernels, then with them,
+    then without again, each against the per-frame path under the same kernels."""
+    scenes = [pk.synth.make_scene(128, 96, seed=60 + f, disparity=("field", 0.0, 4.0)) for f in range(2)]
+    bt = pk.GPUTileBatch(2, 128, 96)
+    bt.setCorrSigma(0.8)
+    for f, sc in enumerate(scenes):
+        bt.setBayerImagesFrame(f, sc.images)
+        bt.setTasksFrame(f, sc.tasks)
+
+    def per_frame(f, with_kernels):
+        tp = pk.GPUTileProcessor(128, 96)
+        if with_kernels:
+            tp.setConvolutionKernels(scenes[0].kernels, offsets=scenes[0].offsets, force=True)
+        tp.setBayerImages(scenes[f].images, force=True)
+        tp.setTasks(scenes[f].tasks)
+        tp.setCorrSigma(0.8)
+        tp.execConverCorrectTiles()
+        tp.execCorrelate2D()
+        c, _, d = tp.getCorr2D()
+        tp.close()
+        return c, d
+
+    for step, with_kernels in enumerate((False, True, False)):
+        if with_kernels:
+            bt.setConvolutionKernels(scenes[0].kernels, offsets=scenes[0].offsets, force=True)
+        elif step:
+            bt.clearConvolutionKernels()
+        bt.execBatch()
+        for f in range(2):
+            c1, d1 = per_frame(f, with_kernels)
+            c2, _, d2 = bt.getCorr2DFrame(f)
+            err = float(np.abs(c2 - c1).max())
+            log("case: batch step %d (kernels %s), frame %d: corr max |diff| %.3g" % (step, with_kernels, f, err))
+            assert err <= 1e-6 * max(1.0, float(np.abs(c1).max()))
+            assert np.array_equal(np.isnan(d1), np.isnan(d2))
     bt.close()
