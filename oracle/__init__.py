@@ -227,6 +227,9 @@ def lib(precision=64):
     L.orc_max_xs_ortho.argtypes = [_D, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_double, _D]
     L.orc_tile_debayer.argtypes = [ctypes.c_int, _D, _D]
+    L.orc_tile_combine_rgba.argtypes = [ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, ctypes.POINTER(TexParams),
+                                        ctypes.c_int, _D, _VP]
+    L.orc_dust_remove.argtypes = [_D, ctypes.c_int]
     L.orc_convert_frame.argtypes = [ctypes.POINTER(Geom), _VP, ctypes.c_int, _VP, _VP, _VP,
                                     ctypes.c_int, ctypes.c_int, _VP, ctypes.c_int, _D, _VP, ctypes.c_int]
     L.orc_imclt_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, _VP, _D, ctypes.c_int]
@@ -417,6 +420,33 @@ def tile_debayer(rb, tile, precision=64):
     out = np.zeros(256)
     lib(precision).orc_tile_debayer(int(rb), _dp(t), _dp(out))
     return out.reshape(16, 16)
+
+
+def tile_combine_rgba(deb, iclt, lt_window, port_mask=None, tex_params=None, chn_weights=None, precision=64):
+    """tile_combine_rgba (ID:2830-3000) on the debayered tiles deb [ports][3][256] followed by the
+    final mix (ID:2789-2801) of iclt [ports][3][256]; lt_window [256]; colour weights R, B, G
+    (default: corr_red 0.5, corr_blue 0.2, ID:1571-1581).  Returns (tile [planes][256], stats
+    [4*ports]) with planes = 4 + ports + 4 when tex_params.keep_weights, else 4."""
+    deb = _f64(deb).reshape(-1, 3, 256)
+    ports = deb.shape[0]
+    tp = tex_params or default_tex_params(ports)
+    mask = (1 << ports) - 1 if port_mask is None else port_mask
+    cw = _f64(np.array([0.5, 0.2, 1.0]) / 1.7 if chn_weights is None else chn_weights)
+    planes = 4 + ports + 4 if tp.keep_weights else 4
+    out = np.zeros((planes, 256))
+    stats = np.zeros(4 * ports)
+    lib(precision).orc_tile_combine_rgba(ports, mask, _dp(deb), _dp(_f64(iclt).reshape(ports, 3, 256)),
+                                         _dp(_f64(lt_window).reshape(256)), _dp(cw), ctypes.byref(tp), planes,
+                                         _dp(out), _dp(stats))
+    return out, stats
+
+
+def dust_remove(pw):
+    """dust_remove (ID:3002-3019) of port weights [ports][256]; returns the new weights."""
+    w = np.array(pw, np.float64, copy=True, order="C").reshape(len(pw), -1)
+    assert w.shape[1] == 256
+    lib().orc_dust_remove(_dp(w), w.shape[0])
+    return w
 
 
 # ---------------------------------------------------------------- frame level

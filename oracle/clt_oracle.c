@@ -1515,44 +1515,13 @@ typedef struct {
     double* stats_out;   /* [task][4*C] or NULL */
 } tex_ctx;
 
-/* ID:2659-2812 for one task tile, then tile_combine_rgba (ID:2830-3000) */
-static void tex_task(void* vctx, int t) {
-    tex_ctx* c = (tex_ctx*)vctx;
-    const clt_geom* g = c->f.g;
-    const int ports = g->num_cams, numcol = 3;
-    double* out = c->tiles_out + (size_t)t * c->planes * 256;
-    double* stats = c->stats_out ? c->stats_out + (size_t)t * 4 * ports : NULL;
-    int32_t flags = task_flags(g, c->tasks, t);
-    if (flags == 0) {
-        for (int i = 0; i < c->planes * 256; i++) out[i] = NAN;
-        if (stats)
-            for (int i = 0; i < 4 * ports; i++) stats[i] = NAN;
-        return;
-    }
-    int port_mask = flags & 0xff & ((1 << ports) - 1);
-    int tx, ty;
-    task_txy(g, c->tasks, t, &tx, &ty);
-    size_t tile = (size_t)ty * c->f.tiles_x + tx;
-    double extra = c->extra_disparity ? c->extra_disparity[t] : 0.0;
-    if (isnan(extra)) extra = 0;
-    int forced = (flags >> 24) & 1;   /* C2:97 getForcedDisparity */
-    real (*iclt)[3][256] = malloc(sizeof(real) * ports * 3 * 256);
-    real (*deb)[3][256] = malloc(sizeof(real) * ports * 3 * 256);
-    for (int ip = 0; ip < ports; ip++) {
-        for (int chn = 0; chn < numcol; chn++) {
-            real clt[4][64];
-            load_clt(&c->f, c->clt, ip, tile, chn, clt);
-            if (extra != 0 && !forced)
-                fract_shift(clt, (real)(extra * c->prm.port_offsets[ip][0] / c->prm.corr_magic_scale),
-                            (real)(extra * c->prm.port_offsets[ip][1] / c->prm.corr_magic_scale));
-            iclt_tile(clt, (real)0.25, c->lpf, iclt[ip][chn]);
-            tile_debayer_shot_corr(chn != 2, iclt[ip][chn], c->window2, (real)c->prm.min_shot,
-                                   (real)c->prm.scale_shot, deb[ip][chn]);
-        }
-    }
-    /* ---- tile_combine_rgba (ID:2830-3000), keep_weights = true ---- */
-    const real* lt_window = c->window2;
-    const real* chw = c->col_weights;
+/* tile_combine_rgba (ID:2830-3000) with keep_weights, on the debayered tiles deb[port][colour][256],
+ * then the final mix of the tiles iclt[port][colour][256] with the port weights (ID:2789-2801):
+ * out [planes][256] (RGBA, then with planes > 4 one weight per port and the rms of r, b, g and the
+ * weighted rms), stats [4 * ports] (max_diff, then the mean tones r, b, g of every port) or NULL. */
+static void combine_rgba(int ports, int port_mask, real (*deb)[3][256], real (*iclt)[3][256], const real* lt_window,
+                         const real* chw, const clt_tex_params* prm, int planes, double* out, double* stats) {
+    const int numcol = 3;
     int used = 0;
     for (int ip = 0; ip < ports; ip++) used += (port_mask >> ip) & 1;
     wreal (*pw)[256] = calloc((size_t)ports, sizeof(wreal) * 256);
@@ -1582,15 +1551,15 @@ static void tex_task(void* vctx, int t) {
         rms[3][i] = (real)sqrt((double)sw);
     }
     if (used > 1) {
-        real ksigma = (real)(1.0 / (2.0 * c->prm.diff_sigma * c->prm.diff_sigma));
+        real ksigma = (real)(1.0 / (2.0 * prm->diff_sigma * prm->diff_sigma));
         int npairs = 0, pp[CLT_MAX_CAMS * CLT_MAX_CAMS / 2][2];
         real pd2r[CLT_MAX_CAMS * CLT_MAX_CAMS / 2];
         for (int i = 0; i < ports; i++)
             if ((port_mask >> i) & 1)
                 for (int j = i + 1; j < ports; j++)
                     if ((port_mask >> j) & 1) {
-                        real dx = c->prm.port_offsets[j][0] - c->prm.port_offsets[i][0];
-                        real dy = c->prm.port_offsets[j][1] - c->prm.port_offsets[i][1];
+                        real dx = prm->port_offsets[j][0] - prm->port_offsets[i][0];
+                        real dy = prm->port_offsets[j][1] - prm->port_offsets[i][1];
                         pp[npairs][0] = i;
                         pp[npairs][1] = j;
                         pd2r[npairs++] = ksigma / (dx * dx + dy * dy);
@@ -1643,7 +1612,7 @@ static void tex_task(void* vctx, int t) {
                     pw[ip][i] = 1;
                 }
     }
-    if (c->prm.dust_remove && (used == 4)) dust_remove(pw, ports);
+    if (prm->dust_remove && (used == 4)) dust_remove(pw, ports);
     for (int i = 0; i < 256; i++) {
         wreal a = 0;
         for (int ip = 0; ip < ports; ip++)
@@ -1683,12 +1652,50 @@ static void tex_task(void* vctx, int t) {
             out[chn * 256 + i] = s;
         }
         out[3 * 256 + i] = alpha[i];
-        if (c->planes > 4) {
+        if (planes > 4) {
             for (int ip = 0; ip < ports; ip++) out[(4 + ip) * 256 + i] = pw[ip][i];
             for (int k = 0; k < 4; k++) out[(4 + ports + k) * 256 + i] = rms[k][i];
         }
     }
     free(pw);
+}
+
+/* ID:2659-2812 for one task tile, then tile_combine_rgba (ID:2830-3000) */
+static void tex_task(void* vctx, int t) {
+    tex_ctx* c = (tex_ctx*)vctx;
+    const clt_geom* g = c->f.g;
+    const int ports = g->num_cams, numcol = 3;
+    double* out = c->tiles_out + (size_t)t * c->planes * 256;
+    double* stats = c->stats_out ? c->stats_out + (size_t)t * 4 * ports : NULL;
+    int32_t flags = task_flags(g, c->tasks, t);
+    if (flags == 0) {
+        for (int i = 0; i < c->planes * 256; i++) out[i] = NAN;
+        if (stats)
+            for (int i = 0; i < 4 * ports; i++) stats[i] = NAN;
+        return;
+    }
+    int port_mask = flags & 0xff & ((1 << ports) - 1);
+    int tx, ty;
+    task_txy(g, c->tasks, t, &tx, &ty);
+    size_t tile = (size_t)ty * c->f.tiles_x + tx;
+    double extra = c->extra_disparity ? c->extra_disparity[t] : 0.0;
+    if (isnan(extra)) extra = 0;
+    int forced = (flags >> 24) & 1;   /* C2:97 getForcedDisparity */
+    real (*iclt)[3][256] = malloc(sizeof(real) * ports * 3 * 256);
+    real (*deb)[3][256] = malloc(sizeof(real) * ports * 3 * 256);
+    for (int ip = 0; ip < ports; ip++) {
+        for (int chn = 0; chn < numcol; chn++) {
+            real clt[4][64];
+            load_clt(&c->f, c->clt, ip, tile, chn, clt);
+            if (extra != 0 && !forced)
+                fract_shift(clt, (real)(extra * c->prm.port_offsets[ip][0] / c->prm.corr_magic_scale),
+                            (real)(extra * c->prm.port_offsets[ip][1] / c->prm.corr_magic_scale));
+            iclt_tile(clt, (real)0.25, c->lpf, iclt[ip][chn]);
+            tile_debayer_shot_corr(chn != 2, iclt[ip][chn], c->window2, (real)c->prm.min_shot,
+                                   (real)c->prm.scale_shot, deb[ip][chn]);
+        }
+    }
+    combine_rgba(ports, port_mask, deb, iclt, c->window2, c->col_weights, &c->prm, c->planes, out, stats);
     free(iclt);
     free(deb);
 }
@@ -2097,6 +2104,27 @@ EXPORT void orc_tile_debayer(int rb, const double* tile, double* out) {
 }
 
 /* ---- frame level ---- */
+/* tile_combine_rgba + the final mix (combine_rgba) of one tile: deb, iclt [ports][3][256],
+ * lt_window [256], chw [3] colour weights; out [planes][256], stats [4 * ports] or NULL. */
+EXPORT void orc_tile_combine_rgba(int ports, int port_mask, const double* deb, const double* iclt, const double* lt_window,
+                                  const double* chw, const clt_tex_params* prm, int planes, double* out, double* stats) {
+    real (*d)[3][256] = malloc(sizeof(real) * ports * 3 * 256);
+    real (*ic)[3][256] = malloc(sizeof(real) * ports * 3 * 256);
+    real w[256], cw[3];
+    for (int i = 0; i < ports * 3 * 256; i++) {
+        ((real*)d)[i] = (real)deb[i];
+        ((real*)ic)[i] = (real)iclt[i];
+    }
+    for (int i = 0; i < 256; i++) w[i] = (real)lt_window[i];
+    for (int i = 0; i < 3; i++) cw[i] = (real)chw[i];
+    combine_rgba(ports, port_mask, d, ic, w, cw, prm, planes, out, stats);
+    free(d);
+    free(ic);
+}
+
+/* dust_remove (ID:3002-3019) in place on pw [np][256]. */
+EXPORT void orc_dust_remove(double* pw, int np) { dust_remove((wreal(*)[256])pw, np); }
+
 EXPORT int orc_convert_frame(const clt_geom* g, const float* const* images, int stride, const float* const* kernels,
                              const float* const* offsets, const int32_t* tasks, int ntasks, int lpf_mask,
                              const double* lpf3x64, int no_fract_shift, double* clt_out, double* residual_out,

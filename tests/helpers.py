@@ -57,30 +57,27 @@ def gpu_clt_cpu_layout(tp, sc):
     return np.stack([tp.getCLT(c) for c in range(sc.num_cams)])
 
 
-def robust_rel_rms(a, b, frac=1e-3, tol=1e-3):
-    """For outputs of discontinuous selections (argmax/argmin of port weights in the reference's
-    texture combiner): FP32 and FP64 may resolve near-ties differently on a few pixels.
-    Returns (rel RMS over all but the worst `frac` of the entries, fraction of entries whose error
-    exceeds `tol` times the RMS of b)."""
-    a = np.asarray(a, np.float64).ravel()
-    b = np.asarray(b, np.float64).ravel()
-    m = np.isfinite(b)
-    a, b = a[m], b[m]
-    err = np.abs(a - b)
-    scale = np.sqrt(np.mean(b ** 2)) or 1.0
-    keep = np.argsort(err)[: max(1, int(np.ceil(len(err) * (1 - frac))))]
-    r = float(np.sqrt(np.mean(err[keep] ** 2)) / scale)
-    return r, float(np.mean(err > tol * scale))
+# Bars of the texture planes (relative RMS against the FP64 oracle): RGB and the rms planes as the
+# reconstructed images; alpha and the port weights are exponentials of squared differences of
+# nearly equal port values chosen by comparison (two best ports), where a float32 restatement of
+# the reference is itself 4e-5 .. 7e-5 (alpha) and 3e-4 .. 9e-4 (weights) off.
+TEX_TOL = {"rgb": 1e-4, "alpha": 2e-4, "weights": 2e-3, "rms": 1e-4}
 
 
-def fp32_sensitive(run, clt64, moved, draws=3, seed=1):
+def tex_planes(num_cams):
+    return (("rgb", slice(0, 3)), ("alpha", slice(3, 4)), ("weights", slice(4, 4 + num_cams)),
+            ("rms", slice(4 + num_cams, 8 + num_cams)))
+
+
+def fp32_sensitive(run, clt64, moved, draws=3, seed=1, ulps=1.0):
     """Tasks whose FP64 outputs are not determined at float32 precision.
 
     run(clt) returns the FP64 oracle's outputs of a stage on the CLT clt [C][tiles][K][4][8][8];
     moved(out, base) returns a bool mask over the tasks of outputs moved past the bars.  The CLT is
-    perturbed `draws` times by Gaussian noise of half a float32 ulp of the largest coefficient of
-    each (camera, tile, colour) block (2^-24 of it: one rounding of a float32 result) and the
-    stage rerun.  Returns (mask of the tasks moved by any draw, the unperturbed outputs).
+    perturbed `draws` times by Gaussian noise of `ulps` float32 ulps of the largest coefficient of
+    each (camera, tile, colour) block (2^-23 of it per ulp: about the error of a float32 transform
+    chain, whose CLT is 2e-7 relative RMS off the FP64 one) and the stage rerun.  Returns (mask of
+    the tasks moved by any draw, the unperturbed outputs).
 
     Such tasks occur where a camera's window lies mostly outside the frame: the border clamp
     leaves many bins with rounding noise only, which the phase correlation normalises to unit
@@ -92,7 +89,7 @@ def fp32_sensitive(run, clt64, moved, draws=3, seed=1):
     base = run(clt64)
     C, T, K = clt64.shape[:3]
     blk = clt64.reshape(C, T, K, 256)
-    scale = np.abs(blk).max(-1, keepdims=True) * 2.0 ** -24
+    scale = np.abs(blk).max(-1, keepdims=True) * (ulps * 2.0 ** -23)
     bad = None
     for _ in range(draws):
         noisy = (blk + rng.standard_normal(blk.shape) * scale).reshape(clt64.shape)

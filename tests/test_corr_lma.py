@@ -4,14 +4,16 @@ over the combined pairs, the four one-direction fits and the foreground correcti
 Same input: the oracle's FP64 chain runs on the GPU's own pair correlations and pre-LMA records, so
 both sides take the same discrete decisions (sample selection, accepted steps) up to the order of
 the FP64 sums; the final disparity, strength and LMA disparity agree to 1e-4 px on every tile.
-End to end (oracle FP64 correlations from its own convert): the float32 correlations can flip a
-discrete decision (a convexity test, the rounding of the start point, an accepted step) on a few
-tiles; all but 1% of the tiles agree within 0.01 px.
+End to end (oracle FP64 correlations from its own convert): every tile within 0.01 px, except the
+tiles whose FP64 result is not determined at float32 precision (helpers.fp32_sensitive: perturbing
+the FP64 CLT by one float32 ulp moves the oracle's own disparity past 0.01 px, e.g. a convexity test,
+the rounding of the start point or an accepted step within rounding of its threshold); those are
+found from the oracle alone and counted.
 """
 import numpy as np
 import pytest
 
-from helpers import gpu_processor, log, oracle_geom
+from helpers import fp32_ill_conditioned, gpu_processor, log, oracle_geom
 
 pytestmark = pytest.mark.gpu
 
@@ -31,14 +33,15 @@ def _same_input(orc, g_corr, g_disp, lma=None):
     return res
 
 
-def _end_to_end(g_disp, e_disp):
-    res = {}
+def _end_to_end(g_disp, e_disp, ok):
+    """Tiles past 0.01 px (and NaN mismatches) among the well-conditioned tiles `ok`."""
+    res = {"ill_conditioned": int((~ok).sum())}
     for name, k in (("final", 1), ("poly", 8)):
-        fo, fg = np.isfinite(e_disp[:, k]), np.isfinite(g_disp[:, k])
+        fo, fg = np.isfinite(e_disp[ok, k]), np.isfinite(g_disp[ok, k])
         both = fo & fg
-        d = np.abs(g_disp[both, k] - e_disp[both, k])
+        d = np.abs(g_disp[ok][both, k] - e_disp[ok][both, k])
         res[name + "_nan"] = int((fo != fg).sum())
-        res[name + "_off"] = float((d > DISP_TOL).mean()) if both.any() else 0.0
+        res[name + "_off"] = int((d > DISP_TOL).sum())
         res[name + "_median"] = float(np.median(d)) if both.any() else 0.0
     return res
 
@@ -65,15 +68,16 @@ def test_corr_lma_quad(pk, orc, name, size, kw):
     g_corr, g_disp = _run(pk, orc, sc)
     r = _same_input(orc, g_corr, g_disp)
     log("  same input: " + ", ".join("%s=%.3g" % kv for kv in r.items()))
-    lp = orc.default_corr_lma_params()
-    _, _, e_disp = orc.corr_frame(oracle_geom(orc, sc), orc.convert_frame(oracle_geom(orc, sc), sc.images, sc.tasks,
-                                  kernels=sc.kernels, offsets=sc.offsets), sc.tasks, lma=lp)
-    e = _end_to_end(g_disp, e_disp)
+    g = oracle_geom(orc, sc)
+    o_clt = orc.convert_frame(g, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
+    ill, (_, _, e_disp) = fp32_ill_conditioned(orc, g, o_clt, sc.tasks)
+    e = _end_to_end(g_disp, e_disp, ~ill)
     log("  end to end: " + ", ".join("%s=%.3g" % kv for kv in e.items()))
     assert r["fitted"] > 0.5 * len(sc.tasks)
     for nm, _ in LAYERS:
         assert r[nm + "_nan"] == 0 and r[nm] < 1e-4, nm
-    assert e["final_off"] <= 0.01 and e["poly_off"] <= 0.01
+    assert e["final_off"] == 0 and e["poly_off"] == 0 and e["final_nan"] == 0 and e["poly_nan"] == 0
+    assert e["ill_conditioned"] <= 0.01 * len(sc.tasks)
 
 
 def test_corr_lma_disabled_and_parameters(pk, orc):

@@ -16,13 +16,13 @@ Oracle pins (CPU):
 * the low-texture pass equals the analysis of the cos-weighted neighbour averages;
 * the notch form ignores the strip's axis row; skipped tasks give zeros and a NaN target.
 GPU parity: the device rig against the oracle's analysis of the device's own correlations (1e-4,
-same discrete decisions) and against the oracle end to end (at most 1 % of the tiles off by more
-than 0.01 px).
+same discrete decisions) and against the oracle end to end (no tile off by more than 0.01 px among
+the tiles whose FP64 result is determined at float32 precision, helpers.fp32_sensitive).
 """
 import numpy as np
 import pytest
 
-from helpers import gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, robust_rel_rms
+from helpers import TEX_TOL, fp32_sensitive, gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, tex_planes
 
 RIG_OFF = np.array([2.0, 0.0])   # aux quad centre, in main-quad disparity units
 AUX_SCALE = 0.75                 # aux baseline / main baseline
@@ -223,17 +223,34 @@ def test_gpu_rig_parity(pk, orc, name, rig, fz):
         log("  same input %-10s nan mismatch %d, max|d| %.3g" % (nm, int((fo != fg).sum()), d))
         assert (fo != fg).sum() == 0 and d < 1e-4, nm
     clt = orc.convert_frame(g8, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets)
-    e = orc.rig_frame(g8, clt, sc.tasks, params=prm, rig=rp)["bimap"]
+    e_layers = [LAYERS[nm] for nm in ("disp_full", "adisp_full", "cross", "cross_dx")]
+
+    def moved(out, base):   # an end-to-end layer moved past 0.01 px or between NaN and finite
+        bad = np.zeros(len(base), bool)
+        for k in e_layers:
+            fa, fb = np.isfinite(out[:, k]), np.isfinite(base[:, k])
+            bad |= fa != fb
+            both = fa & fb
+            bad[both] |= np.abs(out[both, k] - base[both, k]) > 0.01
+        return bad
+
+    ill, e = fp32_sensitive(lambda c: orc.rig_frame(g8, c, sc.tasks, params=prm, rig=rp)["bimap"], clt, moved)
+    ok = ~ill
+    log("  end to end: %d of %d tasks not determined at float32 precision" % (ill.sum(), len(ill)))
+    assert ill.sum() <= 0.02 * len(ill)
     for nm in ("disp_full", "adisp_full", "cross", "cross_dx"):
         k = LAYERS[nm]
         if nm == "cross_dx" and rig.get("notch_mode"):   # the notch form runs no LMA: no dx, dy
             assert np.isnan(e[:, k]).all() and np.isnan(gb[:, k]).all()
             continue
         both = np.isfinite(e[:, k]) & np.isfinite(gb[:, k])
-        off = float((np.abs(gb[both, k] - e[both, k]) > 0.01).mean())
+        nan_mis = int((np.isfinite(e[ok, k]) != np.isfinite(gb[ok, k])).sum())
+        sel = both & ok
+        off = int((np.abs(gb[sel, k] - e[sel, k]) > 0.01).sum())
         med = float(np.median(np.abs(gb[both, k] - e[both, k]))) if both.any() else 0.0
-        log("  end to end %-10s %d both, %.4f off by > 0.01 px, median %.2g" % (nm, both.sum(), off, med))
-        assert both.sum() > 0.5 * len(sc.tasks) and off <= 0.01, nm
+        log("  end to end %-10s %d both, %d off by > 0.01 px, %d NaN mismatches, median %.2g" % (
+            nm, both.sum(), off, nan_mis, med))
+        assert both.sum() > 0.5 * len(sc.tasks) and off == 0 and nan_mis == 0, nm
 
 
 @pytest.mark.gpu
@@ -267,12 +284,12 @@ def test_gpu_rig_textures(pk, orc):
         tasks4 = np.ascontiguousarray(np.concatenate([sc.tasks[:, :2], sc.tasks[:, 2 + 8 * q:10 + 8 * q]], axis=1))
         o_tiles, o_stats = orc.textures_frame(g4, clt[4 * q:4 * q + 4], tasks4, extra_disparity=extra)
         res = {}
-        for name, sl in (("rgb", slice(0, 3)), ("alpha", slice(3, 4)), ("weights", slice(4, 8)), ("rms", slice(8, None))):
-            res[name], res[name + "_flips"] = robust_rel_rms(tiles[:, sl], o_tiles[:, sl])
+        for name, sl in tex_planes(4):
+            res[name] = rel_rms(tiles[:, sl], o_tiles[:, sl])
         res["stats"] = rel_rms(stats, o_stats)
         log("  quad %d: " % q + ", ".join("%s=%.3g" % kv for kv in res.items()))
-        for name in ("rgb", "alpha", "weights", "rms"):
-            assert res[name] < 1e-4 and res[name + "_flips"] <= 1e-3, name
+        for name, tol in TEX_TOL.items():
+            assert res[name] < tol, name
         assert res["stats"] < 1e-4
 
 

@@ -9,7 +9,7 @@ tasks on one tile)."""
 import numpy as np
 import pytest
 
-from helpers import gpu_processor, log
+from helpers import TEX_TOL, gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, tex_planes, valid_map
 
 pytestmark = pytest.mark.gpu
 
@@ -122,3 +122,30 @@ def test_async_side_stream_ordering(pk):
     same_clt = np.array_equal(c0.view(np.int32), c1.view(np.int32))
     log("  bit-identical: images=%s, second CLT=%s" % (same_img, same_clt))
     assert same_img and same_clt
+
+
+@pytest.mark.parametrize("frac", [1.0, 0.6])
+def test_one_pass_against_oracle(pk, orc, frac):
+    """The one-pass k_tex_imclt outputs against the oracle directly: textures (ID:2662-2812) and the
+    LPF'd inverse CLT with overlap-add (ID:3559-3666) of the device CLT (the same input)."""
+    log("case: one-pass textures + imclt_rbg vs the FP64 oracle, 136x152, %d %% of the tiles" % (100 * frac))
+    sc = pk.synth.make_scene(136, 152, seed=26, disparity=("field", 0.0, 5.0), task_fraction=frac)
+    tp = gpu_processor(pk, sc)
+    tp.setCorrSigma(0.8)
+    tp.execConverCorrectTiles()
+    tp.setTiming(True)
+    tp.execTexturesImclt(True)
+    assert tp.getTiming("tex_imclt")[1] == 1   # the one-pass kernel ran (not the two-pass fallback)
+    tiles, stats, img = _outputs(tp, sc)
+    clt = gpu_clt_cpu_layout(tp, sc).astype(np.float64)
+    tp.close()
+    g = oracle_geom(orc, sc)
+    o_tiles, o_stats = orc.textures_frame(g, clt, sc.tasks)
+    o_img = orc.imclt_frame(g, clt, valid_map(sc), lpf64=orc.lpf(0.8))
+    err = {name: rel_rms(tiles[:, sl], o_tiles[:, sl]) for name, sl in tex_planes(4)}
+    err["stats"] = rel_rms(stats, o_stats)
+    err["image"] = rel_rms(img.reshape(o_img.shape), o_img)
+    log("  " + ", ".join("%s=%.3g" % kv for kv in err.items()))
+    for name, tol in TEX_TOL.items():
+        assert err[name] < tol, name
+    assert err["stats"] < 1e-4 and err["image"] < 1e-4
