@@ -122,3 +122,16 @@ def fp32_ill_conditioned(orc, g, clt64, tasks, draws=3, seed=1, pairs=None, nthr
     """fp32_sensitive for the oracle's correlate2D with the LMA chain: (mask, (corr, combo, disp))."""
     return fp32_sensitive(lambda c: orc.corr_frame(g, c, tasks, pairs=pairs, nthreads=nthreads, lma=lma), clt64,
                           corr_moved(), draws=draws, seed=seed)
+
+
+def confirm_ill_conditioned(orc, g, clt64, tasks, idx, draws=24, seed=7, pairs=None, nthreads=None, lma=None):
+    """The subset of the tasks idx that fp32_ill_conditioned marks with `draws` perturbations of
+    those tasks alone (more draws than a whole-frame pass, where a few random draws can miss a
+    decision that sits within rounding of its threshold)."""
+    import numpy as np
+    idx = np.asarray(idx, np.int64)
+    if idx.size == 0:
+        return idx
+    sub = np.ascontiguousarray(tasks[idx])
+    ill, _ = fp32_ill_conditioned(orc, g, clt64, sub, draws=draws, seed=seed, pairs=pairs, nthreads=nthreads, lma=lma)
+    return idx[ill]

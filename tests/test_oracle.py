@@ -36,7 +36,22 @@ Each oracle function is pinned by what the reference itself holds or what mathem
   textures (ID:2830-3147)             identical ports: RGB = input, weights 1, alpha = window^2;
                                       single port copies; debayer keeps constants; overlap-add of
                                       window^2 alpha is 1 in the interior
-Parity of the textures' exp-weight details beyond these invariants is unpinned (no reference run).
+  tile_combine_rgba (ID:2830-3000)    hand-worked pixels: a quad with one outlier port (pair
+                                      weights, best pair, second pass, dust_remove, alpha, the final
+                                      mix of the non-debayered tiles, rms, max_diff, mean tones) at
+                                      window 1 and 0.5; three ports; a quad whose best pair is mixed;
+                                      an exact tie of the second-best ports (the first wins)
+  dust_remove (ID:3002-3019)          hand values that show its loops start at port 1
+  tile_debayer (ID:3048-3147)         unit deltas inside, on an edge and in two corners: the kernels
+                                      of ID:43-50 with the dropped taps and the 16/9, 4/3 (red, blue)
+                                      and 4/3, 8/7 (green) corner/side factors
+  fat zero (C2:197-231)               equal data in both cameras: a2 / (a2 + mean(a2) fz^2) per bin,
+                                      through the DCT-IIe and unfold pinned above
+  getMaxXCm (C2:936-1032)             an asymmetric peak with a negative sample and a far sample,
+                                      windows and offsets worked by hand, at two integer centres
+The planted-mistake check of these pins (final mix of the debayered tiles, ksigma not halved,
+dust_remove from port 0, the tie taken by the last port, fat zero not squared, a wrong side
+factor) is tests/test_oracle_mutants.py.
 """
 import numpy as np
 import pytest
@@ -554,3 +569,222 @@ def test_foreground_correct_branches(orc):
     # far objects (fo_far): the minimum
     lp = orc.default_corr_lma_params(fo_far=1)
     assert orc.foreground_correct([strong(1.0), strong(1.5), strong(1.2), strong(1.1)], full, lma=lp) == 1.0
+
+
+# ---------------------------------------------------------------- hand-worked texture combiner
+G_W = np.array([0.5, 0.2, 1.0]) / 1.7     # R, B, G colour weights (corr_red 0.5, corr_blue 0.2, ID:1571-1581)
+KS = 1.0 / (2 * 5.0 ** 2)                 # ksigma of diff_sigma 5 (ID:2880)
+BASE = np.array([100.0, 50.0, 200.0])     # R, B, G of the agreeing ports
+
+
+def _tiles(vals, window):
+    """deb [ports][3][256] with port ip's colour values vals[ip] at every pixel; iclt = deb plus
+    (ip + 1) * (1, 2, 3), so any use of one for the other changes the result."""
+    vals = np.asarray(vals, np.float64)
+    deb = np.repeat(vals[:, :, None], 256, axis=2)
+    iclt = deb + (np.arange(len(vals))[:, None, None] + 1) * np.array([1.0, 2.0, 3.0])[None, :, None]
+    return deb, iclt, np.broadcast_to(np.asarray(window, np.float64), (256,)).copy()
+
+
+@pytest.mark.parametrize("w", [1.0, 0.5])
+def test_combine_rgba_quad_one_outlier(orc, w):
+    """Quad, ports 0-2 agree, port 3 is off by D in green (ID:2880-2935, 3002-3019, 2789-2801).
+    Pair weights: the agreeing pairs 1; with X = g (D / w)^2 the pairs with port 3 are
+    exp(-KS X / 2) (diagonal 0-3, distance^2 2) and exp(-KS X) (1-3, 2-3, distance^2 1).  The best
+    port is 0 (2 + exp(-KS X / 2)); the second the first of the tied ports 1, 2.  Their mean is the
+    common value, so the second pass gives 1, 1, 1, e = exp(-KS X).  dust_remove (loop from 1): the
+    worst is port 3, avg = (1 + 1) / 3, ports 1 and 2 scaled by 1 + (2/3 - e) / 2, port 0 kept,
+    port 3 -> e^2 / (2/3)."""
+    D = 10.0
+    vals = np.array([BASE, BASE, BASE, BASE + [0, 0, D]])
+    deb, iclt, win = _tiles(vals, w)
+    out, stats = orc.tile_combine_rgba(deb, iclt, win)
+    e = np.exp(-KS * G_W[2] * (D / w) ** 2)
+    sc = 1 + (2.0 / 3 - e) / 2
+    pw = np.array([1.0, sc, sc, 1.5 * e * e])
+    rgb = (pw[:, None] * iclt[:, :, 0]).sum(0) / pw.sum()
+    assert np.allclose(out[0:3, 0], rgb, rtol=1e-13)
+    assert np.allclose(out[3], pw.sum() * w / 4, rtol=1e-13)                     # alpha
+    assert np.allclose(out[4:8], pw[:, None], rtol=1e-13)                        # port weights
+    std_g = D * np.sqrt(3) / 4                                                   # population std of (0, 0, 0, D)
+    assert np.allclose(out[8:10], 0, atol=1e-12) and np.allclose(out[10], std_g, rtol=1e-12)
+    assert np.allclose(out[11], np.sqrt(G_W[2]) * std_g, rtol=1e-12)
+    # stats: max_diff against the second-pass mean (green mean moved by e D / (3 + e)), then mean tones
+    m = e * D / (3 + e)
+    md = np.sqrt(G_W[2] * np.array([m, m, m, D - m]) ** 2 * w)
+    assert np.allclose(stats[:4], md, rtol=1e-12)
+    assert np.allclose(stats[4:].reshape(3, 4), vals.T, rtol=1e-14)
+
+
+def test_combine_rgba_three_ports(orc):
+    """Three ports on a circle of radius 0.5 (all pair distances^2 3/4), port 2 off by D in red:
+    pair weights 1 (0-1) and a = exp(-KS X / 0.75) (0-2, 1-2) with X = g_R D^2; best ports 0 and 1
+    (1 + a against 2 a), mean = the common value; second pass 1, 1, e = exp(-KS X); no dust_remove
+    (three ports); alpha (2 + e) / 3; the output mixes the iclt tiles with weights 1, 1, e."""
+    D = 12.0
+    vals = np.array([BASE, BASE, BASE + [D, 0, 0]])
+    deb, iclt, win = _tiles(vals, 1.0)
+    out, _ = orc.tile_combine_rgba(deb, iclt, win, tex_params=orc.default_tex_params(3))
+    e = np.exp(-KS * G_W[0] * D * D)
+    pw = np.array([1.0, 1.0, e])
+    assert np.allclose(out[0:3, 0], (pw[:, None] * iclt[:, :, 0]).sum(0) / pw.sum(), rtol=1e-13)
+    assert np.allclose(out[3], (2 + e) / 3, rtol=1e-13)
+    assert np.allclose(out[4:7], pw[:, None], rtol=1e-13)
+    assert np.allclose(out[7], D * np.sqrt(2) / 3, rtol=1e-12)   # std of (0, 0, D)
+
+
+def test_combine_rgba_best_pair_is_mixed(orc):
+    """Quad, green values g0 = G, g1 = G + 1, g2 = g3 = G + 15 (w = 1): pair weights
+    E01 = exp(-KS g 1), E23 = 1, E02 = exp(-KS g 225), E13 = exp(-KS g 196) (distance^2 1),
+    E03 = exp(-KS g 225 / 2), E12 = exp(-KS g 196 / 2) (diagonals).  Port sums
+    1.3254, 1.4037, 1.3866, 1.3659: the two best ports are 1 and 2, not an agreeing pair; their
+    weighted mean, the second-pass weights exp(-KS g (g_i - mean)^2) and the re-weighted output
+    follow.  dust_remove off."""
+    gval = np.array([0.0, 1.0, 15.0, 15.0])
+    vals = np.array([BASE + [0, 0, d] for d in gval])
+    deb, iclt, win = _tiles(vals, 1.0)
+    out, _ = orc.tile_combine_rgba(deb, iclt, win, tex_params=orc.default_tex_params(4, dust_remove=0))
+    g = G_W[2]
+    E = lambda d, r2: np.exp(-KS / r2 * g * d * d)
+    p = np.array([E(1, 1) + E(15, 1) + E(15, 2), E(1, 1) + E(14, 2) + E(14, 1),
+                  E(15, 1) + E(14, 2) + 1.0, E(15, 2) + E(14, 1) + 1.0])
+    assert np.allclose(p, [1.3254, 1.4037, 1.3866, 1.3659], atol=1e-4)
+    w1 = p[1] / (p[1] + p[2])
+    mean_g = w1 * gval[1] + (1 - w1) * gval[2]
+    pw = np.exp(-KS * g * (gval - mean_g) ** 2)
+    assert np.allclose(out[4:8, 0], pw, rtol=1e-12)
+    assert np.allclose(out[0:3, 0], (pw[:, None] * iclt[:, :, 0]).sum(0) / pw.sum(), rtol=1e-12)
+    assert np.allclose(out[3, 0], pw.sum() / 4, rtol=1e-12)
+
+
+def test_combine_rgba_tie_takes_first(orc):
+    """Quad, green g1 = G + 2, g2 = G - 2, port 3 off by 20 in red: the pairs of port 0 with 1 and 2
+    have equal weights E01 = E02 = exp(-KS g 4), and so do 1-3 and 2-3, so ports 1 and 2 tie
+    exactly (E01 + E12 + E13 = E02 + E12 + E23).  Port 0 is best (2 E01 + E03); the second best is
+    the first of the tie, port 1 (the scan keeps a port only if strictly greater): the mean moves
+    towards G + 2, not G - 2, which the second-pass weights of ports 1 and 2 show."""
+    vals = np.array([BASE, BASE + [0, 0, 2.0], BASE - [0, 0, 2.0], BASE + [20.0, 0, 0]])
+    deb, iclt, win = _tiles(vals, 1.0)
+    out, _ = orc.tile_combine_rgba(deb, iclt, win, tex_params=orc.default_tex_params(4, dust_remove=0))
+    g, r = G_W[2], G_W[0]
+    e01 = np.exp(-KS * g * 4)
+    e12 = np.exp(-KS / 2 * g * 16)
+    e03 = np.exp(-KS / 2 * r * 400)
+    e13 = np.exp(-KS * (r * 400 + g * 4))
+    p0, p1 = 2 * e01 + e03, e01 + e12 + e13
+    assert p0 > p1 > 2 * e13 + e03
+    w1 = p0 / (p0 + p1)
+    mean = BASE + [0, 0, (1 - w1) * 2.0]                  # towards port 1
+    pw = np.exp(-KS * ((vals - mean) ** 2 * G_W).sum(1))
+    assert pw[1] > pw[2]
+    assert np.allclose(out[4:8, 0], pw, rtol=1e-12)
+    assert np.allclose(out[0:3, 0], (pw[:, None] * iclt[:, :, 0]).sum(0) / pw.sum(), rtol=1e-12)
+
+
+def test_dust_remove_loop_from_one(orc):
+    """dust_remove (ID:3002-3019): the worst port gets w^2 / avg and the others (from port 1 on:
+    the loops start at 1) are scaled by 1 + (avg - w) / (avg (n - 1)), avg being the sum of the
+    ports 1.. other than the worst divided by n - 1 (port 0 never enters it)."""
+    w = orc.dust_remove(np.repeat(np.array([[0.9], [0.2], [0.5], [0.7]]), 256, axis=1))
+    # worst = 1, avg = (0.5 + 0.7) / 3 = 0.4, scale = 1 + 0.2 / 1.2; port 0 untouched
+    assert np.allclose(w[:, 0], [0.9, 0.04 / 0.4, 0.5 * 7 / 6, 0.7 * 7 / 6], rtol=1e-14)
+    w = orc.dust_remove(np.repeat(np.array([[0.1], [0.5], [0.9], [0.7]]), 256, axis=1))
+    # worst = 0, avg = (0.5 + 0.9 + 0.7) / 3 = 0.7, scale = 1 + 0.6 / 2.1
+    s = 1 + 0.6 / 2.1
+    assert np.allclose(w[:, 0], [0.01 / 0.7, 0.5 * s, 0.9 * s, 0.7 * s], rtol=1e-14)
+
+
+# ---------------------------------------------------------------- hand-worked debayer
+@pytest.mark.parametrize("rb", [True, False])
+def test_debayer_delta_responses(orc, rb):
+    """tile_debayer (ID:3048-3147) of a unit delta: the 3x3 kernel (ID:43-50) around it, with the
+    taps that fall outside the tile dropped and the corner / side outputs scaled by 16/9 and 4/3
+    (red, blue) or 4/3 and 8/7 (green)."""
+    k = np.array([[0.0625, 0.125, 0.0625], [0.125, 0.25, 0.125], [0.0625, 0.125, 0.0625]]) if rb else \
+        np.array([[0.0, 0.125, 0.0], [0.125, 0.5, 0.125], [0.0, 0.125, 0.0]])
+    k_corn, k_side = (16 / 9, 4 / 3) if rb else (4 / 3, 8 / 7)
+    # interior delta: the kernel itself
+    t = np.zeros((16, 16))
+    t[8, 8] = 1
+    o = orc.tile_debayer(rb, t)
+    exp = np.zeros((16, 16))
+    exp[7:10, 7:10] = k
+    assert np.allclose(o, exp, atol=1e-15)
+    # corner delta (0, 0): corner output k[1][1] * k_corn, side outputs k[1][0] * k_side, inner k[0][0]
+    t = np.zeros((16, 16))
+    t[0, 0] = 1
+    o = orc.tile_debayer(rb, t)
+    exp = np.zeros((16, 16))
+    exp[0, 0] = k[1, 1] * k_corn
+    exp[0, 1] = exp[1, 0] = k[1, 0] * k_side
+    exp[1, 1] = k[0, 0]
+    assert np.allclose(o, exp, atol=1e-15)
+    # bottom-edge delta (15, 8): three side outputs on the edge row, three inner ones above
+    t = np.zeros((16, 16))
+    t[15, 8] = 1
+    o = orc.tile_debayer(rb, t)
+    exp = np.zeros((16, 16))
+    exp[15, 7:10] = k[1] * k_side
+    exp[14, 7:10] = k[0]
+    assert np.allclose(o, exp, atol=1e-15)
+    # bottom-right corner
+    t = np.zeros((16, 16))
+    t[15, 15] = 1
+    o = orc.tile_debayer(rb, t)
+    exp = np.zeros((16, 16))
+    exp[15, 15] = k[1, 1] * k_corn
+    exp[15, 14] = exp[14, 15] = k[1, 0] * k_side
+    exp[14, 14] = k[0, 0]
+    assert np.allclose(o, exp, atol=1e-15)
+
+
+# ---------------------------------------------------------------- fat zero, closed form
+@pytest.mark.parametrize("fz", [0.0, 0.05, 0.3])
+def test_fat_zero_normalisation_closed_form(orc, fz):
+    """correlateSingleColorFD (C2:197-231) with the same data in both cameras and only the CC
+    quadrant set: the product is CC only, sum_k d_k^2 = a2 in every bin, so the frequency-domain
+    correlation is a2 / (a2 + mean(a2) fz^2).  Bins with (h + v) even get a2 = A, odd ones B, so the
+    mean is (A + B) / 2 and the array is symmetric.  The pair correlation is then the DCT-IIe of it
+    times the LPF, unfolded (both pinned above against scipy and the reference's tables), the
+    colour weights summing to 1."""
+    A, B = 4.0e6, 9.0e4
+    hv = np.add.outer(np.arange(8), np.arange(8))
+    a2 = np.where(hv % 2 == 0, A, B)
+    clt = np.zeros((2, 4, 3, 4, 8, 8))
+    clt[:, 0, :, 0] = np.sqrt(a2)
+    g = orc.make_geom(16, 16, num_cams=2, num_colors=3, kernels=False)
+    tasks = np.array([[1, 0, 0, 0, 0, 0]], np.int32)
+    prm = orc.default_corr_params(fat_zero=fz)
+    corr, _, _ = orc.corr_frame(g, clt, tasks, pairs=[(0, 1, 0, 1)], params=prm,
+                                lma=orc.default_corr_lma_params(enable=0))
+    F = a2 / (a2 + (A + B) / 2 * fz * fz)
+    q = np.zeros((4, 64))
+    q[0] = orc.dttt("iie", (F * orc.lpf(0.8).reshape(8, 8)).reshape(64), 0)
+    exp = orc.corr_unfold(q)
+    assert np.allclose(corr[0, 0], exp, atol=1e-13)
+
+
+# ---------------------------------------------------------------- centre of mass, asymmetric peak
+@pytest.mark.parametrize("ixc", [0, -3])
+def test_max_x_cm_asymmetric(orc, ixc):
+    """getMaxXCm (C2:936-1032) with the default windows: rows 0 and 1 only (wy = 0.5, 2 * 0.25);
+    wx = halfFlatTopWindow(9, 6, 5) normalised and x2 = d / 6 with d = 1, 1, 1, 1,
+    (1 + sin 2pi/5) / 2, (1 + sin pi/5) / 2, 1/2, (1 - sin pi/5) / 2, (1 - sin 2pi/5) / 2.  Row 0
+    samples sit at x = 2 (col - x0), row 1 samples at x = 2 (col - x0) + 1; negative values are
+    skipped; result x = ixcenter + sx / s0 / 2, s0, sqrt(s0 sx2 - sx^2) / s0 / 2."""
+    strip = np.zeros((9, 15))
+    strip[2:] = 5.0                       # rows past the window are not read
+    x0 = 7 + ixc
+    for dc, v in ((0, 1.0), (-1, 0.2), (1, 0.5), (2, -0.3), (4, 0.6)):
+        strip[0, x0 + dc] = v
+    for dc, v in ((-1, 0.4), (0, 0.8)):
+        strip[1, x0 + dc] = v
+    r = orc.max_x_cm(strip, ixc)
+    s1, s2 = np.sin(np.pi / 5), np.sin(2 * np.pi / 5)
+    d = np.array([1, 1, 1, 1, (1 + s2) / 2, (1 + s1) / 2, 0.5, (1 - s1) / 2, (1 - s2) / 2]) / 6
+    # (x, value, adx): row 0 (x = 2 dc, adx = 2|dc|), row 1 (x = 2 dc + 1, adx = |x|)
+    samp = [(0, 1.0, 0), (-2, 0.2, 2), (2, 0.5, 2), (8, 0.6, 8), (-1, 0.4, 1), (1, 0.8, 1)]
+    wts = np.array([v * 0.5 * d[a] for _, v, a in samp])
+    xs = np.array([x for x, _, _ in samp], np.float64)
+    s0, sx, sx2 = wts.sum(), (wts * xs).sum(), (wts * xs * xs).sum()
+    assert np.allclose(r, [ixc + sx / s0 / 2, s0, np.sqrt(s0 * sx2 - sx * sx) / s0 / 2], rtol=1e-13)
