@@ -172,6 +172,7 @@ SIGNATURES = {
     "clt_exec_corr_adjoint": (_I, [_H]),
     "clt_get_corr_adjoint": (_I, [_H, _P]),
     "clt_exec_geometry_adjoint": (_I, [_H, _P, _P, _P]),
+    "clt_exec_geometry_adjoint_noalloc": (_I, [_H, _P, _P, _P]),
     "clt_default_lma_params": (None, [ctypes.POINTER(clt_lma_params)]),
     "clt_lma_extrinsics": (_I, [_H, ctypes.POINTER(clt_lma_params), _P, _P, _P, ctypes.POINTER(_I)]),
 }

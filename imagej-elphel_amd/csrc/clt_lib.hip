@@ -9,6 +9,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,47 @@ namespace clt {
 __constant__ Tables c_tab;
 }
 using namespace clt;
+
+// ------------------------------------------------------------------------ static scratch
+// The buffers of clt_exec_geometry_adjoint_noalloc, in one file-scope device array (each GPU holds
+// its own copy until the process ends): per-task loss adjoint records, per-task disparity state,
+// per-block partial normal systems, their column sums and the per-task disparity gradient.  Host
+// code never uses the array's name as a device address: scratch_ptr() is its address on a device,
+// taken once per device by hipGetSymbolAddress when the first handle is created there.
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr int NA_BLOCKS = (CLT_NOALLOC_MAX_TASKS + GEO_ADJ_TASKS - 1) / GEO_ADJ_TASKS;
+constexpr size_t NA_OFF_ADJ = 0;
+constexpr size_t NA_OFF_TSTATE = NA_OFF_ADJ + align256((size_t)CLT_NOALLOC_MAX_TASKS * CORR_ADJ_STRIDE * sizeof(float));
+constexpr size_t NA_OFF_PART = NA_OFF_TSTATE + align256((size_t)CLT_NOALLOC_MAX_TASKS * 15 * sizeof(double));
+constexpr size_t NA_OFF_SUM = NA_OFF_PART + align256((size_t)NA_BLOCKS * GEO_ADJ_SUMS * sizeof(double));
+constexpr size_t NA_OFF_GD = NA_OFF_SUM + align256((size_t)GEO_ADJ_SUMS * sizeof(double));
+constexpr size_t NA_BYTES = NA_OFF_GD + align256((size_t)CLT_NOALLOC_MAX_TASKS * sizeof(float));
+static_assert(NA_BYTES <= CLT_NOALLOC_SCRATCH_BYTES, "scratch layout exceeds its declared size");
+static __device__ __align__(256) unsigned char scratch[CLT_NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// Address of the scratch array on the current device (which must be `device`), taken once.
+static hipError_t scratch_init(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+// Disparity gradient of each task (column 0 of the per-task state) as float.
+__global__ __launch_bounds__(256) void k_take_col(const double* __restrict__ a, int stride, int n, float* __restrict__ out) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < n) out[t] = (float)a[(size_t)t * stride];
+}
 
 // ------------------------------------------------------------------------ host math helpers
 // Closed-form tables (DttRad2.java:671-694, 952-1012; the DCT-IV twiddles of its FFT form)
@@ -190,6 +232,7 @@ struct clt_handle {
     int graph_epoch_cap = -1;
     CorrArgs* d_corr_args = nullptr;
     TexArgs* d_tex_args = nullptr;
+    void* scratch = nullptr;       // the static scratch array on this handle's device
     // two-quad rig (clt_exec_corr2d_rig): cameras 0..3 the main quad, 4..7 the auxiliary one
     clt_rig_params rp{};
     float* d_rig_corr = nullptr;   // [2][max_tasks][6][225] pair correlations of the main / aux quad
@@ -504,6 +547,10 @@ static int create_handle(const clt_geom* geom, int frames, clt_handle** out) {
         h->pairs.assign(p, p + 6);
     }
     int rc;
+    if (scratch_init(h->device, &h->scratch) != hipSuccess) {
+        delete h;
+        return CLT_E_HIP;
+    }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         delete h;
         return CLT_E_HIP;
@@ -1233,17 +1280,12 @@ static int build_lma_args(clt_handle* h, bool have_records) {
     return CLT_OK;
 }
 
-// Correlation arguments and output buffers for the current pairs/parameters (CorrArgs upload).
-static int prepare_corr(clt_handle* h) {
+// CorrArgs of the current pairs and parameters (host only, no allocation); *quad: the reference
+// quad configuration, which runs the specialised kernel (pairs as compile-time constants).
+static int fill_corr_args(clt_handle* h, CorrArgs& a, bool* quad) {
     if (h->pairs.empty()) return fail(h, CLT_E_STATE, "clt_exec_corr2d: no correlation pairs set");
     const clt_geom& g = h->g;
     const int P = (int)h->pairs.size(), hw = h->cp.strip_height;
-    int rc;
-    if ((rc = ensure_capacity(h, &h->d_corr, &h->corr_cap, (size_t)h->cap_tasks * P * CLT_CORR_LEN)) ||
-        (rc = ensure_capacity(h, &h->d_combo, &h->combo_cap, (size_t)h->cap_tasks * 16 * 15)) ||
-        (rc = dalloc(h, (void**)&h->d_disp, (size_t)h->cap_tasks * CLT_DISP_LAYERS * 4)))
-        return rc;
-    CorrArgs a;
     std::memset(&a, 0, sizeof(a));
     a.num_pairs = P;
     int ss_min = 1 << 30;
@@ -1289,21 +1331,35 @@ static int prepare_corr(clt_handle* h) {
         if (i == 7) f = 0.0;
         a.notch[i] = (float)(f * std::sin(M_PI * (i + 1.0) / 16));
     }
-    HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
-    if ((rc = build_lma_args(h, a.num_strip > 0))) return rc;
-    // the reference quad configuration runs the specialised kernel (pairs as compile-time constants)
-    bool quad = g.num_cams == 4 && g.num_colors == 3 && P == 6;
-    if (quad) {
+    bool q = g.num_cams == 4 && g.num_colors == 3 && P == 6;
+    if (q) {
         clt_pair dq[6];
         clt_default_quad_pairs(dq);
         for (int i = 0; i < 6; i++)
-            quad = quad && h->pairs[i].cam1 == dq[i].cam1 && h->pairs[i].cam2 == dq[i].cam2 &&
-                   h->pairs[i].dir == dq[i].dir && h->pairs[i].scale == 1;
+            q = q && h->pairs[i].cam1 == dq[i].cam1 && h->pairs[i].cam2 == dq[i].cam2 && h->pairs[i].dir == dq[i].dir &&
+                h->pairs[i].scale == 1;
     }
+    *quad = q;
+    return CLT_OK;
+}
+
+// Correlation arguments and output buffers for the current pairs/parameters (CorrArgs upload).
+static int prepare_corr(clt_handle* h) {
+    CorrArgs a;
+    bool quad = false;
+    int rc;
+    if ((rc = fill_corr_args(h, a, &quad))) return rc;
+    const int P = (int)h->pairs.size();
+    if ((rc = ensure_capacity(h, &h->d_corr, &h->corr_cap, (size_t)h->cap_tasks * P * CLT_CORR_LEN)) ||
+        (rc = ensure_capacity(h, &h->d_combo, &h->combo_cap, (size_t)h->cap_tasks * 16 * 15)) ||
+        (rc = dalloc(h, (void**)&h->d_disp, (size_t)h->cap_tasks * CLT_DISP_LAYERS * 4)))
+        return rc;
+    HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    if ((rc = build_lma_args(h, a.num_strip > 0))) return rc;
     h->corr_quad = quad;
     h->corr_num_strip = a.num_strip;
     h->corr_pairs_at_exec = P;
-    h->corr_strip_h_at_exec = a.num_strip ? hw : 0;
+    h->corr_strip_h_at_exec = a.num_strip ? h->cp.strip_height : 0;
     return CLT_OK;
 }
 
@@ -1769,10 +1825,10 @@ static int check_adjoint_config(clt_handle* h, const char* who) {
     return CLT_OK;
 }
 
-static void launch_corr_adjoint(clt_handle* h) {
+static void launch_corr_adjoint(clt_handle* h, float* out) {
     hipLaunchKernelGGL(k_corr_adjoint_quad, dim3(h->num_tasks), dim3(192), 0, h->stream, h->d_images,
                        h->use_kernels ? h->d_kernels : nullptr, h->use_kernels ? h->d_koffs : nullptr, h->d_tasks,
-                       h->task_words, h->num_tasks, h->d_lpf_corr, h->d_corr_args, 1.0 / h->g.kernel_step, h->d_adj, h->gd);
+                       h->task_words, h->num_tasks, h->d_lpf_corr, h->d_corr_args, 1.0 / h->g.kernel_step, out, h->gd);
 }
 
 int clt_exec_corr_adjoint(clt_handle* h) {
@@ -1783,7 +1839,7 @@ int clt_exec_corr_adjoint(clt_handle* h) {
     if (!h->corr_quad) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_corr_adjoint: needs the 6 reference quad pairs");
     if ((rc = dalloc(h, (void**)&h->d_adj, (size_t)h->cap_tasks * CORR_ADJ_STRIDE * 4))) return rc;
     tbegin(h, T_ADJ);
-    launch_corr_adjoint(h);
+    launch_corr_adjoint(h, h->d_adj);
     tend(h, T_ADJ);
     HIPCHK(h, hipGetLastError());
     h->adj_done = true;
@@ -1833,7 +1889,7 @@ static int geo_eval(clt_handle* h, const float* disp, double lambda, double* sum
     const int n = h->num_tasks;
     hipLaunchKernelGGL(k_port_centres, dim3((4 * n + 255) / 256), dim3(256), 0, h->stream, h->d_port_geom, h->d_rtab,
                        h->d_gwords, h->d_gtxy, disp, n, h->task_words, h->d_tasks, nullptr, 0, 1.0);
-    launch_corr_adjoint(h);
+    launch_corr_adjoint(h, h->d_adj);
     if (!sums) return CLT_OK;
     const int blocks = (n + GEO_ADJ_TASKS - 1) / GEO_ADJ_TASKS;
     hipLaunchKernelGGL(k_geometry_adjoint, dim3(blocks), dim3(128), 0, h->stream, h->d_port_geom, h->d_rtab, h->d_gwords,
@@ -1869,6 +1925,58 @@ int clt_exec_geometry_adjoint(clt_handle* h, double* grad_extrinsic, float* grad
         HIPCHK(h, hipMemcpy(ts.data(), h->d_tstate, ts.size() * 8, hipMemcpyDeviceToHost));
         for (int t = 0; t < h->num_tasks; t++) grad_disparity[t] = (float)ts[(size_t)t * 15];
     }
+    return CLT_OK;
+}
+
+// clt_exec_geometry_adjoint on the static scratch: no allocation on any call.  The handle's
+// inputs (images, kernels, port geometry, task list of clt_set_tasks_geometry) and its CorrArgs
+// buffer were allocated by earlier calls; the per-task and per-block buffers live in the scratch
+// array, which every call on this GPU shares (each call rewrites all of it that it reads, so
+// nothing has to start at zero; calls that overlap on one GPU race).
+int clt_exec_geometry_adjoint_noalloc(clt_handle* h, double* grad_extrinsic, float* grad_disparity, double* loss) {
+    if (!h) return CLT_E_ARG;
+    const char* who = "clt_exec_geometry_adjoint_noalloc";
+    int rc;
+    if ((rc = check_adjoint_config(h, who))) return rc;
+    if (!h->port_geom_set || !h->tasks_from_geom)
+        return fail(h, CLT_E_STATE, "%s: tasks must come from clt_set_tasks_geometry", who);
+    const int n = h->num_tasks;
+    if (n > CLT_NOALLOC_MAX_TASKS)
+        return fail(h, CLT_E_ARG, "%s: %d tasks, the static scratch holds at most %d", who, n, CLT_NOALLOC_MAX_TASKS);
+    if (!h->scratch) return fail(h, CLT_E_STATE, "%s: no scratch address on device %d", who, h->device);
+    CorrArgs a;
+    bool quad = false;
+    if ((rc = fill_corr_args(h, a, &quad))) return rc;
+    if (!quad) return fail(h, CLT_E_UNSUPPORTED, "%s: needs the 6 reference quad pairs", who);
+    unsigned char* base = (unsigned char*)h->scratch;
+    float* adj = (float*)(base + NA_OFF_ADJ);
+    double* tstate = (double*)(base + NA_OFF_TSTATE);
+    double* part = (double*)(base + NA_OFF_PART);
+    double* gsum = (double*)(base + NA_OFF_SUM);
+    float* gdisp = (float*)(base + NA_OFF_GD);
+    const int blocks = (n + GEO_ADJ_TASKS - 1) / GEO_ADJ_TASKS;
+    join_lma(h);   // kernels of earlier calls on the other streams finish first
+    join_side(h);
+    HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    tbegin(h, T_GEOADJ);
+    hipLaunchKernelGGL(k_port_centres, dim3((4 * n + 255) / 256), dim3(256), 0, h->stream, h->d_port_geom, h->d_rtab,
+                       h->d_gwords, h->d_gtxy, h->d_gdisp, n, h->task_words, h->d_tasks, nullptr, 0, 1.0);
+    launch_corr_adjoint(h, adj);
+    hipLaunchKernelGGL(k_geometry_adjoint, dim3(blocks), dim3(128), 0, h->stream, h->d_port_geom, h->d_rtab, h->d_gwords,
+                       h->d_gtxy, h->d_gdisp, adj, n, 0.0, part, tstate);
+    hipLaunchKernelGGL(k_sum_rows, dim3(GEO_ADJ_SUMS), dim3(256), 0, h->stream, part, blocks, GEO_ADJ_SUMS, gsum);
+    if (grad_disparity)
+        hipLaunchKernelGGL(k_take_col, dim3((n + 255) / 256), dim3(256), 0, h->stream, tstate, 15, n, gdisp);
+    tend(h, T_GEOADJ);
+    HIPCHK(h, hipGetLastError());
+    double sums[GEO_ADJ_SUMS];
+    HIPCHK(h, hipMemcpyAsync(sums, gsum, sizeof(sums), hipMemcpyDeviceToHost, h->stream));
+    if (grad_disparity && n > 0)
+        HIPCHK(h, hipMemcpyAsync(grad_disparity, gdisp, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (grad_extrinsic)
+        for (int i = 0; i < 13; i++) grad_extrinsic[i] = sums[105 + i];
+    if (loss) *loss = sums[104];
     return CLT_OK;
 }
 

@@ -428,6 +428,17 @@ class GPUTileProcessor:
         self._chk(self.lib.clt_exec_geometry_adjoint(self.h, _ptr(ge), _ptr(gd), ctypes.byref(loss)))
         return ge, gd, loss.value
 
+    def getGeometryGradientNoAlloc(self):
+        """getGeometryGradient through clt_exec_geometry_adjoint_noalloc: no device allocation (the
+        buffers are the library's static scratch, at most MAX_TASKS_NOALLOC tasks)."""
+        ge = np.zeros(13, np.float64)
+        gd = np.zeros(self.num_task_tiles, np.float32)
+        loss = ctypes.c_double()
+        self._chk(self.lib.clt_exec_geometry_adjoint_noalloc(self.h, _ptr(ge), _ptr(gd), ctypes.byref(loss)))
+        return ge, gd, loss.value
+
+    MAX_TASKS_NOALLOC = 80000   # CLT_NOALLOC_MAX_TASKS
+
     def lmaExtrinsics(self, num_iter=10, adjust_disparity=True, **kw):
         """LMA of the extrinsics (and disparities) on the correlation-peak loss; returns
         (extrinsics [13], disparities [n], loss history)."""

@@ -386,6 +386,18 @@ int  clt_get_corr_adjoint(clt_handle* h, float* out);
  * geometry derivatives (GC:2235-2306).  Any output may be NULL. */
 int  clt_exec_geometry_adjoint(clt_handle* h, double* grad_extrinsic, float* grad_disparity, double* loss);
 
+/* No counterpart in the reference.  clt_exec_geometry_adjoint without any device allocation, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving or LMA loops, harnesses
+ * that forbid hipMalloc): the same kernels and outputs, with the per-task adjoint records, per-task
+ * state, per-block partial systems and their sums in one static device array of
+ * CLT_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends).  At most CLT_NOALLOC_MAX_TASKS
+ * tasks (a full 2592x1936 frame is 78,408); more fail with CLT_E_ARG.  The handle's inputs come from
+ * earlier calls (images, kernels, clt_set_port_geometry, clt_set_tasks_geometry).  Every call on a
+ * GPU shares the array: calls that overlap on one GPU (other threads, other handles) race. */
+#define CLT_NOALLOC_MAX_TASKS     80000
+#define CLT_NOALLOC_SCRATCH_BYTES (22u << 20)
+int  clt_exec_geometry_adjoint_noalloc(clt_handle* h, double* grad_extrinsic, float* grad_disparity, double* loss);
+
 /* Levenberg-Marquardt over the extrinsics (and the task disparities) minimising the summed loss
  * (the role of the reference's extrinsic LMA, AlignmentCorrection.java:3346-3452): gradient from
  * the adjoint; Gauss-Newton matrix from the port-centre derivatives weighted by the per-pair
