@@ -143,6 +143,7 @@ SIGNATURES = {
     "clt_exec_textures": (_I, [_H, _P]),
     "clt_exec_textures_imclt": (_I, [_H, _P, _I]),
     "clt_exec_combine_textures": (_I, [_H]),
+    "clt_exec_combine_rgba_tiles": (_I, [_H, _I, _I]),
     "clt_get_rbg": (_I, [_H, _I, _P]),
     "clt_get_clt": (_I, [_H, _I, _P]),
     "clt_get_corr2d": (_I, [_H, _P, _P, _P]),

@@ -2197,11 +2197,24 @@ __global__ __launch_bounds__(256) void k_tex_imclt(const float* __restrict__ clt
 // combine textures: overlap-add of the texture tiles (ImageDtt.java:3858-3989, overlap,
 // sharp_alpha = false), gather form: thread = output pixel.
 // ====================================================================================
+// combineRGBATiles (ImageDtt.java:3855-3991) as a per-pixel gather over the tile map.  overlap: the
+// (up to 4) 16x16 tiles at stride 8 covering the pixel are summed; with sharp_alpha the alpha plane
+// comes only from the centre 8x8 of the tile containing the pixel.  Without overlap the output is the
+// (2 height) x (2 width) mosaic of whole tiles.  Every output pixel is written once.
 __global__ __launch_bounds__(256) void k_combine_textures(const float* __restrict__ tiles, const int32_t* __restrict__ tile_map,
-                                                          int planes, float* __restrict__ out, GeomDev g) {
+                                                          int planes, int overlap, int sharp_alpha,
+                                                          float* __restrict__ out, GeomDev g) {
+    const int ow = overlap ? g.width : 2 * g.width, oh = overlap ? g.height : 2 * g.height;
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= g.width || y >= g.height) return;
+    if (x >= ow || y >= oh) return;
+    const size_t plane_sz = (size_t)ow * oh, o = (size_t)y * ow + x;
+    if (!overlap) {
+        const int tk = tile_map[(size_t)(y >> 4) * g.tiles_x + (x >> 4)];
+        const int off = (y & 15) * 16 + (x & 15);
+        for (int p = 0; p < planes; p++) out[p * plane_sz + o] = (tk < 0) ? 0.f : tiles[((size_t)tk * planes + p) * 256 + off];
+        return;
+    }
     const int tyh = (y + 4) >> 3, txh = (x + 4) >> 3;
     const int iy = (y + 4) & 7, ix = (x + 4) & 7;
     int tsk[4], off[4];
@@ -2219,11 +2232,15 @@ __global__ __launch_bounds__(256) void k_combine_textures(const float* __restric
             n++;
         }
     }
-    const size_t plane_sz = (size_t)g.width * g.height;
     for (int p = 0; p < planes; p++) {
         float s = 0.f;
-        for (int k = 0; k < n; k++) s += tiles[((size_t)tsk[k] * planes + p) * 256 + off[k]];
-        out[p * plane_sz + (size_t)y * g.width + x] = s;
+        if (p == 3 && sharp_alpha) {   // the centre 8x8 of the pixel's own tile
+            const int tk = tile_map[(size_t)(y >> 3) * g.tiles_x + (x >> 3)];
+            if (tk >= 0) s = tiles[((size_t)tk * planes + 3) * 256 + ((y & 7) + 4) * 16 + (x & 7) + 4];
+        } else {
+            for (int k = 0; k < n; k++) s += tiles[((size_t)tsk[k] * planes + p) * 256 + off[k]];
+        }
+        out[p * plane_sz + o] = s;
     }
 }
 

@@ -262,6 +262,7 @@ struct clt_handle {
     bool images_set[CLT_MAX_CAMS] = {};
     int num_tasks = 0;
     bool converted = false, corr_done = false, tex_done = false, combined = false;
+    int combine_scale = 1;         // 1: overlap-added planes [H][W]; 2: the mosaic [2H][2W]
     int corr_pairs_at_exec = 0, corr_strip_h_at_exec = 0, tex_planes_at_exec = 0;
     float corr_sigma = 0.8f;
     std::vector<clt_pair> pairs;
@@ -1741,20 +1742,25 @@ int clt_get_textures_rig(clt_handle* h, float* tiles_main, float* tiles_aux, flo
     return CLT_OK;
 }
 
-int clt_exec_combine_textures(clt_handle* h) {
+int clt_exec_combine_textures(clt_handle* h) { return clt_exec_combine_rgba_tiles(h, 1, 0); }
+
+int clt_exec_combine_rgba_tiles(clt_handle* h, int32_t overlap, int32_t sharp_alpha) {
     if (!h) return CLT_E_ARG;
-    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_combine_textures: not available for a batch");
-    if (!h->tex_done) return fail(h, CLT_E_STATE, "clt_exec_combine_textures: no textures");
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_combine_rgba_tiles: not available for a batch");
+    if (!h->tex_done) return fail(h, CLT_E_STATE, "clt_exec_combine_rgba_tiles: no textures");
     const clt_geom& g = h->g;
     const int planes = h->tex_planes_at_exec;
+    const int k = overlap ? 1 : 2;
     int rc;
-    if ((rc = ensure_capacity(h, &h->d_tex_img, &h->tex_img_cap, (size_t)planes * g.width * g.height))) return rc;
-    dim3 grid((g.width + 63) / 64, (g.height + 3) / 4);
+    if ((rc = ensure_capacity(h, &h->d_tex_img, &h->tex_img_cap, (size_t)planes * k * g.width * k * g.height))) return rc;
+    dim3 grid((k * g.width + 63) / 64, (k * g.height + 3) / 4);
     tbegin(h, T_COMBINE);
-    hipLaunchKernelGGL(k_combine_textures, grid, dim3(256), 0, h->stream, h->d_tex, h->d_tile_map, planes, h->d_tex_img, h->gd);
+    hipLaunchKernelGGL(k_combine_textures, grid, dim3(256), 0, h->stream, h->d_tex, h->d_tile_map, planes, overlap ? 1 : 0,
+                       sharp_alpha ? 1 : 0, h->d_tex_img, h->gd);
     tend(h, T_COMBINE);
     HIPCHK(h, hipGetLastError());
     h->combined = true;
+    h->combine_scale = k;
     return finish(h);
 }
 
@@ -1809,7 +1815,9 @@ int clt_get_combined_textures(clt_handle* h, float* out) {
     if (!h || !out) return CLT_E_ARG;
     if (!h->combined) return fail(h, CLT_E_STATE, "clt_get_combined_textures: combine not executed");
     HIPCHK(h, sync_streams(h));
-    HIPCHK(h, hipMemcpy(out, h->d_tex_img, (size_t)h->tex_planes_at_exec * h->g.width * h->g.height * 4, hipMemcpyDeviceToHost));
+    const size_t k = h->combine_scale;
+    HIPCHK(h, hipMemcpy(out, h->d_tex_img, (size_t)h->tex_planes_at_exec * k * h->g.width * k * h->g.height * 4,
+                        hipMemcpyDeviceToHost));
     return CLT_OK;
 }
 

@@ -333,8 +333,11 @@ class GPUTileProcessor:
         ed = None if extra_disparity is None else np.ascontiguousarray(extra_disparity, np.float32)
         self._chk(self.lib.clt_exec_textures_imclt(self.h, _ptr(ed), int(apply_lpf)))
 
-    def execCombineTextures(self):
-        self._chk(self.lib.clt_exec_combine_textures(self.h))
+    def execCombineTextures(self, overlap=True, sharp_alpha=False):
+        """combineRGBATiles (ID:3855-3991) of the last textures: overlap-added planes, or with
+        overlap=False the (2H x 2W) mosaic of whole tiles; sharp_alpha: alpha from tile centres."""
+        self._chk(self.lib.clt_exec_combine_rgba_tiles(self.h, int(bool(overlap)), int(bool(sharp_alpha))))
+        self._combine_scale = 1 if overlap else 2
 
     # ------------------------------------------------------------------ outputs
     def getRBG(self, ncam):
@@ -403,7 +406,8 @@ class GPUTileProcessor:
 
     def getCombinedTextures(self):
         planes = (4 + self.NUM_CAMS + 4) if self.keep_weights else 4
-        out = np.empty((planes, self.IMG_HEIGHT, self.IMG_WIDTH), np.float32)
+        k = getattr(self, "_combine_scale", 1)
+        out = np.empty((planes, k * self.IMG_HEIGHT, k * self.IMG_WIDTH), np.float32)
         self._chk(self.lib.clt_get_combined_textures(self.h, _ptr(out)))
         return out
 

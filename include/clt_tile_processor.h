@@ -279,6 +279,12 @@ int  clt_exec_textures_imclt(clt_handle* h, const float* extra_disparity, int32_
 /* Overlap-add of the texture tiles into full-frame planes (combineRGBATiles, ID:3858-3989,
  * overlap = true, sharp_alpha = false). */
 int  clt_exec_combine_textures(clt_handle* h);
+/* combineRGBATiles(texture_tiles, 8, overlap, sharp_alpha, ...) (ID:3855-3991) of the last textures:
+ * overlap != 0: [planes][height][width], the 16x16 tiles overlap-added at stride 8, with sharp_alpha
+ * the alpha plane from the centre 8x8 of each tile only; overlap == 0: the [planes][2 height]
+ * [2 width] mosaic of whole tiles.  A tile position holds the last task on it with a non-zero task
+ * word (texture_tiles[tileY][tileX]); positions without one stay 0. */
+int  clt_exec_combine_rgba_tiles(clt_handle* h, int32_t overlap, int32_t sharp_alpha);
 
 /* ---- outputs ---------------------------------------------------------------------------------- */
 /* getRBG(int ncam) (GTP:599-626): [num_colors][height+8][width+8] floats (uncropped; the
@@ -306,7 +312,8 @@ int  clt_get_textures_rig(clt_handle* h, float* tiles_main, float* tiles_aux, fl
  * [num_tasks][4*num_cams] = max_diff[num_cams] (ID:2802-2806) then the mean tones
  * r[0..C-1], b[0..C-1], g[0..C-1] (ID:2807-2811), or NULL. */
 int  clt_get_textures(clt_handle* h, float* tiles, float* stats);
-/* [planes][height][width] overlap-added texture planes. */
+/* The planes of the last clt_exec_combine_textures / clt_exec_combine_rgba_tiles:
+ * [planes][height][width], or [planes][2 height][2 width] for the mosaic. */
 int  clt_get_combined_textures(clt_handle* h, float* out);
 
 /* ---- port geometry on the device ---------------------------------------------------------------

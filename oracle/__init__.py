@@ -249,7 +249,8 @@ def lib(precision=64):
     L.orc_textures_frame.argtypes = [ctypes.POINTER(Geom), _D, _VP, ctypes.c_int,
                                      ctypes.POINTER(TexParams), ctypes.POINTER(CorrParams),
                                      ctypes.c_double, _VP, _D, _VP, ctypes.c_int]
-    L.orc_combine_textures.argtypes = [ctypes.POINTER(Geom), _VP, ctypes.c_int, _D, ctypes.c_int, _D]
+    L.orc_combine_textures.argtypes = [ctypes.POINTER(Geom), _VP, ctypes.c_int, _D, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, _D]
     L.orc_rot_matrices.argtypes = [ctypes.POINTER(PortGeometry), _D, _VP]
     L.orc_r_by_rdist.argtypes = [ctypes.POINTER(PortGeometry), ctypes.c_double]
     L.orc_r_by_rdist.restype = ctypes.c_double
@@ -588,14 +589,16 @@ def textures_frame(geom, clt, tasks, tex_params=None, corr_params=None, corr_sig
     return tiles, stats
 
 
-def combine_textures(geom, tasks, tiles):
-    """combineRGBATiles (ID:3858-3989), overlap: [planes][H][W]."""
+def combine_textures(geom, tasks, tiles, overlap=True, sharp_alpha=False):
+    """combineRGBATiles (ID:3855-3991): [planes][H][W] with overlap, else the [planes][2H][2W]
+    mosaic of whole tiles; sharp_alpha: alpha from the centre 8x8 of each tile only."""
     tasks = np.ascontiguousarray(tasks, np.int32)
     tiles = _f64(tiles)
     planes = tiles.shape[1]
-    out = np.zeros((planes, geom.height, geom.width))
+    k = 1 if overlap else 2
+    out = np.zeros((planes, k * geom.height, k * geom.width))
     lib().orc_combine_textures(ctypes.byref(geom), tasks.ctypes.data_as(_VP), tasks.shape[0], _dp(tiles),
-                               planes, _dp(out))
+                               planes, int(bool(overlap)), int(bool(sharp_alpha)), _dp(out))
     return out
 
 

@@ -2276,28 +2276,53 @@ EXPORT int orc_textures_frame(const clt_geom* g, const double* clt, const int32_
     return 0;
 }
 
-/* ID:3858-3989 combineRGBATiles(overlap = true, sharp_alpha = false) on texture tiles placed by
- * the task list; out: [planes][height][width] (zeroed by the caller). */
+/* ID:3855-3991 combineRGBATiles on the texture tiles of a task list: the tile of a position is the
+ * last task on it with a non-zero task word (texture_tiles[tileY][tileX]).  overlap: the 16x16 tiles
+ * at stride 8, the tile's pixel (i, j) added at (8 ty + i - 4, 8 tx + j - 4) inside the frame; with
+ * sharp_alpha the alpha plane (3) takes only the centre 8x8 of each tile (ID:3931-3937).  Without
+ * overlap: each tile copied as a 16x16 block at (16 ty, 16 tx) of a (2 height) x (2 width) mosaic.
+ * out: [planes][H][W] (overlap) or [planes][2H][2W], zeroed by the caller. */
 EXPORT int orc_combine_textures(const clt_geom* g, const int32_t* tasks, int ntasks, const double* tiles,
-                                int planes, double* out) {
+                                int planes, int overlap, int sharp_alpha, double* out) {
     frame_t f;
     frame_init(&f, g);
-    const int W = g->width, H = g->height;
+    const int W = g->width, H = g->height, TX = f.tiles_x, TY = f.tiles_y;
+    int* map = malloc(sizeof(int) * (size_t)TX * TY);
+    for (int i = 0; i < TX * TY; i++) map[i] = -1;
     for (int t = 0; t < ntasks; t++) {
         if (task_flags(g, tasks, t) == 0) continue;
         int tx, ty;
         task_txy(g, tasks, t, &tx, &ty);
-        const double* tt = tiles + (size_t)t * planes * 256;
-        for (int i = 0; i < 16; i++) {
-            int y = ty * 8 + i - 4;
-            if (y < 0 || y >= H) continue;
-            for (int j = 0; j < 16; j++) {
-                int x = tx * 8 + j - 4;
-                if (x < 0 || x >= W) continue;
-                for (int p = 0; p < planes; p++) out[((size_t)p * H + y) * W + x] += tt[p * 256 + i * 16 + j];
+        map[ty * TX + tx] = t;
+    }
+    const int n2 = 16, n_half = 4;
+    for (int ty = 0; ty < TY; ty++)
+        for (int tx = 0; tx < TX; tx++) {
+            const int t = map[ty * TX + tx];
+            if (t < 0) continue;
+            const double* tt = tiles + (size_t)t * planes * 256;
+            if (!overlap) {
+                for (int i = 0; i < n2; i++)
+                    for (int p = 0; p < planes; p++)
+                        for (int j = 0; j < n2; j++)
+                            out[((size_t)p * 2 * H + ty * n2 + i) * 2 * W + tx * n2 + j] = tt[p * 256 + i * n2 + j];
+                continue;
+            }
+            for (int i = 0; i < n2; i++) {
+                const int y = ty * 8 + i - n_half;
+                if (y < 0 || y >= H) continue;
+                for (int p = 0; p < planes; p++) {
+                    const int sharp = (p == 3) && sharp_alpha;
+                    if (sharp && (i < n_half || i >= n2 - n_half)) continue;
+                    for (int j = sharp ? n_half : 0; j < (sharp ? n2 - n_half : n2); j++) {
+                        const int x = tx * 8 + j - n_half;
+                        if (x < 0 || x >= W) continue;
+                        out[((size_t)p * H + y) * W + x] += tt[p * 256 + i * n2 + j];
+                    }
+                }
             }
         }
-    }
+    free(map);
     return 0;
 }
 

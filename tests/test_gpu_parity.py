@@ -355,3 +355,27 @@ def test_batch_follows_kernel_changes(pk):
             assert err <= 1e-6 * max(1.0, float(np.abs(c1).max()))
             assert np.array_equal(np.isnan(d1), np.isnan(d2))
     bt.close()
+
+
+@pytest.mark.parametrize("overlap,sharp", [(True, False), (True, True), (False, False)])
+def test_combine_rgba_tiles_modes(pk, orc, overlap, sharp):
+    """combineRGBATiles (ID:3855-3991) on the device's own texture tiles against the oracle: the
+    overlap-add, its sharp-alpha form and the mosaic of whole tiles, with a sparse task list and two
+    tasks on one tile (the later one holds the position)."""
+    log("case: combineRGBATiles overlap=%s sharp_alpha=%s, 128x96, 60%% of the tiles" % (overlap, sharp))
+    sc = pk.synth.make_scene(128, 96, seed=27, disparity=("field", 0.0, 5.0), task_fraction=0.6)
+    tasks = np.concatenate([sc.tasks, sc.tasks[3:4]])
+    tasks[-1, 0] = 0x7
+    tp = gpu_processor(pk, sc)
+    tp.setTasks(tasks)
+    tp.setCorrSigma(0.8)
+    tp.execConverCorrectTiles()
+    tp.execTextures()
+    tiles, _ = tp.getTextures()
+    tp.execCombineTextures(overlap=overlap, sharp_alpha=sharp)
+    img = tp.getCombinedTextures()
+    tp.close()
+    ref = orc.combine_textures(oracle_geom(orc, sc), tasks, tiles.astype(np.float64), overlap=overlap, sharp_alpha=sharp)
+    err = float(np.abs(img - ref).max() / np.abs(ref).max())
+    log("  shape %s, max |diff| / max %.3g" % (img.shape, err))
+    assert img.shape == ref.shape and err < 1e-6

@@ -788,3 +788,59 @@ def test_max_x_cm_asymmetric(orc, ixc):
     xs = np.array([x for x, _, _ in samp], np.float64)
     s0, sx, sx2 = wts.sum(), (wts * xs).sum(), (wts * xs * xs).sum()
     assert np.allclose(r, [ixc + sx / s0 / 2, s0, np.sqrt(s0 * sx2 - sx * sx) / s0 / 2], rtol=1e-13)
+
+
+# ---------------------------------------------------------------- combineRGBATiles modes
+def _combine_case(orc):
+    """4 x 3 tiles (32 x 24 px), 12 planes; tile (tx, ty) of task t has plane p filled with
+    100 t + p, alpha 1; no task on (3, 2); tile (1, 1) listed twice (the later task wins)."""
+    g = orc.make_geom(32, 24, kernels=False)
+    pos = [(tx, ty) for ty in range(3) for tx in range(4) if (tx, ty) != (3, 2)] + [(1, 1)]
+    tasks = np.zeros((len(pos), 10), np.int32)
+    tasks[:, 0] = 0xF
+    tasks[:, 1] = [tx | (ty << 16) for tx, ty in pos]
+    tiles = np.zeros((len(pos), 12, 256))
+    for t in range(len(pos)):
+        tiles[t] = 100.0 * t + np.arange(12)[:, None]
+        tiles[t, 3] = 1.0
+    return g, pos, tasks, tiles
+
+
+def test_combine_mosaic_places_whole_tiles(orc):
+    """overlap = false (ID:3970-3981): each tile copied to the 16x16 block at (16 ty, 16 tx) of a
+    (2 H) x (2 W) image; a position without a task stays 0; of two tasks on a tile the last wins."""
+    g, pos, tasks, tiles = _combine_case(orc)
+    m = orc.combine_textures(g, tasks, tiles, overlap=False)
+    assert m.shape == (12, 48, 64)
+    last = {p: t for t, p in enumerate(pos)}
+    for ty in range(3):
+        for tx in range(4):
+            blk = m[:, 16 * ty:16 * ty + 16, 16 * tx:16 * tx + 16]
+            if (tx, ty) == (3, 2):
+                assert (blk == 0).all()
+            else:
+                assert np.array_equal(blk.reshape(12, 256), tiles[last[(tx, ty)]])
+    assert last[(1, 1)] == len(pos) - 1
+
+
+def test_combine_sharp_alpha(orc):
+    """overlap = true: alpha planes of 1 overlap-add to the number of tiles covering a pixel (4 inside,
+    fewer at the frame border, where the tiles are clipped, and next to the empty tile); with
+    sharp_alpha (ID:3931-3937) alpha is 1 exactly where the pixel's own tile has a task, the other
+    planes unchanged."""
+    g, pos, tasks, tiles = _combine_case(orc)
+    soft = orc.combine_textures(g, tasks, tiles, overlap=True, sharp_alpha=False)
+    sharp = orc.combine_textures(g, tasks, tiles, overlap=True, sharp_alpha=True)
+    assert soft.shape == (12, 24, 32)
+    have = np.ones((3, 4))
+    have[2, 3] = 0
+    cover = np.zeros((24, 32))
+    for ty in range(3):
+        for tx in range(4):
+            if have[ty, tx]:
+                y0, x0 = max(0, 8 * ty - 4), max(0, 8 * tx - 4)
+                cover[y0:min(24, 8 * ty + 12), x0:min(32, 8 * tx + 12)] += 1
+    assert np.array_equal(soft[3], cover)
+    assert cover[10, 10] == 4 and cover[0, 0] == 1 and cover[2, 10] == 2
+    assert np.array_equal(sharp[3], np.kron(have, np.ones((8, 8))))
+    assert np.array_equal(np.delete(sharp, 3, 0), np.delete(soft, 3, 0))
