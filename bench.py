@@ -2,10 +2,10 @@
 
 Workload (one "step" = one frame through the plain per-frame GPU path): 4 x 2592x1936 Bayer
 (RGGB) float32 mosaics, 78,408 tiles, dense task list, per-tile disparity a smooth random field
-0..20 px, random per-tile aberration kernels; convert_direct -> correlate2D (6 pairs + combined
-strip + disparity, per-tile LMA chain) -> imclt_rbg (with LPF) -> textures (RGBA).  imclt_rbg and
-textures read only the converted tiles, so the LMA chain (on its own stream) overlaps them and the
-next convert.  Inputs are resident in HBM before the timed region; outputs stay on the device.
+0..20 px, random per-tile aberration kernels; convert_direct -> textures (RGBA) -> correlate2D
+(6 pairs + combined strip + disparity, per-tile LMA chain) -> imclt_rbg (with LPF).  textures and
+imclt_rbg read only the converted tiles; the LMA chain (on its own stream) overlaps imclt_rbg (on a
+side stream) and the next convert.  Inputs are resident in HBM before the timed region; outputs stay on the device.
 
 Prints one JSON line: tiles/s, HBM roofline of the dominant kernel (HIP events on the library's
 stream), the CPU baseline (the C restatement of the reference CPU path in FP64 and FP32 on the
@@ -242,10 +242,13 @@ def make_processor(pk, sc):
 
 
 def run_step(tp):
+    """One frame.  textures (vector-ALU bound) runs before correlate2D, so the per-tile LMA chain
+    that correlate2D starts on its own stream overlaps imclt_rbg and the next frame's convert,
+    which wait on memory and leave the vector ALUs idle a third of the time."""
     tp.execConverCorrectTiles(0)
+    tp.execTextures()
     tp.execCorrelate2D()
     tp.execImcltRbg(True)
-    tp.execTextures()
 
 
 def batch_bench(pk, sc, frames=32, seed=7, repeats=3):

@@ -775,7 +775,9 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
                 out_cm = disparity;
                 out_st = strength;
             }
-            dirs = dir_out ? true : (A.fo_correct && strength > 0.0);
+            // the foreground correction is kept only when strength > fo_min_strength (ID:2319-2322):
+            // below that the four fits cannot change the result
+            dirs = dir_out ? true : (A.fo_correct && strength > fmax(0.0, A.fo_min_strength));
         }
     }
     if (dirs) {   // corr4dirsLMA: two one-direction fits at a time, 8 lanes each
