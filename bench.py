@@ -313,8 +313,11 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
 def adjoint_bench(pk, sc, repeats=5, seed=11):
     """Differentiable form on the same frame: forward (convert + correlate2D) and the adjoint
     (port centres from the geometry, the fused convert/loss adjoint, the geometry adjoint and the
-    LMA system reduction) per pass, then 10 LMA iterations of the extrinsics and disparities
-    from extrinsics perturbed N(0, 3e-4)."""
+    LMA system reduction) per pass, then 10 iterations (3 coarse steps on the pairs'
+    correlation-peak offsets, 7 LMA iterations) of the extrinsics and disparities from extrinsics
+    perturbed N(0, 1e-3); the frame was rendered with zero extrinsics, so the extrinsics' distance
+    to 0 shows the convergence (up to the quad's gauge: azimuths along rXY_x and tilts along rXY_y
+    act as a common disparity)."""
     T = len(sc.tasks)
     tx, ty = sc.tasks[:, 1] & 0xFFFF, (sc.tasks[:, 1] >> 16) & 0xFFFF
     d = sc.disparity[ty, tx].astype(np.float32)
@@ -338,15 +341,17 @@ def adjoint_bench(pk, sc, repeats=5, seed=11):
     adj = tp.getTiming("geometry_adjoint")[0] / repeats
     tp.setTiming(False)
     rng = np.random.default_rng(seed)
-    tp.setPortGeometry(pk.GPUTileProcessor.defaultPortGeometry(W, H, extrinsic=list(rng.normal(0, 3e-4, 13))))
+    e0 = rng.normal(0, 1e-3, 13)
+    tp.setPortGeometry(pk.GPUTileProcessor.defaultPortGeometry(W, H, extrinsic=list(e0)))
     tp.getTileSubcamOffsets(words, tx, ty, d)
     t0 = time.perf_counter()
-    _, _, hist = tp.lmaExtrinsics(10)
+    ext, _, hist = tp.lmaExtrinsics(10, coarse_iter=3)
     lma_s = time.perf_counter() - t0
     tp.close()
     return {"tasks": T, "forward_ms": round(fwd, 4), "adjoint_ms": round(adj, 4),
             "forward_tiles_per_s": round(T / (fwd * 1e-3), 1), "adjoint_tiles_per_s": round(T / (adj * 1e-3), 1),
-            "lma_iterations": len(hist) - 1, "lma_ms": round(1e3 * lma_s, 2),
+            "lma_iterations": len(hist) - 1, "lma_coarse_steps": 3, "lma_ms": round(1e3 * lma_s, 2),
+            "extrinsics_rms": [float("%.3g" % np.sqrt(np.mean(e0 ** 2))), float("%.3g" % np.sqrt(np.mean(ext ** 2)))],
             "lma_loss": [round(float(hist[0]), 2), round(float(hist[-1]), 2)]}
 
 

@@ -99,7 +99,8 @@ class clt_rig_offset(ctypes.Structure):
 class clt_lma_params(ctypes.Structure):
     _fields_ = [("lambda_", ctypes.c_double), ("lambda_scale_good", ctypes.c_double),
                 ("lambda_scale_bad", ctypes.c_double), ("lambda_max", ctypes.c_double),
-                ("rms_diff", ctypes.c_double), ("num_iter", ctypes.c_int32), ("adjust_disparity", ctypes.c_int32)]
+                ("rms_diff", ctypes.c_double), ("num_iter", ctypes.c_int32), ("adjust_disparity", ctypes.c_int32),
+                ("coarse_iter", ctypes.c_int32), ("reserved", ctypes.c_int32), ("coarse_radius", ctypes.c_double)]
 
 
 _P = ctypes.c_void_p

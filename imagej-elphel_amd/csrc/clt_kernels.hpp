@@ -1672,6 +1672,69 @@ __global__ __launch_bounds__(256) void k_sum_rows(const double* __restrict__ a, 
 
 // Disparity step of the LMA after the extrinsic step de: dd_t = -(g_d + h . de) / hd, limited to
 // max_step pixels (a tile whose correlations are flat along its disparity has a tiny hd).
+// Correlation-peak offsets of the 6 quad pairs of each task (the measurement of the reference's
+// extrinsic path: clt_mismatch, ID:2342-2420, by mismatchPairsCM, C2:1336-1430): per pair the
+// integer maximum of its 15x15 correlation and the centre of mass of the positive values within
+// `radius` of it (image axes, +-7 px).  The peak sits at the excess of the pair's relative port
+// offset (xy_c1 - xy_c2) over the aligned one, so the correction wanted is minus the peak.  Written
+// as a loss-adjoint record (CORR_ADJ_STRIDE floats) for k_geometry_adjoint, whose Gauss-Newton
+// step then solves min sum_p w_p |D_p delta + peak_p|^2 (D_p = J_c1 - J_c2): [0] sum w |peak|^2,
+// [7..12] and [13..18] the pair weights w_p (the correlation at the maximum when it is at least
+// min_corr, else 0), [19..26] per camera sum_{p: c1 = c} w_p peak_p - sum_{p: c2 = c} w_p peak_p.
+__global__ __launch_bounds__(256) void k_corr_offsets(const float* __restrict__ corr, const int32_t* __restrict__ tasks,
+                                                      int task_words, int num_tasks, float radius, float min_corr,
+                                                      float* __restrict__ adj) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    float* a = adj + (size_t)t * CORR_ADJ_STRIDE;
+    if (tasks[(size_t)t * task_words] == 0) {
+        a[0] = __int_as_float(0x7fc00000);
+        for (int k = 1; k < CORR_ADJ_STRIDE; k++) a[k] = 0.f;
+        return;
+    }
+    const int ir = (int)radius;
+    const float r2 = radius * radius;
+    float g[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float loss = 0.f;
+    for (int p = 0; p < 6; p++) {
+        const float* c = corr + ((size_t)t * 6 + p) * 225;
+        int im = 0;
+        float vm = c[0];
+        for (int k = 1; k < 225; k++)
+            if (c[k] > vm) { vm = c[k]; im = k; }
+        const int iy = im / 15, ix = im % 15;
+        float s0 = 0.f, sx = 0.f, sy = 0.f;
+        for (int y = -ir; y <= ir; y++) {
+            const int yy = iy + y;
+            if (yy < 0 || yy > 14) continue;
+            for (int x = -ir; x <= ir; x++) {
+                const int xx = ix + x;
+                if (xx < 0 || xx > 14 || (float)(x * x + y * y) > r2) continue;
+                const float d = c[yy * 15 + xx];
+                if (d > 0.f) {
+                    s0 += d;
+                    sx += d * (float)(xx - 7);
+                    sy += d * (float)(yy - 7);
+                }
+            }
+        }
+        const bool ok = vm >= min_corr && s0 > 0.f;
+        const float w = ok ? vm : 0.f;
+        const float px = ok ? sx / s0 : 0.f, py = ok ? sy / s0 : 0.f;
+        a[7 + p] = w;
+        a[13 + p] = w;
+        loss += w * (px * px + py * py);
+        g[2 * QP_C1[p]] += w * px;
+        g[2 * QP_C1[p] + 1] += w * py;
+        g[2 * QP_C2[p]] -= w * px;
+        g[2 * QP_C2[p] + 1] -= w * py;
+    }
+    a[0] = loss;
+    for (int k = 0; k < 6; k++) a[1 + k] = 0.f;
+    for (int k = 0; k < 8; k++) a[19 + k] = g[k];
+    for (int k = 27; k < CORR_ADJ_STRIDE; k++) a[k] = 0.f;
+}
+
 __global__ __launch_bounds__(256) void k_disparity_step(const double* __restrict__ tstate, const double* __restrict__ de,
                                                         const float* __restrict__ d_in, float* __restrict__ d_out, int num_tasks,
                                                         double max_step) {

@@ -1872,6 +1872,9 @@ void clt_default_lma_params(clt_lma_params* p) {
     p->rms_diff = 1e-6;
     p->num_iter = 10;
     p->adjust_disparity = 1;
+    p->coarse_iter = 0;
+    p->reserved = 0;
+    p->coarse_radius = 3.9;      // ECP:2566 max_corr_radius
 }
 
 static int geo_prepare(clt_handle* h, const char* who) {
@@ -2038,6 +2041,38 @@ int clt_lma_extrinsics(clt_handle* h, const clt_lma_params* prm, double* extrins
     double loss = sums[104];
     if (history) history[0] = loss;
     int it = 0, hist_n = 1;
+    // coarse stage: Gauss-Newton steps that cancel the pairs' correlation-peak offsets
+    const int blocks = (n + GEO_ADJ_TASKS - 1) / GEO_ADJ_TASKS;
+    for (; it < std::min(prm->coarse_iter, prm->num_iter); it++) {
+        hipLaunchKernelGGL(k_port_centres, dim3((4 * n + 255) / 256), dim3(256), 0, h->stream, h->d_port_geom, h->d_rtab,
+                           h->d_gwords, h->d_gtxy, d_cur, n, h->task_words, h->d_tasks, nullptr, 0, 1.0);
+        launch_convert(h, n, nullptr, 0);
+        launch_corr(h, n, nullptr);
+        hipLaunchKernelGGL(k_corr_offsets, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_corr, h->d_tasks,
+                           h->task_words, n, (float)prm->coarse_radius, h->cp.min_corr, h->d_adj);
+        hipLaunchKernelGGL(k_geometry_adjoint, dim3(blocks), dim3(128), 0, h->stream, h->d_port_geom, h->d_rtab,
+                           h->d_gwords, h->d_gtxy, d_cur, h->d_adj, n, 0.0, h->d_gpart, h->d_tstate);
+        hipLaunchKernelGGL(k_sum_rows, dim3(GEO_ADJ_SUMS), dim3(256), 0, h->stream, h->d_gpart, blocks, GEO_ADJ_SUMS,
+                           h->d_gsum);
+        HIPCHK(h, hipGetLastError());
+        double osums[GEO_ADJ_SUMS], de[13];
+        HIPCHK(h, hipMemcpyAsync(osums, h->d_gsum, sizeof(osums), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, sync_streams(h));
+        if (!solve13(osums, 1e-3, de)) break;
+        for (int i = 0; i < 13; i++) cand.extrinsic[i] = cur.extrinsic[i] + de[i];
+        HIPCHK(h, hipMemcpyAsync(h->d_gsum + GEO_ADJ_SUMS, de, sizeof(de), hipMemcpyHostToDevice, h->stream));
+        if (prm->adjust_disparity)
+            hipLaunchKernelGGL(k_disparity_step, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_tstate,
+                               h->d_gsum + GEO_ADJ_SUMS, d_cur, d_new, n, 7.0);
+        else
+            HIPCHK(h, hipMemcpyAsync(d_new, d_cur, (size_t)n * 4, hipMemcpyDeviceToDevice, h->stream));
+        cur = cand;
+        std::swap(d_cur, d_new);
+        // the loss and the LMA system at the new point (for the history and the next step)
+        if ((rc = upload_geometry(h, cur)) || (rc = geo_eval(h, d_cur, lambda, sums))) return rc;
+        loss = sums[104];
+        if (history) history[hist_n++] = loss;
+    }
     for (; it < prm->num_iter; it++) {
         double de[13];
         while (!solve13(sums, lambda, de)) {   // make the damped system positive definite

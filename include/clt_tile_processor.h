@@ -416,13 +416,23 @@ typedef struct clt_lma_params {
     double  lambda_scale_bad;   /* 8.0                           (IDP lma_lambda_scale_bad)      */
     double  lambda_max;         /* 100                           (IDP lma_lambda_max)            */
     double  rms_diff;           /* stop below this relative loss improvement                     */
-    int32_t num_iter;           /* iterations                                                    */
+    int32_t num_iter;           /* iterations (coarse steps included)                            */
     int32_t adjust_disparity;   /* 1: the task disparities are parameters too                    */
+    /* Coarse stage (the reference's extrinsic path measures correlation-maximum offsets,
+     * clt_mismatch ID:2342-2420, and solves for the extrinsics from them, AlignmentCorrection
+     * AC:2891-3037, 3291-3470): the first coarse_iter iterations run convert + correlate2D, take each
+     * pair's peak offset (centre of mass within coarse_radius of its maximum, +-7 px) and make the
+     * Gauss-Newton step that cancels the offsets in the least-squares sense (weights: the peak
+     * values), the disparities eliminated per task; the loss-adjoint LMA follows.  0: LMA only. */
+    int32_t coarse_iter;        /* 0                                                             */
+    int32_t reserved;
+    double  coarse_radius;      /* 3.9 (max_corr_radius, ECP:2566)                               */
 } clt_lma_params;
 void clt_default_lma_params(clt_lma_params* p);
 /* Runs from the handle's port geometry and task disparities and leaves the handle at the solution
  * (geometry, disparities, task records).  extrinsic_out [13], disparity_out [num_tasks],
- * history [num_iter + 1] losses (initial, then after each iteration); any may be NULL. */
+ * history [num_iter + 1] losses (initial, then after each iteration, coarse ones included); any may
+ * be NULL. */
 int  clt_lma_extrinsics(clt_handle* h, const clt_lma_params* p, double* extrinsic_out, float* disparity_out,
                         double* history, int32_t* iterations);
 
