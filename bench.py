@@ -302,12 +302,19 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
     bt.resetTiming()
     bt.execBatch()
     batch_gpu = bt.getTiming("batch")[0]
+    by_chunks = {}   # graph time of each pipeline depth the autotuner chooses from
+    for chunks in (1, 2, 4, 8):
+        bt.setBatchChunks(chunks)
+        bt.execBatch()   # capture
+        bt.resetTiming()
+        bt.execBatch()
+        by_chunks[chunks] = round(bt.getTiming("batch")[0], 3)
     bt.close()
     return {"frames": frames, "tasks": total, "coverage": "uniform 5..100 % per frame",
             "batched_tiles_per_s": round(total / best_batch, 1), "plain_tiles_per_s": round(total / best_plain, 1),
             "batched_ms": round(1e3 * best_batch, 3), "plain_ms": round(1e3 * best_plain, 3),
             "speedup": round(best_plain / best_batch, 3), "batched_gpu_ms": round(batch_gpu, 3),
-            "plain_gpu_ms": round(plain_gpu, 3), "tune": tune}
+            "plain_gpu_ms": round(plain_gpu, 3), "batched_gpu_ms_by_chunks": by_chunks, "tune": tune}
 
 
 def adjoint_bench(pk, sc, repeats=5, seed=11):

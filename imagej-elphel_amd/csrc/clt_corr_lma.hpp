@@ -29,7 +29,8 @@ constexpr int LMA_GPAIRS = 4;                // pairs per group (the 8-camera ta
 constexpr int LMA_DPAIRS = 2;                // pairs per group of a one-direction fit
 constexpr int LMA_NFITS = 6;                 // all pairs, horizontal, vertical, diag main, diag other, one pair
 constexpr int LMA_FIT_SINGLE = 5;            // a single correlation as a horizontal pair (C2:1903-1935)
-constexpr int LMA_ACC = 28;                  // LDS doubles per fit for the accepted sums
+constexpr int LMA_NMOM = 14;                 // sample moments of one group
+constexpr int LMA_ACC = 28 + LMA_MAXG * LMA_NMOM;   // LDS doubles per fit: accepted sums, then moments
 
 struct LmaGroup {
     int32_t npairs, diag;
@@ -192,8 +193,9 @@ struct LmaOut {
     double disp, strength, wm, wxy;
 };
 
-// Per-fit LDS of one tile: values of the box cells (group g at +81 g), compacted sample lists, and
-// the sums of the accepted parameter vector.
+// Per-fit LDS of one tile: values of the box cells (group g at +81 g), compacted sample lists, the
+// sums of the accepted parameter vector (first 28 doubles of acc) and the sample moments of each
+// group (acc + 28 + LMA_NMOM g).
 struct LmaLds {
     double* v;
     uint16_t* idx;   // cell | weight class << 8 (1 convex, 2 other selected, 0 NaN sample)
@@ -273,7 +275,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
     constexpr int I_CROSS = NC * (NC + 1) / 2, I_DIAG = I_CROSS + 2 * NC, I_BAG = I_DIAG + 2;
     constexpr int I_BC = I_BAG + 2, I_RMS = I_BC + NC, NV = I_RMS + 1;
     constexpr int P_WXY = NC - 1;
-    static_assert(NV <= LMA_ACC, "LDS room for the accepted sums");
+    static_assert(NV <= 28, "LDS room for the accepted sums");
     LmaOut out = {false, 0.0, 0.0, 0.0, 0.0};
     const int G = F->ngroups;
     if (G == 0) return out;
@@ -460,10 +462,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         wc[g][1] = (A.nc_cost * np) * inv;
     }
 
-    // Passes over the samples at parameters q (getFxJt, getWJtJlambda, getWYmFxRms,
-    // Correlations2dLMA:368-461): `rms_at` the weighted residual square sum only, `normal_at` the
-    // normal matrix and J^T W (y - f).  The LMA needs the normal equations only at accepted points
-    // it continues from, so a rejected or final trial point costs one reduction instead of twenty.
+    // sample j of group g: value, weight and coordinates (x along the group's disparity axis)
     auto sample = [&](int g, int j, double& sv, double& sw, double& x, double& y) {
         const LmaGroup& grp = F->g[g];
         const int e = S.idx[g * LMA_CELLS + j];
@@ -532,6 +531,67 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         out.strength = v;
         return out;
     }
+    // Sample moments of each group (once per fit).  The model is a paraboloid in the sample
+    // coordinates: with u = x - X0, Y = y^2, alpha = ws^2 / Wx^2, beta = ws^2 / Wy^2 the shape is
+    // d = 1 - alpha u^2 - beta Y and every derivative is a u, u^2 or Y term, so the weighted
+    // residual square sum and the normal equations at any parameter vector are polynomials in
+    // the sums of sw x'^i (i <= 4), sw x'^i Y (i <= 2), sw Y^2, sw v x'^i (i <= 2), sw v Y and
+    // sw v^2, x' = x - xo the coordinate from the box centre (the same sums as the per-sample
+    // passes of getFxJt / getWJtJlambda / getWYmFxRms, Correlations2dLMA:368-461, reassociated).
+    double* mom = S.acc + 28;
+    double xo[LMA_MAXG] = {0.0, 0.0};
+#pragma unroll
+    for (int g = 0; g < LMA_MAXG; g++) {
+        if (g >= G || ns[g] == 0) continue;
+        const LmaGroup& grp = F->g[g];
+        const int cx = x0s[g] - 7, cy = y0s[g] - 7;   // the box centre cell
+        xo[g] = (double)(grp.diag ? cx + cy : cx) * (grp.diag ? 0.5 * grp.scale : grp.scale);
+        double m[LMA_NMOM];
+#pragma unroll
+        for (int k = 0; k < LMA_NMOM; k++) m[k] = 0.0;
+        for (int j = fl; j < ns[g]; j += W) {
+            double sv, sw, x, y;
+            sample(g, j, sv, sw, x, y);
+            const double xp = x - xo[g], Y = y * y;
+            const double w1 = sw * xp, w2 = w1 * xp, wy = sw * Y, wv = sw * sv, wv1 = wv * xp;
+            m[0] += sw;        m[1] += w1;        m[2] += w2;        m[3] += w2 * xp;   m[4] += w2 * (xp * xp);
+            m[5] += wy;        m[6] += wy * xp;   m[7] += wy * (xp * xp);               m[8] += wy * Y;
+            m[9] += wv;        m[10] += wv1;      m[11] += wv1 * xp; m[12] += wv * Y;   m[13] += wv * sv;
+        }
+#pragma unroll
+        for (int k = 0; k < LMA_NMOM; k++) m[k] = fit_sum<W>(m[k]);
+        if (fl == 0)
+#pragma unroll
+            for (int k = 0; k < LMA_NMOM; k++) mom[g * LMA_NMOM + k] = m[k];
+    }
+    wave_sync();
+    // the moments about X0 (binomial shift by X0 - xo)
+    struct UMom {
+        double s0, s1, s2, s3, s4, t0, t1, t2, q, v0, v1, v2, vy, vv;
+    };
+    auto umom = [&](int g, double x0) {
+        const double* m = mom + g * LMA_NMOM;
+        const double d = x0 - xo[g];
+        UMom u;
+        u.s0 = m[0];
+        u.s1 = m[1] - d * m[0];
+        u.s2 = m[2] - d * (2.0 * m[1] - d * m[0]);
+        u.s3 = m[3] - d * (3.0 * m[2] - d * (3.0 * m[1] - d * m[0]));
+        u.s4 = m[4] - d * (4.0 * m[3] - d * (6.0 * m[2] - d * (4.0 * m[1] - d * m[0])));
+        u.t0 = m[5];
+        u.t1 = m[6] - d * m[5];
+        u.t2 = m[7] - d * (2.0 * m[6] - d * m[5]);
+        u.q = m[8];
+        u.v0 = m[9];
+        u.v1 = m[10] - d * m[9];
+        u.v2 = m[11] - d * (2.0 * m[10] - d * m[9]);
+        u.vy = m[12];
+        u.vv = m[13];
+        return u;
+    };
+    // `rms_at` the weighted residual square sum, `normal_at` the normal matrix and J^T W (y - f),
+    // both from the moments.  The LMA needs the normal equations only at accepted points it
+    // continues from.
     auto rms_at = [&](const double* q) {
         double s2 = 0.0;
 #pragma unroll
@@ -539,21 +599,18 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             if (g >= G || ns[g] == 0) continue;
             const int gs = (g == 1 && ex0) ? 1 : 0;
             const LmaGroup& grp = F->g[g];
-            const double ws = grp.diag ? sqrt2 : 1.0;
+            const double ws = grp.diag ? sqrt2 : 1.0, ws2 = ws * ws;
             const double Wy = q[1] * grp.scale + ((NC == 4) ? q[2] : 0.0), Wx = Wy + q[P_WXY];
             const double iWx = rcp64(Wx), iWy = rcp64(Wy);
             const double Ag = gs ? q[NC + 1] : q[NC];
-            for (int j = fl; j < ns[g]; j += W) {
-                double sv, sw, x, y;
-                sample(g, j, sv, sw, x, y);
-                const double dxw = ws * (x - q[0]) * iWx, dyw = ws * y * iWy;
-                const double r = sv - (1.0 - dxw * dxw - dyw * dyw) * Ag;
-                s2 += sw * r * r;
-            }
+            const double a = ws2 * iWx * iWx, b = ws2 * iWy * iWy;
+            const UMom u = umom(g, q[0]);
+            const double sd2 = u.s0 - 2.0 * (a * u.s2 + b * u.t0) + a * (a * u.s4 + 2.0 * b * u.t2) + b * b * u.q;
+            const double sdv = u.v0 - a * u.v2 - b * u.vy;
+            s2 += u.vv - Ag * (2.0 * sdv - Ag * sd2);
         }
-        s2 = fit_sum<W>(s2);
         const double r0 = A.half_width - q[1], r1 = -q[P_WXY];
-        return s2 + wr0 * r0 * r0 + wr1 * r1 * r1;
+        return fmax(s2, 0.0) + wr0 * r0 * r0 + wr1 * r1 * r1;
     };
     auto normal_at = [&](const double* q) {
 #pragma unroll
@@ -563,39 +620,52 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             if (g >= G || ns[g] == 0) continue;
             const int gs = (g == 1 && ex0) ? 1 : 0;   // amplitude slot of this group
             const LmaGroup& grp = F->g[g];
-            const double ws = grp.diag ? sqrt2 : 1.0, scale = grp.scale;
-            const double Wy = q[1] * scale + ((NC == 4) ? q[2] : 0.0), Wx = Wy + q[P_WXY];
+            const double ws = grp.diag ? sqrt2 : 1.0, ws2 = ws * ws, sc = grp.scale;
+            const double Wy = q[1] * sc + ((NC == 4) ? q[2] : 0.0), Wx = Wy + q[P_WXY];
             const double iWx = rcp64(Wx), iWy = rcp64(Wy);
             const double Ag = gs ? q[NC + 1] : q[NC];
-            for (int j = fl; j < ns[g]; j += W) {
-                double sv, sw, x, y;
-                sample(g, j, sv, sw, x, y);
-                const double dxw = ws * (x - q[0]) * iWx, dyw = ws * y * iWy;
-                const double d = 1.0 - dxw * dxw - dyw * dyw;
-                const double wr = sw * (sv - d * Ag);
-                const double dfdWx = Ag * 2 * dxw * dxw * iWx, dfdWy = Ag * 2 * dyw * dyw * iWy;
-                double jc[NC];
-                jc[0] = ws * Ag * 2 * dxw * iWx;
-                jc[1] = scale * (dfdWx + dfdWy);
-                if constexpr (NC == 4) jc[2] = dfdWx + dfdWy;
-                jc[P_WXY] = dfdWx;
-                const double wd = sw * d;
-#pragma unroll
-                for (int a = 0; a < NC; a++) {
-                    const double wj = sw * jc[a];
-#pragma unroll
-                    for (int b = a; b < NC; b++) acc[lma_tri(NC, a, b)] += wj * jc[b];
-                    acc[I_CROSS + gs * NC + a] += wd * jc[a];
-                    acc[I_BC + a] += jc[a] * wr;
-                }
-                acc[I_DIAG + gs] += wd * d;
-                acc[I_BAG + gs] += d * wr;
+            const double a = ws2 * iWx * iWx, b = ws2 * iWy * iWy;
+            const UMom u = umom(g, q[0]);
+            // derivatives: X0 c0 u, Wm sc (cx u^2 + cy Y), Wyd cx u^2 + cy Y, Wxy cx u^2
+            const double c0 = 2.0 * Ag * a, cx = c0 * iWx, cy = 2.0 * Ag * b * iWy;
+            const double e = cx * (cx * u.s4 + 2.0 * cy * u.t2) + cy * cy * u.q;   // sum sw (cx u^2 + cy Y)^2
+            const double f = cx * u.s3 + cy * u.t1;                               // sum sw u (cx u^2 + cy Y)
+            const double h = cx * u.s4 + cy * u.t2;                               // sum sw u^2 (cx u^2 + cy Y)
+            const double du = u.s1 - a * u.s3 - b * u.t1;                         // sum sw d u
+            const double du2 = u.s2 - a * u.s4 - b * u.t2;                        // sum sw d u^2
+            const double dy = u.t0 - a * u.t2 - b * u.q;                          // sum sw d Y
+            const double sd2 = u.s0 - 2.0 * (a * u.s2 + b * u.t0) + a * (a * u.s4 + 2.0 * b * u.t2) + b * b * u.q;
+            const double sdv = u.v0 - a * u.v2 - b * u.vy;
+            double jj[NC][NC], cr[NC], jv[NC];
+            jj[0][0] = c0 * c0 * u.s2;
+            jj[0][1] = c0 * sc * f;
+            jj[1][1] = sc * sc * e;
+            jj[0][P_WXY] = c0 * cx * u.s3;
+            jj[1][P_WXY] = sc * cx * h;
+            jj[P_WXY][P_WXY] = cx * cx * u.s4;
+            cr[0] = c0 * du;
+            cr[1] = sc * (cx * du2 + cy * dy);
+            cr[P_WXY] = cx * du2;
+            jv[0] = c0 * u.v1;
+            jv[1] = sc * (cx * u.v2 + cy * u.vy);
+            jv[P_WXY] = cx * u.v2;
+            if constexpr (NC == 4) {
+                jj[0][2] = c0 * f;
+                jj[1][2] = sc * e;
+                jj[2][2] = e;
+                jj[2][P_WXY] = cx * h;
+                cr[2] = cx * du2 + cy * dy;
+                jv[2] = cx * u.v2 + cy * u.vy;
             }
-        }
 #pragma unroll
-        for (int i = 0; i < I_RMS; i++) {
-            const bool slot1 = (i >= I_CROSS + NC && i < I_DIAG) || i == I_DIAG + 1 || i == I_BAG + 1;
-            if (!slot1 || ng > 1) acc[i] = fit_sum<W>(acc[i]);
+            for (int i = 0; i < NC; i++) {
+#pragma unroll
+                for (int k = i; k < NC; k++) acc[lma_tri(NC, i, k)] += jj[i][k];
+                acc[I_CROSS + gs * NC + i] += cr[i];
+                acc[I_BC + i] += jv[i] - Ag * cr[i];
+            }
+            acc[I_DIAG + gs] += sd2;
+            acc[I_BAG + gs] += sdv - Ag * sd2;
         }
         // the two regularisation points
         const double r0 = A.half_width - q[1], r1 = -q[P_WXY];

@@ -205,6 +205,7 @@ struct clt_handle {
     int graph_stages = 0;
     // launch shapes chosen by clt_autotune
     int tune_corr_tpb = 4, tune_imclt_rows = 16;
+    int tune_chunks = 1;           // frame chunks of the batch pipeline
     int tex_imclt_rows = 8;        // output block rows per segment of k_tex_imclt
     bool tuned = false;
     bool corr_quad = false;        // correlate2D runs the quad kernel
@@ -344,61 +345,84 @@ static void tend(clt_handle* h, int which, hipStream_t st = nullptr) {
 
 // ------------------------------------------------------------------------ kernel launches
 // n tasks of the device task list; task_frame: frame of each task (batches) or null (one frame).
-static void launch_convert(clt_handle* h, int n, const int32_t* task_frame, int lpf_mask) {
-    join_side(h);   // a running imclt_rbg still reads the CLT
+// Tasks [t0, t0 + n) of the device task list; task_frame: frame of each task of the whole list
+// (batches) or null (one frame).  The per-task kernels index their records from the pointers
+// passed, so a range is the same launch on offset pointers.
+static void launch_convert_range(clt_handle* h, int t0, int n, const int32_t* task_frame, int lpf_mask) {
     const long units = (long)n * h->g.num_cams;   // (task, camera); 4 per block
     const int blocks = (int)((units + 3) / 4);
     const double inv_kstep = 1.0 / h->g.kernel_step;   // exact: checked power of two
     hipLaunchKernelGGL(k_convert, dim3(blocks), dim3(64 * h->g.num_colors), 0, h->stream, h->d_images,
-                       h->use_kernels ? h->d_kernels : nullptr, h->use_kernels ? h->d_koffs : nullptr, h->d_tasks,
-                       h->task_words, n, h->d_clt, lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, inv_kstep, task_frame,
+                       h->use_kernels ? h->d_kernels : nullptr, h->use_kernels ? h->d_koffs : nullptr,
+                       h->d_tasks + (size_t)t0 * h->task_words, h->task_words, n, h->d_clt,
+                       lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, inv_kstep, task_frame ? task_frame + t0 : nullptr,
                        h->img_fstride, h->clt_fstride, h->gd);
 }
-
-static void launch_imclt(clt_handle* h, int frames, int apply_lpf, hipStream_t st = nullptr) {
-    const int rows = h->tune_imclt_rows;
-    dim3 grid((h->tiles_x + 1 + IMCLT_S - 1) / IMCLT_S, (h->tiles_y + 1 + rows - 1) / rows,
-              frames * h->g.num_cams * h->g.num_colors);
-    hipLaunchKernelGGL(k_imclt, grid, dim3(256), 0, st ? st : h->stream, h->d_clt, h->d_tile_map,
-                       apply_lpf ? h->d_lpf_corr : nullptr, h->d_rbg, rows, h->clt_fstride, h->gd);
+static void launch_convert(clt_handle* h, int n, const int32_t* task_frame, int lpf_mask) {
+    join_side(h);   // a running imclt_rbg still reads the CLT
+    launch_convert_range(h, 0, n, task_frame, lpf_mask);
 }
 
-static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
-    join_lma(h);   // the previous LMA still reads d_corr and d_disp
+// Frames [f0, f0 + frames) of the CLT, tile maps and image planes.
+static void launch_imclt(clt_handle* h, int frames, int apply_lpf, hipStream_t st = nullptr, int f0 = 0) {
+    const int rows = h->tune_imclt_rows;
+    const int planes = h->g.num_cams * h->g.num_colors;
+    dim3 grid((h->tiles_x + 1 + IMCLT_S - 1) / IMCLT_S, (h->tiles_y + 1 + rows - 1) / rows, frames * planes);
+    const size_t plane_px = (size_t)(h->g.width + 8) * (h->g.height + 8);
+    hipLaunchKernelGGL(k_imclt, grid, dim3(256), 0, st ? st : h->stream, h->d_clt + (size_t)f0 * h->clt_fstride,
+                       h->d_tile_map + (size_t)f0 * h->num_tiles, apply_lpf ? h->d_lpf_corr : nullptr,
+                       h->d_rbg + (size_t)f0 * planes * plane_px, rows, h->clt_fstride, h->gd);
+}
+
+static void launch_corr_range(clt_handle* h, int t0, int n, const int32_t* task_frame) {
+    const size_t P = h->corr_pairs_at_exec, sh = h->corr_strip_h_at_exec;
+    const int32_t* tasks = h->d_tasks + (size_t)t0 * h->task_words;
+    const int32_t* tf = task_frame ? task_frame + t0 : nullptr;
+    float* corr = h->d_corr + (size_t)t0 * P * CLT_CORR_LEN;
+    float* combo = h->d_combo ? h->d_combo + (size_t)t0 * sh * 15 : nullptr;
+    float* disp = h->d_disp + (size_t)t0 * CLT_DISP_LAYERS;
     if (h->corr_quad) {
         const int tpb = h->tune_corr_tpb;
         auto kern = (h->cp.fat_zero != 0.f) ? k_corr2d_quad<true> : k_corr2d_quad<false>;
         hipLaunchKernelGGL(kern, dim3((n + tpb - 1) / tpb), dim3(64 * tpb), (size_t)tpb * CQ_WAVE_FLOATS * 4,
-                           h->stream, h->d_clt, h->d_tasks, h->task_words, n, h->d_lpf_corr, h->d_corr_args, h->d_corr,
-                           h->d_combo, h->d_disp, task_frame, h->clt_fstride, h->gd, nullptr);
+                           h->stream, h->d_clt, tasks, h->task_words, n, h->d_lpf_corr, h->d_corr_args, corr, combo,
+                           disp, tf, h->clt_fstride, h->gd, nullptr);
     } else {
         const size_t lds = (size_t)4 * ((corr_wave_floats(h->corr_num_strip) + 3) & ~3) * 4;
-        hipLaunchKernelGGL(k_corr2d<false>, dim3((n + 3) / 4), dim3(256), lds, h->stream, h->d_clt, h->d_tasks, h->task_words,
-                           n, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp, task_frame, h->clt_fstride,
-                           h->gd);
+        hipLaunchKernelGGL(k_corr2d<false>, dim3((n + 3) / 4), dim3(256), lds, h->stream, h->d_clt, tasks, h->task_words,
+                           n, h->d_lpf_corr, h->d_corr_args, corr, combo, disp, tf, h->clt_fstride, h->gd);
     }
 }
+static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
+    join_lma(h);   // the previous LMA still reads d_corr and d_disp
+    launch_corr_range(h, 0, n, task_frame);
+}
 
-// Per-tile LMA chain over the records just written by correlate2D (16 tiles per 256-thread block),
-// forked onto its own stream; the caller joins it (join_lma, or the end of a captured batch).
-static void launch_corr_lma(clt_handle* h, int n, bool timed) {
+// Per-tile LMA chain over the records of tasks [t0, t0 + n) just written by correlate2D (16 tiles
+// per 256-thread block), forked onto its own stream; the caller joins it (join_lma, or the end of
+// a captured batch).
+static void launch_corr_lma(clt_handle* h, int n, bool timed, int t0 = 0) {
     if (!h->lma_run) return;
     const int slots = LMA_SLOTS;
     auto kern = h->lma_args.adj_wyd ? k_corr_lma<4> : k_corr_lma<3>;
     (void)hipEventRecord(h->ev_fork, h->stream);
     (void)hipStreamWaitEvent(h->stream_lma, h->ev_fork, 0);
     if (timed) tbegin(h, T_CORR_LMA, h->stream_lma);
-    hipLaunchKernelGGL(kern, dim3((n + slots - 1) / slots), dim3(64 * LMA_WAVES), 0, h->stream_lma, h->d_corr, h->d_disp,
-                       n, h->d_lma_args, nullptr);
+    hipLaunchKernelGGL(kern, dim3((n + slots - 1) / slots), dim3(64 * LMA_WAVES), 0, h->stream_lma,
+                       h->d_corr + (size_t)t0 * h->corr_pairs_at_exec * CLT_CORR_LEN,
+                       h->d_disp + (size_t)t0 * CLT_DISP_LAYERS, n, h->d_lma_args, nullptr);
     if (timed) tend(h, T_CORR_LMA, h->stream_lma);
     (void)hipEventRecord(h->ev_lma, h->stream_lma);
     h->lma_pending = true;
 }
 
 template <int NC>
-static void launch_tex_nc(clt_handle* h, int n, const int32_t* task_frame, const float* extra) {
-    hipLaunchKernelGGL(k_textures<NC>, dim3(n), dim3(256), 0, h->stream, h->d_clt, h->d_tasks, h->task_words, n,
-                       h->d_lpf_corr, extra, h->d_tex_args, h->d_tex, h->d_tex_stats, task_frame, h->clt_fstride, h->gd);
+static void launch_tex_nc(clt_handle* h, int n, const int32_t* task_frame, const float* extra, int t0) {
+    const size_t planes = h->tex_planes_at_exec;
+    hipLaunchKernelGGL(k_textures<NC>, dim3(n), dim3(256), 0, h->stream, h->d_clt, h->d_tasks + (size_t)t0 * h->task_words,
+                       h->task_words, n, h->d_lpf_corr, extra ? extra + t0 : nullptr, h->d_tex_args,
+                       h->d_tex + (size_t)t0 * planes * 256, h->d_tex_stats + (size_t)t0 * 4 * h->g.num_cams,
+                       task_frame ? task_frame + t0 : nullptr, h->clt_fstride, h->gd);
 }
 
 template <int NC>
@@ -411,15 +435,15 @@ static void launch_tex_imclt_nc(clt_handle* h) {
                            h->gd);
 }
 
-static void launch_tex(clt_handle* h, int n, const int32_t* task_frame, const float* extra) {
+static void launch_tex(clt_handle* h, int n, const int32_t* task_frame, const float* extra, int t0 = 0) {
     switch (h->g.num_cams) {
-    case 2: launch_tex_nc<2>(h, n, task_frame, extra); break;
-    case 3: launch_tex_nc<3>(h, n, task_frame, extra); break;
-    case 4: launch_tex_nc<4>(h, n, task_frame, extra); break;
-    case 5: launch_tex_nc<5>(h, n, task_frame, extra); break;
-    case 6: launch_tex_nc<6>(h, n, task_frame, extra); break;
-    case 7: launch_tex_nc<7>(h, n, task_frame, extra); break;
-    default: launch_tex_nc<8>(h, n, task_frame, extra); break;
+    case 2: launch_tex_nc<2>(h, n, task_frame, extra, t0); break;
+    case 3: launch_tex_nc<3>(h, n, task_frame, extra, t0); break;
+    case 4: launch_tex_nc<4>(h, n, task_frame, extra, t0); break;
+    case 5: launch_tex_nc<5>(h, n, task_frame, extra, t0); break;
+    case 6: launch_tex_nc<6>(h, n, task_frame, extra, t0); break;
+    case 7: launch_tex_nc<7>(h, n, task_frame, extra, t0); break;
+    default: launch_tex_nc<8>(h, n, task_frame, extra, t0); break;
     }
 }
 
@@ -2184,16 +2208,37 @@ static int assemble_batch(clt_handle* h) {
 
 enum { STAGE_CONVERT = 1, STAGE_CORR = 2, STAGE_TEX = 4, STAGE_IMCLT = 8 };
 
+// The batch as a software pipeline over tune_chunks runs of whole frames.  Each chunk runs convert,
+// textures and correlate2D on the main stream and forks its imclt_rbg (after convert) and its
+// per-tile LMA (after correlate2D) onto their own streams, where they overlap the next chunk's
+// kernels.  Chunks touch disjoint frames of the CLT, tile maps and image planes and disjoint task
+// records, so the only joins are the fork points and the end of the batch.
 static void launch_batch(clt_handle* h, int stages) {
     const int32_t* tf = h->num_frames > 1 ? h->d_task_frame : nullptr;
-    if (stages & STAGE_CONVERT) launch_convert(h, h->num_tasks, tf, 0);
-    if (stages & STAGE_CORR) {
-        launch_corr(h, h->num_tasks, tf);
-        launch_corr_lma(h, h->num_tasks, false);
+    const int F = h->num_frames;
+    const int chunks = std::max(1, std::min(h->tune_chunks, F));
+    join_side(h);   // a running imclt_rbg still reads the CLT
+    join_lma(h);    // ... and an LMA the correlation records
+    for (int c = 0; c < chunks; c++) {
+        const int f0 = (int)((long)F * c / chunks), f1 = (int)((long)F * (c + 1) / chunks);
+        const int t0 = F > 1 ? h->frame_start[f0] : 0;
+        const int n = (F > 1 ? h->frame_start[f1] : h->num_tasks) - t0;
+        if ((stages & STAGE_CONVERT) && n > 0) launch_convert_range(h, t0, n, tf, 0);
+        if (stages & STAGE_IMCLT) {
+            (void)hipEventRecord(h->ev_side_fork, h->stream);
+            (void)hipStreamWaitEvent(h->stream_side, h->ev_side_fork, 0);
+            launch_imclt(h, f1 - f0, 1, h->stream_side, f0);
+            (void)hipEventRecord(h->ev_side, h->stream_side);
+            h->side_pending = true;
+        }
+        if ((stages & STAGE_TEX) && n > 0) launch_tex(h, n, tf, nullptr, t0);
+        if ((stages & STAGE_CORR) && n > 0) {
+            launch_corr_range(h, t0, n, tf);
+            launch_corr_lma(h, n, false, t0);
+        }
     }
-    if (stages & STAGE_TEX) launch_tex(h, h->num_tasks, tf, nullptr);
-    if (stages & STAGE_IMCLT) launch_imclt(h, h->num_frames, 1);
-    join_lma(h);   // a captured graph must end on the origin stream
+    join_lma(h);    // a captured graph must end on the origin stream
+    join_side(h);
 }
 
 static int prepare_batch(clt_handle* h, int stages) {
@@ -2260,6 +2305,21 @@ int clt_autotune(clt_handle* h) {
         }
         h->tune_imclt_rows = pick;
     }
+    {   // frame chunks of the pipelined batch (overlap of LMA and imclt_rbg with the next chunk)
+        int stages = STAGE_CONVERT | STAGE_CORR | STAGE_IMCLT;
+        if (h->g.num_colors == 3 && h->g.num_cams >= 2 && prepare_tex(h) == CLT_OK) stages |= STAGE_TEX;
+        h->err.clear();
+        const int cand[4] = {1, 2, 4, 8};
+        float best = 1e30f;
+        int pick = h->tune_chunks;
+        for (int c : cand) {
+            if (c > h->num_frames) break;
+            h->tune_chunks = c;
+            const float ms = time_stage(h, stages);
+            if (ms < best) { best = ms; pick = c; }
+        }
+        h->tune_chunks = pick;
+    }
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, sync_streams(h));
     h->tuned = true;
@@ -2272,6 +2332,20 @@ int clt_get_tune(clt_handle* h, int32_t* corr_tiles_per_block, int32_t* imclt_ro
     if (!h) return CLT_E_ARG;
     if (corr_tiles_per_block) *corr_tiles_per_block = h->tune_corr_tpb;
     if (imclt_rows) *imclt_rows = h->tune_imclt_rows;
+    return CLT_OK;
+}
+
+int clt_set_batch_chunks(clt_handle* h, int32_t chunks) {
+    if (!h) return CLT_E_ARG;
+    if (chunks < 1 || chunks > 64) return fail(h, CLT_E_ARG, "clt_set_batch_chunks: %d", chunks);
+    h->tune_chunks = chunks;
+    h->graph_epoch++;
+    return CLT_OK;
+}
+
+int clt_get_batch_chunks(clt_handle* h, int32_t* chunks) {
+    if (!h || !chunks) return CLT_E_ARG;
+    *chunks = std::max(1, std::min(h->tune_chunks, h->num_frames));
     return CLT_OK;
 }
 

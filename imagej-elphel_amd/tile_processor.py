@@ -511,7 +511,13 @@ class GPUTileBatch(GPUTileProcessor):
     def getTune(self):
         a, b = ctypes.c_int32(), ctypes.c_int32()
         self._chk(self.lib.clt_get_tune(self.h, ctypes.byref(a), ctypes.byref(b)))
-        return {"corr_tiles_per_block": a.value, "imclt_rows": b.value}
+        c = ctypes.c_int32()
+        self._chk(self.lib.clt_get_batch_chunks(self.h, ctypes.byref(c)))
+        return {"corr_tiles_per_block": a.value, "imclt_rows": b.value, "chunks": c.value}
+
+    def setBatchChunks(self, chunks):
+        """Frame chunks of the pipelined batch (clt_autotune picks one of 1, 2, 4, 8)."""
+        self._chk(self.lib.clt_set_batch_chunks(self.h, int(chunks)))
 
     def execBatch(self, stages=15):
         self._chk(self.lib.clt_exec_batch(self.h, stages))

@@ -447,10 +447,17 @@ int  clt_set_image_frame(clt_handle* h, int32_t frame, int32_t cam, const float*
 int  clt_set_tasks_frame(clt_handle* h, int32_t frame, const void* tasks, int32_t num_tasks);
 /* stages: 1 convert (no LPF), 2 correlate2D, 4 textures, 8 imclt_rbg with the corr_sigma LPF. */
 int  clt_exec_batch(clt_handle* h, int32_t stages);
+/* The batch runs as a pipeline over chunks of whole frames: each chunk's convert, textures and
+ * correlate2D run in order while the previous chunks' imclt_rbg and per-tile LMA run on their own
+ * streams.  Results do not depend on the chunk count. */
 /* Times the candidate launch shapes on the current batch and keeps the fastest: tiles per
- * workgroup of correlate2D (1, 2, 4) and tile rows per imclt segment (8, 16, 32). */
+ * workgroup of correlate2D (1, 2, 4), tile rows per imclt segment (8, 16, 32) and frame chunks of
+ * the pipeline (1, 2, 4, 8, at most num_frames). */
 int  clt_autotune(clt_handle* h);
 int  clt_get_tune(clt_handle* h, int32_t* corr_tiles_per_block, int32_t* imclt_rows);
+/* chunks 1..64 (used as min(chunks, num_frames)); 1 until clt_autotune or this call. */
+int  clt_set_batch_chunks(clt_handle* h, int32_t chunks);
+int  clt_get_batch_chunks(clt_handle* h, int32_t* chunks);
 /* Results of one frame, laid out as the per-frame getters (tasks in the frame's own order). */
 int  clt_get_corr2d_frame(clt_handle* h, int32_t frame, float* corr, float* combo, float* disparity);
 int  clt_get_textures_frame(clt_handle* h, int32_t frame, float* tiles, float* stats);

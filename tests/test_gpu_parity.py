@@ -317,6 +317,49 @@ def test_batch_matches_per_frame(pk, orc):
     bt.close()
 
 
+def test_batch_chunks_bit_identical(pk):
+    """The pipelined batch over 2, 3 and 5 frame chunks (8 asked, capped at the frame count) gives
+    bit-identical correlations, LMA records, textures and images to the one-chunk batch: chunks
+    touch disjoint frames and task records.  One frame has no tasks (a chunk with no task
+    launches); convert + imclt_rbg alone are checked the same way."""
+    F = 5
+    scenes = [pk.synth.make_scene(128, 96, seed=90 + f, disparity=("field", 0.0, 6.0)) for f in range(F)]
+    rng = np.random.default_rng(11)
+    keep = [rng.random(len(sc.tasks)) < frac for sc, frac in zip(scenes, (0.3, 0.0, 1.0, 0.6, 0.1))]
+    bt = pk.GPUTileBatch(F, 128, 96)
+    bt.setConvolutionKernels(scenes[0].kernels, offsets=scenes[0].offsets, force=True)
+    bt.setCorrSigma(0.8)
+    for f, sc in enumerate(scenes):
+        bt.setBayerImagesFrame(f, sc.images)
+        bt.setTasksFrame(f, sc.tasks[keep[f]])
+
+    def results(stages):
+        bt.execBatch(stages)
+        bt.execBatch(stages)   # graph replay
+        out = []
+        for f in range(F):
+            if stages & bt.CORR2D:
+                out += list(bt.getCorr2DFrame(f))
+            if stages & bt.TEXTURES:
+                out += list(bt.getTexturesFrame(f))
+            out += [bt.getRBGFrame(f, c) for c in range(4)]
+        return out
+
+    for stages in (15, bt.CONVERT | bt.IMCLT):
+        bt.setBatchChunks(1)
+        ref = results(stages)
+        for chunks in (2, 3, 8):
+            bt.setBatchChunks(chunks)
+            log("case: batch of %d frames, stages %d, %d chunks (used %d)"
+                % (F, stages, chunks, bt.getTune()["chunks"]))
+            got = results(stages)
+            assert len(got) == len(ref)
+            for i, (a, b) in enumerate(zip(got, ref)):
+                assert a.shape == b.shape and np.array_equal(a, b, equal_nan=True), (stages, chunks, i)
+    assert bt.autotune()["chunks"] in (1, 2, 4)
+    bt.close()
+
+
 def test_batch_follows_kernel_changes(pk):
     """A captured batch graph holds the kernel pointers of its capture: after setting or clearing
     the kernels the next clt_exec_batch must use the new ones.  Without kernels, then with them,
