@@ -30,7 +30,6 @@ constexpr int LMA_DPAIRS = 2;                // pairs per group of a one-directi
 constexpr int LMA_NFITS = 6;                 // all pairs, horizontal, vertical, diag main, diag other, one pair
 constexpr int LMA_FIT_SINGLE = 5;            // a single correlation as a horizontal pair (C2:1903-1935)
 constexpr int LMA_NMOM = 14;                 // sample moments of one group
-constexpr int LMA_ACC = 28 + LMA_MAXG * LMA_NMOM;   // LDS doubles per fit: accepted sums, then moments
 
 struct LmaGroup {
     int32_t npairs, diag;
@@ -193,13 +192,29 @@ struct LmaOut {
     double disp, strength, wm, wxy;
 };
 
-// Per-fit LDS of one tile: values of the box cells (group g at +81 g), compacted sample lists, the
-// sums of the accepted parameter vector (first 28 doubles of acc) and the sample moments of each
-// group (acc + 28 + LMA_NMOM g).
+// What the iterations of one fit need from its samples (written by lma_fit, read by lma_iterate):
+// the sample moments of each group about xo (the box centre on the group's disparity axis), the
+// group constants and the start.  The model is a paraboloid in the sample coordinates: with
+// u = x - X0, Y = y^2, alpha = ws^2 / Wx^2, beta = ws^2 / Wy^2 its shape is d = 1 - alpha u^2 -
+// beta Y and every derivative is a u, u^2 or Y term, so the weighted residual square sum and the
+// normal equations at any parameter vector are polynomials in the sums of sw x'^i (i <= 4),
+// sw x'^i Y (i <= 2), sw Y^2, sw v x'^i (i <= 2), sw v Y and sw v^2 with x' = x - xo: the same
+// sums as the per-sample passes of getFxJt / getWJtJlambda / getWYmFxRms
+// (Correlations2dLMA:368-461), reassociated.
+struct LmaState {
+    double mom[LMA_MAXG][LMA_NMOM];
+    double xo[LMA_MAXG], scale[LMA_MAXG], ws2[LMA_MAXG], gwn[LMA_MAXG];
+    double x0, vmax, wr0, wr1;
+    int32_t groups;   // bit g: group g has samples
+    int32_t run;      // 1: the fit has samples (lma_iterate runs it), 0: it fails
+};
+
+// Per-fit LDS of one tile: values of the box cells (group g at +81 g), compacted sample lists and
+// the fit's state for the iterations.
 struct LmaLds {
     double* v;
     uint16_t* idx;   // cell | weight class << 8 (1 convex, 2 other selected, 0 NaN sample)
-    double* acc;
+    LmaState* st;
 };
 
 // LDL^T of a symmetric n x n matrix (packed upper triangle a, constant indices only) and the
@@ -258,25 +273,20 @@ __device__ __forceinline__ double tri_norm(const double* a) {
     return norm;
 }
 
-// One Correlations2dLMA fit (C2:1754-1900) over the groups of *F, started at xcenter (pixels along
-// the disparity axis, = -disparity), on W lanes (fl = lane within the fit, fmask = the fit's lanes
-// in the wavefront).  NC core parameters: X0, Wm, Wxy (3) or X0, Wm, Wyd, Wxy (4, when Wyd is
-// adjusted); then one amplitude per group of at most NPM pairs.  The tile's correlations start at
-// corr[toff] (a wave-uniform base and 32-bit offsets keep one register per load address).
-// POLY: instead of the LMA, the 2-D quadratic maximum of the same samples (run_poly_instead,
-// getMaxXYPoly Correlations2dLMA:643-705); then disp = x, wm = y, strength = the value.
+// The samples of one Correlations2dLMA fit (C2:1754-1900) over the groups of *F, started at
+// xcenter (pixels along the disparity axis, = -disparity), on W lanes (fl = lane within the fit,
+// fmask = the fit's lanes in the wavefront): the box, its convex cells, the selection, the weights
+// and the sample moments, left in *S.st for lma_iterate (run = 0 when the fit has no samples).
+// Groups of at most NPM pairs.  The tile's correlations start at corr[toff] (a wave-uniform base
+// and 32-bit offsets keep one register per load address).  POLY: instead, the 2-D quadratic
+// maximum of the same samples (run_poly_instead, getMaxXYPoly Correlations2dLMA:643-705),
+// returned: disp = x, wm = y, strength = the value.
 template <int W, int NC, int NPM, bool POLY = false>
 __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, const float* __restrict__ corr, int toff,
                                           double xcenter, int fl, int lane, uint64_t fmask, const LmaLds& S) {
     constexpr int CPL = (LMA_CELLS + W - 1) / W;   // box cells per lane
-    constexpr int NP = NC + 2;
-    // sums: core triangle, per group {core x amplitude, amplitude^2, J^T r of the amplitude},
-    // J^T r of the core, weighted residual square sum
-    constexpr int I_CROSS = NC * (NC + 1) / 2, I_DIAG = I_CROSS + 2 * NC, I_BAG = I_DIAG + 2;
-    constexpr int I_BC = I_BAG + 2, I_RMS = I_BC + NC, NV = I_RMS + 1;
-    constexpr int P_WXY = NC - 1;
-    static_assert(NV <= 28, "LDS room for the accepted sums");
     LmaOut out = {false, 0.0, 0.0, 0.0, 0.0};
+    if (fl == 0) S.st->run = 0;
     const int G = F->ngroups;
     if (G == 0) return out;
     const int hwin = A.hwin;
@@ -425,35 +435,15 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
     }
     // groups are numbered in the order of their first sample (Correlations2dLMA:196-210)
     const bool ex0 = ns[0] > 0, ex1 = G > 1 && ns[1] > 0;
-    const int ng = (int)ex0 + (int)ex1;
-    if (ng == 0) return out;
+    if (!ex0 && !ex1) return out;
     sw_fit = fit_sum<W>(sw_fit);
     vmax = fit_max<W>(vmax);
     // setWeightsValues (Correlations2dLMA:245-278): unit total weight with the two regularisation
     // points (Wm -> half_width at cost_wm, Wxy -> 0 at cost_wxy)
     const double tot = sw_fit + A.cost_wm + A.cost_wxy;
     const double inv = (tot != 0.0) ? 1.0 / tot : 1.0;
-    const double wr0 = A.cost_wm * inv, wr1 = A.cost_wxy * inv;
-    double gwn[LMA_MAXG];
-#pragma unroll
-    for (int g = 0; g < LMA_MAXG; g++) gwn[g] = (sw_fit > 0.0) ? gw_raw[g] / sw_fit : gw_raw[g];
     wave_sync();
-    // parameters (initVector, Correlations2dLMA:212-243): X0, Wm, [Wyd,] Wxy, Ag of each group
-    bool msk[NP];
-    double p[NP], pn[NP];
-    msk[0] = true;
-    msk[1] = A.adj_wm != 0;
-    if constexpr (NC == 4) msk[2] = A.adj_wyd != 0;
-    msk[P_WXY] = A.adj_wxy != 0;
-    msk[NC] = A.adj_ag != 0;
-    msk[NC + 1] = A.adj_ag != 0 && ng > 1;
-#pragma unroll
-    for (int i = 0; i < NP; i++) p[i] = 0.0;
-    p[0] = xcenter;
-    p[1] = A.half_width;
-    p[NC] = p[NC + 1] = vmax;
     const double sqrt2 = 1.4142135623730951;
-    double acc[NV];
     double wc[LMA_MAXG][2];   // weight of a class-1 / class-2 sample before |v|^pwr, normalised
 #pragma unroll
     for (int g = 0; g < LMA_MAXG; g++) {
@@ -531,28 +521,21 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         out.strength = v;
         return out;
     }
-    // Sample moments of each group (once per fit).  The model is a paraboloid in the sample
-    // coordinates: with u = x - X0, Y = y^2, alpha = ws^2 / Wx^2, beta = ws^2 / Wy^2 the shape is
-    // d = 1 - alpha u^2 - beta Y and every derivative is a u, u^2 or Y term, so the weighted
-    // residual square sum and the normal equations at any parameter vector are polynomials in
-    // the sums of sw x'^i (i <= 4), sw x'^i Y (i <= 2), sw Y^2, sw v x'^i (i <= 2), sw v Y and
-    // sw v^2, x' = x - xo the coordinate from the box centre (the same sums as the per-sample
-    // passes of getFxJt / getWJtJlambda / getWYmFxRms, Correlations2dLMA:368-461, reassociated).
-    double* mom = S.acc + 28;
-    double xo[LMA_MAXG] = {0.0, 0.0};
+    // sample moments of each group about the box centre, and the fit's constants
+    LmaState* st = S.st;
 #pragma unroll
     for (int g = 0; g < LMA_MAXG; g++) {
         if (g >= G || ns[g] == 0) continue;
         const LmaGroup& grp = F->g[g];
         const int cx = x0s[g] - 7, cy = y0s[g] - 7;   // the box centre cell
-        xo[g] = (double)(grp.diag ? cx + cy : cx) * (grp.diag ? 0.5 * grp.scale : grp.scale);
+        const double xo = (double)(grp.diag ? cx + cy : cx) * (grp.diag ? 0.5 * grp.scale : grp.scale);
         double m[LMA_NMOM];
 #pragma unroll
         for (int k = 0; k < LMA_NMOM; k++) m[k] = 0.0;
         for (int j = fl; j < ns[g]; j += W) {
             double sv, sw, x, y;
             sample(g, j, sv, sw, x, y);
-            const double xp = x - xo[g], Y = y * y;
+            const double xp = x - xo, Y = y * y;
             const double w1 = sw * xp, w2 = w1 * xp, wy = sw * Y, wv = sw * sv, wv1 = wv * xp;
             m[0] += sw;        m[1] += w1;        m[2] += w2;        m[3] += w2 * xp;   m[4] += w2 * (xp * xp);
             m[5] += wy;        m[6] += wy * xp;   m[7] += wy * (xp * xp);               m[8] += wy * Y;
@@ -560,18 +543,66 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         }
 #pragma unroll
         for (int k = 0; k < LMA_NMOM; k++) m[k] = fit_sum<W>(m[k]);
-        if (fl == 0)
+        if (fl == 0) {
 #pragma unroll
-            for (int k = 0; k < LMA_NMOM; k++) mom[g * LMA_NMOM + k] = m[k];
+            for (int k = 0; k < LMA_NMOM; k++) st->mom[g][k] = m[k];
+            st->xo[g] = xo;
+            st->scale[g] = grp.scale;
+            const double ws = grp.diag ? sqrt2 : 1.0;
+            st->ws2[g] = ws * ws;
+        }
     }
-    wave_sync();
+    if (fl == 0) {
+#pragma unroll
+        for (int g = 0; g < LMA_MAXG; g++) st->gwn[g] = (sw_fit > 0.0) ? gw_raw[g] / sw_fit : gw_raw[g];
+        st->x0 = xcenter;
+        st->vmax = vmax;
+        st->wr0 = A.cost_wm * inv;
+        st->wr1 = A.cost_wxy * inv;
+        st->groups = (ex0 ? 1 : 0) | (ex1 ? 2 : 0);
+        st->run = 1;
+    }
+    return out;
+}
+
+// The Levenberg-Marquardt iterations of one fit from its state (lmaStep and runLma,
+// Correlations2dLMA:477-640; initVector :212-243; getDisparityStrength :165-185), all in this
+// lane.  NC core parameters: X0, Wm, Wxy (3) or X0, Wm, Wyd, Wxy (4, when Wyd is adjusted); then
+// one amplitude per group with samples.
+template <int NC>
+__device__ __forceinline__ LmaOut lma_iterate(const LmaArgs& A, const LmaState* __restrict__ st) {
+    constexpr int NP = NC + 2;
+    // sums: core triangle, per group {core x amplitude, amplitude^2, J^T r of the amplitude},
+    // J^T r of the core
+    constexpr int I_CROSS = NC * (NC + 1) / 2, I_DIAG = I_CROSS + 2 * NC, I_BAG = I_DIAG + 2;
+    constexpr int I_BC = I_BAG + 2, I_RMS = I_BC + NC;
+    constexpr int P_WXY = NC - 1;
+    LmaOut out = {false, 0.0, 0.0, 0.0, 0.0};
+    const int groups = st->groups;
+    const bool ex0 = groups & 1, ex1 = (groups >> 1) & 1;
+    const double wr0 = st->wr0, wr1 = st->wr1, vmax = st->vmax;
+    // parameters (initVector): X0, Wm, [Wyd,] Wxy, Ag of each group
+    bool msk[NP];
+    double p[NP], pn[NP];
+    msk[0] = true;
+    msk[1] = A.adj_wm != 0;
+    if constexpr (NC == 4) msk[2] = A.adj_wyd != 0;
+    msk[P_WXY] = A.adj_wxy != 0;
+    msk[NC] = A.adj_ag != 0;
+    msk[NC + 1] = A.adj_ag != 0 && ex0 && ex1;
+#pragma unroll
+    for (int i = 0; i < NP; i++) p[i] = 0.0;
+    p[0] = st->x0;
+    p[1] = A.half_width;
+    p[NC] = p[NC + 1] = vmax;
+    double acc[I_RMS];
     // the moments about X0 (binomial shift by X0 - xo)
     struct UMom {
         double s0, s1, s2, s3, s4, t0, t1, t2, q, v0, v1, v2, vy, vv;
     };
     auto umom = [&](int g, double x0) {
-        const double* m = mom + g * LMA_NMOM;
-        const double d = x0 - xo[g];
+        const double* m = st->mom[g];
+        const double d = x0 - st->xo[g];
         UMom u;
         u.s0 = m[0];
         u.s1 = m[1] - d * m[0];
@@ -589,21 +620,18 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         u.vv = m[13];
         return u;
     };
-    // `rms_at` the weighted residual square sum, `normal_at` the normal matrix and J^T W (y - f),
-    // both from the moments.  The LMA needs the normal equations only at accepted points it
-    // continues from.
+    // `rms_at` the weighted residual square sum, `normal_at` the normal matrix and J^T W (y - f).
+    // The LMA needs the normal equations only at accepted points it continues from.
     auto rms_at = [&](const double* q) {
         double s2 = 0.0;
 #pragma unroll
         for (int g = 0; g < LMA_MAXG; g++) {
-            if (g >= G || ns[g] == 0) continue;
+            if (!((groups >> g) & 1)) continue;
             const int gs = (g == 1 && ex0) ? 1 : 0;
-            const LmaGroup& grp = F->g[g];
-            const double ws = grp.diag ? sqrt2 : 1.0, ws2 = ws * ws;
-            const double Wy = q[1] * grp.scale + ((NC == 4) ? q[2] : 0.0), Wx = Wy + q[P_WXY];
+            const double Wy = q[1] * st->scale[g] + ((NC == 4) ? q[2] : 0.0), Wx = Wy + q[P_WXY];
             const double iWx = rcp64(Wx), iWy = rcp64(Wy);
             const double Ag = gs ? q[NC + 1] : q[NC];
-            const double a = ws2 * iWx * iWx, b = ws2 * iWy * iWy;
+            const double ws2 = st->ws2[g], a = ws2 * iWx * iWx, b = ws2 * iWy * iWy;
             const UMom u = umom(g, q[0]);
             const double sd2 = u.s0 - 2.0 * (a * u.s2 + b * u.t0) + a * (a * u.s4 + 2.0 * b * u.t2) + b * b * u.q;
             const double sdv = u.v0 - a * u.v2 - b * u.vy;
@@ -617,14 +645,13 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         for (int i = 0; i < I_RMS; i++) acc[i] = 0.0;
 #pragma unroll
         for (int g = 0; g < LMA_MAXG; g++) {
-            if (g >= G || ns[g] == 0) continue;
+            if (!((groups >> g) & 1)) continue;
             const int gs = (g == 1 && ex0) ? 1 : 0;   // amplitude slot of this group
-            const LmaGroup& grp = F->g[g];
-            const double ws = grp.diag ? sqrt2 : 1.0, ws2 = ws * ws, sc = grp.scale;
+            const double sc = st->scale[g];
             const double Wy = q[1] * sc + ((NC == 4) ? q[2] : 0.0), Wx = Wy + q[P_WXY];
             const double iWx = rcp64(Wx), iWy = rcp64(Wy);
             const double Ag = gs ? q[NC + 1] : q[NC];
-            const double a = ws2 * iWx * iWx, b = ws2 * iWy * iWy;
+            const double ws2 = st->ws2[g], a = ws2 * iWx * iWx, b = ws2 * iWy * iWy;
             const UMom u = umom(g, q[0]);
             // derivatives: X0 c0 u, Wm sc (cx u^2 + cy Y), Wyd cx u^2 + cy Y, Wxy cx u^2
             const double c0 = 2.0 * Ag * a, cx = c0 * iWx, cy = 2.0 * Ag * b * iWy;
@@ -674,17 +701,10 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         acc[I_BC + 1] += wr0 * r0;
         acc[I_BC + P_WXY] += wr1 * r1;
     };
-    auto keep = [&]() {   // the accepted point's normal equations, read by the next solve
-        if (fl == 0)
-#pragma unroll
-            for (int i = 0; i < I_RMS; i++) S.acc[i] = acc[i];
-        wave_sync();
-    };
 
     const double rms0 = sqrt(rms_at(p));
     double rms = rms0, lambda = A.lambda0;
     normal_at(p);
-    keep();
     bool good = false;
     for (int iter = 0; iter < A.num_iter; iter++) {
         // (J^T W J + lambda diag(J^T W J)) delta = J^T W (y - f) over the adjusted parameters
@@ -693,7 +713,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         bool singular = false;
 #pragma unroll
         for (int k = 0; k < NP; k++) {
-            const double v0 = (k < NC) ? S.acc[lma_tri(NC, k, k)] : S.acc[I_DIAG + k - NC];
+            const double v0 = (k < NC) ? acc[lma_tri(NC, k, k)] : acc[I_DIAG + k - NC];
             double dk = msk[k] ? v0 + v0 * lambda : 1.0;
 #pragma unroll
             for (int s = 0; s < k; s++) dk -= L[k][s] * L[k][s] * dd[s];
@@ -704,8 +724,8 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             for (int i = k + 1; i < NP; i++) {
                 double m = 0.0;
                 if (msk[i] && msk[k]) {
-                    if (i < NC) m = S.acc[lma_tri(NC, k, i)];
-                    else if (k < NC) m = S.acc[I_CROSS + (i - NC) * NC + k];
+                    if (i < NC) m = acc[lma_tri(NC, k, i)];
+                    else if (k < NC) m = acc[I_CROSS + (i - NC) * NC + k];
                 }
 #pragma unroll
                 for (int s = 0; s < k; s++) m -= L[i][s] * L[k][s] * dd[s];
@@ -715,7 +735,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         if (singular) break;   // lmaStep returns {false, true}
 #pragma unroll
         for (int i = 0; i < NP; i++) {
-            double m = msk[i] ? ((i < NC) ? S.acc[I_BC + i] : S.acc[I_BAG + i - NC]) : 0.0;
+            double m = msk[i] ? ((i < NC) ? acc[I_BC + i] : acc[I_BAG + i - NC]) : 0.0;
 #pragma unroll
             for (int s = 0; s < i; s++) m -= L[i][s] * x[s];
             x[i] = m;
@@ -737,10 +757,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             rms = rms_n;
 #pragma unroll
             for (int i = 0; i < NP; i++) p[i] = pn[i];
-            if (!done && iter + 1 < A.num_iter) {   // the next step solves from here
-                normal_at(p);
-                keep();
-            }
+            if (!done && iter + 1 < A.num_iter) normal_at(p);   // the next step solves from here
         } else {
             good = false;
         }
@@ -757,8 +774,8 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
     out.ok = good || (rms < rms0);
     if (out.ok) {   // getDisparityStrength (Correlations2dLMA:165-185)
         double sum_amp = 0.0;
-        if (ex0) sum_amp += gwn[0] * p[NC];
-        if (ex1) sum_amp += gwn[1] * (ex0 ? p[NC + 1] : p[NC]);
+        if (ex0) sum_amp += st->gwn[0] * p[NC];
+        if (ex1) sum_amp += st->gwn[1] * (ex0 ? p[NC + 1] : p[NC]);
         const double max_amp = fmax(vmax, 0.0);
         if (sum_amp > 1.25 * max_amp) sum_amp = max_amp;
         out.disp = -p[0];
@@ -818,59 +835,88 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
     const LmaArgs& A = *Ap;   // in device memory: the fits index it per lane
     __shared__ double s_v[LMA_SLOTS][2 * LMA_CELLS];
     __shared__ uint16_t s_idx[LMA_SLOTS][2 * LMA_CELLS];
-    __shared__ double s_acc[LMA_SLOTS][2 * LMA_ACC];
-    __shared__ double s_dds[LMA_SLOTS][4][4];
+    __shared__ LmaState s_st[4 * LMA_SLOTS];   // the fits over all pairs [slot], then the directions [slot][4]
+    __shared__ double s_res[4 * LMA_SLOTS][5];  // ok, disparity, strength, Wm, Wxy of each fit
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int sub = lane >> 4, l = lane & 15, slot = wave * LMA_TILES_PER_WAVE + sub;
     const int t = (xcd_block(blockIdx.x, gridDim.x) * LMA_WAVES + wave) * LMA_TILES_PER_WAVE + sub;
-    if (t >= num_tasks) return;   // whole rows: the reductions never see an exited lane
-    float* rec = disp + (size_t)t * CLT_DISP_LAYERS;
-    const float cm = rec[CLT_DISP_CM_RAW], st = rec[CLT_DISP_STR_RAW];
+    const bool valid = t < num_tasks;   // every thread reaches the block barriers below
+    float* rec = disp + (size_t)(valid ? t : 0) * CLT_DISP_LAYERS;
+    const float cm = valid ? rec[CLT_DISP_CM_RAW] : NAN, st = valid ? rec[CLT_DISP_STR_RAW] : 0.f;
     // the wavefront's first tile as a 64-bit base, the tile within it as a small 32-bit offset
     const float* corr_w = corr + (size_t)__builtin_amdgcn_readfirstlane(t - sub) * A.npairs * CLT_CORR_LEN;
     const int toff = sub * A.npairs * CLT_CORR_LEN;
     double out_cm = cm, out_st = st, poly = NAN, poly_s = NAN;
     double disparity = cm, strength = st;
-    bool dirs = false;
-    if (!isnan(cm) && (double)st > A.min_poly_strength) {   // the fit over all oriented pairs
-        const LmaLds S = {s_v[slot], s_idx[slot], s_acc[slot]};
-        const LmaOut r = lma_fit<16, NC, LMA_GPAIRS>(A, &A.fit[0], corr_w, toff, -(double)cm, l, lane,
-                                                     0xffffull << (16 * sub), S);
-        if (r.ok) {
-            poly = r.disp;
-            poly_s = r.strength;
-            if (A.mix) {
-                disparity = r.disp;
-                strength = r.strength;
-                out_cm = disparity;
-                out_st = strength;
-            }
-            // the foreground correction is kept only when strength > fo_min_strength (ID:2319-2322):
-            // below that the four fits cannot change the result
-            dirs = dir_out ? true : (A.fo_correct && strength > fmax(0.0, A.fo_min_strength));
+    // The samples of each fit are gathered on the tile's lanes (16 per tile, 8 per direction fit);
+    // the iterations need only the fit's state, so one wavefront runs them for every fit of the
+    // block, one lane per fit (one instruction stream instead of one per wavefront and pass).
+    auto iterate_all = [&](int nfits) {
+        if (threadIdx.x < nfits) {
+            const LmaState* S = &s_st[threadIdx.x];
+            LmaOut r = {false, 0.0, 0.0, 0.0, 0.0};
+            if (S->run) r = lma_iterate<NC>(A, S);
+            double* o = s_res[threadIdx.x];
+            o[0] = r.ok ? 1.0 : 0.0;
+            o[1] = r.disp;
+            o[2] = r.strength;
+            o[3] = r.wm;
+            o[4] = r.wxy;
         }
+    };
+    // the fit over all oriented pairs
+    const bool fit0 = valid && !isnan(cm) && (double)st > A.min_poly_strength;
+    if (fit0) {
+        const LmaLds S = {s_v[slot], s_idx[slot], &s_st[slot]};
+        lma_fit<16, NC, LMA_GPAIRS>(A, &A.fit[0], corr_w, toff, -(double)cm, l, lane, 0xffffull << (16 * sub), S);
+    } else if (l == 0) {
+        s_st[slot].run = 0;
     }
-    if (dirs) {   // corr4dirsLMA: two one-direction fits at a time, 8 lanes each
+    __syncthreads();
+    iterate_all(LMA_SLOTS);
+    __syncthreads();
+    bool dirs = false;
+    if (fit0 && s_res[slot][0] != 0.0) {
+        const double* r = s_res[slot];
+        poly = r[1];
+        poly_s = r[2];
+        if (A.mix) {
+            disparity = r[1];
+            strength = r[2];
+            out_cm = disparity;
+            out_st = strength;
+        }
+        // the foreground correction is kept only when strength > fo_min_strength (ID:2319-2322):
+        // below that the four fits cannot change the result
+        dirs = dir_out ? true : (A.fo_correct && strength > fmax(0.0, A.fo_min_strength));
+    }
+    __syncthreads();   // s_res is rewritten by the direction fits
+    // corr4dirsLMA: the samples of two one-direction fits at a time, 8 lanes each
+    if (dirs) {
         const int half = l >> 3, hl = l & 7;
-        const LmaLds S = {s_v[slot] + half * LMA_CELLS, s_idx[slot] + half * LMA_CELLS, s_acc[slot] + half * LMA_ACC};
         for (int pass = 0; pass < 2; pass++) {
             const int f = 1 + 2 * pass + half;
-            const LmaOut q = lma_fit<8, NC, LMA_DPAIRS>(A, &A.fit[f], corr_w, toff, -disparity, hl, lane,
-                                                        0xffull << (16 * sub + 8 * half), S);
-            if (hl == 0) {
-                s_dds[slot][f - 1][0] = q.ok ? q.disp : NAN;
-                s_dds[slot][f - 1][1] = q.strength;
-                s_dds[slot][f - 1][2] = q.wm;
-                s_dds[slot][f - 1][3] = q.wxy;
-            }
+            const LmaLds S = {s_v[slot] + half * LMA_CELLS, s_idx[slot] + half * LMA_CELLS, &s_st[4 * slot + f - 1]};
+            lma_fit<8, NC, LMA_DPAIRS>(A, &A.fit[f], corr_w, toff, -disparity, hl, lane,
+                                       0xffull << (16 * sub + 8 * half), S);
             wave_sync();
         }
+    } else if (l < 4) {
+        s_st[4 * slot + l].run = 0;
+    }
+    __syncthreads();
+    iterate_all(4 * LMA_SLOTS);
+    __syncthreads();
+    if (dirs) {
         bool have[4];
         double dv[4][4];
 #pragma unroll
         for (int d = 0; d < 4; d++) {
-#pragma unroll
-            for (int e = 0; e < 4; e++) dv[d][e] = s_dds[slot][d][e];
+            const double* r = s_res[4 * slot + d];
+            dv[d][0] = (r[0] != 0.0) ? r[1] : NAN;
+            dv[d][1] = r[2];
+            dv[d][2] = r[3];
+            dv[d][3] = r[4];
             have[d] = !isnan(dv[d][0]);
         }
         if (dir_out) {
@@ -884,14 +930,14 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
             const double mod = lma_foreground(A, have, dv, disparity);   // ID:2288-2336
             if (mod != disparity && strength > A.fo_min_strength) out_cm = mod;
         }
-    } else if (dir_out && l == 0) {
+    } else if (dir_out && valid && l == 0) {
 #pragma unroll
         for (int d = 0; d < 4; d++) {
             dir_out[(size_t)t * 8 + d] = NAN;
             dir_out[(size_t)t * 8 + 4 + d] = 0.f;
         }
     }
-    if (l == 0) {
+    if (valid && l == 0) {
         rec[CLT_DISP_CM] = (float)out_cm;
         rec[CLT_DISP_STRENGTH] = (float)out_st;
         rec[CLT_DISP_POLY] = (float)poly;
@@ -911,7 +957,7 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma_inter(const float* __restri
     const LmaArgs& A = *Ap;
     __shared__ double s_v[LMA_SLOTS][LMA_CELLS];
     __shared__ uint16_t s_idx[LMA_SLOTS][LMA_CELLS];
-    __shared__ double s_acc[LMA_SLOTS][LMA_ACC];
+    __shared__ LmaState s_st[LMA_SLOTS];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int sub = lane >> 4, l = lane & 15, slot = wave * LMA_TILES_PER_WAVE + sub;
     const int t = (xcd_block(blockIdx.x, gridDim.x) * LMA_WAVES + wave) * LMA_TILES_PER_WAVE + sub;
@@ -920,11 +966,13 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma_inter(const float* __restri
     double disparity = r[0], strength = r[1], dx = NAN, dy = NAN;
     const float x0 = r[2];
     if (!isnan(x0) && strength > A.min_poly_strength && !notch) {
-        const LmaLds S = {s_v[slot], s_idx[slot], s_acc[slot]};
+        const LmaLds S = {s_v[slot], s_idx[slot], &s_st[slot]};
         const uint64_t fmask = 0xffffull << (16 * sub);
         const float* corr_w = corr + (size_t)__builtin_amdgcn_readfirstlane(t - sub) * CLT_CORR_LEN;   // wave base
-        const LmaOut q = lma_fit<16, NC, 1>(A, &A.fit[LMA_FIT_SINGLE], corr_w, sub * CLT_CORR_LEN, (double)x0, l, lane,
-                                            fmask, S);
+        lma_fit<16, NC, 1>(A, &A.fit[LMA_FIT_SINGLE], corr_w, sub * CLT_CORR_LEN, (double)x0, l, lane, fmask, S);
+        wave_sync();
+        LmaOut q = {false, 0.0, 0.0, 0.0, 0.0};
+        if (s_st[slot].run) q = lma_iterate<NC>(A, &s_st[slot]);   // the same fit on every lane of the tile
         if (q.ok) {
             if (use_poly) {
                 disparity = q.disp;
