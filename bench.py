@@ -2,10 +2,11 @@
 
 Workload (one "step" = one frame through the plain per-frame GPU path): 4 x 2592x1936 Bayer
 (RGGB) float32 mosaics, 78,408 tiles, dense task list, per-tile disparity a smooth random field
-0..20 px, random per-tile aberration kernels; convert_direct -> textures (RGBA) -> correlate2D
-(6 pairs + combined strip + disparity, per-tile LMA chain) -> imclt_rbg (with LPF).  textures and
-imclt_rbg read only the converted tiles; the LMA chain (on its own stream) overlaps imclt_rbg (on a
-side stream) and the next convert.  Inputs are resident in HBM before the timed region; outputs stay on the device.
+0..20 px, random per-tile aberration kernels; convert_direct -> correlate2D (6 pairs + combined
+strip + disparity, per-tile LMA chain) -> textures (RGBA) and imclt_rbg (with LPF) in one pass over
+the converted tiles (clt_exec_textures_imclt).  The LMA chain (on its own stream) overlaps the
+textures/imclt pass and the next convert.  Inputs are resident in HBM before the timed region;
+outputs stay on the device.
 
 Prints one JSON line: tiles/s, HBM roofline of the dominant kernel (HIP events on the library's
 stream), the CPU baseline (the C restatement of the reference CPU path in FP64 and FP32 on the
@@ -49,6 +50,9 @@ def algorithmic_bytes(sc, planes, n_pairs, strip_h):
         "corr2d": clt + T * tw + T * n_pairs * 225 * 4 + T * strip_h * 15 * 4 + T * 12 * 4,
         "textures": clt + T * tw + T * planes * 1024 + T * 4 * C * 4,
         "imclt": clt + C * K * (W + 8) * (H + 8) * 4 + (W // 8) * (H // 8) * 4,
+        # textures and imclt_rbg in one pass: the CLT once, both outputs, the tile map
+        "tex_imclt": clt + T * tw + T * planes * 1024 + T * 4 * C * 4 + C * K * (W + 8) * (H + 8) * 4
+                     + (W // 8) * (H // 8) * 4,
     }
 
 
@@ -173,15 +177,19 @@ def copy_calibration(mib):
     print("bench: copied %d MiB device to device" % mib, file=sys.stderr, flush=True)
 
 
-KERNEL_NAMES = {"convert": "k_convert", "corr2d": "k_corr2d", "textures": "k_textures", "imclt": "k_imclt"}
+KERNEL_NAMES = {"convert": "k_convert", "corr2d": "k_corr2d", "textures": "k_textures", "imclt": "k_imclt",
+                "tex_imclt": "k_tex_imclt"}
+CHILD_FRAMES = 2   # frames of the profiled child run
 COPY_MIB = 256
+STEP_KERNELS = ("convert", "corr2d", "tex_imclt")   # the calls of one step (their event timings)
 
 
 def pmc_traffic(seed):
     """HBM bytes per launch of each kernel from two rocprofv3 counter runs (FETCH_SIZE, then
     WRITE_SIZE, KiB per dispatch) of a short child run of this script.  Each counter is scaled by
     what it reads for a device-to-device copy of COPY_MIB MiB in the same child run.  Returns
-    ({kernel: bytes}, {"fetch": factor, "write": factor}) or (None, reason)."""
+    ({kernel: bytes}, {"fetch": factor, "write": factor}) or (None, reason).  A call of the one-pass
+    textures/imclt is two dispatches (even, then odd tile columns): bytes are per call."""
     exe = shutil.which("rocprofv3")
     if not exe:
         return None, "rocprofv3 not found"
@@ -208,7 +216,7 @@ def pmc_traffic(seed):
             for key, kname in KERNEL_NAMES.items():
                 v = [x for k, xs in per.items() if kname in k for x in xs]
                 if v:
-                    vals.setdefault(key, {})[name] = sum(v) / len(v) * 1024.0
+                    vals.setdefault(key, {})[name] = sum(v) / CHILD_FRAMES * 1024.0
     except subprocess.TimeoutExpired:
         return None, "rocprofv3 timed out"
     finally:
@@ -224,7 +232,7 @@ def pmc_child(args):
     copy_calibration(COPY_MIB)
     sc = pk.synth.make_scene(W, H, seed=args.seed, disparity=("field", 0.0, 20.0), red_row=RED[0], red_col=RED[1])
     tp = make_processor(pk, sc)
-    for _ in range(2):
+    for _ in range(CHILD_FRAMES):
         run_step(tp)
     tp.sync()
     tp.close()
@@ -242,13 +250,13 @@ def make_processor(pk, sc):
 
 
 def run_step(tp):
-    """One frame.  textures (vector-ALU bound) runs before correlate2D, so the per-tile LMA chain
-    that correlate2D starts on its own stream overlaps imclt_rbg and the next frame's convert,
-    which wait on memory and leave the vector ALUs idle a third of the time."""
+    """One frame.  The per-tile LMA chain that correlate2D starts on its own stream overlaps the
+    one-pass textures/imclt_rbg and the next frame's convert (which waits on memory and leaves the
+    vector ALUs idle a third of the time); the one pass runs the inverse CLT of each tile once for
+    both outputs."""
     tp.execConverCorrectTiles(0)
-    tp.execTextures()
     tp.execCorrelate2D()
-    tp.execImcltRbg(True)
+    tp.execTexturesImclt(True)
 
 
 def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
@@ -277,7 +285,7 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
     for ft in frame_tasks:
         tp.setTasks(ft)
         run_step(tp)
-    plain_gpu = sum(tp.getTiming(k)[0] for k in ("convert", "corr2d", "corr_lma", "textures", "imclt"))
+    plain_gpu = sum(tp.getTiming(k)[0] for k in ("convert", "corr2d", "corr_lma", "tex_imclt"))
     tp.close()
     # batched
     bt = pk.GPUTileBatch(frames, W, H, C, K, red_row=RED[0], red_col=RED[1])
@@ -508,14 +516,14 @@ def main():
 
     ab = algorithmic_bytes(sc, planes=4, n_pairs=6, strip_h=9)
     kernels = {}
-    for name in ("convert", "corr2d", "textures", "imclt"):
+    for name in STEP_KERNELS:
         ms, cnt = tp.getTiming(name)
         avg = ms / max(cnt, 1)
         gbs = ab[name] / (avg * 1e-3) / 1e9
         kernels[name] = {"avg_ms": round(avg, 4), "launches": cnt, "alg_MB": round(ab[name] / 1e6, 1),
                          "GB_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
-    total_alg = sum(ab.values())
+    total_alg = sum(ab[k] for k in STEP_KERNELS)
     # the per-tile LMA chain (FP64 fits, data-dependent iteration counts): its time, not a byte rate
     ms, cnt = tp.getTiming("corr_lma")
     lma_ms = ms / max(cnt, 1)
@@ -525,7 +533,7 @@ def main():
             "frac": kernels[dom]["frac"], "traffic": None,
             "frame_alg_MB": round(total_alg / 1e6, 1),
             "frame_frac": round(total_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    # The timed steps run the LMA concurrently with imclt/textures/convert, which stretches their
+    # The timed steps run the LMA concurrently with textures/imclt and convert, which stretches their
     # event-timed durations; a few steps more with every call waiting for its kernels give each
     # kernel's own duration (not part of `value`).
     tp.setAsync(False)
@@ -533,7 +541,7 @@ def main():
     for _ in range(3):
         step()
     isolated = {}
-    for name in ("convert", "corr2d", "corr_lma", "textures", "imclt"):
+    for name in STEP_KERNELS + ("corr_lma",):
         ms, cnt = tp.getTiming(name)
         avg = ms / max(cnt, 1)
         isolated[name] = {"avg_ms": round(avg, 4)}
@@ -580,7 +588,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "quad 2592x1936 Bayer RGGB frame, 78408 tiles, dense tasks, disparity 0-20 px; "
-                               "convert+correlate2D(6 pairs, per-tile LMA chain)+textures(RGBA)+imclt_rbg",
+                               "convert+correlate2D(6 pairs, per-tile LMA chain)+textures(RGBA)+imclt_rbg "
+                               "(one pass)",
                    "tiles": T, "cameras": C, "colors": K, "pairs": 6},
         "roofline": roof, "kernels": kernels, "kernels_isolated": isolated, "cpu_baseline": cpu, "errors": errors,
         "batch": batch, "adjoint": adjoint,

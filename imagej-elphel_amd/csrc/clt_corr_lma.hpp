@@ -327,7 +327,8 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             in_grid |= ing ? (1u << k) : 0u;
             const int r0 = ing ? row : 0, c0 = ing ? col : 0;   // loads stay unconditional and in bounds
 #pragma unroll
-            for (int q = 0; q < NPM; q++) raw[k][q] = corr[base[q] + ra[q] * r0 + cb[q] * c0];
+            for (int q = 0; q < NPM; q++)   // |strides| <= 15, r0, c0 < 15: 24-bit products are exact
+                raw[k][q] = corr[base[q] + __mul24(ra[q], r0) + __mul24(cb[q], c0)];
         }
         __asm__ volatile("" ::: "memory");   // all loads in flight before the first use
 #pragma unroll
