@@ -213,13 +213,23 @@ struct clt_handle {
     clt_corr_lma_params lp{};      // per-tile LMA chain parameters
     LmaArgs lma_args{};            // ... as launched (built by prepare_corr)
     LmaArgs* d_lma_args = nullptr; // device copy read by k_corr_lma
+    LmaArgs lma_args_dev{};        // ... its contents (the upload is skipped when unchanged)
+    bool lma_args_valid = false;
     bool lma_run = false;          // correlate2D is followed by k_corr_lma
-    // k_corr_lma runs on its own stream, concurrently with the HBM-bound kernels that follow
-    // correlate2D (textures, imclt, the next convert); ev_lma joins it back before its outputs are
-    // read or correlate2D overwrites its inputs
+    // k_corr_lma runs on its own stream, concurrently with the kernels that follow correlate2D
+    // (textures, imclt, the next convert); ev_lma joins it back before its outputs are read or
+    // correlate2D overwrites its inputs
     hipStream_t stream_lma = nullptr;
     hipEvent_t ev_fork = nullptr, ev_lma = nullptr;
     bool lma_pending = false;
+    // a second pair of correlation records: clt_exec_corr2d called while the previous LMA may still
+    // read d_corr / d_disp writes the other pair (allocated on first use), so correlate2D does not
+    // wait for that LMA; ev_lma2 / lma2_pending belong to the LMA reading the other pair
+    float* d_corr2 = nullptr;
+    float* d_disp2 = nullptr;
+    size_t corr_cap2 = 0;
+    hipEvent_t ev_lma2 = nullptr;
+    bool lma2_pending = false;
     // imclt_rbg of a single frame runs on a side stream from the point of its call, so it overlaps
     // what the caller launches next (correlate2D reads the same CLT); ev_side joins it back before
     // the CLT, tile map, LPF or image planes are rewritten or the images are read
@@ -307,6 +317,11 @@ static void join_lma(clt_handle* h) {
     (void)hipStreamWaitEvent(h->stream, h->ev_lma, 0);
     h->lma_pending = false;
 }
+static void join_lma2(clt_handle* h) {
+    if (!h->lma2_pending) return;
+    (void)hipStreamWaitEvent(h->stream, h->ev_lma2, 0);
+    h->lma2_pending = false;
+}
 static void join_side(clt_handle* h) {
     if (!h->side_pending) return;
     (void)hipStreamWaitEvent(h->stream, h->ev_side, 0);
@@ -314,6 +329,7 @@ static void join_side(clt_handle* h) {
 }
 static hipError_t sync_streams(clt_handle* h) {
     join_lma(h);
+    join_lma2(h);
     join_side(h);
     return hipStreamSynchronize(h->stream);
 }
@@ -583,6 +599,7 @@ static int create_handle(const clt_geom* geom, int frames, clt_handle** out) {
     if (hipStreamCreateWithFlags(&h->stream_lma, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_lma, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_lma2, hipEventDisableTiming) != hipSuccess ||
         hipStreamCreateWithFlags(&h->stream_side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_side_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev_side, hipEventDisableTiming) != hipSuccess) {
@@ -637,7 +654,7 @@ int clt_destroy(clt_handle* h) {
                     h->d_adj, h->d_tstate, h->d_gpart, h->d_gsum, h->d_gdisp2, h->d_lma_args, h->d_rig_corr,
                     h->d_rig_disp, h->d_rig_mix, h->d_rig_dirs, h->d_inter, h->d_inter_avg, h->d_inter_rec,
                     h->d_inter_res, h->d_bimap, h->d_rig_target, h->d_rig_args, h->d_lma_rig, h->d_rig_extra,
-                    h->d_rig_tex, h->d_rig_tex_stats, h->d_rig_tex_args};
+                    h->d_rig_tex, h->d_rig_tex_stats, h->d_rig_tex_args, h->d_corr2, h->d_disp2};
     if (h->stream) (void)sync_streams(h);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -654,6 +671,7 @@ int clt_destroy(clt_handle* h) {
     if (h->stream_side) (void)hipStreamDestroy(h->stream_side);
     if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
     if (h->ev_lma) (void)hipEventDestroy(h->ev_lma);
+    if (h->ev_lma2) (void)hipEventDestroy(h->ev_lma2);
     delete h;
     return CLT_OK;
 }
@@ -1300,8 +1318,12 @@ static int build_lma_args(clt_handle* h, bool have_records) {
     if ((rc = fill_lma_args(h, h->pairs, h->lp, a))) return rc;
     h->lma_run = a.fit[0].ngroups > 0;
     if ((rc = dalloc(h, (void**)&h->d_lma_args, sizeof(LmaArgs)))) return rc;
-    join_lma(h);   // a running LMA still reads the previous arguments
+    if (h->lma_args_valid && std::memcmp(&a, &h->lma_args_dev, sizeof(a)) == 0) return CLT_OK;   // both zero-filled
+    join_lma(h);    // a running LMA still reads the previous arguments
+    join_lma2(h);
     HIPCHK(h, hipMemcpyAsync(h->d_lma_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    std::memcpy(&h->lma_args_dev, &a, sizeof(a));
+    h->lma_args_valid = true;
     return CLT_OK;
 }
 
@@ -1388,11 +1410,25 @@ static int prepare_corr(clt_handle* h) {
     return CLT_OK;
 }
 
+// When the last LMA may still read the correlation records, the next correlate2D writes the other
+// pair (it waits only for the LMA before that one, which read the other pair).
+static void swap_corr_records(clt_handle* h) {
+    if (!h->lma_pending) return;
+    join_lma2(h);
+    std::swap(h->d_corr, h->d_corr2);
+    std::swap(h->corr_cap, h->corr_cap2);
+    std::swap(h->d_disp, h->d_disp2);
+    std::swap(h->ev_lma, h->ev_lma2);
+    h->lma2_pending = true;
+    h->lma_pending = false;
+}
+
 int clt_exec_corr2d(clt_handle* h) {
     if (!h) return CLT_E_ARG;
     if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_corr2d: a batch handle runs clt_exec_batch");
     if (!h->converted) return fail(h, CLT_E_STATE, "clt_exec_corr2d: no converted tiles");
     int rc;
+    swap_corr_records(h);
     if ((rc = prepare_corr(h))) return rc;
     tbegin(h, T_CORR);
     launch_corr(h, h->num_tasks, nullptr);

@@ -176,6 +176,35 @@ def test_c1_async_imclt_then_convert(pk, c1_scene):
     assert same_img and same_clt
 
 
+@pytest.mark.timeout(600)
+def test_c1_async_corr2d_while_lma_runs(pk, c1_scene):
+    """At this size the per-tile LMA runs about 0.5 ms on its own stream: correlate2D issued again
+    before it ends (async mode) writes the second pair of correlation records, and a third call
+    goes back to the first pair after waiting for the LMA that read it.  Every result must be
+    bit-identical to the synchronous calls."""
+    sc = c1_scene
+    log("case: async correlate2D x2 / x3 at 2592x1936 with the LMA of the previous call running")
+
+    def run(asynchronous, lpf_masks):
+        tp = gpu_processor(pk, sc)
+        tp.setCorrSigma(0.8)
+        tp.setLpfRbg(0.7, 0.7, 0.7)
+        tp.setAsync(asynchronous)
+        for m in lpf_masks:
+            tp.execConverCorrectTiles(m)
+            tp.execCorrelate2D()
+        out = tp.getCorr2D()
+        tp.close()
+        return out
+
+    for masks in ((0, 7), (0, 7, 0)):
+        ref = run(False, masks[-1:])
+        got = run(True, masks)
+        same = [np.array_equal(a.view(np.int32), b.view(np.int32)) for a, b in zip(got, ref)]
+        log("  %d calls: bit-identical corr/combo/disparity = %s" % (len(masks), same))
+        assert all(same)
+
+
 @pytest.mark.timeout(300)
 def test_c2_mono16_full_size(pk, orc):
     log("case: 16 x 640x512 mono, 30% tasks, 120 pairs: convert + correlate2D + imclt vs FP64 oracle")
