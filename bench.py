@@ -325,6 +325,77 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
             "plain_gpu_ms": round(plain_gpu, 3), "batched_gpu_ms_by_chunks": by_chunks, "tune": tune}
 
 
+def configs_bench(pk, steps=50):
+    """The other single-frame configurations, timed outside `value`: c0, the quad 512x512 Bayer
+    RGGB frame (dense tasks, disparity 3 px: convert + correlate2D with the LMA chain + the one-pass
+    textures/imclt_rbg), and c2, 16 monochrome 640x512 cameras on a circle (30 % of the tiles,
+    all 120 pairs: convert + correlate2D + imclt_rbg); each beside the FP64 oracle of the same step
+    on the host cores (one warm-up, then the median of 3 passes)."""
+    import oracle as orc
+    cores = host_cores()
+    out = {}
+    cases = (("c0", "quad 512x512 Bayer RGGB, dense, disparity 3 px",
+              dict(width=512, height=512, seed=7, disparity=3.0, red_row=0, red_col=0), None),
+             ("c2", "16 mono 640x512, 30 % tasks, 120 pairs",
+              dict(width=640, height=512, num_cams=16, num_colors=1, seed=20261206, disparity=("field", 0.0, 6.0),
+                   task_fraction=0.3), [(i, j, -1, 1) for i in range(16) for j in range(i + 1, 16)]))
+    for key, name, kw, pairs in cases:
+        sc = pk.synth.make_scene(**kw)
+        bayer = sc.num_colors == 3
+        tp = pk.GPUTileProcessor(sc.width, sc.height, sc.num_cams, sc.num_colors, red_row=sc.red_row,
+                                 red_col=sc.red_col)
+        tp.setConvolutionKernels(sc.kernels, offsets=sc.offsets, force=True)
+        tp.setBayerImages(sc.images, force=True)
+        tp.setTasks(sc.tasks)
+        tp.setCorrSigma(0.8)
+        if pairs:
+            tp.setCorrPairs(pairs)
+        if bayer:
+            tp.setTexParams(keep_weights=0)
+        tp.setAsync(True)
+
+        def step():
+            tp.execConverCorrectTiles(0)
+            tp.execCorrelate2D()
+            if bayer:
+                tp.execTexturesImclt(True)
+            else:
+                tp.execImcltRbg(True)
+
+        for _ in range(5):
+            step()
+        tp.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        tp.sync()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        tp.close()
+        T = len(sc.tasks)
+        g = orc.make_geom(sc.width, sc.height, sc.num_cams, sc.num_colors, sc.red_row, sc.red_col)
+        valid = np.zeros((sc.height // 8) * (sc.width // 8), np.uint8)
+        valid[((sc.tasks[:, 1] >> 16) * (sc.width // 8) + (sc.tasks[:, 1] & 0xFFFF))] = 1
+
+        def cpu_step():
+            clt = orc.convert_frame(g, sc.images, sc.tasks, kernels=sc.kernels, offsets=sc.offsets, nthreads=cores)
+            orc.corr_frame(g, clt, sc.tasks, pairs=pairs, nthreads=cores)
+            if bayer:
+                orc.textures_frame(g, clt, sc.tasks, tex_params=orc.default_tex_params(keep_weights=0),
+                                   nthreads=cores)
+            orc.imclt_frame(g, clt, valid, lpf64=orc.lpf(0.8), nthreads=cores)
+
+        cpu_step()
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            cpu_step()
+            times.append(time.perf_counter() - t0)
+        cpu_s = float(np.median(times))
+        out[key] = {"workload": name, "tasks": T, "ms_per_step": round(ms, 4), "tiles_per_s": round(T / (ms * 1e-3), 1),
+                    "cpu_port_fp64_tiles_per_s": round(T / cpu_s, 1), "cpu_cores": cores}
+    return out
+
+
 def adjoint_bench(pk, sc, repeats=5, seed=11):
     """Differentiable form on the same frame: forward (convert + correlate2D) and the adjoint
     (port centres from the geometry, the fused convert/loss adjoint, the geometry adjoint and the
@@ -468,6 +539,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/LMA timing")
     ap.add_argument("--no-kprep", action="store_true", help="skip the kernel preparation timing")
     ap.add_argument("--no-rig", action="store_true", help="skip the two-quad rig timing")
+    ap.add_argument("--no-configs", action="store_true", help="skip the c0 / c2 configuration timings")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -563,6 +635,10 @@ def main():
     if not args.no_kprep:
         print("bench: kernel preparation ...", file=sys.stderr, flush=True)
         kprep = kernel_prep_bench(pk)
+    configs = None
+    if not args.no_configs:
+        print("bench: c0 and c2 configurations ...", file=sys.stderr, flush=True)
+        configs = configs_bench(pk)
     rig = None
     if not args.no_rig:
         print("bench: two-quad rig ...", file=sys.stderr, flush=True)
@@ -593,7 +669,7 @@ def main():
                    "tiles": T, "cameras": C, "colors": K, "pairs": 6},
         "roofline": roof, "kernels": kernels, "kernels_isolated": isolated, "cpu_baseline": cpu, "errors": errors,
         "batch": batch, "adjoint": adjoint,
-        "kernel_prep": kprep, "rig": rig,
+        "kernel_prep": kprep, "rig": rig, "configs": configs,
     }
     print(json.dumps(out))
 

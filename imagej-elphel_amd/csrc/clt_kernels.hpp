@@ -1670,8 +1670,6 @@ __global__ __launch_bounds__(256) void k_sum_rows(const double* __restrict__ a, 
     if (threadIdx.x == 0) out[c] = s_part[0];
 }
 
-// Disparity step of the LMA after the extrinsic step de: dd_t = -(g_d + h . de) / hd, limited to
-// max_step pixels (a tile whose correlations are flat along its disparity has a tiny hd).
 // Correlation-peak offsets of the 6 quad pairs of each task (the measurement of the reference's
 // extrinsic path: clt_mismatch, ID:2342-2420, by mismatchPairsCM, C2:1336-1430): per pair the
 // integer maximum of its 15x15 correlation and the centre of mass of the positive values within
@@ -1735,6 +1733,8 @@ __global__ __launch_bounds__(256) void k_corr_offsets(const float* __restrict__ 
     for (int k = 27; k < CORR_ADJ_STRIDE; k++) a[k] = 0.f;
 }
 
+// Disparity step of the LMA after the extrinsic step de: dd_t = -(g_d + h . de) / hd, limited to
+// max_step pixels (a tile whose correlations are flat along its disparity has a tiny hd).
 __global__ __launch_bounds__(256) void k_disparity_step(const double* __restrict__ tstate, const double* __restrict__ de,
                                                         const float* __restrict__ d_in, float* __restrict__ d_out, int num_tasks,
                                                         double max_step) {
@@ -2169,7 +2169,8 @@ __global__ __launch_bounds__(256) void k_tex_imclt(const float* __restrict__ clt
                                                    int task_words, const int32_t* __restrict__ tile_map,
                                                    const float* __restrict__ lpf, const TexArgs* __restrict__ args_p,
                                                    float* __restrict__ tiles_out, float* __restrict__ stats_out,
-                                                   float* __restrict__ rbg, int seg_rows, int parity, GeomDev g) {
+                                                   float* __restrict__ rbg, int seg_rows, int parity, GeomDev g,
+                                                   long long clt_fstride) {
     static_assert(NC * 3 <= 16, "one pass of units");
     __shared__ __attribute__((aligned(16))) TexShared<NC> S;
     const TexArgs& A = *args_p;
@@ -2180,6 +2181,10 @@ __global__ __launch_bounds__(256) void k_tex_imclt(const float* __restrict__ clt
     const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
     const int out_w = g.width + 8, out_h = g.height + 8;
     const size_t plane_sz = (size_t)out_w * out_h;
+    // frame blockIdx.z of a batch: its CLT, tile map (of flat task indices) and image planes
+    clt += blockIdx.z * clt_fstride;
+    tile_map += blockIdx.z * ntiles;
+    rbg += blockIdx.z * (NC * 3) * plane_sz;
     const int l = threadIdx.x & 15;
     // right half stored, not added: the last column when no even column follows it
     const bool add_left = parity == 1;

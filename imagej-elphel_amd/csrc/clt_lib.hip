@@ -198,6 +198,8 @@ struct clt_handle {
     int32_t* d_fcounts = nullptr;                  // [frames] tasks per frame, then [frames] first task
     std::vector<int> frame_count;                  // tasks of each frame
     std::vector<int> frame_start;                  // first task of each frame in the flat list, + total
+    std::vector<char> frame_fusable;               // every task of the frame has its own tile and non-zero flags
+    bool batch_fused = false;                      // the batch runs textures + imclt_rbg as one pass
     int graph_tasks = -1;                          // task count the graph was captured for
     bool batch_dirty = true;
     hipGraph_t graph = nullptr;
@@ -441,14 +443,25 @@ static void launch_tex_nc(clt_handle* h, int n, const int32_t* task_frame, const
                        task_frame ? task_frame + t0 : nullptr, h->clt_fstride, h->gd);
 }
 
+// Frames [f0, f0 + frames) (one frame: f0 = 0, frames = 1): both column launches.
 template <int NC>
-static void launch_tex_imclt_nc(clt_handle* h) {
+static void launch_tex_imclt_nc(clt_handle* h, int f0 = 0, int frames = 1) {
     const int rows = h->tex_imclt_rows;
-    const dim3 grid((h->tiles_x + 1) / 2, (h->tiles_y + 1 + rows - 1) / rows);
+    const dim3 grid((h->tiles_x + 1) / 2, (h->tiles_y + 1 + rows - 1) / rows, frames);
+    const size_t plane_px = (size_t)(h->g.width + 8) * (h->g.height + 8);
     for (int parity = 0; parity < 2; parity++)   // even columns store, odd columns add
-        hipLaunchKernelGGL(k_tex_imclt<NC>, grid, dim3(256), 0, h->stream, h->d_clt, h->d_tasks, h->task_words,
-                           h->d_tile_map, h->d_lpf_corr, h->d_tex_args, h->d_tex, h->d_tex_stats, h->d_rbg, rows, parity,
-                           h->gd);
+        hipLaunchKernelGGL(k_tex_imclt<NC>, grid, dim3(256), 0, h->stream, h->d_clt + (size_t)f0 * h->clt_fstride,
+                           h->d_tasks, h->task_words, h->d_tile_map + (size_t)f0 * h->num_tiles, h->d_lpf_corr,
+                           h->d_tex_args, h->d_tex, h->d_tex_stats, h->d_rbg + (size_t)f0 * NC * 3 * plane_px, rows,
+                           parity, h->gd, h->clt_fstride);
+}
+static void launch_tex_imclt(clt_handle* h, int f0 = 0, int frames = 1) {
+    switch (h->g.num_cams) {
+    case 2: launch_tex_imclt_nc<2>(h, f0, frames); break;
+    case 3: launch_tex_imclt_nc<3>(h, f0, frames); break;
+    case 4: launch_tex_imclt_nc<4>(h, f0, frames); break;
+    default: launch_tex_imclt_nc<5>(h, f0, frames); break;
+    }
 }
 
 static void launch_tex(clt_handle* h, int n, const int32_t* task_frame, const float* extra, int t0 = 0) {
@@ -575,6 +588,7 @@ static int create_handle(const clt_geom* geom, int frames, clt_handle** out) {
     h->clt_fstride = (long long)g.num_cams * h->num_tiles * g.num_colors * 256;
     h->frame_count.assign(frames, 0);
     h->frame_start.assign(frames + 1, 0);
+    h->frame_fusable.assign(frames, 1);
     h->gd = GeomDev{g.width, g.height, g.num_cams, g.num_colors, g.red_row, g.red_col,
                     g.kernels_hor, g.kernels_vert, g.kernel_step, h->tiles_x, h->tiles_y};
     clt_default_corr_params(&h->cp);
@@ -1737,12 +1751,7 @@ int clt_exec_textures_imclt(clt_handle* h, const float* extra_disparity, int32_t
         return rc;
     join_side(h);   // an imclt_rbg still writing the planes
     tbegin(h, T_TEX_IMCLT);
-    switch (g.num_cams) {
-    case 2: launch_tex_imclt_nc<2>(h); break;
-    case 3: launch_tex_imclt_nc<3>(h); break;
-    case 4: launch_tex_imclt_nc<4>(h); break;
-    default: launch_tex_imclt_nc<5>(h); break;
-    }
+    launch_tex_imclt(h);
     tend(h, T_TEX_IMCLT);
     HIPCHK(h, hipGetLastError());
     h->tex_done = true;
@@ -2198,13 +2207,19 @@ int clt_set_tasks_frame(clt_handle* h, int32_t frame, const void* tasks, int32_t
     if (frame < 0 || frame >= h->num_frames || num_tasks < 0 || num_tasks > h->max_tasks || (num_tasks > 0 && !tasks))
         return fail(h, CLT_E_ARG, "clt_set_tasks_frame: frame %d, %d tasks (capacity %d)", frame, num_tasks, h->max_tasks);
     const int32_t* t = (const int32_t*)tasks;
+    std::vector<char> seen((size_t)h->num_tiles, 0);
+    bool fusable = true;
     for (int i = 0; i < num_tasks; i++) {
         const int32_t* r = t + (size_t)i * h->task_words;
         const int tx = r[1] & 0xffff, ty = (r[1] >> 16) & 0xffff;
         if (tx >= h->tiles_x || ty >= h->tiles_y)
             return fail(h, CLT_E_ARG, "clt_set_tasks_frame: task %d tile (%d,%d) outside %dx%d", i, tx, ty, h->tiles_x, h->tiles_y);
+        char& m = seen[(size_t)ty * h->tiles_x + tx];
+        fusable = fusable && r[0] != 0 && !m;
+        m = 1;
     }
     HIPCHK(h, sync_streams(h));   // the staging area may still be read
+    h->frame_fusable[frame] = fusable;
     if (num_tasks > 0)
         HIPCHK(h, hipMemcpy(h->d_staged + (size_t)frame * h->max_tasks * h->task_words, tasks,
                             (size_t)num_tasks * h->task_words * 4, hipMemcpyHostToDevice));
@@ -2247,27 +2262,33 @@ enum { STAGE_CONVERT = 1, STAGE_CORR = 2, STAGE_TEX = 4, STAGE_IMCLT = 8 };
 // The batch as a software pipeline over tune_chunks runs of whole frames.  Each chunk runs convert,
 // textures and correlate2D on the main stream and forks its imclt_rbg (after convert) and its
 // per-tile LMA (after correlate2D) onto their own streams, where they overlap the next chunk's
-// kernels.  Chunks touch disjoint frames of the CLT, tile maps and image planes and disjoint task
-// records, so the only joins are the fork points and the end of the batch.
+// kernels; when every frame's tasks own distinct tiles with non-zero flags, textures and
+// imclt_rbg of the chunk run as one pass (k_tex_imclt over the chunk's frames) instead.  Chunks
+// touch disjoint frames of the CLT, tile maps and image planes and disjoint task records, so the
+// only joins are the fork points and the end of the batch.
 static void launch_batch(clt_handle* h, int stages) {
     const int32_t* tf = h->num_frames > 1 ? h->d_task_frame : nullptr;
     const int F = h->num_frames;
     const int chunks = std::max(1, std::min(h->tune_chunks, F));
+    const bool fused = h->batch_fused && (stages & STAGE_TEX) && (stages & STAGE_IMCLT);
     join_side(h);   // a running imclt_rbg still reads the CLT
     join_lma(h);    // ... and an LMA the correlation records
+    join_lma2(h);
     for (int c = 0; c < chunks; c++) {
         const int f0 = (int)((long)F * c / chunks), f1 = (int)((long)F * (c + 1) / chunks);
         const int t0 = F > 1 ? h->frame_start[f0] : 0;
         const int n = (F > 1 ? h->frame_start[f1] : h->num_tasks) - t0;
         if ((stages & STAGE_CONVERT) && n > 0) launch_convert_range(h, t0, n, tf, 0);
-        if (stages & STAGE_IMCLT) {
+        if (fused) {   // textures and imclt_rbg of the chunk's frames in one pass
+            launch_tex_imclt(h, f0, f1 - f0);
+        } else if (stages & STAGE_IMCLT) {
             (void)hipEventRecord(h->ev_side_fork, h->stream);
             (void)hipStreamWaitEvent(h->stream_side, h->ev_side_fork, 0);
             launch_imclt(h, f1 - f0, 1, h->stream_side, f0);
             (void)hipEventRecord(h->ev_side, h->stream_side);
             h->side_pending = true;
         }
-        if ((stages & STAGE_TEX) && n > 0) launch_tex(h, n, tf, nullptr, t0);
+        if (!fused && (stages & STAGE_TEX) && n > 0) launch_tex(h, n, tf, nullptr, t0);
         if ((stages & STAGE_CORR) && n > 0) {
             launch_corr_range(h, t0, n, tf);
             launch_corr_lma(h, n, false, t0);
@@ -2286,6 +2307,10 @@ static int prepare_batch(clt_handle* h, int stages) {
     if ((stages & STAGE_CORR) && (rc = prepare_corr(h))) return rc;
     if ((stages & STAGE_TEX) && (rc = prepare_tex(h))) return rc;
     const clt_geom& g = h->g;
+    bool fused = g.num_colors == 3 && g.num_cams >= 2 && g.num_cams <= 5;
+    for (int f = 0; f < h->num_frames; f++) fused = fused && h->frame_fusable[f];
+    if (fused != h->batch_fused) h->graph_epoch++;   // the captured launches differ
+    h->batch_fused = fused;
     if ((stages & STAGE_IMCLT) &&
         (rc = dalloc(h, (void**)&h->d_rbg, (size_t)h->num_frames * g.num_cams * g.num_colors * (g.width + 8) * (g.height + 8) * 4)))
         return rc;

@@ -360,6 +360,40 @@ def test_batch_chunks_bit_identical(pk):
     bt.close()
 
 
+def test_batch_frame_with_idle_task(pk):
+    """A frame whose task list holds a task with flags 0 cannot take the one-pass textures/imclt
+    (its tile has no texture work), so the batch runs the two kernels for every frame; both
+    frames still match the per-frame calls (NaN tile for the idle task)."""
+    scenes = [pk.synth.make_scene(128, 96, seed=70 + f, disparity=("field", 0.0, 4.0)) for f in range(2)]
+    tasks = [sc.tasks.copy() for sc in scenes]
+    tasks[1][5, 0] = 0
+    bt = pk.GPUTileBatch(2, 128, 96)
+    bt.setConvolutionKernels(scenes[0].kernels, offsets=scenes[0].offsets, force=True)
+    bt.setCorrSigma(0.8)
+    for f, sc in enumerate(scenes):
+        bt.setBayerImagesFrame(f, sc.images)
+        bt.setTasksFrame(f, tasks[f])
+    bt.execBatch()
+    for f, sc in enumerate(scenes):
+        tp = pk.GPUTileProcessor(128, 96)
+        tp.setConvolutionKernels(scenes[0].kernels, offsets=scenes[0].offsets, force=True)
+        tp.setBayerImages(sc.images, force=True)
+        tp.setTasks(tasks[f])
+        tp.setCorrSigma(0.8)
+        tp.execConverCorrectTiles()
+        tp.execTextures()
+        tp.execImcltRbg(True)
+        t1, s1 = tp.getTextures()
+        t2, s2 = bt.getTexturesFrame(f)
+        log("case: batch frame %d (idle task: %s)" % (f, f == 1))
+        assert np.array_equal(np.isnan(t1), np.isnan(t2))
+        assert np.array_equal(t1, t2, equal_nan=True) and np.array_equal(s1, s2, equal_nan=True)
+        for c in range(4):
+            assert np.abs(bt.getRBGFrame(f, c) - tp.getRBG(c)).max() <= 1e-6 * max(1.0, np.abs(tp.getRBG(c)).max())
+        tp.close()
+    bt.close()
+
+
 def test_batch_follows_kernel_changes(pk):
     """A captured batch graph holds the kernel pointers of its capture: after setting or clearing
     the kernels the next clt_exec_batch must use the new ones.  Without kernels, then with them,
