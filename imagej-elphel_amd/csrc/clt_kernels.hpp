@@ -1752,6 +1752,10 @@ __global__ __launch_bounds__(256) void k_disparity_step(const double* __restrict
 // tile_debayer_shot_corr 3022-3147, tile_combine_rgba 2830-3000).  Block (256 threads) = tile,
 // thread = pixel of the 16x16 tile in the pixel stages.  NC = number of cameras.
 // ====================================================================================
+// LDS texture tiles: rows TEX_ROW floats apart (see TexShared)
+constexpr int TEX_ROW = 20, TEX_TILE = 16 * TEX_ROW;
+static_assert(TEX_TILE <= IMCLT_SCR_UNIT, "a debayered tile fits the unit's IMCLT scratch");
+
 struct TexArgs {
     float col_w[3];
     float min_shot, scale_shot, ksigma, corr_magic_scale;
@@ -1766,7 +1770,7 @@ struct TexArgs {
 // final mix ImageDtt.java:2789-2801).  ALL: every port of the NC is used (no per-port masking).
 template <int NC, bool ALL>
 __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, const float (&deb)[NC][3], float w2,
-                                              float iw2, const float* s_nw, float* out, int i, int planes,
+                                              float iw2, const float* s_nw, int pi, float* out, int i, int planes,
                                               float (&avg)[3]) {
     auto on = [&](int ip) { return ALL || ((port_mask >> ip) & 1); };
     int used = 0;
@@ -1902,7 +1906,7 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
     for (int chn = 0; chn < 3; chn++) {
         float s = 0.f;
 #pragma unroll
-        for (int ip = 0; ip < NC; ip++) s += pw[ip] * s_nw[(ip * 3 + chn) * 256 + i];
+        for (int ip = 0; ip < NC; ip++) s += pw[ip] * s_nw[(ip * 3 + chn) * TEX_TILE + pi];
         out[chn * 256 + i] = sw * s;
     }
     if (planes > 4) {   // keep_weights: weights, then the rms of the ports (ImageDtt.java:2858-2874)
@@ -1930,14 +1934,17 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
 
 // LDS of one texture tile.  scr: the IMCLT scratch of each 16-lane unit; afterwards, when every
 // unit fits one pass (NU <= 16), unit u's debayered tile.  nw: the window-free tiles (the IMCLT
-// tiles divided by the squared window), pixel-major [unit][16 * row + col].
+// tiles divided by the squared window).  Tiles are [unit][TEX_ROW * row + col]: lane l of a unit
+// stores row l with 16-byte stores, and rows TEX_ROW = 20 floats apart put the rows of 8
+// consecutive lanes on distinct banks (16 apart would put every even lane on the same 4 banks);
+// TEX_ROW and TEX_TILE are defined with TexArgs.
 template <int NC>
 struct TexShared {
     static constexpr int NU = NC * 3;
     static constexpr bool DEB_IN_SCR = NU <= 16;
     float scr[16][IMCLT_SCR_UNIT];
-    float nw[NU][256];
-    float deb[DEB_IN_SCR ? 1 : NU][256];
+    float nw[NU][TEX_TILE];
+    float deb[DEB_IN_SCR ? 1 : NU][TEX_TILE];
     __device__ __forceinline__ float* deb_of(int u) { return DEB_IN_SCR ? &scr[u][0] : &deb[u][0]; }
 };
 
@@ -2017,7 +2024,8 @@ __device__ __forceinline__ void tex_units(const float* __restrict__ clt, size_t 
         if (act) {
 #pragma unroll
             for (int c = 0; c < 16; c += 4)
-                *reinterpret_cast<float4*>(&S.nw[uu][l * 16 + c]) = make_float4(nw[c], nw[c + 1], nw[c + 2], nw[c + 3]);
+                *reinterpret_cast<float4*>(&S.nw[uu][l * TEX_ROW + c]) =
+                    make_float4(nw[c], nw[c + 1], nw[c + 2], nw[c + 3]);
         }
         const bool green = color == 2;
         const float f_in = green ? (row_edge ? 8.f / 7.f : 1.f) : (row_edge ? 4.f / 3.f : 1.f);
@@ -2043,7 +2051,7 @@ __device__ __forceinline__ void tex_units(const float* __restrict__ clt, size_t 
             float* d = S.deb_of(uu);
 #pragma unroll
             for (int c = 0; c < 16; c += 4)
-                *reinterpret_cast<float4*>(&d[l * 16 + c]) = make_float4(deb[c], deb[c + 1], deb[c + 2], deb[c + 3]);
+                *reinterpret_cast<float4*>(&d[l * TEX_ROW + c]) = make_float4(deb[c], deb[c + 1], deb[c + 2], deb[c + 3]);
         }
     }
 }
@@ -2056,15 +2064,16 @@ __device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, Tex
     const int i = threadIdx.x;   // pixel index in the 16x16 tile
     const int planes = A.planes;
     const int py = i >> 4, px = i & 15;
+    const int pi = py * TEX_ROW + px;   // ... in the LDS tiles
     const int qy = py < 8 ? py : 15 - py, qx = px < 8 ? px : 15 - px;
     const float w2 = c_tab.hw2[qy] * c_tab.hw2[qx];
     const float iw2 = c_tab.ihw2[qy] * c_tab.ihw2[qx];
     float deb[NC][3];
 #pragma unroll
-    for (int uu = 0; uu < NC * 3; uu++) deb[uu / 3][uu % 3] = S.deb_of(uu)[i];
+    for (int uu = 0; uu < NC * 3; uu++) deb[uu / 3][uu % 3] = S.deb_of(uu)[pi];
     float avg[3];
-    if (port_mask == (1 << NC) - 1) combine_pixel<NC, true>(A, port_mask, deb, w2, iw2, &S.nw[0][0], out, i, planes, avg);
-    else combine_pixel<NC, false>(A, port_mask, deb, w2, iw2, &S.nw[0][0], out, i, planes, avg);
+    if (port_mask == (1 << NC) - 1) combine_pixel<NC, true>(A, port_mask, deb, w2, iw2, &S.nw[0][0], pi, out, i, planes, avg);
+    else combine_pixel<NC, false>(A, port_mask, deb, w2, iw2, &S.nw[0][0], pi, out, i, planes, avg);
     if (stats_t) {   // max_diff (ImageDtt.java:2965-2981) and ports_rgb (ImageDtt.java:2982-2996)
         // transpose through LDS: value k of pixel i at [k][i] (the NC max_diff rows reuse scr, the
         // 3*NC colour rows reuse nw); 16 threads then reduce each row
@@ -2087,16 +2096,17 @@ __device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, Tex
             for (int chn = 0; chn < 3; chn++) red_rgb[(chn * NC + ip) * 256 + i] = deb[ip][chn];
         }
         __syncthreads();
-        // 4*NC values x 256 pixels: 16 threads per value, 16 pixels each, then a 16-lane reduction
+        // 4*NC values x 256 pixels: 16 threads per value, each the 16 pixels j, j + 16, ... (the
+        // 16 threads read consecutive words), then a 16-lane reduction
         for (int kb = 0; kb < 4 * NC; kb += 16) {
             const int k = kb + (i >> 4), j = i & 15;
             float v = 0.f;
             if (k < 4 * NC) {
                 const bool is_max = k < NC;
-                const float* r = (is_max ? red_max + k * 256 : red_rgb + (k - NC) * 256) + j * 16;
+                const float* r = (is_max ? red_max + k * 256 : red_rgb + (k - NC) * 256) + j;
                 v = r[0];
 #pragma unroll
-                for (int q = 1; q < 16; q++) v = is_max ? fmaxf(v, r[q]) : v + r[q];
+                for (int q = 1; q < 16; q++) v = is_max ? fmaxf(v, r[16 * q]) : v + r[16 * q];
 #pragma unroll
                 for (int o = 8; o > 0; o >>= 1) {
                     const float u = __shfl_xor(v, o, 16);
