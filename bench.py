@@ -574,7 +574,7 @@ def main():
     for _ in range(args.warmup):
         step()
     tp.sync()
-    tp.setTiming(True)
+    tp.setTiming(True, reserve=args.steps)   # the events exist before the timed region
     tp.resetTiming()
     tp.sync()
     t0 = time.perf_counter()

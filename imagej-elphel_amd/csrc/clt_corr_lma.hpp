@@ -6,16 +6,15 @@
 // C2:2230-2338; the fit Correlations2dLMA.java), four one-direction fits (corr4dirsLMA,
 // C2:1591-1630) and the foreground correction (foregroundCorrect, C2:1663-1750).
 //
-// Mapping: 16 lanes (one DPP row) per tile, 4 tiles per wavefront.  The fit over all pairs uses
-// the tile's 16 lanes; the four one-direction fits run two at a time on 8 lanes each.  A fit's
-// candidate samples are the 9x9 box around its start point: the lanes own box cells fl, fl+W, ...,
-// the combined correlation values sit in LDS as doubles, the convexity flags are gathered into an
-// 81-bit mask with ballots and the selection grows by shifts of that mask (no memory round trips),
-// and the selected cells are compacted into a per-group sample list, so an evaluation of the
-// model touches only the samples (typically 15-30 per group of the 81 cells).  Every
-// normal-equation sum is an all-reduce over the fit's lanes (DPP on the two halves of each
-// double; all lanes end with bit-identical sums, so the fit's decisions are uniform).  Arithmetic
-// is FP64 like the reference; the damped normal system is solved by LDL^T in registers.
+// Mapping.  Samples: 16 lanes (one DPP row) per tile, 4 tiles per wavefront; the fit over all
+// pairs uses the tile's 16 lanes, the four one-direction fits run two at a time on 8 lanes each.
+// A fit's candidate samples are the 9x9 box around its start point: the lanes own box cells fl,
+// fl+W, ..., the combined correlation values sit in LDS as doubles, the convexity flags are
+// gathered into an 81-bit mask with ballots and the selection grows by shifts of that mask (no
+// memory round trips), the selected cells are compacted into a per-group sample list, and the
+// fit's samples are reduced once to the moments its model needs (LmaState).  Iterations: one
+// thread per fit, from the state alone.  Arithmetic is FP64 like the reference; the damped normal
+// system is solved by LDL^T in registers.
 #pragma once
 
 namespace clt {
@@ -318,6 +317,7 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
         }
         float raw[CPL][NPM];
         uint32_t in_grid = 0;   // bit k: cell fl + W k lies on the 15 x 15 correlation
+        int r0s[CPL], c0s[CPL];
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
             const int c = fl + W * k;
@@ -325,11 +325,26 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
             const int row = y0 - 4 + br, col = x0 - 4 + bc;
             const bool ing = c < LMA_CELLS && row >= 0 && row < 15 && col >= 0 && col < 15;
             in_grid |= ing ? (1u << k) : 0u;
-            const int r0 = ing ? row : 0, c0 = ing ? col : 0;   // loads stay unconditional and in bounds
-#pragma unroll
-            for (int q = 0; q < NPM; q++)   // |strides| <= 15, r0, c0 < 15: 24-bit products are exact
-                raw[k][q] = corr[base[q] + __mul24(ra[q], r0) + __mul24(cb[q], c0)];
+            r0s[k] = ing ? row : 0;   // loads stay unconditional and in bounds
+            c0s[k] = ing ? col : 0;
         }
+        // the group's pairs only (a wave-uniform count); each load a 32-bit byte offset from the
+        // wave-uniform base (|strides| <= 15, r0, c0 < 15: 24-bit products are exact)
+        const char* cbase = reinterpret_cast<const char*>(corr);
+        auto load_box = [&](auto nq) {
+            constexpr int NQ = decltype(nq)::value;
+#pragma unroll
+            for (int k = 0; k < CPL; k++)
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const uint32_t off = (uint32_t)(base[q] + __mul24(ra[q], r0s[k]) + __mul24(cb[q], c0s[k])) * 4u;
+                    raw[k][q] = *reinterpret_cast<const float*>(cbase + off);
+                }
+        };
+        if (npairs >= NPM) load_box(std::integral_constant<int, NPM>{});
+        else if (npairs == 1) load_box(std::integral_constant<int, 1>{});
+        else if (npairs == 2 && NPM > 2) load_box(std::integral_constant<int, (NPM > 2 ? 2 : NPM)>{});
+        else load_box(std::integral_constant<int, NPM>{});   // (loads of pairs past npairs are discarded)
         __asm__ volatile("" ::: "memory");   // all loads in flight before the first use
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
@@ -570,8 +585,13 @@ __device__ __forceinline__ LmaOut lma_fit(const LmaArgs& A, const LmaFit* F, con
 // Correlations2dLMA:477-640; initVector :212-243; getDisparityStrength :165-185), all in this
 // lane.  NC core parameters: X0, Wm, Wxy (3) or X0, Wm, Wyd, Wxy (4, when Wyd is adjusted); then
 // one amplitude per group with samples.
-template <int NC>
-__device__ __forceinline__ LmaOut lma_iterate(const LmaArgs& A, const LmaState* __restrict__ st) {
+template <int NC, bool IN_REGS = true>
+__device__ __forceinline__ LmaOut lma_iterate(const LmaArgs& A, const LmaState* __restrict__ st_in) {
+    // IN_REGS: the state (in device memory) copied to registers once for the whole fit; else read
+    // where it is (LDS)
+    LmaState stv;
+    if constexpr (IN_REGS) stv = *st_in;
+    const LmaState* st = IN_REGS ? &stv : st_in;
     constexpr int NP = NC + 2;
     // sums: core triangle, per group {core x amplitude, amplitude^2, J^T r of the amplitude},
     // J^T r of the core
@@ -823,62 +843,84 @@ __device__ double lma_foreground(const LmaArgs& A, const bool* have, const doubl
     return disp;
 }
 
-// ID:2254-2336 over the task list: reads the raw centre of mass and strength of each record,
-// writes CM, STRENGTH (final), POLY, POLY_STR.  16 lanes per tile, 4 tiles per wavefront.
-// NC = 4 when the extra vertical width Wyd is adjusted, else 3.
-// dir_out (the rig's tileCorrs, ID:7581-7740): when not null, the four one-direction fits run on
-// every tile whose fit over all pairs succeeded and [T][8] receives their disparities (NaN when a
-// fit fails or does not run) and strengths (0 then); no foreground correction.
+// ID:2254-2336 over the task list, as a chain of five kernels on one stream: the samples of each
+// tile's fit over all pairs (k_lma_setup_all), its iterations (k_lma_iterate), the samples of the
+// four one-direction fits (k_lma_setup_dirs), their iterations, and the foreground correction
+// (k_lma_finish).  Every fit's state goes through device memory (LmaState, 328 bytes), so the
+// set-up kernels keep the tile's lanes busy on the per-cell work at their own occupancy and the
+// iterations run one thread per fit, without a wavefront of a workgroup waiting for another.
+// The records are read at CLT_DISP_CM_RAW / CLT_DISP_STR_RAW and written at CM, STRENGTH (final),
+// POLY, POLY_STR.  NC = 4 when the extra vertical width Wyd is adjusted, else 3.  dir_out (the
+// rig's tileCorrs, ID:7581-7740): when not null, the four one-direction fits run on every tile
+// whose fit over all pairs succeeded and [T][8] receives their disparities (NaN when a fit fails or
+// does not run) and strengths (0 then); no foreground correction.
+
+// k_lma_setup_all: 16 lanes per tile, 4 tiles per wavefront; state of tile t at st[t].
 template <int NC>
-__global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ corr, float* __restrict__ disp,
-                                                  int num_tasks, const LmaArgs* __restrict__ Ap,
-                                                  float* __restrict__ dir_out) {
+__global__ __launch_bounds__(256, 4) void k_lma_setup_all(const float* __restrict__ corr, const float* __restrict__ disp,
+                                                       int num_tasks, const LmaArgs* __restrict__ Ap,
+                                                       LmaState* __restrict__ st) {
     const LmaArgs& A = *Ap;   // in device memory: the fits index it per lane
     __shared__ double s_v[LMA_SLOTS][2 * LMA_CELLS];
     __shared__ uint16_t s_idx[LMA_SLOTS][2 * LMA_CELLS];
-    __shared__ LmaState s_st[4 * LMA_SLOTS];   // the fits over all pairs [slot], then the directions [slot][4]
-    __shared__ double s_res[4 * LMA_SLOTS][5];  // ok, disparity, strength, Wm, Wxy of each fit
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int sub = lane >> 4, l = lane & 15, slot = wave * LMA_TILES_PER_WAVE + sub;
     const int t = (xcd_block(blockIdx.x, gridDim.x) * LMA_WAVES + wave) * LMA_TILES_PER_WAVE + sub;
-    const bool valid = t < num_tasks;   // every thread reaches the block barriers below
-    float* rec = disp + (size_t)(valid ? t : 0) * CLT_DISP_LAYERS;
-    const float cm = valid ? rec[CLT_DISP_CM_RAW] : NAN, st = valid ? rec[CLT_DISP_STR_RAW] : 0.f;
+    if (t >= num_tasks) return;   // whole rows: the reductions never see an exited lane
+    const float* rec = disp + (size_t)t * CLT_DISP_LAYERS;
+    const float cm = rec[CLT_DISP_CM_RAW], str = rec[CLT_DISP_STR_RAW];
     // the wavefront's first tile as a 64-bit base, the tile within it as a small 32-bit offset
     const float* corr_w = corr + (size_t)__builtin_amdgcn_readfirstlane(t - sub) * A.npairs * CLT_CORR_LEN;
     const int toff = sub * A.npairs * CLT_CORR_LEN;
-    double out_cm = cm, out_st = st, poly = NAN, poly_s = NAN;
-    double disparity = cm, strength = st;
-    // The samples of each fit are gathered on the tile's lanes (16 per tile, 8 per direction fit);
-    // the iterations need only the fit's state, so one wavefront runs them for every fit of the
-    // block, one lane per fit (one instruction stream instead of one per wavefront and pass).
-    auto iterate_all = [&](int nfits) {
-        if (threadIdx.x < nfits) {
-            const LmaState* S = &s_st[threadIdx.x];
-            LmaOut r = {false, 0.0, 0.0, 0.0, 0.0};
-            if (S->run) r = lma_iterate<NC>(A, S);
-            double* o = s_res[threadIdx.x];
-            o[0] = r.ok ? 1.0 : 0.0;
-            o[1] = r.disp;
-            o[2] = r.strength;
-            o[3] = r.wm;
-            o[4] = r.wxy;
-        }
-    };
-    // the fit over all oriented pairs
-    const bool fit0 = valid && !isnan(cm) && (double)st > A.min_poly_strength;
-    if (fit0) {
-        const LmaLds S = {s_v[slot], s_idx[slot], &s_st[slot]};
+    if (!isnan(cm) && (double)str > A.min_poly_strength) {
+        const LmaLds S = {s_v[slot], s_idx[slot], &st[t]};
         lma_fit<16, NC, LMA_GPAIRS>(A, &A.fit[0], corr_w, toff, -(double)cm, l, lane, 0xffffull << (16 * sub), S);
     } else if (l == 0) {
-        s_st[slot].run = 0;
+        st[t].run = 0;
     }
-    __syncthreads();
-    iterate_all(LMA_SLOTS);
-    __syncthreads();
+}
+
+// k_lma_iterate: one thread per fit; res[i] = {ok, disparity, strength, Wm, Wxy}.
+template <int NC>
+__global__ __launch_bounds__(256) void k_lma_iterate(const LmaState* __restrict__ st, int num_fits,
+                                                     const LmaArgs* __restrict__ Ap, double* __restrict__ res) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_fits) return;
+    LmaOut r = {false, 0.0, 0.0, 0.0, 0.0};
+    if (st[i].run) r = lma_iterate<NC>(*Ap, &st[i]);
+    double* o = res + (size_t)i * 5;
+    o[0] = r.ok ? 1.0 : 0.0;
+    o[1] = r.disp;
+    o[2] = r.strength;
+    o[3] = r.wm;
+    o[4] = r.wxy;
+}
+
+// k_lma_setup_dirs: the result of the fit over all pairs (res_all[t]) mixed into the tile's values
+// (ID:2273-2281); when the foreground correction can apply, the samples of the four one-direction
+// fits (corr4dirsLMA, C2:1591-1630), two at a time on 8 lanes each, states at st[4 t + direction];
+// tmp[t] = {disparity, strength, CM out, strength out, POLY, POLY strength, directions run}.
+template <int NC>
+__global__ __launch_bounds__(256, 3) void k_lma_setup_dirs(const float* __restrict__ corr, const float* __restrict__ disp,
+                                                        int num_tasks, const LmaArgs* __restrict__ Ap,
+                                                        const double* __restrict__ res_all, LmaState* __restrict__ st,
+                                                        double* __restrict__ tmp, int all_dirs) {
+    const LmaArgs& A = *Ap;
+    __shared__ double s_v[LMA_SLOTS][2 * LMA_CELLS];
+    __shared__ uint16_t s_idx[LMA_SLOTS][2 * LMA_CELLS];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int sub = lane >> 4, l = lane & 15, slot = wave * LMA_TILES_PER_WAVE + sub;
+    const int t = (xcd_block(blockIdx.x, gridDim.x) * LMA_WAVES + wave) * LMA_TILES_PER_WAVE + sub;
+    if (t >= num_tasks) return;   // whole rows
+    const float* rec = disp + (size_t)t * CLT_DISP_LAYERS;
+    const float cm = rec[CLT_DISP_CM_RAW], str = rec[CLT_DISP_STR_RAW];
+    const float* corr_w = corr + (size_t)__builtin_amdgcn_readfirstlane(t - sub) * A.npairs * CLT_CORR_LEN;
+    const int toff = sub * A.npairs * CLT_CORR_LEN;
+    double out_cm = cm, out_st = str, poly = NAN, poly_s = NAN;
+    double disparity = cm, strength = str;
     bool dirs = false;
-    if (fit0 && s_res[slot][0] != 0.0) {
-        const double* r = s_res[slot];
+    const double* r = res_all + (size_t)t * 5;
+    if (!isnan(cm) && (double)str > A.min_poly_strength && r[0] != 0.0) {
         poly = r[1];
         poly_s = r[2];
         if (A.mix) {
@@ -889,31 +931,49 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
         }
         // the foreground correction is kept only when strength > fo_min_strength (ID:2319-2322):
         // below that the four fits cannot change the result
-        dirs = dir_out ? true : (A.fo_correct && strength > fmax(0.0, A.fo_min_strength));
+        dirs = all_dirs ? true : (A.fo_correct && strength > fmax(0.0, A.fo_min_strength));
     }
-    __syncthreads();   // s_res is rewritten by the direction fits
-    // corr4dirsLMA: the samples of two one-direction fits at a time, 8 lanes each
     if (dirs) {
         const int half = l >> 3, hl = l & 7;
         for (int pass = 0; pass < 2; pass++) {
             const int f = 1 + 2 * pass + half;
-            const LmaLds S = {s_v[slot] + half * LMA_CELLS, s_idx[slot] + half * LMA_CELLS, &s_st[4 * slot + f - 1]};
+            const LmaLds S = {s_v[slot] + half * LMA_CELLS, s_idx[slot] + half * LMA_CELLS, &st[4 * (size_t)t + f - 1]};
             lma_fit<8, NC, LMA_DPAIRS>(A, &A.fit[f], corr_w, toff, -disparity, hl, lane,
                                        0xffull << (16 * sub + 8 * half), S);
             wave_sync();
         }
     } else if (l < 4) {
-        s_st[4 * slot + l].run = 0;
+        st[4 * (size_t)t + l].run = 0;
     }
-    __syncthreads();
-    iterate_all(4 * LMA_SLOTS);
-    __syncthreads();
-    if (dirs) {
+    if (l == 0) {
+        double* o = tmp + (size_t)t * 8;
+        o[0] = disparity;
+        o[1] = strength;
+        o[2] = out_cm;
+        o[3] = out_st;
+        o[4] = poly;
+        o[5] = poly_s;
+        o[6] = dirs ? 1.0 : 0.0;
+    }
+}
+
+// k_lma_finish: one thread per tile: the foreground correction (ID:2288-2336) from the four fits'
+// results res_dirs[4 t + direction] (or their disparities and strengths to dir_out), then the record.
+__global__ __launch_bounds__(256) void k_lma_finish(float* __restrict__ disp, int num_tasks,
+                                                    const LmaArgs* __restrict__ Ap, const double* __restrict__ tmp,
+                                                    const double* __restrict__ res_dirs, float* __restrict__ dir_out) {
+    const LmaArgs& A = *Ap;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tasks) return;
+    const double* o = tmp + (size_t)t * 8;
+    const double disparity = o[0], strength = o[1];
+    double out_cm = o[2];
+    if (o[6] != 0.0) {
         bool have[4];
         double dv[4][4];
 #pragma unroll
         for (int d = 0; d < 4; d++) {
-            const double* r = s_res[4 * slot + d];
+            const double* r = res_dirs + (4 * (size_t)t + d) * 5;
             dv[d][0] = (r[0] != 0.0) ? r[1] : NAN;
             dv[d][1] = r[2];
             dv[d][2] = r[3];
@@ -921,29 +981,27 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma(const float* __restrict__ c
             have[d] = !isnan(dv[d][0]);
         }
         if (dir_out) {
-            if (l == 0)
 #pragma unroll
-                for (int d = 0; d < 4; d++) {
-                    dir_out[(size_t)t * 8 + d] = (float)dv[d][0];
-                    dir_out[(size_t)t * 8 + 4 + d] = have[d] ? (float)dv[d][1] : 0.f;
-                }
+            for (int d = 0; d < 4; d++) {
+                dir_out[(size_t)t * 8 + d] = (float)dv[d][0];
+                dir_out[(size_t)t * 8 + 4 + d] = have[d] ? (float)dv[d][1] : 0.f;
+            }
         } else {
-            const double mod = lma_foreground(A, have, dv, disparity);   // ID:2288-2336
+            const double mod = lma_foreground(A, have, dv, disparity);
             if (mod != disparity && strength > A.fo_min_strength) out_cm = mod;
         }
-    } else if (dir_out && valid && l == 0) {
+    } else if (dir_out) {
 #pragma unroll
         for (int d = 0; d < 4; d++) {
             dir_out[(size_t)t * 8 + d] = NAN;
             dir_out[(size_t)t * 8 + 4 + d] = 0.f;
         }
     }
-    if (valid && l == 0) {
-        rec[CLT_DISP_CM] = (float)out_cm;
-        rec[CLT_DISP_STRENGTH] = (float)out_st;
-        rec[CLT_DISP_POLY] = (float)poly;
-        rec[CLT_DISP_POLY_STR] = (float)poly_s;
-    }
+    float* rec = disp + (size_t)t * CLT_DISP_LAYERS;
+    rec[CLT_DISP_CM] = (float)out_cm;
+    rec[CLT_DISP_STRENGTH] = (float)o[3];
+    rec[CLT_DISP_POLY] = (float)o[4];
+    rec[CLT_DISP_POLY_STR] = (float)o[5];
 }
 
 // tileInterCamCorrs (ID:7370-7560) after the centre of mass: the LMA of the inter-camera
@@ -973,7 +1031,7 @@ __global__ __launch_bounds__(256, 3) void k_corr_lma_inter(const float* __restri
         lma_fit<16, NC, 1>(A, &A.fit[LMA_FIT_SINGLE], corr_w, sub * CLT_CORR_LEN, (double)x0, l, lane, fmask, S);
         wave_sync();
         LmaOut q = {false, 0.0, 0.0, 0.0, 0.0};
-        if (s_st[slot].run) q = lma_iterate<NC>(A, &s_st[slot]);   // the same fit on every lane of the tile
+        if (s_st[slot].run) q = lma_iterate<NC, false>(A, &s_st[slot]);   // the same fit on every lane of the tile
         if (q.ok) {
             if (use_poly) {
                 disparity = q.disp;

@@ -216,6 +216,13 @@ struct clt_handle {
     LmaArgs lma_args{};            // ... as launched (built by prepare_corr)
     LmaArgs* d_lma_args = nullptr; // device copy read by k_corr_lma
     LmaArgs lma_args_dev{};        // ... its contents (the upload is skipped when unchanged)
+    // LMA chain scratch (fit states, fit results, per-tile values), one set per stream that runs
+    // a chain: 0 the LMA stream, 1 the side stream (the rig's auxiliary quad)
+    struct LmaScratch {
+        LmaState* st = nullptr;    // [4 cap_tasks]
+        double* res = nullptr;     // [4 cap_tasks][5]
+        double* tmp = nullptr;     // [cap_tasks][8]
+    } lma_scr[2];
     bool lma_args_valid = false;
     bool lma_run = false;          // correlate2D is followed by k_corr_lma
     // k_corr_lma runs on its own stream, concurrently with the kernels that follow correlate2D
@@ -419,16 +426,45 @@ static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
 // Per-tile LMA chain over the records of tasks [t0, t0 + n) just written by correlate2D (16 tiles
 // per 256-thread block), forked onto its own stream; the caller joins it (join_lma, or the end of
 // a captured batch).
+// The per-tile LMA chain (five kernels, clt_corr_lma.hpp) over n tasks on stream st with scratch
+// set `which` (allocated by ensure_lma_scratch before; nothing here allocates, so graph capture
+// works).
+template <int NC>
+static void lma_chain_nc(clt_handle* h, hipStream_t st, int which, const float* corr, float* disp, int n,
+                         const LmaArgs* args, float* dir_out) {
+    const clt_handle::LmaScratch& s = h->lma_scr[which];
+    const dim3 g_tiles((n + LMA_SLOTS - 1) / LMA_SLOTS), g_one((n + 255) / 256), g_four((4 * n + 255) / 256);
+    hipLaunchKernelGGL(k_lma_setup_all<NC>, g_tiles, dim3(64 * LMA_WAVES), 0, st, corr, disp, n, args, s.st);
+    hipLaunchKernelGGL(k_lma_iterate<NC>, g_one, dim3(256), 0, st, s.st, n, args, s.res);
+    hipLaunchKernelGGL(k_lma_setup_dirs<NC>, g_tiles, dim3(64 * LMA_WAVES), 0, st, corr, disp, n, args, s.res, s.st,
+                       s.tmp, dir_out ? 1 : 0);
+    hipLaunchKernelGGL(k_lma_iterate<NC>, g_four, dim3(256), 0, st, s.st, 4 * n, args, s.res);
+    hipLaunchKernelGGL(k_lma_finish, g_one, dim3(256), 0, st, disp, n, args, s.tmp, s.res, dir_out);
+}
+static void launch_lma_chain(clt_handle* h, hipStream_t st, int which, int nc, const float* corr, float* disp, int n,
+                             const LmaArgs* args, float* dir_out) {
+    if (n <= 0) return;
+    if (nc == 4) lma_chain_nc<4>(h, st, which, corr, disp, n, args, dir_out);
+    else lma_chain_nc<3>(h, st, which, corr, disp, n, args, dir_out);
+}
+static int ensure_lma_scratch(clt_handle* h, int which) {
+    clt_handle::LmaScratch& s = h->lma_scr[which];
+    const size_t cap = h->cap_tasks;
+    int rc;
+    if ((rc = dalloc(h, (void**)&s.st, 4 * cap * sizeof(LmaState))) || (rc = dalloc(h, (void**)&s.res, 4 * cap * 5 * 8)) ||
+        (rc = dalloc(h, (void**)&s.tmp, cap * 8 * 8)))
+        return rc;
+    return CLT_OK;
+}
+
 static void launch_corr_lma(clt_handle* h, int n, bool timed, int t0 = 0) {
     if (!h->lma_run) return;
-    const int slots = LMA_SLOTS;
-    auto kern = h->lma_args.adj_wyd ? k_corr_lma<4> : k_corr_lma<3>;
     (void)hipEventRecord(h->ev_fork, h->stream);
     (void)hipStreamWaitEvent(h->stream_lma, h->ev_fork, 0);
     if (timed) tbegin(h, T_CORR_LMA, h->stream_lma);
-    hipLaunchKernelGGL(kern, dim3((n + slots - 1) / slots), dim3(64 * LMA_WAVES), 0, h->stream_lma,
-                       h->d_corr + (size_t)t0 * h->corr_pairs_at_exec * CLT_CORR_LEN,
-                       h->d_disp + (size_t)t0 * CLT_DISP_LAYERS, n, h->d_lma_args, nullptr);
+    launch_lma_chain(h, h->stream_lma, 0, h->lma_args.adj_wyd ? 4 : 3,
+                     h->d_corr + (size_t)t0 * h->corr_pairs_at_exec * CLT_CORR_LEN, h->d_disp + (size_t)t0 * CLT_DISP_LAYERS,
+                     n, h->d_lma_args, nullptr);
     if (timed) tend(h, T_CORR_LMA, h->stream_lma);
     (void)hipEventRecord(h->ev_lma, h->stream_lma);
     h->lma_pending = true;
@@ -668,7 +704,9 @@ int clt_destroy(clt_handle* h) {
                     h->d_adj, h->d_tstate, h->d_gpart, h->d_gsum, h->d_gdisp2, h->d_lma_args, h->d_rig_corr,
                     h->d_rig_disp, h->d_rig_mix, h->d_rig_dirs, h->d_inter, h->d_inter_avg, h->d_inter_rec,
                     h->d_inter_res, h->d_bimap, h->d_rig_target, h->d_rig_args, h->d_lma_rig, h->d_rig_extra,
-                    h->d_rig_tex, h->d_rig_tex_stats, h->d_rig_tex_args, h->d_corr2, h->d_disp2};
+                    h->d_rig_tex, h->d_rig_tex_stats, h->d_rig_tex_args, h->d_corr2, h->d_disp2,
+                    h->lma_scr[0].st, h->lma_scr[0].res, h->lma_scr[0].tmp, h->lma_scr[1].st, h->lma_scr[1].res,
+                    h->lma_scr[1].tmp};
     if (h->stream) (void)sync_streams(h);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -1332,6 +1370,7 @@ static int build_lma_args(clt_handle* h, bool have_records) {
     if ((rc = fill_lma_args(h, h->pairs, h->lp, a))) return rc;
     h->lma_run = a.fit[0].ngroups > 0;
     if ((rc = dalloc(h, (void**)&h->d_lma_args, sizeof(LmaArgs)))) return rc;
+    if (h->lma_run && (rc = ensure_lma_scratch(h, 0))) return rc;
     if (h->lma_args_valid && std::memcmp(&a, &h->lma_args_dev, sizeof(a)) == 0) return CLT_OK;   // both zero-filled
     join_lma(h);    // a running LMA still reads the previous arguments
     join_lma2(h);
@@ -1494,7 +1533,8 @@ static int prepare_rig(clt_handle* h, const float* target) {
         (rc = dalloc(h, (void**)&h->d_bimap, T * CLT_BI_LAYERS * 4)) ||
         (rc = dalloc(h, (void**)&h->d_rig_extra, 2 * T * 4)) ||
         (rc = dalloc(h, (void**)&h->d_rig_args, 2 * sizeof(RigArgs))) ||
-        (rc = dalloc(h, (void**)&h->d_lma_rig, sizeof(LmaArgs))) ||
+        (rc = dalloc(h, (void**)&h->d_lma_rig, sizeof(LmaArgs))) || (rc = ensure_lma_scratch(h, 0)) ||
+        (rc = ensure_lma_scratch(h, 1)) ||
         (h->rp.lt_rad > 0 && (rc = dalloc(h, (void**)&h->d_inter_avg, T * CLT_CORR_LEN * 4))) ||
         (target && (rc = dalloc(h, (void**)&h->d_rig_target, T * 4))))
         return rc;
@@ -1596,7 +1636,6 @@ int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
     const int nc = h->lp.lma_adjust_wy ? 4 : 3;
     tbegin(h, T_RIG);
     const dim3 g4((n + 3) / 4), g_lma((n + LMA_SLOTS - 1) / LMA_SLOTS);
-    auto lma = (nc == 4) ? k_corr_lma<4> : k_corr_lma<3>;
     for (int q = 0; q < 2; q++) {   // each quad's correlate2D, with its mean CLT
         const int tpb = h->tune_corr_tpb;
         auto kern = fz ? k_corr2d_quad<true> : k_corr2d_quad<false>;
@@ -1607,8 +1646,7 @@ int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
         if (q == 0) {   // the main quad's LMA on the second stream, beside the aux quad's correlate2D
             HIPCHK(h, hipEventRecord(h->ev_fork, h->stream));
             HIPCHK(h, hipStreamWaitEvent(h->stream_lma, h->ev_fork, 0));
-            hipLaunchKernelGGL(lma, g_lma, dim3(64 * LMA_WAVES), 0, h->stream_lma, h->d_rig_corr, h->d_rig_disp, n,
-                               h->d_lma_rig, h->d_rig_dirs);
+            launch_lma_chain(h, h->stream_lma, 0, nc, h->d_rig_corr, h->d_rig_disp, n, h->d_lma_rig, h->d_rig_dirs);
             HIPCHK(h, hipEventRecord(h->ev_lma, h->stream_lma));
         }
     }
@@ -1620,8 +1658,8 @@ int clt_exec_corr2d_rig(clt_handle* h, const float* target_disparity) {
     join_side(h);
     HIPCHK(h, hipEventRecord(h->ev_side_fork, h->stream));
     HIPCHK(h, hipStreamWaitEvent(h->stream_side, h->ev_side_fork, 0));
-    hipLaunchKernelGGL(lma, g_lma, dim3(64 * LMA_WAVES), 0, h->stream_side, h->d_rig_corr + T * 6 * CLT_CORR_LEN,
-                       h->d_rig_disp + T * CLT_DISP_LAYERS, n, h->d_lma_rig, h->d_rig_dirs + T * 8);
+    launch_lma_chain(h, h->stream_side, 1, nc, h->d_rig_corr + T * 6 * CLT_CORR_LEN, h->d_rig_disp + T * CLT_DISP_LAYERS, n,
+                     h->d_lma_rig, h->d_rig_dirs + T * 8);
     HIPCHK(h, hipEventRecord(h->ev_side, h->stream_side));
     h->side_pending = true;
     auto lma_x = (nc == 4) ? k_corr_lma_inter<4> : k_corr_lma_inter<3>;
@@ -2505,8 +2543,19 @@ int clt_sync(clt_handle* h) {
 }
 
 int clt_set_timing(clt_handle* h, int32_t enable) {
-    if (!h) return CLT_E_ARG;
+    if (!h || enable < 0) return CLT_E_ARG;
     h->timing = enable != 0;
+    for (int i = 0; i < T_NUM && enable > 1; i++)   // events for `enable` launches of each kernel, created now
+        while ((int)h->ev_beg[i].size() < enable) {
+            hipEvent_t b = nullptr, e = nullptr;
+            HIPCHK(h, hipEventCreate(&b));
+            if (hipEventCreate(&e) != hipSuccess) {
+                (void)hipEventDestroy(b);
+                return fail(h, CLT_E_HIP, "clt_set_timing: hipEventCreate failed");
+            }
+            h->ev_beg[i].push_back(b);
+            h->ev_end[i].push_back(e);
+        }
     return CLT_OK;
 }
 

@@ -467,8 +467,10 @@ class GPUTileProcessor:
     def sync(self):
         self._chk(self.lib.clt_sync(self.h))
 
-    def setTiming(self, on):
-        self._chk(self.lib.clt_set_timing(self.h, int(on)))
+    def setTiming(self, on, reserve=0):
+        """HIP events around each kernel; reserve: create the events of that many launches of each
+        kernel now rather than at first use."""
+        self._chk(self.lib.clt_set_timing(self.h, max(int(bool(on)), int(reserve) if on else 0)))
 
     def resetTiming(self):
         self._chk(self.lib.clt_reset_timing(self.h))

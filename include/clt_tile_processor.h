@@ -466,7 +466,9 @@ int  clt_get_rbg_frame(clt_handle* h, int32_t frame, int32_t cam, float* out);
 /* ---- execution control and timing (no reference counterpart) --------------------------------- */
 int  clt_set_async(clt_handle* h, int32_t async);     /* 1: exec_* return after launch        */
 int  clt_sync(clt_handle* h);
-int  clt_set_timing(clt_handle* h, int32_t enable);   /* record HIP events around each kernel  */
+/* record HIP events around each kernel; enable > 1 also creates the events of `enable` launches of
+ * each kernel now (otherwise they are created at first use, inside the caller's timed region) */
+int  clt_set_timing(clt_handle* h, int32_t enable);
 /* Accumulated device time (ms) and launch count of a kernel since the last reset:
  * which = 0 convert, 1 imclt, 2 corr2d, 3 textures, 4 combine_textures, 5 batch graph,
  * 6 correlation adjoint, 7 geometry adjoint, 8 per-tile correlation LMA, 9 kernel preparation
