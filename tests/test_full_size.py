@@ -178,22 +178,26 @@ def test_c1_async_imclt_then_convert(pk, c1_scene):
 
 @pytest.mark.timeout(600)
 def test_c1_async_corr2d_while_lma_runs(pk, c1_scene):
-    """At this size the per-tile LMA runs about 0.5 ms on its own stream: correlate2D issued again
-    before it ends (async mode) writes the second pair of correlation records, and a third call
-    goes back to the first pair after waiting for the LMA that read it.  Every result must be
-    bit-identical to the synchronous calls."""
+    """Async frames at this size: each convert writes the other CLT buffer on its own stream while
+    the previous frame's correlate2D, one-pass textures/imclt and LMA (about 0.4 ms on its own
+    stream) may still run; correlate2D issued before the previous LMA ends writes the second pair
+    of correlation records, and a third frame goes back to the first buffers after waiting for
+    their readers.  The last frame's correlations, disparities, textures and images must be
+    bit-identical to the same frame run synchronously."""
     sc = c1_scene
-    log("case: async correlate2D x2 / x3 at 2592x1936 with the LMA of the previous call running")
+    log("case: async frames x2 / x3 at 2592x1936 (convert, correlate2D + LMA, textures + imclt)")
 
     def run(asynchronous, lpf_masks):
         tp = gpu_processor(pk, sc)
         tp.setCorrSigma(0.8)
         tp.setLpfRbg(0.7, 0.7, 0.7)
+        tp.setTexParams(keep_weights=0)
         tp.setAsync(asynchronous)
         for m in lpf_masks:
             tp.execConverCorrectTiles(m)
             tp.execCorrelate2D()
-        out = tp.getCorr2D()
+            tp.execTexturesImclt(True)
+        out = list(tp.getCorr2D()) + list(tp.getTextures()) + [tp.getRBG(1), tp.getCLT(2)]
         tp.close()
         return out
 
@@ -201,7 +205,7 @@ def test_c1_async_corr2d_while_lma_runs(pk, c1_scene):
         ref = run(False, masks[-1:])
         got = run(True, masks)
         same = [np.array_equal(a.view(np.int32), b.view(np.int32)) for a, b in zip(got, ref)]
-        log("  %d calls: bit-identical corr/combo/disparity = %s" % (len(masks), same))
+        log("  %d frames: bit-identical corr/combo/disparity/textures/stats/image/CLT = %s" % (len(masks), same))
         assert all(same)
 
 
