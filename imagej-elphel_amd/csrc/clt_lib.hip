@@ -2404,10 +2404,21 @@ int clt_autotune(clt_handle* h) {
         }
         h->tune_imclt_rows = pick;
     }
+    int stages = STAGE_CONVERT | STAGE_CORR | STAGE_IMCLT;
+    if (h->g.num_colors == 3 && h->g.num_cams >= 2 && prepare_tex(h) == CLT_OK) stages |= STAGE_TEX;
+    h->err.clear();
+    if ((stages & STAGE_TEX) && h->batch_fused) {   // tile rows per segment of the one-pass textures/imclt:
+        const int cand[3] = {4, 8, 16};           // the halo re-read (1/rows) against the workgroups
+        float best = 1e30f;
+        int pick = h->tex_imclt_rows;
+        for (int c : cand) {
+            h->tex_imclt_rows = c;
+            const float ms = time_stage(h, STAGE_TEX | STAGE_IMCLT);
+            if (ms < best) { best = ms; pick = c; }
+        }
+        h->tex_imclt_rows = pick;
+    }
     {   // frame chunks of the pipelined batch (overlap of LMA and imclt_rbg with the next chunk)
-        int stages = STAGE_CONVERT | STAGE_CORR | STAGE_IMCLT;
-        if (h->g.num_colors == 3 && h->g.num_cams >= 2 && prepare_tex(h) == CLT_OK) stages |= STAGE_TEX;
-        h->err.clear();
         const int cand[4] = {1, 2, 4, 8};
         float best = 1e30f;
         int pick = h->tune_chunks;
@@ -2439,6 +2450,14 @@ int clt_set_batch_chunks(clt_handle* h, int32_t chunks) {
     if (chunks < 1 || chunks > 64) return fail(h, CLT_E_ARG, "clt_set_batch_chunks: %d", chunks);
     h->tune_chunks = chunks;
     h->graph_epoch++;
+    return CLT_OK;
+}
+
+int clt_get_tune_all(clt_handle* h, int32_t* values, int32_t n) {
+    if (!h || !values || n < 0) return CLT_E_ARG;
+    const int32_t v[4] = {h->tune_corr_tpb, h->tune_imclt_rows, std::max(1, std::min(h->tune_chunks, h->num_frames)),
+                          h->tex_imclt_rows};
+    for (int i = 0; i < n && i < 4; i++) values[i] = v[i];
     return CLT_OK;
 }
 

@@ -511,11 +511,9 @@ class GPUTileBatch(GPUTileProcessor):
         return self.getTune()
 
     def getTune(self):
-        a, b = ctypes.c_int32(), ctypes.c_int32()
-        self._chk(self.lib.clt_get_tune(self.h, ctypes.byref(a), ctypes.byref(b)))
-        c = ctypes.c_int32()
-        self._chk(self.lib.clt_get_batch_chunks(self.h, ctypes.byref(c)))
-        return {"corr_tiles_per_block": a.value, "imclt_rows": b.value, "chunks": c.value}
+        v = (ctypes.c_int32 * 4)()
+        self._chk(self.lib.clt_get_tune_all(self.h, v, 4))
+        return {"corr_tiles_per_block": v[0], "imclt_rows": v[1], "chunks": v[2], "tex_imclt_rows": v[3]}
 
     def setBatchChunks(self, chunks):
         """Frame chunks of the pipelined batch (clt_autotune picks one of 1, 2, 4, 8)."""

@@ -451,10 +451,13 @@ int  clt_exec_batch(clt_handle* h, int32_t stages);
  * correlate2D run in order while the previous chunks' imclt_rbg and per-tile LMA run on their own
  * streams.  Results do not depend on the chunk count. */
 /* Times the candidate launch shapes on the current batch and keeps the fastest: tiles per
- * workgroup of correlate2D (1, 2, 4), tile rows per imclt segment (8, 16, 32) and frame chunks of
- * the pipeline (1, 2, 4, 8, at most num_frames). */
+ * workgroup of correlate2D (1, 2, 4: its LDS per workgroup), tile rows per imclt segment (8, 16,
+ * 32), tile rows per segment of the one-pass textures/imclt (4, 8, 16) and frame chunks of the
+ * pipeline (1, 2, 4, 8, at most num_frames). */
 int  clt_autotune(clt_handle* h);
 int  clt_get_tune(clt_handle* h, int32_t* corr_tiles_per_block, int32_t* imclt_rows);
+/* values[0..n-1] (n <= 4): correlate2D tiles per workgroup, imclt rows, frame chunks, one-pass rows */
+int  clt_get_tune_all(clt_handle* h, int32_t* values, int32_t n);
 /* chunks 1..64 (used as min(chunks, num_frames)); 1 until clt_autotune or this call. */
 int  clt_set_batch_chunks(clt_handle* h, int32_t chunks);
 int  clt_get_batch_chunks(clt_handle* h, int32_t* chunks);
