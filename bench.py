@@ -397,13 +397,16 @@ def configs_bench(pk, steps=50):
 
 
 def adjoint_bench(pk, sc, repeats=5, seed=11):
-    """Differentiable form on the same frame: forward (convert + correlate2D) and the adjoint
-    (port centres from the geometry, the fused convert/loss adjoint, the geometry adjoint and the
-    LMA system reduction) per pass, then 10 iterations (3 coarse steps on the pairs'
-    correlation-peak offsets, 7 LMA iterations) of the extrinsics and disparities from extrinsics
-    perturbed N(0, 1e-3); the frame was rendered with zero extrinsics, so the extrinsics' distance
-    to 0 shows the convergence (up to the quad's gauge: azimuths along rXY_x and tilts along rXY_y
-    act as a common disparity)."""
+    """Differentiable form: forward (convert + correlate2D) and the adjoint (port centres from the
+    geometry, the fused convert/loss adjoint, the geometry adjoint and the LMA system reduction)
+    per pass on the bench frame; then the extrinsic/disparity solve on a 2592x1936 frame rendered
+    through the port geometry with extrinsics drawn N(0, 1e-3): 10 steps on the pairs'
+    correlation-peak offsets (the residual form), and 5 such steps followed by 5 loss-adjoint LMA
+    iterations.  Each reports the RMS error of the pairs' relative port offsets (what the
+    correlations see) against the true geometry, from the library's own port geometry, in all and
+    from the extrinsics alone (every tile's disparity re-fitted to the truth); the extrinsics
+    themselves are determined only up to the quad's gauge (azimuths along rXY_x and tilts along
+    rXY_y act as a common disparity)."""
     T = len(sc.tasks)
     tx, ty = sc.tasks[:, 1] & 0xFFFF, (sc.tasks[:, 1] >> 16) & 0xFFFF
     d = sc.disparity[ty, tx].astype(np.float32)
@@ -426,19 +429,63 @@ def adjoint_bench(pk, sc, repeats=5, seed=11):
     fwd = (tp.getTiming("convert")[0] + tp.getTiming("corr2d")[0]) / repeats
     adj = tp.getTiming("geometry_adjoint")[0] / repeats
     tp.setTiming(False)
-    rng = np.random.default_rng(seed)
-    e0 = rng.normal(0, 1e-3, 13)
-    tp.setPortGeometry(pk.GPUTileProcessor.defaultPortGeometry(W, H, extrinsic=list(e0)))
-    tp.getTileSubcamOffsets(words, tx, ty, d)
-    t0 = time.perf_counter()
-    ext, _, hist = tp.lmaExtrinsics(10, coarse_iter=3)
-    lma_s = time.perf_counter() - t0
     tp.close()
+    # the solve on a frame rendered through the port geometry itself (camera images resampled at
+    # the port positions of extrinsics drawn N(0, 1e-3) and a smooth disparity field), started
+    # from zero extrinsics and the true disparities plus N(0, 0.05 px)
+    rng = np.random.default_rng(seed)
+    e_true = rng.normal(0, 1e-3, 13)
+    print("bench: rendering the 2592x1936 geometry frame ...", file=sys.stderr, flush=True)
+    gs = pk.synth.make_geometry_scene(W, H, e_true, seed=seed)
+    inner = (gs.tx > 1) & (gs.tx < W // 8 - 2) & (gs.ty > 1) & (gs.ty < H // 8 - 2)
+    tiles = np.nonzero(inner)[0]
+    d_true = gs.disparity_tiles.ravel()[tiles].astype(np.float32)
+    d0 = (d_true + rng.normal(0, 0.05, len(tiles))).astype(np.float32)
+    gw, gx, gy = np.full(len(tiles), 0xFF, np.int32), gs.tx[tiles], gs.ty[tiles]
+    gp = pk.GPUTileProcessor(W, H, 4, 3, kernels=False)
+    gp.setBayerImages(gs.images, force=True)
+    gp.setCorrSigma(0.8)
+    pairs = [(0, 1), (2, 3), (0, 2), (1, 3), (0, 3), (2, 1)]
+
+    def port_xy(extrinsic, disparity):   # the tasks' port centres from the library's port geometry
+        gp.setPortGeometry(pk.GPUTileProcessor.defaultPortGeometry(W, H, extrinsic=list(extrinsic)))
+        gp.getTileSubcamOffsets(gw, gx, gy, disparity)
+        return gp.getTasks()[:, 2:].view(np.float32).reshape(len(tiles), 4, 2).astype(np.float64)
+
+    true_xy = port_xy(e_true, d_true)
+
+    def rel(xy):
+        return np.stack([xy[:, i] - xy[:, j] for i, j in pairs], 1).reshape(len(xy), 12)
+
+    true_rel = rel(true_xy)
+
+    def errors(extrinsic, disparity):
+        """RMS error of the pairs' relative port offsets against the true geometry, px: all of it,
+        and what the extrinsics alone leave once each tile's disparity is re-fitted to the truth."""
+        r = rel(port_xy(extrinsic, disparity)) - true_rel
+        dd = np.asarray(disparity, np.float64).copy()
+        for _ in range(2):
+            r0 = rel(port_xy(extrinsic, dd)) - true_rel
+            jac = (rel(port_xy(extrinsic, dd + 1e-2)) - rel(port_xy(extrinsic, dd))) / 1e-2
+            dd -= (jac * r0).sum(1) / (jac * jac).sum(1)
+        rx = rel(port_xy(extrinsic, dd)) - true_rel
+        return float(np.sqrt(np.mean(r ** 2))), float(np.sqrt(np.mean(rx ** 2)))
+
+    runs = {}
+    start = errors(np.zeros(13), d0)
+    for name, coarse in (("peak_offsets_10", 10), ("peak_offsets_5_lma_5", 5)):
+        port_xy(np.zeros(13), d0)
+        t0 = time.perf_counter()
+        ext, disp, hist = gp.lmaExtrinsics(10, coarse_iter=coarse)
+        ms = 1e3 * (time.perf_counter() - t0)
+        end = errors(ext, disp)
+        runs[name] = {"coarse_steps": coarse, "iterations": len(hist) - 1, "ms": round(ms, 2),
+                      "rel_port_offset_px": [round(start[0], 4), round(end[0], 4)],
+                      "from_extrinsics_px": [round(start[1], 4), round(end[1], 4)]}
+    gp.close()
     return {"tasks": T, "forward_ms": round(fwd, 4), "adjoint_ms": round(adj, 4),
             "forward_tiles_per_s": round(T / (fwd * 1e-3), 1), "adjoint_tiles_per_s": round(T / (adj * 1e-3), 1),
-            "lma_iterations": len(hist) - 1, "lma_coarse_steps": 3, "lma_ms": round(1e3 * lma_s, 2),
-            "extrinsics_rms": [float("%.3g" % np.sqrt(np.mean(e0 ** 2))), float("%.3g" % np.sqrt(np.mean(ext ** 2)))],
-            "lma_loss": [round(float(hist[0]), 2), round(float(hist[-1]), 2)]}
+            "solve_tasks": len(tiles), "solve": runs}
 
 
 def kernel_prep_bench(pk, seed=13, cams=4):

@@ -100,7 +100,8 @@ class clt_lma_params(ctypes.Structure):
     _fields_ = [("lambda_", ctypes.c_double), ("lambda_scale_good", ctypes.c_double),
                 ("lambda_scale_bad", ctypes.c_double), ("lambda_max", ctypes.c_double),
                 ("rms_diff", ctypes.c_double), ("num_iter", ctypes.c_int32), ("adjust_disparity", ctypes.c_int32),
-                ("coarse_iter", ctypes.c_int32), ("reserved", ctypes.c_int32), ("coarse_radius", ctypes.c_double)]
+                ("coarse_iter", ctypes.c_int32), ("reserved", ctypes.c_int32), ("coarse_radius", ctypes.c_double),
+                ("coarse_strength_floor", ctypes.c_double)]
 
 
 _P = ctypes.c_void_p
@@ -148,6 +149,7 @@ SIGNATURES = {
     "clt_get_rbg": (_I, [_H, _I, _P]),
     "clt_get_clt": (_I, [_H, _I, _P]),
     "clt_get_corr2d": (_I, [_H, _P, _P, _P]),
+    "clt_get_corr_offsets": (_I, [_H, ctypes.c_double, _P]),
     "clt_get_textures": (_I, [_H, _P, _P]),
     "clt_get_combined_textures": (_I, [_H, _P]),
     "clt_set_async": (_I, [_H, _I]),

@@ -1672,27 +1672,43 @@ __global__ __launch_bounds__(256) void k_sum_rows(const double* __restrict__ a, 
 
 // Correlation-peak offsets of the 6 quad pairs of each task (the measurement of the reference's
 // extrinsic path: clt_mismatch, ID:2342-2420, by mismatchPairsCM, C2:1336-1430): per pair the
-// integer maximum of its 15x15 correlation and the centre of mass of the positive values within
-// `radius` of it (image axes, +-7 px).  The peak sits at the excess of the pair's relative port
-// offset (xy_c1 - xy_c2) over the aligned one, so the correction wanted is minus the peak.  Written
-// as a loss-adjoint record (CORR_ADJ_STRIDE floats) for k_geometry_adjoint, whose Gauss-Newton
-// step then solves min sum_p w_p |D_p delta + peak_p|^2 (D_p = J_c1 - J_c2): [0] sum w |peak|^2,
-// [7..12] and [13..18] the pair weights w_p (the correlation at the maximum when it is at least
-// min_corr, else 0), [19..26] per camera sum_{p: c1 = c} w_p peak_p - sum_{p: c2 = c} w_p peak_p.
+// integer maximum of its 15x15 correlation (the first in row order) and the centre of mass of the
+// positive values within `radius` of it (image axes, +-7 px; the reference centres its window at the
+// tile's residual disparity instead, the pairs here being aligned at the task disparity already).
+// The peak sits at the excess of the pair's relative port offset (xy_c1 - xy_c2) over the aligned
+// one, so the correction wanted is minus the peak.  Outputs (either may be null):
+//  adj  a loss-adjoint record (CORR_ADJ_STRIDE floats) for k_geometry_adjoint, whose Gauss-Newton
+//       step then solves min sum_p w_p |D_p delta + peak_p|^2 (D_p = J_c1 - J_c2): [0] sum w |peak|^2,
+//       [7..12] and [13..18] the pair weights w_p (the correlation at the maximum less `floor`
+//       where it exceeds the floor, else 0: the reference's tile weights, AC:1521-1525), [19..26]
+//       per camera sum_{p: c1 = c} w_p peak_p - sum_{p: c2 = c} w_p peak_p;
+//  offs [6][3] per task: dx, dy of the peak (NaN where no value in the window is positive) and the
+//       correlation at the maximum (tasks without work: NaN, NaN, 0).
 __global__ __launch_bounds__(256) void k_corr_offsets(const float* __restrict__ corr, const int32_t* __restrict__ tasks,
-                                                      int task_words, int num_tasks, float radius, float min_corr,
-                                                      float* __restrict__ adj) {
+                                                      int task_words, int num_tasks, float radius, float floor,
+                                                      float* __restrict__ adj, float* __restrict__ offs) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= num_tasks) return;
-    float* a = adj + (size_t)t * CORR_ADJ_STRIDE;
+    const float qnan = __int_as_float(0x7fc00000);
+    float* a = adj ? adj + (size_t)t * CORR_ADJ_STRIDE : nullptr;
+    float* o = offs ? offs + (size_t)t * 18 : nullptr;
     if (tasks[(size_t)t * task_words] == 0) {
-        a[0] = __int_as_float(0x7fc00000);
-        for (int k = 1; k < CORR_ADJ_STRIDE; k++) a[k] = 0.f;
+        if (a) {
+            a[0] = qnan;
+            for (int k = 1; k < CORR_ADJ_STRIDE; k++) a[k] = 0.f;
+        }
+        if (o)
+            for (int p = 0; p < 6; p++) {
+                o[3 * p] = qnan;
+                o[3 * p + 1] = qnan;
+                o[3 * p + 2] = 0.f;
+            }
         return;
     }
     const int ir = (int)radius;
     const float r2 = radius * radius;
     float g[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float wp[6];
     float loss = 0.f;
     for (int p = 0; p < 6; p++) {
         const float* c = corr + ((size_t)t * 6 + p) * 225;
@@ -1716,19 +1732,25 @@ __global__ __launch_bounds__(256) void k_corr_offsets(const float* __restrict__ 
                 }
             }
         }
-        const bool ok = vm >= min_corr && s0 > 0.f;
-        const float w = ok ? vm : 0.f;
+        const bool ok = vm > floor && s0 > 0.f;
+        const float w = ok ? vm - floor : 0.f;
         const float px = ok ? sx / s0 : 0.f, py = ok ? sy / s0 : 0.f;
-        a[7 + p] = w;
-        a[13 + p] = w;
+        if (o) {
+            o[3 * p] = s0 > 0.f ? sx / s0 : qnan;
+            o[3 * p + 1] = s0 > 0.f ? sy / s0 : qnan;
+            o[3 * p + 2] = vm;
+        }
+        wp[p] = w;
         loss += w * (px * px + py * py);
         g[2 * QP_C1[p]] += w * px;
         g[2 * QP_C1[p] + 1] += w * py;
         g[2 * QP_C2[p]] -= w * px;
         g[2 * QP_C2[p] + 1] -= w * py;
     }
+    if (!a) return;
     a[0] = loss;
     for (int k = 0; k < 6; k++) a[1 + k] = 0.f;
+    for (int k = 0; k < 6; k++) a[7 + k] = a[13 + k] = wp[k];
     for (int k = 0; k < 8; k++) a[19 + k] = g[k];
     for (int k = 27; k < CORR_ADJ_STRIDE; k++) a[k] = 0.f;
 }

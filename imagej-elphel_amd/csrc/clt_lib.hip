@@ -183,6 +183,7 @@ struct clt_handle {
     float* d_gdisp = nullptr;      // [max_tasks]
     double* d_gderiv = nullptr;    // [max_tasks][8][14]
     float* d_adj = nullptr;        // [cap_tasks][CORR_ADJ_STRIDE] correlation-peak loss and gradient
+    float* d_offs = nullptr;       // [cap_tasks][6][3] correlation-peak offsets (clt_get_corr_offsets)
     bool adj_done = false;
     clt_port_geometry pgeom{};     // host copy of the current port geometry
     double* d_tstate = nullptr;    // [max_tasks][15] per-task LMA state
@@ -706,7 +707,7 @@ int clt_destroy(clt_handle* h) {
                     h->d_inter_res, h->d_bimap, h->d_rig_target, h->d_rig_args, h->d_lma_rig, h->d_rig_extra,
                     h->d_rig_tex, h->d_rig_tex_stats, h->d_rig_tex_args, h->d_corr2, h->d_disp2,
                     h->lma_scr[0].st, h->lma_scr[0].res, h->lma_scr[0].tmp, h->lma_scr[1].st, h->lma_scr[1].res,
-                    h->lma_scr[1].tmp};
+                    h->lma_scr[1].tmp, h->d_offs};
     if (h->stream) (void)sync_streams(h);
     if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -1895,6 +1896,24 @@ int clt_get_clt(clt_handle* h, int32_t cam, float* out) {
     return CLT_OK;
 }
 
+int clt_get_corr_offsets(clt_handle* h, double radius, float* offsets) {
+    if (!h || !offsets) return CLT_E_ARG;
+    if (!(radius >= 1.0 && radius <= 7.0)) return fail(h, CLT_E_ARG, "clt_get_corr_offsets: radius %g outside [1, 7]", radius);
+    if (!h->corr_done) return fail(h, CLT_E_STATE, "clt_get_corr_offsets: corr2d not executed");
+    if (!h->corr_quad || h->corr_pairs_at_exec != 6)
+        return fail(h, CLT_E_UNSUPPORTED, "clt_get_corr_offsets: needs the 6 reference quad pairs");
+    const int n = h->num_tasks;
+    if (n == 0) return CLT_OK;
+    int rc;
+    if ((rc = dalloc(h, (void**)&h->d_offs, (size_t)h->cap_tasks * 18 * 4))) return rc;
+    hipLaunchKernelGGL(k_corr_offsets, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_corr, h->d_tasks,
+                       h->task_words, n, (float)radius, 0.f, nullptr, h->d_offs);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, sync_streams(h));
+    HIPCHK(h, hipMemcpy(offsets, h->d_offs, (size_t)n * 18 * 4, hipMemcpyDeviceToHost));
+    return CLT_OK;
+}
+
 int clt_get_corr2d(clt_handle* h, float* corr, float* combo, float* disparity) {
     if (!h) return CLT_E_ARG;
     if (!h->corr_done) return fail(h, CLT_E_STATE, "clt_get_corr2d: corr2d not executed");
@@ -1982,6 +2001,7 @@ void clt_default_lma_params(clt_lma_params* p) {
     p->coarse_iter = 0;
     p->reserved = 0;
     p->coarse_radius = 3.9;      // ECP:2566 max_corr_radius
+    p->coarse_strength_floor = 0.3;
 }
 
 static int geo_prepare(clt_handle* h, const char* who) {
@@ -2156,7 +2176,8 @@ int clt_lma_extrinsics(clt_handle* h, const clt_lma_params* prm, double* extrins
         launch_convert(h, n, nullptr, 0);
         launch_corr(h, n, nullptr);
         hipLaunchKernelGGL(k_corr_offsets, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_corr, h->d_tasks,
-                           h->task_words, n, (float)prm->coarse_radius, h->cp.min_corr, h->d_adj);
+                           h->task_words, n, (float)prm->coarse_radius, std::max((float)prm->coarse_strength_floor, h->cp.min_corr),
+                           h->d_adj, nullptr);
         hipLaunchKernelGGL(k_geometry_adjoint, dim3(blocks), dim3(128), 0, h->stream, h->d_port_geom, h->d_rtab,
                            h->d_gwords, h->d_gtxy, d_cur, h->d_adj, n, 0.0, h->d_gpart, h->d_tstate);
         hipLaunchKernelGGL(k_sum_rows, dim3(GEO_ADJ_SUMS), dim3(256), 0, h->stream, h->d_gpart, blocks, GEO_ADJ_SUMS,

@@ -361,6 +361,14 @@ class GPUTileProcessor:
         self._chk(self.lib.clt_get_corr2d(self.h, _ptr(corr), _ptr(combo), _ptr(disp)))
         return corr, combo, disp
 
+    def getCorrPeakOffsets(self, radius=3.9):
+        """Per task and quad pair (dx, dy, correlation at the maximum): the centre of mass of the
+        positive correlation within radius of the pair's integer maximum, pixels from the centre
+        (the clt_mismatch measurement, ID:2342-2420, as the coarse extrinsic stage uses it)."""
+        out = np.empty((self.num_task_tiles, 6, 3), np.float32)
+        self._chk(self.lib.clt_get_corr_offsets(self.h, float(radius), _ptr(out)))
+        return out
+
     def execCorrelate2DRig(self, target_disparity=None):
         """clt_bi_quad (ID:8731-9504) correlation part on an 8-camera handle (main quad 0..3,
         auxiliary 4..7); target_disparity: per task (the BI_TARGET layer) or None."""

@@ -295,6 +295,16 @@ int  clt_get_clt(clt_handle* h, int32_t cam, float* out);
 /* corr: [num_tasks][num_pairs][225]; combo: [num_tasks][strip_height*15] (may be NULL);
  * disparity: [num_tasks][CLT_DISP_LAYERS] (may be NULL). */
 int  clt_get_corr2d(clt_handle* h, float* corr, float* combo, float* disparity);
+/* Correlation-peak offsets of the 6 quad pairs after clt_exec_corr2d: the per-pair measurement of
+ * the reference's extrinsic path (clt_mismatch, ID:2342-2420, mismatchPairsCM C2:1336-1430), which
+ * the coarse stage of clt_lma_extrinsics solves for.  offsets [num_tasks][6][3]: dx, dy of the
+ * centre of mass of the positive correlation values within radius (1..7) of the pair's integer
+ * maximum (image axes, pixels from the centre: the excess of the pair's relative port offset
+ * xy_c1 - xy_c2 over the aligned one; NaN where no value there is positive) and the correlation
+ * at the maximum.  Tasks without work: NaN, NaN, 0.  The reference centres its window at the tile's
+ * residual disparity (xcenter = -disparity) and reports offsets beyond it; here the pairs are
+ * aligned at the task disparity and the disparity is solved for with the extrinsics. */
+int  clt_get_corr_offsets(clt_handle* h, double radius, float* offsets);
 /* Results of clt_exec_corr2d_rig, each may be NULL: bimap [num_tasks][CLT_BI_LAYERS] (the
  * disparity_bimap layers of clt_bi_quad); inter [num_tasks][225] inter-camera correlations
  * (clt_corr_combo[0], before the low-texture averaging); corr_main / corr_aux [num_tasks][6][225]
@@ -423,10 +433,15 @@ typedef struct clt_lma_params {
      * AC:2891-3037, 3291-3470): the first coarse_iter iterations run convert + correlate2D, take each
      * pair's peak offset (centre of mass within coarse_radius of its maximum, +-7 px) and make the
      * Gauss-Newton step that cancels the offsets in the least-squares sense (weights: the peak
-     * values), the disparities eliminated per task; the loss-adjoint LMA follows.  0: LMA only. */
+     * value less coarse_strength_floor, pairs at or below the floor left out, as the reference
+     * weights its tiles, AC:1521-1525), the disparities eliminated per task; the loss-adjoint LMA
+     * follows.  0: LMA only. */
     int32_t coarse_iter;        /* 0                                                             */
     int32_t reserved;
     double  coarse_radius;      /* 3.9 (max_corr_radius, ECP:2566)                               */
+    /* 0.3: the correlation maximum of a pair whose offset lies beyond the +-7 px window is a
+     * noise peak (about 0.23 on average against 0.5 for aligned textured tiles) */
+    double  coarse_strength_floor;
 } clt_lma_params;
 void clt_default_lma_params(clt_lma_params* p);
 /* Runs from the handle's port geometry and task disparities and leaves the handle at the solution

@@ -265,6 +265,7 @@ def lib(precision=64):
                                       ctypes.POINTER(RigOffset), ctypes.c_int, _D, _D, _D, _D]
     L.orc_quadratic_max.argtypes = [_D, _D, _D, _D, ctypes.c_int, _D]
     L.orc_single2d_cm.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D]
+    L.orc_corr_peak_offsets.argtypes = [_D, ctypes.c_int, ctypes.c_double, _D]
     L.orc_single2d_poly.argtypes = [_D, ctypes.c_double, ctypes.POINTER(CorrLmaParams), _D]
     _LIBS[precision] = L
     return L
@@ -695,6 +696,16 @@ def quadratic_max(x, y, z, w):
 def single2d_cm(corr225, xcenter=0.0, radius=3.9):
     out = np.zeros(3)
     lib(64).orc_single2d_cm(_dp(_f64(corr225).ravel()), xcenter, radius, _dp(out))
+    return out
+
+
+def corr_peak_offsets(corr, radius=3.9):
+    """[n][6][225] quad-pair correlations -> [n][6][3] (dx, dy, value at the maximum): centre of
+    mass of the positive values within radius of each pair's integer maximum (mismatchPairsCM,
+    C2:1336-1430, as the clt_mismatch measurement, ID:2342-2420)."""
+    c = np.ascontiguousarray(_f64(corr).reshape(-1, 6, 225))
+    out = np.zeros((c.shape[0], 6, 3))
+    lib(64).orc_corr_peak_offsets(_dp(c), c.shape[0], float(radius), _dp(out))
     return out
 
 

@@ -2751,6 +2751,39 @@ EXPORT int orc_rig_frame(const clt_geom* g, const double* clt, const int32_t* ta
                             bimap, nthreads);
 }
 
+/* Correlation-peak offsets of quad pairs (the clt_mismatch measurement, ID:2342-2420, by the centre
+ * of mass of mismatchPairsCM, C2:1336-1430; the window is centred at the pair's integer maximum,
+ * the first in row order, instead of at the residual disparity): corr [n][6][225], out [n][6][3] =
+ * dx, dy of the centre of mass of the positive values within radius (image axes, pixels from the
+ * centre; NaN where none is positive), the correlation at the maximum. */
+EXPORT void orc_corr_peak_offsets(const double* corr, int n, double radius, double* out) {
+    const int iradius = (int)fabs(radius), ir2 = (int)(radius * radius);
+    for (int k = 0; k < 6 * n; k++) {
+        const double* c = corr + (size_t)k * 225;
+        int im = 0;
+        for (int i = 1; i < 225; i++)
+            if (c[i] > c[im]) im = i;
+        const int iy = im / 15 - 7, ix = im % 15 - 7;
+        double s0 = 0, sx = 0, sy = 0;
+        for (int y = iy - iradius; y <= iy + iradius; y++) {
+            if (y < -7 || y > 7) continue;
+            for (int x = ix - iradius; x <= ix + iradius; x++) {
+                const int r2 = (x - ix) * (x - ix) + (y - iy) * (y - iy);
+                if (x < -7 || x > 7 || r2 > ir2) continue;
+                const double d = c[(y + 7) * 15 + x + 7];
+                if (d > 0.0) {
+                    s0 += d;
+                    sx += d * x;
+                    sy += d * y;
+                }
+            }
+        }
+        out[3 * k] = s0 > 0.0 ? sx / s0 : NAN;
+        out[3 * k + 1] = s0 > 0.0 ? sy / s0 : NAN;
+        out[3 * k + 2] = c[im];
+    }
+}
+
 /* single checks for the tests */
 EXPORT int orc_quadratic_max(const double* x, const double* y, const double* z, const double* w, int n, double* out5) {
     return quadratic_max(x, y, z, w, n, 1.0e-30, out5);

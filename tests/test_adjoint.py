@@ -249,3 +249,84 @@ def test_lma_capture_range_with_coarse_stage(pk, orc, seed):
     assert runs["lma only"][0] > 1.0                       # outside the basin of the loss alone
     assert runs["coarse 3 + lma"][0] < 0.25
     assert runs["coarse 3 + lma"][1][-1] <= loss_true
+
+
+def _rel_port_errors(orc, sc, tiles, e, d, e_true, d_true):
+    """RMS error of the pairs' relative port offsets (what the correlations see) of extrinsics e and
+    disparities d against the truth (FP64 port geometry of the oracle), px: (all of it, what the
+    extrinsics alone leave once every tile's disparity is re-fitted to the truth, the worst tile)."""
+    px, py = sc.tx[tiles] * 8 + 4.0, sc.ty[tiles] * 8 + 4.0
+
+    def rel(ext, disp):
+        x = orc.port_coords(orc.default_port_geometry(sc.width, sc.height, extrinsic=list(ext)), px, py,
+                            np.asarray(disp, np.float64))
+        return np.stack([x[:, i] - x[:, j] for i, j in PAIRS], 1).reshape(len(tiles), 12)
+
+    truth = rel(e_true, d_true)
+    r = rel(e, d) - truth
+    dd = np.asarray(d, np.float64).copy()
+    for _ in range(2):                    # Gauss-Newton on each tile's disparity (nearly linear)
+        r0 = rel(e, dd) - truth
+        j = (rel(e, dd + 1e-3) - rel(e, dd)) / 1e-3
+        dd -= (j * r0).sum(1) / (j * j).sum(1)
+    rx = rel(e, dd) - truth
+    return (float(np.sqrt(np.mean(r ** 2))), float(np.sqrt(np.mean(rx ** 2))),
+            float(np.sqrt(np.mean(r ** 2, 1)).max()))
+
+
+@pytest.mark.gpu
+def test_corr_peak_offsets_match_oracle(pk, orc):
+    """clt_get_corr_offsets on a misaligned scene (extrinsics N(0, 1e-3) rendered, zero assumed:
+    offsets of up to several px, some pairs beyond the +-7 px window) against the oracle's peak
+    offsets of the same correlations: the same maxima, the centres of mass to float32 rounding."""
+    rng = np.random.default_rng(4)
+    sc = pk.synth.make_geometry_scene(512, 384, rng.normal(0, 1e-3, 13), seed=4)
+    inner = (sc.tx > 1) & (sc.tx < sc.width // 8 - 2) & (sc.ty > 1) & (sc.ty < sc.height // 8 - 2)
+    tiles = np.nonzero(inner)[0]
+    d = sc.disparity_tiles.ravel()[tiles].astype(np.float32)
+    tp = _geometry_processor(pk, sc, np.zeros(13), tiles, d)
+    tp.execConverCorrectTiles(0)
+    tp.execCorrelate2D()
+    corr, _, _ = tp.getCorr2D()
+    for radius in (3.9, 1.5, 7.0):
+        got = tp.getCorrPeakOffsets(radius)
+        want = orc.corr_peak_offsets(corr, radius)
+        nan = np.isnan(want[..., 0])
+        log("case: peak offsets radius %.1f, %d tasks x 6 pairs: max |offset| %.2f px, %d without a positive value, "
+            "max |d| %.2g px" % (radius, len(tiles), np.nanmax(np.abs(want[..., :2])), nan.sum(),
+                                 np.nanmax(np.abs(got[..., :2] - want[..., :2]))))
+        assert np.array_equal(np.isnan(got[..., 0]), nan) and np.array_equal(np.isnan(got[..., 1]), nan)
+        assert np.array_equal(got[..., 2], corr.reshape(len(tiles), 6, 225).max(2))
+        assert np.nanmax(np.abs(got[..., :2] - want[..., :2])) < 2e-5
+    with pytest.raises(RuntimeError):
+        tp.getCorrPeakOffsets(0.5)
+    tp.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [11, 12])
+def test_coarse_stage_reaches_hundredth_pixel(pk, orc, seed):
+    """Extrinsics drawn N(0, 1e-3) (relative port offsets about 4.4 px off, up to 5.5 px per tile) on a
+    1296x968 frame rendered through the port geometry: 10 coarse steps on the pairs' peak offsets
+    leave at most 0.01 px of relative port-offset error from the extrinsics (each tile's disparity
+    re-fitted to the truth), at most 0.05 px in all (the rest is the per-tile disparities, measured
+    by each tile's own correlations: about 0.05 px, as precise as the 0.05 px start) and no tile
+    beyond 0.5 px.  Correlation maxima at or below the strength floor 0.3, the noise peaks of pairs
+    beyond the +-7 px window, are left out."""
+    rng = np.random.default_rng(seed)
+    e_true = rng.normal(0, 1e-3, 13)
+    sc = pk.synth.make_geometry_scene(1296, 968, e_true, seed=seed)
+    inner = (sc.tx > 1) & (sc.tx < sc.width // 8 - 2) & (sc.ty > 1) & (sc.ty < sc.height // 8 - 2)
+    tiles = np.nonzero(inner)[0]
+    d_true = sc.disparity_tiles.ravel()[tiles]
+    d0 = (d_true + rng.normal(0, 0.05, len(tiles))).astype(np.float32)
+    tp = _geometry_processor(pk, sc, np.zeros(13), tiles, d0)
+    ext, disp, hist = tp.lmaExtrinsics(10, coarse_iter=10)
+    tp.close()
+    start = _rel_port_errors(orc, sc, tiles, np.zeros(13), d0, e_true, d_true)
+    end = _rel_port_errors(orc, sc, tiles, ext, disp, e_true, d_true)
+    log("case: coarse stage, 1296x968, extrinsics N(0, 1e-3), seed %d, %d tasks: relative port offset error "
+        "%.3f -> %.4f px, from the extrinsics %.3f -> %.4f px, worst tile %.2f -> %.3f px"
+        % (seed, len(tiles), start[0], end[0], start[1], end[1], start[2], end[2]))
+    assert start[0] > 3.0
+    assert end[1] <= 0.01 and end[0] <= 0.05 and end[2] < 0.5

@@ -49,6 +49,8 @@ Each oracle function is pinned by what the reference itself holds or what mathem
                                       through the DCT-IIe and unfold pinned above
   getMaxXCm (C2:936-1032)             an asymmetric peak with a negative sample and a far sample,
                                       windows and offsets worked by hand, at two integer centres
+  corr_peak_offsets (ID:2342-2420,    hand-worked windows (negative, far and clipped samples, a
+   C2:1336-1430)                      tie, nothing positive); single2dCM (C2:1538-1586) on the row
 The planted-mistake check of these pins (final mix of the debayered tiles, ksigma not halved,
 dust_remove from port 0, the tie taken by the last port, fat zero not squared, a wrong side
 factor) is tests/test_oracle_mutants.py.
@@ -844,3 +846,54 @@ def test_combine_sharp_alpha(orc):
     assert cover[10, 10] == 4 and cover[0, 0] == 1 and cover[2, 10] == 2
     assert np.array_equal(sharp[3], np.kron(have, np.ones((8, 8))))
     assert np.array_equal(np.delete(sharp, 3, 0), np.delete(soft, 3, 0))
+
+
+def test_corr_peak_offsets_hand_cases(orc):
+    """Peak offsets of the coarse extrinsic stage (the clt_mismatch measurement, ID:2342-2420, by the
+    centre of mass of mismatchPairsCM, C2:1336-1430, around each pair's integer maximum): worked by
+    hand, with the window (radius 3.9: r^2 <= 15), negative samples, a clipped corner, a tie and an
+    all-negative pair."""
+    c = np.full((6, 15, 15), -0.01)
+    # pair 0: maximum at (x 3, y -2); right and lower neighbours, a negative left one, a sample at
+    # (3, -2) away (r^2 = 13, inside) and one 4 px to the right (r^2 = 16, outside)
+    for (x, y), v in (((3, -2), 1.0), ((4, -2), 0.5), ((3, -1), 0.25), ((2, -2), -0.3), ((6, -4), 0.2), ((7, -2), 0.9)):
+        c[0, 7 + y, 7 + x] = v
+    # pair 1: maximum in the corner (7, 7), neighbours clipped by the border
+    for (x, y), v in (((7, 7), 1.0), ((6, 7), 0.5), ((7, 6), 0.5)):
+        c[1, 7 + y, 7 + x] = v
+    # pair 2: two equal maxima 6 px apart: the first in row order (y = 1) wins, alone in its window
+    c[2, 8, 5] = c[2, 10, 11] = 0.7
+    # pair 3: nothing positive
+    c[3] = -0.2
+    c[3, 0, 0] = -0.1
+    # pairs 4, 5: a single positive sample on the centre and one on the left edge
+    c[4, 7, 7] = 0.3
+    c[5, 2, 0] = 0.6
+    out = orc.corr_peak_offsets(c[None])[0]
+    assert np.allclose(out[0], [139 / 39, -27 / 13, 1.0], atol=1e-14)
+    assert np.allclose(out[1], [6.75, 6.75, 1.0], atol=1e-14)
+    assert np.allclose(out[2], [-2.0, 1.0, 0.7], atol=1e-14)
+    assert np.isnan(out[3, 0]) and np.isnan(out[3, 1]) and out[3, 2] == -0.1
+    assert np.allclose(out[4], [0.0, 0.0, 0.3], atol=1e-14)
+    assert np.allclose(out[5], [-7.0, -5.0, 0.6], atol=1e-14)
+    # the window radius: 7 takes the far sample of pair 0 as well
+    wide = orc.corr_peak_offsets(c[None], radius=7.0)[0, 0]
+    assert np.allclose(wide[:2], [(6.95 + 6.3) / 2.85, (-4.05 - 1.8) / 2.85], atol=1e-14)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_corr_peak_offsets_against_single2d_cm(orc, seed):
+    """On the centre row the window around the integer maximum is single2dCM's (C2:1538-1586, pinned
+    in test_rig.py) centred at xcenter = that maximum, which reports xcenter - xm and -ym."""
+    rng = np.random.default_rng(seed)
+    c = rng.normal(0, 0.2, (4, 6, 15, 15))
+    xs = rng.integers(-6, 7, (4, 6))
+    for t in range(4):
+        for p in range(6):
+            c[t, p, 7, 7 + xs[t, p]] = 2.0
+    out = orc.corr_peak_offsets(c)
+    for t in range(4):
+        for p in range(6):
+            ref = orc.single2d_cm(c[t, p], float(xs[t, p]), 3.9)
+            assert abs(out[t, p, 0] - (xs[t, p] - ref[0])) < 1e-12 and abs(out[t, p, 1] + ref[1]) < 1e-12
+            assert out[t, p, 2] == 2.0
