@@ -21,6 +21,10 @@ struct Tables {
     float d4post[4][2]; // exp(-i pi k / 8): DCT-IV post-twiddle
     float ciie[8][8];   // DCT-IIe                                    (DttRad2.java:952-968)
     float siie[8][8];   // DST-IIe                                    (DttRad2.java:1014-1030)
+    // 1-D inverse lapped transform as a 16x16 matrix: sample t of the 16-long windowed output from
+    // mode m (0: DCT-IV, 1: DST-IV) coefficient k at column m*8+k (the IDCT-IV / IDST-IV, unfold and
+    // window of DttRad2.java:392-434), the operand of the matrix-core inverse CLT
+    float iclt_u[16][16];
 };
 extern __constant__ Tables c_tab;
 

@@ -1776,7 +1776,6 @@ __global__ __launch_bounds__(256) void k_disparity_step(const double* __restrict
 // ====================================================================================
 // LDS texture tiles: rows TEX_ROW floats apart (see TexShared)
 constexpr int TEX_ROW = 20, TEX_TILE = 16 * TEX_ROW;
-static_assert(TEX_TILE <= IMCLT_SCR_UNIT, "a debayered tile fits the unit's IMCLT scratch");
 
 struct TexArgs {
     float col_w[3];
@@ -1954,127 +1953,182 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
     }
 }
 
-// LDS of one texture tile.  scr: the IMCLT scratch of each 16-lane unit; afterwards, when every
-// unit fits one pass (NU <= 16), unit u's debayered tile.  nw: the window-free tiles (the IMCLT
-// tiles divided by the squared window).  Tiles are [unit][TEX_ROW * row + col]: lane l of a unit
-// stores row l with 16-byte stores, and rows TEX_ROW = 20 floats apart put the rows of 8
-// consecutive lanes on distinct banks (16 apart would put every even lane on the same 4 banks);
-// TEX_ROW and TEX_TILE are defined with TexArgs.
+// LDS of one texture tile: the window-free tiles (the IMCLT tiles divided by the squared window)
+// and the debayered tiles, [unit][TEX_ROW * row + col].  Lane (g, n) of a wavefront stores rows
+// 4g .. 4g+3 of column n; rows TEX_ROW = 20 floats apart put the two lane groups of each
+// half-wavefront on distinct banks.  TEX_ROW and TEX_TILE are defined with TexArgs.
 template <int NC>
 struct TexShared {
     static constexpr int NU = NC * 3;
-    static constexpr bool DEB_IN_SCR = NU <= 16;
-    float scr[16][IMCLT_SCR_UNIT];
     float nw[NU][TEX_TILE];
-    float deb[DEB_IN_SCR ? 1 : NU][TEX_TILE];
-    __device__ __forceinline__ float* deb_of(int u) { return DEB_IN_SCR ? &scr[u][0] : &deb[u][0]; }
+    float deb[NU][TEX_TILE];
+    __device__ __forceinline__ float* deb_of(int u) { return &deb[u][0]; }
 };
 
-// Value of the lane one row above / below within the 16-lane unit (DPP row shifts; 0 past the
-// tile's first / last row, which is the zero border of the 3x3 debayer).  Every lane active.
+// Value of the lane one column to the left / right within the 16-lane row (DPP row shifts; 0 past
+// the tile's first / last column, which is the zero border of the 3x3 debayer).  Every lane active.
 __device__ __forceinline__ float unit_row_prev(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
 }
 __device__ __forceinline__ float unit_row_next(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf, false));
 }
+// value of lane (lane + d) mod 64 (LDS crossbar, no LDS memory); every lane active
+__device__ __forceinline__ float lane_from(float v, int lane, int d) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + d) & 63) << 2, __float_as_int(v)));
+}
+
+typedef float float4x __attribute__((ext_vector_type(4)));
+
+// Inverse CLT of a unit on the matrix cores.  The 2-D inverse is out = U X U^T with X the 16x16
+// coefficient matrix (rows (vs, kv), columns (hs, kh)) and U the 1-D inverse (Tables::iclt_u), two
+// chains of four v_mfma_f32_16x16x4_f32 with lane (g, n) = (lane >> 4, lane & 15).  Pass 1
+// (P = X^T U^T, contraction over (vs, kv)): A = X[(vs, kv)][(hs, kh) = n] with vs = g >> 1,
+// kv = 4 (g & 1) + s for step s (four consecutive coefficients: one 16-byte load), B = U[y = n][4g + s];
+// P lands as rows (hs, kh) = 4g + r, column y = n.  Pass 2 (out^T = U P, contraction over (hs, kh))
+// takes P's register s as its B operand for step s (the contraction order follows the rows each
+// lane holds, so no data moves between the passes), with A = U[x = n][4g + s], the same constants;
+// pixel (y = n, x = 4g + r) lands in register r.
+// Per-lane constants of the texture units (set once per block): the matrix operand, LPF * 0.25 of
+// the lane's coefficients (kh = n & 7, kv = 4 (g & 1) + s), and for its pixels the inverse squared
+// window, the squared window, and the renormalisation of the 3x3 debayer kernels for the taps
+// dropped at the tile border (ImageDtt.java:43-50, ID:3048-3147: red/blue 4/3 per border
+// dimension, green 8/7 at a side and 4/3 at a corner).
+struct TexLane {
+    float u[4], lpf[4], nwf[4], w2f[4];
+    bool redge, cedge_lo, cedge_hi;   // the lane's row / its register 0 / register 3 on the tile border
+};
+__device__ __forceinline__ void tex_lane_setup(TexLane& L, const float* __restrict__ lpf) {
+    const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
+    const int qy = n < 8 ? n : 15 - n;
+    const bool redge = n == 0 || n == 15;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        L.u[s] = c_tab.iclt_u[n][4 * g + s];
+        L.lpf[s] = 0.25f * lpf[(n & 7) * 8 + 4 * (g & 1) + s];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int x = 4 * g + r, qx = x < 8 ? x : 15 - x;
+        L.nwf[r] = c_tab.ihw2[qy] * c_tab.ihw2[qx];
+        L.w2f[r] = c_tab.hw2[qy] * c_tab.hw2[qx];
+    }
+    L.redge = redge;
+    L.cedge_lo = g == 0;
+    L.cedge_hi = g == 3;
+}
+// renormalisation of the debayer kernel at pixel (n, 4g + r) of lane L (see TexLane)
+__device__ __forceinline__ float deb_factor(const TexLane& L, int r, bool green) {
+    const bool cedge = (r == 0 && L.cedge_lo) || (r == 3 && L.cedge_hi);
+    if (green) return L.redge ? (cedge ? 4.f / 3.f : 8.f / 7.f) : (cedge ? 8.f / 7.f : 1.f);
+    return (L.redge ? 4.f / 3.f : 1.f) * (cedge ? 4.f / 3.f : 1.f);
+}
 
 // IMCLT of the NC*3 camera/colour units of one tile after LPF * 0.25 (ImageDtt.java:2684-2716),
 // with the optional fract_shift by the extra disparity (ID:2663-2677), then the debayer with
-// shot-noise correction and the re-applied window (tile_debayer_shot_corr, ID:3022-3147) on the
-// unit's lanes: lane l holds output row l, the vertical taps come from the neighbouring lanes, the
-// horizontal ones from the lane's own row; taps outside the tile are dropped and the kernel sum
-// renormalised (corner / side factors).  Leaves S.nw and the debayered tiles in LDS; sink(uu, l,
-// row) receives each unit row of the windowed IMCLT tile (the imclt_rbg tile) while it is in
-// registers.  All 256 threads call this.
-template <int NC, class Sink>
-__device__ __forceinline__ void tex_units(const float* __restrict__ clt, size_t tile, size_t ntiles,
-                                          const float* __restrict__ lpf, const TexArgs& A, float extra, bool do_shift,
-                                          TexShared<NC>& S, Sink sink) {
-    constexpr int NU = NC * 3;
-    const int unit = threadIdx.x >> 4, l = threadIdx.x & 15, hs = l >> 3, kh = l & 7;
-    // 3x3 kernels (ImageDtt.java:43-50) with the renormalisation of the dropped taps folded in:
-    // kern_rb = (1 2 1) x (1 2 1) / 16 (tap sums 9/16 corner, 12/16 side), kern_g = cross (0.75,
-    // 0.875); per lane the factors of an inner (k_in) and an edge column (k_out) of its row
-    const bool row_edge = (l == 0) || (l == 15);
-    const float kshot = fast_rcp(fast_sqrt(A.min_shot));
-    const float wy = c_tab.hw2[l < 8 ? l : 15 - l], iwy = c_tab.ihw2[l < 8 ? l : 15 - l];
-    for (int ub = 0; ub < NU; ub += 16) {
-        if (ub + (threadIdx.x >> 6) * 4 >= NU) continue;   // wavefront without a unit
-        const int uu = ub + unit;
-        const bool act = uu < NU;
-        const int cam = act ? uu / 3 : 0, color = act ? uu % 3 : 0;
-        const float* src = clt + (((size_t)cam * ntiles + tile) * 3 + color) * 256 + kh * 8;
-        float p[8], q[8];
-        {
-            float4 v0 = *reinterpret_cast<const float4*>(src + hs * 64);
-            float4 v1 = *reinterpret_cast<const float4*>(src + hs * 64 + 4);
-            float4 v2 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64);
-            float4 v3 = *reinterpret_cast<const float4*>(src + (hs | 2) * 64 + 4);
-            p[0] = v0.x; p[1] = v0.y; p[2] = v0.z; p[3] = v0.w; p[4] = v1.x; p[5] = v1.y; p[6] = v1.z; p[7] = v1.w;
-            q[0] = v2.x; q[1] = v2.y; q[2] = v2.z; q[3] = v2.w; q[4] = v3.x; q[5] = v3.y; q[6] = v3.z; q[7] = v3.w;
-        }
-        if (do_shift) {   // fract_shift by extra * port_offset / corr_magic_scale (ID:2663-2677)
+// shot-noise correction and the re-applied window (tile_debayer_shot_corr, ID:3022-3147).  Wavefront
+// w takes units w, w + 4, ... (all their loads first, then their matrix-core chains interleaved);
+// the unit's colour is uniform in the wavefront.  Lane (g, n) holds pixels (n, 4g .. 4g+3): the
+// vertical debayer taps come from the neighbouring lanes by DPP, the horizontal ones from the
+// lane's own registers and, at its first / last column, from the lane group to the left / right.
+// x: the coefficients from tex_fetch; next() is called once they are consumed.  Leaves S.nw and the debayered tiles in LDS; sink(uu, j, px)
+// receives the windowed IMCLT tile of unit uu (the wavefront's j-th unit) while it is in registers.
+// All 256 threads call this.
+// Coefficients of the wavefront's units of one tile (see tex_units): lane (g, n) reads mode
+// (n >> 3) | ((g >> 1) << 1), kh = n & 7, kv = 4 (g & 1) .. + 3 (one 16-byte load per unit).
+template <int NC>
+__device__ __forceinline__ void tex_fetch(const float* __restrict__ clt, size_t tile, size_t ntiles,
+                                          float (&x)[(NC * 3 + 3) / 4][4]) {
+    constexpr int NU = NC * 3, J = (NU + 3) / 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const int uu = min(wave + 4 * j, NU - 1);   // a wavefront without a j-th unit repeats its last
+        const int cam = uu / 3, color = uu - 3 * cam;
+        const float4 v = *reinterpret_cast<const float4*>(clt + (((size_t)cam * ntiles + tile) * 3 + color) * 256 +
+                                                         (n >> 3) * 64 + (g >> 1) * 128 + (n & 7) * 8 + 4 * (g & 1));
+        x[j][0] = v.x; x[j][1] = v.y; x[j][2] = v.z; x[j][3] = v.w;
+    }
+}
+
+template <int NC, class Sink, class Next>
+__device__ __forceinline__ void tex_units(float (&x)[(NC * 3 + 3) / 4][4], const TexLane& L, const TexArgs& A,
+                                          float extra, bool do_shift, TexShared<NC>& S, Sink sink, Next next) {
+    constexpr int NU = NC * 3, J = (NU + 3) / 4;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
+    const int hs = n >> 3, kh = n & 7;
+    if (do_shift) {   // fract_shift by extra * port_offset / corr_magic_scale (ID:2663-2677)
+        const bool vodd = g >= 2;   // vs = 1
+        const float sigma = hs ? 1.f : -1.f;
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int cam = min(wave + 4 * j, NU - 1) / 3;
             const float sxp = extra * A.port_offsets[cam][0] / A.corr_magic_scale;
             const float syp = extra * A.port_offsets[cam][1] / A.corr_magic_scale;
-            const float sigma = hs ? 1.f : -1.f;
-            float sh, ch, cv[8], sv[8];
+            float sh, ch;
             sincos_pi((kh + 0.5f) * sxp * 0.125f, &sh, &ch);
-            half_phase8(syp * 0.125f, cv, sv);
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const float pa = lane_xor8(p[k]) * sigma;
-                const float pb = lane_xor8(q[k]) * sigma;
-                const float ta = p[k] * ch + pa * sh;
-                const float tb = q[k] * ch + pb * sh;
-                p[k] = ta * cv[k] - tb * sv[k];
-                q[k] = tb * cv[k] + ta * sv[k];
+            for (int s = 0; s < 4; s++) {
+                // horizontal: the other horizontal mode is lane n ^ 8; vertical: lane (lane + 32) mod 64
+                const float t = x[j][s] * ch + lane_xor8(x[j][s]) * sigma * sh;
+                const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(t), __float_as_uint(t), false, false);
+                const float tw = __uint_as_float(vodd ? sw[0] : sw[1]);
+                float sv, cv;
+                sincos_pi((4 * (g & 1) + s + 0.5f) * syp * 0.125f, &sv, &cv);
+                x[j][s] = vodd ? t * cv + tw * sv : t * cv - tw * sv;
             }
         }
+    }
+    float4x d1[J], d2[J];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const float f = 0.25f * lpf[kh * 8 + k];
-            p[k] *= f;
-            q[k] *= f;
+    for (int j = 0; j < J; j++) {
+        d1[j] = (float4x){0.f, 0.f, 0.f, 0.f};
+        d2[j] = (float4x){0.f, 0.f, 0.f, 0.f};
+    }
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int j = 0; j < J; j++) d1[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[j][s] * L.lpf[s], L.u[s], d1[j], 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int j = 0; j < J; j++) d2[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(L.u[s], d1[j][s], d2[j], 0, 0, 0);
+    next();   // x is free again: the caller may load the next tile's coefficients into it
+    const float kshot = fast_rcp(fast_sqrt(A.min_shot));
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const int uu = wave + 4 * j;
+        if (uu >= NU) break;   // wave-uniform
+        const int color = uu % 3;
+        float px[4] = {d2[j][0], d2[j][1], d2[j][2], d2[j][3]};
+        sink(uu, j, px);
+        float nw[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) nw[r] = px[r] * L.nwf[r];
+        *reinterpret_cast<float4*>(&S.nw[uu][n * TEX_ROW + 4 * g]) = make_float4(nw[0], nw[1], nw[2], nw[3]);
+        // horizontal neighbour sums (columns 4g - 1 and 4g + 4 from the lane groups to the left / right)
+        const float left = lane_from(nw[3], lane, 48), right = lane_from(nw[0], lane, 16);
+        float hsum[4];
+        hsum[0] = (g > 0 ? left : 0.f) + nw[1];
+        hsum[1] = nw[0] + nw[2];
+        hsum[2] = nw[1] + nw[3];
+        hsum[3] = nw[2] + (g < 3 ? right : 0.f);
+        const bool green = color == 2;   // wave-uniform
+        float dv[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float vsum = unit_row_prev(nw[r]) + unit_row_next(nw[r]);
+            const float side = vsum + hsum[r];
+            if (green) {   // kern_g: centre 1/2, sides 1/8
+                dv[r] = deb_factor(L, r, true) * (0.5f * nw[r] + 0.125f * side);
+            } else {       // kern_rb: centre 1/4, sides 1/8, corners 1/16
+                const float corner = unit_row_prev(hsum[r]) + unit_row_next(hsum[r]);
+                dv[r] = deb_factor(L, r, false) * (0.25f * nw[r] + 0.125f * side + 0.0625f * corner);
+            }
+            if (A.scale_shot > 0.f) dv[r] = A.scale_shot * ((dv[r] > A.min_shot) ? fast_sqrt(dv[r]) : (kshot * dv[r]));
+            dv[r] *= L.w2f[r];
         }
-        float row[16];
-        imclt_unit(l, p, q, S.scr[unit], row);   // the unit's scratch is free again on return
-        if (act) sink(uu, l, row);
-        float nw[16];
-#pragma unroll
-        for (int c = 0; c < 16; c++) nw[c] = row[c] * (iwy * c_tab.ihw2[c < 8 ? c : 15 - c]);
-        if (act) {
-#pragma unroll
-            for (int c = 0; c < 16; c += 4)
-                *reinterpret_cast<float4*>(&S.nw[uu][l * TEX_ROW + c]) =
-                    make_float4(nw[c], nw[c + 1], nw[c + 2], nw[c + 3]);
-        }
-        const bool green = color == 2;
-        const float f_in = green ? (row_edge ? 8.f / 7.f : 1.f) : (row_edge ? 4.f / 3.f : 1.f);
-        const float f_out = green ? (row_edge ? 4.f / 3.f : 8.f / 7.f) : (row_edge ? 16.f / 9.f : 4.f / 3.f);
-        const float k_corner = green ? 0.f : 0.0625f, k_side = 0.125f, k_centre = green ? 0.5f : 0.25f;
-        const float kc_in = k_centre * f_in, ks_in = k_side * f_in, kk_in = k_corner * f_in;
-        const float kc_out = k_centre * f_out, ks_out = k_side * f_out, kk_out = k_corner * f_out;
-        float vs[16];   // sum of the pixels above and below
-#pragma unroll
-        for (int c = 0; c < 16; c++) vs[c] = unit_row_prev(nw[c]) + unit_row_next(nw[c]);
-        float deb[16];
-#pragma unroll
-        for (int c = 0; c < 16; c++) {
-            float side = vs[c], corner = 0.f;
-            if (c > 0) { side += nw[c - 1]; corner += vs[c - 1]; }
-            if (c < 15) { side += nw[c + 1]; corner += vs[c + 1]; }
-            const bool edge = (c == 0 || c == 15);
-            float s = (edge ? kc_out : kc_in) * nw[c] + (edge ? ks_out : ks_in) * side + (edge ? kk_out : kk_in) * corner;
-            if (A.scale_shot > 0.f) s = A.scale_shot * ((s > A.min_shot) ? fast_sqrt(s) : (kshot * s));
-            deb[c] = s * (wy * c_tab.hw2[c < 8 ? c : 15 - c]);
-        }
-        if (act) {
-            float* d = S.deb_of(uu);
-#pragma unroll
-            for (int c = 0; c < 16; c += 4)
-                *reinterpret_cast<float4*>(&d[l * TEX_ROW + c]) = make_float4(deb[c], deb[c + 1], deb[c + 2], deb[c + 3]);
-        }
+        *reinterpret_cast<float4*>(&S.deb[uu][n * TEX_ROW + 4 * g]) = make_float4(dv[0], dv[1], dv[2], dv[3]);
     }
 }
 
@@ -2097,9 +2151,9 @@ __device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, Tex
     if (port_mask == (1 << NC) - 1) combine_pixel<NC, true>(A, port_mask, deb, w2, iw2, &S.nw[0][0], pi, out, i, planes, avg);
     else combine_pixel<NC, false>(A, port_mask, deb, w2, iw2, &S.nw[0][0], pi, out, i, planes, avg);
     if (stats_t) {   // max_diff (ImageDtt.java:2965-2981) and ports_rgb (ImageDtt.java:2982-2996)
-        // transpose through LDS: value k of pixel i at [k][i] (the NC max_diff rows reuse scr, the
+        // transpose through LDS: value k of pixel i at [k][i] (the NC max_diff rows reuse deb, the
         // 3*NC colour rows reuse nw); 16 threads then reduce each row
-        float* red_max = &S.scr[0][0];
+        float* red_max = &S.deb[0][0];
         float* red_rgb = &S.nw[0][0];
         __syncthreads();
 #pragma unroll
@@ -2178,32 +2232,37 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
     float extra = extra_disp ? extra_disp[t] : 0.f;
     if (isnan(extra)) extra = 0.f;
     const bool do_shift = (extra != 0.f) && !((flags >> 24) & 1);
-    tex_units<NC>(clt, tile, ntiles, lpf, A, extra, do_shift, S, NoRowSink{});
+    float x[(NC * 3 + 3) / 4][4];
+    tex_fetch<NC>(clt, tile, ntiles, x);
+    TexLane L;
+    tex_lane_setup(L, lpf);
+    tex_units<NC>(x, L, A, extra, do_shift, S, NoRowSink{}, [] {});
     __syncthreads();
     tex_combine<NC>(A, port_mask, S, out, stats_t);
 }
 
 // ====================================================================================
 // textures + imclt_rbg in one pass over the converted tiles: each tile's NC*3 units go through
-// the IMCLT once; the windowed IMCLT rows feed both the texture tile (tex_units / tex_combine) and
+// the IMCLT once; the windowed IMCLT tiles feed both the texture tile (tex_units / tex_combine) and
 // the overlap-add of the (W+8) x (H+8) image planes (imclt_rbg, same LPF * 0.25).
 // A block sweeps one tile column down a segment of seg_rows output block rows, starting one tile
-// above the segment (its bottom half only); the lower half of each tile row stays in the lanes
-// (lane l >= 8 of a unit) until the next tile adds its upper half.  The two halves of a tile's
-// 16 columns overlap the neighbouring columns, so the columns run in two launches: parity 0
-// (even tx) stores its 16 output columns, parity 1 (odd tx) adds to them; when tiles_x is even the
-// last column's right half has no even neighbour and is stored.  Requires NC * 3 <= 16, no extra
-// disparity, and at most one task per tile with every task's flags non-zero (tile_map then names
-// every task exactly once).
+// above the segment (its bottom half only); the lower half of each unit's tile (lanes n >= 8) waits
+// in LDS until the next tile adds its upper half (lanes n < 8).  The two halves
+// of a tile's 16 columns overlap the neighbouring columns, so the columns run in two launches:
+// parity 0 (even tx) stores its 16 output columns, parity 1 (odd tx) adds to them; when tiles_x is
+// even the last column's right half has no even neighbour and is stored.  Requires NC * 3 <= 16, no
+// extra disparity, and at most one task per tile with every task's flags non-zero (tile_map then
+// names every task exactly once).
 // ====================================================================================
 template <int NC>
-__global__ __launch_bounds__(256) void k_tex_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+__global__ __launch_bounds__(256, NC <= 4 ? 4 : 3) void k_tex_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                    int task_words, const int32_t* __restrict__ tile_map,
                                                    const float* __restrict__ lpf, const TexArgs* __restrict__ args_p,
                                                    float* __restrict__ tiles_out, float* __restrict__ stats_out,
                                                    float* __restrict__ rbg, int seg_rows, int parity, GeomDev g,
                                                    long long clt_fstride) {
-    static_assert(NC * 3 <= 16, "one pass of units");
+    static_assert(NC * 3 <= 16, "at most four units per wavefront");
+    constexpr int NU = NC * 3, J = (NU + 3) / 4;
     __shared__ __attribute__((aligned(16))) TexShared<NC> S;
     const TexArgs& A = *args_p;
     const int tx = 2 * (int)blockIdx.x + parity;
@@ -2217,78 +2276,66 @@ __global__ __launch_bounds__(256) void k_tex_imclt(const float* __restrict__ clt
     clt += blockIdx.z * clt_fstride;
     tile_map += blockIdx.z * ntiles;
     rbg += blockIdx.z * (NC * 3) * plane_sz;
-    const int l = threadIdx.x & 15;
+    TexLane L;
+    tex_lane_setup(L, lpf);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, gq = lane >> 4, n = lane & 15;
     // right half stored, not added: the last column when no even column follows it
     const bool add_left = parity == 1;
     const bool add_right = parity == 1 && !(tx == g.tiles_x - 1 && (g.tiles_x & 1) == 0);
-    float carry[16];
-#pragma unroll
-    for (int c = 0; c < 16; c++) carry[c] = 0.f;
+    const bool add_mine = (gq < 2) ? add_left : add_right;   // columns 4 gq .. 4 gq + 3 of the tile
+    // lower halves (rows 8-15) of the previous tile's units, [unit][row - 8][lane group]: written by
+    // lanes n >= 8, read by lanes n < 8 of the same wavefront one tile later (the first tile of a
+    // segment is never stored, so the array needs no clearing)
+    __shared__ float4 s_carry[NU][8][4];
+    // coefficients of the next tile down the column, loaded one tile ahead (rows outside the frame
+    // read a clamped row and are not used)
+    auto row_tile = [&](int ty) { return (size_t)min(max(ty, 0), g.tiles_y - 1) * g.tiles_x + tx; };
+    float x[J][4];
+    tex_fetch<NC>(clt, row_tile(by0 - 1), ntiles, x);
+    int tn = (by0 >= 1) ? tile_map[row_tile(by0 - 1)] : -1;
     for (int ty = by0 - 1; ty < by0 + nby; ty++) {
-        const bool in_frame = ty >= 0 && ty < g.tiles_y;
-        const int t = in_frame ? tile_map[(size_t)ty * g.tiles_x + tx] : -1;
+        const int t = tn;
+        const bool more = ty + 1 < by0 + nby;
+        tn = (more && ty + 1 < g.tiles_y) ? tile_map[row_tile(ty + 1)] : -1;
+        auto next = [&] {
+            if (more) tex_fetch<NC>(clt, row_tile(ty + 1), ntiles, x);
+        };
         const bool own = ty >= by0;   // the segment writes output block row ty
+        // pixels (n, 4 gq .. 4 gq + 3) of the upper half (n < 8): output row ty * 8 + n, columns tx * 8 + 4 gq ..
+        const size_t o0 = (size_t)(max(ty, 0) * 8 + (n & 7)) * out_w + tx * 8 + 4 * gq;
         // the odd columns' read of the even columns' sums, issued before this tile's transforms
-        const int unit = threadIdx.x >> 4;
-        float* o = rbg + (size_t)min(unit, NC * 3 - 1) * plane_sz + (size_t)(max(ty, 0) * 8 + (l & 7)) * out_w + tx * 8;
-        float4 pre[4];
+        float4 pre[J];
 #pragma unroll
-        for (int c = 0; c < 4; c++) pre[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (own && l < 8 && unit < NC * 3) {
-            if (add_left) {
-                pre[0] = *reinterpret_cast<const float4*>(o);
-                pre[1] = *reinterpret_cast<const float4*>(o + 4);
-            }
-            if (add_right) {
-                pre[2] = *reinterpret_cast<const float4*>(o + 8);
-                pre[3] = *reinterpret_cast<const float4*>(o + 12);
-            }
+        for (int j = 0; j < J; j++) {
+            const int uu = min(wave + 4 * j, NU - 1);
+            pre[j] = (own && n < 8 && add_mine && wave + 4 * j < NU)
+                         ? *reinterpret_cast<const float4*>(rbg + uu * plane_sz + o0)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        auto sink = [&](int uu, int ll, const float* row) {
-            if (own && ll < 8) {
-#pragma unroll
-                for (int c = 0; c < 16; c += 4) {
-                    const float4 u = pre[c >> 2];
-                    const float4 w = make_float4(row[c] + carry[c] + u.x, row[c + 1] + carry[c + 1] + u.y,
-                                                 row[c + 2] + carry[c + 2] + u.z, row[c + 3] + carry[c + 3] + u.w);
-                    *reinterpret_cast<float4*>(o + c) = w;
-                }
-            }
+        auto sink = [&](int uu, int j, const float* px) {
+            // the lower half of the tile above, read before this tile's lower half replaces it
+            const float4 up = s_carry[uu][n & 7][gq];
+            if (own && n < 8)
+                *reinterpret_cast<float4*>(rbg + uu * plane_sz + o0) =
+                    make_float4(px[0] + up.x + pre[j].x, px[1] + up.y + pre[j].y, px[2] + up.z + pre[j].z,
+                                px[3] + up.w + pre[j].w);
+            if (n >= 8) s_carry[uu][n & 7][gq] = make_float4(px[0], px[1], px[2], px[3]);
         };
         if (t >= 0) {
             const int32_t* rec = tasks + (size_t)t * task_words;
             const int port_mask = rec[0] & 0xff & ((1 << NC) - 1);
-            tex_units<NC>(clt, (size_t)ty * g.tiles_x + tx, ntiles, lpf, A, 0.f, false, S,
-                          [&](int uu, int ll, const float* row) {
-                              // lane l < 8 adds the lower half of the tile above (lane l + 8)
-                              float up[16];
-#pragma unroll
-                              for (int c = 0; c < 16; c++) up[c] = lane_xor8(carry[c]);
-#pragma unroll
-                              for (int c = 0; c < 16; c++) carry[c] = up[c];
-                              sink(uu, ll, row);
-#pragma unroll
-                              for (int c = 0; c < 16; c++) carry[c] = row[c];
-                          });
+            tex_units<NC>(x, L, A, 0.f, false, S, sink, next);
             __syncthreads();
             if (own && ty < g.tiles_y)
                 tex_combine<NC>(A, port_mask, S, tiles_out + (size_t)t * A.planes * 256,
                                 stats_out ? stats_out + (size_t)t * 4 * NC : nullptr);
             __syncthreads();   // the next tile reuses the LDS
         } else {   // no task tile (or outside the frame): nothing to add but the lower half above
-            if (unit < NC * 3) {
-                float up[16], zero[16];
+            next();
+            const float zero[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int c = 0; c < 16; c++) {
-                    up[c] = lane_xor8(carry[c]);
-                    zero[c] = 0.f;
-                }
-#pragma unroll
-                for (int c = 0; c < 16; c++) carry[c] = up[c];
-                sink(unit, l, zero);
-#pragma unroll
-                for (int c = 0; c < 16; c++) carry[c] = 0.f;
-            }
+            for (int j = 0; j < J; j++)
+                if (wave + 4 * j < NU) sink(wave + 4 * j, j, zero);
         }
     }
 }

@@ -86,6 +86,17 @@ static void make_tables(Tables* t) {
             t->ciie[j][k] = (float)(sc * std::cos(j * (2 * k + 1) * M_PI / 16.0));
             t->siie[j][k] = (float)(sc * std::sin((j + 1) * (2 * k + 1) * M_PI / 16.0));
         }
+    // inverse lapped transform: orthonormal DCT-IV / DST-IV (their own inverses), unfolded to 16
+    // samples with the signs of DttRad2.java:392-403 and the half window
+    for (int x = 0; x < 16; x++) {
+        const int j = unfold_idx(x);
+        const double w = std::sin(M_PI / 16.0 * (unfold_w(x) + 0.5));
+        for (int k = 0; k < 8; k++) {
+            const double a = M_PI / 8.0 * (j + 0.5) * (k + 0.5);
+            t->iclt_u[x][k] = (float)(w * unfold_sc(x) * sc * std::cos(a));
+            t->iclt_u[x][8 + k] = (float)(w * unfold_ss(x) * sc * std::sin(a));
+        }
+    }
 }
 
 // setCltLpf (GPUTileProcessor.java:814-848) followed by dttt_iiie * 2n (GTP:788-798), in double.
