@@ -173,6 +173,7 @@ SIGNATURES = {
     "clt_set_batch_chunks": (_I, [_H, _I]),
     "clt_get_batch_chunks": (_I, [_H, ctypes.POINTER(_I)]),
     "clt_get_tune_all": (_I, [_H, ctypes.POINTER(_I), _I]),
+    "clt_set_tune_all": (_I, [_H, ctypes.POINTER(_I), _I]),
     "clt_get_corr2d_frame": (_I, [_H, _I, _P, _P, _P]),
     "clt_get_textures_frame": (_I, [_H, _I, _P, _P]),
     "clt_get_rbg_frame": (_I, [_H, _I, _I, _P]),

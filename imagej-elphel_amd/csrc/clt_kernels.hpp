@@ -1973,6 +1973,11 @@ __device__ __forceinline__ float unit_row_prev(float v) {
 __device__ __forceinline__ float unit_row_next(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf, false));
 }
+// value of lane (l + R) mod 16 of the lane's 16-lane row (DPP row_ror); the row's lanes active
+template <int R>
+__device__ __forceinline__ float row_ror(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + R, 0xf, 0xf, false));
+}
 // value of lane (lane + d) mod 64 (LDS crossbar, no LDS memory); every lane active
 __device__ __forceinline__ float lane_from(float v, int lane, int d) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + d) & 63) << 2, __float_as_int(v)));
@@ -2173,22 +2178,32 @@ __device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, Tex
         }
         __syncthreads();
         // 4*NC values x 256 pixels: 16 threads per value, each the 16 pixels j, j + 16, ... (the
-        // 16 threads read consecutive words), then a 16-lane reduction
+        // 16 threads read consecutive words), then a reduction over the 16-lane row by DPP row
+        // rotations; a value's kind (maximum or mean) is uniform in its 16 lanes, and in the whole
+        // wavefront when NC is a multiple of 4
         for (int kb = 0; kb < 4 * NC; kb += 16) {
             const int k = kb + (i >> 4), j = i & 15;
-            float v = 0.f;
             if (k < 4 * NC) {
                 const bool is_max = k < NC;
                 const float* r = (is_max ? red_max + k * 256 : red_rgb + (k - NC) * 256) + j;
-                v = r[0];
+                float v = r[0];
+                if (is_max) {
 #pragma unroll
-                for (int q = 1; q < 16; q++) v = is_max ? fmaxf(v, r[16 * q]) : v + r[16 * q];
+                    for (int q = 1; q < 16; q++) v = fmaxf(v, r[16 * q]);
+                    v = fmaxf(v, row_ror<8>(v));
+                    v = fmaxf(v, row_ror<4>(v));
+                    v = fmaxf(v, row_ror<2>(v));
+                    v = fmaxf(v, row_ror<1>(v));
+                    if (j == 0) stats_t[k] = sqrtf(v);
+                } else {
 #pragma unroll
-                for (int o = 8; o > 0; o >>= 1) {
-                    const float u = __shfl_xor(v, o, 16);
-                    v = is_max ? fmaxf(v, u) : v + u;
+                    for (int q = 1; q < 16; q++) v += r[16 * q];
+                    v += row_ror<8>(v);
+                    v += row_ror<4>(v);
+                    v += row_ror<2>(v);
+                    v += row_ror<1>(v);
+                    if (j == 0) stats_t[k] = v * (1.f / 256.f);
                 }
-                if (j == 0) stats_t[k] = is_max ? sqrtf(v) : v / 256.f;
             }
         }
     }

@@ -2493,6 +2493,20 @@ int clt_get_tune_all(clt_handle* h, int32_t* values, int32_t n) {
     return CLT_OK;
 }
 
+int clt_set_tune_all(clt_handle* h, const int32_t* values, int32_t n) {
+    if (!h || (!values && n > 0) || n < 0 || n > 4) return CLT_E_ARG;
+    const int32_t lo[4] = {1, 1, 1, 1}, hi[4] = {4, 64, 64, 64};
+    for (int i = 0; i < n; i++)
+        if (values[i] < lo[i] || values[i] > hi[i] || (i == 0 && values[i] == 3))
+            return fail(h, CLT_E_ARG, "clt_set_tune_all: value %d = %d", i, values[i]);
+    if (n > 0) h->tune_corr_tpb = values[0];
+    if (n > 1) h->tune_imclt_rows = values[1];
+    if (n > 2) h->tune_chunks = values[2];
+    if (n > 3) h->tex_imclt_rows = values[3];
+    h->graph_epoch++;
+    return CLT_OK;
+}
+
 int clt_get_batch_chunks(clt_handle* h, int32_t* chunks) {
     if (!h || !chunks) return CLT_E_ARG;
     *chunks = std::max(1, std::min(h->tune_chunks, h->num_frames));

@@ -469,6 +469,25 @@ class GPUTileProcessor:
         return ext, disp, hist[:its.value + 1]
 
     # ------------------------------------------------------------------ execution control
+    TUNE_KNOBS = ("corr_tiles_per_block", "imclt_rows", "chunks", "tex_imclt_rows")
+
+    def getTune(self):
+        """Launch shapes in use (clt_get_tune_all): correlate2D tiles per workgroup, imclt rows per
+        segment, batch frame chunks, one-pass textures/imclt rows per segment."""
+        v = (ctypes.c_int32 * 4)()
+        self._chk(self.lib.clt_get_tune_all(self.h, v, 4))
+        return dict(zip(self.TUNE_KNOBS, v))
+
+    def setTune(self, **knobs):
+        """Pins launch shapes (clt_set_tune_all), e.g. the result of GPUTileBatch.autotune()."""
+        unknown = set(knobs) - set(self.TUNE_KNOBS)
+        if unknown:
+            raise ValueError("unknown tuning knobs: %s" % sorted(unknown))
+        cur = self.getTune()
+        cur.update(knobs)
+        v = (ctypes.c_int32 * 4)(*[int(cur[k]) for k in self.TUNE_KNOBS])
+        self._chk(self.lib.clt_set_tune_all(self.h, v, 4))
+
     def setAsync(self, on):
         self._chk(self.lib.clt_set_async(self.h, int(on)))
 
@@ -517,11 +536,6 @@ class GPUTileBatch(GPUTileProcessor):
     def autotune(self):
         self._chk(self.lib.clt_autotune(self.h))
         return self.getTune()
-
-    def getTune(self):
-        v = (ctypes.c_int32 * 4)()
-        self._chk(self.lib.clt_get_tune_all(self.h, v, 4))
-        return {"corr_tiles_per_block": v[0], "imclt_rows": v[1], "chunks": v[2], "tex_imclt_rows": v[3]}
 
     def setBatchChunks(self, chunks):
         """Frame chunks of the pipelined batch (clt_autotune picks one of 1, 2, 4, 8)."""

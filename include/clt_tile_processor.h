@@ -473,6 +473,10 @@ int  clt_autotune(clt_handle* h);
 int  clt_get_tune(clt_handle* h, int32_t* corr_tiles_per_block, int32_t* imclt_rows);
 /* values[0..n-1] (n <= 4): correlate2D tiles per workgroup, imclt rows, frame chunks, one-pass rows */
 int  clt_get_tune_all(clt_handle* h, int32_t* values, int32_t n);
+/* Sets the first n (<= 4) of the same knobs, e.g. to reuse a clt_autotune result on another handle
+ * or to pin the per-frame path's launch shapes: correlate2D tiles per workgroup 1, 2 or 4; rows
+ * 1..64; chunks 1..64.  No reference counterpart (the reference's launch shapes are fixed). */
+int  clt_set_tune_all(clt_handle* h, const int32_t* values, int32_t n);
 /* chunks 1..64 (used as min(chunks, num_frames)); 1 until clt_autotune or this call. */
 int  clt_set_batch_chunks(clt_handle* h, int32_t chunks);
 int  clt_get_batch_chunks(clt_handle* h, int32_t* chunks);
