@@ -28,6 +28,8 @@ struct Tables {
 };
 extern __constant__ Tables c_tab;
 
+typedef float float4x __attribute__((ext_vector_type(4)));   // v_mfma_f32_16x16x4_f32 accumulator
+
 struct GeomDev {
     int width, height, cams, colors, red_row, red_col;
     int kh, kv, kstep;            // kernel grid and step

@@ -1023,8 +1023,58 @@ __device__ __forceinline__ void corr_fd_to_tile(const float* acc, float lp, floa
     if (xx > 0 && y > 0) { co[o_mm] = v_mm; cs[o_mm] = v_mm; }
 }
 
+// The same on the matrix cores: the four quadrants as one 16x16 matrix Q[(qv, kv)][(qh, kh)]
+// (q = 2 qv + qh, qv / qh: DST-IIe vertically / horizontally), out = By Q Bx^T with the
+// block-diagonal By = Bx = diag(DCT-IIe, DST-IIe); two chains of four v_mfma_f32_16x16x4_f32.  Lane
+// (g, n) = (lane >> 4, lane & 15): pass 1 (contraction over (qv, kv)) takes A = Q^T[(qh, kh) = n]
+// [(qv, kv) = (g >> 1, 4 (g & 1) + s)] (one 16-byte LDS read) and B = u[s], the block-diagonal
+// matrix at [row n][column 4g + s]; pass 2 (contraction over (qh, kh)) takes A = u[s] and B = pass
+// 1's register s.  The result, (qh, x) = (g >> 1, 4 (g & 1) + r) and (qv, y) = (n >> 3, n & 7) in
+// register r, goes back to S as [q][y][x] (one 16-byte write) for the unfold.
+__device__ __forceinline__ void corr_fd_to_tile_mx(const float* acc, float lp, float* S, const float (&u)[4], int lane,
+                                                   float* co, float* cs) {
+    const int y = lane >> 3, xx = lane & 7, g = lane >> 4, n = lane & 15;
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 4; q++) S[q * 64 + lane] = acc[q] * lp;
+    wave_sync();
+    const float4 a = *reinterpret_cast<const float4*>(S + (2 * (g >> 1) + (n >> 3)) * 64 + (n & 7) * 8 + 4 * (g & 1));
+    const float av[4] = {a.x, a.y, a.z, a.w};
+    float4x d1 = {0.f, 0.f, 0.f, 0.f}, d2 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; s++) d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], u[s], d1, 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 4; s++) d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(u[s], d1[s], d2, 0, 0, 0);
+    wave_sync();
+    *reinterpret_cast<float4*>(S + (2 * (n >> 3) + (g >> 1)) * 64 + (n & 7) * 8 + 4 * (g & 1)) =
+        make_float4(d2[0], d2[1], d2[2], d2[3]);
+    wave_sync();
+    // unfold to 15x15: CC[ay][ax] +- SC[ay][ax-1] +- CS[ay-1][ax] +- SS[ay-1][ax-1]; the lane (ay, ax) =
+    // (y, xx) writes (7 +- y, 7 +- xx)
+    const float va = S[y * 8 + xx];
+    const float vb = xx > 0 ? S[64 + y * 8 + xx - 1] : 0.f;
+    const float vc = y > 0 ? S[128 + (y - 1) * 8 + xx] : 0.f;
+    const float vd = (xx > 0 && y > 0) ? S[192 + (y - 1) * 8 + xx - 1] : 0.f;
+    const float v_pp = 0.25f * (va + vb + vc + vd), v_pm = 0.25f * (va - vb + vc - vd);
+    const float v_mp = 0.25f * (va + vb - vc - vd), v_mm = 0.25f * (va - vb - vc + vd);
+    const int o_pp = (7 + y) * 15 + 7 + xx, o_pm = (7 + y) * 15 + 7 - xx;
+    const int o_mp = (7 - y) * 15 + 7 + xx, o_mm = (7 - y) * 15 + 7 - xx;
+    co[o_pp] = v_pp;
+    cs[o_pp] = v_pp;
+    if (xx > 0) { co[o_pm] = v_pm; cs[o_pm] = v_pm; }
+    if (y > 0) { co[o_mp] = v_mp; cs[o_mp] = v_mp; }
+    if (xx > 0 && y > 0) { co[o_mm] = v_mm; cs[o_mm] = v_mm; }
+}
+// per-lane operand of corr_fd_to_tile_mx: the block-diagonal DCT-IIe / DST-IIe at [n][4g + s]
+__device__ __forceinline__ void corr_mx_operand(int lane, float (&u)[4]) {
+    const int g = lane >> 4, n = lane & 15;
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+        u[s] = ((n >> 3) != (g >> 1)) ? 0.f : ((g >> 1) ? c_tab.siie[n & 7][4 * (g & 1) + s] : c_tab.ciie[n & 7][4 * (g & 1) + s]);
+}
+
 template <bool FZ>   // FZ: fat zero in use (the per-bin normaliser needs the mean over the bins)
-__global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+__global__ __launch_bounds__(256, FZ ? 4 : 5) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                      int task_words, int num_tasks, const float* __restrict__ lpf,
                                                      const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
                                                      float* __restrict__ combo_out, float* __restrict__ disp_out,
@@ -1091,15 +1141,8 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
             }
         }
     }
-    const int y = lane >> 3, xx = lane & 7;
-    float cy_[8], sy_[8], cx_[8], sx_[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        cy_[k] = c_tab.ciie[y][k];
-        sy_[k] = c_tab.siie[y][k];
-        cx_[k] = c_tab.ciie[xx][k];
-        sx_[k] = c_tab.siie[xx][k];
-    }
+    float umx[4];
+    corr_mx_operand(lane, umx);
     const float lp = lpf[lane];
     float* corr_t = corr_out + (size_t)t * 6 * CLT_CORR_LEN;
 #pragma unroll
@@ -1126,7 +1169,7 @@ __global__ __launch_bounds__(256) void k_corr2d_quad(const float* __restrict__ c
             acc[2] = fmaf(t2, scale, acc[2]);
             acc[3] = fmaf(t3, scale, acc[3]);
         }
-        corr_fd_to_tile(acc, lp, S, cy_, sy_, cx_, sx_, lane, corr_t + p * CLT_CORR_LEN, s_corr + p * CLT_CORR_LEN);
+        corr_fd_to_tile_mx(acc, lp, S, umx, lane, corr_t + p * CLT_CORR_LEN, s_corr + p * CLT_CORR_LEN);
     }
     wave_sync();
     // ---- combined strip (C2:730-790) of the 6 rotated strips (C2:556-718)
@@ -1982,8 +2025,6 @@ __device__ __forceinline__ float row_ror(float v) {
 __device__ __forceinline__ float lane_from(float v, int lane, int d) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(((lane + d) & 63) << 2, __float_as_int(v)));
 }
-
-typedef float float4x __attribute__((ext_vector_type(4)));
 
 // Inverse CLT of a unit on the matrix cores.  The 2-D inverse is out = U X U^T with X the 16x16
 // coefficient matrix (rows (vs, kv), columns (hs, kh)) and U the 1-D inverse (Tables::iclt_u), two
