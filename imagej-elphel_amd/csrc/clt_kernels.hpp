@@ -540,7 +540,7 @@ __device__ __forceinline__ float wave_sum(float v) {
 
 // LDS floats per wavefront of k_corr2d
 __host__ __device__ inline int corr_wave_floats(int num_strip) {
-    return 512 + num_strip * CLT_CORR_LEN + 240 + 32 + 3;
+    return 256 + num_strip * CLT_CORR_LEN + 240 + 32 + 3;
 }
 
 // x^(1/n) for x > 0 with the hardware log2/exp2
@@ -590,6 +590,65 @@ constexpr int QP_LG[6] = {1, 1, 1, 1, 0, 0};
 
 // QUAD: 4 cameras, 3 colours, the 6 reference pairs -- the tile's CLT is held in registers and
 // every pair/strip index is a compile-time constant; otherwise the pair table comes from CorrArgs.
+// 2-D DCT-IIe / DST-IIe of the four quadrants of a pair's product (C2:316-319, DR:595-614) on the
+// matrix cores, in place in the wavefront's 256 floats of LDS at S ([q][kh * 8 + kv] in, [q][y][x]
+// out): the four quadrants as one 16x16 matrix Q[(qv, kv)][(qh, kh)]
+// (q = 2 qv + qh, qv / qh: DST-IIe vertically / horizontally), out = By Q Bx^T with the
+// block-diagonal By = Bx = diag(DCT-IIe, DST-IIe); two chains of four v_mfma_f32_16x16x4_f32.  Lane
+// (g, n) = (lane >> 4, lane & 15): pass 1 (contraction over (qv, kv)) takes A = Q^T[(qh, kh) = n]
+// [(qv, kv) = (g >> 1, 4 (g & 1) + s)] (one 16-byte LDS read) and B = u[s], the block-diagonal
+// matrix at [row n][column 4g + s]; pass 2 (contraction over (qh, kh)) takes A = u[s] and B = pass
+// 1's register s.  The result, (qh, x) = (g >> 1, 4 (g & 1) + r) and (qv, y) = (n >> 3, n & 7) in
+// register r, goes back to S as [q][y][x] (one 16-byte write) for the unfold.
+__device__ __forceinline__ void corr_dct_mx(float* S, const float (&u)[4], int lane) {
+    const int g = lane >> 4, n = lane & 15;
+    const float4 a = *reinterpret_cast<const float4*>(S + (2 * (g >> 1) + (n >> 3)) * 64 + (n & 7) * 8 + 4 * (g & 1));
+    const float av[4] = {a.x, a.y, a.z, a.w};
+    float4x d1 = {0.f, 0.f, 0.f, 0.f}, d2 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; s++) d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], u[s], d1, 0, 0, 0);
+#pragma unroll
+    for (int s = 0; s < 4; s++) d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(u[s], d1[s], d2, 0, 0, 0);
+    wave_sync();
+    *reinterpret_cast<float4*>(S + (2 * (n >> 3) + (g >> 1)) * 64 + (n & 7) * 8 + 4 * (g & 1)) =
+        make_float4(d2[0], d2[1], d2[2], d2[3]);
+    wave_sync();
+}
+// One pair's correlation from its frequency-domain product (one bin per lane, the four modes in
+// acc): LPF, the 2-D DCT-IIe / DST-IIe above through the wavefront's 256 floats of LDS at S, unfold
+// to the 15x15 tile (DttRad2.java:443-489), written to co (global) and cs (LDS).
+__device__ __forceinline__ void corr_fd_to_tile_mx(const float* acc, float lp, float* S, const float (&u)[4], int lane,
+                                                   float* co, float* cs) {
+    const int y = lane >> 3, xx = lane & 7;
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 4; q++) S[q * 64 + lane] = acc[q] * lp;
+    wave_sync();
+    corr_dct_mx(S, u, lane);
+    // unfold to 15x15: CC[ay][ax] +- SC[ay][ax-1] +- CS[ay-1][ax] +- SS[ay-1][ax-1]; the lane (ay, ax) =
+    // (y, xx) writes (7 +- y, 7 +- xx)
+    const float va = S[y * 8 + xx];
+    const float vb = xx > 0 ? S[64 + y * 8 + xx - 1] : 0.f;
+    const float vc = y > 0 ? S[128 + (y - 1) * 8 + xx] : 0.f;
+    const float vd = (xx > 0 && y > 0) ? S[192 + (y - 1) * 8 + xx - 1] : 0.f;
+    const float v_pp = 0.25f * (va + vb + vc + vd), v_pm = 0.25f * (va - vb + vc - vd);
+    const float v_mp = 0.25f * (va + vb - vc - vd), v_mm = 0.25f * (va - vb - vc + vd);
+    const int o_pp = (7 + y) * 15 + 7 + xx, o_pm = (7 + y) * 15 + 7 - xx;
+    const int o_mp = (7 - y) * 15 + 7 + xx, o_mm = (7 - y) * 15 + 7 - xx;
+    co[o_pp] = v_pp;
+    cs[o_pp] = v_pp;
+    if (xx > 0) { co[o_pm] = v_pm; cs[o_pm] = v_pm; }
+    if (y > 0) { co[o_mp] = v_mp; cs[o_mp] = v_mp; }
+    if (xx > 0 && y > 0) { co[o_mm] = v_mm; cs[o_mm] = v_mm; }
+}
+// per-lane operand of corr_dct_mx: the block-diagonal DCT-IIe / DST-IIe at [n][4g + s]
+__device__ __forceinline__ void corr_mx_operand(int lane, float (&u)[4]) {
+    const int g = lane >> 4, n = lane & 15;
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+        u[s] = ((n >> 3) != (g >> 1)) ? 0.f : ((g >> 1) ? c_tab.siie[n & 7][4 * (g & 1) + s] : c_tab.ciie[n & 7][4 * (g & 1) + s]);
+}
+
 template <bool QUAD>
 __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                 int task_words, int num_tasks, const float* __restrict__ lpf,
@@ -606,8 +665,7 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
     if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
     const int wfl = (corr_wave_floats(NS) + 3) & ~3;
     float* S0 = s_dyn + wave * wfl;                   // [256]
-    float* S1 = S0 + 256;                             // [256]
-    float* s_corr = S1 + 256;                         // [NS][225]
+    float* s_corr = S0 + 256;                         // [NS][225]
     float* s_combo = s_corr + NS * CLT_CORR_LEN;      // [240]
     float* s_c1d = s_combo + 240;                     // [2][16]
     const float qnan = __int_as_float(0x7fc00000);
@@ -639,16 +697,8 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
 #pragma unroll
                 for (int n = 0; n < 4; n++) D[c][col][n] = clt[(((size_t)c * ntiles + tile) * 3 + col) * 256 + n * 64 + lane];
     }
-    // per-lane DCT-IIe / DST-IIe rows: pass 1 uses row y = lane >> 3, pass 2 row x = lane & 7
-    const int y = lane >> 3, xx = lane & 7;
-    float cy_[8], sy_[8], cx_[8], sx_[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        cy_[k] = c_tab.ciie[y][k];
-        sy_[k] = c_tab.siie[y][k];
-        cx_[k] = c_tab.ciie[xx][k];
-        sx_[k] = c_tab.siie[xx][k];
-    }
+    float umx[4];   // operand of the matrix-core DCT-IIe (corr_dct_mx)
+    corr_mx_operand(lane, umx);
     const float lp = lpf[lane];
     float* corr_t = corr_out + (size_t)t * P * CLT_CORR_LEN;
 #pragma unroll
@@ -693,38 +743,7 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
 #pragma unroll
             for (int n = 0; n < 4; n++) S0[n * 64 + lane] = acc[n] * lp;
             wave_sync();
-            // pass 1 along kv (contiguous index): lane (y, kh); quadrant q uses DST-IIe if q bit 1
-            {
-                const int khh = lane & 7;
-                float tq[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const float* src = S0 + q * 64 + khh * 8;
-                    float s = 0.f;
-#pragma unroll
-                    for (int k = 0; k < 8; k++) s = fmaf((q & 2) ? sy_[k] : cy_[k], src[k], s);
-                    tq[q] = s;
-                }
-#pragma unroll
-                for (int q = 0; q < 4; q++) S1[q * 64 + y * 8 + khh] = tq[q];
-            }
-            wave_sync();
-            // pass 2 along kh: lane (y, x); quadrant q uses DST-IIe if q bit 0
-            {
-                float oq[4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const float* src = S1 + q * 64 + y * 8;
-                    float s = 0.f;
-#pragma unroll
-                    for (int k = 0; k < 8; k++) s = fmaf((q & 1) ? sx_[k] : cx_[k], src[k], s);
-                    oq[q] = s;
-                }
-                wave_sync();
-#pragma unroll
-                for (int q = 0; q < 4; q++) S0[q * 64 + lane] = oq[q];
-            }
-            wave_sync();
+            corr_dct_mx(S0, umx, lane);
             // unfold the 4 quadrants to 15x15 (DttRad2.java:443-489)
             const int slot = QUAD ? p : A.strip_slot[pp];
 #pragma unroll
@@ -971,107 +990,6 @@ __device__ __forceinline__ int argmax16(float v, int idx) {
     return idx;
 }
 
-// One pair's correlation from its frequency-domain product (one bin per lane, the four modes in
-// acc): LPF, DCT-IIe/DST-IIe of each quadrant (C2:316-319, DR:595-614) through the wavefront's 256
-// floats of LDS at S, unfold to the 15x15 tile (DttRad2.java:443-489), written to co (global) and
-// cs (LDS).  cy_/sy_: DCT-IIe/DST-IIe rows of the lane's y = lane >> 3, cx_/sx_ of x = lane & 7.
-__device__ __forceinline__ void corr_fd_to_tile(const float* acc, float lp, float* S, const float* cy_, const float* sy_,
-                                                const float* cx_, const float* sx_, int lane, float* co, float* cs) {
-    const int y = lane >> 3, xx = lane & 7;
-    wave_sync();
-#pragma unroll
-    for (int n = 0; n < 4; n++) S[n * 64 + lane] = acc[n] * lp;
-    wave_sync();
-    // pass 1 along kv: lane (y, kh); quadrant q uses DST-IIe if q bit 1
-    float tq[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const float4 u0 = *reinterpret_cast<const float4*>(S + q * 64 + xx * 8);
-        const float4 u1 = *reinterpret_cast<const float4*>(S + q * 64 + xx * 8 + 4);
-        const float* m = (q & 2) ? sy_ : cy_;
-        tq[q] = m[0] * u0.x + m[1] * u0.y + m[2] * u0.z + m[3] * u0.w + m[4] * u1.x + m[5] * u1.y + m[6] * u1.z + m[7] * u1.w;
-    }
-    wave_sync();
-#pragma unroll
-    for (int q = 0; q < 4; q++) S[q * 64 + y * 8 + xx] = tq[q];
-    wave_sync();
-    // pass 2 along kh: lane (y, x); quadrant q uses DST-IIe if q bit 0
-    float oq[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const float4 u0 = *reinterpret_cast<const float4*>(S + q * 64 + y * 8);
-        const float4 u1 = *reinterpret_cast<const float4*>(S + q * 64 + y * 8 + 4);
-        const float* m = (q & 1) ? sx_ : cx_;
-        oq[q] = m[0] * u0.x + m[1] * u0.y + m[2] * u0.z + m[3] * u0.w + m[4] * u1.x + m[5] * u1.y + m[6] * u1.z + m[7] * u1.w;
-    }
-    // unfold to 15x15: CC[ay][ax] +- SC[ay][ax-1] +- CS[ay-1][ax] +- SS[ay-1][ax-1]; the lane (ay, ax) =
-    // (y, xx) writes (7 +- y, 7 +- xx)
-    const float a = oq[0];
-    float b = __shfl(oq[1], (lane + 63) & 63);
-    float c = __shfl(oq[2], (lane + 56) & 63);
-    float d = __shfl(oq[3], (lane + 55) & 63);
-    if (xx == 0) { b = 0.f; d = 0.f; }
-    if (y == 0) { c = 0.f; d = 0.f; }
-    const float v_pp = 0.25f * (a + b + c + d), v_pm = 0.25f * (a - b + c - d);
-    const float v_mp = 0.25f * (a + b - c - d), v_mm = 0.25f * (a - b - c + d);
-    const int o_pp = (7 + y) * 15 + 7 + xx, o_pm = (7 + y) * 15 + 7 - xx;
-    const int o_mp = (7 - y) * 15 + 7 + xx, o_mm = (7 - y) * 15 + 7 - xx;
-    co[o_pp] = v_pp;
-    cs[o_pp] = v_pp;
-    if (xx > 0) { co[o_pm] = v_pm; cs[o_pm] = v_pm; }
-    if (y > 0) { co[o_mp] = v_mp; cs[o_mp] = v_mp; }
-    if (xx > 0 && y > 0) { co[o_mm] = v_mm; cs[o_mm] = v_mm; }
-}
-
-// The same on the matrix cores: the four quadrants as one 16x16 matrix Q[(qv, kv)][(qh, kh)]
-// (q = 2 qv + qh, qv / qh: DST-IIe vertically / horizontally), out = By Q Bx^T with the
-// block-diagonal By = Bx = diag(DCT-IIe, DST-IIe); two chains of four v_mfma_f32_16x16x4_f32.  Lane
-// (g, n) = (lane >> 4, lane & 15): pass 1 (contraction over (qv, kv)) takes A = Q^T[(qh, kh) = n]
-// [(qv, kv) = (g >> 1, 4 (g & 1) + s)] (one 16-byte LDS read) and B = u[s], the block-diagonal
-// matrix at [row n][column 4g + s]; pass 2 (contraction over (qh, kh)) takes A = u[s] and B = pass
-// 1's register s.  The result, (qh, x) = (g >> 1, 4 (g & 1) + r) and (qv, y) = (n >> 3, n & 7) in
-// register r, goes back to S as [q][y][x] (one 16-byte write) for the unfold.
-__device__ __forceinline__ void corr_fd_to_tile_mx(const float* acc, float lp, float* S, const float (&u)[4], int lane,
-                                                   float* co, float* cs) {
-    const int y = lane >> 3, xx = lane & 7, g = lane >> 4, n = lane & 15;
-    wave_sync();
-#pragma unroll
-    for (int q = 0; q < 4; q++) S[q * 64 + lane] = acc[q] * lp;
-    wave_sync();
-    const float4 a = *reinterpret_cast<const float4*>(S + (2 * (g >> 1) + (n >> 3)) * 64 + (n & 7) * 8 + 4 * (g & 1));
-    const float av[4] = {a.x, a.y, a.z, a.w};
-    float4x d1 = {0.f, 0.f, 0.f, 0.f}, d2 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int s = 0; s < 4; s++) d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], u[s], d1, 0, 0, 0);
-#pragma unroll
-    for (int s = 0; s < 4; s++) d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(u[s], d1[s], d2, 0, 0, 0);
-    wave_sync();
-    *reinterpret_cast<float4*>(S + (2 * (n >> 3) + (g >> 1)) * 64 + (n & 7) * 8 + 4 * (g & 1)) =
-        make_float4(d2[0], d2[1], d2[2], d2[3]);
-    wave_sync();
-    // unfold to 15x15: CC[ay][ax] +- SC[ay][ax-1] +- CS[ay-1][ax] +- SS[ay-1][ax-1]; the lane (ay, ax) =
-    // (y, xx) writes (7 +- y, 7 +- xx)
-    const float va = S[y * 8 + xx];
-    const float vb = xx > 0 ? S[64 + y * 8 + xx - 1] : 0.f;
-    const float vc = y > 0 ? S[128 + (y - 1) * 8 + xx] : 0.f;
-    const float vd = (xx > 0 && y > 0) ? S[192 + (y - 1) * 8 + xx - 1] : 0.f;
-    const float v_pp = 0.25f * (va + vb + vc + vd), v_pm = 0.25f * (va - vb + vc - vd);
-    const float v_mp = 0.25f * (va + vb - vc - vd), v_mm = 0.25f * (va - vb - vc + vd);
-    const int o_pp = (7 + y) * 15 + 7 + xx, o_pm = (7 + y) * 15 + 7 - xx;
-    const int o_mp = (7 - y) * 15 + 7 + xx, o_mm = (7 - y) * 15 + 7 - xx;
-    co[o_pp] = v_pp;
-    cs[o_pp] = v_pp;
-    if (xx > 0) { co[o_pm] = v_pm; cs[o_pm] = v_pm; }
-    if (y > 0) { co[o_mp] = v_mp; cs[o_mp] = v_mp; }
-    if (xx > 0 && y > 0) { co[o_mm] = v_mm; cs[o_mm] = v_mm; }
-}
-// per-lane operand of corr_fd_to_tile_mx: the block-diagonal DCT-IIe / DST-IIe at [n][4g + s]
-__device__ __forceinline__ void corr_mx_operand(int lane, float (&u)[4]) {
-    const int g = lane >> 4, n = lane & 15;
-#pragma unroll
-    for (int s = 0; s < 4; s++)
-        u[s] = ((n >> 3) != (g >> 1)) ? 0.f : ((g >> 1) ? c_tab.siie[n & 7][4 * (g & 1) + s] : c_tab.ciie[n & 7][4 * (g & 1) + s]);
-}
 
 template <bool FZ>   // FZ: fat zero in use (the per-bin normaliser needs the mean over the bins)
 __global__ __launch_bounds__(256, FZ ? 4 : 5) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,

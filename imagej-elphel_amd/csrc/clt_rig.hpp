@@ -79,16 +79,9 @@ __global__ __launch_bounds__(256) void k_corr_inter(const float* __restrict__ mi
             acc[2] = fmaf(t2, scale, acc[2]);
             acc[3] = fmaf(t3, scale, acc[3]);
         }
-        const int y = lane >> 3, xx = lane & 7;
-        float cy_[8], sy_[8], cx_[8], sx_[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            cy_[k] = c_tab.ciie[y][k];
-            sy_[k] = c_tab.siie[y][k];
-            cx_[k] = c_tab.ciie[xx][k];
-            sx_[k] = c_tab.siie[xx][k];
-        }
-        corr_fd_to_tile(acc, lpf[lane], S, cy_, sy_, cx_, sx_, lane, corr_out + (size_t)t * CLT_CORR_LEN, s_corr);
+        float umx[4];
+        corr_mx_operand(lane, umx);
+        corr_fd_to_tile_mx(acc, lpf[lane], S, umx, lane, corr_out + (size_t)t * CLT_CORR_LEN, s_corr);
     } else {
         for (int o = lane; o < CLT_CORR_LEN; o += 64) s_corr[o] = src[(size_t)t * CLT_CORR_LEN + o];
     }
