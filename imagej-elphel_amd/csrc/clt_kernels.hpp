@@ -59,7 +59,7 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
                                                  long long clt_fstride, GeomDev g) {
     __shared__ __attribute__((aligned(16))) float s_buf[12][CONV_UNIT];
     const int blk = xcd_block(blockIdx.x, gridDim.x);
-    const int color = threadIdx.x >> 6;                     // wave-uniform
+    const int color = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     const int unit_in_block = threadIdx.x >> 4, l = threadIdx.x & 15;
     const int total = num_tasks * g.cams;
     int u = blk * 4 + ((threadIdx.x >> 4) & 3);             // (task, camera) unit
@@ -660,7 +660,7 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
     const int K = QUAD ? 3 : g.colors;
     const int P = QUAD ? 6 : A.num_pairs;
     const int NS = QUAD ? 6 : A.num_strip;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int t = xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
     if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
     const int wfl = (corr_wave_floats(NS) + 3) & ~3;
@@ -1000,7 +1000,7 @@ __global__ __launch_bounds__(256, FZ ? 4 : 5) void k_corr2d_quad(const float* __
                                                      GeomDev g, float* __restrict__ mix_out) {
     extern __shared__ __attribute__((aligned(16))) float s_dyn[];
     const CorrArgs& A = *args_p;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int t = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wave;   // one tile per wavefront
     if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
     float* S = s_dyn + wave * CQ_WAVE_FLOATS;         // [4][64] transform staging
@@ -1258,7 +1258,7 @@ __global__ __launch_bounds__(192) void k_corr_adjoint_quad(const float* __restri
     const CorrArgs& A = *args_p;
     const int t = xcd_block(blockIdx.x, gridDim.x);
     if (t >= num_tasks) return;                              // block-uniform
-    const int color = threadIdx.x >> 6;                     // wave = colour
+    const int color = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave = colour
     const int unit_in_block = threadIdx.x >> 4, l = threadIdx.x & 15;
     const int cam = (threadIdx.x >> 4) & 3;
     const int32_t* rec = tasks + (size_t)t * task_words;
@@ -2004,7 +2004,7 @@ template <int NC>
 __device__ __forceinline__ void tex_fetch(const float* __restrict__ clt, size_t tile, size_t ntiles,
                                           float (&x)[(NC * 3 + 3) / 4][4]) {
     constexpr int NU = NC * 3, J = (NU + 3) / 4;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
 #pragma unroll
     for (int j = 0; j < J; j++) {
         const int uu = min(wave + 4 * j, NU - 1);   // a wavefront without a j-th unit repeats its last
@@ -2019,7 +2019,7 @@ template <int NC, class Sink, class Next>
 __device__ __forceinline__ void tex_units(float (&x)[(NC * 3 + 3) / 4][4], const TexLane& L, const TexArgs& A,
                                           float extra, bool do_shift, TexShared<NC>& S, Sink sink, Next next) {
     constexpr int NU = NC * 3, J = (NU + 3) / 4;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
     const int hs = n >> 3, kh = n & 7;
     if (do_shift) {   // fract_shift by extra * port_offset / corr_magic_scale (ID:2663-2677)
         const bool vodd = g >= 2;   // vs = 1
@@ -2252,7 +2252,7 @@ __global__ __launch_bounds__(256, NC <= 4 ? 4 : 3) void k_tex_imclt(const float*
     rbg += blockIdx.z * (NC * 3) * plane_sz;
     TexLane L;
     tex_lane_setup(L, lpf);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, gq = lane >> 4, n = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, gq = lane >> 4, n = lane & 15;
     // right half stored, not added: the last column when no even column follows it
     const bool add_left = parity == 1;
     const bool add_right = parity == 1 && !(tx == g.tiles_x - 1 && (g.tiles_x & 1) == 0);
