@@ -1735,8 +1735,21 @@ __global__ __launch_bounds__(256) void k_disparity_step(const double* __restrict
 // tile_debayer_shot_corr 3022-3147, tile_combine_rgba 2830-3000).  Block (256 threads) = tile,
 // thread = pixel of the 16x16 tile in the pixel stages.  NC = number of cameras.
 // ====================================================================================
-// LDS texture tiles: rows TEX_ROW floats apart (see TexShared)
-constexpr int TEX_ROW = 20, TEX_TILE = 16 * TEX_ROW;
+// LDS layouts of a 16x16 texture tile (TexShared); both keep the 16-byte stores of a lane's four
+// pixels (row n, columns 4g .. 4g+3) and the per-pixel reads of the combination free of bank
+// conflicts.  LAY 0: rows 20 floats apart (the two lane groups of a half-wavefront on distinct
+// banks).  LAY 1: rows 16 floats apart, the 16-byte column group g of row y stored at group
+// g ^ ((y >> 1) & 3), 20 % less LDS per tile.
+template <int LAY>
+struct TexLayout {
+    static constexpr int ROW = LAY ? 16 : 20, TILE = 16 * ROW;
+    __device__ __forceinline__ static int at(int y, int x) {   // pixel (y, x)
+        return LAY ? y * 16 + ((((x >> 2) ^ ((y >> 1) & 3)) << 2) | (x & 3)) : y * 20 + x;
+    }
+    __device__ __forceinline__ static int group(int y, int g) {   // pixels (y, 4g .. 4g+3)
+        return LAY ? y * 16 + ((g ^ ((y >> 1) & 3)) << 2) : y * 20 + 4 * g;
+    }
+};
 
 struct TexArgs {
     float col_w[3];
@@ -1750,7 +1763,7 @@ struct TexArgs {
 
 // Per-pixel port combination of one tile (tile_combine_rgba, ImageDtt.java:2830-3000, with the
 // final mix ImageDtt.java:2789-2801).  ALL: every port of the NC is used (no per-port masking).
-template <int NC, bool ALL>
+template <int NC, bool ALL, int TILE>
 __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, const float (&deb)[NC][3], float w2,
                                               float iw2, const float* s_nw, int pi, float* out, int i, int planes,
                                               float (&avg)[3]) {
@@ -1888,7 +1901,7 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
     for (int chn = 0; chn < 3; chn++) {
         float s = 0.f;
 #pragma unroll
-        for (int ip = 0; ip < NC; ip++) s += pw[ip] * s_nw[(ip * 3 + chn) * TEX_TILE + pi];
+        for (int ip = 0; ip < NC; ip++) s += pw[ip] * s_nw[(ip * 3 + chn) * TILE + pi];
         out[chn * 256 + i] = sw * s;
     }
     if (planes > 4) {   // keep_weights: weights, then the rms of the ports (ImageDtt.java:2858-2874)
@@ -1915,14 +1928,13 @@ __device__ __forceinline__ void combine_pixel(const TexArgs& A, int port_mask, c
 }
 
 // LDS of one texture tile: the window-free tiles (the IMCLT tiles divided by the squared window)
-// and the debayered tiles, [unit][TEX_ROW * row + col].  Lane (g, n) of a wavefront stores rows
-// 4g .. 4g+3 of column n; rows TEX_ROW = 20 floats apart put the two lane groups of each
-// half-wavefront on distinct banks.  TEX_ROW and TEX_TILE are defined with TexArgs.
-template <int NC>
+// and the debayered tiles, [unit][tile in layout LAY] (TexLayout, defined with TexArgs).
+template <int NC, int LAY>
 struct TexShared {
     static constexpr int NU = NC * 3;
-    float nw[NU][TEX_TILE];
-    float deb[NU][TEX_TILE];
+    using Lay = TexLayout<LAY>;
+    float nw[NU][Lay::TILE];
+    float deb[NU][Lay::TILE];
     __device__ __forceinline__ float* deb_of(int u) { return &deb[u][0]; }
 };
 
@@ -2015,9 +2027,9 @@ __device__ __forceinline__ void tex_fetch(const float* __restrict__ clt, size_t 
     }
 }
 
-template <int NC, class Sink, class Next>
+template <int NC, int LAY, class Sink, class Next>
 __device__ __forceinline__ void tex_units(float (&x)[(NC * 3 + 3) / 4][4], const TexLane& L, const TexArgs& A,
-                                          float extra, bool do_shift, TexShared<NC>& S, Sink sink, Next next) {
+                                          float extra, bool do_shift, TexShared<NC, LAY>& S, Sink sink, Next next) {
     constexpr int NU = NC * 3, J = (NU + 3) / 4;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15;
     const int hs = n >> 3, kh = n & 7;
@@ -2069,7 +2081,7 @@ __device__ __forceinline__ void tex_units(float (&x)[(NC * 3 + 3) / 4][4], const
         float nw[4];
 #pragma unroll
         for (int r = 0; r < 4; r++) nw[r] = px[r] * L.nwf[r];
-        *reinterpret_cast<float4*>(&S.nw[uu][n * TEX_ROW + 4 * g]) = make_float4(nw[0], nw[1], nw[2], nw[3]);
+        *reinterpret_cast<float4*>(&S.nw[uu][TexLayout<LAY>::group(n, g)]) = make_float4(nw[0], nw[1], nw[2], nw[3]);
         // horizontal neighbour sums (columns 4g - 1 and 4g + 4 from the lane groups to the left / right)
         const float left = lane_from(nw[3], lane, 48), right = lane_from(nw[0], lane, 16);
         float hsum[4];
@@ -2092,19 +2104,19 @@ __device__ __forceinline__ void tex_units(float (&x)[(NC * 3 + 3) / 4][4], const
             if (A.scale_shot > 0.f) dv[r] = A.scale_shot * ((dv[r] > A.min_shot) ? fast_sqrt(dv[r]) : (kshot * dv[r]));
             dv[r] *= L.w2f[r];
         }
-        *reinterpret_cast<float4*>(&S.deb[uu][n * TEX_ROW + 4 * g]) = make_float4(dv[0], dv[1], dv[2], dv[3]);
+        *reinterpret_cast<float4*>(&S.deb[uu][TexLayout<LAY>::group(n, g)]) = make_float4(dv[0], dv[1], dv[2], dv[3]);
     }
 }
 
 // Port combination (pixel per thread) and the tile statistics of one texture tile, from the LDS
 // that tex_units left (the caller's barrier between them).  All 256 threads call this.
-template <int NC>
-__device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, TexShared<NC>& S, float* __restrict__ out,
+template <int NC, int LAY>
+__device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, TexShared<NC, LAY>& S, float* __restrict__ out,
                                             float* __restrict__ stats_t) {
     const int i = threadIdx.x;   // pixel index in the 16x16 tile
     const int planes = A.planes;
     const int py = i >> 4, px = i & 15;
-    const int pi = py * TEX_ROW + px;   // ... in the LDS tiles
+    const int pi = TexLayout<LAY>::at(py, px);   // ... in the LDS tiles
     const int qy = py < 8 ? py : 15 - py, qx = px < 8 ? px : 15 - px;
     const float w2 = c_tab.hw2[qy] * c_tab.hw2[qx];
     const float iw2 = c_tab.ihw2[qy] * c_tab.ihw2[qx];
@@ -2112,8 +2124,9 @@ __device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, Tex
 #pragma unroll
     for (int uu = 0; uu < NC * 3; uu++) deb[uu / 3][uu % 3] = S.deb_of(uu)[pi];
     float avg[3];
-    if (port_mask == (1 << NC) - 1) combine_pixel<NC, true>(A, port_mask, deb, w2, iw2, &S.nw[0][0], pi, out, i, planes, avg);
-    else combine_pixel<NC, false>(A, port_mask, deb, w2, iw2, &S.nw[0][0], pi, out, i, planes, avg);
+    constexpr int TILE = TexLayout<LAY>::TILE;
+    if (port_mask == (1 << NC) - 1) combine_pixel<NC, true, TILE>(A, port_mask, deb, w2, iw2, &S.nw[0][0], pi, out, i, planes, avg);
+    else combine_pixel<NC, false, TILE>(A, port_mask, deb, w2, iw2, &S.nw[0][0], pi, out, i, planes, avg);
     if (stats_t) {   // max_diff (ImageDtt.java:2965-2981) and ports_rgb (ImageDtt.java:2982-2996)
         // transpose through LDS: value k of pixel i at [k][i] (the NC max_diff rows reuse deb, the
         // 3*NC colour rows reuse nw); 16 threads then reduce each row
@@ -2186,7 +2199,7 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
                                                   const float* __restrict__ extra_disp, const TexArgs* __restrict__ args_p,
                                                   float* __restrict__ tiles_out, float* __restrict__ stats_out,
                                                   const int32_t* __restrict__ task_frame, long long clt_fstride, GeomDev g) {
-    __shared__ __attribute__((aligned(16))) TexShared<NC> S;
+    __shared__ __attribute__((aligned(16))) TexShared<NC, 1> S;
     const TexArgs& A = *args_p;
     const int t = xcd_block(blockIdx.x, gridDim.x);
     if (t >= num_tasks) return;
@@ -2212,7 +2225,7 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
     tex_lane_setup(L, lpf);
     tex_units<NC>(x, L, A, extra, do_shift, S, NoRowSink{}, [] {});
     __syncthreads();
-    tex_combine<NC>(A, port_mask, S, out, stats_t);
+    tex_combine<NC, 1>(A, port_mask, S, out, stats_t);
 }
 
 // ====================================================================================
@@ -2228,7 +2241,7 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
 // extra disparity, and at most one task per tile with every task's flags non-zero (tile_map then
 // names every task exactly once).
 // ====================================================================================
-template <int NC>
+template <int NC, int LAY>
 __global__ __launch_bounds__(256, NC <= 4 ? 4 : 3) void k_tex_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                    int task_words, const int32_t* __restrict__ tile_map,
                                                    const float* __restrict__ lpf, const TexArgs* __restrict__ args_p,
@@ -2237,7 +2250,7 @@ __global__ __launch_bounds__(256, NC <= 4 ? 4 : 3) void k_tex_imclt(const float*
                                                    long long clt_fstride) {
     static_assert(NC * 3 <= 16, "at most four units per wavefront");
     constexpr int NU = NC * 3, J = (NU + 3) / 4;
-    __shared__ __attribute__((aligned(16))) TexShared<NC> S;
+    __shared__ __attribute__((aligned(16))) TexShared<NC, LAY> S;
     const TexArgs& A = *args_p;
     const int tx = 2 * (int)blockIdx.x + parity;
     if (tx >= g.tiles_x) return;   // whole block
@@ -2301,7 +2314,7 @@ __global__ __launch_bounds__(256, NC <= 4 ? 4 : 3) void k_tex_imclt(const float*
             tex_units<NC>(x, L, A, 0.f, false, S, sink, next);
             __syncthreads();
             if (own && ty < g.tiles_y)
-                tex_combine<NC>(A, port_mask, S, tiles_out + (size_t)t * A.planes * 256,
+                tex_combine<NC, LAY>(A, port_mask, S, tiles_out + (size_t)t * A.planes * 256,
                                 stats_out ? stats_out + (size_t)t * 4 * NC : nullptr);
             __syncthreads();   // the next tile reuses the LDS
         } else {   // no task tile (or outside the frame): nothing to add but the lower half above

@@ -221,6 +221,7 @@ struct clt_handle {
     int tune_corr_tpb = 4, tune_imclt_rows = 16;
     int tune_chunks = 1;           // frame chunks of the batch pipeline
     int tex_imclt_rows = 8;        // output block rows per segment of k_tex_imclt
+    int tex_lds_layout = 0;        // LDS tile layout of k_tex_imclt (TexLayout: 0 padded rows, 1 swizzled groups)
     bool tuned = false;
     bool corr_quad = false;        // correlate2D runs the quad kernel
     int corr_num_strip = 0;
@@ -491,24 +492,30 @@ static void launch_tex_nc(clt_handle* h, int n, const int32_t* task_frame, const
                        task_frame ? task_frame + t0 : nullptr, h->clt_fstride, h->gd);
 }
 
-// Frames [f0, f0 + frames) (one frame: f0 = 0, frames = 1): both column launches.
-template <int NC>
-static void launch_tex_imclt_nc(clt_handle* h, int f0 = 0, int frames = 1) {
+// Frames [f0, f0 + frames) (one frame: f0 = 0, frames = 1): both column launches, in the LDS tile
+// layout h->tex_lds_layout.
+template <int NC, int LAY>
+static void launch_tex_imclt_nc(clt_handle* h, int f0, int frames) {
     const int rows = h->tex_imclt_rows;
     const dim3 grid((h->tiles_x + 1) / 2, (h->tiles_y + 1 + rows - 1) / rows, frames);
     const size_t plane_px = (size_t)(h->g.width + 8) * (h->g.height + 8);
     for (int parity = 0; parity < 2; parity++)   // even columns store, odd columns add
-        hipLaunchKernelGGL(k_tex_imclt<NC>, grid, dim3(256), 0, h->stream, h->d_clt + (size_t)f0 * h->clt_fstride,
+        hipLaunchKernelGGL((k_tex_imclt<NC, LAY>), grid, dim3(256), 0, h->stream, h->d_clt + (size_t)f0 * h->clt_fstride,
                            h->d_tasks, h->task_words, h->d_tile_map + (size_t)f0 * h->num_tiles, h->d_lpf_corr,
                            h->d_tex_args, h->d_tex, h->d_tex_stats, h->d_rbg + (size_t)f0 * NC * 3 * plane_px, rows,
                            parity, h->gd, h->clt_fstride);
 }
+template <int NC>
+static void launch_tex_imclt_lay(clt_handle* h, int f0, int frames) {
+    if (h->tex_lds_layout) launch_tex_imclt_nc<NC, 1>(h, f0, frames);
+    else launch_tex_imclt_nc<NC, 0>(h, f0, frames);
+}
 static void launch_tex_imclt(clt_handle* h, int f0 = 0, int frames = 1) {
     switch (h->g.num_cams) {
-    case 2: launch_tex_imclt_nc<2>(h, f0, frames); break;
-    case 3: launch_tex_imclt_nc<3>(h, f0, frames); break;
-    case 4: launch_tex_imclt_nc<4>(h, f0, frames); break;
-    default: launch_tex_imclt_nc<5>(h, f0, frames); break;
+    case 2: launch_tex_imclt_lay<2>(h, f0, frames); break;
+    case 3: launch_tex_imclt_lay<3>(h, f0, frames); break;
+    case 4: launch_tex_imclt_lay<4>(h, f0, frames); break;
+    default: launch_tex_imclt_lay<5>(h, f0, frames); break;
     }
 }
 
@@ -2439,16 +2446,21 @@ int clt_autotune(clt_handle* h) {
     int stages = STAGE_CONVERT | STAGE_CORR | STAGE_IMCLT;
     if (h->g.num_colors == 3 && h->g.num_cams >= 2 && prepare_tex(h) == CLT_OK) stages |= STAGE_TEX;
     h->err.clear();
-    if ((stages & STAGE_TEX) && h->batch_fused) {   // tile rows per segment of the one-pass textures/imclt:
-        const int cand[3] = {4, 8, 16};           // the halo re-read (1/rows) against the workgroups
+    if ((stages & STAGE_TEX) && h->batch_fused) {   // the one-pass textures/imclt: tile rows per segment
+        // (the halo re-read, 1/rows, against the workgroups) and the LDS tile layout (padded rows or
+        // swizzled column groups: LDS per workgroup against the address arithmetic)
+        const int cand[3] = {4, 8, 16};
         float best = 1e30f;
-        int pick = h->tex_imclt_rows;
-        for (int c : cand) {
-            h->tex_imclt_rows = c;
-            const float ms = time_stage(h, STAGE_TEX | STAGE_IMCLT);
-            if (ms < best) { best = ms; pick = c; }
-        }
+        int pick = h->tex_imclt_rows, pick_lay = h->tex_lds_layout;
+        for (int lay = 0; lay < 2; lay++)
+            for (int c : cand) {
+                h->tex_imclt_rows = c;
+                h->tex_lds_layout = lay;
+                const float ms = time_stage(h, STAGE_TEX | STAGE_IMCLT);
+                if (ms < best) { best = ms; pick = c; pick_lay = lay; }
+            }
         h->tex_imclt_rows = pick;
+        h->tex_lds_layout = pick_lay;
     }
     {   // frame chunks of the pipelined batch (overlap of LMA and imclt_rbg with the next chunk)
         const int cand[4] = {1, 2, 4, 8};
@@ -2487,15 +2499,15 @@ int clt_set_batch_chunks(clt_handle* h, int32_t chunks) {
 
 int clt_get_tune_all(clt_handle* h, int32_t* values, int32_t n) {
     if (!h || !values || n < 0) return CLT_E_ARG;
-    const int32_t v[4] = {h->tune_corr_tpb, h->tune_imclt_rows, std::max(1, std::min(h->tune_chunks, h->num_frames)),
-                          h->tex_imclt_rows};
-    for (int i = 0; i < n && i < 4; i++) values[i] = v[i];
+    const int32_t v[5] = {h->tune_corr_tpb, h->tune_imclt_rows, std::max(1, std::min(h->tune_chunks, h->num_frames)),
+                          h->tex_imclt_rows, h->tex_lds_layout};
+    for (int i = 0; i < n && i < 5; i++) values[i] = v[i];
     return CLT_OK;
 }
 
 int clt_set_tune_all(clt_handle* h, const int32_t* values, int32_t n) {
-    if (!h || (!values && n > 0) || n < 0 || n > 4) return CLT_E_ARG;
-    const int32_t lo[4] = {1, 1, 1, 1}, hi[4] = {4, 64, 64, 64};
+    if (!h || (!values && n > 0) || n < 0 || n > 5) return CLT_E_ARG;
+    const int32_t lo[5] = {1, 1, 1, 1, 0}, hi[5] = {4, 64, 64, 64, 1};
     for (int i = 0; i < n; i++)
         if (values[i] < lo[i] || values[i] > hi[i] || (i == 0 && values[i] == 3))
             return fail(h, CLT_E_ARG, "clt_set_tune_all: value %d = %d", i, values[i]);
@@ -2503,6 +2515,7 @@ int clt_set_tune_all(clt_handle* h, const int32_t* values, int32_t n) {
     if (n > 1) h->tune_imclt_rows = values[1];
     if (n > 2) h->tune_chunks = values[2];
     if (n > 3) h->tex_imclt_rows = values[3];
+    if (n > 4) h->tex_lds_layout = values[4];
     h->graph_epoch++;
     return CLT_OK;
 }

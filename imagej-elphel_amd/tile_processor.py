@@ -469,13 +469,14 @@ class GPUTileProcessor:
         return ext, disp, hist[:its.value + 1]
 
     # ------------------------------------------------------------------ execution control
-    TUNE_KNOBS = ("corr_tiles_per_block", "imclt_rows", "chunks", "tex_imclt_rows")
+    TUNE_KNOBS = ("corr_tiles_per_block", "imclt_rows", "chunks", "tex_imclt_rows", "tex_lds_layout")
 
     def getTune(self):
         """Launch shapes in use (clt_get_tune_all): correlate2D tiles per workgroup, imclt rows per
-        segment, batch frame chunks, one-pass textures/imclt rows per segment."""
-        v = (ctypes.c_int32 * 4)()
-        self._chk(self.lib.clt_get_tune_all(self.h, v, 4))
+        segment, batch frame chunks, one-pass textures/imclt rows per segment and LDS tile layout."""
+        n = len(self.TUNE_KNOBS)
+        v = (ctypes.c_int32 * n)()
+        self._chk(self.lib.clt_get_tune_all(self.h, v, n))
         return dict(zip(self.TUNE_KNOBS, v))
 
     def setTune(self, **knobs):
@@ -485,8 +486,8 @@ class GPUTileProcessor:
             raise ValueError("unknown tuning knobs: %s" % sorted(unknown))
         cur = self.getTune()
         cur.update(knobs)
-        v = (ctypes.c_int32 * 4)(*[int(cur[k]) for k in self.TUNE_KNOBS])
-        self._chk(self.lib.clt_set_tune_all(self.h, v, 4))
+        v = (ctypes.c_int32 * len(self.TUNE_KNOBS))(*[int(cur[k]) for k in self.TUNE_KNOBS])
+        self._chk(self.lib.clt_set_tune_all(self.h, v, len(self.TUNE_KNOBS)))
 
     def setAsync(self, on):
         self._chk(self.lib.clt_set_async(self.h, int(on)))

@@ -467,15 +467,18 @@ int  clt_exec_batch(clt_handle* h, int32_t stages);
  * streams.  Results do not depend on the chunk count. */
 /* Times the candidate launch shapes on the current batch and keeps the fastest: tiles per
  * workgroup of correlate2D (1, 2, 4: its LDS per workgroup), tile rows per imclt segment (8, 16,
- * 32), tile rows per segment of the one-pass textures/imclt (4, 8, 16) and frame chunks of the
+ * 32), tile rows per segment of the one-pass textures/imclt (4, 8, 16) with its LDS tile layout
+ * (rows 20 floats apart, or 16 with swizzled 16-byte column groups) and frame chunks of the
  * pipeline (1, 2, 4, 8, at most num_frames). */
 int  clt_autotune(clt_handle* h);
 int  clt_get_tune(clt_handle* h, int32_t* corr_tiles_per_block, int32_t* imclt_rows);
-/* values[0..n-1] (n <= 4): correlate2D tiles per workgroup, imclt rows, frame chunks, one-pass rows */
+/* values[0..n-1] (n <= 5): correlate2D tiles per workgroup, imclt rows, frame chunks, one-pass rows,
+ * one-pass LDS tile layout */
 int  clt_get_tune_all(clt_handle* h, int32_t* values, int32_t n);
-/* Sets the first n (<= 4) of the same knobs, e.g. to reuse a clt_autotune result on another handle
+/* Sets the first n (<= 5) of the same knobs, e.g. to reuse a clt_autotune result on another handle
  * or to pin the per-frame path's launch shapes: correlate2D tiles per workgroup 1, 2 or 4; rows
- * 1..64; chunks 1..64.  No reference counterpart (the reference's launch shapes are fixed). */
+ * 1..64; chunks 1..64; layout 0 or 1.  No reference counterpart (the reference's launch shapes are
+ * fixed). */
 int  clt_set_tune_all(clt_handle* h, const int32_t* values, int32_t n);
 /* chunks 1..64 (used as min(chunks, num_frames)); 1 until clt_autotune or this call. */
 int  clt_set_batch_chunks(clt_handle* h, int32_t chunks);

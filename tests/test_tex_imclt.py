@@ -149,3 +149,32 @@ def test_one_pass_against_oracle(pk, orc, frac):
     for name, tol in TEX_TOL.items():
         assert err[name] < tol, name
     assert err["stats"] < 1e-4 and err["image"] < 1e-4
+
+
+@pytest.mark.parametrize("cams", [3, 4, 5])
+def test_lds_layouts_bit_identical(pk, cams):
+    """The one-pass kernel's two LDS tile layouts (tuning knob tex_lds_layout: rows 20 floats apart,
+    or 16 with swizzled 16-byte column groups) move the same values through LDS: every output is
+    bit-identical."""
+    log("case: LDS layouts 0 and 1 of the one-pass kernel, %d cameras, 136x152, 70 %% of the tiles" % cams)
+    sc = pk.synth.make_scene(136, 152, num_cams=cams, seed=27 + cams, disparity=("field", 0.0, 5.0),
+                             task_fraction=0.7)
+
+    def run(layout):
+        tp = gpu_processor(pk, sc)
+        tp.setCorrSigma(0.8)
+        tp.setTexParams(keep_weights=1)
+        tp.setTune(tex_lds_layout=layout)
+        assert tp.getTune()["tex_lds_layout"] == layout
+        tp.execConverCorrectTiles()
+        tp.setTiming(True)
+        tp.execTexturesImclt(True)
+        assert tp.getTiming("tex_imclt")[1] == 1
+        out = _outputs(tp, sc)
+        tp.close()
+        return out
+
+    a, b = run(0), run(1)
+    same = [np.array_equal(x.view(np.int32), y.view(np.int32)) for x, y in zip(a, b)]
+    log("  bit-identical tiles, stats, images: %s" % same)
+    assert all(same)
