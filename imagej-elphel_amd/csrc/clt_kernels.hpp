@@ -2132,6 +2132,10 @@ __device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, Tex
         // 3*NC colour rows reuse nw); 16 threads then reduce each row
         float* red_max = &S.deb[0][0];
         float* red_rgb = &S.nw[0][0];
+        // rows RS floats apart: 272 puts the rows read by the two 16-lane halves of a half-wavefront
+        // on distinct banks (256 would put all four rows of a wavefront on the same banks); it fits
+        // the tiles of the padded layout
+        constexpr int RS = (TexLayout<LAY>::TILE >= 272) ? 272 : 256;
         __syncthreads();
 #pragma unroll
         for (int ip = 0; ip < NC; ip++) {
@@ -2144,9 +2148,9 @@ __device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, Tex
                 }
                 d2 *= w2;
             }
-            red_max[ip * 256 + i] = d2;
+            red_max[ip * RS + i] = d2;
 #pragma unroll
-            for (int chn = 0; chn < 3; chn++) red_rgb[(chn * NC + ip) * 256 + i] = deb[ip][chn];
+            for (int chn = 0; chn < 3; chn++) red_rgb[(chn * NC + ip) * RS + i] = deb[ip][chn];
         }
         __syncthreads();
         // 4*NC values x 256 pixels: 16 threads per value, each the 16 pixels j, j + 16, ... (the
@@ -2157,7 +2161,7 @@ __device__ __forceinline__ void tex_combine(const TexArgs& A, int port_mask, Tex
             const int k = kb + (i >> 4), j = i & 15;
             if (k < 4 * NC) {
                 const bool is_max = k < NC;
-                const float* r = (is_max ? red_max + k * 256 : red_rgb + (k - NC) * 256) + j;
+                const float* r = (is_max ? red_max + k * RS : red_rgb + (k - NC) * RS) + j;
                 float v = r[0];
                 if (is_max) {
 #pragma unroll
