@@ -429,6 +429,41 @@ def adjoint_bench(pk, sc, repeats=5, seed=11):
     fwd = (tp.getTiming("convert")[0] + tp.getTiming("corr2d")[0]) / repeats
     adj = tp.getTiming("geometry_adjoint")[0] / repeats
     tp.setTiming(False)
+    # One gradient (13 extrinsics + every tile's disparity) by the adjoint, wall clock per call,
+    # against the same gradient through the plain per-frame path: central differences of the loss,
+    # one +/- pair of passes per extrinsic and one for all the disparities at once (each tile's loss
+    # depends on its own disparity only), each pass the device port centres, convert and
+    # correlate2D without the per-tile LMA.  (Reading the 28 losses back, a sum over the pairs'
+    # correlation centres, is left out of the plain path's time.)
+    tp.setAsync(True)
+    tp.sync()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        tp.getGeometryGradient()
+    grad_adj = 1e3 * (time.perf_counter() - t0) / repeats
+    tp.setCorrLmaParams(enable=0)
+    pg0 = list(pk.GPUTileProcessor.defaultPortGeometry(W, H).extrinsic)
+    h_e, h_d = 1e-4, 1e-2
+
+    def plain_pass(ext, disp):
+        tp.setPortGeometry(pk.GPUTileProcessor.defaultPortGeometry(W, H, extrinsic=ext))
+        tp.getTileSubcamOffsets(words, tx, ty, disp)
+        tp.execConverCorrectTiles(0)
+        tp.execCorrelate2D()
+
+    plain_pass(pg0, d)
+    tp.sync()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        for k in range(13):
+            for sgn in (1.0, -1.0):
+                e = list(pg0)
+                e[k] += sgn * h_e
+                plain_pass(e, d)
+        for sgn in (1.0, -1.0):
+            plain_pass(pg0, (d + sgn * h_d).astype(np.float32))
+        tp.sync()
+    grad_fd = 1e3 * (time.perf_counter() - t0) / repeats
     tp.close()
     # the solve on a frame rendered through the port geometry itself (camera images resampled at
     # the port positions of extrinsics drawn N(0, 1e-3) and a smooth disparity field), started
@@ -485,6 +520,8 @@ def adjoint_bench(pk, sc, repeats=5, seed=11):
     gp.close()
     return {"tasks": T, "forward_ms": round(fwd, 4), "adjoint_ms": round(adj, 4),
             "forward_tiles_per_s": round(T / (fwd * 1e-3), 1), "adjoint_tiles_per_s": round(T / (adj * 1e-3), 1),
+            "gradient": {"adjoint_ms": round(grad_adj, 3), "plain_central_differences_ms": round(grad_fd, 3),
+                         "plain_passes": 28, "speedup": round(grad_fd / grad_adj, 2)},
             "solve_tasks": len(tiles), "solve": runs}
 
 
