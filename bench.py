@@ -2,10 +2,10 @@
 
 Workload (one "step" = one frame through the plain per-frame GPU path): 4 x 2592x1936 Bayer
 (RGGB) float32 mosaics, 78,408 tiles, dense task list, per-tile disparity a smooth random field
-0..20 px, random per-tile aberration kernels; convert_direct -> correlate2D (6 pairs + combined
-strip + disparity, per-tile LMA chain) -> textures (RGBA) and imclt_rbg (with LPF) in one pass over
-the converted tiles (clt_exec_textures_imclt).  The LMA chain (on its own stream) overlaps the
-textures/imclt pass and the next convert.  Inputs are resident in HBM before the timed region;
+0..20 px, random per-tile aberration kernels; convert_direct -> textures (RGBA) and imclt_rbg
+(with LPF) in one pass over the converted tiles (clt_exec_textures_imclt) -> correlate2D (6 pairs +
+combined strip + disparity, per-tile LMA chain).  The LMA chain (on its own stream) overlaps the
+next frame's convert and textures/imclt pass.  Inputs are resident in HBM before the timed region;
 outputs stay on the device.
 
 Prints one JSON line: tiles/s, HBM roofline of the dominant kernel (HIP events on the library's
@@ -251,12 +251,13 @@ def make_processor(pk, sc):
 
 def run_step(tp):
     """One frame.  The per-tile LMA chain that correlate2D starts on its own stream overlaps the
-    one-pass textures/imclt_rbg and the next frame's convert (which waits on memory and leaves the
-    vector ALUs idle a third of the time); the one pass runs the inverse CLT of each tile once for
-    both outputs."""
+    next frame's convert (which waits on memory and leaves the vector ALUs idle a third of the
+    time) and one-pass textures/imclt_rbg; the one pass runs the inverse CLT of each tile once for
+    both outputs.  Textures/imclt before correlate2D: the one-pass kernel (4 waves per SIMD) then
+    shares the GPU with the previous frame's LMA chain for less of its run."""
     tp.execConverCorrectTiles(0)
-    tp.execCorrelate2D()
     tp.execTexturesImclt(True)
+    tp.execCorrelate2D()
 
 
 def batch_bench(pk, sc, frames=32, seed=7, repeats=3):

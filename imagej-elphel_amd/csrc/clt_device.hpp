@@ -65,6 +65,22 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks) {
     return (x < rem) ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
 }
 
+// Order of a task list over the blocks: the list is cut into bands of BAND rows of a dense
+// row-major list W tiles wide (positions pBW .. pBW + BW - 1), and a band is taken column by
+// column, so the tiles that share an aberration kernel tile (kernel grid at twice the tile step,
+// above and beside each other) and window rows run within a few blocks of each other, while those
+// are in L2; in row order the tile row below comes a whole row later, after they have left it.  A
+// bijection on 0 .. n-1 (positions past the last whole band keep their place), so any list is
+// still covered exactly once.
+constexpr int TASK_BAND = 8;
+__device__ __forceinline__ int band_order(int i, int n, int w) {
+    const unsigned gsz = TASK_BAND * (unsigned)w;
+    const unsigned p = (unsigned)i / gsz;
+    if ((p + 1) * gsz > (unsigned)n) return i;
+    const unsigned q = (unsigned)i - p * gsz;
+    return (int)(p * gsz + (q % TASK_BAND) * (unsigned)w + q / TASK_BAND);
+}
+
 // ---------------------------------------------------------------- 8-point transforms
 // 8-point orthonormal DCT-IV (its own inverse) through a 4-point complex DFT between two twiddle
 // stages: t_n = (x[2n] + i x[7-2n]) pre_n, T = DFT4(t), u_k = T_k post_k, y[2k] = Re u_k,

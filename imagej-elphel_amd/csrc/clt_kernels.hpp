@@ -65,7 +65,7 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     int u = blk * 4 + ((threadIdx.x >> 4) & 3);             // (task, camera) unit
     bool active = u < total;
     if (!active) u = 0;
-    const int t = u / g.cams, cam = u % g.cams;
+    const int t = band_order(u / g.cams, num_tasks, g.tiles_x), cam = u % g.cams;
     const int32_t* rec = tasks + (size_t)t * task_words;
     active = active && (rec[0] != 0);
     if (!__any(active)) return;   // whole wave idle (no block-level barriers in this kernel)
@@ -263,13 +263,21 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
             b[k] *= f;
         }
     }
+    // ---- store through the unit's LDS: the rows of lane (hs, kh) go to their places in the
+    // unit's 256 coefficients, then lane l stores coefficients 4l .. 4l + 3 of each mode, so every
+    // store instruction writes whole 256-byte runs (instead of 16 bytes in every 32)
+    wave_sync();   // stage V's reads of sb are done
+    *reinterpret_cast<float4*>(&sb[hs * 64 + kh * 8]) = make_float4(a[0], a[1], a[2], a[3]);
+    *reinterpret_cast<float4*>(&sb[hs * 64 + kh * 8 + 4]) = make_float4(a[4], a[5], a[6], a[7]);
+    *reinterpret_cast<float4*>(&sb[(hs | 2) * 64 + kh * 8]) = make_float4(b[0], b[1], b[2], b[3]);
+    *reinterpret_cast<float4*>(&sb[(hs | 2) * 64 + kh * 8 + 4]) = make_float4(b[4], b[5], b[6], b[7]);
+    wave_sync();
     if (active) {
         const size_t tile = (size_t)ty * g.tiles_x + tx;
-        float* o = clt + frame * clt_fstride + (((size_t)cam * g.tiles_x * g.tiles_y + tile) * g.colors + color) * 256 + kh * 8;
-        *reinterpret_cast<float4*>(o + hs * 64) = make_float4(a[0], a[1], a[2], a[3]);
-        *reinterpret_cast<float4*>(o + hs * 64 + 4) = make_float4(a[4], a[5], a[6], a[7]);
-        *reinterpret_cast<float4*>(o + (hs | 2) * 64) = make_float4(b[0], b[1], b[2], b[3]);
-        *reinterpret_cast<float4*>(o + (hs | 2) * 64 + 4) = make_float4(b[4], b[5], b[6], b[7]);
+        float* o = clt + frame * clt_fstride + (((size_t)cam * g.tiles_x * g.tiles_y + tile) * g.colors + color) * 256 + 4 * l;
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+            *reinterpret_cast<float4*>(o + m * 64) = *reinterpret_cast<const float4*>(&sb[m * 64 + 4 * l]);
     }
 }
 
