@@ -624,7 +624,7 @@ __device__ __forceinline__ void corr_dct_mx(float* S, const float (&u)[4], int l
 }
 // One pair's correlation from its frequency-domain product (one bin per lane, the four modes in
 // acc): LPF, the 2-D DCT-IIe / DST-IIe above through the wavefront's 256 floats of LDS at S, unfold
-// to the 15x15 tile (DttRad2.java:443-489), written to co (global) and cs (LDS).
+// to the 15x15 tile (DttRad2.java:443-489), written to cs (LDS) and, unless co is null, co (global).
 __device__ __forceinline__ void corr_fd_to_tile_mx(const float* acc, float lp, float* S, const float (&u)[4], int lane,
                                                    float* co, float* cs) {
     const int y = lane >> 3, xx = lane & 7;
@@ -643,11 +643,16 @@ __device__ __forceinline__ void corr_fd_to_tile_mx(const float* acc, float lp, f
     const float v_mp = 0.25f * (va + vb - vc - vd), v_mm = 0.25f * (va - vb - vc + vd);
     const int o_pp = (7 + y) * 15 + 7 + xx, o_pm = (7 + y) * 15 + 7 - xx;
     const int o_mp = (7 - y) * 15 + 7 + xx, o_mm = (7 - y) * 15 + 7 - xx;
-    co[o_pp] = v_pp;
     cs[o_pp] = v_pp;
-    if (xx > 0) { co[o_pm] = v_pm; cs[o_pm] = v_pm; }
-    if (y > 0) { co[o_mp] = v_mp; cs[o_mp] = v_mp; }
-    if (xx > 0 && y > 0) { co[o_mm] = v_mm; cs[o_mm] = v_mm; }
+    if (xx > 0) cs[o_pm] = v_pm;
+    if (y > 0) cs[o_mp] = v_mp;
+    if (xx > 0 && y > 0) cs[o_mm] = v_mm;
+    if (co) {
+        co[o_pp] = v_pp;
+        if (xx > 0) co[o_pm] = v_pm;
+        if (y > 0) co[o_mp] = v_mp;
+        if (xx > 0 && y > 0) co[o_mm] = v_mm;
+    }
 }
 // per-lane operand of corr_dct_mx: the block-diagonal DCT-IIe / DST-IIe at [n][4g + s]
 __device__ __forceinline__ void corr_mx_operand(int lane, float (&u)[4]) {
@@ -1095,9 +1100,13 @@ __global__ __launch_bounds__(256, FZ ? 4 : 5) void k_corr2d_quad(const float* __
             acc[2] = fmaf(t2, scale, acc[2]);
             acc[3] = fmaf(t3, scale, acc[3]);
         }
-        corr_fd_to_tile_mx(acc, lp, S, umx, lane, corr_t + p * CLT_CORR_LEN, s_corr + p * CLT_CORR_LEN);
+        corr_fd_to_tile_mx(acc, lp, S, umx, lane, nullptr, s_corr + p * CLT_CORR_LEN);
     }
     wave_sync();
+    // the six correlation tiles, contiguous in LDS and in the output: stored as 8-byte pairs, each
+    // store instruction a contiguous 512-byte run (the tile's base is 8-byte aligned)
+    for (int o = lane; o < 3 * CLT_CORR_LEN; o += 64)
+        reinterpret_cast<float2*>(corr_t)[o] = reinterpret_cast<const float2*>(s_corr)[o];
     // ---- combined strip (C2:730-790) of the 6 rotated strips (C2:556-718)
     {
         const int ncomb = 6 * (A.twice_diagonal ? 2 : 1);
