@@ -265,19 +265,22 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     }
     // ---- store through the unit's LDS: the rows of lane (hs, kh) go to their places in the
     // unit's 256 coefficients, then lane l stores coefficients 4l .. 4l + 3 of each mode, so every
-    // store instruction writes whole 256-byte runs (instead of 16 bytes in every 32)
+    // store instruction writes whole 256-byte runs (instead of 16 bytes in every 32).  The modes sit
+    // CONV_MODE floats apart in LDS, so the 16-byte writes of the two lane halves (hs = 0, 1) land on
+    // distinct banks.
+    constexpr int CONV_MODE = 68;
     wave_sync();   // stage V's reads of sb are done
-    *reinterpret_cast<float4*>(&sb[hs * 64 + kh * 8]) = make_float4(a[0], a[1], a[2], a[3]);
-    *reinterpret_cast<float4*>(&sb[hs * 64 + kh * 8 + 4]) = make_float4(a[4], a[5], a[6], a[7]);
-    *reinterpret_cast<float4*>(&sb[(hs | 2) * 64 + kh * 8]) = make_float4(b[0], b[1], b[2], b[3]);
-    *reinterpret_cast<float4*>(&sb[(hs | 2) * 64 + kh * 8 + 4]) = make_float4(b[4], b[5], b[6], b[7]);
+    *reinterpret_cast<float4*>(&sb[hs * CONV_MODE + kh * 8]) = make_float4(a[0], a[1], a[2], a[3]);
+    *reinterpret_cast<float4*>(&sb[hs * CONV_MODE + kh * 8 + 4]) = make_float4(a[4], a[5], a[6], a[7]);
+    *reinterpret_cast<float4*>(&sb[(hs | 2) * CONV_MODE + kh * 8]) = make_float4(b[0], b[1], b[2], b[3]);
+    *reinterpret_cast<float4*>(&sb[(hs | 2) * CONV_MODE + kh * 8 + 4]) = make_float4(b[4], b[5], b[6], b[7]);
     wave_sync();
     if (active) {
         const size_t tile = (size_t)ty * g.tiles_x + tx;
         float* o = clt + frame * clt_fstride + (((size_t)cam * g.tiles_x * g.tiles_y + tile) * g.colors + color) * 256 + 4 * l;
 #pragma unroll
         for (int m = 0; m < 4; m++)
-            *reinterpret_cast<float4*>(o + m * 64) = *reinterpret_cast<const float4*>(&sb[m * 64 + 4 * l]);
+            *reinterpret_cast<float4*>(o + m * 64) = *reinterpret_cast<const float4*>(&sb[m * CONV_MODE + 4 * l]);
     }
 }
 
