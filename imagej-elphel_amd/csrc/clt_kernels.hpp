@@ -89,21 +89,16 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
         ce = koffsets + kidx * 8;
         kt = kernels + kidx * 256;
     }
-    // aberration kernel of this lane's coefficients, loaded early so the load overlaps the window
-    // loads and the transforms
-    float k0[8], k1[8], k2[8], k3[8];
+    // aberration kernel, loaded early so the load overlaps the window loads and the transforms:
+    // the four quadrants of row kh = l & 7 at kv = 4 hs .. 4 hs + 3, hs = l >> 3 (the kv half this
+    // lane takes in the frequency-domain stages below)
+    float kk[4][4];
     if (kt) {
-        const float* kq = kt + (l & 7) * 8;
+        const float* kq = kt + (l & 7) * 8 + 4 * (l >> 3);
 #pragma unroll
-        for (int k = 0; k < 8; k += 4) {
-            float4 v0 = *reinterpret_cast<const float4*>(kq + k);
-            float4 v1 = *reinterpret_cast<const float4*>(kq + 64 + k);
-            float4 v2 = *reinterpret_cast<const float4*>(kq + 128 + k);
-            float4 v3 = *reinterpret_cast<const float4*>(kq + 192 + k);
-            k0[k] = v0.x; k0[k + 1] = v0.y; k0[k + 2] = v0.z; k0[k + 3] = v0.w;
-            k1[k] = v1.x; k1[k + 1] = v1.y; k1[k + 2] = v1.z; k1[k + 3] = v1.w;
-            k2[k] = v2.x; k2[k + 1] = v2.y; k2[k + 2] = v2.z; k2[k + 3] = v2.w;
-            k3[k] = v3.x; k3[k + 1] = v3.y; k3[k + 2] = v3.z; k3[k + 3] = v3.w;
+        for (int q = 0; q < 4; q++) {
+            const float4 v = *reinterpret_cast<const float4*>(kq + 64 * q);
+            kk[q][0] = v.x; kk[q][1] = v.y; kk[q][2] = v.z; kk[q][3] = v.w;
         }
     }
     const TilePos pos = tile_position(cx, cy, ce, g.kstep, ktx, kty);
@@ -224,63 +219,80 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     }
     const float sigma = hs ? 1.f : -1.f;
 
-    // ---- aberration kernel multiply (ImageDtt.java:5429-5488), partner lane holds modes hs^1
+    // ---- frequency-domain stages: aberration kernel multiply (ImageDtt.java:5429-5488), fractional
+    // shift (ImageDtt.java:5490-5584) and LPF.  Each mixes the modes of the partner lanes (hs = 0, 1
+    // of row kh) at equal kv, so the two lanes split the work by kv: lane hs takes kv = 4 hs + j,
+    // j = 0..3, of both lanes' modes.  One exchange gives each lane the partner's a, b of its half;
+    // oa, ob are the lane's own modes (hs, hs | 2), pa, pb the partner's (hs ^ 1, (hs ^ 1) | 2).
+    float oa[4], ob[4], pa[4], pb[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        pa[j] = lane_xor8(hs ? a[j] : a[j + 4]);   // the partner sends what this lane takes
+        pb[j] = lane_xor8(hs ? b[j] : b[j + 4]);
+        oa[j] = hs ? a[j + 4] : a[j];
+        ob[j] = hs ? b[j + 4] : b[j];
+    }
+    // with sign sigma of the lane's own modes, the partner's sign is -sigma
     if (kt) {
-        float pa[8], pb[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            pa[k] = lane_xor8(a[k]) * sigma;
-            pb[k] = lane_xor8(b[k]) * sigma;
-        }
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float oa = a[k] * k0[k] + pa[k] * k1[k] - b[k] * k2[k] - pb[k] * k3[k];
-            float ob = b[k] * k0[k] + pb[k] * k1[k] + a[k] * k2[k] + pa[k] * k3[k];
-            a[k] = oa;
-            b[k] = ob;
+        for (int j = 0; j < 4; j++) {
+            const float k0 = kk[0][j], k1 = kk[1][j], k2 = kk[2][j], k3 = kk[3][j];
+            const float spa = sigma * pa[j], spb = sigma * pb[j], soa = sigma * oa[j], sob = sigma * ob[j];
+            const float na = oa[j] * k0 + spa * k1 - ob[j] * k2 - spb * k3;
+            const float nb = ob[j] * k0 + spb * k1 + oa[j] * k2 + spa * k3;
+            const float npa = pa[j] * k0 - soa * k1 - pb[j] * k2 + sob * k3;
+            const float npb = pb[j] * k0 - sob * k1 + pa[j] * k2 - soa * k3;
+            oa[j] = na; ob[j] = nb; pa[j] = npa; pb[j] = npb;
         }
     }
-    // ---- fractional shift (ImageDtt.java:5490-5584): horizontal by rx (kh), vertical by ry (kv)
-    {
-        float sh, ch, cv[8], sv[8];
+    {   // horizontal shift by rx (kh), vertical by ry (kv = 4 hs + j)
+        float sh, ch, cv, sv, c1, s1;
         sincos_pi((kh + 0.5f) * rx * 0.125f, &sh, &ch);
-        half_phase8(ry * 0.125f, cv, sv);
+        sincos_pi((4 * hs + 0.5f) * ry * 0.125f, &sv, &cv);   // then rotations by pi ry / 8
+        sincos_pi(ry * 0.125f, &s1, &c1);
+        const float ssh = sigma * sh;
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float pa = lane_xor8(a[k]) * sigma;
-            float pb = lane_xor8(b[k]) * sigma;
-            float ta = a[k] * ch + pa * sh;
-            float tb = b[k] * ch + pb * sh;
-            a[k] = ta * cv[k] - tb * sv[k];
-            b[k] = tb * cv[k] + ta * sv[k];
+        for (int j = 0; j < 4; j++) {
+            const float ta = oa[j] * ch + pa[j] * ssh, tb = ob[j] * ch + pb[j] * ssh;
+            const float tpa = pa[j] * ch - oa[j] * ssh, tpb = pb[j] * ch - ob[j] * ssh;
+            oa[j] = ta * cv - tb * sv;
+            ob[j] = tb * cv + ta * sv;
+            pa[j] = tpa * cv - tpb * sv;
+            pb[j] = tpb * cv + tpa * sv;
+            const float cn = fmaf(cv, c1, -sv * s1), sn = fmaf(sv, c1, cv * s1);
+            cv = cn;
+            sv = sn;
         }
     }
     if (lpf && ((lpf_mask >> color) & 1)) {
+        const float4 f = *reinterpret_cast<const float4*>(lpf + color * 64 + kh * 8 + 4 * hs);
+        const float fv[4] = {f.x, f.y, f.z, f.w};
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float f = lpf[color * 64 + kh * 8 + k];
-            a[k] *= f;
-            b[k] *= f;
+        for (int j = 0; j < 4; j++) {
+            oa[j] *= fv[j];
+            ob[j] *= fv[j];
+            pa[j] *= fv[j];
+            pb[j] *= fv[j];
         }
     }
-    // ---- store through the unit's LDS: the rows of lane (hs, kh) go to their places in the
-    // unit's 256 coefficients, then lane l stores coefficients 4l .. 4l + 3 of each mode, so every
-    // store instruction writes whole 256-byte runs (instead of 16 bytes in every 32).  The modes sit
-    // CONV_MODE floats apart in LDS, so the 16-byte writes of the two lane halves (hs = 0, 1) land on
-    // distinct banks.
-    constexpr int CONV_MODE = 68;
+    // ---- store through the unit's LDS: each lane's four kv-halves go to their places in the
+    // unit's 256 coefficients (the two lanes of a pair write disjoint banks), then lane l stores
+    // coefficients 4l .. 4l + 3 of each mode, so every store instruction writes whole 256-byte runs
     wave_sync();   // stage V's reads of sb are done
-    *reinterpret_cast<float4*>(&sb[hs * CONV_MODE + kh * 8]) = make_float4(a[0], a[1], a[2], a[3]);
-    *reinterpret_cast<float4*>(&sb[hs * CONV_MODE + kh * 8 + 4]) = make_float4(a[4], a[5], a[6], a[7]);
-    *reinterpret_cast<float4*>(&sb[(hs | 2) * CONV_MODE + kh * 8]) = make_float4(b[0], b[1], b[2], b[3]);
-    *reinterpret_cast<float4*>(&sb[(hs | 2) * CONV_MODE + kh * 8 + 4]) = make_float4(b[4], b[5], b[6], b[7]);
+    {
+        const int o = kh * 8 + 4 * hs;
+        *reinterpret_cast<float4*>(&sb[hs * 64 + o]) = make_float4(oa[0], oa[1], oa[2], oa[3]);
+        *reinterpret_cast<float4*>(&sb[(hs | 2) * 64 + o]) = make_float4(ob[0], ob[1], ob[2], ob[3]);
+        *reinterpret_cast<float4*>(&sb[(hs ^ 1) * 64 + o]) = make_float4(pa[0], pa[1], pa[2], pa[3]);
+        *reinterpret_cast<float4*>(&sb[((hs ^ 1) | 2) * 64 + o]) = make_float4(pb[0], pb[1], pb[2], pb[3]);
+    }
     wave_sync();
     if (active) {
         const size_t tile = (size_t)ty * g.tiles_x + tx;
         float* o = clt + frame * clt_fstride + (((size_t)cam * g.tiles_x * g.tiles_y + tile) * g.colors + color) * 256 + 4 * l;
 #pragma unroll
         for (int m = 0; m < 4; m++)
-            *reinterpret_cast<float4*>(o + m * 64) = *reinterpret_cast<const float4*>(&sb[m * CONV_MODE + 4 * l]);
+            *reinterpret_cast<float4*>(o + m * 64) = *reinterpret_cast<const float4*>(&sb[m * 64 + 4 * l]);
     }
 }
 
