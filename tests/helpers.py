@@ -118,6 +118,34 @@ def corr_moved(corr_tol=1e-3, disp_tol=0.01, layers=(0, 10, 1, 8, 3, 5)):
     return moved
 
 
+def tex_moved(bar=1e-2, planes=slice(0, 3)):
+    """moved() for texture tiles [task][planes][16][16]: a pixel of `planes` (RGB) moved by more
+    than `bar`."""
+    import numpy as np
+
+    def moved(out, base):
+        n = len(base)
+        return np.nan_to_num(np.abs(out[:, planes] - base[:, planes]).reshape(n, -1), nan=0.0).max(1) > bar
+    return moved
+
+
+def confirm_tex_flips(orc, g, clt64, tasks, idx, extra=None, draws=24, seed=7, bar=1e-2):
+    """The subset of the tasks idx whose FP64 texture tiles move past `bar` (an RGB pixel) under
+    1-ulp perturbations of the CLT, over `draws` draws of those tasks alone.  Such a tile has a
+    pixel whose two-best-port choice (ID:2893-2910) is a near-tie below float32 resolution (the
+    port weights of the second and third port equal to a few 1e-6): any float32 computation may
+    pick either port there, and the pixel then takes the other pair's mix."""
+    import numpy as np
+    idx = np.asarray(idx, np.int64)
+    if idx.size == 0:
+        return idx
+    sub = np.ascontiguousarray(tasks[idx])
+    ex = None if extra is None else np.ascontiguousarray(np.asarray(extra, np.float64)[idx])
+    bad, _ = fp32_sensitive(lambda c: orc.textures_frame(g, c, sub, extra_disparity=ex)[0], clt64, tex_moved(bar),
+                            draws=draws, seed=seed)
+    return idx[bad]
+
+
 def fp32_ill_conditioned(orc, g, clt64, tasks, draws=3, seed=1, pairs=None, nthreads=None, lma=None):
     """fp32_sensitive for the oracle's correlate2D with the LMA chain: (mask, (corr, combo, disp))."""
     return fp32_sensitive(lambda c: orc.corr_frame(g, c, tasks, pairs=pairs, nthreads=nthreads, lma=lma), clt64,

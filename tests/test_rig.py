@@ -22,7 +22,8 @@ the tiles whose FP64 result is determined at float32 precision, helpers.fp32_sen
 import numpy as np
 import pytest
 
-from helpers import TEX_TOL, fp32_sensitive, gpu_clt_cpu_layout, gpu_processor, log, oracle_geom, rel_rms, tex_planes
+from helpers import (TEX_TOL, confirm_tex_flips, fp32_sensitive, gpu_clt_cpu_layout, gpu_processor, log, oracle_geom,
+                     rel_rms, tex_moved, tex_planes)
 
 RIG_OFF = np.array([2.0, 0.0])   # aux quad centre, in main-quad disparity units
 AUX_SCALE = 0.75                 # aux baseline / main baseline
@@ -266,7 +267,9 @@ def test_gpu_rig_rejects_other_camera_counts(pk):
 @pytest.mark.gpu
 def test_gpu_rig_textures(pk, orc):
     """generateTextureTiles of each quad (ID:9360-9398) against the oracle's textures of the same
-    cameras, on the device CLT, with the quad's device disparity as the extra disparity."""
+    cameras, on the device CLT, with the quad's device disparity as the extra disparity.  A tile with
+    an RGB pixel off by more than 1e-2 must be a confirmed float32 near-tie of the two-best-port
+    choice (confirm_tex_flips); at most 2 % of the tiles may be, and the bars hold on the rest."""
     log("case: two-quad rig 128x96 textures of both quads")
     sc = rig_scene(pk, 128, 96, 9, disparity=("field", 1.0, 4.0), target_disparity=2.0)
     tp = gpu_processor(pk, sc)
@@ -283,11 +286,17 @@ def test_gpu_rig_textures(pk, orc):
         extra = np.nan_to_num(b[:, k].astype(np.float64))
         tasks4 = np.ascontiguousarray(np.concatenate([sc.tasks[:, :2], sc.tasks[:, 2 + 8 * q:10 + 8 * q]], axis=1))
         o_tiles, o_stats = orc.textures_frame(g4, clt[4 * q:4 * q + 4], tasks4, extra_disparity=extra)
+        over = np.nonzero(tex_moved()(tiles, o_tiles))[0]
+        flips = confirm_tex_flips(orc, g4, clt[4 * q:4 * q + 4], tasks4, over, extra=extra)
+        keep = np.ones(len(tiles), bool)
+        keep[over] = False
         res = {}
         for name, sl in tex_planes(4):
-            res[name] = rel_rms(tiles[:, sl], o_tiles[:, sl])
-        res["stats"] = rel_rms(stats, o_stats)
-        log("  quad %d: " % q + ", ".join("%s=%.3g" % kv for kv in res.items()))
+            res[name] = rel_rms(tiles[keep, sl], o_tiles[keep, sl])
+        res["stats"] = rel_rms(stats[keep], o_stats[keep])
+        log("  quad %d: " % q + ", ".join("%s=%.3g" % kv for kv in res.items()) +
+            "; tiles past 1e-2: %s, confirmed near-ties: %s" % (over.tolist(), flips.tolist()))
+        assert np.array_equal(np.sort(flips), over) and len(over) <= 0.02 * len(tiles)
         for name, tol in TEX_TOL.items():
             assert res[name] < tol, name
         assert res["stats"] < 1e-4
