@@ -947,7 +947,9 @@ __global__ __launch_bounds__(256) void k_corr2d(const float* __restrict__ clt, c
 // (lane (ay, ax) writes the up to 4 outputs (7 +- ay, 7 +- ax)); strips with tap addresses shared
 // by the pairs of one orientation.
 // ------------------------------------------------------------------------------------
-constexpr int CQ_WAVE_FLOATS = 256 + 6 * CLT_CORR_LEN + 240 + 32 + 2;   // per wavefront, multiple of 4
+// LDS per wavefront, a multiple of 4: the transform staging (256 floats, then the combined strip),
+// the six correlation tiles and the two 1-D profiles
+constexpr int CQ_WAVE_FLOATS = 256 + 6 * CLT_CORR_LEN + 32 + 2;
 
 // One sample of a strip element (C2:610-718): position of (row, scol) for orientation DIR with
 // interpolation denominator 1 << LG, sample side down = -1 / +1.  Returns false outside the
@@ -1020,7 +1022,7 @@ __device__ __forceinline__ int argmax16(float v, int idx) {
 
 
 template <bool FZ>   // FZ: fat zero in use (the per-bin normaliser needs the mean over the bins)
-__global__ __launch_bounds__(256, FZ ? 4 : 5) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+__global__ __launch_bounds__(256, FZ ? 4 : 6) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                      int task_words, int num_tasks, const float* __restrict__ lpf,
                                                      const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
                                                      float* __restrict__ combo_out, float* __restrict__ disp_out,
@@ -1033,8 +1035,8 @@ __global__ __launch_bounds__(256, FZ ? 4 : 5) void k_corr2d_quad(const float* __
     if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
     float* S = s_dyn + wave * CQ_WAVE_FLOATS;         // [4][64] transform staging
     float* s_corr = S + 256;                          // [6][225]
-    float* s_combo = s_corr + 6 * CLT_CORR_LEN;       // [240]
-    float* s_c1d = s_combo + 240;                     // [2][16]
+    float* s_combo = S;                               // [240], once the pairs are done with S
+    float* s_c1d = s_corr + 6 * CLT_CORR_LEN;         // [2][16]
     const float qnan = __int_as_float(0x7fc00000);
     const int hw = A.strip_height, slen = hw * 15;
     const int32_t* rec = tasks + (size_t)t * task_words;
