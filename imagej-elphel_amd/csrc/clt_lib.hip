@@ -237,6 +237,11 @@ struct clt_handle {
         double* tmp = nullptr;     // [cap_tasks][8]
     } lma_scr[2];
     bool lma_args_valid = false;
+    // the correlation / texture arguments last uploaded to d_corr_args / d_tex_args
+    CorrArgs corr_args_dev;
+    bool corr_args_valid = false;
+    TexArgs tex_args_dev;
+    bool tex_args_valid = false;
     bool lma_run = false;          // correlate2D is followed by k_corr_lma
     // k_corr_lma runs on its own stream, concurrently with the kernels that follow correlate2D
     // (textures, imclt, the next convert); ev_lma joins it back before its outputs are read or
@@ -323,6 +328,21 @@ static int fail(clt_handle* h, int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return fail((h), CLT_E_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+// Upload of a launch-argument struct to dst on the main stream, skipped when the device already
+// holds the same bytes (an upload from pageable memory holds the stream for about 20 us, which a
+// frame of unchanged parameters need not pay).  The kernels reading dst run on the main stream,
+// after the upload that wrote their bytes.
+template <class T>
+static hipError_t upload_args(clt_handle* h, T* dst, const T& a, T& shadow, bool& valid) {
+    if (valid && std::memcmp(&a, &shadow, sizeof(T)) == 0) return hipSuccess;
+    valid = false;
+    const hipError_t e = hipMemcpyAsync(dst, &a, sizeof(T), hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) return e;
+    std::memcpy(&shadow, &a, sizeof(T));
+    valid = true;
+    return hipSuccess;
+}
+
 static int dalloc(clt_handle* h, void** p, size_t bytes) {
     if (*p) return CLT_OK;
     hipError_t e = hipMalloc(p, bytes);
@@ -366,8 +386,12 @@ static void tbegin(clt_handle* h, int which, hipStream_t st = nullptr) {
     if (!h->timing) return;
     const int u = h->ev_used[which];
     if (u == (int)h->ev_beg[which].size()) {
+        // timing only (read after the stream is synchronised): no system-scope fence, which would
+        // write back and invalidate the caches between the kernels it brackets
         hipEvent_t b = nullptr, e = nullptr;
-        if (hipEventCreate(&b) != hipSuccess || hipEventCreate(&e) != hipSuccess) return;
+        if (hipEventCreateWithFlags(&b, hipEventDisableSystemFence) != hipSuccess ||
+            hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess)
+            return;
         h->ev_beg[which].push_back(b);
         h->ev_end[which].push_back(e);
     }
@@ -1473,7 +1497,7 @@ static int prepare_corr(clt_handle* h) {
         (rc = ensure_capacity(h, &h->d_combo, &h->combo_cap, (size_t)h->cap_tasks * 16 * 15)) ||
         (rc = dalloc(h, (void**)&h->d_disp, (size_t)h->cap_tasks * CLT_DISP_LAYERS * 4)))
         return rc;
-    HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, upload_args(h, h->d_corr_args, a, h->corr_args_dev, h->corr_args_valid));
     if ((rc = build_lma_args(h, a.num_strip > 0))) return rc;
     h->corr_quad = quad;
     h->corr_num_strip = a.num_strip;
@@ -1634,7 +1658,7 @@ static int prepare_rig(clt_handle* h, const float* target) {
         r.square = rp.max_corr_radius < 0;
     }
     join_lma(h);
-    HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, upload_args(h, h->d_corr_args, a, h->corr_args_dev, h->corr_args_valid));
     HIPCHK(h, hipMemcpyAsync(h->d_lma_rig, &la, sizeof(la), hipMemcpyHostToDevice, h->stream));
     HIPCHK(h, hipMemcpyAsync(h->d_rig_args, ra, sizeof(ra), hipMemcpyHostToDevice, h->stream));
     h->rig_target = target != nullptr;
@@ -1766,7 +1790,7 @@ static int prepare_tex(clt_handle* h) {
         return rc;
     TexArgs a;
     fill_tex_args(h, g.num_cams, h->tp.port_offsets, a);
-    HIPCHK(h, hipMemcpyAsync(h->d_tex_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, upload_args(h, h->d_tex_args, a, h->tex_args_dev, h->tex_args_valid));
     h->tex_planes_at_exec = planes;
     return CLT_OK;
 }
@@ -2113,7 +2137,7 @@ int clt_exec_geometry_adjoint_noalloc(clt_handle* h, double* grad_extrinsic, flo
     const int blocks = (n + GEO_ADJ_TASKS - 1) / GEO_ADJ_TASKS;
     join_lma(h);   // kernels of earlier calls on the other streams finish first
     join_side(h);
-    HIPCHK(h, hipMemcpyAsync(h->d_corr_args, &a, sizeof(a), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, upload_args(h, h->d_corr_args, a, h->corr_args_dev, h->corr_args_valid));
     tbegin(h, T_GEOADJ);
     hipLaunchKernelGGL(k_port_centres, dim3((4 * n + 255) / 256), dim3(256), 0, h->stream, h->d_port_geom, h->d_rtab,
                        h->d_gwords, h->d_gtxy, h->d_gdisp, n, h->task_words, h->d_tasks, nullptr, 0, 1.0);
@@ -2625,9 +2649,9 @@ int clt_set_timing(clt_handle* h, int32_t enable) {
     h->timing = enable != 0;
     for (int i = 0; i < T_NUM && enable > 1; i++)   // events for `enable` launches of each kernel, created now
         while ((int)h->ev_beg[i].size() < enable) {
-            hipEvent_t b = nullptr, e = nullptr;
-            HIPCHK(h, hipEventCreate(&b));
-            if (hipEventCreate(&e) != hipSuccess) {
+            hipEvent_t b = nullptr, e = nullptr;   // timing only, as in tbegin
+            HIPCHK(h, hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
+            if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
                 (void)hipEventDestroy(b);
                 return fail(h, CLT_E_HIP, "clt_set_timing: hipEventCreate failed");
             }
