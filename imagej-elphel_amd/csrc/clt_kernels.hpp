@@ -2274,23 +2274,24 @@ __global__ __launch_bounds__(256) void k_textures(const float* __restrict__ clt,
 // above the segment (its bottom half only); the lower half of each unit's tile (lanes n >= 8) waits
 // in LDS until the next tile adds its upper half (lanes n < 8).  The two halves
 // of a tile's 16 columns overlap the neighbouring columns, so the columns run in two launches:
-// parity 0 (even tx) stores its 16 output columns, parity 1 (odd tx) adds to them; when tiles_x is
-// even the last column's right half has no even neighbour and is stored.  Requires NC * 3 <= 16, no
-// extra disparity, and at most one task per tile with every task's flags non-zero (tile_map then
-// names every task exactly once).
+// PAR = 0 (even tx) stores its 16 output columns, PAR = 1 (odd tx) adds to them; when tiles_x is
+// even the last column's right half has no even neighbour and is stored.  The even launch reads no
+// image planes, so with the 16-float LDS rows (LAY 1) it fits 96 VGPRs and runs 5 waves per SIMD.
+// Requires NC * 3 <= 16, no extra disparity, and at most one task per tile with every task's flags
+// non-zero (tile_map then names every task exactly once).
 // ====================================================================================
-template <int NC, int LAY>
-__global__ __launch_bounds__(256, NC <= 4 ? 4 : 3) void k_tex_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+template <int NC, int LAY, int PAR>
+__global__ __launch_bounds__(256, NC <= 4 ? ((PAR == 0 && LAY == 1) ? 5 : 4) : 3) void k_tex_imclt(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
                                                    int task_words, const int32_t* __restrict__ tile_map,
                                                    const float* __restrict__ lpf, const TexArgs* __restrict__ args_p,
                                                    float* __restrict__ tiles_out, float* __restrict__ stats_out,
-                                                   float* __restrict__ rbg, int seg_rows, int parity, GeomDev g,
+                                                   float* __restrict__ rbg, int seg_rows, GeomDev g,
                                                    long long clt_fstride) {
     static_assert(NC * 3 <= 16, "at most four units per wavefront");
     constexpr int NU = NC * 3, J = (NU + 3) / 4;
     __shared__ __attribute__((aligned(16))) TexShared<NC, LAY> S;
     const TexArgs& A = *args_p;
-    const int tx = 2 * (int)blockIdx.x + parity;
+    const int tx = 2 * (int)blockIdx.x + PAR;
     if (tx >= g.tiles_x) return;   // whole block
     const int by0 = blockIdx.y * seg_rows;
     const int nby = min(seg_rows, g.tiles_y + 1 - by0);
@@ -2305,8 +2306,8 @@ __global__ __launch_bounds__(256, NC <= 4 ? 4 : 3) void k_tex_imclt(const float*
     tex_lane_setup(L, lpf);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, gq = lane >> 4, n = lane & 15;
     // right half stored, not added: the last column when no even column follows it
-    const bool add_left = parity == 1;
-    const bool add_right = parity == 1 && !(tx == g.tiles_x - 1 && (g.tiles_x & 1) == 0);
+    const bool add_left = PAR == 1;
+    const bool add_right = PAR == 1 && !(tx == g.tiles_x - 1 && (g.tiles_x & 1) == 0);
     const bool add_mine = (gq < 2) ? add_left : add_right;   // columns 4 gq .. 4 gq + 3 of the tile
     // lower halves (rows 8-15) of the previous tile's units, [unit][row - 8][lane group]: written by
     // lanes n >= 8, read by lanes n < 8 of the same wavefront one tile later (the first tile of a

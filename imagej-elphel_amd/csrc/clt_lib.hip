@@ -523,11 +523,16 @@ static void launch_tex_imclt_nc(clt_handle* h, int f0, int frames) {
     const int rows = h->tex_imclt_rows;
     const dim3 grid((h->tiles_x + 1) / 2, (h->tiles_y + 1 + rows - 1) / rows, frames);
     const size_t plane_px = (size_t)(h->g.width + 8) * (h->g.height + 8);
-    for (int parity = 0; parity < 2; parity++)   // even columns store, odd columns add
-        hipLaunchKernelGGL((k_tex_imclt<NC, LAY>), grid, dim3(256), 0, h->stream, h->d_clt + (size_t)f0 * h->clt_fstride,
-                           h->d_tasks, h->task_words, h->d_tile_map + (size_t)f0 * h->num_tiles, h->d_lpf_corr,
-                           h->d_tex_args, h->d_tex, h->d_tex_stats, h->d_rbg + (size_t)f0 * NC * 3 * plane_px, rows,
-                           parity, h->gd, h->clt_fstride);
+    // even columns store, odd columns add (with the swizzled 16-float LDS rows the even launch, which
+    // reads no image planes, fits 96 VGPRs: 5 waves per SIMD for NC <= 4)
+    hipLaunchKernelGGL((k_tex_imclt<NC, LAY, 0>), grid, dim3(256), 0, h->stream, h->d_clt + (size_t)f0 * h->clt_fstride,
+                       h->d_tasks, h->task_words, h->d_tile_map + (size_t)f0 * h->num_tiles, h->d_lpf_corr,
+                       h->d_tex_args, h->d_tex, h->d_tex_stats, h->d_rbg + (size_t)f0 * NC * 3 * plane_px, rows, h->gd,
+                       h->clt_fstride);
+    hipLaunchKernelGGL((k_tex_imclt<NC, LAY, 1>), grid, dim3(256), 0, h->stream, h->d_clt + (size_t)f0 * h->clt_fstride,
+                       h->d_tasks, h->task_words, h->d_tile_map + (size_t)f0 * h->num_tiles, h->d_lpf_corr,
+                       h->d_tex_args, h->d_tex, h->d_tex_stats, h->d_rbg + (size_t)f0 * NC * 3 * plane_px, rows, h->gd,
+                       h->clt_fstride);
 }
 template <int NC>
 static void launch_tex_imclt_lay(clt_handle* h, int f0, int frames) {
