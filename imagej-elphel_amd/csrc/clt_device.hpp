@@ -65,7 +65,7 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks) {
     return (x < rem) ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
 }
 
-// Order of a task list over the blocks: the list is cut into bands of BAND rows of a dense
+// Order of a task list over the blocks: the list is cut into bands of B = TASK_BAND rows of a dense
 // row-major list W tiles wide (positions pBW .. pBW + BW - 1), and a band is taken column by
 // column, so the tiles that share an aberration kernel tile (kernel grid at twice the tile step,
 // above and beside each other) and window rows run within a few blocks of each other, while those
