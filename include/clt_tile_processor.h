@@ -468,8 +468,9 @@ int  clt_exec_batch(clt_handle* h, int32_t stages);
 /* Times the candidate launch shapes on the current batch and keeps the fastest: tiles per
  * workgroup of correlate2D (1, 2, 4: its LDS per workgroup), tile rows per imclt segment (8, 16,
  * 32), tile rows per segment of the one-pass textures/imclt (4, 8, 16) with its LDS tile layout
- * (rows 20 floats apart, or 16 with swizzled 16-byte column groups) and frame chunks of the
- * pipeline (1, 2, 4, 8, at most num_frames). */
+ * (rows 20 floats apart, or 16 with swizzled 16-byte column groups, with which its even-column
+ * launch runs 5 wavefronts per SIMD instead of 4) and frame chunks of the pipeline (1, 2, 4, 8, at
+ * most num_frames). */
 int  clt_autotune(clt_handle* h);
 int  clt_get_tune(clt_handle* h, int32_t* corr_tiles_per_block, int32_t* imclt_rows);
 /* values[0..n-1] (n <= 5): correlate2D tiles per workgroup, imclt rows, frame chunks, one-pass rows,
