@@ -2,11 +2,12 @@
 
 Workload (one "step" = one frame through the plain per-frame GPU path): 4 x 2592x1936 Bayer
 (RGGB) float32 mosaics, 78,408 tiles, dense task list, per-tile disparity a smooth random field
-0..20 px, random per-tile aberration kernels; convert_direct -> textures (RGBA) and imclt_rbg
-(with LPF) in one pass over the converted tiles (clt_exec_textures_imclt) -> correlate2D (6 pairs +
-combined strip + disparity, per-tile LMA chain).  The LMA chain (on its own stream) overlaps the
-next frame's convert and textures/imclt pass.  Inputs are resident in HBM before the timed region;
-outputs stay on the device.
+0..20 px, random per-tile aberration kernels; convert_direct and correlate2D (6 pairs + combined
+strip + disparity, per-tile LMA chain) in one kernel (clt_exec_convert_corr2d: the CLT passes from
+one to the other in LDS) -> textures (RGBA) and imclt_rbg (with LPF) in one pass over the converted
+tiles (clt_exec_textures_imclt).  The LMA chain (on its own stream) overlaps the textures/imclt
+pass and the next frame's conversion.  Inputs are resident in HBM before the timed region; outputs
+stay on the device.
 
 Prints one JSON line: tiles/s, HBM roofline of the dominant kernel (HIP events on the library's
 stream), the CPU baseline (the C restatement of the reference CPU path in FP64 and FP32 on the
@@ -18,6 +19,7 @@ import glob
 import importlib
 import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -48,6 +50,9 @@ def algorithmic_bytes(sc, planes, n_pairs, strip_h):
     return {
         "convert": C * W * H * 4 + used * K * (256 + 8) * 4 + T * tw + clt,
         "corr2d": clt + T * tw + T * n_pairs * 225 * 4 + T * strip_h * 15 * 4 + T * 12 * 4,
+        # convert and correlate2D in one kernel: the CLT written once (for the textures), not read back
+        "convert_corr2d": C * W * H * 4 + used * K * (256 + 8) * 4 + T * tw + clt + T * n_pairs * 225 * 4
+                          + T * strip_h * 15 * 4 + T * 12 * 4,
         "textures": clt + T * tw + T * planes * 1024 + T * 4 * C * 4,
         "imclt": clt + C * K * (W + 8) * (H + 8) * 4 + (W // 8) * (H // 8) * 4,
         # textures and imclt_rbg in one pass: the CLT once, both outputs, the tile map
@@ -177,11 +182,12 @@ def copy_calibration(mib):
     print("bench: copied %d MiB device to device" % mib, file=sys.stderr, flush=True)
 
 
-KERNEL_NAMES = {"convert": "k_convert", "corr2d": "k_corr2d", "textures": "k_textures", "imclt": "k_imclt",
-                "tex_imclt": "k_tex_imclt"}
+# kernel name patterns (demangled names of the counter runs)
+KERNEL_NAMES = {"convert": r"k_convert\(", "corr2d": r"k_corr2d(_quad)?<", "textures": r"k_textures<",
+                "imclt": r"k_imclt\(", "tex_imclt": r"k_tex_imclt<", "convert_corr2d": r"k_convert_corr<"}
 CHILD_FRAMES = 2   # frames of the profiled child run
 COPY_MIB = 256
-STEP_KERNELS = ("convert", "corr2d", "tex_imclt")   # the calls of one step (their event timings)
+STEP_KERNELS = ("convert_corr2d", "tex_imclt")   # the calls of one step (their event timings)
 
 
 def pmc_traffic(seed):
@@ -214,7 +220,7 @@ def pmc_traffic(seed):
                 return None, "no copy dispatch in the %s run" % counter
             factors[name.split("_")[0]] = (COPY_MIB << 20) / (copy * 1024.0)
             for key, kname in KERNEL_NAMES.items():
-                v = [x for k, xs in per.items() if kname in k for x in xs]
+                v = [x for k, xs in per.items() if re.search(kname, k) for x in xs]
                 if v:
                     vals.setdefault(key, {})[name] = sum(v) / CHILD_FRAMES * 1024.0
     except subprocess.TimeoutExpired:
@@ -250,14 +256,13 @@ def make_processor(pk, sc):
 
 
 def run_step(tp):
-    """One frame.  The per-tile LMA chain that correlate2D starts on its own stream overlaps the
-    next frame's convert (which waits on memory and leaves the vector ALUs idle a third of the
-    time) and one-pass textures/imclt_rbg; the one pass runs the inverse CLT of each tile once for
-    both outputs.  Textures/imclt before correlate2D: the one-pass kernel (4 waves per SIMD) then
-    shares the GPU with the previous frame's LMA chain for less of its run."""
-    tp.execConverCorrectTiles(0)
+    """One frame: execConverCorrectTiles(0) and execCorrelate2D() as one call (one kernel per task
+    block, the CLT handed from the conversion to the correlation in LDS and written once, for the
+    textures), then textures and imclt_rbg in one pass over the converted tiles (the inverse CLT of
+    each tile once for both outputs).  The per-tile LMA chain that correlate2D starts on its own
+    stream overlaps the one-pass kernel and the next frame's conversion."""
+    tp.execConvertCorrelate2D(0)
     tp.execTexturesImclt(True)
-    tp.execCorrelate2D()
 
 
 def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
@@ -286,7 +291,7 @@ def batch_bench(pk, sc, frames=32, seed=7, repeats=3):
     for ft in frame_tasks:
         tp.setTasks(ft)
         run_step(tp)
-    plain_gpu = sum(tp.getTiming(k)[0] for k in ("convert", "corr2d", "corr_lma", "tex_imclt"))
+    plain_gpu = sum(tp.getTiming(k)[0] for k in ("convert_corr2d", "corr_lma", "tex_imclt"))
     tp.close()
     # batched
     bt = pk.GPUTileBatch(frames, W, H, C, K, red_row=RED[0], red_col=RED[1])

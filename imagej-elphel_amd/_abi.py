@@ -18,7 +18,7 @@ DISP_NAMES = ["int", "cm", "strength", "hor", "hor_strength", "vert", "vert_stre
               "poly", "poly_strength", "cm_raw", "strength_raw"]
 TIMING_KERNELS = {"convert": 0, "imclt": 1, "corr2d": 2, "textures": 3, "combine_textures": 4, "batch": 5,
                   "corr_adjoint": 6, "geometry_adjoint": 7, "corr_lma": 8,
-                  "kernel_prep": 9, "rig": 10, "tex_imclt": 11}
+                  "kernel_prep": 9, "rig": 10, "tex_imclt": 11, "convert_corr2d": 12}
 
 
 class clt_geom(ctypes.Structure):
@@ -142,6 +142,7 @@ SIGNATURES = {
     "clt_exec_convert": (_I, [_H, _I]),
     "clt_exec_imclt": (_I, [_H, _I]),
     "clt_exec_corr2d": (_I, [_H]),
+    "clt_exec_convert_corr2d": (_I, [_H, _I]),
     "clt_exec_textures": (_I, [_H, _P]),
     "clt_exec_textures_imclt": (_I, [_H, _P, _I]),
     "clt_exec_combine_textures": (_I, [_H]),

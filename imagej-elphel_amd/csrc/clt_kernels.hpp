@@ -51,13 +51,16 @@ constexpr int CONV_UNIT = 336;
 // per unit.  Every row of a Bayer colour plane holds samples of one column parity only, so its
 // sine fold transform is the reversed cosine one: DST-IV(fold_S x)[k] = s * DCT-IV(fold_C x)[7-k],
 // s = -1 when the samples sit at odd window columns (same along columns for red and blue).
-__global__ __launch_bounds__(192) void k_convert(const float* __restrict__ images, const float* __restrict__ kernels,
-                                                 const float* __restrict__ koffsets, const int32_t* __restrict__ tasks,
-                                                 int task_words, int num_tasks, float* __restrict__ clt,
-                                                 const float* __restrict__ lpf, int lpf_mask, double inv_kstep,
-                                                 const int32_t* __restrict__ task_frame, long long img_fstride,
-                                                 long long clt_fstride, GeomDev g) {
-    __shared__ __attribute__((aligned(16))) float s_buf[12][CONV_UNIT];
+// The block's work; returns false when the wavefront has no active unit (it stored nothing), t_out
+// = the task of the wavefront's first unit.  On return true the units' CLT is also in s_buf
+// ([unit][mode * 64 + kh * 8 + kv], unit = colour * 4 + unit of the wavefront).
+__device__ __forceinline__ bool convert_tiles(const float* __restrict__ images, const float* __restrict__ kernels,
+                                              const float* __restrict__ koffsets, const int32_t* __restrict__ tasks,
+                                              int task_words, int num_tasks, float* __restrict__ clt,
+                                              const float* __restrict__ lpf, int lpf_mask, double inv_kstep,
+                                              const int32_t* __restrict__ task_frame, long long img_fstride,
+                                              long long clt_fstride, const GeomDev& g, float (*s_buf)[CONV_UNIT],
+                                              int& t_out) {
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     const int color = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // wave-uniform
     const int unit_in_block = threadIdx.x >> 4, l = threadIdx.x & 15;
@@ -66,9 +69,10 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
     bool active = u < total;
     if (!active) u = 0;
     const int t = band_order(u / g.cams, num_tasks, g.tiles_x), cam = u % g.cams;
+    t_out = t;
     const int32_t* rec = tasks + (size_t)t * task_words;
     active = active && (rec[0] != 0);
-    if (!__any(active)) return;   // whole wave idle (no block-level barriers in this kernel)
+    if (!__any(active)) return false;   // whole wave idle (no block-level barriers here)
     const long long frame = (task_frame && active) ? task_frame[t] : 0;   // batched frames
     const bool bayer = g.colors == 3;
     const int txy = rec[1];
@@ -294,6 +298,18 @@ __global__ __launch_bounds__(192) void k_convert(const float* __restrict__ image
         for (int m = 0; m < 4; m++)
             *reinterpret_cast<float4*>(o + m * 64) = *reinterpret_cast<const float4*>(&sb[m * 64 + 4 * l]);
     }
+    return true;
+}
+__global__ __launch_bounds__(192) void k_convert(const float* __restrict__ images, const float* __restrict__ kernels,
+                                                 const float* __restrict__ koffsets, const int32_t* __restrict__ tasks,
+                                                 int task_words, int num_tasks, float* __restrict__ clt,
+                                                 const float* __restrict__ lpf, int lpf_mask, double inv_kstep,
+                                                 const int32_t* __restrict__ task_frame, long long img_fstride,
+                                                 long long clt_fstride, GeomDev g) {
+    __shared__ __attribute__((aligned(16))) float s_buf[12][CONV_UNIT];
+    int t;
+    (void)convert_tiles(images, kernels, koffsets, tasks, task_words, num_tasks, clt, lpf, lpf_mask, inv_kstep,
+                        task_frame, img_fstride, clt_fstride, g, s_buf, t);
 }
 
 // ====================================================================================
@@ -623,9 +639,11 @@ constexpr int QP_LG[6] = {1, 1, 1, 1, 0, 0};
 // matrix at [row n][column 4g + s]; pass 2 (contraction over (qh, kh)) takes A = u[s] and B = pass
 // 1's register s.  The result, (qh, x) = (g >> 1, 4 (g & 1) + r) and (qv, y) = (n >> 3, n & 7) in
 // register r, goes back to S as [q][y][x] (one 16-byte write) for the unfold.
+// QS: floats between the quadrants' 64-float blocks at S (16-byte aligned).
+template <int QS = 64>
 __device__ __forceinline__ void corr_dct_mx(float* S, const float (&u)[4], int lane) {
     const int g = lane >> 4, n = lane & 15;
-    const float4 a = *reinterpret_cast<const float4*>(S + (2 * (g >> 1) + (n >> 3)) * 64 + (n & 7) * 8 + 4 * (g & 1));
+    const float4 a = *reinterpret_cast<const float4*>(S + (2 * (g >> 1) + (n >> 3)) * QS + (n & 7) * 8 + 4 * (g & 1));
     const float av[4] = {a.x, a.y, a.z, a.w};
     float4x d1 = {0.f, 0.f, 0.f, 0.f}, d2 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -633,41 +651,50 @@ __device__ __forceinline__ void corr_dct_mx(float* S, const float (&u)[4], int l
 #pragma unroll
     for (int s = 0; s < 4; s++) d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(u[s], d1[s], d2, 0, 0, 0);
     wave_sync();
-    *reinterpret_cast<float4*>(S + (2 * (n >> 3) + (g >> 1)) * 64 + (n & 7) * 8 + 4 * (g & 1)) =
+    *reinterpret_cast<float4*>(S + (2 * (n >> 3) + (g >> 1)) * QS + (n & 7) * 8 + 4 * (g & 1)) =
         make_float4(d2[0], d2[1], d2[2], d2[3]);
     wave_sync();
 }
 // One pair's correlation from its frequency-domain product (one bin per lane, the four modes in
 // acc): LPF, the 2-D DCT-IIe / DST-IIe above through the wavefront's 256 floats of LDS at S, unfold
-// to the 15x15 tile (DttRad2.java:443-489), written to cs (LDS) and, unless co is null, co (global).
-__device__ __forceinline__ void corr_fd_to_tile_mx(const float* acc, float lp, float* S, const float (&u)[4], int lane,
-                                                   float* co, float* cs) {
+// to the 15x15 tile (DttRad2.java:443-489): lane (ay, ax) = (lane >> 3, lane & 7) gets the values
+// at (7 + ay, 7 + ax), (7 + ay, 7 - ax), (7 - ay, 7 + ax), (7 - ay, 7 - ax) in v[0..3].  The
+// staging holds the four quadrants QS floats apart.
+template <int QS = 64>
+__device__ __forceinline__ void corr_fd_unfold(const float* acc, float lp, float* S, const float (&u)[4], int lane,
+                                               float (&v)[4]) {
     const int y = lane >> 3, xx = lane & 7;
     wave_sync();
 #pragma unroll
-    for (int q = 0; q < 4; q++) S[q * 64 + lane] = acc[q] * lp;
+    for (int q = 0; q < 4; q++) S[q * QS + lane] = acc[q] * lp;
     wave_sync();
-    corr_dct_mx(S, u, lane);
-    // unfold to 15x15: CC[ay][ax] +- SC[ay][ax-1] +- CS[ay-1][ax] +- SS[ay-1][ax-1]; the lane (ay, ax) =
-    // (y, xx) writes (7 +- y, 7 +- xx)
+    corr_dct_mx<QS>(S, u, lane);
+    // CC[ay][ax] +- SC[ay][ax-1] +- CS[ay-1][ax] +- SS[ay-1][ax-1]
     const float va = S[y * 8 + xx];
-    const float vb = xx > 0 ? S[64 + y * 8 + xx - 1] : 0.f;
-    const float vc = y > 0 ? S[128 + (y - 1) * 8 + xx] : 0.f;
-    const float vd = (xx > 0 && y > 0) ? S[192 + (y - 1) * 8 + xx - 1] : 0.f;
-    const float v_pp = 0.25f * (va + vb + vc + vd), v_pm = 0.25f * (va - vb + vc - vd);
-    const float v_mp = 0.25f * (va + vb - vc - vd), v_mm = 0.25f * (va - vb - vc + vd);
-    const int o_pp = (7 + y) * 15 + 7 + xx, o_pm = (7 + y) * 15 + 7 - xx;
-    const int o_mp = (7 - y) * 15 + 7 + xx, o_mm = (7 - y) * 15 + 7 - xx;
-    cs[o_pp] = v_pp;
-    if (xx > 0) cs[o_pm] = v_pm;
-    if (y > 0) cs[o_mp] = v_mp;
-    if (xx > 0 && y > 0) cs[o_mm] = v_mm;
-    if (co) {
-        co[o_pp] = v_pp;
-        if (xx > 0) co[o_pm] = v_pm;
-        if (y > 0) co[o_mp] = v_mp;
-        if (xx > 0 && y > 0) co[o_mm] = v_mm;
-    }
+    const float vb = xx > 0 ? S[QS + y * 8 + xx - 1] : 0.f;
+    const float vc = y > 0 ? S[2 * QS + (y - 1) * 8 + xx] : 0.f;
+    const float vd = (xx > 0 && y > 0) ? S[3 * QS + (y - 1) * 8 + xx - 1] : 0.f;
+    v[0] = 0.25f * (va + vb + vc + vd);
+    v[1] = 0.25f * (va - vb + vc - vd);
+    v[2] = 0.25f * (va + vb - vc - vd);
+    v[3] = 0.25f * (va - vb - vc + vd);
+}
+// the unfolded values of corr_fd_unfold into the 15x15 tile at c (the lanes with ay or ax = 0 skip
+// the duplicates)
+__device__ __forceinline__ void corr_tile_put(float* c, const float (&v)[4], int lane) {
+    const int y = lane >> 3, xx = lane & 7;
+    c[(7 + y) * 15 + 7 + xx] = v[0];
+    if (xx > 0) c[(7 + y) * 15 + 7 - xx] = v[1];
+    if (y > 0) c[(7 - y) * 15 + 7 + xx] = v[2];
+    if (xx > 0 && y > 0) c[(7 - y) * 15 + 7 - xx] = v[3];
+}
+// corr_fd_unfold written to cs (LDS) and, unless co is null, co (global)
+__device__ __forceinline__ void corr_fd_to_tile_mx(const float* acc, float lp, float* S, const float (&u)[4], int lane,
+                                                   float* co, float* cs) {
+    float v[4];
+    corr_fd_unfold(acc, lp, S, u, lane, v);
+    corr_tile_put(cs, v, lane);
+    if (co) corr_tile_put(co, v, lane);
 }
 // per-lane operand of corr_dct_mx: the block-diagonal DCT-IIe / DST-IIe at [n][4g + s]
 __device__ __forceinline__ void corr_mx_operand(int lane, float (&u)[4]) {
@@ -1021,109 +1048,27 @@ __device__ __forceinline__ int argmax16(float v, int idx) {
 }
 
 
-template <bool FZ>   // FZ: fat zero in use (the per-bin normaliser needs the mean over the bins)
-__global__ __launch_bounds__(256, FZ ? 4 : 6) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
-                                                     int task_words, int num_tasks, const float* __restrict__ lpf,
-                                                     const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
-                                                     float* __restrict__ combo_out, float* __restrict__ disp_out,
-                                                     const int32_t* __restrict__ task_frame, long long clt_fstride,
-                                                     GeomDev g, float* __restrict__ mix_out) {
-    extern __shared__ __attribute__((aligned(16))) float s_dyn[];
-    const CorrArgs& A = *args_p;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int t = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wave;   // one tile per wavefront
-    if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
-    float* S = s_dyn + wave * CQ_WAVE_FLOATS;         // [4][64] transform staging
-    float* s_corr = S + 256;                          // [6][225]
-    float* s_combo = S;                               // [240], once the pairs are done with S
-    float* s_c1d = s_corr + 6 * CLT_CORR_LEN;         // [2][16]
+// Outputs of a quad task without correlation: NaN tiles and strip, zero strengths and weights.
+__device__ __forceinline__ void corr_quad_empty(int t, int slen, int lane, float* __restrict__ corr_out,
+                                                float* __restrict__ combo_out, float* __restrict__ disp_out) {
+    const float qnan = __int_as_float(0x7fc00000);
+    for (int o = lane; o < 6 * CLT_CORR_LEN; o += 64) corr_out[(size_t)t * 6 * CLT_CORR_LEN + o] = qnan;
+    if (combo_out)
+        for (int e = lane; e < slen; e += 64) combo_out[(size_t)t * slen + e] = qnan;
+    if (lane < CLT_DISP_LAYERS && disp_out)
+        disp_out[(size_t)t * CLT_DISP_LAYERS + lane] =
+            (lane == CLT_DISP_STRENGTH || lane == CLT_DISP_HOR_STR || lane == CLT_DISP_VERT_STR ||
+             lane == CLT_DISP_CM_WEIGHT || lane == CLT_DISP_STR_RAW) ? 0.f : qnan;
+}
+
+// The rest of correlate2D for a quad task (one wavefront) from its six tiles in LDS at s_corr: the
+// combined strip (s_combo: LDS of 15 * strip_height floats, and combo_out), the two 1-D profiles
+// (s_c1d: LDS of 32 floats) and the task's disparity layers.
+__device__ __forceinline__ void corr_quad_tail(const CorrArgs& A, const float* s_corr, float* s_combo, float* s_c1d,
+                                               int lane, int t, float* __restrict__ combo_out,
+                                               float* __restrict__ disp_out) {
     const float qnan = __int_as_float(0x7fc00000);
     const int hw = A.strip_height, slen = hw * 15;
-    const int32_t* rec = tasks + (size_t)t * task_words;
-    if (rec[0] == 0) {
-        for (int o = lane; o < 6 * CLT_CORR_LEN; o += 64) corr_out[(size_t)t * 6 * CLT_CORR_LEN + o] = qnan;
-        if (combo_out)
-            for (int e = lane; e < slen; e += 64) combo_out[(size_t)t * slen + e] = qnan;
-        if (lane < CLT_DISP_LAYERS && disp_out)
-            disp_out[(size_t)t * CLT_DISP_LAYERS + lane] =
-                (lane == CLT_DISP_STRENGTH || lane == CLT_DISP_HOR_STR || lane == CLT_DISP_VERT_STR ||
-                 lane == CLT_DISP_CM_WEIGHT || lane == CLT_DISP_STR_RAW) ? 0.f : qnan;
-        return;
-    }
-    const int txy = rec[1];
-    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
-    const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
-    if (task_frame) clt += task_frame[t] * clt_fstride;   // batched frames
-    float D[4][3][4];
-#pragma unroll
-    for (int c = 0; c < 4; c++)
-#pragma unroll
-        for (int col = 0; col < 3; col++)
-#pragma unroll
-            for (int n = 0; n < 4; n++) D[c][col][n] = clt[(((size_t)c * ntiles + tile) * 3 + col) * 256 + n * 64 + lane];
-    if (mix_out) {   // cltMixCameras (C2:425-447): the mean of the 4 cameras, for the rig's inter-camera pair
-#pragma unroll
-        for (int col = 0; col < 3; col++)
-#pragma unroll
-            for (int n = 0; n < 4; n++)
-                mix_out[((size_t)t * 3 + col) * 256 + n * 64 + lane] =
-                    0.25f * (D[0][col][n] + D[1][col][n] + D[2][col][n] + D[3][col][n]);
-    }
-    constexpr bool fz = FZ;
-    float nrm[4][3];   // per bin sum over the 4 modes of the squares (C2:205-212)
-#pragma unroll
-    for (int c = 0; c < 4; c++)
-#pragma unroll
-        for (int col = 0; col < 3; col++)
-            nrm[c][col] = D[c][col][0] * D[c][col][0] + D[c][col][1] * D[c][col][1] + D[c][col][2] * D[c][col][2] +
-                          D[c][col][3] * D[c][col][3];
-    if (!fz) {   // 1 / sqrt(n1 n2) * w = (sqrt(w) / sqrt(n1)) * (sqrt(w) / sqrt(n2)): scale each camera once
-#pragma unroll
-        for (int col = 0; col < 3; col++) {
-            const float sw = fast_sqrt(A.col_w[col]);
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const float k = sw * __builtin_amdgcn_rsqf(nrm[c][col]);
-#pragma unroll
-                for (int n = 0; n < 4; n++) D[c][col][n] *= k;
-            }
-        }
-    }
-    float umx[4];
-    corr_mx_operand(lane, umx);
-    const float lp = lpf[lane];
-    float* corr_t = corr_out + (size_t)t * 6 * CLT_CORR_LEN;
-#pragma unroll
-    for (int p = 0; p < 6; p++) {
-        const int c1 = QP_C1[p], c2 = QP_C2[p];
-        float acc[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int col = 0; col < 3; col++) {
-            const float* d1 = D[c1][col];
-            const float* d2 = D[c2][col];
-            float scale = 1.f;
-            if (fz) {   // fat zero: mean over the bins of sqrt(n1 n2) (C2:213-218)
-                const float a2 = fast_sqrt(nrm[c1][col] * nrm[c2][col]);
-                const float fz2 = wave_sum(a2) * (1.f / 64.f) * A.fat_zero * A.fat_zero;
-                scale = fast_rcp(a2 + fz2) * A.col_w[col];
-            }
-            // product with the ZI sign/index table (Correlation2d.java:86-90)
-            const float t0 = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2] + d1[3] * d2[3];
-            const float t1 = -d1[1] * d2[0] + d1[0] * d2[1] - d1[3] * d2[2] + d1[2] * d2[3];
-            const float t2 = -d1[2] * d2[0] - d1[3] * d2[1] + d1[0] * d2[2] + d1[1] * d2[3];
-            const float t3 = d1[3] * d2[0] - d1[2] * d2[1] - d1[1] * d2[2] + d1[0] * d2[3];
-            acc[0] = fmaf(t0, scale, acc[0]);
-            acc[1] = fmaf(t1, scale, acc[1]);
-            acc[2] = fmaf(t2, scale, acc[2]);
-            acc[3] = fmaf(t3, scale, acc[3]);
-        }
-        corr_fd_to_tile_mx(acc, lp, S, umx, lane, nullptr, s_corr + p * CLT_CORR_LEN);
-    }
-    wave_sync();
-    // the six correlation tiles, contiguous in LDS and in the output: stored as 8-byte pairs, each
-    // store instruction a contiguous 512-byte run (the tile's base is 8-byte aligned)
-    for (int o = lane; o < 3 * CLT_CORR_LEN; o += 64)
-        reinterpret_cast<float2*>(corr_t)[o] = reinterpret_cast<const float2*>(s_corr)[o];
     // ---- combined strip (C2:730-790) of the 6 rotated strips (C2:556-718)
     {
         const int ncomb = 6 * (A.twice_diagonal ? 2 : 1);
@@ -1259,6 +1204,204 @@ __global__ __launch_bounds__(256, FZ ? 4 : 6) void k_corr2d_quad(const float* __
         default: r = have_max ? s0 : 0.f; break;   // CLT_DISP_CM_WEIGHT
         }
         disp_out[(size_t)t * CLT_DISP_LAYERS + lane] = r;
+    }
+}
+
+template <bool FZ>   // FZ: fat zero in use (the per-bin normaliser needs the mean over the bins)
+__global__ __launch_bounds__(256, FZ ? 4 : 6) void k_corr2d_quad(const float* __restrict__ clt, const int32_t* __restrict__ tasks,
+                                                     int task_words, int num_tasks, const float* __restrict__ lpf,
+                                                     const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
+                                                     float* __restrict__ combo_out, float* __restrict__ disp_out,
+                                                     const int32_t* __restrict__ task_frame, long long clt_fstride,
+                                                     GeomDev g, float* __restrict__ mix_out) {
+    extern __shared__ __attribute__((aligned(16))) float s_dyn[];
+    const CorrArgs& A = *args_p;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wave;   // one tile per wavefront
+    if (t >= num_tasks) return;                       // wave-uniform: no block barriers below
+    float* S = s_dyn + wave * CQ_WAVE_FLOATS;         // [4][64] transform staging
+    float* s_corr = S + 256;                          // [6][225]
+    float* s_combo = S;                               // [240], once the pairs are done with S
+    float* s_c1d = s_corr + 6 * CLT_CORR_LEN;         // [2][16]
+    const int32_t* rec = tasks + (size_t)t * task_words;
+    if (rec[0] == 0) {
+        corr_quad_empty(t, A.strip_height * 15, lane, corr_out, combo_out, disp_out);
+        return;
+    }
+    const int txy = rec[1];
+    const size_t ntiles = (size_t)g.tiles_x * g.tiles_y;
+    const size_t tile = (size_t)((txy >> 16) & 0xffff) * g.tiles_x + (txy & 0xffff);
+    if (task_frame) clt += task_frame[t] * clt_fstride;   // batched frames
+    float D[4][3][4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+#pragma unroll
+            for (int n = 0; n < 4; n++) D[c][col][n] = clt[(((size_t)c * ntiles + tile) * 3 + col) * 256 + n * 64 + lane];
+    if (mix_out) {   // cltMixCameras (C2:425-447): the mean of the 4 cameras, for the rig's inter-camera pair
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+#pragma unroll
+            for (int n = 0; n < 4; n++)
+                mix_out[((size_t)t * 3 + col) * 256 + n * 64 + lane] =
+                    0.25f * (D[0][col][n] + D[1][col][n] + D[2][col][n] + D[3][col][n]);
+    }
+    constexpr bool fz = FZ;
+    float nrm[4][3];   // per bin sum over the 4 modes of the squares (C2:205-212)
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int col = 0; col < 3; col++)
+            nrm[c][col] = D[c][col][0] * D[c][col][0] + D[c][col][1] * D[c][col][1] + D[c][col][2] * D[c][col][2] +
+                          D[c][col][3] * D[c][col][3];
+    if (!fz) {   // 1 / sqrt(n1 n2) * w = (sqrt(w) / sqrt(n1)) * (sqrt(w) / sqrt(n2)): scale each camera once
+#pragma unroll
+        for (int col = 0; col < 3; col++) {
+            const float sw = fast_sqrt(A.col_w[col]);
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const float k = sw * __builtin_amdgcn_rsqf(nrm[c][col]);
+#pragma unroll
+                for (int n = 0; n < 4; n++) D[c][col][n] *= k;
+            }
+        }
+    }
+    float umx[4];
+    corr_mx_operand(lane, umx);
+    const float lp = lpf[lane];
+    float* corr_t = corr_out + (size_t)t * 6 * CLT_CORR_LEN;
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+        const int c1 = QP_C1[p], c2 = QP_C2[p];
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int col = 0; col < 3; col++) {
+            const float* d1 = D[c1][col];
+            const float* d2 = D[c2][col];
+            float scale = 1.f;
+            if (fz) {   // fat zero: mean over the bins of sqrt(n1 n2) (C2:213-218)
+                const float a2 = fast_sqrt(nrm[c1][col] * nrm[c2][col]);
+                const float fz2 = wave_sum(a2) * (1.f / 64.f) * A.fat_zero * A.fat_zero;
+                scale = fast_rcp(a2 + fz2) * A.col_w[col];
+            }
+            // product with the ZI sign/index table (Correlation2d.java:86-90)
+            const float t0 = d1[0] * d2[0] + d1[1] * d2[1] + d1[2] * d2[2] + d1[3] * d2[3];
+            const float t1 = -d1[1] * d2[0] + d1[0] * d2[1] - d1[3] * d2[2] + d1[2] * d2[3];
+            const float t2 = -d1[2] * d2[0] - d1[3] * d2[1] + d1[0] * d2[2] + d1[1] * d2[3];
+            const float t3 = d1[3] * d2[0] - d1[2] * d2[1] - d1[1] * d2[2] + d1[0] * d2[3];
+            acc[0] = fmaf(t0, scale, acc[0]);
+            acc[1] = fmaf(t1, scale, acc[1]);
+            acc[2] = fmaf(t2, scale, acc[2]);
+            acc[3] = fmaf(t3, scale, acc[3]);
+        }
+        corr_fd_to_tile_mx(acc, lp, S, umx, lane, nullptr, s_corr + p * CLT_CORR_LEN);
+    }
+    wave_sync();
+    // the six correlation tiles, contiguous in LDS and in the output: stored as 8-byte pairs, each
+    // store instruction a contiguous 512-byte run (the tile's base is 8-byte aligned)
+    for (int o = lane; o < 3 * CLT_CORR_LEN; o += 64)
+        reinterpret_cast<float2*>(corr_t)[o] = reinterpret_cast<const float2*>(s_corr)[o];
+    corr_quad_tail(A, s_corr, s_combo, s_c1d, lane, t, combo_out, disp_out);
+}
+
+// convert_direct and correlate2D of one quad task (4 cameras, 3 colours, the 6 pairs) per
+// 3-wavefront block, the CLT passed from one to the other in LDS: after the conversion (the CLT
+// also goes to memory for the textures) wavefront w correlates pairs 2 w and 2 w + 1 from the
+// block's 12 (camera, colour) units in LDS, then the six tiles meet in LDS for wavefront 0's strip,
+// profiles and disparity layers while wavefronts 1 and 2 store the tiles.  Same per-pair arithmetic
+// as k_corr2d_quad (pair products of normalised vectors, the matrix-core DCT-IIe, the unfold).  The
+// transform staging of wavefront w is in the 80 unused floats after each of its units' 256
+// coefficients (one quadrant per unit), so the block needs no LDS beyond the conversion's; at 72
+// VGPRs 7 wavefronts per SIMD.
+template <bool FZ>
+__global__ __launch_bounds__(192, 7) void k_convert_corr(const float* __restrict__ images, const float* __restrict__ kernels,
+                                                      const float* __restrict__ koffsets, const int32_t* __restrict__ tasks,
+                                                      int task_words, int num_tasks, float* __restrict__ clt,
+                                                      const float* __restrict__ lpf, int lpf_mask, double inv_kstep,
+                                                      const int32_t* __restrict__ task_frame, long long img_fstride,
+                                                      long long clt_fstride, GeomDev g, const float* __restrict__ lpf_corr,
+                                                      const CorrArgs* __restrict__ args_p, float* __restrict__ corr_out,
+                                                      float* __restrict__ combo_out, float* __restrict__ disp_out) {
+    static_assert(CONV_UNIT >= 256 + 64 && CONV_UNIT % 4 == 0, "a quadrant in each unit's tail");
+    __shared__ __attribute__((aligned(16))) float s_lds[12 * CONV_UNIT];
+    float (*units)[CONV_UNIT] = reinterpret_cast<float (*)[CONV_UNIT]>(s_lds);
+    int t;
+    // one task per block: every wavefront gets the same answer, so the return is block-uniform
+    const bool busy = convert_tiles(images, kernels, koffsets, tasks, task_words, num_tasks, clt, lpf, lpf_mask,
+                                    inv_kstep, task_frame, img_fstride, clt_fstride, g, units, t);
+    const CorrArgs& A = *args_p;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (!busy) {
+        if (wave == 0) corr_quad_empty(t, A.strip_height * 15, lane, corr_out, combo_out, disp_out);
+        return;
+    }
+    __syncthreads();   // the 12 units' CLT in LDS
+    float* S = units[wave * 4] + 256;   // quadrant q in the tail of unit 4 wave + q
+    float umx[4];
+    corr_mx_operand(lane, umx);
+    const float lp = lpf_corr[lane];
+    float v[2][4];
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+        if ((p >> 1) != wave) continue;   // wave-uniform
+        const int c1 = QP_C1[p], c2 = QP_C2[p];
+        float d1[3][4], d2[3][4], n1[3], n2[3];
+#pragma unroll
+        for (int col = 0; col < 3; col++) {
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                d1[col][n] = units[col * 4 + c1][n * 64 + lane];
+                d2[col][n] = units[col * 4 + c2][n * 64 + lane];
+            }
+            n1[col] = d1[col][0] * d1[col][0] + d1[col][1] * d1[col][1] + d1[col][2] * d1[col][2] + d1[col][3] * d1[col][3];
+            n2[col] = d2[col][0] * d2[col][0] + d2[col][1] * d2[col][1] + d2[col][2] * d2[col][2] + d2[col][3] * d2[col][3];
+            if (!FZ) {   // (sqrt(w) / sqrt(n1)) * (sqrt(w) / sqrt(n2)), as k_corr2d_quad
+                const float sw = fast_sqrt(A.col_w[col]);
+                const float k1 = sw * __builtin_amdgcn_rsqf(n1[col]), k2 = sw * __builtin_amdgcn_rsqf(n2[col]);
+#pragma unroll
+                for (int n = 0; n < 4; n++) {
+                    d1[col][n] *= k1;
+                    d2[col][n] *= k2;
+                }
+            }
+        }
+        float acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int col = 0; col < 3; col++) {
+            const float* a = d1[col];
+            const float* b = d2[col];
+            float scale = 1.f;
+            if (FZ) {   // fat zero: mean over the bins of sqrt(n1 n2) (C2:213-218)
+                const float a2 = fast_sqrt(n1[col] * n2[col]);
+                const float fz2 = wave_sum(a2) * (1.f / 64.f) * A.fat_zero * A.fat_zero;
+                scale = fast_rcp(a2 + fz2) * A.col_w[col];
+            }
+            // product with the ZI sign/index table (Correlation2d.java:86-90)
+            const float t0 = a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+            const float t1 = -a[1] * b[0] + a[0] * b[1] - a[3] * b[2] + a[2] * b[3];
+            const float t2 = -a[2] * b[0] - a[3] * b[1] + a[0] * b[2] + a[1] * b[3];
+            const float t3 = a[3] * b[0] - a[2] * b[1] - a[1] * b[2] + a[0] * b[3];
+            acc[0] = fmaf(t0, scale, acc[0]);
+            acc[1] = fmaf(t1, scale, acc[1]);
+            acc[2] = fmaf(t2, scale, acc[2]);
+            acc[3] = fmaf(t3, scale, acc[3]);
+        }
+        corr_fd_unfold<CONV_UNIT>(acc, lp, S, umx, lane, v[p & 1]);
+    }
+    __syncthreads();   // every wavefront is done with the units: the tiles go where they were
+    float* s_corr = s_lds;                    // [6][225]
+    float* s_c1d = s_lds + 6 * CLT_CORR_LEN + 2;   // [2][16]
+    float* s_combo = s_c1d + 32;                    // [15 * strip_height <= 240]
+    corr_tile_put(s_corr + (2 * wave) * CLT_CORR_LEN, v[0], lane);
+    corr_tile_put(s_corr + (2 * wave + 1) * CLT_CORR_LEN, v[1], lane);
+    __syncthreads();
+    if (wave) {   // the six tiles, contiguous in LDS and in the output, as 8-byte pairs
+        float* corr_t = corr_out + (size_t)t * 6 * CLT_CORR_LEN;
+        for (int o = threadIdx.x - 64; o < 3 * CLT_CORR_LEN; o += 128)
+            reinterpret_cast<float2*>(corr_t)[o] = reinterpret_cast<const float2*>(s_corr)[o];
+    } else {
+        corr_quad_tail(A, s_corr, s_combo, s_c1d, lane, t, combo_out, disp_out);
     }
 }
 

@@ -157,7 +157,7 @@ static void half_flat_top(int n, double hwidth, double blur, bool normalize, boo
 }
 
 // ------------------------------------------------------------------------ handle
-enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_CORR_LMA, T_KPREP, T_RIG, T_TEX_IMCLT, T_NUM };
+enum { T_CONVERT = 0, T_IMCLT, T_CORR, T_TEX, T_COMBINE, T_BATCH, T_ADJ, T_GEOADJ, T_CORR_LMA, T_KPREP, T_RIG, T_TEX_IMCLT, T_CONV_CORR, T_NUM };
 
 struct clt_handle {
     clt_geom g{};
@@ -1522,6 +1522,47 @@ static void swap_corr_records(clt_handle* h) {
     std::swap(h->ev_lma, h->ev_lma2);
     h->lma2_pending = true;
     h->lma_pending = false;
+}
+
+// execConverCorrectTiles then execCorrelate2D over the same task list.  In the quad layout (4
+// cameras, 3 colours, the 6 default pairs) one kernel does both, the CLT passed through LDS (it still
+// goes to memory for the textures and imclt); otherwise the two launches of clt_exec_convert and
+// clt_exec_corr2d.
+int clt_exec_convert_corr2d(clt_handle* h, int32_t lpf_mask) {
+    if (!h) return CLT_E_ARG;
+    for (int c = 0; c < h->g.num_cams; c++)
+        if (!h->images_set[c]) return fail(h, CLT_E_STATE, "clt_exec_convert_corr2d: image of camera %d not set", c);
+    if (h->num_tasks <= 0) return fail(h, CLT_E_STATE, "clt_exec_convert_corr2d: no tasks");
+    if (h->num_frames > 1) return fail(h, CLT_E_UNSUPPORTED, "clt_exec_convert_corr2d: a batch handle runs clt_exec_batch");
+    int rc;
+    swap_corr_records(h);
+    if ((rc = prepare_corr(h))) return rc;
+    const int n = h->num_tasks;
+    if (h->corr_quad) {
+        join_side(h);   // a running imclt_rbg still reads the CLT
+        join_lma(h);    // the previous LMA still reads d_corr and d_disp
+        const double inv_kstep = 1.0 / h->g.kernel_step;   // exact: checked power of two
+        auto kern = (h->cp.fat_zero != 0.f) ? k_convert_corr<true> : k_convert_corr<false>;
+        tbegin(h, T_CONV_CORR);
+        hipLaunchKernelGGL(kern, dim3(n), dim3(192), 0, h->stream, h->d_images, h->use_kernels ? h->d_kernels : nullptr,
+                           h->use_kernels ? h->d_koffs : nullptr, h->d_tasks, h->task_words, n, h->d_clt,
+                           lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, inv_kstep, nullptr, h->img_fstride,
+                           h->clt_fstride, h->gd, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp);
+        tend(h, T_CONV_CORR);
+    } else {
+        tbegin(h, T_CONVERT);
+        launch_convert(h, n, nullptr, lpf_mask);
+        tend(h, T_CONVERT);
+        tbegin(h, T_CORR);
+        launch_corr(h, n, nullptr);
+        tend(h, T_CORR);
+    }
+    launch_corr_lma(h, n, true);
+    HIPCHK(h, hipGetLastError());
+    h->converted = true;
+    h->tex_done = h->combined = h->rig_done = false;
+    h->corr_done = true;
+    return finish(h);
 }
 
 int clt_exec_corr2d(clt_handle* h) {

@@ -322,6 +322,11 @@ class GPUTileProcessor:
     def execCorrelate2D(self):
         self._chk(self.lib.clt_exec_corr2d(self.h))
 
+    def execConvertCorrelate2D(self, lpf_mask=0):
+        """execConverCorrectTiles(lpf_mask) then execCorrelate2D() with the same outputs; in the quad
+        layout one kernel that hands each task's CLT from the conversion to the correlation on chip."""
+        self._chk(self.lib.clt_exec_convert_corr2d(self.h, lpf_mask))
+
     def execTextures(self, extra_disparity=None):
         ed = None if extra_disparity is None else np.ascontiguousarray(extra_disparity, np.float32)
         self._chk(self.lib.clt_exec_textures(self.h, _ptr(ed)))

@@ -254,6 +254,11 @@ int  clt_exec_imclt(clt_handle* h, int32_t apply_lpf);
  * per-tile disparity (ID:2186-2238) and, when enabled, the per-tile LMA chain (ID:2254-2336) over
  * the pairs with a known orientation (at most 2 compatible groups, C2:72-78). */
 int  clt_exec_corr2d(clt_handle* h);
+/* clt_exec_convert(h, lpf_mask) then clt_exec_corr2d(h) with the same outputs (GTP:530-558 then
+ * execCorrelate2D, the order of the reference's frame loop): in the quad layout one kernel, the CLT
+ * of each task handed from the conversion to the correlation on chip (it is still written for the
+ * textures and imclt); otherwise the two calls. */
+int  clt_exec_convert_corr2d(clt_handle* h, int32_t lpf_mask);
 /* The correlation part of clt_bi_quad(...) (ID:8731-9504) for a two-quad rig held as one 8-camera
  * frame: cameras 0..3 the main quad, 4..7 the auxiliary quad (tasks carry TpTask.xy then xy_aux,
  * GTP:132-166).  Per task: tileCorrs (ID:7581-7740) of each quad over the quad pair table
@@ -500,7 +505,7 @@ int  clt_set_timing(clt_handle* h, int32_t enable);
  * 6 correlation adjoint, 7 geometry adjoint, 8 per-tile correlation LMA, 9 kernel preparation
  * (clt_prepare_kernels: one count per band of kernel rows, plus one for the centre slopes),
  * 10 the whole of clt_exec_corr2d_rig, 11 the one-pass textures + imclt of
- * clt_exec_textures_imclt (both column launches). */
+ * clt_exec_textures_imclt (both column launches), 12 the fused kernel of clt_exec_convert_corr2d. */
 int  clt_get_timing(clt_handle* h, int32_t which, double* total_ms, int32_t* count);
 int  clt_reset_timing(clt_handle* h);
 

@@ -131,14 +131,17 @@ def test_imclt_sparse_tasks(pk, orc):
     tp.close()
 
 
-def _corr_compare(pk, orc, sc, fat_zero=0.0, pairs=None, strip=True):
+def _corr_compare(pk, orc, sc, fat_zero=0.0, pairs=None, strip=True, fused=False):
     tp = gpu_processor(pk, sc)
     tp.setCorrSigma(0.8)
     tp.setCorrParams(fat_zero=fat_zero)
     if pairs is not None:
         tp.setCorrPairs(pairs)
-    tp.execConverCorrectTiles()
-    tp.execCorrelate2D()
+    if fused:
+        tp.execConvertCorrelate2D()
+    else:
+        tp.execConverCorrectTiles()
+        tp.execCorrelate2D()
     g_corr, g_combo, g_disp = tp.getCorr2D()
     g_clt = gpu_clt_cpu_layout(tp, sc).astype(np.float64)
     prm = orc.default_corr_params(fat_zero=fat_zero)
@@ -173,6 +176,49 @@ def test_corr2d_fat_zero(pk, orc, quad_scene):
     r = _corr_compare(pk, orc, quad_scene, fat_zero=0.05)
     log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
     assert r["corr"] < CLT_TOL and r["cm"] < DISP_TOL
+
+
+@pytest.mark.parametrize("fat_zero", [0.0, 0.05])
+def test_convert_corr2d_fused_against_oracle(pk, orc, quad_scene, fat_zero):
+    log("case: convert + correlate2D in one kernel, quad 128x96, fat_zero %g" % fat_zero)
+    r = _corr_compare(pk, orc, quad_scene, fat_zero=fat_zero, fused=True)
+    log("  " + ", ".join("%s=%.3g" % kv for kv in r.items()))
+    assert r["corr"] < CLT_TOL and r["corr_e2e"] < CLT_TOL and r["combo"] < CLT_TOL
+    assert r["int"] == 0 and r["cm"] < DISP_TOL and r["hor"] < DISP_TOL and r["vert"] < DISP_TOL
+    assert r["cm_e2e"] < DISP_TOL and r["strength"] < CLT_TOL
+
+
+@pytest.mark.parametrize("fat_zero", [0.0, 0.05])
+def test_convert_corr2d_fused_matches_two_calls(pk, quad_scene, fat_zero):
+    log("case: execConvertCorrelate2D against execConverCorrectTiles + execCorrelate2D, every 3rd task "
+        "word 0, fat_zero %g, twice in a row" % fat_zero)
+    sc = quad_scene
+    tasks = sc.tasks.copy()
+    tasks[::3, 0] = 0
+    out = []
+    for fused in (False, True):
+        tp = gpu_processor(pk, sc)
+        tp.setTasks(tasks)
+        tp.setCorrSigma(0.8)
+        tp.setCorrParams(fat_zero=fat_zero)
+        for _ in range(2):   # the second call writes the other correlation record set
+            if fused:
+                tp.execConvertCorrelate2D()
+            else:
+                tp.execConverCorrectTiles()
+                tp.execCorrelate2D()
+        out.append((gpu_clt_cpu_layout(tp, sc),) + tp.getCorr2D())
+        tp.close()
+    (c0, k0, s0, d0), (c1, k1, s1, d1) = out
+    assert np.array_equal(c0, c1), "CLT"
+    for name, a, b in (("corr", k0, k1), ("combo", s0, s1), ("disp", d0, d1)):
+        assert np.array_equal(np.isnan(a), np.isnan(b)), name + " NaN pattern"
+        fin = np.isfinite(a)
+        err = float(np.abs(a[fin] - b[fin]).max()) if fin.any() else 0.0
+        log("  %s max |fused - two calls| = %.3g" % (name, err))
+        assert err <= 1e-6 * max(1.0, float(np.abs(a[fin]).max())), name
+    live = tasks[:, 0] != 0
+    assert np.isnan(k1[~live]).all()
 
 
 def test_corr2d_mono16_all_pairs(pk, orc):
