@@ -459,6 +459,19 @@ static void launch_corr(clt_handle* h, int n, const int32_t* task_frame) {
     join_lma(h);   // the previous LMA still reads d_corr and d_disp
     launch_corr_range(h, 0, n, task_frame);
 }
+// convert and correlate2D of tasks [t0, t0 + n) in one kernel (h->corr_quad only: one task per
+// 3-wavefront block)
+static void launch_convert_corr_range(clt_handle* h, int t0, int n, const int32_t* task_frame, int lpf_mask) {
+    const size_t P = h->corr_pairs_at_exec, sh = h->corr_strip_h_at_exec;
+    const double inv_kstep = 1.0 / h->g.kernel_step;   // exact: checked power of two
+    auto kern = (h->cp.fat_zero != 0.f) ? k_convert_corr<true> : k_convert_corr<false>;
+    hipLaunchKernelGGL(kern, dim3(n), dim3(192), 0, h->stream, h->d_images, h->use_kernels ? h->d_kernels : nullptr,
+                       h->use_kernels ? h->d_koffs : nullptr, h->d_tasks + (size_t)t0 * h->task_words, h->task_words, n,
+                       h->d_clt, lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, inv_kstep,
+                       task_frame ? task_frame + t0 : nullptr, h->img_fstride, h->clt_fstride, h->gd, h->d_lpf_corr,
+                       h->d_corr_args, h->d_corr + (size_t)t0 * P * CLT_CORR_LEN,
+                       h->d_combo ? h->d_combo + (size_t)t0 * sh * 15 : nullptr, h->d_disp + (size_t)t0 * CLT_DISP_LAYERS);
+}
 
 // Per-tile LMA chain over the records of tasks [t0, t0 + n) just written by correlate2D (16 tiles
 // per 256-thread block), forked onto its own stream; the caller joins it (join_lma, or the end of
@@ -1541,13 +1554,8 @@ int clt_exec_convert_corr2d(clt_handle* h, int32_t lpf_mask) {
     if (h->corr_quad) {
         join_side(h);   // a running imclt_rbg still reads the CLT
         join_lma(h);    // the previous LMA still reads d_corr and d_disp
-        const double inv_kstep = 1.0 / h->g.kernel_step;   // exact: checked power of two
-        auto kern = (h->cp.fat_zero != 0.f) ? k_convert_corr<true> : k_convert_corr<false>;
         tbegin(h, T_CONV_CORR);
-        hipLaunchKernelGGL(kern, dim3(n), dim3(192), 0, h->stream, h->d_images, h->use_kernels ? h->d_kernels : nullptr,
-                           h->use_kernels ? h->d_koffs : nullptr, h->d_tasks, h->task_words, n, h->d_clt,
-                           lpf_mask ? h->d_lpf_rbg : nullptr, lpf_mask, inv_kstep, nullptr, h->img_fstride,
-                           h->clt_fstride, h->gd, h->d_lpf_corr, h->d_corr_args, h->d_corr, h->d_combo, h->d_disp);
+        launch_convert_corr_range(h, 0, n, nullptr, lpf_mask);
         tend(h, T_CONV_CORR);
     } else {
         tbegin(h, T_CONVERT);
@@ -2407,9 +2415,9 @@ static int assemble_batch(clt_handle* h) {
 enum { STAGE_CONVERT = 1, STAGE_CORR = 2, STAGE_TEX = 4, STAGE_IMCLT = 8 };
 
 // The batch as a software pipeline over tune_chunks runs of whole frames.  Each chunk runs convert,
-// textures and correlate2D on the main stream and forks its imclt_rbg (after convert) and its
-// per-tile LMA (after correlate2D) onto their own streams, where they overlap the next chunk's
-// kernels; when every frame's tasks own distinct tiles with non-zero flags, textures and
+// textures and correlate2D on the main stream (in the quad layout convert and correlate2D as one
+// kernel, before the textures) and forks its imclt_rbg (after convert) and its per-tile LMA (after
+// correlate2D) onto their own streams, where they overlap the next chunk's kernels; when every frame's tasks own distinct tiles with non-zero flags, textures and
 // imclt_rbg of the chunk run as one pass (k_tex_imclt over the chunk's frames) instead.  Chunks
 // touch disjoint frames of the CLT, tile maps and image planes and disjoint task records, so the
 // only joins are the fork points and the end of the batch.
@@ -2418,6 +2426,8 @@ static void launch_batch(clt_handle* h, int stages) {
     const int F = h->num_frames;
     const int chunks = std::max(1, std::min(h->tune_chunks, F));
     const bool fused = h->batch_fused && (stages & STAGE_TEX) && (stages & STAGE_IMCLT);
+    // convert and correlate2D of a chunk in one kernel (quad layout), its LMA forked before the textures
+    const bool conv_corr = (stages & STAGE_CONVERT) && (stages & STAGE_CORR) && h->corr_quad;
     join_side(h);   // a running imclt_rbg still reads the CLT
     join_lma(h);    // ... and an LMA the correlation records
     join_lma2(h);
@@ -2425,7 +2435,12 @@ static void launch_batch(clt_handle* h, int stages) {
         const int f0 = (int)((long)F * c / chunks), f1 = (int)((long)F * (c + 1) / chunks);
         const int t0 = F > 1 ? h->frame_start[f0] : 0;
         const int n = (F > 1 ? h->frame_start[f1] : h->num_tasks) - t0;
-        if ((stages & STAGE_CONVERT) && n > 0) launch_convert_range(h, t0, n, tf, 0);
+        if (conv_corr && n > 0) {
+            launch_convert_corr_range(h, t0, n, tf, 0);
+            launch_corr_lma(h, n, false, t0);
+        } else if ((stages & STAGE_CONVERT) && n > 0) {
+            launch_convert_range(h, t0, n, tf, 0);
+        }
         if (fused) {   // textures and imclt_rbg of the chunk's frames in one pass
             launch_tex_imclt(h, f0, f1 - f0);
         } else if (stages & STAGE_IMCLT) {
@@ -2436,7 +2451,7 @@ static void launch_batch(clt_handle* h, int stages) {
             h->side_pending = true;
         }
         if (!fused && (stages & STAGE_TEX) && n > 0) launch_tex(h, n, tf, nullptr, t0);
-        if ((stages & STAGE_CORR) && n > 0) {
+        if (!conv_corr && (stages & STAGE_CORR) && n > 0) {
             launch_corr_range(h, t0, n, tf);
             launch_corr_lma(h, n, false, t0);
         }
