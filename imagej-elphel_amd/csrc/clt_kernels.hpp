@@ -1336,6 +1336,21 @@ __global__ __launch_bounds__(192, 7) void k_convert_corr(const float* __restrict
         if (wave == 0) corr_quad_empty(t, A.strip_height * 15, lane, corr_out, combo_out, disp_out);
         return;
     }
+    if (!FZ) {   // each unit's bins pre-normalised once, sqrt(w) / |D| (C2:205-212), in place (the
+                 // lane rescales the 16 floats it just stored from, so no other lane is involved)
+        float* u = units[threadIdx.x >> 4] + 4 * (threadIdx.x & 15);
+        const float sw = fast_sqrt(A.col_w[wave]);   // wavefront w converted colour w
+        float4 m[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) m[q] = *reinterpret_cast<const float4*>(u + q * 64);
+        const float k0 = sw * __builtin_amdgcn_rsqf(m[0].x * m[0].x + m[1].x * m[1].x + m[2].x * m[2].x + m[3].x * m[3].x);
+        const float k1 = sw * __builtin_amdgcn_rsqf(m[0].y * m[0].y + m[1].y * m[1].y + m[2].y * m[2].y + m[3].y * m[3].y);
+        const float k2 = sw * __builtin_amdgcn_rsqf(m[0].z * m[0].z + m[1].z * m[1].z + m[2].z * m[2].z + m[3].z * m[3].z);
+        const float k3 = sw * __builtin_amdgcn_rsqf(m[0].w * m[0].w + m[1].w * m[1].w + m[2].w * m[2].w + m[3].w * m[3].w);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            *reinterpret_cast<float4*>(u + q * 64) = make_float4(m[q].x * k0, m[q].y * k1, m[q].z * k2, m[q].w * k3);
+    }
     __syncthreads();   // the 12 units' CLT in LDS
     float* S = units[wave * 4] + 256;   // quadrant q in the tail of unit 4 wave + q
     float umx[4];
@@ -1346,7 +1361,7 @@ __global__ __launch_bounds__(192, 7) void k_convert_corr(const float* __restrict
     for (int p = 0; p < 6; p++) {
         if ((p >> 1) != wave) continue;   // wave-uniform
         const int c1 = QP_C1[p], c2 = QP_C2[p];
-        float d1[3][4], d2[3][4], n1[3], n2[3];
+        float d1[3][4], d2[3][4], n1[3], n2[3];   // (without fat zero: already normalised)
 #pragma unroll
         for (int col = 0; col < 3; col++) {
 #pragma unroll
@@ -1354,16 +1369,9 @@ __global__ __launch_bounds__(192, 7) void k_convert_corr(const float* __restrict
                 d1[col][n] = units[col * 4 + c1][n * 64 + lane];
                 d2[col][n] = units[col * 4 + c2][n * 64 + lane];
             }
-            n1[col] = d1[col][0] * d1[col][0] + d1[col][1] * d1[col][1] + d1[col][2] * d1[col][2] + d1[col][3] * d1[col][3];
-            n2[col] = d2[col][0] * d2[col][0] + d2[col][1] * d2[col][1] + d2[col][2] * d2[col][2] + d2[col][3] * d2[col][3];
-            if (!FZ) {   // (sqrt(w) / sqrt(n1)) * (sqrt(w) / sqrt(n2)), as k_corr2d_quad
-                const float sw = fast_sqrt(A.col_w[col]);
-                const float k1 = sw * __builtin_amdgcn_rsqf(n1[col]), k2 = sw * __builtin_amdgcn_rsqf(n2[col]);
-#pragma unroll
-                for (int n = 0; n < 4; n++) {
-                    d1[col][n] *= k1;
-                    d2[col][n] *= k2;
-                }
+            if (FZ) {
+                n1[col] = d1[col][0] * d1[col][0] + d1[col][1] * d1[col][1] + d1[col][2] * d1[col][2] + d1[col][3] * d1[col][3];
+                n2[col] = d2[col][0] * d2[col][0] + d2[col][1] * d2[col][1] + d2[col][2] * d2[col][2] + d2[col][3] * d2[col][3];
             }
         }
         float acc[4] = {0.f, 0.f, 0.f, 0.f};
