@@ -231,6 +231,30 @@ def test_corr2d_mono16_all_pairs(pk, orc):
     assert r["corr"] < CLT_TOL and r["corr_e2e"] < CLT_TOL
 
 
+def test_convert_corr2d_other_layouts_run_the_two_calls(pk):
+    log("case: execConvertCorrelate2D outside the quad layout (3-camera Bayer, quad with 5 pairs) "
+        "against execConverCorrectTiles + execCorrelate2D")
+    cases = ((pk.synth.make_scene(96, 64, num_cams=3, seed=21, disparity=2.0), [(0, 1, 0, 1), (0, 2, 1, 1), (1, 2, -1, 1)]),
+             (pk.synth.make_scene(96, 64, seed=22, disparity=2.0), [(0, 1, 0, 1), (2, 3, 0, 1), (0, 2, 1, 1),
+                                                                    (1, 3, 1, 1), (0, 3, 2, 1)]))
+    for sc, pairs in cases:
+        out = []
+        for fused in (False, True):
+            tp = gpu_processor(pk, sc)
+            tp.setCorrSigma(0.8)
+            if pairs is not None:
+                tp.setCorrPairs(pairs)
+            if fused:
+                tp.execConvertCorrelate2D()
+            else:
+                tp.execConverCorrectTiles()
+                tp.execCorrelate2D()
+            out.append((gpu_clt_cpu_layout(tp, sc),) + tp.getCorr2D())
+            tp.close()
+        for i, (a, b) in enumerate(zip(*out)):
+            assert a.shape == b.shape and np.array_equal(a, b, equal_nan=True), (sc.num_cams, i)
+
+
 def _tex_compare(pk, orc, sc, extra=None, tasks=None):
     tasks = sc.tasks if tasks is None else tasks
     tp = gpu_processor(pk, sc)
